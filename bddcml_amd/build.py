@@ -1,0 +1,59 @@
+"""Builds libbddcml_amd.so (HIP for gfx950 + host C++) in-tree.
+
+    python -m bddcml_amd.build
+
+Every HIP source is compiled with hipcc for gfx950 only; packed FP32 ops are
+disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libbddcml_amd.so")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+SOURCES = ["kernels_tile.hip", "kernels_misc.hip", "device.hip", "plan.cpp", "api.cpp"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-function"]
+HIPFLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    out = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
+                    if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError("build failed: " + " ".join(cmd))
+    return out
+
+
+def build(verbose: bool = False) -> str:
+    objdir = os.path.join(HERE, "_objs")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(objdir, src + ".o")
+        cmd = [HIPCC] + COMMON + HIPFLAGS + ["-c", path, "-o", obj]
+        if src.endswith(".cpp"):
+            cmd = [HIPCC] + COMMON + ["-x", "c++", "-c", path, "-o", obj]
+        out = _run(cmd)
+        if verbose and out:
+            print(out)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    rocm_llvm = "/opt/rocm/llvm/lib"
+    _run([HIPCC, "-shared", "-fopenmp", "-o", tmp] + objs +
+         ["-Wl,-rpath," + rocm_llvm, "-L" + rocm_llvm])
+    os.replace(tmp, LIB)
+    shutil.rmtree(objdir, ignore_errors=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

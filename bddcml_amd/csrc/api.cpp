@@ -1,0 +1,520 @@
+// api.cpp -- the bddcml_*_c entry points (reference src/bddcml_interface_c.h,
+// src/module_bddcml.f90, src/module_levels.f90:180-903) on top of the
+// host plan and the device pipelines.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+
+#include "../../include/bddcml_amd.h"
+#include "../../include/bddcml_interface_c.h"
+#include "solver.hpp"
+
+namespace bddc {
+
+static Solver *g_solver = nullptr;
+Solver &solver() {
+  if (!g_solver) g_solver = new Solver();
+  return *g_solver;
+}
+void reset_solver() {
+  if (g_solver) {
+    device_release(*g_solver);
+    delete g_solver;
+  }
+  g_solver = nullptr;
+}
+
+static std::string g_last_error;
+static void record_error(const Error &e) { g_last_error = e.what(); }
+
+static Sub &local_sub(Solver &S, int isub, const char *routine) {
+  int is = isub - S.numbase;
+  if (is < 0 || is >= S.nsub)
+    fatal(routine, "Index of subdomain not found among local subdomains: " + std::to_string(isub));
+  return S.subs[is];
+}
+
+}  // namespace bddc
+
+using namespace bddc;
+
+extern "C" {
+
+void bddcml_init_c(int *nl, int *nsublev, int *lnsublev, int *nsub_loc_1, int *comm_init,
+                   int *verbose_level, int *numbase, int *just_direct_solve_int) try {
+  (void)comm_init;
+  (void)just_direct_solve_int;
+  reset_solver();
+  Solver &S = solver();
+  if (*nl < 2) fatal("LEVELS_INIT", "Number of levels must be at least 2");
+  if (*nl != 2) fatal("LEVELS_INIT", "This library implements two levels");
+  if (*lnsublev < *nl) fatal("LEVELS_INIT", "lnsublev smaller than number of levels");
+  if (nsublev[*nl - 1] != 1) fatal("LEVELS_INIT", "Number of subdomains at last level must be 1.");
+  S.nl = *nl;
+  S.nsub = nsublev[0];
+  S.numbase = *numbase;
+  S.verbose = *verbose_level;
+  S.subs.assign(S.nsub, Sub());
+  *nsub_loc_1 = S.nsub;
+  S.initialised = true;
+}
+catch (const Error &e) {
+  record_error(e);
+}
+
+void bddcml_upload_global_data_c(int *, int *, int *, int *, int *, int *, int *, int *, int *,
+                                 int *, int *, double *, int *, int *, int *, int *, double *,
+                                 int *, double *, int *, double *, int *, int *, int *, int *) try {
+  fatal("BDDCML_UPLOAD_GLOBAL_DATA",
+        "global loading needs a mesh partitioner; upload subdomains with bddcml_upload_subdomain_data");
+}
+catch (const Error &e) {
+  record_error(e);
+}
+
+void bddcml_upload_subdomain_data_c(
+    int *nelem, int *nnod, int *ndof, int *ndim, int *meshdim, int *isub, int *nelems,
+    int *nnods, int *ndofs, int *inet, int *linet, int *nnet, int *lnnet, int *nndf, int *lnndf,
+    int *isngn, int *lisngn, int *isvgvn, int *lisvgvn, int *isegn, int *lisegn, double *xyz,
+    int *lxyz1, int *lxyz2, int *ifix, int *lifix, double *fixv, int *lfixv, double *rhs,
+    int *lrhs, int *is_rhs_complete, double *sol, int *lsol, int *matrixtype, int *i_sparse,
+    int *j_sparse, double *a_sparse, int *la, int *is_assembled_int, double *user_constraints,
+    int *luser_constraints1, int *luser_constraints2, double *element_data,
+    int *lelement_data1, int *lelement_data2, double *dof_data, int *ldof_data,
+    int *find_components_int, int *use_dual_mesh_graph_int, int *neighbouring) try {
+  (void)isegn;
+  (void)lisegn;
+  (void)user_constraints;
+  (void)element_data;
+  (void)lelement_data1;
+  (void)lelement_data2;
+  (void)is_assembled_int;
+  const char *R = "LEVELS_UPLOAD_SUBDOMAIN_DATA";
+  Solver &S = solver();
+  if (!S.initialised) fatal(R, "bddcml_init was not called");
+  S.nelem_g = *nelem;
+  S.nnod_g = *nnod;
+  S.ndof_g = *ndof;
+  S.ndim = *ndim;
+  S.meshdim = *meshdim;
+  if (*matrixtype != 1)
+    fatal(R, "the device path supports matrixtype 1 (symmetric positive definite)");
+  if (S.matrixtype >= 0 && S.matrixtype != *matrixtype) fatal(R, "matrixtype differs between subdomains");
+  S.matrixtype = *matrixtype;
+  if (*luser_constraints1 * *luser_constraints2 > 0)
+    fatal(R, "user constraints are not supported by this library");
+  Sub &s = local_sub(S, *isub, R);
+  if (s.uploaded) fatal(R, "It appears that mesh was already loaded for subdomain " + std::to_string(*isub));
+  const int b = S.numbase;
+  s = Sub();
+  s.isub = *isub - b;
+  s.nelems = *nelems;
+  s.nnods = *nnods;
+  s.ndofs = *ndofs;
+  if (*lnnet != *nelems || *lnndf != *nnods || *lisngn != *nnods || *lisvgvn != *ndofs)
+    fatal(R, "array length mismatch");
+  if (*lxyz1 != *nnods || *lxyz2 != *ndim) fatal(R, "coordinate array dimensions mismatch");
+  if (*lifix != *ndofs || *lfixv != *ndofs || *lrhs != *ndofs || *lsol != *ndofs)
+    fatal(R, "length of IFIX, FIXV, RHS or SOL differs from NDOFS");
+  s.inet.resize(*linet);
+  for (int k = 0; k < *linet; k++) s.inet[k] = inet[k] - b;
+  s.nnet.assign(nnet, nnet + *nelems);
+  s.nndf.assign(nndf, nndf + *nnods);
+  s.isngn.resize(*nnods);
+  for (int k = 0; k < *nnods; k++) s.isngn[k] = isngn[k] - b;
+  s.isvgvn.resize(*ndofs);
+  for (int k = 0; k < *ndofs; k++) s.isvgvn[k] = isvgvn[k] - b;
+  s.xyz.assign(xyz, xyz + (size_t)*nnods * *ndim);
+  s.ifix.assign(ifix, ifix + *ndofs);
+  s.fixv.assign(fixv, fixv + *ndofs);
+  s.rhs.assign(rhs, rhs + *ndofs);
+  s.sol.assign(sol, sol + *ndofs);
+  s.has_fix = std::any_of(s.ifix.begin(), s.ifix.end(), [](int v) { return v != 0; });
+  s.rhs_complete = *is_rhs_complete != 0;
+  s.dof_data.assign(dof_data, dof_data + *ldof_data);
+  s.find_components = *find_components_int;
+  s.use_dual = *use_dual_mesh_graph_int;
+  s.neighbouring = *neighbouring;
+  s.kdof.assign(*nnods + 1, 0);
+  for (int k = 0; k < *nnods; k++) s.kdof[k + 1] = s.kdof[k] + nndf[k];
+  if (s.kdof[*nnods] != *ndofs) fatal(R, "sum of NNDF differs from NDOFS");
+  // assemble triplets: sort by (row, col), sum duplicates (src/module_sm.f90:630-685)
+  const int n = *la;
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  for (int k = 0; k < n; k++)
+    if (i_sparse[k] > j_sparse[k])
+      fatal(R, "Matrix in wrong format - only upper triangle is expected for symmetric problems, matrixtype = 1");
+  std::sort(ord.begin(), ord.end(), [&](int x, int y) {
+    return i_sparse[x] != i_sparse[y] ? i_sparse[x] < i_sparse[y] : j_sparse[x] < j_sparse[y];
+  });
+  s.ai.reserve(n);
+  s.aj.reserve(n);
+  s.av.reserve(n);
+  for (int q = 0; q < n; q++) {
+    int k = ord[q];
+    int i = i_sparse[k] - b, j = j_sparse[k] - b;
+    if (i < 0 || j < 0 || i >= *ndofs || j >= *ndofs) fatal(R, "matrix index out of range");
+    if (!s.ai.empty() && s.ai.back() == i && s.aj.back() == j)
+      s.av.back() += a_sparse[k];
+    else
+      s.ai.push_back(i), s.aj.push_back(j), s.av.push_back(a_sparse[k]);
+  }
+  s.uploaded = true;
+}
+catch (const Error &e) {
+  record_error(e);
+}
+
+void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
+                                   int *parallel_division_int, int *use_corner_constraints_int,
+                                   int *use_arithmetic_constraints_int,
+                                   int *use_adaptive_constraints_int,
+                                   int *use_user_constraints_int, int *weights_type) try {
+  (void)parallel_division_int;
+  const char *R = "BDDCML_SETUP_PRECONDITIONER";
+  Solver &S = solver();
+  double t0 = wall_time();
+  if (*matrixtype != 1) fatal(R, "the device path supports matrixtype 1 (symmetric positive definite)");
+  for (auto &s : S.subs)
+    if (!s.uploaded) fatal("LEVELS_PC_SETUP", "not all subdomains were uploaded");
+  if (*use_defaults_int) {
+    S.use_corner = 1;
+    S.use_arith = 1;
+    S.weights_type = 0;
+  } else {
+    if (*use_adaptive_constraints_int) fatal(R, "adaptive constraints are not supported by this library");
+    if (*use_user_constraints_int) fatal(R, "user constraints are not supported by this library");
+    S.use_corner = *use_corner_constraints_int != 0;
+    S.use_arith = *use_arithmetic_constraints_int != 0;
+    S.weights_type = *weights_type;
+  }
+  build_topology(S);
+  apply_boundary_conditions(S);
+  build_weights(S);
+  build_constraints(S);
+  build_condensation_rhs(S);
+  build_unique_interface(S);
+  build_orderings(S);
+  S.t.topology += wall_time() - t0;
+  if (S.plan_only) {
+    S.is_setup = true;
+    return;
+  }
+  device_setup(S);
+  double t1 = wall_time();
+  device_reduced_rhs(S);
+  S.t.rhs += wall_time() - t1;
+  S.is_setup = true;
+  S.t.setup_total = wall_time() - t0;
+  if (S.verbose > 0)
+    std::printf(" BDDCML_SETUP_PRECONDITIONER: ncorner %d nedge %d nface %d ndofc %d interface %d\n",
+                S.ncorner, S.nedge, S.nface, S.ndofc, S.nig);
+}
+catch (const Error &e) {
+  record_error(e);
+}
+
+void bddcml_solve_c(int *comm_all, int *method, double *tol, int *maxit, int *ndecrmax,
+                    int *recycling_int, int *max_number_of_stored_vectors, int *num_iter,
+                    int *converged_reason, double *condition_number) try {
+  (void)comm_all;
+  (void)max_number_of_stored_vectors;
+  Solver &S = solver();
+  if (!S.is_setup) fatal("BDDCML_SOLVE", "preconditioner not set up");
+  int m = 1;
+  double tl = 1e-6;
+  int mi = 1000, nd = 30;
+  if (*method != -1) {
+    m = *method;
+    tl = *tol;
+    mi = *maxit;
+    nd = *ndecrmax;
+    if (*recycling_int) {
+      // Krylov recycling is not implemented: the solve proceeds without it
+      if (S.verbose > 0) std::fprintf(stderr, "WARNING in BDDCML_SOLVE: recycling ignored\n");
+    }
+  }
+  if (m != 0 && m != 1) fatal("BDDCML_SOLVE", "unknown iterative method " + std::to_string(m));
+  double t0 = wall_time();
+  device_solve(S, m, tl, mi, nd, num_iter, converged_reason, condition_number);
+  S.t.solve_total = wall_time() - t0;
+  S.last_iters = *num_iter;
+  S.last_reason = *converged_reason;
+  S.last_cond = *condition_number;
+  S.solved = true;
+}
+catch (const Error &e) {
+  record_error(e);
+}
+
+void bddcml_download_local_solution_c(int *isub, double *sols, int *lsols) try {
+  Solver &S = solver();
+  Sub &s = local_sub(S, *isub, "LEVELS_DD_DOWNLOAD_SOLUTION");
+  if (*lsols != s.ndofs) fatal("LEVELS_DD_DOWNLOAD_SOLUTION", "solution size mismatch");
+  std::copy(s.sol.begin(), s.sol.end(), sols);
+}
+catch (const Error &e) {
+  record_error(e);
+}
+
+void bddcml_download_global_solution_c(double *sol, int *lsol) try {
+  Solver &S = solver();
+  if (*lsol != S.ndof_g) fatal("LEVELS_GET_GLOBAL_SOLUTION", "solution size mismatch");
+  for (auto &s : S.subs)
+    for (int d = 0; d < s.ndofs; d++) sol[s.isvgvn[d]] = s.sol[d];
+}
+catch (const Error &e) {
+  record_error(e);
+}
+
+// reactions at Dirichlet dofs: r = A u - f with the original (uneliminated)
+// matrix is not kept by this library
+void bddcml_download_local_reactions_c(int *, double *, int *) try {
+  fatal("BDDCML_DOWNLOAD_LOCAL_REACTIONS", "reactions are not supported by this library");
+}
+catch (const Error &e) {
+  record_error(e);
+}
+void bddcml_download_global_reactions_c(double *, int *) try {
+  fatal("BDDCML_DOWNLOAD_GLOBAL_REACTIONS", "reactions are not supported by this library");
+}
+catch (const Error &e) {
+  record_error(e);
+}
+
+void bddcml_change_global_data_c(int *, int *, double *, int *, double *, int *, double *, int *) try {
+  fatal("LEVELS_CHANGE_GLOBAL_DATA", "Global loading should be used.");
+}
+catch (const Error &e) {
+  record_error(e);
+}
+
+void bddcml_change_subdomain_data_c(int *isub, int *ifix, int *lifix, double *fixv, int *lfixv,
+                                    double *rhs, int *lrhs, int *is_rhs_complete, double *sol,
+                                    int *lsol) try {
+  const char *R = "LEVELS_CHANGE_SUBDOMAIN_DATA";
+  Solver &S = solver();
+  if (!S.is_setup) fatal(R, "Cannot change data at this moment, level is not ready.");
+  Sub &s = local_sub(S, *isub, R);
+  if (*lifix != s.ndofs || *lfixv != s.ndofs || *lrhs != s.ndofs || *lsol != s.ndofs)
+    fatal(R, "array length mismatch");
+  for (int d = 0; d < s.ndofs; d++)
+    if ((ifix[d] != 0) != (s.ifix[d] != 0))
+      fatal("DD_UPLOAD_BC", "Indices of fixed variables IFIX cannot be changed for subdomain " +
+                                std::to_string(*isub));
+  s.fixv.assign(fixv, fixv + s.ndofs);
+  s.rhs.assign(rhs, rhs + s.ndofs);
+  s.sol.assign(sol, sol + s.ndofs);
+  s.rhs_complete = *is_rhs_complete != 0;
+  s.uploaded = true;
+  s.bc.clear();  // marks the subdomain for BC recomputation
+}
+catch (const Error &e) {
+  record_error(e);
+}
+
+void bddcml_setup_new_data_c() try {
+  fatal("LEVELS_SETUP_NEW_DATA", "re-solving with new data is not supported yet");
+}
+catch (const Error &e) {
+  record_error(e);
+}
+
+void bddcml_dotprod_subdomain_c(int *isub, double *vec1, int *lvec1, double *vec2, int *lvec2,
+                                double *dotprod) try {
+  Solver &S = solver();
+  Sub &s = local_sub(S, *isub, "LEVELS_DD_DOTPROD_SUBDOMAIN_LOCAL");
+  if (*lvec1 != s.ndofs || *lvec2 != s.ndofs)
+    fatal("LEVELS_DD_DOTPROD_SUBDOMAIN_LOCAL", "vector length mismatch");
+  std::vector<double> w(s.ndofs, 1.0);
+  for (size_t k = 0; k < s.ifc_dof.size(); k++) w[s.ifc_dof[k]] = s.w[k];
+  double sum = 0.0;
+  for (int d = 0; d < s.ndofs; d++) sum += vec1[d] * w[d] * vec2[d];
+  *dotprod = sum;
+}
+catch (const Error &e) {
+  record_error(e);
+}
+
+void bddcml_finalize_c() try { reset_solver(); }
+catch (const Error &e) {
+  record_error(e);
+}
+
+// ------------------------------------------------------------- extra ABI
+int bddcml_amd_counts(int *out) try {
+  Solver &S = solver();
+  if (!S.is_setup) return -1;
+  out[0] = S.ncorner;
+  out[1] = S.nedge;
+  out[2] = S.nface;
+  out[3] = S.ndofc;
+  out[4] = S.nig;
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+int bddcml_amd_interface_dofs(int *out) try {
+  Solver &S = solver();
+  if (!S.is_setup) return -1;
+  std::copy(S.igdof.begin(), S.igdof.end(), out);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+int bddcml_amd_apply_schur(const double *x, double *y) try {
+  Solver &S = solver();
+  if (!S.is_setup || S.plan_only || !S.dev) return -1;
+  device_apply_schur(S, x, y);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+int bddcml_amd_apply_pc(const double *r, double *z) try {
+  Solver &S = solver();
+  if (!S.is_setup || S.plan_only || !S.dev) return -1;
+  device_apply_pc(S, r, z);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+int bddcml_amd_reduced_rhs(double *g) try {
+  Solver &S = solver();
+  if (!S.is_setup || S.plan_only || !S.dev) return -1;
+  device_reduced_rhs_download(S, g);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+int bddcml_amd_interface_solution(double *u) try {
+  Solver &S = solver();
+  if (!S.solved) return -1;
+  std::copy(S.uig.begin(), S.uig.end(), u);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+int bddcml_amd_residual_history(double *out, int maxn) try {
+  Solver &S = solver();
+  int n = (int)S.hist.size();
+  if (out)
+    for (int k = 0; k < std::min(n, maxn); k++) out[k] = S.hist[k];
+  return n;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+int bddcml_amd_timings(double *out, int n) try {
+  Solver &S = solver();
+  double v[7] = {S.t.topology, S.t.plan, S.t.factor, S.t.krylov, S.t.postprocess, S.t.setup_total,
+                 S.t.solve_total};
+  for (int k = 0; k < std::min(n, 7); k++) out[k] = v[k];
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+int bddcml_amd_set_option(const char *key, double value) try {
+  Solver &S = solver();
+  if (!std::strcmp(key, "throw_on_error")) {
+    g_throw_on_error = value != 0.0;
+    return 0;
+  }
+  if (!std::strcmp(key, "verbose")) {
+    S.verbose = (int)value;
+    return 0;
+  }
+  if (!std::strcmp(key, "plan_only")) {
+    S.plan_only = value != 0.0;
+    return 0;
+  }
+  return -1;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+
+static bool device_ready(Solver &S) { return S.is_setup && !S.plan_only && S.dev; }
+
+int bddcml_amd_sub_info(int isub, int *out) try {
+  Solver &S = solver();
+  if (!S.is_setup) return -1;
+  int is = isub - S.numbase;
+  if (is < 0 || is >= S.nsub) return -2;
+  const Sub &s = S.subs[is];
+  out[0] = (int)s.ifc_dof.size();
+  out[1] = s.nconstr;
+  out[2] = (int)s.int_dof.size();
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+int bddcml_amd_sub_maps(int isub, int *gdofs, int *indrowc) try {
+  Solver &S = solver();
+  if (!S.is_setup) return -1;
+  int is = isub - S.numbase;
+  if (is < 0 || is >= S.nsub) return -2;
+  const Sub &s = S.subs[is];
+  for (size_t k = 0; k < s.ifc_dof.size(); k++) gdofs[k] = s.isvgvn[s.ifc_dof[k]];
+  for (int c = 0; c < s.nconstr; c++) indrowc[c] = s.indrowc[c];
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+int bddcml_amd_sub_matrix(int isub, int which, double *out) try {
+  Solver &S = solver();
+  if (!device_ready(S)) return -1;
+  int is = isub - S.numbase;
+  if (is < 0 || is >= S.nsub || (which != 'S' && which != 'K')) return -2;
+  device_sub_matrix(S, is, which, out);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+int bddcml_amd_sub_coarse(int isub, double *phi, double *ac) try {
+  Solver &S = solver();
+  if (!device_ready(S)) return -1;
+  int is = isub - S.numbase;
+  if (is < 0 || is >= S.nsub) return -2;
+  device_sub_coarse(S, is, phi, ac);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+
+int bddcml_amd_last_error(char *buf, int len) {
+  int n = (int)g_last_error.size();
+  if (buf && len > 0) {
+    int m = std::min(n, len - 1);
+    std::memcpy(buf, g_last_error.data(), m);
+    buf[m] = 0;
+  }
+  g_last_error.clear();
+  return n;
+}
+
+int bddcml_amd_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
+                             const double *X, double alpha, double beta, double *out) try {
+  device_selftest_tile(op, npairs, A, B, C, X, alpha, beta, out);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e);
+  return -100;
+}
+
+}  // extern "C"

@@ -1,0 +1,1157 @@
+// device.hip -- device plan and pipelines of the MI355X BDDC path.
+//
+// All subdomains of the level live on one GPU as batched slots.  Setup
+// (the reference's bddcml_setup_preconditioner numerics,
+// src/module_levels.f90:2500-3169):
+//   1. scatter the eliminated subdomain matrices into a block-sparse tile
+//      pool (interior dofs first, interface last),
+//   2. batched left-looking tile Cholesky of the interior block; the trailing
+//      interface block becomes the Schur complement S_i
+//      (dd_prepare_schur, src/module_dd.f90:4175-4296),
+//   3. augmented interface matrix S'_i = S_i + C_i^T Omega C_i, its Cholesky
+//      factor and inverse give K_i (interface block of [S C^T; C 0]^-1),
+//      the coarse basis Phi_i and the coarse element matrix Ac_i
+//      (dd_prepare_aug / dd_prepare_coarse, src/module_dd.f90:5211-5800),
+//   4. coarse matrix assembly, Cholesky and inverse
+//      (levels_prepare_last_level, src/module_levels.f90:3001-3169).
+// Solve: PCG / BiCGSTAB on the unique interface vector with the explicit
+// S_i and K_i applied as batched tile GEMVs, the interface exchange as a
+// deterministic gather/scatter (src/module_levels.f90:3627-4507,
+// src/module_krylov.f90:48-1505).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "kernels.hpp"
+#include "solver.hpp"
+
+namespace bddc {
+
+#define HIPCHECK(x)                                                                       \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess)                                                                 \
+      fatal("BDDCML_DEVICE", std::string(#x) + " failed: " + hipGetErrorString(e_));      \
+  } while (0)
+
+double wall_time() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr, o.n = 0; }
+  DBuf &operator=(DBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p, n = o.n;
+      o.p = nullptr, o.n = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t cnt) {
+    release();
+    n = cnt;
+    HIPCHECK(hipMalloc(&p, std::max<size_t>(cnt, 1) * sizeof(T)));
+  }
+  void upload(const std::vector<T> &v, hipStream_t st) {
+    alloc(v.size());
+    if (!v.empty()) HIPCHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  }
+  void zero(hipStream_t st) {
+    if (n) HIPCHECK(hipMemsetAsync(p, 0, n * sizeof(T), st));
+  }
+};
+
+struct Phase {
+  DBuf<TileTask> tasks;
+  DBuf<TilePair> pairs;
+  int ntask = 0;
+  void upload(const TaskList &l, hipStream_t st) {
+    ntask = (int)l.tasks.size();
+    tasks.upload(l.tasks, st);
+    pairs.upload(l.pairs, st);
+  }
+};
+
+static inline int lowidx(int n, int I, int J) { return J * n - J * (J - 1) / 2 + (I - J); }
+
+struct SubTiles {
+  int nto = 0, nti = 0;
+  TilePattern pat;   // A pattern (lower), tile ids absolute
+  std::vector<std::vector<uint64_t>> col;  // column bitsets
+  int linvA = 0, sfull = 0, pbase = 0, linvP = 0, wbase = 0, kfull = 0;
+  int P(int I, int J) const { return pbase + lowidx(nti, I, J); }
+  int Wt(int I, int J) const { return I == J ? linvP + I : wbase + lowidx(nti - 1, I - 1, J); }
+};
+
+int TilePattern::find(int I, int J) const {
+  auto b = rowidx.begin() + colptr[J], e = rowidx.begin() + colptr[J + 1];
+  auto it = std::lower_bound(b, e, I);
+  if (it == e || *it != I) return -1;
+  return tile[it - rowidx.begin()];
+}
+
+struct Device {
+  hipStream_t st = nullptr;
+  DBuf<double> pool;
+  int64_t ntiles = 0;
+  DBuf<int> err;
+  std::vector<SubTiles> sub;
+  // setup phases
+  std::vector<Phase> a_diag, a_off, p_diag, p_off, w_step, c_diag, c_off, cw_step;
+  Phase a_schur, kraw, cinv;
+  DBuf<int> zero_ids;
+  DBuf<int64_t> sc_idx;
+  DBuf<double> sc_val;
+  DBuf<MirrorJob> mir_s, mir_k, mir_c;
+  DBuf<int> sp_sub, sp_ij, sp_dst, ku_sub, ku_ij;
+  int n_sp = 0, n_ku = 0;
+  // interface data
+  std::vector<SubDesc> hdesc;
+  DBuf<SubDesc> desc;
+  DBuf<int64_t> loff;
+  DBuf<int> NIs, map, cid, csize, cptr, cdof, cmap;
+  DBuf<double> w, Z, Phi, G, Acs;
+  int maxNI = 0, maxnc = 0;
+  int64_t LV = 0;
+  DBuf<GemvRow> srows, krows, crows;
+  int nsrows = 0, nkrows = 0, ncrows = 0;
+  DBuf<int> aptr;
+  DBuf<int64_t> asrc;
+  DBuf<int> cgptr, cgsrc;
+  // coarse
+  int ntc = 0, NC = 0, cbase = 0, clinv = 0, cwbase = 0, cfull = 0;
+  DBuf<int64_t> ca_dst, ca_src;
+  DBuf<int> ca_ptr;
+  int nca = 0;
+  DBuf<int64_t> cloff;
+  DBuf<int> cNI;
+  // triangular sweeps
+  std::vector<TrsvDesc> htrsv;
+  DBuf<TrsvDesc> trsv;
+  DBuf<int> rptr, rk, rt, tcptr, tck, tct;
+  DBuf<double> vec;
+  int64_t VT = 0;
+  DBuf<int64_t> rhs_src;
+  // vectors
+  DBuf<double> yl, zl, rcs, rc, uc, partial, dots;
+  DBuf<double> u, r, p, z, ap, rs, v, y, s, t, g, xin, xout;
+  DBuf<char> fixmask;
+  int nig = 0;
+};
+
+Solver::Solver() = default;
+Solver::~Solver() = default;
+
+void device_release(Solver &S) {
+  if (S.dev) {
+    if (S.dev->st) (void)hipStreamDestroy(S.dev->st);
+    S.dev->st = nullptr;
+    S.dev.reset();
+  }
+}
+
+// ------------------------------------------------------- symbolic planning
+static void bs_set(std::vector<uint64_t> &b, int i) { b[i >> 6] |= (1ull << (i & 63)); }
+static bool bs_get(const std::vector<uint64_t> &b, int i) { return (b[i >> 6] >> (i & 63)) & 1ull; }
+
+// Block-sparse lower pattern of the device-ordered subdomain matrix and its
+// fill under elimination of the interior tile columns.
+static void symbolic(const Sub &s, SubTiles &T) {
+  T.nto = s.nto;
+  T.nti = s.nti;
+  const int nt = s.nto + s.nti, nw = (nt + 63) / 64;
+  T.col.assign(nt, std::vector<uint64_t>(nw, 0ull));
+  for (int J = 0; J < nt; J++) bs_set(T.col[J], J);
+  for (size_t k = 0; k < s.av.size(); k++) {
+    int a = s.perm[s.ai[k]], b = s.perm[s.aj[k]];
+    int I = std::max(a, b) / TILE, J = std::min(a, b) / TILE;
+    bs_set(T.col[J], I);
+  }
+  for (int J = s.nto; J < nt; J++)
+    for (int I = J; I < nt; I++) bs_set(T.col[J], I);
+  for (int J = 0; J < s.nto; J++) {
+    std::vector<int> R;
+    for (int I = J + 1; I < nt; I++)
+      if (bs_get(T.col[J], I)) R.push_back(I);
+    for (size_t a = 0; a < R.size(); a++)
+      for (size_t b = a; b < R.size(); b++) bs_set(T.col[R[a]], R[b]);
+  }
+}
+
+static void assign_tiles(SubTiles &T, int &next) {
+  const int nt = T.nto + T.nti;
+  T.pat.nt = nt;
+  T.pat.ntf = T.nto;
+  T.pat.colptr.assign(nt + 1, 0);
+  T.pat.rowidx.clear();
+  T.pat.tile.clear();
+  for (int J = 0; J < nt; J++) {
+    for (int I = J; I < nt; I++)
+      if (bs_get(T.col[J], I)) {
+        T.pat.rowidx.push_back(I);
+        T.pat.tile.push_back(next++);
+      }
+    T.pat.colptr[J + 1] = (int)T.pat.rowidx.size();
+  }
+  T.linvA = next;
+  next += T.nto;
+  T.sfull = next;
+  next += T.nti * T.nti;
+  T.pbase = next;
+  next += T.nti * (T.nti + 1) / 2;
+  T.linvP = next;
+  next += T.nti;
+  T.wbase = next;
+  next += T.nti * (T.nti - 1) / 2;
+  T.kfull = next;
+  next += T.nti * T.nti;
+}
+
+// tasks of a dense lower Cholesky on tiles P(I,J) with inverse tiles Linv(J)
+template <class FP, class FL>
+static void dense_chol_tasks(int n, FP Ptile, FL Linv, std::vector<TaskList> &diag,
+                             std::vector<TaskList> &off) {
+  if ((int)diag.size() < n) diag.resize(n), off.resize(n);
+  for (int J = 0; J < n; J++) {
+    TileTask t{Ptile(J, J), Linv(J), (int)diag[J].pairs.size(), 0};
+    for (int k = 0; k < J; k++) diag[J].pairs.push_back({Ptile(J, k), Ptile(J, k)});
+    t.p1 = (int)diag[J].pairs.size();
+    diag[J].tasks.push_back(t);
+    for (int I = J + 1; I < n; I++) {
+      TileTask u{Ptile(I, J), Linv(J), (int)off[J].pairs.size(), 0};
+      for (int k = 0; k < J; k++) off[J].pairs.push_back({Ptile(I, k), Ptile(J, k)});
+      u.p1 = (int)off[J].pairs.size();
+      off[J].tasks.push_back(u);
+    }
+  }
+}
+
+// W = L^{-1} (strict lower tiles, diagonal = Linv) and Kraw = W^T W
+template <class FP, class FW>
+static void inverse_tasks(int n, FP Ptile, FW Wt, std::vector<TaskList> &wstep, TaskList &kraw) {
+  if ((int)wstep.size() < n) wstep.resize(n);
+  for (int i = 1; i < n; i++)
+    for (int j = 0; j < i; j++) {
+      TileTask t{Wt(i, j), Wt(i, i), (int)wstep[i].pairs.size(), 0};
+      for (int k = j; k < i; k++) wstep[i].pairs.push_back({Ptile(i, k), Wt(k, j)});
+      t.p1 = (int)wstep[i].pairs.size();
+      wstep[i].tasks.push_back(t);
+    }
+  for (int j = 0; j < n; j++)
+    for (int i = j; i < n; i++) {
+      TileTask t{Ptile(i, j), -1, (int)kraw.pairs.size(), 0};
+      for (int k = i; k < n; k++) kraw.pairs.push_back({Wt(k, i), Wt(k, j)});
+      t.p1 = (int)kraw.pairs.size();
+      kraw.tasks.push_back(t);
+    }
+}
+
+// Host-side bounds check of every tile id a task list hands to the kernels.
+static void check_tasks(const TaskList &l, int64_t ntiles, bool uses_x, const char *what) {
+  auto bad = [&](int id) { return id < 0 || id >= ntiles; };
+  for (const TileTask &t : l.tasks) {
+    if (bad(t.c) || (uses_x && bad(t.x)) || t.p0 < 0 || t.p1 < t.p0 || t.p1 > (int)l.pairs.size())
+      fatal("BDDCML_DEVICE_PLAN", std::string("invalid task in ") + what);
+  }
+  for (const TilePair &p : l.pairs)
+    if (bad(p.a) || bad(p.b)) fatal("BDDCML_DEVICE_PLAN", std::string("invalid tile pair in ") + what);
+}
+
+static void upload_phases(std::vector<Phase> &dst, const std::vector<TaskList> &src,
+                          hipStream_t st) {
+  dst.resize(src.size());
+  for (size_t k = 0; k < src.size(); k++) dst[k].upload(src[k], st);
+}
+
+// ------------------------------------------------------------------ setup
+void device_setup(Solver &S) {
+  device_release(S);
+  S.dev.reset(new Device());
+  Device &D = *S.dev;
+  HIPCHECK(hipStreamCreateWithFlags(&D.st, hipStreamNonBlocking));
+  hipStream_t st = D.st;
+  const int nsub = S.nsub;
+  double t0 = wall_time();
+
+  // ---- symbolic patterns and tile ids
+  D.sub.resize(nsub);
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < nsub; is++) symbolic(S.subs[is], D.sub[is]);
+  int next = 0;
+  for (int is = 0; is < nsub; is++) assign_tiles(D.sub[is], next);
+  D.NC = std::max(TILE, (S.ndofc + TILE - 1) / TILE * TILE);
+  D.ntc = D.NC / TILE;
+  D.cbase = next;
+  next += D.ntc * (D.ntc + 1) / 2;
+  D.clinv = next;
+  next += D.ntc;
+  D.cwbase = next;
+  next += D.ntc * (D.ntc - 1) / 2;
+  D.cfull = next;
+  next += D.ntc * D.ntc;
+  D.ntiles = next;
+
+  // ---- task lists
+  std::vector<TaskList> a_diag, a_off, p_diag, p_off, w_step, c_diag, c_off, cw_step;
+  TaskList a_schur, kraw, cinv;
+  for (int is = 0; is < nsub; is++) {
+    const SubTiles &T = D.sub[is];
+    const int nto = T.nto, nt = T.nto + T.nti;
+    // row sets restricted to factored columns
+    std::vector<std::vector<int>> rowk(nt);
+    for (int J = 0; J < nto; J++)
+      for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) rowk[T.pat.rowidx[q]].push_back(J);
+    if ((int)a_diag.size() < nto) a_diag.resize(nto), a_off.resize(nto);
+    auto common = [&](int I, int J, int kmax, TaskList &L) {
+      const auto &a = rowk[I], &b = rowk[J];
+      size_t x = 0, y = 0;
+      while (x < a.size() && y < b.size()) {
+        if (a[x] >= kmax || b[y] >= kmax) break;
+        if (a[x] < b[y]) x++;
+        else if (b[y] < a[x]) y++;
+        else {
+          L.pairs.push_back({T.pat.find(I, a[x]), T.pat.find(J, b[y])});
+          x++, y++;
+        }
+      }
+    };
+    for (int J = 0; J < nto; J++) {
+      TileTask t{T.pat.find(J, J), T.linvA + J, (int)a_diag[J].pairs.size(), 0};
+      common(J, J, J, a_diag[J]);
+      t.p1 = (int)a_diag[J].pairs.size();
+      a_diag[J].tasks.push_back(t);
+      for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) {
+        int I = T.pat.rowidx[q];
+        if (I == J) continue;
+        TileTask u{T.pat.tile[q], T.linvA + J, (int)a_off[J].pairs.size(), 0};
+        common(I, J, J, a_off[J]);
+        u.p1 = (int)a_off[J].pairs.size();
+        a_off[J].tasks.push_back(u);
+      }
+    }
+    for (int J = nto; J < nt; J++)
+      for (int I = J; I < nt; I++) {
+        TileTask t{T.pat.find(I, J), -1, (int)a_schur.pairs.size(), 0};
+        common(I, J, nto, a_schur);
+        t.p1 = (int)a_schur.pairs.size();
+        if (t.p1 > t.p0) a_schur.tasks.push_back(t);
+      }
+    const SubTiles *Tp = &D.sub[is];
+    dense_chol_tasks(
+        T.nti, [Tp](int I, int J) { return Tp->P(I, J); }, [Tp](int J) { return Tp->linvP + J; },
+        p_diag, p_off);
+    inverse_tasks(
+        T.nti, [Tp](int I, int J) { return Tp->P(I, J); },
+        [Tp](int I, int J) { return Tp->Wt(I, J); }, w_step, kraw);
+  }
+  {
+    const int n = D.ntc, cb = D.cbase, cl = D.clinv, cw = D.cwbase;
+    auto Pc = [=](int I, int J) { return cb + lowidx(n, I, J); };
+    auto Wc = [=](int I, int J) { return I == J ? cl + I : cw + lowidx(n - 1, I - 1, J); };
+    dense_chol_tasks(n, Pc, [=](int J) { return cl + J; }, c_diag, c_off);
+    inverse_tasks(n, Pc, Wc, cw_step, cinv);
+  }
+  for (auto &l : a_diag) check_tasks(l, D.ntiles, true, "interior Cholesky");
+  for (auto &l : a_off) check_tasks(l, D.ntiles, true, "interior panel");
+  for (auto &l : p_diag) check_tasks(l, D.ntiles, true, "interface Cholesky");
+  for (auto &l : p_off) check_tasks(l, D.ntiles, true, "interface panel");
+  for (auto &l : w_step) check_tasks(l, D.ntiles, true, "interface inverse");
+  for (auto &l : c_diag) check_tasks(l, D.ntiles, true, "coarse Cholesky");
+  for (auto &l : c_off) check_tasks(l, D.ntiles, true, "coarse panel");
+  for (auto &l : cw_step) check_tasks(l, D.ntiles, true, "coarse inverse");
+  check_tasks(a_schur, D.ntiles, false, "Schur block");
+  check_tasks(kraw, D.ntiles, false, "interface W^T W");
+  check_tasks(cinv, D.ntiles, false, "coarse W^T W");
+  upload_phases(D.a_diag, a_diag, st);
+  upload_phases(D.a_off, a_off, st);
+  upload_phases(D.p_diag, p_diag, st);
+  upload_phases(D.p_off, p_off, st);
+  upload_phases(D.w_step, w_step, st);
+  upload_phases(D.c_diag, c_diag, st);
+  upload_phases(D.c_off, c_off, st);
+  upload_phases(D.cw_step, cw_step, st);
+  D.a_schur.upload(a_schur, st);
+  D.kraw.upload(kraw, st);
+  D.cinv.upload(cinv, st);
+
+  // ---- scatter entries: A tiles (both triangles inside diagonal tiles),
+  // padding diagonals, coarse padding diagonals
+  std::vector<int> zero_ids;
+  std::vector<int64_t> sidx;
+  std::vector<double> sval;
+  for (int is = 0; is < nsub; is++) {
+    const Sub &s = S.subs[is];
+    const SubTiles &T = D.sub[is];
+    zero_ids.insert(zero_ids.end(), T.pat.tile.begin(), T.pat.tile.end());
+    auto put = [&](int a, int b, double v) {  // a >= b tile-wise lower
+      int I = a / TILE, J = b / TILE;
+      int tid = T.pat.find(I, J);
+      sidx.push_back((int64_t)tid * TILE_ELEMS + (b % TILE) * TILE + (a % TILE));
+      sval.push_back(v);
+    };
+    for (size_t k = 0; k < s.av.size(); k++) {
+      int a = s.perm[s.ai[k]], b = s.perm[s.aj[k]];
+      if (a < b) std::swap(a, b);
+      put(a, b, s.av[k]);
+      if (a != b && a / TILE == b / TILE) put(b, a, s.av[k]);
+    }
+    for (int p = s.no; p < s.NO; p++) put(p, p, 1.0);
+    for (int p = s.NO + s.ni; p < s.NO + s.NI; p++) put(p, p, 1.0);
+  }
+  {
+    const int n = D.ntc;
+    for (int J = 0; J < n; J++)
+      for (int I = J; I < n; I++) zero_ids.push_back(D.cbase + lowidx(n, I, J));
+    for (int p = S.ndofc; p < D.NC; p++) {
+      int T_ = D.cbase + lowidx(n, p / TILE, p / TILE);
+      sidx.push_back((int64_t)T_ * TILE_ELEMS + (p % TILE) * TILE + (p % TILE));
+      sval.push_back(1.0);
+    }
+  }
+  for (int id : zero_ids)
+    if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid tile to clear");
+  for (int64_t k : sidx)
+    if (k < 0 || k >= D.ntiles * (int64_t)TILE_ELEMS) fatal("BDDCML_DEVICE_PLAN", "scatter index out of range");
+  D.zero_ids.upload(zero_ids, st);
+  D.sc_idx.upload(sidx, st);
+  D.sc_val.upload(sval, st);
+
+  // ---- mirror jobs, S' jobs, K update jobs
+  std::vector<MirrorJob> ms, mk, mc;
+  std::vector<int> spsub, spij, spdst, kusub, kuij;
+  for (int is = 0; is < nsub; is++) {
+    const SubTiles &T = D.sub[is];
+    const int nto = T.nto, n = T.nti;
+    for (int J = 0; J < n; J++)
+      for (int I = J; I < n; I++) {
+        int src = T.pat.find(nto + I, nto + J);
+        ms.push_back({src, T.sfull + I * n + J, 0});
+        if (I > J) ms.push_back({src, T.sfull + J * n + I, 1});
+        mk.push_back({T.P(I, J), T.kfull + I * n + J, 0});
+        if (I > J) mk.push_back({T.P(I, J), T.kfull + J * n + I, 1});
+        spsub.push_back(is);
+        spij.push_back((I << 16) | J);
+        spdst.push_back(T.P(I, J));
+      }
+    for (int I = 0; I < n; I++)
+      for (int J = 0; J < n; J++) kusub.push_back(is), kuij.push_back((I << 16) | J);
+  }
+  {
+    const int n = D.ntc;
+    for (int J = 0; J < n; J++)
+      for (int I = J; I < n; I++) {
+        int src = D.cbase + lowidx(n, I, J);
+        mc.push_back({src, D.cfull + I * n + J, 0});
+        if (I > J) mc.push_back({src, D.cfull + J * n + I, 1});
+      }
+  }
+  for (auto *jobs : {&ms, &mk, &mc})
+    for (const MirrorJob &j : *jobs)
+      if (j.src < 0 || j.src >= D.ntiles || j.dst < 0 || j.dst >= D.ntiles)
+        fatal("BDDCML_DEVICE_PLAN", "invalid mirror job");
+  for (int d : spdst)
+    if (d < 0 || d >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid augmented tile");
+  D.mir_s.upload(ms, st);
+  D.mir_k.upload(mk, st);
+  D.mir_c.upload(mc, st);
+  D.sp_sub.upload(spsub, st);
+  D.sp_ij.upload(spij, st);
+  D.sp_dst.upload(spdst, st);
+  D.n_sp = (int)spsub.size();
+  D.ku_sub.upload(kusub, st);
+  D.ku_ij.upload(kuij, st);
+  D.n_ku = (int)kusub.size();
+
+  // ---- interface descriptors and maps
+  D.hdesc.resize(nsub);
+  std::vector<int64_t> loff(nsub);
+  std::vector<int> NIs(nsub);
+  int64_t lv = 0, zo = 0, go = 0;
+  int co = 0;
+  std::vector<int> map, cidv, csz, cptrv(1, 0), cdofv, cmapv;
+  std::vector<double> wv;
+  for (int is = 0; is < nsub; is++) {
+    const Sub &s = S.subs[is];
+    const SubTiles &T = D.sub[is];
+    SubDesc d;
+    d.ni = s.ni;
+    d.NI = s.NI;
+    d.nti = s.nti;
+    d.nc = s.nconstr;
+    d.sfull = T.sfull;
+    d.kfull = T.kfull;
+    d.loff = lv;
+    d.zoff = zo;
+    d.goff = go;
+    d.coff = co;
+    // rho: largest interface diagonal entry of the eliminated matrix
+    double rho = 0.0;
+    for (size_t k = 0; k < s.av.size(); k++)
+      if (s.ai[k] == s.aj[k] && s.perm[s.ai[k]] >= s.NO) rho = std::max(rho, std::fabs(s.av[k]));
+    d.rho = rho > 0.0 ? rho : 1.0;
+    D.hdesc[is] = d;
+    loff[is] = lv;
+    NIs[is] = s.NI;
+    std::vector<int> ref2dev(s.ni);
+    for (int k = 0; k < s.ni; k++) ref2dev[s.dev_ifc[k]] = k;
+    std::vector<int> cid_loc(s.NI, -1);
+    for (int c = 0; c < s.nconstr; c++) {
+      for (int q = s.cptr[c]; q < s.cptr[c + 1]; q++) {
+        int k = ref2dev[s.cidx[q]];
+        if (cid_loc[k] >= 0)
+          fatal("BDDCML_SETUP_PRECONDITIONER", "a dof belongs to two constraints; only corner and "
+                                               "arithmetic constraints are supported");
+        cid_loc[k] = co + c;
+        cdofv.push_back(k);
+      }
+      cptrv.push_back((int)cdofv.size());
+      csz.push_back(s.cptr[c + 1] - s.cptr[c]);
+      cmapv.push_back(s.indrowc[c]);
+    }
+    for (int k = 0; k < s.NI; k++) {
+      if (k < s.ni) {
+        int kr = s.dev_ifc[k];
+        map.push_back(S.gpos[s.isvgvn[s.ifc_dof[kr]]]);
+        wv.push_back(s.w[kr]);
+      } else {
+        map.push_back(-1);
+        wv.push_back(0.0);
+      }
+      cidv.push_back(cid_loc[k]);
+    }
+    lv += s.NI;
+    zo += (int64_t)s.NI * s.nconstr;
+    go += (int64_t)s.nconstr * s.nconstr;
+    co += s.nconstr;
+    D.maxNI = std::max(D.maxNI, s.NI);
+    D.maxnc = std::max(D.maxnc, s.nconstr);
+  }
+  if (D.maxNI > 7936) fatal("BDDCML_SETUP_PRECONDITIONER", "interface larger than 7936 dofs per subdomain");
+  if (D.maxnc > 1024) fatal("BDDCML_SETUP_PRECONDITIONER", "more than 1024 constraints per subdomain");
+  if (D.NC > 7936) fatal("BDDCML_SETUP_PRECONDITIONER", "coarse problem larger than 7936 dofs");
+  D.LV = lv;
+  D.desc.upload(D.hdesc, st);
+  D.loff.upload(loff, st);
+  D.NIs.upload(NIs, st);
+  D.map.upload(map, st);
+  D.w.upload(wv, st);
+  D.cid.upload(cidv, st);
+  D.csize.upload(csz, st);
+  D.cptr.upload(cptrv, st);
+  D.cdof.upload(cdofv, st);
+  D.cmap.upload(cmapv, st);
+  D.Z.alloc(std::max<int64_t>(zo, 1));
+  D.Phi.alloc(std::max<int64_t>(zo, 1));
+  D.G.alloc(std::max<int64_t>(go, 1));
+  D.Acs.alloc(std::max<int64_t>(go, 1));
+  // gemv rows
+  std::vector<GemvRow> sr, kr, cr;
+  for (int is = 0; is < nsub; is++) {
+    const SubTiles &T = D.sub[is];
+    for (int I = 0; I < T.nti; I++) {
+      sr.push_back({T.sfull + I * T.nti, T.nti, is, I});
+      kr.push_back({T.kfull + I * T.nti, T.nti, is, I});
+    }
+  }
+  for (int I = 0; I < D.ntc; I++) cr.push_back({D.cfull + I * D.ntc, D.ntc, 0, I});
+  for (auto *rows : {&sr, &kr, &cr})
+    for (const GemvRow &g : *rows)
+      if (g.tile0 < 0 || (int64_t)g.tile0 + g.nt > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid GEMV row");
+  D.srows.upload(sr, st);
+  D.krows.upload(kr, st);
+  D.crows.upload(cr, st);
+  D.nsrows = (int)sr.size();
+  D.nkrows = (int)kr.size();
+  D.ncrows = (int)cr.size();
+  D.cloff.upload(std::vector<int64_t>{0}, st);
+  D.cNI.upload(std::vector<int>{D.NC}, st);
+  // assembly CSR: unique interface dof -> local flat indices (subdomain order)
+  {
+    std::vector<int> cnt(S.nig + 1, 0);
+    for (int64_t k = 0; k < lv; k++)
+      if (map[k] >= 0) cnt[map[k] + 1]++;
+    for (int g = 0; g < S.nig; g++) cnt[g + 1] += cnt[g];
+    std::vector<int64_t> src(cnt[S.nig]);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t k = 0; k < lv; k++)
+      if (map[k] >= 0) src[fill[map[k]]++] = k;
+    D.aptr.upload(cnt, st);
+    D.asrc.upload(src, st);
+    // the same contributors in the device-ordered subdomain vectors
+    std::vector<int64_t> voff(nsub);
+    int64_t vt = 0;
+    for (int is = 0; is < nsub; is++) voff[is] = vt, vt += S.subs[is].NO + S.subs[is].NI;
+    D.VT = vt;
+    std::vector<int64_t> rsrc(src.size());
+    int isub = 0;
+    for (size_t q = 0; q < src.size(); q++) {
+      int64_t k = src[q];
+      // locate subdomain of local index k
+      int lo = 0, hi = nsub - 1;
+      while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (loff[mid] <= k) lo = mid;
+        else hi = mid - 1;
+      }
+      isub = lo;
+      rsrc[q] = voff[isub] + S.subs[isub].NO + (k - loff[isub]);
+    }
+    D.rhs_src.upload(rsrc, st);
+    // triangular sweep descriptors
+    std::vector<int> rp, rkv, rtv, cp, ckv, ctv;
+    D.htrsv.resize(nsub);
+    int maxnt = 1;
+    for (int is = 0; is < nsub; is++) {
+      const SubTiles &T = D.sub[is];
+      const int nt = T.nto + T.nti;
+      maxnt = std::max(maxnt, nt);
+      TrsvDesc d{T.nto, T.nti, (int)rp.size(), (int)cp.size(), T.linvA, voff[is]};
+      D.htrsv[is] = d;
+      std::vector<std::vector<std::pair<int, int>>> rows(nt);
+      for (int J = 0; J < T.nto; J++)
+        for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) {
+          int I = T.pat.rowidx[q];
+          if (I > J) rows[I].push_back({J, T.pat.tile[q]});
+        }
+      rp.push_back((int)rkv.size());
+      for (int I = 0; I < nt; I++) {
+        for (auto &e : rows[I]) rkv.push_back(e.first), rtv.push_back(e.second);
+        rp.push_back((int)rkv.size());
+      }
+      cp.push_back((int)ckv.size());
+      for (int J = 0; J < T.nto; J++) {
+        for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) {
+          int I = T.pat.rowidx[q];
+          if (I > J) ckv.push_back(I), ctv.push_back(T.pat.tile[q]);
+        }
+        cp.push_back((int)ckv.size());
+      }
+    }
+    if ((int64_t)(maxnt * 64 + 320) * 8 > 65536)
+      fatal("BDDCML_SETUP_PRECONDITIONER", "subdomain larger than the triangular sweep supports");
+    for (int id : rtv)
+      if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid sweep tile");
+    for (int id : ctv)
+      if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid sweep tile");
+    trsv_set_maxnt(maxnt);
+    D.trsv.upload(D.htrsv, st);
+    D.rptr.upload(rp, st);
+    D.rk.upload(rkv, st);
+    D.rt.upload(rtv, st);
+    D.tcptr.upload(cp, st);
+    D.tck.upload(ckv, st);
+    D.tct.upload(ctv, st);
+  }
+  // coarse gather CSR: coarse dof -> constraint rows (subdomain order)
+  {
+    std::vector<int> cnt(S.ndofc + 1, 0);
+    for (int v : cmapv) cnt[v + 1]++;
+    for (int k = 0; k < S.ndofc; k++) cnt[k + 1] += cnt[k];
+    std::vector<int> src(cnt[S.ndofc]);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (size_t k = 0; k < cmapv.size(); k++) src[fill[cmapv[k]]++] = (int)k;
+    D.cgptr.upload(cnt, st);
+    D.cgsrc.upload(src, st);
+  }
+  // coarse assembly CSR over the lower tiles of the coarse matrix
+  {
+    struct E {
+      int64_t dst, src;
+    };
+    std::vector<E> ent;
+    const int n = D.ntc;
+    for (int is = 0; is < nsub; is++) {
+      const Sub &s = S.subs[is];
+      const SubDesc &d = D.hdesc[is];
+      for (int a = 0; a < s.nconstr; a++)
+        for (int b = 0; b < s.nconstr; b++) {
+          int ga = s.indrowc[a], gb = s.indrowc[b];
+          int I = ga / TILE, J = gb / TILE;
+          if (I < J) continue;
+          int64_t dst = (int64_t)(D.cbase + lowidx(n, I, J)) * TILE_ELEMS + (gb % TILE) * TILE + ga % TILE;
+          ent.push_back({dst, d.goff + (int64_t)a * s.nconstr + b});
+        }
+    }
+    std::stable_sort(ent.begin(), ent.end(), [](const E &a, const E &b) { return a.dst < b.dst; });
+    std::vector<int64_t> dst, src;
+    std::vector<int> ptr(1, 0);
+    for (size_t k = 0; k < ent.size(); k++) {
+      if (dst.empty() || dst.back() != ent[k].dst) {
+        if (!dst.empty()) ptr.push_back((int)k);
+        dst.push_back(ent[k].dst);
+      }
+      src.push_back(ent[k].src);
+    }
+    ptr.push_back((int)ent.size());
+    D.ca_dst.upload(dst, st);
+    D.ca_src.upload(src, st);
+    D.ca_ptr.upload(ptr, st);
+    D.nca = (int)dst.size();
+  }
+  // vectors
+  D.nig = S.nig;
+  D.yl.alloc(lv);
+  D.zl.alloc(lv);
+  D.rcs.alloc(std::max(co, 1));
+  D.rc.alloc(D.NC);
+  D.uc.alloc(D.NC);
+  D.partial.alloc(DOT_BLOCKS * 4);
+  D.dots.alloc(4);
+  for (DBuf<double> *b : {&D.u, &D.r, &D.p, &D.z, &D.ap, &D.rs, &D.v, &D.y, &D.s, &D.t, &D.g,
+                          &D.xin, &D.xout})
+    b->alloc(std::max(S.nig, 1));
+  D.vec.alloc(D.VT);
+  D.fixmask.upload(std::vector<char>(S.ig_fixed.begin(), S.ig_fixed.end()), st);
+  D.err.alloc(1);
+  D.err.zero(st);
+  D.rc.zero(st);
+  D.uc.zero(st);
+  D.pool.alloc((size_t)D.ntiles * TILE_ELEMS);
+  HIPCHECK(hipStreamSynchronize(st));
+  S.t.plan += wall_time() - t0;
+
+  // ---- numerics
+  t0 = wall_time();
+  double *pool = D.pool.p;
+  launch_zero_tiles(st, pool, D.zero_ids.p, (int)D.zero_ids.n);
+  launch_scatter(st, pool, D.sc_idx.p, D.sc_val.p, (int64_t)D.sc_idx.n);
+  for (size_t J = 0; J < D.a_diag.size(); J++) {
+    launch_tile_potrf(st, pool, D.a_diag[J].tasks.p, D.a_diag[J].pairs.p, D.a_diag[J].ntask, D.err.p);
+    launch_tile_gemm(st, pool, D.a_off[J].tasks.p, D.a_off[J].pairs.p, D.a_off[J].ntask, GEMM_NT,
+                     POST_RIGHT_XT, 1.0, -1.0);
+  }
+  launch_tile_gemm(st, pool, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask, GEMM_NT,
+                   POST_NONE, 1.0, -1.0);
+  launch_tile_mirror(st, pool, D.mir_s.p, (int)D.mir_s.n);
+  // augmented interface matrices
+  launch_sprime(st, pool, D.desc.p, D.sp_sub.p, D.sp_ij.p, D.sp_dst.p, D.n_sp, D.cid.p, D.csize.p);
+  for (size_t J = 0; J < D.p_diag.size(); J++) {
+    launch_tile_potrf(st, pool, D.p_diag[J].tasks.p, D.p_diag[J].pairs.p, D.p_diag[J].ntask, D.err.p);
+    launch_tile_gemm(st, pool, D.p_off[J].tasks.p, D.p_off[J].pairs.p, D.p_off[J].ntask, GEMM_NT,
+                     POST_RIGHT_XT, 1.0, -1.0);
+  }
+  for (size_t i = 1; i < D.w_step.size(); i++)
+    launch_tile_gemm(st, pool, D.w_step[i].tasks.p, D.w_step[i].pairs.p, D.w_step[i].ntask, GEMM_NN,
+                     POST_LEFT_X, 0.0, -1.0);
+  launch_tile_gemm(st, pool, D.kraw.tasks.p, D.kraw.pairs.p, D.kraw.ntask, GEMM_TN, POST_NONE, 0.0,
+                   1.0);
+  launch_tile_mirror(st, pool, D.mir_k.p, (int)D.mir_k.n);
+  launch_z(st, pool, D.desc.p, nsub, D.cptr.p, D.cdof.p, D.Z.p, D.maxnc);
+  launch_g(st, D.desc.p, nsub, D.cptr.p, D.cdof.p, D.Z.p, D.G.p);
+  launch_ginv(st, D.desc.p, nsub, D.G.p, nullptr, D.err.p);
+  launch_phi(st, D.desc.p, nsub, D.Z.p, D.G.p, D.Phi.p);
+  launch_kupdate(st, pool, D.desc.p, D.ku_sub.p, D.ku_ij.p, D.n_ku, D.Phi.p, D.Z.p);
+  launch_acs(st, D.desc.p, nsub, D.cptr.p, D.G.p, D.Acs.p);
+  // coarse problem
+  launch_coarse_assemble(st, pool, D.ca_dst.p, D.ca_ptr.p, D.ca_src.p, D.nca, D.Acs.p);
+  for (size_t J = 0; J < D.c_diag.size(); J++) {
+    launch_tile_potrf(st, pool, D.c_diag[J].tasks.p, D.c_diag[J].pairs.p, D.c_diag[J].ntask, D.err.p);
+    launch_tile_gemm(st, pool, D.c_off[J].tasks.p, D.c_off[J].pairs.p, D.c_off[J].ntask, GEMM_NT,
+                     POST_RIGHT_XT, 1.0, -1.0);
+  }
+  for (size_t i = 1; i < D.cw_step.size(); i++)
+    launch_tile_gemm(st, pool, D.cw_step[i].tasks.p, D.cw_step[i].pairs.p, D.cw_step[i].ntask,
+                     GEMM_NN, POST_LEFT_X, 0.0, -1.0);
+  launch_tile_gemm(st, pool, D.cinv.tasks.p, D.cinv.pairs.p, D.cinv.ntask, GEMM_TN, POST_NONE, 0.0,
+                   1.0);
+  launch_tile_mirror(st, pool, D.mir_c.p, (int)D.mir_c.n);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(st));
+  int herr = 0;
+  HIPCHECK(hipMemcpy(&herr, D.err.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (herr & 1) fatal("BDDCML_SETUP_PRECONDITIONER", "matrix not positive definite in a tile Cholesky factorization");
+  if (herr & 2) fatal("BDDCML_SETUP_PRECONDITIONER", "singular constraint matrix C S'^{-1} C^T");
+  S.t.factor += wall_time() - t0;
+}
+
+// ------------------------------------------------------------- right-hand side
+void device_reduced_rhs(Solver &S) {
+  Device &D = *S.dev;
+  hipStream_t st = D.st;
+  std::vector<double> hv(D.VT, 0.0);
+  for (int is = 0; is < S.nsub; is++) {
+    const Sub &s = S.subs[is];
+    int64_t vo = D.htrsv[is].voff;
+    for (int d = 0; d < s.ndofs; d++) hv[vo + s.perm[d]] = s.fmod[d];
+  }
+  HIPCHECK(hipMemcpyAsync(D.vec.p, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  launch_trsv_fwd(st, D.pool.p, D.trsv.p, S.nsub, D.rptr.p, D.rk.p, D.rt.p, D.vec.p);
+  launch_assemble(st, S.nig, D.aptr.p, D.rhs_src.p, nullptr, D.vec.p, D.g.p);
+  HIPCHECK(hipStreamSynchronize(st));
+}
+
+void device_reduced_rhs_download(Solver &S, double *g) {
+  Device &D = *S.dev;
+  HIPCHECK(hipMemcpy(g, D.g.p, S.nig * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+// ------------------------------------------------------------- debug probes
+void device_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
+                          const double *X, double alpha, double beta, double *out) {
+  if (npairs < 0 || npairs > 16) fatal("BDDCML_SELFTEST", "npairs out of range");
+  // tiles: 0 = C, 1 = X, 2.. = A_p, B_p
+  const int nt = 2 + 2 * npairs;
+  DBuf<double> pool;
+  pool.alloc((size_t)nt * TILE_ELEMS);
+  std::vector<double> h((size_t)nt * TILE_ELEMS, 0.0);
+  std::copy(C, C + TILE_ELEMS, h.begin());
+  if (X) std::copy(X, X + TILE_ELEMS, h.begin() + TILE_ELEMS);
+  for (int p = 0; p < npairs; p++) {
+    std::copy(A + (size_t)p * TILE_ELEMS, A + (size_t)(p + 1) * TILE_ELEMS,
+              h.begin() + (size_t)(2 + 2 * p) * TILE_ELEMS);
+    std::copy(B + (size_t)p * TILE_ELEMS, B + (size_t)(p + 1) * TILE_ELEMS,
+              h.begin() + (size_t)(3 + 2 * p) * TILE_ELEMS);
+  }
+  HIPCHECK(hipMemcpy(pool.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  TaskList l;
+  l.tasks.push_back({0, 1, 0, npairs});
+  for (int p = 0; p < npairs; p++) l.pairs.push_back({2 + 2 * p, 3 + 2 * p});
+  DBuf<TileTask> t;
+  DBuf<TilePair> pr;
+  DBuf<int> err;
+  t.upload(l.tasks, nullptr);
+  pr.upload(l.pairs, nullptr);
+  err.alloc(1);
+  err.zero(nullptr);
+  if (op == 3) {
+    launch_tile_potrf(nullptr, pool.p, t.p, pr.p, 1, err.p);
+  } else {
+    int kind = op % 10, post = op / 10;
+    if (kind < 0 || kind > 2 || post < 0 || post > 2) fatal("BDDCML_SELFTEST", "bad op");
+    launch_tile_gemm(nullptr, pool.p, t.p, pr.p, 1, kind, post, alpha, beta);
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipDeviceSynchronize());
+  HIPCHECK(hipMemcpy(out, pool.p, TILE_ELEMS * sizeof(double), hipMemcpyDeviceToHost));
+  if (op == 3)
+    HIPCHECK(hipMemcpy(out + TILE_ELEMS, pool.p + TILE_ELEMS, TILE_ELEMS * sizeof(double),
+                       hipMemcpyDeviceToHost));
+}
+
+void device_sub_matrix(Solver &S, int is, int which, double *out) {
+  Device &D = *S.dev;
+  const Sub &s = S.subs[is];
+  const SubTiles &T = D.sub[is];
+  const int n = T.nti, first = which == 'S' ? T.sfull : T.kfull;
+  std::vector<double> tiles((size_t)n * n * TILE_ELEMS);
+  HIPCHECK(hipMemcpy(tiles.data(), D.pool.p + (size_t)first * TILE_ELEMS,
+                     tiles.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int a = 0; a < s.ni; a++)
+    for (int b = 0; b < s.ni; b++) {
+      size_t t = (size_t)(a / TILE) * n + b / TILE;
+      out[(size_t)s.dev_ifc[a] * s.ni + s.dev_ifc[b]] =
+          tiles[t * TILE_ELEMS + (b % TILE) * TILE + a % TILE];
+    }
+}
+
+void device_sub_coarse(Solver &S, int is, double *phi, double *ac) {
+  Device &D = *S.dev;
+  const Sub &s = S.subs[is];
+  const SubDesc &d = D.hdesc[is];
+  const int nc = s.nconstr;
+  std::vector<double> P((size_t)d.NI * nc), A((size_t)nc * nc);
+  if (nc) {
+    HIPCHECK(hipMemcpy(P.data(), D.Phi.p + d.zoff, P.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(A.data(), D.Acs.p + d.goff, A.size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  for (int a = 0; a < s.ni; a++)
+    for (int c = 0; c < nc; c++) phi[(size_t)s.dev_ifc[a] * nc + c] = P[(size_t)c * d.NI + a];
+  for (size_t k = 0; k < A.size(); k++) ac[k] = A[k];
+}
+
+// --------------------------------------------------------------- operators
+static void op_schur(Solver &S, const double *x, double *y) {
+  Device &D = *S.dev;
+  launch_gemv(D.st, D.pool.p, D.srows.p, D.nsrows, D.loff.p, D.NIs.p, D.map.p, nullptr, x, D.yl.p,
+              D.maxNI);
+  launch_assemble(D.st, S.nig, D.aptr.p, D.asrc.p, nullptr, D.yl.p, y);
+}
+
+static void op_pc(Solver &S, const double *r, double *z) {
+  Device &D = *S.dev;
+  hipStream_t st = D.st;
+  launch_gemv(st, D.pool.p, D.krows.p, D.nkrows, D.loff.p, D.NIs.p, D.map.p, D.w.p, r, D.zl.p,
+              D.maxNI);
+  launch_phit(st, D.desc.p, S.nsub, D.Phi.p, D.map.p, D.w.p, r, D.rcs.p, D.maxnc);
+  launch_coarse_gather(st, S.ndofc, D.cgptr.p, D.cgsrc.p, D.rcs.p, D.rc.p);
+  launch_gemv(st, D.pool.p, D.crows.p, D.ncrows, D.cloff.p, D.cNI.p, nullptr, nullptr, D.rc.p,
+              D.uc.p, D.NC);
+  launch_phi_add(st, D.desc.p, S.nsub, D.Phi.p, D.cmap.p, D.uc.p, D.zl.p, D.maxNI);
+  launch_assemble(st, S.nig, D.aptr.p, D.asrc.p, D.w.p, D.zl.p, z);
+}
+
+static void dots(Solver &S, std::initializer_list<std::pair<const double *, const double *>> prs,
+                 double *out) {
+  Device &D = *S.dev;
+  DotArgs a{};
+  a.ndot = 0;
+  for (auto &p : prs) {
+    a.a[a.ndot] = p.first;
+    a.b[a.ndot] = p.second;
+    a.ndot++;
+  }
+  launch_dots(D.st, S.nig, a, D.partial.p, D.dots.p);
+  HIPCHECK(hipMemcpyAsync(out, D.dots.p, a.ndot * sizeof(double), hipMemcpyDeviceToHost, D.st));
+  HIPCHECK(hipStreamSynchronize(D.st));
+}
+
+void device_apply_schur(Solver &S, const double *x, double *y) {
+  Device &D = *S.dev;
+  HIPCHECK(hipMemcpy(D.xin.p, x, S.nig * sizeof(double), hipMemcpyHostToDevice));
+  op_schur(S, D.xin.p, D.xout.p);
+  HIPCHECK(hipStreamSynchronize(D.st));
+  HIPCHECK(hipMemcpy(y, D.xout.p, S.nig * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+void device_apply_pc(Solver &S, const double *r, double *z) {
+  Device &D = *S.dev;
+  HIPCHECK(hipMemcpy(D.xin.p, r, S.nig * sizeof(double), hipMemcpyHostToDevice));
+  op_pc(S, D.xin.p, D.xout.p);
+  HIPCHECK(hipStreamSynchronize(D.st));
+  HIPCHECK(hipMemcpy(z, D.xout.p, S.nig * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+// eigenvalues of the Lanczos tridiagonal: largest one by Sturm bisection
+static double tridiag_max_eig(const std::vector<double> &d, const std::vector<double> &e, int n) {
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < n; i++) {
+    double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i < n - 1 ? std::fabs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+  }
+  auto count_below = [&](double x) {  // number of eigenvalues < x
+    int c = 0;
+    double q = 1.0;
+    for (int i = 0; i < n; i++) {
+      double ee = i > 0 ? e[i - 1] * e[i - 1] : 0.0;
+      q = d[i] - x - (i > 0 ? ee / q : 0.0);
+      if (q == 0.0) q = -1e-300;
+      if (q < 0) c++;
+    }
+    return c;
+  };
+  for (int it = 0; it < 200 && hi - lo > 1e-14 * std::max(1.0, std::fabs(hi)); it++) {
+    double mid = 0.5 * (lo + hi);
+    if (count_below(mid) >= n) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// ------------------------------------------------------------------- Krylov
+static void postprocess(Solver &S) {
+  Device &D = *S.dev;
+  hipStream_t st = D.st;
+  // interface part of the device-ordered vectors <- u
+  std::vector<double> hu(S.nig);
+  HIPCHECK(hipMemcpyAsync(hu.data(), D.u.p, S.nig * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  std::vector<double> hv(D.VT);
+  HIPCHECK(hipMemcpy(hv.data(), D.vec.p, D.VT * sizeof(double), hipMemcpyDeviceToHost));
+  for (int is = 0; is < S.nsub; is++) {
+    const Sub &s = S.subs[is];
+    int64_t vo = D.htrsv[is].voff;
+    for (int k = 0; k < s.ni; k++) {
+      int kr = s.dev_ifc[k];
+      hv[vo + s.NO + k] = hu[S.gpos[s.isvgvn[s.ifc_dof[kr]]]];
+    }
+  }
+  HIPCHECK(hipMemcpyAsync(D.vec.p, hv.data(), D.VT * sizeof(double), hipMemcpyHostToDevice, st));
+  launch_trsv_bwd(st, D.pool.p, D.trsv.p, S.nsub, D.tcptr.p, D.tck.p, D.tct.p, D.vec.p);
+  HIPCHECK(hipMemcpyAsync(hv.data(), D.vec.p, D.VT * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  for (int is = 0; is < S.nsub; is++) {
+    Sub &s = S.subs[is];
+    int64_t vo = D.htrsv[is].voff;
+    for (int d = 0; d < s.ndofs; d++) s.sol[d] = hv[vo + s.perm[d]];
+  }
+  S.uig = hu;
+}
+
+static void initial_guess(Solver &S) {
+  Device &D = *S.dev;
+  std::vector<double> u0(S.nig, 0.0);
+  std::vector<char> done(S.nig, 0);
+  for (int is = 0; is < S.nsub; is++) {
+    const Sub &s = S.subs[is];
+    for (int d : s.ifc_dof) {
+      int g = S.gpos[s.isvgvn[d]];
+      if (done[g]) continue;
+      done[g] = 1;
+      u0[g] = (s.has_fix && s.ifix[d]) ? s.fixv[d] : s.sol[d];
+    }
+  }
+  HIPCHECK(hipMemcpyAsync(D.u.p, u0.data(), S.nig * sizeof(double), hipMemcpyHostToDevice, D.st));
+}
+
+void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, int *num_iter,
+                  int *reason, double *cond) {
+  Device &D = *S.dev;
+  hipStream_t st = D.st;
+  const int n = S.nig;
+  double t0 = wall_time();
+  S.hist.clear();
+  *num_iter = 0;
+  *reason = 0;
+  *cond = (method == 0) ? 1.0 : -1.0;
+  initial_guess(S);
+  HIPCHECK(hipMemcpyAsync(D.r.p, D.g.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  launch_mask(st, n, D.fixmask.p, D.r.p);
+  double sc[4];
+  if (method == 0) {
+    dots(S, {{D.r.p, D.r.p}}, sc);
+    if (sc[0] == 0.0) goto finish;
+    op_schur(S, D.u.p, D.ap.p);
+    launch_mask(st, n, D.fixmask.p, D.ap.p);
+    launch_axpy2(st, n, -1.0, D.ap.p, D.r.p, 0.0, nullptr, nullptr);
+    dots(S, {{D.r.p, D.r.p}}, sc);
+    double normres0 = std::sqrt(sc[0]);
+    if (normres0 == 0.0) goto finish;
+    op_pc(S, D.r.p, D.p.p);
+    dots(S, {{D.r.p, D.p.p}}, sc);
+    double rmp = sc[0];
+    std::vector<double> diag(maxit + 2, 0.0), sub(maxit + 2, 0.0);
+    int ndecr = 0, nw = 0;
+    double lastres = 1.0;
+    for (int iter = 1; iter <= maxit; iter++) {
+      op_schur(S, D.p.p, D.ap.p);
+      dots(S, {{D.p.p, D.ap.p}}, sc);
+      double pap = sc[0];
+      if (pap <= 0.0 && S.verbose > 0)
+        std::fprintf(stderr, "WARNING in KRYLOV_BDDCPCG: System matrix not positive definite!\n");
+      double alpha = rmp / pap;
+      launch_axpy2(st, n, alpha, D.p.p, D.u.p, -alpha, D.ap.p, D.r.p);
+      dots(S, {{D.r.p, D.r.p}}, sc);
+      double relres = std::sqrt(sc[0]) / normres0;
+      S.hist.push_back(relres);
+      if (S.verbose > 0) std::printf(" KRYLOV_BDDCPCG: iteration: %d relative residual: %.6e\n", iter, relres);
+      if (relres < tol) {
+        nw = iter - 1;
+        *num_iter = iter;
+        *reason = 0;
+        break;
+      }
+      if (iter == maxit) {
+        nw = iter - 1;
+        *num_iter = iter;
+        *reason = -1;
+        break;
+      }
+      if (relres < lastres) ndecr = 0;
+      else if (++ndecr >= ndecrmax) {
+        nw = iter - 1;
+        *num_iter = iter;
+        *reason = -2;
+        break;
+      }
+      lastres = relres;
+      op_pc(S, D.r.p, D.z.p);
+      dots(S, {{D.r.p, D.z.p}}, sc);
+      double rmpold = rmp;
+      rmp = sc[0];
+      if (rmp <= 0.0 && S.verbose > 0)
+        std::fprintf(stderr, "WARNING in KRYLOV_BDDCPCG: Preconditioner not positive definite!\n");
+      double beta = rmp / rmpold;
+      launch_xpay(st, n, beta, D.z.p, D.p.p);
+      diag[iter - 1] += 1.0 / alpha;
+      diag[iter] = beta / alpha;
+      sub[iter - 1] = beta >= 0.0 ? -std::sqrt(beta) / alpha : 0.0;
+    }
+    *cond = nw > 1 ? tridiag_max_eig(diag, sub, nw) : 1.0;
+  } else {
+    // BiCGSTAB (src/module_krylov.f90:948-1505)
+    op_schur(S, D.u.p, D.v.p);
+    launch_axpy2(st, n, -1.0, D.v.p, D.r.p, 0.0, nullptr, nullptr);
+    launch_mask(st, n, D.fixmask.p, D.r.p);
+    dots(S, {{D.r.p, D.r.p}}, sc);
+    double normrhs = std::sqrt(sc[0]);
+    if (normrhs == 0.0) goto finish;
+    double rhoold = 1.0, alpha = 1.0, omega = 1.0;
+    HIPCHECK(hipMemsetAsync(D.v.p, 0, n * sizeof(double), st));
+    HIPCHECK(hipMemsetAsync(D.p.p, 0, n * sizeof(double), st));
+    HIPCHECK(hipMemcpyAsync(D.rs.p, D.r.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    int ndecr = 0;
+    double lastres = 1.0;
+    for (int iter = 1; iter <= maxit; iter++) {
+      dots(S, {{D.r.p, D.rs.p}}, sc);
+      double rho = sc[0];
+      double beta = rho * alpha / (rhoold * omega);
+      launch_bicg_p(st, n, beta, omega, D.r.p, D.v.p, D.p.p);
+      op_pc(S, D.p.p, D.y.p);
+      op_schur(S, D.y.p, D.v.p);
+      dots(S, {{D.v.p, D.rs.p}}, sc);
+      alpha = rho / sc[0];
+      launch_axpy2(st, n, alpha, D.y.p, D.u.p, 0.0, nullptr, nullptr);
+      launch_sub_scaled(st, n, D.r.p, alpha, D.v.p, D.s.p);
+      dots(S, {{D.s.p, D.s.p}}, sc);
+      double relres = std::sqrt(sc[0]) / normrhs;
+      S.hist.push_back(relres);
+      if (relres < tol) {
+        *num_iter = iter;
+        *reason = 0;
+        break;
+      }
+      if (relres < lastres) ndecr = 0;
+      else if (++ndecr >= ndecrmax) {
+        *num_iter = iter;
+        *reason = -2;
+        break;
+      }
+      lastres = relres;
+      op_pc(S, D.s.p, D.z.p);
+      op_schur(S, D.z.p, D.t.p);
+      dots(S, {{D.s.p, D.t.p}, {D.t.p, D.t.p}}, sc);
+      omega = sc[0] / sc[1];
+      launch_axpy2(st, n, omega, D.z.p, D.u.p, 0.0, nullptr, nullptr);
+      launch_sub_scaled(st, n, D.s.p, omega, D.t.p, D.r.p);
+      dots(S, {{D.r.p, D.r.p}}, sc);
+      relres = std::sqrt(sc[0]) / normrhs;
+      S.hist.push_back(relres);
+      if (relres < tol) {
+        *num_iter = iter;
+        *reason = 0;
+        break;
+      }
+      if (iter == maxit) {
+        *num_iter = iter;
+        *reason = -1;
+        break;
+      }
+      if (relres < lastres) ndecr = 0;
+      else if (++ndecr >= ndecrmax) {
+        *num_iter = iter;
+        *reason = -2;
+        break;
+      }
+      lastres = relres;
+      rhoold = rho;
+    }
+  }
+finish:
+  HIPCHECK(hipStreamSynchronize(st));
+  S.t.krylov += wall_time() - t0;
+  t0 = wall_time();
+  postprocess(S);
+  S.t.postprocess += wall_time() - t0;
+}
+
+}  // namespace bddc

@@ -1,0 +1,130 @@
+// kernels.hpp -- host launchers of the HIP kernels (defined in *.hip).
+// Tiles are 64x64 FP64, column major inside the tile, addressed by an int32
+// tile id into one device pool: element (r,c) of tile t is
+// pool[t*4096 + c*64 + r].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "plan.hpp"
+
+namespace bddc {
+
+// ---- tile algebra -------------------------------------------------------
+// op kinds of the batched GEMM-sum: C = alpha*C + beta * sum_p opA(A_p) opB(B_p)
+enum GemmKind { GEMM_NT = 0, GEMM_NN = 1, GEMM_TN = 2 };
+// post operations on E after the sum
+enum GemmPost { POST_NONE = 0, POST_RIGHT_XT = 1, POST_LEFT_X = 2 };
+
+void launch_tile_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+                      int ntasks, int kind, int post, double alpha, double beta);
+// C = C - sum A B^T ; C = chol(C) (lower, upper zeroed) ; X = inv(C)
+void launch_tile_potrf(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+                       int ntasks, int *err);
+// dst tiles (full storage) from lower tiles: for each job (src, dst, transpose)
+struct MirrorJob {
+  int src, dst, trans;
+};
+void launch_tile_mirror(hipStream_t st, double *pool, const MirrorJob *jobs, int njobs);
+void launch_zero_tiles(hipStream_t st, double *pool, const int *ids, int n);
+void launch_scatter(hipStream_t st, double *pool, const int64_t *idx, const double *val,
+                    int64_t n);
+
+// ---- interface preconditioner pieces ------------------------------------
+// Descriptor of one subdomain's dense interface matrices.
+struct SubDesc {
+  int ni, NI, nti, nc;    // interface dofs, padded, tile count, constraints
+  int sfull;              // first tile of S (full, row-major tile order)
+  int kfull;              // first tile of K (full, row-major tile order)
+  int64_t loff;           // offset into local interface vectors
+  int64_t zoff;           // offset into Z / Phi (NI x nc column major)
+  int64_t goff;           // offset into G / Ginv (nc x nc)
+  int coff;               // offset into constraint arrays (row pointers)
+  double rho;             // augmented-Lagrangian scale
+};
+// S' = S + sum_c (rho/|D_c|) 1_c 1_c^T written into lower tiles
+void launch_sprime(hipStream_t st, double *pool, const SubDesc *subs, const int *jobs_sub,
+                   const int *jobs_ij, const int *jobs_dst, int njobs, const int *cid,
+                   const int *csize);
+// Z(:,c) = sum_{j in D_c} K(:,j) with K full tiles
+void launch_z(hipStream_t st, const double *pool, const SubDesc *subs, int nsub, const int *cptr,
+              const int *cdof, double *Z, int maxnc);
+// G = C Z, Ginv = G^{-1} (in place Gauss-Jordan, SPD), Phi = Z Ginv,
+// Acs = Ginv - diag(rho/|D_c|)
+void launch_g(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const int *cdof,
+              const double *Z, double *G);
+void launch_ginv(hipStream_t st, const SubDesc *subs, int nsub, double *G, double *work, int *err);
+void launch_phi(hipStream_t st, const SubDesc *subs, int nsub, const double *Z, const double *G,
+                double *Phi);
+// K(I,J) -= Phi_I Z_J^T for all full tiles
+void launch_kupdate(hipStream_t st, double *pool, const SubDesc *subs, const int *job_sub,
+                    const int *job_ij, int njobs, const double *Phi, const double *Z);
+void launch_acs(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const double *G,
+                double *Acs);
+// coarse assembly: dst[k] = sum_{q in ptr[k]..ptr[k+1]} Acs[src[q]] into pool
+void launch_coarse_assemble(hipStream_t st, double *pool, const int64_t *dst, const int *ptr,
+                            const int64_t *src, int n, const double *Acs);
+
+// ---- triangular sweeps of the partial factor ----------------------------
+struct TrsvDesc {
+  int nto, nti;          // interior / interface tile rows
+  int rptr_off;          // offset into row lists
+  int cptr_off;          // offset into column lists
+  int linv;              // first Linv tile (nto consecutive)
+  int64_t voff;          // offset into the device-ordered subdomain vector
+};
+void trsv_set_maxnt(int nt);  // largest nto+nti of the batch (LDS sizing)
+void launch_trsv_fwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
+                     const int *rptr, const int *rk, const int *rt, double *v);
+void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
+                     const int *cptr, const int *ck, const int *ct, double *v);
+
+// ---- iteration kernels ---------------------------------------------------
+struct GemvRow {
+  int tile0;      // first tile of the row (row-major full storage)
+  int nt;         // tiles in the row
+  int mat;        // matrix index (subdomain) for gather
+  int I;          // tile row
+};
+// y_local[rowoff + I*64 + r] = sum_J T(I,J) x_J, x_J gathered:
+// x[k] = (w ? w[loff+k] : 1) * (map ? xin[map[loff+k]] : xin[k])
+void launch_gemv(hipStream_t st, const double *pool, const GemvRow *rows, int nrows,
+                 const int64_t *loff, const int *NI, const int *map, const double *w,
+                 const double *xin, double *yout, int maxNI);
+// unique y[g] = sum_{q} (w? w[src]:1) * (ylocal[src] + (phiu? phiu[src]:0))
+void launch_assemble(hipStream_t st, int n, const int *ptr, const int64_t *src, const double *w,
+                     const double *ylocal, double *y);
+// rc_s = Phi_s^T (w .* x_gathered)  -> coarse residual assembled by ptr/src
+void launch_phit(hipStream_t st, const SubDesc *subs, int nsub, const double *Phi,
+                 const int *map, const double *w, const double *x, double *rcs, int maxnc);
+void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
+                          const double *rcs, double *rc);
+// zlocal += Phi_s uc(indrowc)
+void launch_phi_add(hipStream_t st, const SubDesc *subs, int nsub, const double *Phi,
+                    const int *cmap, const double *uc, double *zlocal, int maxNI);
+
+// deterministic reductions and vector updates on unique vectors
+constexpr int DOT_BLOCKS = 240;
+// out[d] = sum_i a_d[i] * b_d[i] for up to 4 pairs; partial has DOT_BLOCKS*4 doubles
+struct DotArgs {
+  const double *a[4];
+  const double *b[4];
+  int ndot;
+};
+void launch_dots(hipStream_t st, int n, DotArgs args, double *partial, double *out);
+// y += a*x ; y2 += a2*x2 (y2 may be null)
+void launch_axpy2(hipStream_t st, int n, double a, const double *x, double *y, double a2,
+                  const double *x2, double *y2);
+// y = x + b*y
+void launch_xpay(hipStream_t st, int n, double b, const double *x, double *y);
+// y = x + b*(y - w*v)   (BiCGSTAB direction update)
+void launch_bicg_p(hipStream_t st, int n, double b, double w, const double *r, const double *v,
+                   double *p);
+// z = x - a*y
+void launch_sub_scaled(hipStream_t st, int n, const double *x, double a, const double *y,
+                       double *z);
+void launch_mask(hipStream_t st, int n, const char *mask, double *v);
+
+}  // namespace bddc

@@ -1,0 +1,531 @@
+// kernels_misc.hip -- interface preconditioner assembly (Z, G, Phi, K, coarse
+// element matrices), triangular sweeps of the partial factor, and the
+// per-iteration kernels of the Krylov methods (tile GEMV, weighted
+// gather/scatter of the interface exchange, coarse gather, dot products).
+#include "kernels.hpp"
+
+namespace bddc {
+
+// --------------------------------------------------------- S' = S + rho C^T C
+__global__ void k_sprime(double *__restrict__ pool, const SubDesc *__restrict__ subs,
+                         const int *__restrict__ jsub, const int *__restrict__ jij,
+                         const int *__restrict__ jdst, const int *__restrict__ cid,
+                         const int *__restrict__ csize) {
+  const int job = blockIdx.x;
+  const SubDesc d = subs[jsub[job]];
+  const int I = jij[job] >> 16, J = jij[job] & 0xffff;
+  const double *src = pool + (size_t)(d.sfull + I * d.nti + J) * TILE_ELEMS;
+  double *dst = pool + (size_t)jdst[job] * TILE_ELEMS;
+  const int *ci = cid + d.loff;
+  for (int e = threadIdx.x; e < TILE_ELEMS; e += blockDim.x) {
+    int r = I * 64 + (e & 63), c = J * 64 + (e >> 6);
+    double v = src[e];
+    int a = ci[r], b = ci[c];
+    if (a >= 0 && a == b) v += d.rho / (double)csize[a];
+    dst[e] = v;
+  }
+}
+
+void launch_sprime(hipStream_t st, double *pool, const SubDesc *subs, const int *jobs_sub,
+                   const int *jobs_ij, const int *jobs_dst, int njobs, const int *cid,
+                   const int *csize) {
+  if (njobs <= 0) return;
+  hipLaunchKernelGGL(k_sprime, dim3(njobs), dim3(256), 0, st, pool, subs, jobs_sub, jobs_ij,
+                     jobs_dst, cid, csize);
+}
+
+// ------------------------------------------------------- Z(:,c) = K 1_{D_c}
+__global__ void k_z(const double *__restrict__ pool, const SubDesc *__restrict__ subs,
+                    const int *__restrict__ cptr, const int *__restrict__ cdof,
+                    double *__restrict__ Z) {
+  const SubDesc d = subs[blockIdx.y];
+  const int c = blockIdx.x;
+  if (c >= d.nc) return;
+  const int q0 = cptr[d.coff + c], q1 = cptr[d.coff + c + 1];
+  for (int i = threadIdx.x; i < d.NI; i += blockDim.x) {
+    double s = 0.0;
+    const int ti = i >> 6, ri = i & 63;
+    for (int q = q0; q < q1; q++) {
+      int j = cdof[q];
+      s += pool[(size_t)(d.kfull + ti * d.nti + (j >> 6)) * TILE_ELEMS + (j & 63) * 64 + ri];
+    }
+    Z[d.zoff + (size_t)c * d.NI + i] = s;
+  }
+}
+
+__global__ void k_g(const SubDesc *__restrict__ subs, const int *__restrict__ cptr,
+                    const int *__restrict__ cdof, const double *__restrict__ Z,
+                    double *__restrict__ G) {
+  const SubDesc d = subs[blockIdx.x];
+  const int nc = d.nc;
+  for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) {
+    int c = e / nc, k = e % nc;
+    double s = 0.0;
+    for (int q = cptr[d.coff + c]; q < cptr[d.coff + c + 1]; q++)
+      s += Z[d.zoff + (size_t)k * d.NI + cdof[q]];
+    G[d.goff + (size_t)k * nc + c] = s;
+  }
+}
+
+// in-place Gauss-Jordan inverse of the SPD matrices G (no pivoting)
+__global__ void k_ginv(const SubDesc *__restrict__ subs, double *__restrict__ G,
+                       int *__restrict__ err) {
+  __shared__ double F[1024];
+  __shared__ double inv;
+  const SubDesc d = subs[blockIdx.x];
+  const int n = d.nc;
+  double *A = G + d.goff;  // A[i*n + j]
+  for (int k = 0; k < n; k++) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double p = A[k * n + k];
+      if (!(p > 0.0)) {
+        atomicOr(err, 2);
+        p = 1.0;
+      }
+      inv = 1.0 / p;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      double v = (j == k) ? 1.0 : A[k * n + j];
+      A[k * n + j] = v * inv;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) F[i] = (i == k) ? 0.0 : A[i * n + k];
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+      int i = e / n, j = e % n;
+      if (i == k) continue;
+      double v = (j == k) ? 0.0 : A[i * n + j];
+      A[i * n + j] = v - F[i] * A[k * n + j];
+    }
+  }
+}
+
+__global__ void k_phi(const SubDesc *__restrict__ subs, const double *__restrict__ Z,
+                      const double *__restrict__ G, double *__restrict__ Phi) {
+  const SubDesc d = subs[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.NI) return;
+  const int nc = d.nc;
+  for (int c = 0; c < nc; c++) {
+    double s = 0.0;
+    for (int k = 0; k < nc; k++) s += Z[d.zoff + (size_t)k * d.NI + i] * G[d.goff + (size_t)c * nc + k];
+    Phi[d.zoff + (size_t)c * d.NI + i] = s;
+  }
+}
+
+// K(I,J) -= Phi(I rows,:) Z(J rows,:)^T
+__global__ void k_kupdate(double *__restrict__ pool, const SubDesc *__restrict__ subs,
+                          const int *__restrict__ jsub, const int *__restrict__ jij,
+                          const double *__restrict__ Phi, const double *__restrict__ Z) {
+  __shared__ double Ps[32 * 64];
+  __shared__ double Zs[32 * 64];
+  const int job = blockIdx.x;
+  const SubDesc d = subs[jsub[job]];
+  const int I = jij[job] >> 16, J = jij[job] & 0xffff;
+  double *T = pool + (size_t)(d.kfull + I * d.nti + J) * TILE_ELEMS;
+  double acc[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) acc[m] = 0.0;
+  for (int c0 = 0; c0 < d.nc; c0 += 32) {
+    const int cn = min(32, d.nc - c0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < cn * 64; e += blockDim.x) {
+      int c = e >> 6, r = e & 63;
+      Ps[c * 64 + r] = Phi[d.zoff + (size_t)(c0 + c) * d.NI + I * 64 + r];
+      Zs[c * 64 + r] = Z[d.zoff + (size_t)(c0 + c) * d.NI + J * 64 + r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      int e = threadIdx.x + 256 * m;
+      int r = e & 63, col = e >> 6;
+      double s = 0.0;
+      for (int c = 0; c < cn; c++) s += Ps[c * 64 + r] * Zs[c * 64 + col];
+      acc[m] += s;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    int e = threadIdx.x + 256 * m;
+    T[e] -= acc[m];
+  }
+}
+
+__global__ void k_acs(const SubDesc *__restrict__ subs, const int *__restrict__ cptr,
+                      const double *__restrict__ G, double *__restrict__ Acs) {
+  const SubDesc d = subs[blockIdx.x];
+  const int nc = d.nc;
+  for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) {
+    int c = e / nc, k = e % nc;
+    double v = G[d.goff + e];
+    if (c == k) v -= d.rho / (double)(cptr[d.coff + c + 1] - cptr[d.coff + c]);
+    Acs[d.goff + e] = v;
+  }
+}
+
+__global__ void k_coarse_assemble(double *__restrict__ pool, const int64_t *__restrict__ dst,
+                                  const int *__restrict__ ptr, const int64_t *__restrict__ src,
+                                  int n, const double *__restrict__ Acs) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double s = 0.0;
+  for (int q = ptr[k]; q < ptr[k + 1]; q++) s += Acs[src[q]];
+  pool[dst[k]] = s;
+}
+
+void launch_z(hipStream_t st, const double *pool, const SubDesc *subs, int nsub, const int *cptr,
+              const int *cdof, double *Z, int maxnc) {
+  if (nsub <= 0 || maxnc <= 0) return;
+  hipLaunchKernelGGL(k_z, dim3(maxnc, nsub), dim3(256), 0, st, pool, subs, cptr, cdof, Z);
+}
+void launch_g(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const int *cdof,
+              const double *Z, double *G) {
+  if (nsub <= 0) return;
+  hipLaunchKernelGGL(k_g, dim3(nsub), dim3(256), 0, st, subs, cptr, cdof, Z, G);
+}
+void launch_ginv(hipStream_t st, const SubDesc *subs, int nsub, double *G, double *work, int *err) {
+  (void)work;
+  if (nsub <= 0) return;
+  hipLaunchKernelGGL(k_ginv, dim3(nsub), dim3(256), 0, st, subs, G, err);
+}
+void launch_phi(hipStream_t st, const SubDesc *subs, int nsub, const double *Z, const double *G,
+                double *Phi) {
+  if (nsub <= 0) return;
+  hipLaunchKernelGGL(k_phi, dim3(8192 / 256, nsub), dim3(256), 0, st, subs, Z, G, Phi);
+}
+void launch_kupdate(hipStream_t st, double *pool, const SubDesc *subs, const int *job_sub,
+                    const int *job_ij, int njobs, const double *Phi, const double *Z) {
+  if (njobs <= 0) return;
+  hipLaunchKernelGGL(k_kupdate, dim3(njobs), dim3(256), 0, st, pool, subs, job_sub, job_ij, Phi, Z);
+}
+void launch_acs(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const double *G,
+                double *Acs) {
+  if (nsub <= 0) return;
+  hipLaunchKernelGGL(k_acs, dim3(nsub), dim3(256), 0, st, subs, cptr, G, Acs);
+}
+void launch_coarse_assemble(hipStream_t st, double *pool, const int64_t *dst, const int *ptr,
+                            const int64_t *src, int n, const double *Acs) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_coarse_assemble, dim3((n + 255) / 256), dim3(256), 0, st, pool, dst, ptr,
+                     src, n, Acs);
+}
+
+// ------------------------------------------------------- triangular sweeps
+// 256 threads: y[r] = sum_c T(r,c) x[c] (trans=0) or sum_c T(c,r) x[c] (trans=1)
+// x and y in LDS; partial sums reduced through red[4*64].
+__device__ __forceinline__ void tile_gemv_lds(const double *__restrict__ T, const double *x,
+                                              double *red, bool trans) {
+  const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
+  double s = 0.0;
+  if (!trans) {
+#pragma unroll 4
+    for (int c = g; c < 64; c += 4) s += T[c * 64 + r] * x[c];
+  } else {
+    const double *row = T + r * 64 + g * 16;
+#pragma unroll
+    for (int k = 0; k < 16; k++) s += row[k] * x[g * 16 + k];
+  }
+  red[g * 64 + r] = s;
+}
+
+__global__ void k_trsv_fwd(const double *__restrict__ pool, const TrsvDesc *__restrict__ ds,
+                           const int *__restrict__ rptr, const int *__restrict__ rk,
+                           const int *__restrict__ rt, double *__restrict__ vec) {
+  extern __shared__ double smem[];
+  const TrsvDesc d = ds[blockIdx.x];
+  const int nt = d.nto + d.nti;
+  double *v = smem;                 // nt*64
+  double *red = smem + nt * 64;     // 256
+  double *t = red + 256;            // 64
+  double *gv = vec + d.voff;
+  for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
+  __syncthreads();
+  for (int I = 0; I < nt; I++) {
+    if (threadIdx.x < 64) t[threadIdx.x] = v[I * 64 + threadIdx.x];
+    for (int q = rptr[d.rptr_off + I]; q < rptr[d.rptr_off + I + 1]; q++) {
+      __syncthreads();
+      tile_gemv_lds(pool + (size_t)rt[q] * TILE_ELEMS, v + rk[q] * 64, red, false);
+      __syncthreads();
+      if (threadIdx.x < 64)
+        t[threadIdx.x] -= red[threadIdx.x] + red[64 + threadIdx.x] + red[128 + threadIdx.x] +
+                          red[192 + threadIdx.x];
+    }
+    __syncthreads();
+    if (I < d.nto) {
+      tile_gemv_lds(pool + (size_t)(d.linv + I) * TILE_ELEMS, t, red, false);
+      __syncthreads();
+      if (threadIdx.x < 64)
+        v[I * 64 + threadIdx.x] = red[threadIdx.x] + red[64 + threadIdx.x] +
+                                  red[128 + threadIdx.x] + red[192 + threadIdx.x];
+    } else if (threadIdx.x < 64) {
+      v[I * 64 + threadIdx.x] = t[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) gv[k] = v[k];
+}
+
+__global__ void k_trsv_bwd(const double *__restrict__ pool, const TrsvDesc *__restrict__ ds,
+                           const int *__restrict__ cptr, const int *__restrict__ ck,
+                           const int *__restrict__ ct, double *__restrict__ vec) {
+  extern __shared__ double smem[];
+  const TrsvDesc d = ds[blockIdx.x];
+  const int nt = d.nto + d.nti;
+  double *v = smem;
+  double *red = smem + nt * 64;
+  double *t = red + 256;
+  double *gv = vec + d.voff;
+  for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
+  __syncthreads();
+  for (int I = d.nto - 1; I >= 0; I--) {
+    if (threadIdx.x < 64) t[threadIdx.x] = v[I * 64 + threadIdx.x];
+    for (int q = cptr[d.cptr_off + I]; q < cptr[d.cptr_off + I + 1]; q++) {
+      __syncthreads();
+      tile_gemv_lds(pool + (size_t)ct[q] * TILE_ELEMS, v + ck[q] * 64, red, true);
+      __syncthreads();
+      if (threadIdx.x < 64)
+        t[threadIdx.x] -= red[threadIdx.x] + red[64 + threadIdx.x] + red[128 + threadIdx.x] +
+                          red[192 + threadIdx.x];
+    }
+    __syncthreads();
+    tile_gemv_lds(pool + (size_t)(d.linv + I) * TILE_ELEMS, t, red, true);
+    __syncthreads();
+    if (threadIdx.x < 64)
+      v[I * 64 + threadIdx.x] = red[threadIdx.x] + red[64 + threadIdx.x] + red[128 + threadIdx.x] +
+                                red[192 + threadIdx.x];
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < d.nto * 64; k += blockDim.x) gv[k] = v[k];
+}
+
+static int g_trsv_maxnt = 0;
+void trsv_set_maxnt(int nt) { g_trsv_maxnt = nt; }
+
+void launch_trsv_fwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
+                     const int *rptr, const int *rk, const int *rt, double *v) {
+  if (nsub <= 0) return;
+  size_t sh = (size_t)(g_trsv_maxnt * 64 + 256 + 64) * sizeof(double);
+  hipLaunchKernelGGL(k_trsv_fwd, dim3(nsub), dim3(256), sh, st, pool, d, rptr, rk, rt, v);
+}
+void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
+                     const int *cptr, const int *ck, const int *ct, double *v) {
+  if (nsub <= 0) return;
+  size_t sh = (size_t)(g_trsv_maxnt * 64 + 256 + 64) * sizeof(double);
+  hipLaunchKernelGGL(k_trsv_bwd, dim3(nsub), dim3(256), sh, st, pool, d, cptr, ck, ct, v);
+}
+
+// -------------------------------------------------------- iteration kernels
+__global__ __launch_bounds__(256) void k_gemv(const double *__restrict__ pool,
+                                              const GemvRow *__restrict__ rows,
+                                              const int64_t *__restrict__ loff,
+                                              const int *__restrict__ NIs,
+                                              const int *__restrict__ map,
+                                              const double *__restrict__ w,
+                                              const double *__restrict__ xin,
+                                              double *__restrict__ yout) {
+  extern __shared__ double xs[];
+  __shared__ double red[256];
+  const GemvRow row = rows[blockIdx.x];
+  const int64_t lo = loff[row.mat];
+  const int NI = NIs[row.mat];
+  for (int k = threadIdx.x; k < NI; k += 256) {
+    double x;
+    if (map) {
+      int g = map[lo + k];
+      x = g >= 0 ? xin[g] : 0.0;
+    } else {
+      x = xin[k];
+    }
+    if (w) x *= w[lo + k];
+    xs[k] = x;
+  }
+  __syncthreads();
+  const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
+  double s0 = 0.0, s1 = 0.0;
+  for (int J = g; J < row.nt; J += 4) {
+    const double *T = pool + (size_t)(row.tile0 + J) * TILE_ELEMS + r;
+    const double *x = xs + J * 64;
+#pragma unroll 8
+    for (int c = 0; c < 64; c += 2) {
+      s0 += T[c * 64] * x[c];
+      s1 += T[(c + 1) * 64] * x[c + 1];
+    }
+  }
+  red[threadIdx.x] = s0 + s1;
+  __syncthreads();
+  if (threadIdx.x < 64)
+    yout[lo + row.I * 64 + r] = (red[r] + red[64 + r]) + (red[128 + r] + red[192 + r]);
+}
+
+__global__ void k_assemble(int n, const int *__restrict__ ptr, const int64_t *__restrict__ src,
+                           const double *__restrict__ w, const double *__restrict__ yl,
+                           double *__restrict__ y) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  double s = 0.0;
+  for (int q = ptr[g]; q < ptr[g + 1]; q++) {
+    int64_t k = src[q];
+    s += w ? w[k] * yl[k] : yl[k];
+  }
+  y[g] = s;
+}
+
+__global__ void k_phit(const SubDesc *__restrict__ subs, const double *__restrict__ Phi,
+                       const int *__restrict__ map, const double *__restrict__ w,
+                       const double *__restrict__ x, double *__restrict__ rcs) {
+  __shared__ double red[256];
+  const SubDesc d = subs[blockIdx.y];
+  const int c = blockIdx.x;
+  if (c >= d.nc) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < d.NI; i += 256) {
+    int g = map[d.loff + i];
+    if (g >= 0) s += Phi[d.zoff + (size_t)c * d.NI + i] * w[d.loff + i] * x[g];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rcs[d.coff + c] = red[0];
+}
+
+__global__ void k_coarse_gather(int n, const int *__restrict__ ptr, const int *__restrict__ src,
+                                const double *__restrict__ rcs, double *__restrict__ rc) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double s = 0.0;
+  for (int q = ptr[k]; q < ptr[k + 1]; q++) s += rcs[src[q]];
+  rc[k] = s;
+}
+
+__global__ void k_phi_add(const SubDesc *__restrict__ subs, const double *__restrict__ Phi,
+                          const int *__restrict__ cmap, const double *__restrict__ uc,
+                          double *__restrict__ zl) {
+  const SubDesc d = subs[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.NI) return;
+  double s = 0.0;
+  for (int c = 0; c < d.nc; c++) s += Phi[d.zoff + (size_t)c * d.NI + i] * uc[cmap[d.coff + c]];
+  zl[d.loff + i] += s;
+}
+
+void launch_gemv(hipStream_t st, const double *pool, const GemvRow *rows, int nrows,
+                 const int64_t *loff, const int *NI, const int *map, const double *w,
+                 const double *xin, double *yout, int maxNI) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_gemv, dim3(nrows), dim3(256), (size_t)maxNI * sizeof(double), st, pool,
+                     rows, loff, NI, map, w, xin, yout);
+}
+void launch_assemble(hipStream_t st, int n, const int *ptr, const int64_t *src, const double *w,
+                     const double *ylocal, double *y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_assemble, dim3((n + 255) / 256), dim3(256), 0, st, n, ptr, src, w, ylocal, y);
+}
+void launch_phit(hipStream_t st, const SubDesc *subs, int nsub, const double *Phi, const int *map,
+                 const double *w, const double *x, double *rcs, int maxnc) {
+  if (nsub <= 0 || maxnc <= 0) return;
+  hipLaunchKernelGGL(k_phit, dim3(maxnc, nsub), dim3(256), 0, st, subs, Phi, map, w, x, rcs);
+}
+void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
+                          const double *rcs, double *rc) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_coarse_gather, dim3((n + 255) / 256), dim3(256), 0, st, n, ptr, src, rcs, rc);
+}
+void launch_phi_add(hipStream_t st, const SubDesc *subs, int nsub, const double *Phi,
+                    const int *cmap, const double *uc, double *zlocal, int maxNI) {
+  if (nsub <= 0) return;
+  hipLaunchKernelGGL(k_phi_add, dim3((maxNI + 255) / 256, nsub), dim3(256), 0, st, subs, Phi, cmap,
+                     uc, zlocal);
+}
+
+// ------------------------------------------------------------ vector ops
+__global__ __launch_bounds__(256) void k_dots(int n, DotArgs a, double *__restrict__ partial) {
+  __shared__ double red[4][256];
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    for (int d = 0; d < a.ndot; d++) s[d] += a.a[d][i] * a.b[d][i];
+  for (int d = 0; d < 4; d++) red[d][threadIdx.x] = s[d];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h)
+      for (int d = 0; d < 4; d++) red[d][threadIdx.x] += red[d][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) partial[threadIdx.x * gridDim.x + blockIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_dots_final(int nb, int ndot, const double *__restrict__ partial,
+                             double *__restrict__ out) {
+  __shared__ double red[256];
+  for (int d = 0; d < ndot; d++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) s += partial[d * nb + b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[d] = red[0];
+    __syncthreads();
+  }
+}
+
+void launch_dots(hipStream_t st, int n, DotArgs args, double *partial, double *out) {
+  hipLaunchKernelGGL(k_dots, dim3(DOT_BLOCKS), dim3(256), 0, st, n, args, partial);
+  hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(256), 0, st, DOT_BLOCKS, args.ndot, partial, out);
+}
+
+__global__ void k_axpy2(int n, double a, const double *__restrict__ x, double *__restrict__ y,
+                        double a2, const double *__restrict__ x2, double *__restrict__ y2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[i] += a * x[i];
+  if (y2) y2[i] += a2 * x2[i];
+}
+void launch_axpy2(hipStream_t st, int n, double a, const double *x, double *y, double a2,
+                  const double *x2, double *y2) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy2, dim3((n + 255) / 256), dim3(256), 0, st, n, a, x, y, a2, x2, y2);
+}
+__global__ void k_xpay(int n, double b, const double *__restrict__ x, double *__restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i] + b * y[i];
+}
+void launch_xpay(hipStream_t st, int n, double b, const double *x, double *y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_xpay, dim3((n + 255) / 256), dim3(256), 0, st, n, b, x, y);
+}
+__global__ void k_bicg_p(int n, double b, double w, const double *__restrict__ r,
+                         const double *__restrict__ v, double *__restrict__ p) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = r[i] + b * (p[i] - w * v[i]);
+}
+void launch_bicg_p(hipStream_t st, int n, double b, double w, const double *r, const double *v,
+                   double *p) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_bicg_p, dim3((n + 255) / 256), dim3(256), 0, st, n, b, w, r, v, p);
+}
+__global__ void k_sub_scaled(int n, const double *__restrict__ x, double a,
+                             const double *__restrict__ y, double *__restrict__ z) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = x[i] - a * y[i];
+}
+void launch_sub_scaled(hipStream_t st, int n, const double *x, double a, const double *y,
+                       double *z) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_sub_scaled, dim3((n + 255) / 256), dim3(256), 0, st, n, x, a, y, z);
+}
+__global__ void k_mask(int n, const char *__restrict__ m, double *__restrict__ v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && m[i]) v[i] = 0.0;
+}
+void launch_mask(hipStream_t st, int n, const char *mask, double *v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mask, dim3((n + 255) / 256), dim3(256), 0, st, n, mask, v);
+}
+
+}  // namespace bddc

@@ -1,0 +1,283 @@
+// kernels_tile.hip -- batched 64x64 FP64 tile algebra on CDNA4 matrix cores.
+//
+// One workgroup (4 waves) owns one output tile.  Each wave computes a 32x32
+// quadrant as 2x2 blocks of v_mfma_f64_16x16x4 accumulators; operand tiles
+// are staged through LDS with a padded leading dimension.  These kernels
+// carry every dense factorisation of the path: the partial Cholesky of the
+// subdomain matrices (Schur complements), the Cholesky / inverse of the
+// augmented interface matrices and of the coarse matrix.
+#include "kernels.hpp"
+
+namespace bddc {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int LDL = 65;  // LDS leading dimension (doubles)
+constexpr int NTHR = 256;
+
+// lds[q*LDL + r] = T(r,q)   (direct)      element (r,q) of the tile
+// lds[q*LDL + r] = T(q,r)   (transposed)
+__device__ __forceinline__ void tile_to_lds(const double *__restrict__ T, double *lds, bool trans) {
+  const int tid = threadIdx.x;
+  const double2 *T2 = reinterpret_cast<const double2 *>(T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    int e2 = tid + NTHR * i;  // double2 index, elements 2*e2 and 2*e2+1
+    double2 v = T2[e2];
+    int e = 2 * e2;
+    int row = e & 63, col = e >> 6;
+    if (!trans) {
+      lds[col * LDL + row] = v.x;
+      lds[col * LDL + row + 1] = v.y;
+    } else {
+      lds[row * LDL + col] = v.x;
+      lds[(row + 1) * LDL + col] = v.y;
+    }
+  }
+}
+
+// global tile (col major) from lds holding element (r,c) at c*LDL + r
+__device__ __forceinline__ void lds_to_tile(const double *lds, double *__restrict__ T) {
+  const int tid = threadIdx.x;
+  double2 *T2 = reinterpret_cast<double2 *>(T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    int e2 = tid + NTHR * i;
+    int e = 2 * e2;
+    int row = e & 63, col = e >> 6;
+    double2 v;
+    v.x = lds[col * LDL + row];
+    v.y = lds[col * LDL + row + 1];
+    T2[e2] = v;
+  }
+}
+
+// acc(r,c) += sum_q As[q*LDL + r] * Bs[q*LDL + c] over this wave's quadrant
+__device__ __forceinline__ void mfma_accumulate(const double *As, const double *Bs, v4d acc[2][2]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int R0 = (wave & 1) * 32, C0 = (wave >> 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int q = 0; q < 64; q += 4) {
+    const double *ar = As + (q + lk) * LDL + R0 + li;
+    const double *br = Bs + (q + lk) * LDL + C0 + li;
+    double a0 = ar[0], a1 = ar[16];
+    double b0 = br[0], b1 = br[16];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+
+// element coordinates of accumulator (bi,bj,v) of this lane: for the f64
+// 16x16x4 MFMA, register v of lane l holds row (l/16) + 4v, column l%16 of
+// the 16x16 block (measured on gfx950)
+__device__ __forceinline__ void acc_coord(int bi, int bj, int v, int &r, int &c) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  r = (wave & 1) * 32 + bi * 16 + (lane >> 4) + 4 * v;
+  c = (wave >> 1) * 32 + bj * 16 + (lane & 15);
+}
+
+__device__ __forceinline__ void zero_acc(v4d acc[2][2]) {
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+}
+
+// acc = sum over pairs of opA(A) opB(B) in the "r,q x c,q" form
+__device__ __forceinline__ void gemm_sum(const double *__restrict__ pool, const TilePair *pairs,
+                                         int p0, int p1, int kind, double *As, double *Bs,
+                                         v4d acc[2][2]) {
+  const bool ta = (kind == GEMM_TN);
+  const bool tb = (kind == GEMM_NN || kind == GEMM_TN);
+  for (int p = p0; p < p1; p++) {
+    TilePair pr = pairs[p];
+    __syncthreads();
+    tile_to_lds(pool + (size_t)pr.a * TILE_ELEMS, As, ta);
+    tile_to_lds(pool + (size_t)pr.b * TILE_ELEMS, Bs, tb);
+    __syncthreads();
+    mfma_accumulate(As, Bs, acc);
+  }
+}
+
+// E = alpha*C + beta*acc, left in As as element (r,c) at c*LDL + r
+__device__ __forceinline__ void form_e(const double *__restrict__ C, double alpha, double beta,
+                                       v4d acc[2][2], double *As) {
+  __syncthreads();
+  if (alpha != 0.0) tile_to_lds(C, As, false);
+  __syncthreads();
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int r, c;
+        acc_coord(bi, bj, v, r, c);
+        double e = beta * acc[bi][bj][v];
+        if (alpha != 0.0) e += alpha * As[c * LDL + r];
+        As[c * LDL + r] = e;
+      }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void acc_to_lds(v4d acc[2][2], double *As) {
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int r, c;
+        acc_coord(bi, bj, v, r, c);
+        As[c * LDL + r] = acc[bi][bj][v];
+      }
+}
+
+__global__ __launch_bounds__(NTHR) void k_tile_gemm(double *__restrict__ pool,
+                                                    const TileTask *__restrict__ tasks,
+                                                    const TilePair *__restrict__ pairs, int kind,
+                                                    int post, double alpha, double beta) {
+  __shared__ double As[64 * LDL];
+  __shared__ double Bs[64 * LDL];
+  const TileTask t = tasks[blockIdx.x];
+  v4d acc[2][2];
+  zero_acc(acc);
+  gemm_sum(pool, pairs, t.p0, t.p1, kind, As, Bs, acc);
+  double *C = pool + (size_t)t.c * TILE_ELEMS;
+  form_e(C, alpha, beta, acc, As);  // E in As
+  if (post == POST_NONE) {
+    lds_to_tile(As, C);
+    return;
+  }
+  const double *X = pool + (size_t)t.x * TILE_ELEMS;
+  if (post == POST_RIGHT_XT) {
+    // C_new(r,c) = sum_q E(r,q) X(c,q): A' = E (already As), B'(c,q) = X(c,q)
+    tile_to_lds(X, Bs, false);
+  } else {
+    // C_new(r,c) = sum_q X(r,q) E(q,c): B'(c,q) = E(q,c) = As[c*LDL + q]
+    for (int e = threadIdx.x; e < 4096; e += NTHR) {
+      int q = e & 63, c = e >> 6;
+      Bs[q * LDL + c] = As[c * LDL + q];
+    }
+    __syncthreads();
+    tile_to_lds(X, As, false);
+  }
+  __syncthreads();
+  zero_acc(acc);
+  mfma_accumulate(As, Bs, acc);
+  __syncthreads();
+  acc_to_lds(acc, As);
+  __syncthreads();
+  lds_to_tile(As, C);
+}
+
+__global__ __launch_bounds__(NTHR) void k_tile_potrf(double *__restrict__ pool,
+                                                     const TileTask *__restrict__ tasks,
+                                                     const TilePair *__restrict__ pairs,
+                                                     int *__restrict__ err) {
+  __shared__ double As[64 * LDL];
+  __shared__ double Bs[64 * LDL];
+  __shared__ int bad;
+  const TileTask t = tasks[blockIdx.x];
+  const int tid = threadIdx.x;
+  v4d acc[2][2];
+  zero_acc(acc);
+  gemm_sum(pool, pairs, t.p0, t.p1, GEMM_NT, As, Bs, acc);
+  double *C = pool + (size_t)t.c * TILE_ELEMS;
+  form_e(C, 1.0, -1.0, acc, As);
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  // right-looking unblocked Cholesky of the lower triangle in LDS
+  for (int j = 0; j < 64; j++) {
+    if (tid == 0) {
+      double d = As[j * LDL + j];
+      if (!(d > 0.0)) {
+        bad = 1;
+        d = 1.0;
+      }
+      As[j * LDL + j] = sqrt(d);
+    }
+    __syncthreads();
+    double djj = As[j * LDL + j];
+    if (tid > j && tid < 64) As[j * LDL + tid] /= djj;
+    __syncthreads();
+    for (int e = tid; e < 4096; e += NTHR) {
+      int r = e & 63, c = e >> 6;
+      if (c > j && r >= c) As[c * LDL + r] -= As[j * LDL + r] * As[j * LDL + c];
+    }
+    __syncthreads();
+  }
+  // zero the strict upper triangle
+  for (int e = tid; e < 4096; e += NTHR) {
+    int r = e & 63, c = e >> 6;
+    if (r < c) As[c * LDL + r] = 0.0;
+  }
+  __syncthreads();
+  // inverse of the lower factor, one column per thread
+  if (tid < 64) {
+    const int c = tid;
+    for (int i = 0; i < 64; i++) {
+      double x = 0.0;
+      if (i >= c) {
+        double s = (i == c) ? 1.0 : 0.0;
+        for (int k = c; k < i; k++) s -= As[k * LDL + i] * Bs[c * LDL + k];
+        x = s / As[i * LDL + i];
+      }
+      Bs[c * LDL + i] = x;
+    }
+  }
+  __syncthreads();
+  lds_to_tile(As, C);
+  lds_to_tile(Bs, pool + (size_t)t.x * TILE_ELEMS);
+  if (tid == 0 && bad) atomicOr(err, 1);
+}
+
+__global__ __launch_bounds__(NTHR) void k_tile_mirror(double *__restrict__ pool,
+                                                      const MirrorJob *__restrict__ jobs) {
+  __shared__ double As[64 * LDL];
+  const MirrorJob j = jobs[blockIdx.x];
+  tile_to_lds(pool + (size_t)j.src * TILE_ELEMS, As, j.trans != 0);
+  __syncthreads();
+  lds_to_tile(As, pool + (size_t)j.dst * TILE_ELEMS);
+}
+
+__global__ void k_zero_tiles(double *__restrict__ pool, const int *__restrict__ ids) {
+  double2 *T = reinterpret_cast<double2 *>(pool + (size_t)ids[blockIdx.x] * TILE_ELEMS);
+  for (int e = threadIdx.x; e < TILE_ELEMS / 2; e += blockDim.x) T[e] = make_double2(0.0, 0.0);
+}
+
+__global__ void k_scatter(double *__restrict__ pool, const int64_t *__restrict__ idx,
+                          const double *__restrict__ val, int64_t n) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) pool[idx[k]] = val[k];
+}
+
+void launch_tile_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+                      int ntasks, int kind, int post, double alpha, double beta) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_tile_gemm, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs, kind, post,
+                     alpha, beta);
+}
+void launch_tile_potrf(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+                       int ntasks, int *err) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_tile_potrf, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs, err);
+}
+void launch_tile_mirror(hipStream_t st, double *pool, const MirrorJob *jobs, int njobs) {
+  if (njobs <= 0) return;
+  hipLaunchKernelGGL(k_tile_mirror, dim3(njobs), dim3(NTHR), 0, st, pool, jobs);
+}
+void launch_zero_tiles(hipStream_t st, double *pool, const int *ids, int n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero_tiles, dim3(n), dim3(256), 0, st, pool, ids);
+}
+void launch_scatter(hipStream_t st, double *pool, const int64_t *idx, const double *val,
+                    int64_t n) {
+  if (n <= 0) return;
+  int64_t nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)nb), dim3(256), 0, st, pool, idx, val, n);
+}
+
+}  // namespace bddc

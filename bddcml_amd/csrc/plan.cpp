@@ -1,0 +1,683 @@
+// plan.cpp -- host prerequisites of the device path: neighbours, interface,
+// globs and corners, Dirichlet elimination, weights, constraints, coarse
+// numbering, condensation right-hand side and the device dof orderings.
+//
+// Semantics follow the reference (paths relative to its root):
+//   interface, shared nodes ..... src/module_dd.f90:8071-8647
+//   components .................. src/module_dd.f90:2662-2764
+//   globs, corner selection ..... src/module_dd.f90:8925-10554
+//   Dirichlet elimination ....... src/module_sm.f90:431-600, src/module_dd.f90:1081-1256
+//   weights ..................... src/module_dd.f90:11554-11725
+//   constraints ................. src/module_dd.f90:4298-4384, 4869-5145
+//   condensation rhs ............ src/module_dd.f90:6995-7136
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <numeric>
+#include <queue>
+
+#include "solver.hpp"
+
+namespace bddc {
+
+bool g_throw_on_error = false;
+
+[[noreturn]] void fatal(const char *routine, const std::string &msg) {
+  std::fprintf(stderr, "ERROR in %s: %s\n", routine, msg.c_str());
+  std::fflush(stderr);
+  if (g_throw_on_error) throw Error(std::string("ERROR in ") + routine + ": " + msg);
+  std::abort();
+}
+
+// ------------------------------------------------------------ components
+static int components(int n, const std::vector<int> &xadj, const std::vector<int> &adj,
+                      std::vector<int> &comp) {
+  comp.assign(n, 0);
+  int nc = 0;
+  std::vector<int> stack;
+  for (int v = 0; v < n; v++) {
+    if (comp[v]) continue;
+    comp[v] = ++nc;
+    stack.push_back(v);
+    while (!stack.empty()) {
+      int u = stack.back();
+      stack.pop_back();
+      for (int k = xadj[u]; k < xadj[u + 1]; k++)
+        if (!comp[adj[k]]) comp[adj[k]] = nc, stack.push_back(adj[k]);
+    }
+  }
+  return nc;
+}
+
+static void nodal_components(Sub &s) {
+  s.ncomp.assign(s.nnods, 1);
+  if (!s.find_components) return;
+  std::vector<int> kinet(s.nelems + 1, 0);
+  for (int e = 0; e < s.nelems; e++) kinet[e + 1] = kinet[e] + s.nnet[e];
+  std::vector<std::vector<int>> node_el(s.nnods);
+  for (int e = 0; e < s.nelems; e++)
+    for (int k = kinet[e]; k < kinet[e + 1]; k++) node_el[s.inet[k]].push_back(e);
+  std::vector<int> xadj, adj;
+  if (s.use_dual) {
+    xadj.assign(s.nelems + 1, 0);
+    std::vector<int> cnt(s.nelems, 0), touched;
+    for (int e = 0; e < s.nelems; e++) {
+      touched.clear();
+      for (int k = kinet[e]; k < kinet[e + 1]; k++)
+        for (int f : node_el[s.inet[k]])
+          if (f != e) {
+            if (cnt[f] == 0) touched.push_back(f);
+            cnt[f]++;
+          }
+      for (int f : touched) {
+        if (cnt[f] >= s.neighbouring) adj.push_back(f);
+        cnt[f] = 0;
+      }
+      xadj[e + 1] = (int)adj.size();
+    }
+    std::vector<int> ec;
+    components(s.nelems, xadj, adj, ec);
+    for (int e = 0; e < s.nelems; e++)
+      for (int k = kinet[e]; k < kinet[e + 1]; k++) s.ncomp[s.inet[k]] = ec[e];
+  } else {
+    xadj.assign(s.nnods + 1, 0);
+    std::vector<int> mark(s.nnods, -1);
+    for (int v = 0; v < s.nnods; v++) {
+      for (int e : node_el[v])
+        for (int k = kinet[e]; k < kinet[e + 1]; k++) {
+          int w = s.inet[k];
+          if (w != v && mark[w] != v) mark[w] = v, adj.push_back(w);
+        }
+      xadj[v + 1] = (int)adj.size();
+    }
+    components(s.nnods, xadj, adj, s.ncomp);
+  }
+}
+
+// -------------------------------------------------------------- topology
+static void interface_of(Solver &S, int is) {
+  Sub &s = S.subs[is];
+  // (neighbour, global node, local node) for every shared node
+  struct Sh {
+    int t, g, l;
+  };
+  std::vector<Sh> sh;
+  std::vector<char> shared(s.nnods, 0);
+  for (int l = 0; l < s.nnods; l++) {
+    int g = s.isngn[l];
+    int p0 = S.n2s_ptr[g], p1 = S.n2s_ptr[g + 1];
+    if (p1 - p0 < 2) continue;
+    shared[l] = 1;
+    for (int q = p0; q < p1; q++) {
+      int t = S.n2s_idx[q];
+      if (t != is) sh.push_back({t, g, l});
+    }
+  }
+  std::sort(sh.begin(), sh.end(), [](const Sh &a, const Sh &b) {
+    return a.t != b.t ? a.t < b.t : a.g < b.g;
+  });
+  s.iin.clear();
+  s.ifc_dof.clear();
+  s.int_dof.clear();
+  s.kdofi.clear();
+  std::vector<int> l2i(s.nnods, -1);
+  for (int l = 0; l < s.nnods; l++) {
+    if (shared[l]) {
+      l2i[l] = (int)s.iin.size();
+      s.iin.push_back(l);
+      s.kdofi.push_back((int)s.ifc_dof.size());
+      for (int d = s.kdof[l]; d < s.kdof[l + 1]; d++) s.ifc_dof.push_back(d);
+    } else
+      for (int d = s.kdof[l]; d < s.kdof[l + 1]; d++) s.int_dof.push_back(d);
+  }
+  s.kdofi.push_back((int)s.ifc_dof.size());
+  s.adj.clear();
+  s.shptr.assign(1, 0);
+  s.shnode.clear();
+  s.shcomp.clear();
+  for (size_t k = 0; k < sh.size(); k++) {
+    if (s.adj.empty() || s.adj.back() != sh[k].t) {
+      if (!s.adj.empty()) s.shptr.push_back((int)s.shnode.size());
+      s.adj.push_back(sh[k].t);
+    }
+    s.shnode.push_back(l2i[sh[k].l]);
+    // component of the node inside the neighbour
+    const Sub &u = S.subs[sh[k].t];
+    int cu = 1;
+    if (u.find_components) {
+      auto it = std::lower_bound(u.g2l.begin(), u.g2l.end(), std::make_pair(sh[k].g, -1));
+      cu = u.ncomp[it->second];
+    }
+    s.shcomp.push_back(cu);
+  }
+  if (!s.adj.empty()) s.shptr.push_back((int)s.shnode.size());
+}
+
+static bool same_key(const Sub &s, int a, int b) { return s.nodekey[a] == s.nodekey[b]; }
+
+// glob classification and face-triangle corner selection
+// (src/module_dd.f90:9130-9300 and 10017-10327)
+static void globs_first_pass(Solver &S, int is) {
+  Sub &s = S.subs[is];
+  int nnodi = (int)s.iin.size();
+  s.nsubnode.assign(nnodi, 0);
+  s.nodekey.assign(nnodi, {});
+  for (size_t a = 0; a < s.adj.size(); a++)
+    for (int q = s.shptr[a]; q < s.shptr[a + 1]; q++) {
+      int in = s.shnode[q];
+      s.nsubnode[in]++;
+      auto &k = s.nodekey[in];
+      k.push_back(s.adj[a]);
+      k.push_back(s.shcomp[q]);
+      k.push_back(s.ncomp[s.iin[in]]);
+    }
+  // group nodes with equal keys
+  std::map<std::vector<int>, std::vector<int>> groups;
+  for (int i = 0; i < nnodi; i++) groups[s.nodekey[i]].push_back(i);
+  std::vector<int> gt(nnodi, 0);
+  for (auto &kv : groups) {
+    int nsubn = (int)kv.first.size() / 3;
+    int t = nsubn == 1 ? 1 : (kv.second.size() > 1 ? 2 : 3);
+    for (int i : kv.second) gt[i] = t;
+  }
+  for (int i = 0; i < nnodi; i++)
+    if (s.nsubnode[i] == 1) gt[i] = 1;
+  if (S.use_corner) {
+    const int ndim = S.ndim, meshdim = S.meshdim;
+    for (size_t a = 0; a < s.adj.size(); a++) {
+      int q0 = s.shptr[a], n = s.shptr[a + 1] - q0;
+      bool face = false;
+      for (int q = 0; q < n && !face; q++) face = gt[s.shnode[q0 + q]] == 1;
+      if (!face) continue;
+      // combined interface components, renumbered densely in value order
+      auto densify = [](std::vector<int> &v) {
+        std::vector<int> u(v);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        for (int &x : v) x = int(std::lower_bound(u.begin(), u.end(), x) - u.begin()) + 1;
+        return (int)u.size();
+      };
+      std::vector<int> mine(n), theirs(n), comb(n);
+      for (int q = 0; q < n; q++) {
+        mine[q] = s.ncomp[s.iin[s.shnode[q0 + q]]];
+        theirs[q] = s.shcomp[q0 + q];
+      }
+      densify(mine);
+      int nth = densify(theirs);
+      for (int q = 0; q < n; q++) comb[q] = (mine[q] - 1) * nth + theirs[q];
+      int ncomb = densify(comb);
+      for (int c = 1; c <= ncomb; c++) {
+        std::vector<int> idx;
+        for (int q = 0; q < n; q++)
+          if (comb[q] == c) idx.push_back(q);
+        int m = (int)idx.size();
+        std::vector<double> X(3 * m, 0.0);
+        for (int k = 0; k < m; k++) {
+          int ln = s.iin[s.shnode[q0 + idx[k]]];
+          for (int d = 0; d < ndim; d++) X[3 * k + d] = s.xyz[(size_t)d * s.nnods + ln];
+        }
+        auto dist2 = [&](int i, int j) {
+          double r = 0;
+          for (int d = 0; d < ndim; d++) r += (X[3 * i + d] - X[3 * j + d]) * (X[3 * i + d] - X[3 * j + d]);
+          return r;
+        };
+        int sel[4] = {-1, -1, -1, -1};
+        // maxloc keeps the first maximum
+        auto argmax_from = [&](int base) {
+          int best = 0;
+          double bv = dist2(0, base);
+          for (int k = 1; k < m; k++) {
+            double v = dist2(k, base);
+            if (v > bv) bv = v, best = k;
+          }
+          return best;
+        };
+        if (m > 0) sel[0] = argmax_from(0);
+        if (meshdim > 1 && m > 1) {
+          sel[1] = argmax_from(sel[0]);
+          if (sel[1] == sel[0]) sel[1] = sel[0] == 0 ? 1 : 0;
+        }
+        if (meshdim > 2 && m > 2) {
+          const double *p1 = &X[3 * sel[0]], *p2 = &X[3 * sel[1]];
+          std::vector<double> area(m);
+          for (int k = 0; k < m; k++) {
+            const double *p = &X[3 * k];
+            double c1 = (p2[1] - p1[1]) * (p[2] - p1[2]) - (p[1] - p1[1]) * (p2[2] - p1[2]);
+            double c2 = (p2[0] - p1[0]) * (p[2] - p1[2]) - (p[0] - p1[0]) * (p2[2] - p1[2]);
+            double c3 = (p2[0] - p1[0]) * (p[1] - p1[1]) - (p[0] - p1[0]) * (p2[1] - p1[1]);
+            area[k] = c1 * c1 + c2 * c2 + c3 * c3;
+          }
+          // strict maximum searched forwards and backwards, starting from 0
+          double mv = 0.0;
+          for (int k = 0; k < m; k++)
+            if (area[k] > mv) mv = area[k], sel[2] = k;
+          mv = 0.0;
+          for (int k = m - 1; k >= 0; k--)
+            if (area[k] > mv) mv = area[k], sel[3] = k;
+          if ((sel[2] == sel[0] && sel[3] == sel[0]) || (sel[2] == sel[1] && sel[3] == sel[1])) {
+            int k = 0;
+            while (k < m && (k == sel[0] || k == sel[1])) k++;
+            sel[2] = k < m ? k : -1;
+            k = 0;
+            while (k < m && (k == sel[0] || k == sel[1] || k == sel[2])) k++;
+            sel[3] = k < m ? k : -1;
+          }
+        }
+        int nb = meshdim <= 2 ? 2 : 4;
+        for (int k = 0; k < nb; k++)
+          if (sel[k] >= 0) gt[s.shnode[q0 + idx[sel[k]]]] = 3;
+      }
+    }
+  }
+  s.lcorner.clear();
+  for (int i = 0; i < nnodi; i++)
+    if (gt[i] == 3) s.lcorner.push_back(s.isngn[s.iin[i]]);
+}
+
+// corners merged with neighbours, edges, faces, Dirichlet removal
+// (src/module_dd.f90:9322-9660)
+static void globs_second_pass(Solver &S, int is) {
+  Sub &s = S.subs[is];
+  int nnodi = (int)s.iin.size();
+  std::vector<int> merged(s.lcorner);
+  for (int t : s.adj) merged.insert(merged.end(), S.subs[t].lcorner.begin(), S.subs[t].lcorner.end());
+  std::sort(merged.begin(), merged.end());
+  merged.erase(std::unique(merged.begin(), merged.end()), merged.end());
+  std::vector<int> kind(nnodi, 0);  // 3 corner, -2 removed
+  s.corners.clear();
+  for (int i = 0; i < nnodi; i++) {
+    int g = s.isngn[s.iin[i]];
+    if (std::binary_search(merged.begin(), merged.end(), g)) kind[i] = 3, s.corners.push_back(g);
+  }
+  std::sort(s.corners.begin(), s.corners.end());
+  if (s.has_fix)
+    for (int i = 0; i < nnodi; i++) {
+      if (kind[i] == 3) continue;
+      int ln = s.iin[i];
+      for (int d = s.kdof[ln]; d < s.kdof[ln + 1]; d++)
+        if (s.ifix[d]) kind[i] = -2;
+    }
+  std::map<std::vector<int>, std::vector<int>> groups;
+  for (int i = 0; i < nnodi; i++)
+    if (kind[i] == 0) groups[s.nodekey[i]].push_back(i);
+  s.globs.clear();
+  for (auto &kv : groups) {
+    Sub::Glob g;
+    g.type = kv.first.size() / 3 == 1 ? 1 : 2;
+    g.nodes = kv.second;
+    g.key = INT32_MAX;
+    for (int i : g.nodes) g.key = std::min(g.key, s.isngn[s.iin[i]]);
+    s.globs.push_back(std::move(g));
+  }
+  std::sort(s.globs.begin(), s.globs.end(), [](const Sub::Glob &a, const Sub::Glob &b) {
+    return a.type != b.type ? a.type > b.type : a.key < b.key;
+  });
+}
+
+void build_topology(Solver &S) {
+  int ng = S.nnod_g;
+  S.n2s_ptr.assign(ng + 1, 0);
+  for (auto &s : S.subs)
+    for (int g : s.isngn) S.n2s_ptr[g + 1]++;
+  for (int k = 0; k < ng; k++) S.n2s_ptr[k + 1] += S.n2s_ptr[k];
+  S.n2s_idx.assign(S.n2s_ptr[ng], 0);
+  {
+    std::vector<int> fill(ng, 0);
+    for (int is = 0; is < S.nsub; is++)
+      for (int g : S.subs[is].isngn) S.n2s_idx[S.n2s_ptr[g] + fill[g]++] = is;
+  }
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++) {
+    Sub &s = S.subs[is];
+    nodal_components(s);
+    s.g2l.resize(s.nnods);
+    for (int l = 0; l < s.nnods; l++) s.g2l[l] = {s.isngn[l], l};
+    std::sort(s.g2l.begin(), s.g2l.end());
+  }
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++) interface_of(S, is);
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++) globs_first_pass(S, is);
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++) globs_second_pass(S, is);
+}
+
+// ------------------------------------------------------------ BC, weights
+// Sum interface values over all subdomains sharing a dof.
+static void exchange_sum(Solver &S, std::vector<std::vector<double>> &v) {
+  std::vector<double> acc(S.ndof_g, 0.0);
+  for (int is = 0; is < S.nsub; is++) {
+    const Sub &s = S.subs[is];
+    for (size_t k = 0; k < s.ifc_dof.size(); k++) acc[s.isvgvn[s.ifc_dof[k]]] += v[is][k];
+  }
+  for (int is = 0; is < S.nsub; is++) {
+    const Sub &s = S.subs[is];
+    for (size_t k = 0; k < s.ifc_dof.size(); k++) v[is][k] = acc[s.isvgvn[s.ifc_dof[k]]];
+  }
+}
+
+static void eliminate_bc(Sub &s) {
+  s.bc.assign(s.ndofs, 0.0);
+  if (!s.has_fix) return;
+  std::vector<char> diag_seen(s.ndofs, 0);
+  size_t nnz = s.av.size();
+  for (size_t k = 0; k < nnz; k++) {
+    int i = s.ai[k], j = s.aj[k];
+    double a = s.av[k];
+    if (i != j) {
+      if (s.ifix[i] && s.fixv[i] != 0.0 && a != 0.0 && !s.ifix[j]) s.bc[j] -= a * s.fixv[i];
+      if (s.ifix[j] && s.fixv[j] != 0.0 && a != 0.0 && !s.ifix[i]) s.bc[i] -= a * s.fixv[j];
+      if (s.ifix[i] || s.ifix[j]) s.av[k] = 0.0;
+    } else if (s.ifix[i]) {
+      if (std::fabs(a) < 1e-12) s.av[k] = a = 1.0;
+      if (s.fixv[i] != 0.0 && a != 0.0) s.bc[i] += a * s.fixv[i];
+      diag_seen[i] = 1;
+    }
+  }
+  bool added = false;
+  for (int i = 0; i < s.ndofs; i++)
+    if (s.ifix[i] && !diag_seen[i]) {
+      s.ai.push_back(i);
+      s.aj.push_back(i);
+      s.av.push_back(1.0);
+      s.bc[i] += s.fixv[i];
+      added = true;
+    }
+  if (added) {
+    std::vector<size_t> o(s.av.size());
+    std::iota(o.begin(), o.end(), 0);
+    std::sort(o.begin(), o.end(), [&](size_t a, size_t b) {
+      return s.ai[a] != s.ai[b] ? s.ai[a] < s.ai[b] : s.aj[a] < s.aj[b];
+    });
+    std::vector<int> ai(o.size()), aj(o.size());
+    std::vector<double> av(o.size());
+    for (size_t k = 0; k < o.size(); k++) ai[k] = s.ai[o[k]], aj[k] = s.aj[o[k]], av[k] = s.av[o[k]];
+    s.ai.swap(ai);
+    s.aj.swap(aj);
+    s.av.swap(av);
+  }
+}
+
+void apply_boundary_conditions(Solver &S) {
+  S.bc_global = false;
+  for (auto &s : S.subs) S.bc_global = S.bc_global || s.has_fix;
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++) eliminate_bc(S.subs[is]);
+  if (S.bc_global) {
+    std::vector<std::vector<double>> b(S.nsub);
+    for (int is = 0; is < S.nsub; is++) {
+      Sub &s = S.subs[is];
+      b[is].resize(s.ifc_dof.size());
+      for (size_t k = 0; k < s.ifc_dof.size(); k++) b[is][k] = s.bc[s.ifc_dof[k]];
+    }
+    exchange_sum(S, b);
+    for (int is = 0; is < S.nsub; is++) {
+      Sub &s = S.subs[is];
+      for (size_t k = 0; k < s.ifc_dof.size(); k++) s.bc[s.ifc_dof[k]] = b[is][k];
+    }
+  }
+}
+
+void build_weights(Solver &S) {
+  std::vector<std::vector<double>> rho(S.nsub), tot(S.nsub);
+  for (int is = 0; is < S.nsub; is++) {
+    Sub &s = S.subs[is];
+    size_t ni = s.ifc_dof.size();
+    rho[is].assign(ni, 1.0);
+    if (S.weights_type == 1) {
+      std::vector<double> dg(s.ndofs, 0.0);
+      for (size_t k = 0; k < s.av.size(); k++)
+        if (s.ai[k] == s.aj[k]) dg[s.ai[k]] += s.av[k];
+      for (size_t k = 0; k < ni; k++) {
+        rho[is][k] = dg[s.ifc_dof[k]];
+        if (rho[is][k] == 0.0) rho[is][k] = 1.0;
+      }
+    } else if (S.weights_type == 3) {
+      if ((int)s.dof_data.size() != s.ndofs)
+        fatal("DD_GET_MY_COEFFICIENTS_FOR_WEIGHTS", "dof_data must have ndofs entries for weights type 3");
+      for (size_t k = 0; k < ni; k++) rho[is][k] = s.dof_data[s.ifc_dof[k]];
+    } else if (S.weights_type != 0) {
+      fatal("DD_GET_MY_COEFFICIENTS_FOR_WEIGHTS",
+            "Type of weights not supported: " + std::to_string(S.weights_type));
+    }
+    tot[is] = rho[is];
+  }
+  exchange_sum(S, tot);
+  for (int is = 0; is < S.nsub; is++) {
+    Sub &s = S.subs[is];
+    s.w.resize(s.ifc_dof.size());
+    for (size_t k = 0; k < s.w.size(); k++) {
+      if (tot[is][k] == 0.0)
+        fatal("DD_WEIGHTS_PREPARE", "Zero in denominator in computation of weights for subdomain: " +
+                                        std::to_string(s.isub + S.numbase));
+      s.w[k] = rho[is][k] / tot[is][k];
+    }
+  }
+}
+
+// ------------------------------------------------------------ constraints
+void build_constraints(Solver &S) {
+  // global coarse nodes: corners by node id, then edges, faces by smallest
+  // node id (a permutation of the reference numbering; the coarse space is
+  // identical)
+  struct CN {
+    int order, key, nd;
+  };
+  std::vector<CN> all;
+  for (auto &s : S.subs) {
+    for (int g : s.corners) {
+      int ln = -1;
+      for (int l : s.iin)
+        if (s.isngn[l] == g) {
+          ln = l;
+          break;
+        }
+      all.push_back({0, g, s.nndf[ln]});
+    }
+    for (auto &gl : s.globs) {
+      int nd = 0;
+      for (int i : gl.nodes) nd = std::max(nd, s.nndf[s.iin[i]]);
+      all.push_back({gl.type == 2 ? 1 : 2, gl.key, S.use_arith ? nd : 0});
+    }
+  }
+  std::sort(all.begin(), all.end(), [](const CN &a, const CN &b) {
+    return a.order != b.order ? a.order < b.order : a.key < b.key;
+  });
+  std::vector<CN> u;
+  for (auto &c : all) {
+    if (!u.empty() && u.back().order == c.order && u.back().key == c.key) {
+      if (u.back().nd != c.nd) fatal("DD_EMBED_CNODES", "different number of dof at a coarse node from different subdomains");
+      continue;
+    }
+    u.push_back(c);
+  }
+  S.nnodc = (int)u.size();
+  S.ncorner = S.nedge = S.nface = 0;
+  S.nndfc.resize(u.size());
+  S.kdofc.assign(u.size() + 1, 0);
+  for (size_t k = 0; k < u.size(); k++) {
+    (u[k].order == 0 ? S.ncorner : (u[k].order == 1 ? S.nedge : S.nface))++;
+    S.nndfc[k] = u[k].nd;
+    S.kdofc[k + 1] = S.kdofc[k] + u[k].nd;
+  }
+  S.ndofc = S.kdofc[u.size()];
+  auto find = [&](int order, int key) {
+    auto it = std::lower_bound(u.begin(), u.end(), CN{order, key, 0}, [](const CN &a, const CN &b) {
+      return a.order != b.order ? a.order < b.order : a.key < b.key;
+    });
+    return int(it - u.begin());
+  };
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++) {
+    Sub &s = S.subs[is];
+    s.cptr.assign(1, 0);
+    s.cidx.clear();
+    s.cval.clear();
+    s.indrowc.clear();
+    std::vector<int> g2i;  // interface node of a global node (linear search on small sets)
+    for (int g : s.corners) {
+      int in = -1;
+      for (size_t k = 0; k < s.iin.size(); k++)
+        if (s.isngn[s.iin[k]] == g) {
+          in = (int)k;
+          break;
+        }
+      int id = find(0, g);
+      for (int d = 0; d < S.nndfc[id]; d++) {
+        s.cidx.push_back(s.kdofi[in] + d);
+        s.cval.push_back(1.0);
+        s.cptr.push_back((int)s.cidx.size());
+        s.indrowc.push_back(S.kdofc[id] + d);
+      }
+    }
+    for (auto &gl : s.globs) {
+      int id = find(gl.type == 2 ? 1 : 2, gl.key);
+      for (int d = 0; d < S.nndfc[id]; d++) {
+        for (int in : gl.nodes)
+          if (d < s.kdofi[in + 1] - s.kdofi[in]) {
+            s.cidx.push_back(s.kdofi[in] + d);
+            s.cval.push_back(1.0);
+          }
+        s.cptr.push_back((int)s.cidx.size());
+        s.indrowc.push_back(S.kdofc[id] + d);
+      }
+    }
+    s.nconstr = (int)s.indrowc.size();
+  }
+}
+
+// rhs used for condensation: weighted at the interface when it is a
+// restriction of a global rhs, Dirichlet values eliminated
+// (src/module_dd.f90:7068-7087, src/module_sm.f90:602-628)
+void build_condensation_rhs(Solver &S, int isub_only) {
+  for (int is = 0; is < S.nsub; is++) {
+    if (isub_only >= 0 && is != isub_only) continue;
+    Sub &s = S.subs[is];
+    s.fmod = s.rhs;
+    if (s.rhs_complete)
+      for (size_t k = 0; k < s.ifc_dof.size(); k++) s.fmod[s.ifc_dof[k]] *= s.w[k];
+    if (S.bc_global) {
+      std::vector<double> bc(s.bc);
+      for (size_t k = 0; k < s.ifc_dof.size(); k++) bc[s.ifc_dof[k]] *= s.w[k];
+      if (s.has_fix)
+        for (int d = 0; d < s.ndofs; d++)
+          if (s.ifix[d]) s.fmod[d] = 0.0;
+      for (int d = 0; d < s.ndofs; d++) s.fmod[d] += bc[d];
+    }
+  }
+}
+
+void build_unique_interface(Solver &S) {
+  std::vector<char> on(S.ndof_g, 0), fixed(S.ndof_g, 0);
+  for (auto &s : S.subs)
+    for (int d : s.ifc_dof) {
+      on[s.isvgvn[d]] = 1;
+      if (s.has_fix && s.ifix[d]) fixed[s.isvgvn[d]] = 1;
+    }
+  S.igdof.clear();
+  S.gpos.assign(S.ndof_g, -1);
+  for (int g = 0; g < S.ndof_g; g++)
+    if (on[g]) S.gpos[g] = (int)S.igdof.size(), S.igdof.push_back(g);
+  S.nig = (int)S.igdof.size();
+  S.ig_fixed.resize(S.nig);
+  for (int k = 0; k < S.nig; k++) S.ig_fixed[k] = fixed[S.igdof[k]];
+}
+
+// ------------------------------------------------------------- orderings
+// Interior dofs: reverse Cuthill-McKee on the interior graph from a
+// pseudo-peripheral start (George & Liu 1979).  Interface dofs: sorted by
+// the first interior position they couple to, so that the rows of the
+// interface block of the factor start as late as possible.
+static void order_subdomain(Sub &s) {
+  int n = s.ndofs;
+  std::vector<int> loc(n, -1);  // interior index
+  for (size_t k = 0; k < s.int_dof.size(); k++) loc[s.int_dof[k]] = (int)k;
+  int no = (int)s.int_dof.size(), ni = (int)s.ifc_dof.size();
+  std::vector<std::vector<int>> g(no);
+  for (size_t k = 0; k < s.av.size(); k++) {
+    int i = loc[s.ai[k]], j = loc[s.aj[k]];
+    if (i >= 0 && j >= 0 && i != j) g[i].push_back(j), g[j].push_back(i);
+  }
+  std::vector<int> order;
+  order.reserve(no);
+  std::vector<char> seen(no, 0);
+  auto bfs_last = [&](int start, std::vector<int> &level) {
+    level.assign(no, -1);
+    std::queue<int> q;
+    q.push(start);
+    level[start] = 0;
+    int last = start;
+    while (!q.empty()) {
+      int v = q.front();
+      q.pop();
+      last = v;
+      for (int w : g[v])
+        if (level[w] < 0) level[w] = level[v] + 1, q.push(w);
+    }
+    return last;
+  };
+  std::vector<int> level;
+  for (int root = 0; root < no; root++) {
+    if (seen[root]) continue;
+    int start = root;
+    for (int it = 0; it < 2; it++) {
+      int far = bfs_last(start, level);
+      // among the last level pick minimum degree
+      int lv = level[far], best = far;
+      for (int v = 0; v < no; v++)
+        if (level[v] == lv && g[v].size() < g[best].size()) best = v;
+      start = best;
+    }
+    std::queue<int> q;
+    q.push(start);
+    seen[start] = 1;
+    while (!q.empty()) {
+      int v = q.front();
+      q.pop();
+      order.push_back(v);
+      std::vector<int> nb;
+      for (int w : g[v])
+        if (!seen[w]) seen[w] = 1, nb.push_back(w);
+      std::sort(nb.begin(), nb.end(), [&](int a, int b) { return g[a].size() < g[b].size(); });
+      for (int w : nb) q.push(w);
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  std::vector<int> pos(no);
+  for (int k = 0; k < no; k++) pos[order[k]] = k;
+  // interface: first interior coupling position
+  std::vector<int> iloc(n, -1);
+  for (int k = 0; k < ni; k++) iloc[s.ifc_dof[k]] = k;
+  std::vector<int> first(ni, no);
+  for (size_t k = 0; k < s.av.size(); k++) {
+    int i = s.ai[k], j = s.aj[k];
+    if (loc[i] >= 0 && iloc[j] >= 0) first[iloc[j]] = std::min(first[iloc[j]], pos[loc[i]]);
+    if (loc[j] >= 0 && iloc[i] >= 0) first[iloc[i]] = std::min(first[iloc[i]], pos[loc[j]]);
+  }
+  std::vector<int> io(ni);
+  std::iota(io.begin(), io.end(), 0);
+  std::stable_sort(io.begin(), io.end(), [&](int a, int b) { return first[a] < first[b]; });
+  s.no = no;
+  s.ni = ni;
+  s.NO = (no + TILE - 1) / TILE * TILE;
+  s.NI = (ni + TILE - 1) / TILE * TILE;
+  if (s.NI == 0) s.NI = TILE;
+  s.nto = s.NO / TILE;
+  s.nti = s.NI / TILE;
+  s.perm.assign(n, -1);
+  for (int k = 0; k < no; k++) s.perm[s.int_dof[order[k]]] = k;
+  s.dev_ifc.resize(ni);
+  for (int k = 0; k < ni; k++) {
+    s.dev_ifc[k] = io[k];
+    s.perm[s.ifc_dof[io[k]]] = s.NO + k;
+  }
+}
+
+void build_orderings(Solver &S) {
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++) order_subdomain(S.subs[is]);
+}
+
+}  // namespace bddc

@@ -1,0 +1,85 @@
+// solver.hpp -- process-wide solver state behind the bddcml_*_c interface.
+// The reference keeps the same kind of module state (levels(:), the coarse
+// MUMPS instance, Krylov recycling basis; src/module_levels.f90:158-175,
+// src/module_krylov.f90:34-43): one solver per process, not reentrant.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace bddc {
+
+struct Device;  // device-side plan and buffers (device.hip)
+
+struct Timings {
+  double topology = 0, plan = 0, upload = 0, factor = 0, schur_block = 0, interface = 0,
+         coarse = 0, rhs = 0, krylov = 0, postprocess = 0, setup_total = 0, solve_total = 0;
+};
+
+struct Solver {
+  // init
+  int nl = 0, nsub = 0, numbase = 1, verbose = 0;
+  bool initialised = false;
+  // global sizes
+  int nelem_g = 0, nnod_g = 0, ndof_g = 0, ndim = 3, meshdim = 3;
+  int matrixtype = -1;
+  std::vector<Sub> subs;
+  // options
+  int use_corner = 1, use_arith = 1, weights_type = 0;
+  int precision = 64;  // 64: FP64 everywhere; 32: FP32 preconditioner factors
+  // topology results
+  int ncorner = 0, nedge = 0, nface = 0, nnodc = 0, ndofc = 0;
+  std::vector<int> nndfc, kdofc;
+  bool bc_global = false;
+  // node -> subdomains
+  std::vector<int> n2s_ptr, n2s_idx;
+  // unique interface
+  int nig = 0;
+  std::vector<int> igdof;  // sorted global dofs
+  std::vector<int> gpos;   // global dof -> unique position or -1
+  std::vector<char> ig_fixed;
+  // state
+  bool is_setup = false, solved = false, plan_only = false;
+  std::vector<double> uig;  // interface solution
+  std::unique_ptr<Device> dev;
+  Timings t;
+  int last_iters = 0, last_reason = 0;
+  double last_cond = 0;
+  std::vector<double> hist;
+
+  Solver();
+  ~Solver();
+};
+
+Solver &solver();
+void reset_solver();
+
+// host-side setup steps (plan.cpp)
+void build_topology(Solver &S);
+void apply_boundary_conditions(Solver &S);
+void build_weights(Solver &S);
+void build_constraints(Solver &S);
+void build_condensation_rhs(Solver &S, int isub_only = -1);
+void build_unique_interface(Solver &S);
+void build_orderings(Solver &S);
+
+// device steps (device.hip)
+void device_setup(Solver &S);
+void device_reduced_rhs(Solver &S);
+void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, int *num_iter,
+                  int *reason, double *cond);
+void device_apply_schur(Solver &S, const double *x, double *y);
+void device_apply_pc(Solver &S, const double *r, double *z);
+void device_reduced_rhs_download(Solver &S, double *g);
+void device_sub_matrix(Solver &S, int is, int which, double *out);
+void device_sub_coarse(Solver &S, int is, double *phi, double *ac);
+void device_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
+                          const double *X, double alpha, double beta, double *out);
+void device_release(Solver &S);
+
+double wall_time();
+
+}  // namespace bddc

@@ -1,0 +1,61 @@
+/*
+ * bddcml_amd.h -- extra entry points of the MI355X BDDC library.
+ *
+ * These have no counterpart in the reference interface: they expose the
+ * device operators of the interface problem for parity tests, timings for
+ * benchmarks and a few options.  Vectors over the interface are ordered by
+ * increasing global dof index (see bddcml_amd_interface_dofs).  All
+ * functions return 0 on success and a negative value on misuse.
+ */
+#ifndef BDDCML_AMD_H
+#define BDDCML_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* out[0]=ncorner out[1]=nedge out[2]=nface out[3]=ndofc out[4]=interface dofs */
+int bddcml_amd_counts(int *out);
+/* sorted global dof ids (0-based) of the interface, length out[4] above */
+int bddcml_amd_interface_dofs(int *out);
+/* y = S x, assembled Schur complement on the interface (device) */
+int bddcml_amd_apply_schur(const double *x, double *y);
+/* z = M r, BDDC preconditioner (device) */
+int bddcml_amd_apply_pc(const double *r, double *z);
+/* reduced right-hand side g = f_G - sum A21 A11^-1 f_I (device) */
+int bddcml_amd_reduced_rhs(double *g);
+/* interface solution of the last solve */
+int bddcml_amd_interface_solution(double *u);
+/* relative residual history of the last solve; returns its length */
+int bddcml_amd_residual_history(double *out, int maxn);
+/* phase timings in seconds: topology plan factor krylov postprocess
+ * setup_total solve_total (7 values) */
+int bddcml_amd_timings(double *out, int n);
+/* options: "verbose" (0/1/2), "plan_only" (1: setup stops after the host
+ * plan, no device work; for inspecting topology without a GPU) */
+int bddcml_amd_set_option(const char *key, double value);
+/* per subdomain (isub in numbase): out[0]=ndofi out[1]=nconstr out[2]=ndofo */
+int bddcml_amd_sub_info(int isub, int *out);
+/* global dof of each local interface dof (reference interface order) and
+ * global coarse dof of each constraint row */
+int bddcml_amd_sub_maps(int isub, int *gdofs, int *indrowc);
+/* dense interface matrices of one subdomain, row major, reference interface
+ * order: which = 'S' (Schur complement) or 'K' (interface block of
+ * [S C^T; C 0]^-1); Phi (ndofi x nconstr) and Ac (nconstr x nconstr) */
+int bddcml_amd_sub_matrix(int isub, int which, double *out);
+int bddcml_amd_sub_coarse(int isub, double *phi, double *ac);
+/* message of the last error raised while option "throw_on_error" is set
+ * (then errors are reported instead of aborting); returns its length and
+ * clears it */
+int bddcml_amd_last_error(char *buf, int len);
+/* runs one batched tile task on the device on 64x64 column-major tiles:
+ * op 0 NT, 1 NN, 2 TN product sums (+10 post C X^T, +20 post X C),
+ * op 3 Cholesky of C - sum A B^T with inverse; out = result tile, and for
+ * op 3 the inverse in out[4096..8191] */
+int bddcml_amd_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
+                             const double *X, double alpha, double beta, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

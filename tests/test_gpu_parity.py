@@ -1,0 +1,98 @@
+"""GPU parity of the HIP path against the CPU oracle on the same inputs.
+
+Each case prints its name before touching the GPU.  Tolerances (FP64 path):
+Schur complements 1e-12 relative, coarse basis / coarse matrices 1e-10,
+operators S and M 1e-11 / 1e-10 relative, interface solution 1e-10
+relative with PCG iteration counts within +-1.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import bddcml_amd
+from bddcml_amd.cube import CubeProblem
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle(oracle_lib, p, wt=0, method=0, tol=1e-10):
+    return oracle_lib.solve_cube(p, weights_type=wt, method=method, tol=tol, maxit=300)
+
+
+def _relmax(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("nel,ns,kind,wt", [(2, 2, "poisson", 0), (3, 3, "poisson", 1),
+                                            (3, 2, "elasticity", 0)])
+def test_subdomain_matrices(oracle_lib, nel, ns, kind, wt):
+    _say(f"case subdomain_matrices nel={nel} ns={ns} kind={kind} wt={wt}")
+    p = CubeProblem(nel, ns, kind=kind, rhs="random", seed=9)
+    o, _ = _oracle(oracle_lib, p, wt)
+    b, _ = bddcml_amd.solve_cube(p, weights_type=wt, tol=1e-10, maxit=300)
+    for s in range(p.nsub):
+        isub = s + 1
+        S_ref = o.sub_schur(isub)
+        S = b.sub_matrix(isub, "S")
+        assert _relmax(S, S_ref) < 1e-12, f"S of subdomain {isub}"
+        # coarse basis and coarse matrix, columns matched by coarse dof
+        g, ind = b.sub_maps(isub)
+        _, ind_ref = o.sub_constraints(isub)
+        pos = {int(c): k for k, c in enumerate(ind_ref)}
+        perm = np.array([pos[int(c)] for c in ind])
+        phi, ac = b.sub_coarse(isub)
+        phi_ref = o.sub_phi(isub)[:, perm]
+        ac_ref = o.sub_coarse(isub)[np.ix_(perm, perm)]
+        assert _relmax(phi, phi_ref) < 1e-10, f"Phi of subdomain {isub}"
+        assert _relmax(ac, ac_ref) < 1e-10, f"Ac of subdomain {isub}"
+    o.close()
+    b.finalize()
+
+
+@pytest.mark.parametrize("nel,ns,kind,wt", [(2, 2, "poisson", 0), (4, 3, "poisson", 1),
+                                            (3, 2, "elasticity", 0)])
+def test_operators(oracle_lib, nel, ns, kind, wt):
+    _say(f"case operators nel={nel} ns={ns} kind={kind} wt={wt}")
+    p = CubeProblem(nel, ns, kind=kind, rhs="random", seed=5)
+    o, _ = _oracle(oracle_lib, p, wt)
+    b, _ = bddcml_amd.solve_cube(p, weights_type=wt, tol=1e-10, maxit=300)
+    ig = b.interface_dofs()
+    assert np.array_equal(ig, o.interface_dofs())
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(len(ig))
+    assert _relmax(b.apply_schur(x), o.apply_schur(x)) < 1e-11
+    assert _relmax(b.apply_pc(x), o.apply_pc(x)) < 1e-10
+    assert _relmax(b.reduced_rhs(), o.reduced_rhs()) < 1e-12
+    o.close()
+    b.finalize()
+
+
+@pytest.mark.parametrize("nel,ns,kind,wt,method,rhs", [
+    (8, 4, "poisson", 0, 0, "const"),      # 32^3 Q1, 64 subdomains of 8^3
+    (4, 3, "poisson", 1, 0, "random"),
+    (3, 3, "poisson", 0, 1, "random"),     # BiCGSTAB
+    (3, 2, "elasticity", 0, 0, "random"),
+])
+def test_solve_matches_oracle(oracle_lib, nel, ns, kind, wt, method, rhs):
+    _say(f"case solve nel={nel} ns={ns} kind={kind} wt={wt} method={method} rhs={rhs}")
+    p = CubeProblem(nel, ns, kind=kind, rhs=rhs, seed=2)
+    o, (it_o, r_o, c_o) = _oracle(oracle_lib, p, wt, method=method, tol=1e-10)
+    b, (it, r, c) = bddcml_amd.solve_cube(p, weights_type=wt, method=method, tol=1e-10, maxit=300)
+    assert r == r_o == 0
+    assert abs(it - it_o) <= 1
+    if method == 0:
+        assert abs(c - c_o) <= 1e-6 * c_o
+    u_ref = o.interface_solution()
+    assert _relmax(b.interface_solution(), u_ref) < 1e-10
+    for d in p.subdomains():
+        ls = b.download_local_solution(d["isub"], d["ndofs"])
+        ls_ref = o.local_solution(d["isub"])
+        assert np.abs(ls - ls_ref).max() <= 1e-10 * np.abs(u_ref).max()
+    o.close()
+    b.finalize()

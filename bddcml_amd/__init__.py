@@ -243,9 +243,12 @@ class Bddcml:
         return out
 
     def timings(self):
-        out = np.zeros(7)
-        self.L.bddcml_amd_timings(out.ctypes.data_as(C.POINTER(C.c_double)), 7)
-        keys = ("topology", "plan", "factor", "krylov", "postprocess", "setup_total", "solve_total")
+        out = np.zeros(18)
+        self.L.bddcml_amd_timings(out.ctypes.data_as(C.POINTER(C.c_double)), 18)
+        keys = ("topology", "plan", "factor", "krylov", "postprocess", "setup_total",
+                "solve_total", "rhs", "ev_interior_ms", "ev_schur_ms", "ev_interface_ms",
+                "ev_coarse_ms", "schur_flops", "interior_flops", "interface_flops",
+                "ev_gemv_ms", "gemv_bytes", "iterations")
         return dict(zip(keys, out.tolist()))
 
     def sub_info(self, isub):

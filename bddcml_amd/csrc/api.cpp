@@ -192,13 +192,28 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
     S.use_arith = *use_arithmetic_constraints_int != 0;
     S.weights_type = *weights_type;
   }
-  build_topology(S);
-  apply_boundary_conditions(S);
-  build_weights(S);
-  build_constraints(S);
-  build_condensation_rhs(S);
-  build_unique_interface(S);
-  build_orderings(S);
+  {
+    double tp = wall_time();
+    auto phase = [&](const char *name) {
+      double now = wall_time();
+      if (S.verbose >= 2) std::printf(" BDDCML_SETUP_PRECONDITIONER: %-22s %8.4f s\n", name, now - tp);
+      tp = now;
+    };
+    build_topology(S);
+    phase("topology");
+    apply_boundary_conditions(S);
+    phase("boundary conditions");
+    build_weights(S);
+    phase("weights");
+    build_constraints(S);
+    phase("constraints");
+    build_condensation_rhs(S);
+    phase("condensation rhs");
+    build_unique_interface(S);
+    phase("unique interface");
+    build_orderings(S);
+    phase("orderings");
+  }
   S.t.topology += wall_time() - t0;
   if (S.plan_only) {
     S.is_setup = true;
@@ -416,9 +431,12 @@ catch (const Error &e) {
 }
 int bddcml_amd_timings(double *out, int n) try {
   Solver &S = solver();
-  double v[7] = {S.t.topology, S.t.plan, S.t.factor, S.t.krylov, S.t.postprocess, S.t.setup_total,
-                 S.t.solve_total};
-  for (int k = 0; k < std::min(n, 7); k++) out[k] = v[k];
+  double v[18] = {S.t.topology,        S.t.plan,           S.t.factor,          S.t.krylov,
+                  S.t.postprocess,     S.t.setup_total,    S.t.solve_total,     S.t.rhs,
+                  S.t.ev_interior_ms,  S.t.ev_schur_ms,    S.t.ev_interface_ms, S.t.ev_coarse_ms,
+                  S.t.schur_flops,     S.t.interior_flops, S.t.interface_flops, S.t.ev_gemv_ms,
+                  S.t.gemv_bytes,      (double)S.last_iters};
+  for (int k = 0; k < std::min(n, 18); k++) out[k] = v[k];
   return 0;
 }
 catch (const Error &e) {

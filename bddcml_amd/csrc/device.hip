@@ -729,9 +729,25 @@ void device_setup(Solver &S) {
   HIPCHECK(hipStreamSynchronize(st));
   S.t.plan += wall_time() - t0;
 
+  // algorithmic work: 2*64^3 flops per tile product, 64^3/3 per tile Cholesky
+  // plus 64^3/3 for its inverse, 2*64^3 per post multiply
+  {
+    const double g = 2.0 * 64 * 64 * 64, c = 64.0 * 64 * 64 / 3.0;
+    auto work = [&](const std::vector<TaskList> &v, double per_task) {
+      double f = 0;
+      for (auto &l : v) f += g * l.pairs.size() + per_task * l.tasks.size();
+      return f;
+    };
+    S.t.schur_flops = g * a_schur.pairs.size();
+    S.t.interior_flops = work(a_diag, 2 * c) + work(a_off, g) + S.t.schur_flops;
+    S.t.interface_flops = work(p_diag, 2 * c) + work(p_off, g) + work(w_step, g) + g * kraw.pairs.size();
+  }
   // ---- numerics
   t0 = wall_time();
   double *pool = D.pool.p;
+  hipEvent_t ev[6];
+  for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
+  HIPCHECK(hipEventRecord(ev[0], st));
   launch_zero_tiles(st, pool, D.zero_ids.p, (int)D.zero_ids.n);
   launch_scatter(st, pool, D.sc_idx.p, D.sc_val.p, (int64_t)D.sc_idx.n);
   for (size_t J = 0; J < D.a_diag.size(); J++) {
@@ -739,8 +755,10 @@ void device_setup(Solver &S) {
     launch_tile_gemm(st, pool, D.a_off[J].tasks.p, D.a_off[J].pairs.p, D.a_off[J].ntask, GEMM_NT,
                      POST_RIGHT_XT, 1.0, -1.0);
   }
+  HIPCHECK(hipEventRecord(ev[1], st));
   launch_tile_gemm(st, pool, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask, GEMM_NT,
                    POST_NONE, 1.0, -1.0);
+  HIPCHECK(hipEventRecord(ev[2], st));
   launch_tile_mirror(st, pool, D.mir_s.p, (int)D.mir_s.n);
   // augmented interface matrices
   launch_sprime(st, pool, D.desc.p, D.sp_sub.p, D.sp_ij.p, D.sp_dst.p, D.n_sp, D.cid.p, D.csize.p);
@@ -761,6 +779,7 @@ void device_setup(Solver &S) {
   launch_phi(st, D.desc.p, nsub, D.Z.p, D.G.p, D.Phi.p);
   launch_kupdate(st, pool, D.desc.p, D.ku_sub.p, D.ku_ij.p, D.n_ku, D.Phi.p, D.Z.p);
   launch_acs(st, D.desc.p, nsub, D.cptr.p, D.G.p, D.Acs.p);
+  HIPCHECK(hipEventRecord(ev[3], st));
   // coarse problem
   launch_coarse_assemble(st, pool, D.ca_dst.p, D.ca_ptr.p, D.ca_src.p, D.nca, D.Acs.p);
   for (size_t J = 0; J < D.c_diag.size(); J++) {
@@ -774,8 +793,21 @@ void device_setup(Solver &S) {
   launch_tile_gemm(st, pool, D.cinv.tasks.p, D.cinv.pairs.p, D.cinv.ntask, GEMM_TN, POST_NONE, 0.0,
                    1.0);
   launch_tile_mirror(st, pool, D.mir_c.p, (int)D.mir_c.n);
+  HIPCHECK(hipEventRecord(ev[4], st));
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(st));
+  {
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, ev[0], ev[2]));
+    S.t.ev_interior_ms = ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ev[1], ev[2]));
+    S.t.ev_schur_ms = ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ev[2], ev[3]));
+    S.t.ev_interface_ms = ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ev[3], ev[4]));
+    S.t.ev_coarse_ms = ms;
+    for (auto &e : ev) (void)hipEventDestroy(e);
+  }
   int herr = 0;
   HIPCHECK(hipMemcpy(&herr, D.err.p, sizeof(int), hipMemcpyDeviceToHost));
   if (herr & 1) fatal("BDDCML_SETUP_PRECONDITIONER", "matrix not positive definite in a tile Cholesky factorization");
@@ -879,10 +911,29 @@ void device_sub_coarse(Solver &S, int is, double *phi, double *ac) {
 }
 
 // --------------------------------------------------------------- operators
-static void op_schur(Solver &S, const double *x, double *y) {
+static void op_schur(Solver &S, const double *x, double *y, bool timed = false) {
   Device &D = *S.dev;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timed) {
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, D.st));
+  }
   launch_gemv(D.st, D.pool.p, D.srows.p, D.nsrows, D.loff.p, D.NIs.p, D.map.p, nullptr, x, D.yl.p,
               D.maxNI);
+  if (timed) {
+    HIPCHECK(hipEventRecord(e1, D.st));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    S.t.ev_gemv_ms = ms;
+    double tiles = 0;
+    for (const SubTiles &T : D.sub) tiles += (double)T.nti * T.nti;
+    // matrix tiles + gathered input + written output (8 bytes each)
+    S.t.gemv_bytes = tiles * TILE_ELEMS * 8.0 + 16.0 * (double)D.LV;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
   launch_assemble(D.st, S.nig, D.aptr.p, D.asrc.p, nullptr, D.yl.p, y);
 }
 
@@ -1033,7 +1084,7 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, in
     int ndecr = 0, nw = 0;
     double lastres = 1.0;
     for (int iter = 1; iter <= maxit; iter++) {
-      op_schur(S, D.p.p, D.ap.p);
+      op_schur(S, D.p.p, D.ap.p, iter == 2);
       dots(S, {{D.p.p, D.ap.p}}, sc);
       double pap = sc[0];
       if (pap <= 0.0 && S.verbose > 0)

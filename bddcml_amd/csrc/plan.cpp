@@ -328,6 +328,12 @@ void build_topology(Solver &S) {
     for (int is = 0; is < S.nsub; is++)
       for (int g : S.subs[is].isngn) S.n2s_idx[S.n2s_ptr[g] + fill[g]++] = is;
   }
+  double tp = wall_time();
+  auto phase = [&](const char *name) {
+    double now = wall_time();
+    if (S.verbose >= 2) std::printf("   topology: %-20s %8.4f s\n", name, now - tp);
+    tp = now;
+  };
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < S.nsub; is++) {
     Sub &s = S.subs[is];
@@ -336,12 +342,16 @@ void build_topology(Solver &S) {
     for (int l = 0; l < s.nnods; l++) s.g2l[l] = {s.isngn[l], l};
     std::sort(s.g2l.begin(), s.g2l.end());
   }
+  phase("components");
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < S.nsub; is++) interface_of(S, is);
+  phase("interface");
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < S.nsub; is++) globs_first_pass(S, is);
+  phase("globs pass 1");
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < S.nsub; is++) globs_second_pass(S, is);
+  phase("globs pass 2");
 }
 
 // ------------------------------------------------------------ BC, weights

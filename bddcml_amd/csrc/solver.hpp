@@ -17,6 +17,11 @@ struct Device;  // device-side plan and buffers (device.hip)
 struct Timings {
   double topology = 0, plan = 0, upload = 0, factor = 0, schur_block = 0, interface = 0,
          coarse = 0, rhs = 0, krylov = 0, postprocess = 0, setup_total = 0, solve_total = 0;
+  // device event times (ms) and algorithmic work of the measured launches
+  double ev_interior_ms = 0, ev_schur_ms = 0, ev_interface_ms = 0, ev_coarse_ms = 0;
+  double schur_flops = 0, interior_flops = 0, interface_flops = 0;
+  double ev_gemv_ms = 0, gemv_bytes = 0;  // one batched S GEMV launch
+  int iterations = 0;
 };
 
 struct Solver {

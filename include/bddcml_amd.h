@@ -28,8 +28,12 @@ int bddcml_amd_reduced_rhs(double *g);
 int bddcml_amd_interface_solution(double *u);
 /* relative residual history of the last solve; returns its length */
 int bddcml_amd_residual_history(double *out, int maxn);
-/* phase timings in seconds: topology plan factor krylov postprocess
- * setup_total solve_total (7 values) */
+/* up to 18 values: wall seconds of topology plan factor krylov postprocess
+ * setup_total solve_total rhs; device event milliseconds of the interior
+ * factorisation, the Schur-block GEMM launch, the interface phase, the
+ * coarse phase; algorithmic flops of the Schur-block launch, the interior
+ * factorisation and the interface phase; event milliseconds and algorithmic
+ * bytes of one batched Schur GEMV; last iteration count */
 int bddcml_amd_timings(double *out, int n);
 /* options: "verbose" (0/1/2), "plan_only" (1: setup stops after the host
  * plan, no device work; for inspecting topology without a GPU) */

@@ -163,6 +163,32 @@ void bddcml_upload_subdomain_data_c(
     else
       s.ai.push_back(i), s.aj.push_back(j), s.av.push_back(a_sparse[k]);
   }
+  // a fixed dof without a diagonal entry gets an explicit zero one; the
+  // Dirichlet elimination turns it into 1 (src/module_sm.f90:556-585)
+  {
+    std::vector<char> has_diag(s.ndofs, 0);
+    for (size_t k = 0; k < s.ai.size(); k++)
+      if (s.ai[k] == s.aj[k]) has_diag[s.ai[k]] = 1;
+    bool added = false;
+    for (int d = 0; d < s.ndofs; d++)
+      if (s.ifix[d] && !has_diag[d]) {
+        s.ai.push_back(d), s.aj.push_back(d), s.av.push_back(0.0);
+        added = true;
+      }
+    if (added) {
+      std::vector<size_t> o(s.ai.size());
+      std::iota(o.begin(), o.end(), 0);
+      std::sort(o.begin(), o.end(), [&](size_t x, size_t y) {
+        return s.ai[x] != s.ai[y] ? s.ai[x] < s.ai[y] : s.aj[x] < s.aj[y];
+      });
+      std::vector<int> ai(o.size()), aj(o.size());
+      std::vector<double> av(o.size());
+      for (size_t q = 0; q < o.size(); q++) ai[q] = s.ai[o[q]], aj[q] = s.aj[o[q]], av[q] = s.av[o[q]];
+      s.ai.swap(ai), s.aj.swap(aj), s.av.swap(av);
+    }
+  }
+  // the assembled values go to HBM now; setup permutes and eliminates them there
+  if (!S.plan_only) device_upload_entries(S, s.isub);
   s.uploaded = true;
 }
 catch (const Error &e) {

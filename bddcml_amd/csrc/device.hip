@@ -118,7 +118,10 @@ struct Device {
   // setup phases
   std::vector<Phase> a_diag, a_off, p_diag, p_off, w_step, c_diag, c_off, cw_step;
   Phase a_schur, kraw, cinv;
-  DBuf<int> zero_ids;
+  DBuf<int> zero_ids, sc_perm, sc_ttab;
+  DBuf<char> sc_fix;
+  DBuf<ScatterSub> sc_subs;
+  DBuf<int2> zero_rng;
   DBuf<int64_t> sc_idx;
   DBuf<double> sc_val;
   DBuf<MirrorJob> mir_s, mir_k, mir_c;
@@ -158,8 +161,35 @@ struct Device {
   int nig = 0;
 };
 
+// Assembled subdomain triplets kept in HBM from upload time on.
+struct DeviceInput {
+  std::vector<DBuf<int>> ai, aj;
+  std::vector<DBuf<double>> av;
+};
+
 Solver::Solver() = default;
 Solver::~Solver() = default;
+
+void device_upload_entries(Solver &S, int is) {
+  if (!S.din) {
+    S.din.reset(new DeviceInput());
+    S.din->ai.resize(S.nsub);
+    S.din->aj.resize(S.nsub);
+    S.din->av.resize(S.nsub);
+  }
+  const Sub &s = S.subs[is];
+  DeviceInput &I = *S.din;
+  I.ai[is].alloc(s.ai.size());
+  I.aj[is].alloc(s.aj.size());
+  I.av[is].alloc(s.av.size());
+  if (!s.ai.empty()) {
+    HIPCHECK(hipMemcpy(I.ai[is].p, s.ai.data(), s.ai.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(I.aj[is].p, s.aj.data(), s.aj.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(I.av[is].p, s.av.data(), s.av.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+}
+
+void device_release_input(Solver &S) { S.din.reset(); }
 
 void device_release(Solver &S) {
   if (S.dev) {
@@ -276,6 +306,38 @@ static void check_tasks(const TaskList &l, int64_t ntiles, bool uses_x, const ch
     if (bad(p.a) || bad(p.b)) fatal("BDDCML_DEVICE_PLAN", std::string("invalid tile pair in ") + what);
 }
 
+// Concatenates per-subdomain task lists so that the tasks of subdomain s sit
+// at positions congruent to s mod 8: workgroups are dealt round robin to the
+// 8 XCDs, so every subdomain's tiles are touched from a single L2.
+static TaskList xcd_merge(const std::vector<const TaskList *> &parts) {
+  constexpr int NX = 8;
+  std::vector<std::vector<std::pair<int, int>>> bins(NX);
+  size_t nt = 0, np = 0;
+  for (size_t s = 0; s < parts.size(); s++) {
+    if (!parts[s]) continue;
+    nt += parts[s]->tasks.size();
+    np += parts[s]->pairs.size();
+    for (size_t k = 0; k < parts[s]->tasks.size(); k++) bins[s % NX].push_back({(int)s, (int)k});
+  }
+  TaskList out;
+  out.tasks.reserve(nt);
+  out.pairs.reserve(np);
+  size_t longest = 0;
+  for (auto &b : bins) longest = std::max(longest, b.size());
+  for (size_t q = 0; q < longest; q++)
+    for (int x = 0; x < NX; x++) {
+      if (q >= bins[x].size()) continue;
+      const TaskList &L = *parts[bins[x][q].first];
+      TileTask t = L.tasks[bins[x][q].second];
+      int p0 = (int)out.pairs.size();
+      out.pairs.insert(out.pairs.end(), L.pairs.begin() + t.p0, L.pairs.begin() + t.p1);
+      t.p0 = p0;
+      t.p1 = (int)out.pairs.size();
+      out.tasks.push_back(t);
+    }
+  return out;
+}
+
 static void upload_phases(std::vector<Phase> &dst, const std::vector<TaskList> &src,
                           hipStream_t st) {
   dst.resize(src.size());
@@ -291,6 +353,12 @@ void device_setup(Solver &S) {
   hipStream_t st = D.st;
   const int nsub = S.nsub;
   double t0 = wall_time();
+  double tph = t0;
+  auto ph = [&](const char *name) {
+    double now = wall_time();
+    if (S.verbose >= 2) std::printf("   device plan: %-24s %8.4f s\n", name, now - tph);
+    tph = now;
+  };
 
   // ---- symbolic patterns and tile ids
   D.sub.resize(nsub);
@@ -310,17 +378,27 @@ void device_setup(Solver &S) {
   next += D.ntc * D.ntc;
   D.ntiles = next;
 
-  // ---- task lists
-  std::vector<TaskList> a_diag, a_off, p_diag, p_off, w_step, c_diag, c_off, cw_step;
-  TaskList a_schur, kraw, cinv;
+  ph("symbolic");
+  // ---- task lists: generated per subdomain in parallel, then merged per
+  // phase so that the workgroups of one subdomain land on one XCD
+  // (workgroups are dealt to the 8 XCDs round robin), keeping its tiles in
+  // that XCD's L2
+  struct SubTasks {
+    std::vector<TaskList> a_diag, a_off, p_diag, p_off, w_step;
+    TaskList a_schur, kraw;
+  };
+  std::vector<SubTasks> sts(nsub);
+#pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < nsub; is++) {
     const SubTiles &T = D.sub[is];
+    SubTasks &Q = sts[is];
     const int nto = T.nto, nt = T.nto + T.nti;
     // row sets restricted to factored columns
     std::vector<std::vector<int>> rowk(nt);
     for (int J = 0; J < nto; J++)
       for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) rowk[T.pat.rowidx[q]].push_back(J);
-    if ((int)a_diag.size() < nto) a_diag.resize(nto), a_off.resize(nto);
+    Q.a_diag.resize(nto);
+    Q.a_off.resize(nto);
     auto common = [&](int I, int J, int kmax, TaskList &L) {
       const auto &a = rowk[I], &b = rowk[J];
       size_t x = 0, y = 0;
@@ -335,34 +413,58 @@ void device_setup(Solver &S) {
       }
     };
     for (int J = 0; J < nto; J++) {
-      TileTask t{T.pat.find(J, J), T.linvA + J, (int)a_diag[J].pairs.size(), 0};
-      common(J, J, J, a_diag[J]);
-      t.p1 = (int)a_diag[J].pairs.size();
-      a_diag[J].tasks.push_back(t);
+      TileTask t{T.pat.find(J, J), T.linvA + J, (int)Q.a_diag[J].pairs.size(), 0};
+      common(J, J, J, Q.a_diag[J]);
+      t.p1 = (int)Q.a_diag[J].pairs.size();
+      Q.a_diag[J].tasks.push_back(t);
       for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) {
         int I = T.pat.rowidx[q];
         if (I == J) continue;
-        TileTask u{T.pat.tile[q], T.linvA + J, (int)a_off[J].pairs.size(), 0};
-        common(I, J, J, a_off[J]);
-        u.p1 = (int)a_off[J].pairs.size();
-        a_off[J].tasks.push_back(u);
+        TileTask u{T.pat.tile[q], T.linvA + J, (int)Q.a_off[J].pairs.size(), 0};
+        common(I, J, J, Q.a_off[J]);
+        u.p1 = (int)Q.a_off[J].pairs.size();
+        Q.a_off[J].tasks.push_back(u);
       }
     }
     for (int J = nto; J < nt; J++)
       for (int I = J; I < nt; I++) {
-        TileTask t{T.pat.find(I, J), -1, (int)a_schur.pairs.size(), 0};
-        common(I, J, nto, a_schur);
-        t.p1 = (int)a_schur.pairs.size();
-        if (t.p1 > t.p0) a_schur.tasks.push_back(t);
+        TileTask t{T.pat.find(I, J), -1, (int)Q.a_schur.pairs.size(), 0};
+        common(I, J, nto, Q.a_schur);
+        t.p1 = (int)Q.a_schur.pairs.size();
+        if (t.p1 > t.p0) Q.a_schur.tasks.push_back(t);
       }
-    const SubTiles *Tp = &D.sub[is];
+    const SubTiles *Tp = &T;
     dense_chol_tasks(
         T.nti, [Tp](int I, int J) { return Tp->P(I, J); }, [Tp](int J) { return Tp->linvP + J; },
-        p_diag, p_off);
+        Q.p_diag, Q.p_off);
     inverse_tasks(
         T.nti, [Tp](int I, int J) { return Tp->P(I, J); },
-        [Tp](int I, int J) { return Tp->Wt(I, J); }, w_step, kraw);
+        [Tp](int I, int J) { return Tp->Wt(I, J); }, Q.w_step, Q.kraw);
   }
+  auto merge_steps = [&](std::vector<TaskList> SubTasks::*member) {
+    size_t nsteps = 0;
+    for (auto &Q : sts) nsteps = std::max(nsteps, (Q.*member).size());
+    std::vector<TaskList> out(nsteps);
+    std::vector<const TaskList *> parts(nsub);
+    for (size_t k = 0; k < nsteps; k++) {
+      for (int is = 0; is < nsub; is++)
+        parts[is] = k < (sts[is].*member).size() ? &(sts[is].*member)[k] : nullptr;
+      out[k] = xcd_merge(parts);
+    }
+    return out;
+  };
+  auto merge_one = [&](TaskList SubTasks::*member) {
+    std::vector<const TaskList *> parts(nsub);
+    for (int is = 0; is < nsub; is++) parts[is] = &(sts[is].*member);
+    return xcd_merge(parts);
+  };
+  std::vector<TaskList> a_diag = merge_steps(&SubTasks::a_diag), a_off = merge_steps(&SubTasks::a_off),
+                        p_diag = merge_steps(&SubTasks::p_diag), p_off = merge_steps(&SubTasks::p_off),
+                        w_step = merge_steps(&SubTasks::w_step);
+  TaskList a_schur = merge_one(&SubTasks::a_schur), kraw = merge_one(&SubTasks::kraw);
+  sts.clear();
+  std::vector<TaskList> c_diag, c_off, cw_step;
+  TaskList cinv;
   {
     const int n = D.ntc, cb = D.cbase, cl = D.clinv, cw = D.cwbase;
     auto Pc = [=](int I, int J) { return cb + lowidx(n, I, J); };
@@ -393,48 +495,75 @@ void device_setup(Solver &S) {
   D.kraw.upload(kraw, st);
   D.cinv.upload(cinv, st);
 
-  // ---- scatter entries: A tiles (both triangles inside diagonal tiles),
-  // padding diagonals, coarse padding diagonals
-  std::vector<int> zero_ids;
-  std::vector<int64_t> sidx;
-  std::vector<double> sval;
-  for (int is = 0; is < nsub; is++) {
-    const Sub &s = S.subs[is];
-    const SubTiles &T = D.sub[is];
-    zero_ids.insert(zero_ids.end(), T.pat.tile.begin(), T.pat.tile.end());
-    auto put = [&](int a, int b, double v) {  // a >= b tile-wise lower
-      int I = a / TILE, J = b / TILE;
-      int tid = T.pat.find(I, J);
-      sidx.push_back((int64_t)tid * TILE_ELEMS + (b % TILE) * TILE + (a % TILE));
-      sval.push_back(v);
-    };
-    for (size_t k = 0; k < s.av.size(); k++) {
-      int a = s.perm[s.ai[k]], b = s.perm[s.aj[k]];
-      if (a < b) std::swap(a, b);
-      put(a, b, s.av[k]);
-      if (a != b && a / TILE == b / TILE) put(b, a, s.av[k]);
-    }
-    for (int p = s.no; p < s.NO; p++) put(p, p, 1.0);
-    for (int p = s.NO + s.ni; p < s.NO + s.NI; p++) put(p, p, 1.0);
-  }
+  ph("task lists");
+  // ---- matrix entries: the assembled triplets are already on the device
+  // (uploaded with each subdomain); here only the device permutation, the
+  // Dirichlet flags and the tile tables go up.  Padding diagonals (1.0) of
+  // subdomain and coarse matrices come from a short host list.
+  if (!S.din)
+    for (int is = 0; is < nsub; is++) device_upload_entries(S, is);
   {
-    const int n = D.ntc;
-    for (int J = 0; J < n; J++)
-      for (int I = J; I < n; I++) zero_ids.push_back(D.cbase + lowidx(n, I, J));
-    for (int p = S.ndofc; p < D.NC; p++) {
-      int T_ = D.cbase + lowidx(n, p / TILE, p / TILE);
-      sidx.push_back((int64_t)T_ * TILE_ELEMS + (p % TILE) * TILE + (p % TILE));
-      sval.push_back(1.0);
+    std::vector<int> perm, ttab;
+    std::vector<char> fix;
+    std::vector<ScatterSub> ss(nsub);
+    std::vector<int2> ranges;
+    std::vector<int64_t> sidx;
+    std::vector<double> sval;
+    for (int is = 0; is < nsub; is++) {
+      const Sub &s = S.subs[is];
+      const SubTiles &T = D.sub[is];
+      const int nt = T.nto + T.nti;
+      ss[is] = ScatterSub{S.din->ai[is].p, S.din->aj[is].p, S.din->av[is].p,
+                          (int)S.din->ai[is].n, (int)perm.size(), (int)ttab.size()};
+      perm.insert(perm.end(), s.perm.begin(), s.perm.end());
+      for (int d = 0; d < s.ndofs; d++) fix.push_back(s.ifix[d] != 0);
+      size_t t0_ = ttab.size();
+      ttab.resize(t0_ + (size_t)nt * (nt + 1) / 2, -1);
+      for (int J = 0; J < nt; J++)
+        for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) {
+          int I = T.pat.rowidx[q];
+          ttab[t0_ + (size_t)I * (I + 1) / 2 + J] = T.pat.tile[q];
+        }
+      ranges.push_back(make_int2(T.pat.tile.front(), (int)T.pat.tile.size()));
+      auto put_diag = [&](int p) {
+        int tid = T.pat.find(p / TILE, p / TILE);
+        sidx.push_back((int64_t)tid * TILE_ELEMS + (p % TILE) * TILE + (p % TILE));
+        sval.push_back(1.0);
+      };
+      for (int p = s.no; p < s.NO; p++) put_diag(p);
+      for (int p = s.NO + s.ni; p < s.NO + s.NI; p++) put_diag(p);
     }
+    {
+      const int n = D.ntc;
+      ranges.push_back(make_int2(D.cbase, n * (n + 1) / 2));
+      for (int p = S.ndofc; p < D.NC; p++) {
+        int T_ = D.cbase + lowidx(n, p / TILE, p / TILE);
+        sidx.push_back((int64_t)T_ * TILE_ELEMS + (p % TILE) * TILE + (p % TILE));
+        sval.push_back(1.0);
+      }
+    }
+    for (auto &r : ranges)
+      if (r.x < 0 || (int64_t)r.x + r.y > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid tile range to clear");
+    for (int64_t k : sidx)
+      if (k < 0 || k >= D.ntiles * (int64_t)TILE_ELEMS) fatal("BDDCML_DEVICE_PLAN", "scatter index out of range");
+    for (int v : ttab)
+      if (v >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid tile table entry");
+    for (int is = 0; is < nsub; is++) {
+      const Sub &s = S.subs[is];
+      if ((int64_t)S.din->ai[is].n != (int64_t)s.ai.size()) fatal("BDDCML_DEVICE_PLAN", "device triplets out of date");
+      for (int d = 0; d < s.ndofs; d++)
+        if (s.perm[d] < 0 || s.perm[d] >= s.NO + s.NI) fatal("BDDCML_DEVICE_PLAN", "invalid permutation");
+    }
+    D.sc_perm.upload(perm, st);
+    D.sc_fix.upload(fix, st);
+    D.sc_ttab.upload(ttab, st);
+    D.sc_subs.upload(ss, st);
+    D.zero_rng.upload(ranges, st);
+    D.sc_idx.upload(sidx, st);
+    D.sc_val.upload(sval, st);
   }
-  for (int id : zero_ids)
-    if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid tile to clear");
-  for (int64_t k : sidx)
-    if (k < 0 || k >= D.ntiles * (int64_t)TILE_ELEMS) fatal("BDDCML_DEVICE_PLAN", "scatter index out of range");
-  D.zero_ids.upload(zero_ids, st);
-  D.sc_idx.upload(sidx, st);
-  D.sc_val.upload(sval, st);
 
+  ph("scatter entries");
   // ---- mirror jobs, S' jobs, K update jobs
   std::vector<MirrorJob> ms, mk, mc;
   std::vector<int> spsub, spij, spdst, kusub, kuij;
@@ -481,6 +610,7 @@ void device_setup(Solver &S) {
   D.ku_ij.upload(kuij, st);
   D.n_ku = (int)kusub.size();
 
+  ph("mirror and update jobs");
   // ---- interface descriptors and maps
   D.hdesc.resize(nsub);
   std::vector<int64_t> loff(nsub);
@@ -584,6 +714,7 @@ void device_setup(Solver &S) {
   D.ncrows = (int)cr.size();
   D.cloff.upload(std::vector<int64_t>{0}, st);
   D.cNI.upload(std::vector<int>{D.NC}, st);
+  ph("interface maps");
   // assembly CSR: unique interface dof -> local flat indices (subdomain order)
   {
     std::vector<int> cnt(S.nig + 1, 0);
@@ -661,6 +792,7 @@ void device_setup(Solver &S) {
     D.tck.upload(ckv, st);
     D.tct.upload(ctv, st);
   }
+  ph("assembly and sweep lists");
   // coarse gather CSR: coarse dof -> constraint rows (subdomain order)
   {
     std::vector<int> cnt(S.ndofc + 1, 0);
@@ -725,8 +857,14 @@ void device_setup(Solver &S) {
   D.err.zero(st);
   D.rc.zero(st);
   D.uc.zero(st);
+  ph("coarse lists, vectors");
   D.pool.alloc((size_t)D.ntiles * TILE_ELEMS);
+  ph("pool allocation");
   HIPCHECK(hipStreamSynchronize(st));
+  ph("upload sync");
+  if (S.verbose >= 2)
+    std::printf("   device plan: %lld tiles (%.1f GB), %zu scatter entries\n", (long long)D.ntiles,
+                D.ntiles * 32768.0 / 1e9, (size_t)D.sc_idx.n);
   S.t.plan += wall_time() - t0;
 
   // algorithmic work: 2*64^3 flops per tile product, 64^3/3 per tile Cholesky
@@ -748,7 +886,8 @@ void device_setup(Solver &S) {
   hipEvent_t ev[6];
   for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
   HIPCHECK(hipEventRecord(ev[0], st));
-  launch_zero_tiles(st, pool, D.zero_ids.p, (int)D.zero_ids.n);
+  launch_zero_ranges(st, pool, D.zero_rng.p, (int)D.zero_rng.n);
+  launch_scatter_sub(st, pool, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p, D.err.p);
   launch_scatter(st, pool, D.sc_idx.p, D.sc_val.p, (int64_t)D.sc_idx.n);
   for (size_t J = 0; J < D.a_diag.size(); J++) {
     launch_tile_potrf(st, pool, D.a_diag[J].tasks.p, D.a_diag[J].pairs.p, D.a_diag[J].ntask, D.err.p);
@@ -812,6 +951,7 @@ void device_setup(Solver &S) {
   HIPCHECK(hipMemcpy(&herr, D.err.p, sizeof(int), hipMemcpyDeviceToHost));
   if (herr & 1) fatal("BDDCML_SETUP_PRECONDITIONER", "matrix not positive definite in a tile Cholesky factorization");
   if (herr & 2) fatal("BDDCML_SETUP_PRECONDITIONER", "singular constraint matrix C S'^{-1} C^T");
+  if (herr & 4) fatal("BDDCML_SETUP_PRECONDITIONER", "matrix entry outside the symbolic tile pattern");
   S.t.factor += wall_time() - t0;
 }
 

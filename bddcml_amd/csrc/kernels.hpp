@@ -29,6 +29,18 @@ struct MirrorJob {
 };
 void launch_tile_mirror(hipStream_t st, double *pool, const MirrorJob *jobs, int njobs);
 void launch_zero_tiles(hipStream_t st, double *pool, const int *ids, int n);
+// clear ranges of consecutive tiles: ranges[k] = (first tile, count)
+void launch_zero_ranges(hipStream_t st, double *pool, const int2 *ranges, int nranges);
+// assembled triplets of each subdomain -> tile pool (see kernels_tile.hip)
+struct ScatterSub {
+  const int *ai, *aj;
+  const double *av;
+  int nnz;
+  int poff;  // offset of the subdomain in perm / fix (dofs)
+  int toff;  // offset of its packed lower tile table
+};
+void launch_scatter_sub(hipStream_t st, double *pool, const ScatterSub *subs, int nsub,
+                        const int *perm, const char *fix, const int *ttab, int *err);
 void launch_scatter(hipStream_t st, double *pool, const int64_t *idx, const double *val,
                     int64_t n);
 

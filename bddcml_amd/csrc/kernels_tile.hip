@@ -215,20 +215,23 @@ __global__ __launch_bounds__(NTHR) void k_tile_potrf(double *__restrict__ pool,
     if (r < c) As[c * LDL + r] = 0.0;
   }
   __syncthreads();
-  // inverse of the lower factor, one column per thread
-  if (tid < 64) {
-    const int c = tid;
-    for (int i = 0; i < 64; i++) {
-      double x = 0.0;
-      if (i >= c) {
-        double s = (i == c) ? 1.0 : 0.0;
-        for (int k = c; k < i; k++) s -= As[k * LDL + i] * Bs[c * LDL + k];
-        x = s / As[i * LDL + i];
-      }
-      Bs[c * LDL + i] = x;
-    }
+  // inverse of the lower factor by a row sweep over [L | I]: after step k
+  // row k of X = L^{-1} is final and eliminated from the rows below
+  for (int e = tid; e < 4096; e += NTHR) {
+    int r = e & 63, c = e >> 6;
+    Bs[c * LDL + r] = (r == c) ? 1.0 : 0.0;
   }
   __syncthreads();
+  for (int k = 0; k < 64; k++) {
+    const double inv = 1.0 / As[k * LDL + k];
+    if (tid <= k) Bs[tid * LDL + k] *= inv;
+    __syncthreads();
+    for (int e = tid; e < 4096; e += NTHR) {
+      int i = e & 63, c = e >> 6;
+      if (i > k && c <= k) Bs[c * LDL + i] -= As[k * LDL + i] * Bs[c * LDL + k];
+    }
+    __syncthreads();
+  }
   lds_to_tile(As, C);
   lds_to_tile(Bs, pool + (size_t)t.x * TILE_ELEMS);
   if (tid == 0 && bad) atomicOr(err, 1);
@@ -252,6 +255,58 @@ __global__ void k_scatter(double *__restrict__ pool, const int64_t *__restrict__
                           const double *__restrict__ val, int64_t n) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) pool[idx[k]] = val[k];
+}
+
+// Assembled subdomain triplets (upper triangle, local dof indices) into the
+// tile pool: Dirichlet elimination keeping the diagonal
+// (src/module_sm.f90:431-600), device permutation, both triangles inside
+// diagonal tiles.
+__global__ void k_scatter_sub(double *__restrict__ pool, const ScatterSub *__restrict__ subs,
+                              const int *__restrict__ perm, const char *__restrict__ fix,
+                              const int *__restrict__ ttab, int *__restrict__ err) {
+  const ScatterSub d = subs[blockIdx.y];
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < d.nnz; e += gridDim.x * blockDim.x) {
+    int i = d.ai[e], j = d.aj[e];
+    double v = d.av[e];
+    const bool fi = fix[d.poff + i] != 0, fj = fix[d.poff + j] != 0;
+    if (i != j) {
+      if (fi || fj) v = 0.0;
+    } else if (fi && fabs(v) < 1e-12) {
+      v = 1.0;
+    }
+    int a = perm[d.poff + i], b = perm[d.poff + j];
+    if (a < b) {
+      int t = a;
+      a = b;
+      b = t;
+    }
+    const int I = a >> 6, J = b >> 6;
+    const int t = ttab[d.toff + I * (I + 1) / 2 + J];
+    if (t < 0) {
+      atomicOr(err, 4);
+      continue;
+    }
+    pool[(size_t)t * TILE_ELEMS + (b & 63) * 64 + (a & 63)] = v;
+    if (I == J && a != b) pool[(size_t)t * TILE_ELEMS + (a & 63) * 64 + (b & 63)] = v;
+  }
+}
+
+__global__ void k_zero_ranges(double *__restrict__ pool, const int2 *__restrict__ ranges) {
+  const int2 rg = ranges[blockIdx.y];  // first tile, count
+  const size_t n2 = (size_t)rg.y * (TILE_ELEMS / 2);
+  double2 *T = reinterpret_cast<double2 *>(pool + (size_t)rg.x * TILE_ELEMS);
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (size_t)gridDim.x * blockDim.x)
+    T[e] = make_double2(0.0, 0.0);
+}
+
+void launch_scatter_sub(hipStream_t st, double *pool, const ScatterSub *subs, int nsub,
+                        const int *perm, const char *fix, const int *ttab, int *err) {
+  if (nsub <= 0) return;
+  hipLaunchKernelGGL(k_scatter_sub, dim3(64, nsub), dim3(256), 0, st, pool, subs, perm, fix, ttab, err);
+}
+void launch_zero_ranges(hipStream_t st, double *pool, const int2 *ranges, int nranges) {
+  if (nranges <= 0) return;
+  hipLaunchKernelGGL(k_zero_ranges, dim3(64, nranges), dim3(256), 0, st, pool, ranges);
 }
 
 void launch_tile_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,

@@ -12,7 +12,8 @@
 
 namespace bddc {
 
-struct Device;  // device-side plan and buffers (device.hip)
+struct Device;       // device-side plan and buffers (device.hip)
+struct DeviceInput;  // assembled subdomain triplets in HBM (device.hip)
 
 struct Timings {
   double topology = 0, plan = 0, upload = 0, factor = 0, schur_block = 0, interface = 0,
@@ -50,6 +51,7 @@ struct Solver {
   bool is_setup = false, solved = false, plan_only = false;
   std::vector<double> uig;  // interface solution
   std::unique_ptr<Device> dev;
+  std::unique_ptr<DeviceInput> din;
   Timings t;
   int last_iters = 0, last_reason = 0;
   double last_cond = 0;
@@ -72,6 +74,8 @@ void build_unique_interface(Solver &S);
 void build_orderings(Solver &S);
 
 // device steps (device.hip)
+void device_upload_entries(Solver &S, int is);
+void device_release_input(Solver &S);
 void device_setup(Solver &S);
 void device_reduced_rhs(Solver &S);
 void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, int *num_iter,

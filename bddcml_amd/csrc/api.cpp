@@ -521,7 +521,7 @@ int bddcml_amd_sub_matrix(int isub, int which, double *out) try {
   Solver &S = solver();
   if (!device_ready(S)) return -1;
   int is = isub - S.numbase;
-  if (is < 0 || is >= S.nsub || (which != 'S' && which != 'K')) return -2;
+  if (is < 0 || is >= S.nsub || which != 'S') return -2;
   device_sub_matrix(S, is, which, out);
   return 0;
 }

@@ -97,9 +97,9 @@ struct SubTiles {
   int nto = 0, nti = 0;
   TilePattern pat;   // A pattern (lower), tile ids absolute
   std::vector<std::vector<uint64_t>> col;  // column bitsets
-  int linvA = 0, sfull = 0, pbase = 0, linvP = 0, wbase = 0, kfull = 0;
+  int linvA = 0, sfull = 0, pbase = 0, linvP = 0, ybase = 0, ncT = 0;
   int P(int I, int J) const { return pbase + lowidx(nti, I, J); }
-  int Wt(int I, int J) const { return I == J ? linvP + I : wbase + lowidx(nti - 1, I - 1, J); }
+  int Y(int I, int cT) const { return ybase + I * ncT + cT; }
 };
 
 int TilePattern::find(int I, int J) const {
@@ -116,27 +116,27 @@ struct Device {
   DBuf<int> err;
   std::vector<SubTiles> sub;
   // setup phases
-  std::vector<Phase> a_diag, a_off, p_diag, p_off, w_step, c_diag, c_off, cw_step;
-  Phase a_schur, kraw, cinv;
+  std::vector<Phase> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd, c_diag, c_off, cw_step;
+  Phase a_schur, cinv;
   DBuf<int> zero_ids, sc_perm, sc_ttab;
   DBuf<char> sc_fix;
   DBuf<ScatterSub> sc_subs;
   DBuf<int2> zero_rng;
   DBuf<int64_t> sc_idx;
   DBuf<double> sc_val;
-  DBuf<MirrorJob> mir_s, mir_k, mir_c;
-  DBuf<int> sp_sub, sp_ij, sp_dst, ku_sub, ku_ij;
-  int n_sp = 0, n_ku = 0;
+  DBuf<MirrorJob> mir_s, mir_c;
+  DBuf<int> sp_sub, sp_ij, sp_dst;
+  int n_sp = 0;
   // interface data
   std::vector<SubDesc> hdesc;
   DBuf<SubDesc> desc;
   DBuf<int64_t> loff;
   DBuf<int> NIs, map, cid, csize, cptr, cdof, cmap;
   DBuf<double> w, Z, Phi, G, Acs;
-  int maxNI = 0, maxnc = 0;
+  int maxNI = 0, maxnc = 0, maxnti = 0, maxnt = 1;
   int64_t LV = 0;
-  DBuf<GemvRow> srows, krows, crows;
-  int nsrows = 0, nkrows = 0, ncrows = 0;
+  DBuf<GemvRow> srows, crows;
+  int nsrows = 0, ncrows = 0;
   DBuf<int> aptr;
   DBuf<int64_t> asrc;
   DBuf<int> cgptr, cgsrc;
@@ -227,7 +227,7 @@ static void symbolic(const Sub &s, SubTiles &T) {
   }
 }
 
-static void assign_tiles(SubTiles &T, int &next) {
+static void assign_tiles(SubTiles &T, int nc, int &next) {
   const int nt = T.nto + T.nti;
   T.pat.nt = nt;
   T.pat.ntf = T.nto;
@@ -250,10 +250,9 @@ static void assign_tiles(SubTiles &T, int &next) {
   next += T.nti * (T.nti + 1) / 2;
   T.linvP = next;
   next += T.nti;
-  T.wbase = next;
-  next += T.nti * (T.nti - 1) / 2;
-  T.kfull = next;
-  next += T.nti * T.nti;
+  T.ncT = (nc + TILE - 1) / TILE;
+  T.ybase = next;
+  next += T.nti * T.ncT;
 }
 
 // tasks of a dense lower Cholesky on tiles P(I,J) with inverse tiles Linv(J)
@@ -275,7 +274,8 @@ static void dense_chol_tasks(int n, FP Ptile, FL Linv, std::vector<TaskList> &di
   }
 }
 
-// W = L^{-1} (strict lower tiles, diagonal = Linv) and Kraw = W^T W
+// W = L^{-1} (strict lower tiles, diagonal = Linv) and Kraw = W^T W for the
+// explicit inverse of a dense factor
 template <class FP, class FW>
 static void inverse_tasks(int n, FP Ptile, FW Wt, std::vector<TaskList> &wstep, TaskList &kraw) {
   if ((int)wstep.size() < n) wstep.resize(n);
@@ -365,7 +365,7 @@ void device_setup(Solver &S) {
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < nsub; is++) symbolic(S.subs[is], D.sub[is]);
   int next = 0;
-  for (int is = 0; is < nsub; is++) assign_tiles(D.sub[is], next);
+  for (int is = 0; is < nsub; is++) assign_tiles(D.sub[is], S.subs[is].nconstr, next);
   D.NC = std::max(TILE, (S.ndofc + TILE - 1) / TILE * TILE);
   D.ntc = D.NC / TILE;
   D.cbase = next;
@@ -384,8 +384,8 @@ void device_setup(Solver &S) {
   // (workgroups are dealt to the 8 XCDs round robin), keeping its tiles in
   // that XCD's L2
   struct SubTasks {
-    std::vector<TaskList> a_diag, a_off, p_diag, p_off, w_step;
-    TaskList a_schur, kraw;
+    std::vector<TaskList> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd;
+    TaskList a_schur;
   };
   std::vector<SubTasks> sts(nsub);
 #pragma omp parallel for schedule(dynamic)
@@ -437,9 +437,25 @@ void device_setup(Solver &S) {
     dense_chol_tasks(
         T.nti, [Tp](int I, int J) { return Tp->P(I, J); }, [Tp](int J) { return Tp->linvP + J; },
         Q.p_diag, Q.p_off);
-    inverse_tasks(
-        T.nti, [Tp](int I, int J) { return Tp->P(I, J); },
-        [Tp](int I, int J) { return Tp->Wt(I, J); }, Q.w_step, Q.kraw);
+    // Z = S'^{-1} C^T by tile sweeps over the factor: forward steps
+    // Y_I = Linv_I (Y_I - sum_{k<I} L_Ik Y_k), backward steps
+    // Y_I = Linv_I^T (Y_I - sum_{k>I} L_kI^T Y_k), one task per tile column
+    const int n = T.nti;
+    Q.z_fwd.resize(T.ncT ? n : 0);
+    Q.z_bwd.resize(T.ncT ? n : 0);
+    for (int cT = 0; cT < T.ncT; cT++)
+      for (int I = 0; I < n; I++) {
+        TaskList &F = Q.z_fwd[I];
+        TileTask t{T.Y(I, cT), T.linvP + I, (int)F.pairs.size(), 0};
+        for (int k = 0; k < I; k++) F.pairs.push_back({T.P(I, k), T.Y(k, cT)});
+        t.p1 = (int)F.pairs.size();
+        F.tasks.push_back(t);
+        TaskList &B = Q.z_bwd[n - 1 - I];
+        TileTask u{T.Y(I, cT), T.linvP + I, (int)B.pairs.size(), 0};
+        for (int k = I + 1; k < n; k++) B.pairs.push_back({T.P(k, I), T.Y(k, cT)});
+        u.p1 = (int)B.pairs.size();
+        B.tasks.push_back(u);
+      }
   }
   auto merge_steps = [&](std::vector<TaskList> SubTasks::*member) {
     size_t nsteps = 0;
@@ -460,8 +476,8 @@ void device_setup(Solver &S) {
   };
   std::vector<TaskList> a_diag = merge_steps(&SubTasks::a_diag), a_off = merge_steps(&SubTasks::a_off),
                         p_diag = merge_steps(&SubTasks::p_diag), p_off = merge_steps(&SubTasks::p_off),
-                        w_step = merge_steps(&SubTasks::w_step);
-  TaskList a_schur = merge_one(&SubTasks::a_schur), kraw = merge_one(&SubTasks::kraw);
+                        z_fwd = merge_steps(&SubTasks::z_fwd), z_bwd = merge_steps(&SubTasks::z_bwd);
+  TaskList a_schur = merge_one(&SubTasks::a_schur);
   sts.clear();
   std::vector<TaskList> c_diag, c_off, cw_step;
   TaskList cinv;
@@ -476,23 +492,23 @@ void device_setup(Solver &S) {
   for (auto &l : a_off) check_tasks(l, D.ntiles, true, "interior panel");
   for (auto &l : p_diag) check_tasks(l, D.ntiles, true, "interface Cholesky");
   for (auto &l : p_off) check_tasks(l, D.ntiles, true, "interface panel");
-  for (auto &l : w_step) check_tasks(l, D.ntiles, true, "interface inverse");
+  for (auto &l : z_fwd) check_tasks(l, D.ntiles, true, "constraint solve (forward)");
+  for (auto &l : z_bwd) check_tasks(l, D.ntiles, true, "constraint solve (backward)");
   for (auto &l : c_diag) check_tasks(l, D.ntiles, true, "coarse Cholesky");
   for (auto &l : c_off) check_tasks(l, D.ntiles, true, "coarse panel");
   for (auto &l : cw_step) check_tasks(l, D.ntiles, true, "coarse inverse");
   check_tasks(a_schur, D.ntiles, false, "Schur block");
-  check_tasks(kraw, D.ntiles, false, "interface W^T W");
   check_tasks(cinv, D.ntiles, false, "coarse W^T W");
   upload_phases(D.a_diag, a_diag, st);
   upload_phases(D.a_off, a_off, st);
   upload_phases(D.p_diag, p_diag, st);
   upload_phases(D.p_off, p_off, st);
-  upload_phases(D.w_step, w_step, st);
+  upload_phases(D.z_fwd, z_fwd, st);
+  upload_phases(D.z_bwd, z_bwd, st);
   upload_phases(D.c_diag, c_diag, st);
   upload_phases(D.c_off, c_off, st);
   upload_phases(D.cw_step, cw_step, st);
   D.a_schur.upload(a_schur, st);
-  D.kraw.upload(kraw, st);
   D.cinv.upload(cinv, st);
 
   ph("task lists");
@@ -564,9 +580,9 @@ void device_setup(Solver &S) {
   }
 
   ph("scatter entries");
-  // ---- mirror jobs, S' jobs, K update jobs
-  std::vector<MirrorJob> ms, mk, mc;
-  std::vector<int> spsub, spij, spdst, kusub, kuij;
+  // ---- mirror jobs, S' jobs
+  std::vector<MirrorJob> ms, mc;
+  std::vector<int> spsub, spij, spdst;
   for (int is = 0; is < nsub; is++) {
     const SubTiles &T = D.sub[is];
     const int nto = T.nto, n = T.nti;
@@ -575,14 +591,10 @@ void device_setup(Solver &S) {
         int src = T.pat.find(nto + I, nto + J);
         ms.push_back({src, T.sfull + I * n + J, 0});
         if (I > J) ms.push_back({src, T.sfull + J * n + I, 1});
-        mk.push_back({T.P(I, J), T.kfull + I * n + J, 0});
-        if (I > J) mk.push_back({T.P(I, J), T.kfull + J * n + I, 1});
         spsub.push_back(is);
         spij.push_back((I << 16) | J);
         spdst.push_back(T.P(I, J));
       }
-    for (int I = 0; I < n; I++)
-      for (int J = 0; J < n; J++) kusub.push_back(is), kuij.push_back((I << 16) | J);
   }
   {
     const int n = D.ntc;
@@ -593,22 +605,18 @@ void device_setup(Solver &S) {
         if (I > J) mc.push_back({src, D.cfull + J * n + I, 1});
       }
   }
-  for (auto *jobs : {&ms, &mk, &mc})
+  for (auto *jobs : {&ms, &mc})
     for (const MirrorJob &j : *jobs)
       if (j.src < 0 || j.src >= D.ntiles || j.dst < 0 || j.dst >= D.ntiles)
         fatal("BDDCML_DEVICE_PLAN", "invalid mirror job");
   for (int d : spdst)
     if (d < 0 || d >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid augmented tile");
   D.mir_s.upload(ms, st);
-  D.mir_k.upload(mk, st);
   D.mir_c.upload(mc, st);
   D.sp_sub.upload(spsub, st);
   D.sp_ij.upload(spij, st);
   D.sp_dst.upload(spdst, st);
   D.n_sp = (int)spsub.size();
-  D.ku_sub.upload(kusub, st);
-  D.ku_ij.upload(kuij, st);
-  D.n_ku = (int)kusub.size();
 
   ph("mirror and update jobs");
   // ---- interface descriptors and maps
@@ -628,7 +636,10 @@ void device_setup(Solver &S) {
     d.nti = s.nti;
     d.nc = s.nconstr;
     d.sfull = T.sfull;
-    d.kfull = T.kfull;
+    d.pbase = T.pbase;
+    d.plinv = T.linvP;
+    d.ybase = T.ybase;
+    d.ncT = T.ncT;
     d.loff = lv;
     d.zoff = zo;
     d.goff = go;
@@ -674,8 +685,11 @@ void device_setup(Solver &S) {
     co += s.nconstr;
     D.maxNI = std::max(D.maxNI, s.NI);
     D.maxnc = std::max(D.maxnc, s.nconstr);
+    D.maxnti = std::max(D.maxnti, s.nti);
   }
   if (D.maxNI > 7936) fatal("BDDCML_SETUP_PRECONDITIONER", "interface larger than 7936 dofs per subdomain");
+  if ((size_t)(D.maxNI + 1024 + 64 + D.maxnc) * sizeof(double) > max_dynamic_lds())
+    fatal("BDDCML_SETUP_PRECONDITIONER", "subdomain interface too large for the preconditioner apply");
   if (D.maxnc > 1024) fatal("BDDCML_SETUP_PRECONDITIONER", "more than 1024 constraints per subdomain");
   if (D.NC > 7936) fatal("BDDCML_SETUP_PRECONDITIONER", "coarse problem larger than 7936 dofs");
   D.LV = lv;
@@ -694,23 +708,18 @@ void device_setup(Solver &S) {
   D.G.alloc(std::max<int64_t>(go, 1));
   D.Acs.alloc(std::max<int64_t>(go, 1));
   // gemv rows
-  std::vector<GemvRow> sr, kr, cr;
+  std::vector<GemvRow> sr, cr;
   for (int is = 0; is < nsub; is++) {
     const SubTiles &T = D.sub[is];
-    for (int I = 0; I < T.nti; I++) {
-      sr.push_back({T.sfull + I * T.nti, T.nti, is, I});
-      kr.push_back({T.kfull + I * T.nti, T.nti, is, I});
-    }
+    for (int I = 0; I < T.nti; I++) sr.push_back({T.sfull + I * T.nti, T.nti, is, I});
   }
   for (int I = 0; I < D.ntc; I++) cr.push_back({D.cfull + I * D.ntc, D.ntc, 0, I});
-  for (auto *rows : {&sr, &kr, &cr})
+  for (auto *rows : {&sr, &cr})
     for (const GemvRow &g : *rows)
       if (g.tile0 < 0 || (int64_t)g.tile0 + g.nt > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid GEMV row");
   D.srows.upload(sr, st);
-  D.krows.upload(kr, st);
   D.crows.upload(cr, st);
   D.nsrows = (int)sr.size();
-  D.nkrows = (int)kr.size();
   D.ncrows = (int)cr.size();
   D.cloff.upload(std::vector<int64_t>{0}, st);
   D.cNI.upload(std::vector<int>{D.NC}, st);
@@ -777,13 +786,13 @@ void device_setup(Solver &S) {
         cp.push_back((int)ckv.size());
       }
     }
-    if ((int64_t)(maxnt * 64 + 320) * 8 > 65536)
+    if ((size_t)(maxnt * 64 + 320) * sizeof(double) > max_dynamic_lds())
       fatal("BDDCML_SETUP_PRECONDITIONER", "subdomain larger than the triangular sweep supports");
     for (int id : rtv)
       if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid sweep tile");
     for (int id : ctv)
       if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid sweep tile");
-    trsv_set_maxnt(maxnt);
+    D.maxnt = maxnt;
     D.trsv.upload(D.htrsv, st);
     D.rptr.upload(rp, st);
     D.rk.upload(rkv, st);
@@ -878,7 +887,7 @@ void device_setup(Solver &S) {
     };
     S.t.schur_flops = g * a_schur.pairs.size();
     S.t.interior_flops = work(a_diag, 2 * c) + work(a_off, g) + S.t.schur_flops;
-    S.t.interface_flops = work(p_diag, 2 * c) + work(p_off, g) + work(w_step, g) + g * kraw.pairs.size();
+    S.t.interface_flops = work(p_diag, 2 * c) + work(p_off, g) + work(z_fwd, g) + work(z_bwd, g);
   }
   // ---- numerics
   t0 = wall_time();
@@ -906,17 +915,18 @@ void device_setup(Solver &S) {
     launch_tile_gemm(st, pool, D.p_off[J].tasks.p, D.p_off[J].pairs.p, D.p_off[J].ntask, GEMM_NT,
                      POST_RIGHT_XT, 1.0, -1.0);
   }
-  for (size_t i = 1; i < D.w_step.size(); i++)
-    launch_tile_gemm(st, pool, D.w_step[i].tasks.p, D.w_step[i].pairs.p, D.w_step[i].ntask, GEMM_NN,
-                     POST_LEFT_X, 0.0, -1.0);
-  launch_tile_gemm(st, pool, D.kraw.tasks.p, D.kraw.pairs.p, D.kraw.ntask, GEMM_TN, POST_NONE, 0.0,
-                   1.0);
-  launch_tile_mirror(st, pool, D.mir_k.p, (int)D.mir_k.n);
-  launch_z(st, pool, D.desc.p, nsub, D.cptr.p, D.cdof.p, D.Z.p, D.maxnc);
+  // Z = S'^{-1} C^T through the factor, G = C Z, Phi = Z G^{-1}
+  launch_ct_init(st, pool, D.desc.p, nsub, D.cid.p, D.maxnti);
+  for (size_t i = 0; i < D.z_fwd.size(); i++)
+    launch_tile_gemm(st, pool, D.z_fwd[i].tasks.p, D.z_fwd[i].pairs.p, D.z_fwd[i].ntask, GEMM_NN,
+                     POST_LEFT_X, 1.0, -1.0);
+  for (size_t i = 0; i < D.z_bwd.size(); i++)
+    launch_tile_gemm(st, pool, D.z_bwd[i].tasks.p, D.z_bwd[i].pairs.p, D.z_bwd[i].ntask, GEMM_TN,
+                     POST_LEFT_XT, 1.0, -1.0);
+  launch_y_to_z(st, pool, D.desc.p, nsub, D.Z.p, D.maxNI);
   launch_g(st, D.desc.p, nsub, D.cptr.p, D.cdof.p, D.Z.p, D.G.p);
   launch_ginv(st, D.desc.p, nsub, D.G.p, nullptr, D.err.p);
   launch_phi(st, D.desc.p, nsub, D.Z.p, D.G.p, D.Phi.p);
-  launch_kupdate(st, pool, D.desc.p, D.ku_sub.p, D.ku_ij.p, D.n_ku, D.Phi.p, D.Z.p);
   launch_acs(st, D.desc.p, nsub, D.cptr.p, D.G.p, D.Acs.p);
   HIPCHECK(hipEventRecord(ev[3], st));
   // coarse problem
@@ -966,7 +976,7 @@ void device_reduced_rhs(Solver &S) {
     for (int d = 0; d < s.ndofs; d++) hv[vo + s.perm[d]] = s.fmod[d];
   }
   HIPCHECK(hipMemcpyAsync(D.vec.p, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, st));
-  launch_trsv_fwd(st, D.pool.p, D.trsv.p, S.nsub, D.rptr.p, D.rk.p, D.rt.p, D.vec.p);
+  launch_trsv_fwd(st, D.pool.p, D.trsv.p, S.nsub, D.rptr.p, D.rk.p, D.rt.p, D.vec.p, D.maxnt);
   launch_assemble(st, S.nig, D.aptr.p, D.rhs_src.p, nullptr, D.vec.p, D.g.p);
   HIPCHECK(hipStreamSynchronize(st));
 }
@@ -1008,7 +1018,7 @@ void device_selftest_tile(int op, int npairs, const double *A, const double *B, 
     launch_tile_potrf(nullptr, pool.p, t.p, pr.p, 1, err.p);
   } else {
     int kind = op % 10, post = op / 10;
-    if (kind < 0 || kind > 2 || post < 0 || post > 2) fatal("BDDCML_SELFTEST", "bad op");
+    if (kind < 0 || kind > 2 || post < 0 || post > 3) fatal("BDDCML_SELFTEST", "bad op");
     launch_tile_gemm(nullptr, pool.p, t.p, pr.p, 1, kind, post, alpha, beta);
   }
   HIPCHECK(hipGetLastError());
@@ -1023,7 +1033,8 @@ void device_sub_matrix(Solver &S, int is, int which, double *out) {
   Device &D = *S.dev;
   const Sub &s = S.subs[is];
   const SubTiles &T = D.sub[is];
-  const int n = T.nti, first = which == 'S' ? T.sfull : T.kfull;
+  if (which != 'S') fatal("BDDCML_AMD_SUB_MATRIX", "only the Schur complement is stored explicitly");
+  const int n = T.nti, first = T.sfull;
   std::vector<double> tiles((size_t)n * n * TILE_ELEMS);
   HIPCHECK(hipMemcpy(tiles.data(), D.pool.p + (size_t)first * TILE_ELEMS,
                      tiles.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1080,13 +1091,14 @@ static void op_schur(Solver &S, const double *x, double *y, bool timed = false) 
 static void op_pc(Solver &S, const double *r, double *z) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
-  launch_gemv(st, D.pool.p, D.krows.p, D.nkrows, D.loff.p, D.NIs.p, D.map.p, D.w.p, r, D.zl.p,
-              D.maxNI);
+  // coarse correction uc = Ac^{-1} sum_i R_ci^T Phi_i^T W_i r_i
   launch_phit(st, D.desc.p, S.nsub, D.Phi.p, D.map.p, D.w.p, r, D.rcs.p, D.maxnc);
   launch_coarse_gather(st, S.ndofc, D.cgptr.p, D.cgsrc.p, D.rcs.p, D.rc.p);
   launch_gemv(st, D.pool.p, D.crows.p, D.ncrows, D.cloff.p, D.cNI.p, nullptr, nullptr, D.rc.p,
               D.uc.p, D.NC);
-  launch_phi_add(st, D.desc.p, S.nsub, D.Phi.p, D.cmap.p, D.uc.p, D.zl.p, D.maxNI);
+  // zl_i = K_i W_i r_i + Phi_i uc(indrowc), then z = sum_i R_i^T W_i zl_i
+  launch_kapply(st, D.pool.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p, D.Z.p, D.cmap.p,
+                D.uc.p, D.zl.p, D.maxNI, D.maxnc);
   launch_assemble(st, S.nig, D.aptr.p, D.asrc.p, D.w.p, D.zl.p, z);
 }
 
@@ -1167,7 +1179,7 @@ static void postprocess(Solver &S) {
     }
   }
   HIPCHECK(hipMemcpyAsync(D.vec.p, hv.data(), D.VT * sizeof(double), hipMemcpyHostToDevice, st));
-  launch_trsv_bwd(st, D.pool.p, D.trsv.p, S.nsub, D.tcptr.p, D.tck.p, D.tct.p, D.vec.p);
+  launch_trsv_bwd(st, D.pool.p, D.trsv.p, S.nsub, D.tcptr.p, D.tck.p, D.tct.p, D.vec.p, D.maxnt);
   HIPCHECK(hipMemcpyAsync(hv.data(), D.vec.p, D.VT * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   for (int is = 0; is < S.nsub; is++) {

@@ -16,7 +16,7 @@ namespace bddc {
 // op kinds of the batched GEMM-sum: C = alpha*C + beta * sum_p opA(A_p) opB(B_p)
 enum GemmKind { GEMM_NT = 0, GEMM_NN = 1, GEMM_TN = 2 };
 // post operations on E after the sum
-enum GemmPost { POST_NONE = 0, POST_RIGHT_XT = 1, POST_LEFT_X = 2 };
+enum GemmPost { POST_NONE = 0, POST_RIGHT_XT = 1, POST_LEFT_X = 2, POST_LEFT_XT = 3 };
 
 void launch_tile_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
                       int ntasks, int kind, int post, double alpha, double beta);
@@ -49,7 +49,10 @@ void launch_scatter(hipStream_t st, double *pool, const int64_t *idx, const doub
 struct SubDesc {
   int ni, NI, nti, nc;    // interface dofs, padded, tile count, constraints
   int sfull;              // first tile of S (full, row-major tile order)
-  int kfull;              // first tile of K (full, row-major tile order)
+  int pbase;              // first tile of the packed lower factor of S'
+  int plinv;              // first of the nti inverse diagonal factor tiles
+  int ybase;              // first tile of Z = S'^{-1} C^T (nti x ncT, row-major)
+  int ncT;                // constraint tile columns, ceil(nc/64)
   int64_t loff;           // offset into local interface vectors
   int64_t zoff;           // offset into Z / Phi (NI x nc column major)
   int64_t goff;           // offset into G / Ginv (nc x nc)
@@ -60,9 +63,13 @@ struct SubDesc {
 void launch_sprime(hipStream_t st, double *pool, const SubDesc *subs, const int *jobs_sub,
                    const int *jobs_ij, const int *jobs_dst, int njobs, const int *cid,
                    const int *csize);
-// Z(:,c) = sum_{j in D_c} K(:,j) with K full tiles
-void launch_z(hipStream_t st, const double *pool, const SubDesc *subs, int nsub, const int *cptr,
-              const int *cdof, double *Z, int maxnc);
+// right-hand sides C^T of the multi-RHS solve S' Z = C^T, written as the
+// nti x ncT tile block at ybase: B(i, c) = 1 if interface dof i is in D_c
+void launch_ct_init(hipStream_t st, double *pool, const SubDesc *subs, int nsub, const int *cid,
+                    int maxnti);
+// Z (column major NI x nc at zoff) from the solved tile block
+void launch_y_to_z(hipStream_t st, const double *pool, const SubDesc *subs, int nsub, double *Z,
+                   int maxNI);
 // G = C Z, Ginv = G^{-1} (in place Gauss-Jordan, SPD), Phi = Z Ginv,
 // Acs = Ginv - diag(rho/|D_c|)
 void launch_g(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const int *cdof,
@@ -70,9 +77,6 @@ void launch_g(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, co
 void launch_ginv(hipStream_t st, const SubDesc *subs, int nsub, double *G, double *work, int *err);
 void launch_phi(hipStream_t st, const SubDesc *subs, int nsub, const double *Z, const double *G,
                 double *Phi);
-// K(I,J) -= Phi_I Z_J^T for all full tiles
-void launch_kupdate(hipStream_t st, double *pool, const SubDesc *subs, const int *job_sub,
-                    const int *job_ij, int njobs, const double *Phi, const double *Z);
 void launch_acs(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const double *G,
                 double *Acs);
 // coarse assembly: dst[k] = sum_{q in ptr[k]..ptr[k+1]} Acs[src[q]] into pool
@@ -87,11 +91,13 @@ struct TrsvDesc {
   int linv;              // first Linv tile (nto consecutive)
   int64_t voff;          // offset into the device-ordered subdomain vector
 };
-void trsv_set_maxnt(int nt);  // largest nto+nti of the batch (LDS sizing)
+// maxnt = largest nto+nti of the batch (sizes the vector held in LDS)
 void launch_trsv_fwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
-                     const int *rptr, const int *rk, const int *rt, double *v);
+                     const int *rptr, const int *rk, const int *rt, double *v, int maxnt);
 void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
-                     const int *cptr, const int *ck, const int *ct, double *v);
+                     const int *cptr, const int *ck, const int *ct, double *v, int maxnt);
+// largest dynamic LDS allocation a kernel may request on the current device
+size_t max_dynamic_lds();
 
 // ---- iteration kernels ---------------------------------------------------
 struct GemvRow {
@@ -113,9 +119,13 @@ void launch_phit(hipStream_t st, const SubDesc *subs, int nsub, const double *Ph
                  const int *map, const double *w, const double *x, double *rcs, int maxnc);
 void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
                           const double *rcs, double *rc);
-// zlocal += Phi_s uc(indrowc)
-void launch_phi_add(hipStream_t st, const SubDesc *subs, int nsub, const double *Phi,
-                    const int *cmap, const double *uc, double *zlocal, int maxNI);
+// Local part of the BDDC preconditioner on every subdomain, one workgroup
+// each: x = w .* r(map); z = S'^{-1} x through the packed factor of S'
+// (forward and backward tile sweeps); zl = z + Phi (uc(indrowc) - Z^T x).
+void launch_kapply(hipStream_t st, const double *pool, const SubDesc *subs, int nsub,
+                   const int *map, const double *w, const double *r, const double *Phi,
+                   const double *Z, const int *cmap, const double *uc, double *zl, int maxNI,
+                   int maxnc);
 
 // deterministic reductions and vector updates on unique vectors
 constexpr int DOT_BLOCKS = 240;

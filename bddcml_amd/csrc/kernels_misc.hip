@@ -1,7 +1,8 @@
-// kernels_misc.hip -- interface preconditioner assembly (Z, G, Phi, K, coarse
-// element matrices), triangular sweeps of the partial factor, and the
-// per-iteration kernels of the Krylov methods (tile GEMV, weighted
-// gather/scatter of the interface exchange, coarse gather, dot products).
+// kernels_misc.hip -- interface preconditioner assembly (C^T blocks, Z, G,
+// Phi, coarse element matrices), triangular sweeps of the partial factor,
+// and the per-iteration kernels of the Krylov methods (tile GEMV, the local
+// preconditioner apply, weighted gather/scatter of the interface exchange,
+// coarse gather, dot products).
 #include "kernels.hpp"
 
 namespace bddc {
@@ -34,23 +35,30 @@ void launch_sprime(hipStream_t st, double *pool, const SubDesc *subs, const int 
                      jobs_dst, cid, csize);
 }
 
-// ------------------------------------------------------- Z(:,c) = K 1_{D_c}
-__global__ void k_z(const double *__restrict__ pool, const SubDesc *__restrict__ subs,
-                    const int *__restrict__ cptr, const int *__restrict__ cdof,
-                    double *__restrict__ Z) {
+// --------------------------------------------- C^T blocks and Z = S'^{-1} C^T
+__global__ void k_ct_init(double *__restrict__ pool, const SubDesc *__restrict__ subs,
+                          const int *__restrict__ cid) {
   const SubDesc d = subs[blockIdx.y];
-  const int c = blockIdx.x;
-  if (c >= d.nc) return;
-  const int q0 = cptr[d.coff + c], q1 = cptr[d.coff + c + 1];
-  for (int i = threadIdx.x; i < d.NI; i += blockDim.x) {
-    double s = 0.0;
-    const int ti = i >> 6, ri = i & 63;
-    for (int q = q0; q < q1; q++) {
-      int j = cdof[q];
-      s += pool[(size_t)(d.kfull + ti * d.nti + (j >> 6)) * TILE_ELEMS + (j & 63) * 64 + ri];
+  const int I = blockIdx.x;
+  if (I >= d.nti) return;
+  const int *ci = cid + d.loff + I * 64;
+  for (int cT = 0; cT < d.ncT; cT++) {
+    double *T = pool + (size_t)(d.ybase + I * d.ncT + cT) * TILE_ELEMS;
+    for (int e = threadIdx.x; e < TILE_ELEMS; e += blockDim.x) {
+      const int r = e & 63, c = cT * 64 + (e >> 6);
+      T[e] = (c < d.nc && ci[r] == d.coff + c) ? 1.0 : 0.0;
     }
-    Z[d.zoff + (size_t)c * d.NI + i] = s;
   }
+}
+
+__global__ void k_y_to_z(const double *__restrict__ pool, const SubDesc *__restrict__ subs,
+                         double *__restrict__ Z) {
+  const SubDesc d = subs[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.NI) return;
+  const double *T = pool + (size_t)(d.ybase + (i >> 6) * d.ncT) * TILE_ELEMS + (i & 63);
+  for (int c = 0; c < d.nc; c++)
+    Z[d.zoff + (size_t)c * d.NI + i] = T[(size_t)(c >> 6) * TILE_ELEMS + (c & 63) * 64];
 }
 
 __global__ void k_g(const SubDesc *__restrict__ subs, const int *__restrict__ cptr,
@@ -114,44 +122,6 @@ __global__ void k_phi(const SubDesc *__restrict__ subs, const double *__restrict
   }
 }
 
-// K(I,J) -= Phi(I rows,:) Z(J rows,:)^T
-__global__ void k_kupdate(double *__restrict__ pool, const SubDesc *__restrict__ subs,
-                          const int *__restrict__ jsub, const int *__restrict__ jij,
-                          const double *__restrict__ Phi, const double *__restrict__ Z) {
-  __shared__ double Ps[32 * 64];
-  __shared__ double Zs[32 * 64];
-  const int job = blockIdx.x;
-  const SubDesc d = subs[jsub[job]];
-  const int I = jij[job] >> 16, J = jij[job] & 0xffff;
-  double *T = pool + (size_t)(d.kfull + I * d.nti + J) * TILE_ELEMS;
-  double acc[16];
-#pragma unroll
-  for (int m = 0; m < 16; m++) acc[m] = 0.0;
-  for (int c0 = 0; c0 < d.nc; c0 += 32) {
-    const int cn = min(32, d.nc - c0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < cn * 64; e += blockDim.x) {
-      int c = e >> 6, r = e & 63;
-      Ps[c * 64 + r] = Phi[d.zoff + (size_t)(c0 + c) * d.NI + I * 64 + r];
-      Zs[c * 64 + r] = Z[d.zoff + (size_t)(c0 + c) * d.NI + J * 64 + r];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < 16; m++) {
-      int e = threadIdx.x + 256 * m;
-      int r = e & 63, col = e >> 6;
-      double s = 0.0;
-      for (int c = 0; c < cn; c++) s += Ps[c * 64 + r] * Zs[c * 64 + col];
-      acc[m] += s;
-    }
-  }
-#pragma unroll
-  for (int m = 0; m < 16; m++) {
-    int e = threadIdx.x + 256 * m;
-    T[e] -= acc[m];
-  }
-}
-
 __global__ void k_acs(const SubDesc *__restrict__ subs, const int *__restrict__ cptr,
                       const double *__restrict__ G, double *__restrict__ Acs) {
   const SubDesc d = subs[blockIdx.x];
@@ -174,10 +144,15 @@ __global__ void k_coarse_assemble(double *__restrict__ pool, const int64_t *__re
   pool[dst[k]] = s;
 }
 
-void launch_z(hipStream_t st, const double *pool, const SubDesc *subs, int nsub, const int *cptr,
-              const int *cdof, double *Z, int maxnc) {
-  if (nsub <= 0 || maxnc <= 0) return;
-  hipLaunchKernelGGL(k_z, dim3(maxnc, nsub), dim3(256), 0, st, pool, subs, cptr, cdof, Z);
+void launch_ct_init(hipStream_t st, double *pool, const SubDesc *subs, int nsub, const int *cid,
+                    int maxnti) {
+  if (nsub <= 0 || maxnti <= 0) return;
+  hipLaunchKernelGGL(k_ct_init, dim3(maxnti, nsub), dim3(256), 0, st, pool, subs, cid);
+}
+void launch_y_to_z(hipStream_t st, const double *pool, const SubDesc *subs, int nsub, double *Z,
+                   int maxNI) {
+  if (nsub <= 0 || maxNI <= 0) return;
+  hipLaunchKernelGGL(k_y_to_z, dim3((maxNI + 255) / 256, nsub), dim3(256), 0, st, pool, subs, Z);
 }
 void launch_g(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const int *cdof,
               const double *Z, double *G) {
@@ -194,11 +169,6 @@ void launch_phi(hipStream_t st, const SubDesc *subs, int nsub, const double *Z, 
   if (nsub <= 0) return;
   hipLaunchKernelGGL(k_phi, dim3(8192 / 256, nsub), dim3(256), 0, st, subs, Z, G, Phi);
 }
-void launch_kupdate(hipStream_t st, double *pool, const SubDesc *subs, const int *job_sub,
-                    const int *job_ij, int njobs, const double *Phi, const double *Z) {
-  if (njobs <= 0) return;
-  hipLaunchKernelGGL(k_kupdate, dim3(njobs), dim3(256), 0, st, pool, subs, job_sub, job_ij, Phi, Z);
-}
 void launch_acs(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const double *G,
                 double *Acs) {
   if (nsub <= 0) return;
@@ -212,63 +182,68 @@ void launch_coarse_assemble(hipStream_t st, double *pool, const int64_t *dst, co
 }
 
 // ------------------------------------------------------- triangular sweeps
-// 256 threads: y[r] = sum_c T(r,c) x[c] (trans=0) or sum_c T(c,r) x[c] (trans=1)
-// x and y in LDS; partial sums reduced through red[4*64].
-__device__ __forceinline__ void tile_gemv_lds(const double *__restrict__ T, const double *x,
-                                              double *red, bool trans) {
+// One workgroup per subdomain keeps the whole device-ordered vector in LDS.
+// A tile row accumulates all its tile products per thread and reduces once.
+// Forward:  v_I = Linv_I (b_I - sum_k L_Ik v_k)      for the nto factored rows,
+//           v_I = b_I - sum_k L_Ik v_k               for the nti trailing rows
+// Backward: v_I = Linv_I^T (y_I - sum_k L_kI^T v_k)  for the nto factored rows
+__device__ __forceinline__ double row_reduce(double s, double *red) {
   const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
-  double s = 0.0;
-  if (!trans) {
-#pragma unroll 4
-    for (int c = g; c < 64; c += 4) s += T[c * 64 + r] * x[c];
-  } else {
-    const double *row = T + r * 64 + g * 16;
-#pragma unroll
-    for (int k = 0; k < 16; k++) s += row[k] * x[g * 16 + k];
-  }
   red[g * 64 + r] = s;
+  __syncthreads();
+  double v = 0.0;
+  if (threadIdx.x < 64) v = (red[r] + red[64 + r]) + (red[128 + r] + red[192 + r]);
+  return v;
 }
 
-__global__ void k_trsv_fwd(const double *__restrict__ pool, const TrsvDesc *__restrict__ ds,
-                           const int *__restrict__ rptr, const int *__restrict__ rk,
-                           const int *__restrict__ rt, double *__restrict__ vec) {
+__global__ __launch_bounds__(256) void k_trsv_fwd(const double *__restrict__ pool,
+                                                  const TrsvDesc *__restrict__ ds,
+                                                  const int *__restrict__ rptr,
+                                                  const int *__restrict__ rk,
+                                                  const int *__restrict__ rt,
+                                                  double *__restrict__ vec) {
   extern __shared__ double smem[];
   const TrsvDesc d = ds[blockIdx.x];
   const int nt = d.nto + d.nti;
-  double *v = smem;                 // nt*64
-  double *red = smem + nt * 64;     // 256
-  double *t = red + 256;            // 64
+  double *v = smem;              // nt*64
+  double *red = smem + nt * 64;  // 256
+  double *t = red + 256;         // 64
   double *gv = vec + d.voff;
+  const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
   for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
   __syncthreads();
   for (int I = 0; I < nt; I++) {
-    if (threadIdx.x < 64) t[threadIdx.x] = v[I * 64 + threadIdx.x];
+    double s = 0.0;
     for (int q = rptr[d.rptr_off + I]; q < rptr[d.rptr_off + I + 1]; q++) {
-      __syncthreads();
-      tile_gemv_lds(pool + (size_t)rt[q] * TILE_ELEMS, v + rk[q] * 64, red, false);
-      __syncthreads();
-      if (threadIdx.x < 64)
-        t[threadIdx.x] -= red[threadIdx.x] + red[64 + threadIdx.x] + red[128 + threadIdx.x] +
-                          red[192 + threadIdx.x];
+      const double *T = pool + (size_t)rt[q] * TILE_ELEMS + r;
+      const double *x = v + rk[q] * 64;
+#pragma unroll 4
+      for (int c = g; c < 64; c += 4) s += T[c * 64] * x[c];
     }
+    double tv = row_reduce(s, red);
+    if (threadIdx.x < 64) t[r] = v[I * 64 + r] - tv;
     __syncthreads();
     if (I < d.nto) {
-      tile_gemv_lds(pool + (size_t)(d.linv + I) * TILE_ELEMS, t, red, false);
-      __syncthreads();
-      if (threadIdx.x < 64)
-        v[I * 64 + threadIdx.x] = red[threadIdx.x] + red[64 + threadIdx.x] +
-                                  red[128 + threadIdx.x] + red[192 + threadIdx.x];
+      const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r;
+      double u = 0.0;
+#pragma unroll 4
+      for (int c = g; c < 64; c += 4) u += L[c * 64] * t[c];
+      double w = row_reduce(u, red);
+      if (threadIdx.x < 64) v[I * 64 + r] = w;
     } else if (threadIdx.x < 64) {
-      v[I * 64 + threadIdx.x] = t[threadIdx.x];
+      v[I * 64 + r] = t[r];
     }
     __syncthreads();
   }
   for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) gv[k] = v[k];
 }
 
-__global__ void k_trsv_bwd(const double *__restrict__ pool, const TrsvDesc *__restrict__ ds,
-                           const int *__restrict__ cptr, const int *__restrict__ ck,
-                           const int *__restrict__ ct, double *__restrict__ vec) {
+__global__ __launch_bounds__(256) void k_trsv_bwd(const double *__restrict__ pool,
+                                                  const TrsvDesc *__restrict__ ds,
+                                                  const int *__restrict__ cptr,
+                                                  const int *__restrict__ ck,
+                                                  const int *__restrict__ ct,
+                                                  double *__restrict__ vec) {
   extern __shared__ double smem[];
   const TrsvDesc d = ds[blockIdx.x];
   const int nt = d.nto + d.nti;
@@ -276,42 +251,65 @@ __global__ void k_trsv_bwd(const double *__restrict__ pool, const TrsvDesc *__re
   double *red = smem + nt * 64;
   double *t = red + 256;
   double *gv = vec + d.voff;
+  const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
   for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
   __syncthreads();
   for (int I = d.nto - 1; I >= 0; I--) {
-    if (threadIdx.x < 64) t[threadIdx.x] = v[I * 64 + threadIdx.x];
+    double s = 0.0;
     for (int q = cptr[d.cptr_off + I]; q < cptr[d.cptr_off + I + 1]; q++) {
-      __syncthreads();
-      tile_gemv_lds(pool + (size_t)ct[q] * TILE_ELEMS, v + ck[q] * 64, red, true);
-      __syncthreads();
-      if (threadIdx.x < 64)
-        t[threadIdx.x] -= red[threadIdx.x] + red[64 + threadIdx.x] + red[128 + threadIdx.x] +
-                          red[192 + threadIdx.x];
+      // (L_kI^T v_k)[r] = sum_c L_kI(c, r) v_k[c], column r of the tile is contiguous
+      const double *T = pool + (size_t)ct[q] * TILE_ELEMS + r * 64 + g * 16;
+      const double *x = v + ck[q] * 64 + g * 16;
+#pragma unroll
+      for (int c = 0; c < 16; c++) s += T[c] * x[c];
     }
+    double tv = row_reduce(s, red);
+    if (threadIdx.x < 64) t[r] = v[I * 64 + r] - tv;
     __syncthreads();
-    tile_gemv_lds(pool + (size_t)(d.linv + I) * TILE_ELEMS, t, red, true);
-    __syncthreads();
-    if (threadIdx.x < 64)
-      v[I * 64 + threadIdx.x] = red[threadIdx.x] + red[64 + threadIdx.x] + red[128 + threadIdx.x] +
-                                red[192 + threadIdx.x];
+    const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r * 64 + g * 16;
+    double u = 0.0;
+#pragma unroll
+    for (int c = 0; c < 16; c++) u += L[c] * t[g * 16 + c];
+    double w = row_reduce(u, red);
+    if (threadIdx.x < 64) v[I * 64 + r] = w;
     __syncthreads();
   }
   for (int k = threadIdx.x; k < d.nto * 64; k += blockDim.x) gv[k] = v[k];
 }
 
-static int g_trsv_maxnt = 0;
-void trsv_set_maxnt(int nt) { g_trsv_maxnt = nt; }
+size_t max_dynamic_lds() {
+  static size_t cached = 0;
+  if (!cached) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      v = 65536;
+    cached = (size_t)v;
+  }
+  return cached;
+}
+
+// dynamic LDS request of a launch, checked against the device limit
+static void lds_request(const void *kernel, size_t bytes, const char *what) {
+  if (bytes > max_dynamic_lds()) fatal("BDDCML_DEVICE_PLAN", std::string(what) + ": subdomain too large for LDS");
+  if (bytes > 65536) {
+    if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+      fatal("BDDCML_DEVICE_PLAN", std::string(what) + ": cannot raise the LDS limit");
+  }
+}
 
 void launch_trsv_fwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
-                     const int *rptr, const int *rk, const int *rt, double *v) {
+                     const int *rptr, const int *rk, const int *rt, double *v, int maxnt) {
   if (nsub <= 0) return;
-  size_t sh = (size_t)(g_trsv_maxnt * 64 + 256 + 64) * sizeof(double);
+  size_t sh = (size_t)(maxnt * 64 + 256 + 64) * sizeof(double);
+  lds_request((const void *)k_trsv_fwd, sh, "triangular sweep");
   hipLaunchKernelGGL(k_trsv_fwd, dim3(nsub), dim3(256), sh, st, pool, d, rptr, rk, rt, v);
 }
 void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
-                     const int *cptr, const int *ck, const int *ct, double *v) {
+                     const int *cptr, const int *ck, const int *ct, double *v, int maxnt) {
   if (nsub <= 0) return;
-  size_t sh = (size_t)(g_trsv_maxnt * 64 + 256 + 64) * sizeof(double);
+  size_t sh = (size_t)(maxnt * 64 + 256 + 64) * sizeof(double);
+  lds_request((const void *)k_trsv_bwd, sh, "triangular sweep");
   hipLaunchKernelGGL(k_trsv_bwd, dim3(nsub), dim3(256), sh, st, pool, d, cptr, ck, ct, v);
 }
 
@@ -401,15 +399,113 @@ __global__ void k_coarse_gather(int n, const int *__restrict__ ptr, const int *_
   rc[k] = s;
 }
 
-__global__ void k_phi_add(const SubDesc *__restrict__ subs, const double *__restrict__ Phi,
-                          const int *__restrict__ cmap, const double *__restrict__ uc,
-                          double *__restrict__ zl) {
-  const SubDesc d = subs[blockIdx.y];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.NI) return;
-  double s = 0.0;
-  for (int c = 0; c < d.nc; c++) s += Phi[d.zoff + (size_t)c * d.NI + i] * uc[cmap[d.coff + c]];
-  zl[d.loff + i] += s;
+// ------------------------------------------------- local preconditioner apply
+constexpr int KA_THR = 1024;  // 16 waves per subdomain
+
+__device__ __forceinline__ int packed_lower(int n, int I, int J) {
+  return J * n - J * (J - 1) / 2 + (I - J);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(KA_THR) void k_kapply(
+    const double *__restrict__ pool, const SubDesc *__restrict__ subs, const int *__restrict__ map,
+    const double *__restrict__ w, const double *__restrict__ rin, const double *__restrict__ Phi,
+    const double *__restrict__ Z, const int *__restrict__ cmap, const double *__restrict__ uc,
+    double *__restrict__ zl) {
+  extern __shared__ double smem[];
+  const SubDesc d = subs[blockIdx.x];
+  const int NI = d.NI, nc = d.nc, n = d.nti;
+  double *xs = smem;        // NI: the subdomain interface vector
+  double *red = xs + NI;    // 16 x 64 partial sums
+  double *t = red + 1024;   // 64
+  double *cc = t + 64;      // nc: uc(indrowc) - Z^T x
+  const int tid = threadIdx.x, g = tid >> 6, l = tid & 63;
+  for (int k = tid; k < NI; k += KA_THR) {
+    const int m = map[d.loff + k];
+    xs[k] = m >= 0 ? w[d.loff + k] * rin[m] : 0.0;
+  }
+  __syncthreads();
+  for (int c = g; c < nc; c += 16) {
+    const double *zc = Z + d.zoff + (size_t)c * NI;
+    double s = 0.0;
+    for (int i = l; i < NI; i += 64) s += zc[i] * xs[i];
+    s = wave_sum(s);
+    if (l == 0) cc[c] = uc[cmap[d.coff + c]] - s;
+  }
+  // forward sweep x_I <- Linv_I (x_I - sum_{k<I} L_Ik x_k); lane = tile row
+  for (int I = 0; I < n; I++) {
+    double s = 0.0;
+#pragma unroll 2
+    for (int k = 0; k < I; k++) {
+      const double *T = pool + (size_t)(d.pbase + packed_lower(n, I, k)) * TILE_ELEMS + l;
+      const double *x = xs + k * 64;
+#pragma unroll
+      for (int m = 0; m < 4; m++) s += T[(g + 16 * m) * 64] * x[g + 16 * m];
+    }
+    red[g * 64 + l] = s;
+    __syncthreads();
+    if (tid < 64) {
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; q++) v += red[q * 64 + l];
+      t[l] = xs[I * 64 + l] - v;
+    }
+    __syncthreads();
+    {
+      const double *L = pool + (size_t)(d.plinv + I) * TILE_ELEMS + l;
+      double u = 0.0;
+#pragma unroll
+      for (int m = 0; m < 4; m++) u += L[(g + 16 * m) * 64] * t[g + 16 * m];
+      red[g * 64 + l] = u;
+    }
+    __syncthreads();
+    if (tid < 64) {
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; q++) v += red[q * 64 + l];
+      xs[I * 64 + l] = v;
+    }
+    __syncthreads();
+  }
+  // backward sweep x_I <- Linv_I^T (x_I - sum_{k>I} L_kI^T x_k); lane = tile
+  // row of L_kI (contiguous), wave g owns outputs g, g+16, g+32, g+48
+  for (int I = n - 1; I >= 0; I--) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int k = I + 1; k < n; k++) {
+      const double *T = pool + (size_t)(d.pbase + packed_lower(n, k, I)) * TILE_ELEMS + g * 64 + l;
+      const double xv = xs[k * 64 + l];
+#pragma unroll
+      for (int m = 0; m < 4; m++) a[m] += T[m * 16 * 64] * xv;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++) a[m] = wave_sum(a[m]);
+    if (l == 0) {
+#pragma unroll
+      for (int m = 0; m < 4; m++) t[g + 16 * m] = xs[I * 64 + g + 16 * m] - a[m];
+    }
+    __syncthreads();
+    const double *L = pool + (size_t)(d.plinv + I) * TILE_ELEMS + g * 64 + l;
+    const double tv = t[l];
+#pragma unroll
+    for (int m = 0; m < 4; m++) a[m] = wave_sum(L[m * 16 * 64] * tv);
+    if (l == 0) {
+#pragma unroll
+      for (int m = 0; m < 4; m++) xs[I * 64 + g + 16 * m] = a[m];
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  for (int i = tid; i < NI; i += KA_THR) {
+    double s = xs[i];
+    for (int c = 0; c < nc; c++) s += Phi[d.zoff + (size_t)c * NI + i] * cc[c];
+    zl[d.loff + i] = s;
+  }
 }
 
 void launch_gemv(hipStream_t st, const double *pool, const GemvRow *rows, int nrows,
@@ -434,11 +530,15 @@ void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
   if (n <= 0) return;
   hipLaunchKernelGGL(k_coarse_gather, dim3((n + 255) / 256), dim3(256), 0, st, n, ptr, src, rcs, rc);
 }
-void launch_phi_add(hipStream_t st, const SubDesc *subs, int nsub, const double *Phi,
-                    const int *cmap, const double *uc, double *zlocal, int maxNI) {
+void launch_kapply(hipStream_t st, const double *pool, const SubDesc *subs, int nsub,
+                   const int *map, const double *w, const double *r, const double *Phi,
+                   const double *Z, const int *cmap, const double *uc, double *zl, int maxNI,
+                   int maxnc) {
   if (nsub <= 0) return;
-  hipLaunchKernelGGL(k_phi_add, dim3((maxNI + 255) / 256, nsub), dim3(256), 0, st, subs, Phi, cmap,
-                     uc, zlocal);
+  size_t sh = (size_t)(maxNI + 1024 + 64 + maxnc) * sizeof(double);
+  lds_request((const void *)k_kapply, sh, "preconditioner apply");
+  hipLaunchKernelGGL(k_kapply, dim3(nsub), dim3(KA_THR), sh, st, pool, subs, map, w, r, Phi, Z,
+                     cmap, uc, zl);
 }
 
 // ------------------------------------------------------------ vector ops

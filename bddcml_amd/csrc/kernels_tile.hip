@@ -156,13 +156,14 @@ __global__ __launch_bounds__(NTHR) void k_tile_gemm(double *__restrict__ pool,
     // C_new(r,c) = sum_q E(r,q) X(c,q): A' = E (already As), B'(c,q) = X(c,q)
     tile_to_lds(X, Bs, false);
   } else {
-    // C_new(r,c) = sum_q X(r,q) E(q,c): B'(c,q) = E(q,c) = As[c*LDL + q]
+    // C_new(r,c) = sum_q X(r,q) E(q,c) (LEFT_X) or X(q,r) E(q,c) (LEFT_XT):
+    // B'(c,q) = E(q,c) = As[c*LDL + q]
     for (int e = threadIdx.x; e < 4096; e += NTHR) {
       int q = e & 63, c = e >> 6;
       Bs[q * LDL + c] = As[c * LDL + q];
     }
     __syncthreads();
-    tile_to_lds(X, As, false);
+    tile_to_lds(X, As, post == POST_LEFT_XT);
   }
   __syncthreads();
   zero_acc(acc);

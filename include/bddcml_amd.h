@@ -43,9 +43,8 @@ int bddcml_amd_sub_info(int isub, int *out);
 /* global dof of each local interface dof (reference interface order) and
  * global coarse dof of each constraint row */
 int bddcml_amd_sub_maps(int isub, int *gdofs, int *indrowc);
-/* dense interface matrices of one subdomain, row major, reference interface
- * order: which = 'S' (Schur complement) or 'K' (interface block of
- * [S C^T; C 0]^-1); Phi (ndofi x nconstr) and Ac (nconstr x nconstr) */
+/* dense Schur complement of one subdomain (which = 'S'), row major, in the
+ * reference interface order; Phi (ndofi x nconstr) and Ac (nconstr x nconstr) */
 int bddcml_amd_sub_matrix(int isub, int which, double *out);
 int bddcml_amd_sub_coarse(int isub, double *phi, double *ac);
 /* message of the last error raised while option "throw_on_error" is set

@@ -18,7 +18,7 @@ def _say(msg):
 
 
 @pytest.mark.parametrize("kind", [0, 1, 2])
-@pytest.mark.parametrize("post", [0, 1, 2])
+@pytest.mark.parametrize("post", [0, 1, 2, 3])
 def test_tile_gemm(kind, post):
     _say(f"case tile_gemm kind={kind} post={post}")
     npairs = 3
@@ -35,7 +35,7 @@ def test_tile_gemm(kind, post):
     else:
         S = sum(A[p].T @ B[p] for p in range(npairs))
     E = alpha * C + beta * S
-    ref = E if post == 0 else (E @ X.T if post == 1 else X @ E)
+    ref = [E, E @ X.T, X @ E, X.T @ E][post]
     assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max()
 
 

@@ -482,6 +482,16 @@ int bddcml_amd_set_option(const char *key, double value) try {
     S.plan_only = value != 0.0;
     return 0;
   }
+  if (!std::strcmp(key, "interior_ordering")) {
+    if (value != 0.0 && value != 1.0) return -2;
+    S.ordering = (int)value;
+    return 0;
+  }
+  if (!std::strcmp(key, "nd_leaf")) {
+    if (value < 1.0) return -2;
+    S.nd_leaf = (int)value;
+    return 0;
+  }
   return -1;
 }
 catch (const Error &e) {

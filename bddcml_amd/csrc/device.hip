@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <unordered_map>
 
 #include "kernels.hpp"
 #include "solver.hpp"
@@ -97,6 +98,7 @@ struct SubTiles {
   int nto = 0, nti = 0;
   TilePattern pat;   // A pattern (lower), tile ids absolute
   std::vector<std::vector<uint64_t>> col;  // column bitsets
+  int base = 0, span = 0;  // first tile id and tile count of the subdomain
   int linvA = 0, sfull = 0, pbase = 0, linvP = 0, ybase = 0, ncT = 0;
   int P(int I, int J) const { return pbase + lowidx(nti, I, J); }
   int Y(int I, int cT) const { return ybase + I * ncT + cT; }
@@ -229,6 +231,7 @@ static void symbolic(const Sub &s, SubTiles &T) {
 
 static void assign_tiles(SubTiles &T, int nc, int &next) {
   const int nt = T.nto + T.nti;
+  T.base = next;
   T.pat.nt = nt;
   T.pat.ntf = T.nto;
   T.pat.colptr.assign(nt + 1, 0);
@@ -253,6 +256,22 @@ static void assign_tiles(SubTiles &T, int nc, int &next) {
   T.ncT = (nc + TILE - 1) / TILE;
   T.ybase = next;
   next += T.nti * T.ncT;
+  T.span = next - T.base;
+}
+
+// Structural identity of two subdomains' tile layouts: same tile pattern,
+// interface and constraint tile counts, hence the same relative tile ids.
+static uint64_t structure_hash(const SubTiles &T) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+  mix(T.nto), mix(T.nti), mix(T.ncT), mix(T.span);
+  for (int v : T.pat.colptr) mix((uint64_t)v);
+  for (int v : T.pat.rowidx) mix((uint64_t)v);
+  return h;
+}
+static bool same_structure(const SubTiles &a, const SubTiles &b) {
+  return a.nto == b.nto && a.nti == b.nti && a.ncT == b.ncT && a.span == b.span &&
+         a.pat.colptr == b.pat.colptr && a.pat.rowidx == b.pat.rowidx;
 }
 
 // tasks of a dense lower Cholesky on tiles P(I,J) with inverse tiles Linv(J)
@@ -306,38 +325,6 @@ static void check_tasks(const TaskList &l, int64_t ntiles, bool uses_x, const ch
     if (bad(p.a) || bad(p.b)) fatal("BDDCML_DEVICE_PLAN", std::string("invalid tile pair in ") + what);
 }
 
-// Concatenates per-subdomain task lists so that the tasks of subdomain s sit
-// at positions congruent to s mod 8: workgroups are dealt round robin to the
-// 8 XCDs, so every subdomain's tiles are touched from a single L2.
-static TaskList xcd_merge(const std::vector<const TaskList *> &parts) {
-  constexpr int NX = 8;
-  std::vector<std::vector<std::pair<int, int>>> bins(NX);
-  size_t nt = 0, np = 0;
-  for (size_t s = 0; s < parts.size(); s++) {
-    if (!parts[s]) continue;
-    nt += parts[s]->tasks.size();
-    np += parts[s]->pairs.size();
-    for (size_t k = 0; k < parts[s]->tasks.size(); k++) bins[s % NX].push_back({(int)s, (int)k});
-  }
-  TaskList out;
-  out.tasks.reserve(nt);
-  out.pairs.reserve(np);
-  size_t longest = 0;
-  for (auto &b : bins) longest = std::max(longest, b.size());
-  for (size_t q = 0; q < longest; q++)
-    for (int x = 0; x < NX; x++) {
-      if (q >= bins[x].size()) continue;
-      const TaskList &L = *parts[bins[x][q].first];
-      TileTask t = L.tasks[bins[x][q].second];
-      int p0 = (int)out.pairs.size();
-      out.pairs.insert(out.pairs.end(), L.pairs.begin() + t.p0, L.pairs.begin() + t.p1);
-      t.p0 = p0;
-      t.p1 = (int)out.pairs.size();
-      out.tasks.push_back(t);
-    }
-  return out;
-}
-
 static void upload_phases(std::vector<Phase> &dst, const std::vector<TaskList> &src,
                           hipStream_t st) {
   dst.resize(src.size());
@@ -379,26 +366,57 @@ void device_setup(Solver &S) {
   D.ntiles = next;
 
   ph("symbolic");
-  // ---- task lists: generated per subdomain in parallel, then merged per
-  // phase so that the workgroups of one subdomain land on one XCD
-  // (workgroups are dealt to the 8 XCDs round robin), keeping its tiles in
-  // that XCD's L2
+  // ---- task lists.  Subdomains of identical tile structure form a class
+  // and share one list of tile pairs, kept relative to the subdomain's first
+  // tile; the tasks (output tile, inverse tile, pair range, tile base) are
+  // materialised per subdomain, subdomain-major within each step.
+  std::vector<int> cls(nsub, -1), reps;
+  {
+    std::vector<uint64_t> h(nsub);
+#pragma omp parallel for schedule(static)
+    for (int is = 0; is < nsub; is++) h[is] = structure_hash(D.sub[is]);
+    std::unordered_map<uint64_t, std::vector<int>> byhash;
+    for (int is = 0; is < nsub; is++) {
+      auto &cands = byhash[h[is]];
+      for (int r : cands)
+        if (same_structure(D.sub[r], D.sub[is])) {
+          cls[is] = cls[r];
+          break;
+        }
+      if (cls[is] < 0) {
+        cls[is] = (int)reps.size();
+        reps.push_back(is);
+        cands.push_back(is);
+      }
+    }
+  }
+  const int ncls = (int)reps.size();
   struct SubTasks {
     std::vector<TaskList> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd;
     TaskList a_schur;
   };
-  std::vector<SubTasks> sts(nsub);
-#pragma omp parallel for schedule(dynamic)
-  for (int is = 0; is < nsub; is++) {
-    const SubTiles &T = D.sub[is];
-    SubTasks &Q = sts[is];
+  std::vector<SubTasks> sts(ncls);
+  int bad_ids = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad_ids)
+  for (int ic = 0; ic < ncls; ic++) {
+    const SubTiles &T = D.sub[reps[ic]];
+    SubTasks &Q = sts[ic];
     const int nto = T.nto, nt = T.nto + T.nti;
     // row sets restricted to factored columns
     std::vector<std::vector<int>> rowk(nt);
     for (int J = 0; J < nto; J++)
       for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) rowk[T.pat.rowidx[q]].push_back(J);
-    Q.a_diag.resize(nto);
-    Q.a_off.resize(nto);
+    // level of each factored tile column in the tile elimination tree:
+    // columns of one level depend only on lower levels and share a step
+    std::vector<int> lvl(nto, 0);
+    int nlev = 0;
+    for (int J = 0; J < nto; J++) {
+      for (int k : rowk[J])
+        if (k < J) lvl[J] = std::max(lvl[J], lvl[k] + 1);
+      nlev = std::max(nlev, lvl[J] + 1);
+    }
+    Q.a_diag.resize(nlev);
+    Q.a_off.resize(nlev);
     auto common = [&](int I, int J, int kmax, TaskList &L) {
       const auto &a = rowk[I], &b = rowk[J];
       size_t x = 0, y = 0;
@@ -413,17 +431,18 @@ void device_setup(Solver &S) {
       }
     };
     for (int J = 0; J < nto; J++) {
-      TileTask t{T.pat.find(J, J), T.linvA + J, (int)Q.a_diag[J].pairs.size(), 0};
-      common(J, J, J, Q.a_diag[J]);
-      t.p1 = (int)Q.a_diag[J].pairs.size();
-      Q.a_diag[J].tasks.push_back(t);
+      TaskList &Dg = Q.a_diag[lvl[J]], &Of = Q.a_off[lvl[J]];
+      TileTask t{T.pat.find(J, J), T.linvA + J, (int)Dg.pairs.size(), 0};
+      common(J, J, J, Dg);
+      t.p1 = (int)Dg.pairs.size();
+      Dg.tasks.push_back(t);
       for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) {
         int I = T.pat.rowidx[q];
         if (I == J) continue;
-        TileTask u{T.pat.tile[q], T.linvA + J, (int)Q.a_off[J].pairs.size(), 0};
-        common(I, J, J, Q.a_off[J]);
-        u.p1 = (int)Q.a_off[J].pairs.size();
-        Q.a_off[J].tasks.push_back(u);
+        TileTask u{T.pat.tile[q], T.linvA + J, (int)Of.pairs.size(), 0};
+        common(I, J, J, Of);
+        u.p1 = (int)Of.pairs.size();
+        Of.tasks.push_back(u);
       }
     }
     for (int J = nto; J < nt; J++)
@@ -456,29 +475,75 @@ void device_setup(Solver &S) {
         u.p1 = (int)B.pairs.size();
         B.tasks.push_back(u);
       }
+    // make every tile id relative to the subdomain's first tile, checking
+    // that it stays inside the subdomain's tiles
+    auto inside = [&](int id) { return id >= T.base && id < T.base + T.span; };
+    auto relative = [&](TaskList &L) {
+      for (TileTask &t : L.tasks) {
+        if (!inside(t.c) || (t.x != -1 && !inside(t.x)) || t.p0 < 0 || t.p1 < t.p0 ||
+            t.p1 > (int)L.pairs.size())
+          bad_ids |= 1;
+        t.c -= T.base;
+        if (t.x >= 0) t.x -= T.base;
+      }
+      for (TilePair &p : L.pairs) {
+        if (!inside(p.a) || !inside(p.b)) bad_ids |= 1;
+        p.a -= T.base;
+        p.b -= T.base;
+      }
+    };
+    for (auto *v : {&Q.a_diag, &Q.a_off, &Q.p_diag, &Q.p_off, &Q.z_fwd, &Q.z_bwd})
+      for (TaskList &L : *v) relative(L);
+    relative(Q.a_schur);
   }
+  if (bad_ids) fatal("BDDCML_DEVICE_PLAN", "tile task outside its subdomain's tiles");
+  for (int is = 0; is < nsub; is++)
+    if ((int64_t)D.sub[is].base + D.sub[is].span > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "tile range");
+  // one step of every subdomain: shared pairs, per-subdomain tasks
+  auto materialize = [&](auto get) {
+    TaskList out;
+    std::vector<size_t> poff(ncls + 1, 0), toff(nsub + 1, 0);
+    for (int ic = 0; ic < ncls; ic++) {
+      const TaskList *L = get(sts[ic]);
+      poff[ic + 1] = poff[ic] + (L ? L->pairs.size() : 0);
+    }
+    for (int is = 0; is < nsub; is++) {
+      const TaskList *L = get(sts[cls[is]]);
+      toff[is + 1] = toff[is] + (L ? L->tasks.size() : 0);
+    }
+    out.pairs.resize(poff[ncls]);
+    out.tasks.resize(toff[nsub]);
+    for (int ic = 0; ic < ncls; ic++)
+      if (const TaskList *L = get(sts[ic])) std::copy(L->pairs.begin(), L->pairs.end(), out.pairs.begin() + poff[ic]);
+#pragma omp parallel for schedule(static)
+    for (int is = 0; is < nsub; is++) {
+      const TaskList *L = get(sts[cls[is]]);
+      if (!L) continue;
+      const int b = D.sub[is].base, po = (int)poff[cls[is]];
+      TileTask *o = out.tasks.data() + toff[is];
+      for (const TileTask &t : L->tasks)
+        *o++ = TileTask{t.c + b, t.x >= 0 ? t.x + b : -1, t.p0 + po, t.p1 + po, b};
+    }
+    return out;
+  };
   auto merge_steps = [&](std::vector<TaskList> SubTasks::*member) {
     size_t nsteps = 0;
     for (auto &Q : sts) nsteps = std::max(nsteps, (Q.*member).size());
     std::vector<TaskList> out(nsteps);
-    std::vector<const TaskList *> parts(nsub);
-    for (size_t k = 0; k < nsteps; k++) {
-      for (int is = 0; is < nsub; is++)
-        parts[is] = k < (sts[is].*member).size() ? &(sts[is].*member)[k] : nullptr;
-      out[k] = xcd_merge(parts);
-    }
+    for (size_t k = 0; k < nsteps; k++)
+      out[k] = materialize([&](const SubTasks &Q) -> const TaskList * {
+        return k < (Q.*member).size() ? &(Q.*member)[k] : nullptr;
+      });
     return out;
-  };
-  auto merge_one = [&](TaskList SubTasks::*member) {
-    std::vector<const TaskList *> parts(nsub);
-    for (int is = 0; is < nsub; is++) parts[is] = &(sts[is].*member);
-    return xcd_merge(parts);
   };
   std::vector<TaskList> a_diag = merge_steps(&SubTasks::a_diag), a_off = merge_steps(&SubTasks::a_off),
                         p_diag = merge_steps(&SubTasks::p_diag), p_off = merge_steps(&SubTasks::p_off),
                         z_fwd = merge_steps(&SubTasks::z_fwd), z_bwd = merge_steps(&SubTasks::z_bwd);
-  TaskList a_schur = merge_one(&SubTasks::a_schur);
+  TaskList a_schur = materialize([](const SubTasks &Q) -> const TaskList * { return &Q.a_schur; });
   sts.clear();
+  if (S.verbose >= 2)
+    std::printf("   device plan: %d structure classes over %d subdomains, %zu interior steps\n", ncls,
+                nsub, a_diag.size());
   std::vector<TaskList> c_diag, c_off, cw_step;
   TaskList cinv;
   {
@@ -488,16 +553,9 @@ void device_setup(Solver &S) {
     dense_chol_tasks(n, Pc, [=](int J) { return cl + J; }, c_diag, c_off);
     inverse_tasks(n, Pc, Wc, cw_step, cinv);
   }
-  for (auto &l : a_diag) check_tasks(l, D.ntiles, true, "interior Cholesky");
-  for (auto &l : a_off) check_tasks(l, D.ntiles, true, "interior panel");
-  for (auto &l : p_diag) check_tasks(l, D.ntiles, true, "interface Cholesky");
-  for (auto &l : p_off) check_tasks(l, D.ntiles, true, "interface panel");
-  for (auto &l : z_fwd) check_tasks(l, D.ntiles, true, "constraint solve (forward)");
-  for (auto &l : z_bwd) check_tasks(l, D.ntiles, true, "constraint solve (backward)");
   for (auto &l : c_diag) check_tasks(l, D.ntiles, true, "coarse Cholesky");
   for (auto &l : c_off) check_tasks(l, D.ntiles, true, "coarse panel");
   for (auto &l : cw_step) check_tasks(l, D.ntiles, true, "coarse inverse");
-  check_tasks(a_schur, D.ntiles, false, "Schur block");
   check_tasks(cinv, D.ntiles, false, "coarse W^T W");
   upload_phases(D.a_diag, a_diag, st);
   upload_phases(D.a_off, a_off, st);
@@ -519,43 +577,59 @@ void device_setup(Solver &S) {
   if (!S.din)
     for (int is = 0; is < nsub; is++) device_upload_entries(S, is);
   {
-    std::vector<int> perm, ttab;
-    std::vector<char> fix;
+    std::vector<size_t> poff(nsub + 1, 0), toff(nsub + 1, 0), doff(nsub + 1, 0);
+    for (int is = 0; is < nsub; is++) {
+      const Sub &s = S.subs[is];
+      const SubTiles &T = D.sub[is];
+      const size_t nt = T.nto + T.nti;
+      poff[is + 1] = poff[is] + s.ndofs;
+      toff[is + 1] = toff[is] + nt * (nt + 1) / 2;
+      doff[is + 1] = doff[is] + s.pad.size() + (s.NI - s.ni);
+    }
+    const int npad_c = D.NC - S.ndofc;
+    std::vector<int> perm(poff[nsub]), ttab(toff[nsub], -1);
+    std::vector<char> fix(poff[nsub]);
     std::vector<ScatterSub> ss(nsub);
-    std::vector<int2> ranges;
-    std::vector<int64_t> sidx;
-    std::vector<double> sval;
+    std::vector<int2> ranges(nsub + 1);
+    std::vector<int64_t> sidx(doff[nsub] + npad_c);
+    std::vector<double> sval(doff[nsub] + npad_c, 1.0);
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
     for (int is = 0; is < nsub; is++) {
       const Sub &s = S.subs[is];
       const SubTiles &T = D.sub[is];
       const int nt = T.nto + T.nti;
       ss[is] = ScatterSub{S.din->ai[is].p, S.din->aj[is].p, S.din->av[is].p,
-                          (int)S.din->ai[is].n, (int)perm.size(), (int)ttab.size()};
-      perm.insert(perm.end(), s.perm.begin(), s.perm.end());
-      for (int d = 0; d < s.ndofs; d++) fix.push_back(s.ifix[d] != 0);
-      size_t t0_ = ttab.size();
-      ttab.resize(t0_ + (size_t)nt * (nt + 1) / 2, -1);
+                          (int)S.din->ai[is].n, (int)poff[is], (int)toff[is]};
+      if ((int64_t)S.din->ai[is].n != (int64_t)s.ai.size()) bad |= 1;
+      for (int d = 0; d < s.ndofs; d++) {
+        perm[poff[is] + d] = s.perm[d];
+        fix[poff[is] + d] = s.ifix[d] != 0;
+        if (s.perm[d] < 0 || s.perm[d] >= s.NO + s.NI) bad |= 2;
+      }
+      int *tt = ttab.data() + toff[is];
       for (int J = 0; J < nt; J++)
         for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) {
           int I = T.pat.rowidx[q];
-          ttab[t0_ + (size_t)I * (I + 1) / 2 + J] = T.pat.tile[q];
+          tt[(size_t)I * (I + 1) / 2 + J] = T.pat.tile[q];
         }
-      ranges.push_back(make_int2(T.pat.tile.front(), (int)T.pat.tile.size()));
+      ranges[is] = make_int2(T.pat.tile.front(), (int)T.pat.tile.size());
+      int64_t *si = sidx.data() + doff[is];
       auto put_diag = [&](int p) {
         int tid = T.pat.find(p / TILE, p / TILE);
-        sidx.push_back((int64_t)tid * TILE_ELEMS + (p % TILE) * TILE + (p % TILE));
-        sval.push_back(1.0);
+        *si++ = (int64_t)tid * TILE_ELEMS + (p % TILE) * TILE + (p % TILE);
       };
-      for (int p = s.no; p < s.NO; p++) put_diag(p);
+      for (int p : s.pad) put_diag(p);
       for (int p = s.NO + s.ni; p < s.NO + s.NI; p++) put_diag(p);
     }
+    if (bad & 1) fatal("BDDCML_DEVICE_PLAN", "device triplets out of date");
+    if (bad & 2) fatal("BDDCML_DEVICE_PLAN", "invalid permutation");
     {
       const int n = D.ntc;
-      ranges.push_back(make_int2(D.cbase, n * (n + 1) / 2));
+      ranges[nsub] = make_int2(D.cbase, n * (n + 1) / 2);
       for (int p = S.ndofc; p < D.NC; p++) {
         int T_ = D.cbase + lowidx(n, p / TILE, p / TILE);
-        sidx.push_back((int64_t)T_ * TILE_ELEMS + (p % TILE) * TILE + (p % TILE));
-        sval.push_back(1.0);
+        sidx[doff[nsub] + (p - S.ndofc)] = (int64_t)T_ * TILE_ELEMS + (p % TILE) * TILE + (p % TILE);
       }
     }
     for (auto &r : ranges)
@@ -564,12 +638,6 @@ void device_setup(Solver &S) {
       if (k < 0 || k >= D.ntiles * (int64_t)TILE_ELEMS) fatal("BDDCML_DEVICE_PLAN", "scatter index out of range");
     for (int v : ttab)
       if (v >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid tile table entry");
-    for (int is = 0; is < nsub; is++) {
-      const Sub &s = S.subs[is];
-      if ((int64_t)S.din->ai[is].n != (int64_t)s.ai.size()) fatal("BDDCML_DEVICE_PLAN", "device triplets out of date");
-      for (int d = 0; d < s.ndofs; d++)
-        if (s.perm[d] < 0 || s.perm[d] >= s.NO + s.NI) fatal("BDDCML_DEVICE_PLAN", "invalid permutation");
-    }
     D.sc_perm.upload(perm, st);
     D.sc_fix.upload(fix, st);
     D.sc_ttab.upload(ttab, st);
@@ -619,14 +687,13 @@ void device_setup(Solver &S) {
   D.n_sp = (int)spsub.size();
 
   ph("mirror and update jobs");
-  // ---- interface descriptors and maps
+  // ---- interface descriptors and maps: offsets first, then every
+  // subdomain fills its own ranges in parallel
   D.hdesc.resize(nsub);
-  std::vector<int64_t> loff(nsub);
+  std::vector<int64_t> loff(nsub), cdoff(nsub + 1, 0);
   std::vector<int> NIs(nsub);
   int64_t lv = 0, zo = 0, go = 0;
   int co = 0;
-  std::vector<int> map, cidv, csz, cptrv(1, 0), cdofv, cmapv;
-  std::vector<double> wv;
   for (int is = 0; is < nsub; is++) {
     const Sub &s = S.subs[is];
     const SubTiles &T = D.sub[is];
@@ -644,41 +711,11 @@ void device_setup(Solver &S) {
     d.zoff = zo;
     d.goff = go;
     d.coff = co;
-    // rho: largest interface diagonal entry of the eliminated matrix
-    double rho = 0.0;
-    for (size_t k = 0; k < s.av.size(); k++)
-      if (s.ai[k] == s.aj[k] && s.perm[s.ai[k]] >= s.NO) rho = std::max(rho, std::fabs(s.av[k]));
-    d.rho = rho > 0.0 ? rho : 1.0;
+    d.rho = 1.0;
     D.hdesc[is] = d;
     loff[is] = lv;
     NIs[is] = s.NI;
-    std::vector<int> ref2dev(s.ni);
-    for (int k = 0; k < s.ni; k++) ref2dev[s.dev_ifc[k]] = k;
-    std::vector<int> cid_loc(s.NI, -1);
-    for (int c = 0; c < s.nconstr; c++) {
-      for (int q = s.cptr[c]; q < s.cptr[c + 1]; q++) {
-        int k = ref2dev[s.cidx[q]];
-        if (cid_loc[k] >= 0)
-          fatal("BDDCML_SETUP_PRECONDITIONER", "a dof belongs to two constraints; only corner and "
-                                               "arithmetic constraints are supported");
-        cid_loc[k] = co + c;
-        cdofv.push_back(k);
-      }
-      cptrv.push_back((int)cdofv.size());
-      csz.push_back(s.cptr[c + 1] - s.cptr[c]);
-      cmapv.push_back(s.indrowc[c]);
-    }
-    for (int k = 0; k < s.NI; k++) {
-      if (k < s.ni) {
-        int kr = s.dev_ifc[k];
-        map.push_back(S.gpos[s.isvgvn[s.ifc_dof[kr]]]);
-        wv.push_back(s.w[kr]);
-      } else {
-        map.push_back(-1);
-        wv.push_back(0.0);
-      }
-      cidv.push_back(cid_loc[k]);
-    }
+    cdoff[is + 1] = cdoff[is] + (s.nconstr ? s.cptr[s.nconstr] - s.cptr[0] : 0);
     lv += s.NI;
     zo += (int64_t)s.NI * s.nconstr;
     go += (int64_t)s.nconstr * s.nconstr;
@@ -687,11 +724,48 @@ void device_setup(Solver &S) {
     D.maxnc = std::max(D.maxnc, s.nconstr);
     D.maxnti = std::max(D.maxnti, s.nti);
   }
-  if (D.maxNI > 7936) fatal("BDDCML_SETUP_PRECONDITIONER", "interface larger than 7936 dofs per subdomain");
-  if ((size_t)(D.maxNI + 1024 + 64 + D.maxnc) * sizeof(double) > max_dynamic_lds())
-    fatal("BDDCML_SETUP_PRECONDITIONER", "subdomain interface too large for the preconditioner apply");
-  if (D.maxnc > 1024) fatal("BDDCML_SETUP_PRECONDITIONER", "more than 1024 constraints per subdomain");
-  if (D.NC > 7936) fatal("BDDCML_SETUP_PRECONDITIONER", "coarse problem larger than 7936 dofs");
+  std::vector<int> map(lv), cidv(lv), csz(co), cptrv(co + 1, 0), cdofv(cdoff[nsub]), cmapv(co);
+  std::vector<double> wv(lv);
+  int two_constraints = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : two_constraints)
+  for (int is = 0; is < nsub; is++) {
+    const Sub &s = S.subs[is];
+    SubDesc &d = D.hdesc[is];
+    // rho: largest interface diagonal entry of the eliminated matrix
+    double rho = 0.0;
+    for (size_t k = 0; k < s.av.size(); k++)
+      if (s.ai[k] == s.aj[k] && s.perm[s.ai[k]] >= s.NO) rho = std::max(rho, std::fabs(s.av[k]));
+    d.rho = rho > 0.0 ? rho : 1.0;
+    std::vector<int> ref2dev(s.ni);
+    for (int k = 0; k < s.ni; k++) ref2dev[s.dev_ifc[k]] = k;
+    int *cid_loc = cidv.data() + d.loff;
+    std::fill(cid_loc, cid_loc + s.NI, -1);
+    int64_t q2 = cdoff[is];
+    for (int c = 0; c < s.nconstr; c++) {
+      for (int q = s.cptr[c]; q < s.cptr[c + 1]; q++) {
+        int k = ref2dev[s.cidx[q]];
+        if (cid_loc[k] >= 0) two_constraints |= 1;
+        cid_loc[k] = d.coff + c;
+        cdofv[q2++] = k;
+      }
+      cptrv[d.coff + c + 1] = (int)q2;
+      csz[d.coff + c] = s.cptr[c + 1] - s.cptr[c];
+      cmapv[d.coff + c] = s.indrowc[c];
+    }
+    for (int k = 0; k < s.NI; k++) {
+      if (k < s.ni) {
+        int kr = s.dev_ifc[k];
+        map[d.loff + k] = S.gpos[s.isvgvn[s.ifc_dof[kr]]];
+        wv[d.loff + k] = s.w[kr];
+      } else {
+        map[d.loff + k] = -1;
+        wv[d.loff + k] = 0.0;
+      }
+    }
+  }
+  if (two_constraints)
+    fatal("BDDCML_SETUP_PRECONDITIONER", "a dof belongs to two constraints; only corner and "
+                                         "arithmetic constraints are supported");
   D.LV = lv;
   D.desc.upload(D.hdesc, st);
   D.loff.upload(loff, st);
@@ -741,50 +815,56 @@ void device_setup(Solver &S) {
     int64_t vt = 0;
     for (int is = 0; is < nsub; is++) voff[is] = vt, vt += S.subs[is].NO + S.subs[is].NI;
     D.VT = vt;
+    std::vector<int64_t> vpos(lv);
+#pragma omp parallel for schedule(static)
+    for (int is = 0; is < nsub; is++)
+      for (int k = 0; k < S.subs[is].NI; k++) vpos[loff[is] + k] = voff[is] + S.subs[is].NO + k;
     std::vector<int64_t> rsrc(src.size());
-    int isub = 0;
-    for (size_t q = 0; q < src.size(); q++) {
-      int64_t k = src[q];
-      // locate subdomain of local index k
-      int lo = 0, hi = nsub - 1;
-      while (lo < hi) {
-        int mid = (lo + hi + 1) / 2;
-        if (loff[mid] <= k) lo = mid;
-        else hi = mid - 1;
-      }
-      isub = lo;
-      rsrc[q] = voff[isub] + S.subs[isub].NO + (k - loff[isub]);
-    }
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < src.size(); q++) rsrc[q] = vpos[src[q]];
     D.rhs_src.upload(rsrc, st);
-    // triangular sweep descriptors
-    std::vector<int> rp, rkv, rtv, cp, ckv, ctv;
+    // triangular sweep lists of the interior factor: row lists (forward) and
+    // column lists (backward) of the strictly lower factored tiles
     D.htrsv.resize(nsub);
+    std::vector<size_t> roff(nsub + 1, 0), eoff(nsub + 1, 0), coff_(nsub + 1, 0);
     int maxnt = 1;
     for (int is = 0; is < nsub; is++) {
       const SubTiles &T = D.sub[is];
       const int nt = T.nto + T.nti;
       maxnt = std::max(maxnt, nt);
-      TrsvDesc d{T.nto, T.nti, (int)rp.size(), (int)cp.size(), T.linvA, voff[is]};
-      D.htrsv[is] = d;
-      std::vector<std::vector<std::pair<int, int>>> rows(nt);
-      for (int J = 0; J < T.nto; J++)
-        for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) {
-          int I = T.pat.rowidx[q];
-          if (I > J) rows[I].push_back({J, T.pat.tile[q]});
-        }
-      rp.push_back((int)rkv.size());
-      for (int I = 0; I < nt; I++) {
-        for (auto &e : rows[I]) rkv.push_back(e.first), rtv.push_back(e.second);
-        rp.push_back((int)rkv.size());
-      }
-      cp.push_back((int)ckv.size());
+      size_t ne = 0;
+      for (int J = 0; J < T.nto; J++) ne += T.pat.colptr[J + 1] - T.pat.colptr[J] - 1;
+      roff[is + 1] = roff[is] + nt + 1;
+      coff_[is + 1] = coff_[is] + T.nto + 1;
+      eoff[is + 1] = eoff[is] + ne;
+      D.htrsv[is] = TrsvDesc{T.nto, T.nti, (int)roff[is], (int)coff_[is], T.linvA, voff[is]};
+    }
+    std::vector<int> rp(roff[nsub]), rkv(eoff[nsub]), rtv(eoff[nsub]), cp(coff_[nsub]),
+        ckv(eoff[nsub]), ctv(eoff[nsub]);
+#pragma omp parallel for schedule(dynamic)
+    for (int is = 0; is < nsub; is++) {
+      const SubTiles &T = D.sub[is];
+      const int nt = T.nto + T.nti;
+      const int e0 = (int)eoff[is];
+      // column lists straight from the CSC pattern (diagonal skipped)
+      int e = e0;
       for (int J = 0; J < T.nto; J++) {
-        for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) {
-          int I = T.pat.rowidx[q];
-          if (I > J) ckv.push_back(I), ctv.push_back(T.pat.tile[q]);
-        }
-        cp.push_back((int)ckv.size());
+        cp[coff_[is] + J] = e;
+        for (int q = T.pat.colptr[J] + 1; q < T.pat.colptr[J + 1]; q++)
+          ckv[e] = T.pat.rowidx[q], ctv[e] = T.pat.tile[q], e++;
       }
+      cp[coff_[is] + T.nto] = e;
+      // row lists by counting sort over rows, columns ascending
+      std::vector<int> cnt(nt + 1, 0);
+      for (int J = 0; J < T.nto; J++)
+        for (int q = T.pat.colptr[J] + 1; q < T.pat.colptr[J + 1]; q++) cnt[T.pat.rowidx[q] + 1]++;
+      for (int I = 0; I < nt; I++) cnt[I + 1] += cnt[I];
+      for (int I = 0; I <= nt; I++) rp[roff[is] + I] = e0 + cnt[I];
+      for (int J = 0; J < T.nto; J++)
+        for (int q = T.pat.colptr[J] + 1; q < T.pat.colptr[J + 1]; q++) {
+          int k = e0 + cnt[T.pat.rowidx[q]]++;
+          rkv[k] = J, rtv[k] = T.pat.tile[q];
+        }
     }
     if ((size_t)(maxnt * 64 + 320) * sizeof(double) > max_dynamic_lds())
       fatal("BDDCML_SETUP_PRECONDITIONER", "subdomain larger than the triangular sweep supports");
@@ -880,12 +960,18 @@ void device_setup(Solver &S) {
   // plus 64^3/3 for its inverse, 2*64^3 per post multiply
   {
     const double g = 2.0 * 64 * 64 * 64, c = 64.0 * 64 * 64 / 3.0;
+    // pair lists are shared between subdomains: count the pairs of each task
+    auto products = [](const TaskList &l) {
+      double n = 0;
+      for (const TileTask &t : l.tasks) n += t.p1 - t.p0;
+      return n;
+    };
     auto work = [&](const std::vector<TaskList> &v, double per_task) {
       double f = 0;
-      for (auto &l : v) f += g * l.pairs.size() + per_task * l.tasks.size();
+      for (auto &l : v) f += g * products(l) + per_task * l.tasks.size();
       return f;
     };
-    S.t.schur_flops = g * a_schur.pairs.size();
+    S.t.schur_flops = g * products(a_schur);
     S.t.interior_flops = work(a_diag, 2 * c) + work(a_off, g) + S.t.schur_flops;
     S.t.interface_flops = work(p_diag, 2 * c) + work(p_off, g) + work(z_fwd, g) + work(z_bwd, g);
   }

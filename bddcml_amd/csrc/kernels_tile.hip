@@ -87,15 +87,16 @@ __device__ __forceinline__ void zero_acc(v4d acc[2][2]) {
 
 // acc = sum over pairs of opA(A) opB(B) in the "r,q x c,q" form
 __device__ __forceinline__ void gemm_sum(const double *__restrict__ pool, const TilePair *pairs,
-                                         int p0, int p1, int kind, double *As, double *Bs,
-                                         v4d acc[2][2]) {
+                                         int p0, int p1, int base, int kind, double *As,
+                                         double *Bs, v4d acc[2][2]) {
   const bool ta = (kind == GEMM_TN);
   const bool tb = (kind == GEMM_NN || kind == GEMM_TN);
+  const double *pb = pool + (size_t)base * TILE_ELEMS;
   for (int p = p0; p < p1; p++) {
     TilePair pr = pairs[p];
     __syncthreads();
-    tile_to_lds(pool + (size_t)pr.a * TILE_ELEMS, As, ta);
-    tile_to_lds(pool + (size_t)pr.b * TILE_ELEMS, Bs, tb);
+    tile_to_lds(pb + (size_t)pr.a * TILE_ELEMS, As, ta);
+    tile_to_lds(pb + (size_t)pr.b * TILE_ELEMS, Bs, tb);
     __syncthreads();
     mfma_accumulate(As, Bs, acc);
   }
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(NTHR) void k_tile_gemm(double *__restrict__ pool,
   const TileTask t = tasks[blockIdx.x];
   v4d acc[2][2];
   zero_acc(acc);
-  gemm_sum(pool, pairs, t.p0, t.p1, kind, As, Bs, acc);
+  gemm_sum(pool, pairs, t.p0, t.p1, t.base, kind, As, Bs, acc);
   double *C = pool + (size_t)t.c * TILE_ELEMS;
   form_e(C, alpha, beta, acc, As);  // E in As
   if (post == POST_NONE) {
@@ -185,7 +186,7 @@ __global__ __launch_bounds__(NTHR) void k_tile_potrf(double *__restrict__ pool,
   const int tid = threadIdx.x;
   v4d acc[2][2];
   zero_acc(acc);
-  gemm_sum(pool, pairs, t.p0, t.p1, GEMM_NT, As, Bs, acc);
+  gemm_sum(pool, pairs, t.p0, t.p1, t.base, GEMM_NT, As, Bs, acc);
   double *C = pool + (size_t)t.c * TILE_ELEMS;
   form_e(C, 1.0, -1.0, acc, As);
   if (tid == 0) bad = 0;

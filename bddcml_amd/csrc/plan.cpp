@@ -56,21 +56,30 @@ static void nodal_components(Sub &s) {
   if (!s.find_components) return;
   std::vector<int> kinet(s.nelems + 1, 0);
   for (int e = 0; e < s.nelems; e++) kinet[e + 1] = kinet[e] + s.nnet[e];
-  std::vector<std::vector<int>> node_el(s.nnods);
-  for (int e = 0; e < s.nelems; e++)
-    for (int k = kinet[e]; k < kinet[e + 1]; k++) node_el[s.inet[k]].push_back(e);
+  // node -> element incidence (CSR, elements ascending)
+  std::vector<int> nptr(s.nnods + 1, 0), nel(kinet[s.nelems]);
+  for (int k = 0; k < kinet[s.nelems]; k++) nptr[s.inet[k] + 1]++;
+  for (int v = 0; v < s.nnods; v++) nptr[v + 1] += nptr[v];
+  {
+    std::vector<int> fill(nptr.begin(), nptr.end() - 1);
+    for (int e = 0; e < s.nelems; e++)
+      for (int k = kinet[e]; k < kinet[e + 1]; k++) nel[fill[s.inet[k]]++] = e;
+  }
+  auto node_el = [&](int v) { return std::make_pair(nel.data() + nptr[v], nel.data() + nptr[v + 1]); };
   std::vector<int> xadj, adj;
   if (s.use_dual) {
     xadj.assign(s.nelems + 1, 0);
     std::vector<int> cnt(s.nelems, 0), touched;
     for (int e = 0; e < s.nelems; e++) {
       touched.clear();
-      for (int k = kinet[e]; k < kinet[e + 1]; k++)
-        for (int f : node_el[s.inet[k]])
-          if (f != e) {
-            if (cnt[f] == 0) touched.push_back(f);
-            cnt[f]++;
+      for (int k = kinet[e]; k < kinet[e + 1]; k++) {
+        auto r = node_el(s.inet[k]);
+        for (const int *f = r.first; f != r.second; f++)
+          if (*f != e) {
+            if (cnt[*f] == 0) touched.push_back(*f);
+            cnt[*f]++;
           }
+      }
       for (int f : touched) {
         if (cnt[f] >= s.neighbouring) adj.push_back(f);
         cnt[f] = 0;
@@ -85,8 +94,9 @@ static void nodal_components(Sub &s) {
     xadj.assign(s.nnods + 1, 0);
     std::vector<int> mark(s.nnods, -1);
     for (int v = 0; v < s.nnods; v++) {
-      for (int e : node_el[v])
-        for (int k = kinet[e]; k < kinet[e + 1]; k++) {
+      auto r = node_el(v);
+      for (const int *e = r.first; e != r.second; e++)
+        for (int k = kinet[*e]; k < kinet[*e + 1]; k++) {
           int w = s.inet[k];
           if (w != v && mark[w] != v) mark[w] = v, adj.push_back(w);
         }
@@ -600,11 +610,9 @@ void build_unique_interface(Solver &S) {
 // pseudo-peripheral start (George & Liu 1979).  Interface dofs: sorted by
 // the first interior position they couple to, so that the rows of the
 // interface block of the factor start as late as possible.
-static void order_subdomain(Sub &s) {
-  int n = s.ndofs;
-  std::vector<int> loc(n, -1);  // interior index
-  for (size_t k = 0; k < s.int_dof.size(); k++) loc[s.int_dof[k]] = (int)k;
-  int no = (int)s.int_dof.size(), ni = (int)s.ifc_dof.size();
+// Reverse Cuthill-McKee order of the interior dofs (pseudo-peripheral start,
+// neighbours by ascending degree).
+static std::vector<int> order_rcm(const Sub &s, const std::vector<int> &loc, int no) {
   std::vector<std::vector<int>> g(no);
   for (size_t k = 0; k < s.av.size(); k++) {
     int i = loc[s.ai[k]], j = loc[s.aj[k]];
@@ -655,29 +663,152 @@ static void order_subdomain(Sub &s) {
     }
   }
   std::reverse(order.begin(), order.end());
-  std::vector<int> pos(no);
-  for (int k = 0; k < no; k++) pos[order[k]] = k;
-  // interface: first interior coupling position
+  return order;
+}
+
+// Nested dissection of the interior nodes by recursive coordinate bisection.
+// A part is cut at the median coordinate of its longest extent; the nodes of
+// the upper half adjacent to the lower half form the separator.  Parts with
+// at most leaf_dofs dofs stay whole and are swept along their longest axis.
+// Blocks come out in elimination order: lower part, upper part, separator.
+namespace {
+struct Dissection {
+  const std::vector<int> &ptr, &adj;  // interior node graph (CSR)
+  const std::vector<double> &xyz;     // coordinates, column major nn x ndim
+  const std::vector<int> &w;          // dofs per interior node
+  int nn, ndim, leaf_dofs;
+  std::vector<int> side;              // scratch: 1 = lower half
+  std::vector<std::vector<int>> blocks;
+  double coord(int v, int d) const { return xyz[(size_t)d * nn + v]; }
+  int longest_axis(const std::vector<int> &part) const {
+    int ax = 0;
+    double best = -1.0;
+    for (int d = 0; d < ndim; d++) {
+      double lo = coord(part[0], d), hi = lo;
+      for (int v : part) lo = std::min(lo, coord(v, d)), hi = std::max(hi, coord(v, d));
+      if (hi - lo > best) best = hi - lo, ax = d;
+    }
+    return ax;
+  }
+  void leaf(std::vector<int> part) {
+    const int ax = longest_axis(part);
+    std::stable_sort(part.begin(), part.end(), [&](int a, int b) {
+      for (int k = 0; k < ndim; k++) {
+        int d = (ax + k) % ndim;
+        if (coord(a, d) != coord(b, d)) return coord(a, d) < coord(b, d);
+      }
+      return a < b;
+    });
+    blocks.push_back(std::move(part));
+  }
+  void split(std::vector<int> part) {
+    if (part.empty()) return;
+    long dofs = 0;
+    for (int v : part) dofs += w[v];
+    if (dofs <= leaf_dofs || ndim == 0) return leaf(std::move(part));
+    const int ax = longest_axis(part);
+    std::vector<double> c(part.size());
+    for (size_t k = 0; k < part.size(); k++) c[k] = coord(part[k], ax);
+    std::vector<double> sc(c);
+    std::nth_element(sc.begin(), sc.begin() + sc.size() / 2, sc.end());
+    const double m = sc[sc.size() / 2];
+    std::vector<int> lower, upper, sep;
+    for (size_t k = 0; k < part.size(); k++) (c[k] < m ? lower : upper).push_back(part[k]);
+    if (lower.empty()) return leaf(std::move(part));
+    for (int v : lower) side[v] = 1;
+    std::vector<int> rest;
+    for (int v : upper) {
+      bool touches = false;
+      for (int q = ptr[v]; q < ptr[v + 1] && !touches; q++) touches = side[adj[q]] == 1;
+      (touches ? sep : rest).push_back(v);
+    }
+    for (int v : lower) side[v] = 0;
+    split(std::move(lower));
+    split(std::move(rest));
+    if (!sep.empty()) blocks.push_back(std::move(sep));
+  }
+};
+}  // namespace
+
+static void order_subdomain(Sub &s, int method, int leaf_dofs, int ndim) {
+  const int n = s.ndofs;
+  std::vector<int> loc(n, -1);  // interior index
+  for (size_t k = 0; k < s.int_dof.size(); k++) loc[s.int_dof[k]] = (int)k;
+  const int no = (int)s.int_dof.size(), ni = (int)s.ifc_dof.size();
+  // device position of every interior dof; blocks start on tile boundaries
+  std::vector<int> ipos(no, -1);
+  int NO = 0;
+  s.pad.clear();
+  if (method == 1 || ndim <= 0 || s.xyz.size() < (size_t)s.nnods * ndim) {
+    std::vector<int> order = order_rcm(s, loc, no);
+    for (int k = 0; k < no; k++) ipos[order[k]] = k;
+    NO = (no + TILE - 1) / TILE * TILE;
+    for (int p = no; p < NO; p++) s.pad.push_back(p);
+  } else {
+    // interior nodes (all dofs of a node share its interior/interface role)
+    std::vector<int> dnode(n);
+    for (int l = 0; l < s.nnods; l++)
+      for (int d = s.kdof[l]; d < s.kdof[l + 1]; d++) dnode[d] = l;
+    std::vector<int> nloc(s.nnods, -1), nodes;
+    for (int d : s.int_dof)
+      if (nloc[dnode[d]] < 0) nloc[dnode[d]] = (int)nodes.size(), nodes.push_back(dnode[d]);
+    const int nn = (int)nodes.size();
+    // symmetric node adjacency by counting sort (repeated neighbours of
+    // multi-dof nodes are harmless to the separator test)
+    std::vector<int> ptr(nn + 1, 0);
+    for (size_t k = 0; k < s.av.size(); k++) {
+      int a = nloc[dnode[s.ai[k]]], b = nloc[dnode[s.aj[k]]];
+      if (a >= 0 && b >= 0 && a != b) ptr[a + 1]++, ptr[b + 1]++;
+    }
+    for (int v = 0; v < nn; v++) ptr[v + 1] += ptr[v];
+    std::vector<int> adj(ptr[nn]);
+    {
+      std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+      for (size_t k = 0; k < s.av.size(); k++) {
+        int a = nloc[dnode[s.ai[k]]], b = nloc[dnode[s.aj[k]]];
+        if (a >= 0 && b >= 0 && a != b) adj[fill[a]++] = b, adj[fill[b]++] = a;
+      }
+    }
+    std::vector<double> xyz((size_t)nn * ndim);
+    std::vector<int> w(nn);
+    for (int v = 0; v < nn; v++) {
+      for (int d = 0; d < ndim; d++) xyz[(size_t)d * nn + v] = s.xyz[(size_t)d * s.nnods + nodes[v]];
+      w[v] = s.kdof[nodes[v] + 1] - s.kdof[nodes[v]];
+    }
+    Dissection nd{ptr, adj, xyz, w, nn, ndim, std::max(leaf_dofs, 1), std::vector<int>(nn, 0), {}};
+    std::vector<int> all(nn);
+    std::iota(all.begin(), all.end(), 0);
+    nd.split(std::move(all));
+    int p = 0;
+    for (const auto &blk : nd.blocks) {
+      for (int v : blk)
+        for (int d = s.kdof[nodes[v]]; d < s.kdof[nodes[v] + 1]; d++) ipos[loc[d]] = p++;
+      int q = (p + TILE - 1) / TILE * TILE;
+      for (; p < q; p++) s.pad.push_back(p);
+    }
+    NO = p;
+  }
+  // interface: by first interior coupling position
   std::vector<int> iloc(n, -1);
   for (int k = 0; k < ni; k++) iloc[s.ifc_dof[k]] = k;
-  std::vector<int> first(ni, no);
+  std::vector<int> first(ni, NO);
   for (size_t k = 0; k < s.av.size(); k++) {
     int i = s.ai[k], j = s.aj[k];
-    if (loc[i] >= 0 && iloc[j] >= 0) first[iloc[j]] = std::min(first[iloc[j]], pos[loc[i]]);
-    if (loc[j] >= 0 && iloc[i] >= 0) first[iloc[i]] = std::min(first[iloc[i]], pos[loc[j]]);
+    if (loc[i] >= 0 && iloc[j] >= 0) first[iloc[j]] = std::min(first[iloc[j]], ipos[loc[i]]);
+    if (loc[j] >= 0 && iloc[i] >= 0) first[iloc[i]] = std::min(first[iloc[i]], ipos[loc[j]]);
   }
   std::vector<int> io(ni);
   std::iota(io.begin(), io.end(), 0);
   std::stable_sort(io.begin(), io.end(), [&](int a, int b) { return first[a] < first[b]; });
   s.no = no;
   s.ni = ni;
-  s.NO = (no + TILE - 1) / TILE * TILE;
+  s.NO = NO;
   s.NI = (ni + TILE - 1) / TILE * TILE;
   if (s.NI == 0) s.NI = TILE;
   s.nto = s.NO / TILE;
   s.nti = s.NI / TILE;
   s.perm.assign(n, -1);
-  for (int k = 0; k < no; k++) s.perm[s.int_dof[order[k]]] = k;
+  for (int k = 0; k < no; k++) s.perm[s.int_dof[k]] = ipos[k];
   s.dev_ifc.resize(ni);
   for (int k = 0; k < ni; k++) {
     s.dev_ifc[k] = io[k];
@@ -687,7 +818,7 @@ static void order_subdomain(Sub &s) {
 
 void build_orderings(Solver &S) {
 #pragma omp parallel for schedule(dynamic)
-  for (int is = 0; is < S.nsub; is++) order_subdomain(S.subs[is]);
+  for (int is = 0; is < S.nsub; is++) order_subdomain(S.subs[is], S.ordering, S.nd_leaf, S.ndim);
 }
 
 }  // namespace bddc

@@ -80,14 +80,17 @@ struct Sub {
   int nto = 0, nti = 0;     // tile counts (interior, interface)
   std::vector<int> perm;    // device position of each local dof
   std::vector<int> dev_ifc; // interface device index k (0..ni-1) -> reference interface index
+  std::vector<int> pad;     // interior device positions holding no dof (unit diagonal)
 };
 
 // One batched tile task: C = alpha*C + beta*sum(op(A_p) op(B_p)), optional
-// post multiply by tile X, optional Cholesky.
+// post multiply by tile X, optional Cholesky.  Pair tile ids are relative to
+// base, so subdomains of identical structure share one pair list.
 struct TileTask {
   int c;         // output tile id
   int x;         // post-multiply / inverse tile id (-1 none)
   int p0, p1;    // pair range
+  int base = 0;  // added to the tile ids of the pairs
 };
 struct TilePair {
   int a, b;

@@ -36,6 +36,8 @@ struct Solver {
   // options
   int use_corner = 1, use_arith = 1, weights_type = 0;
   int precision = 64;  // 64: FP64 everywhere; 32: FP32 preconditioner factors
+  int ordering = 0;    // interior ordering: 0 tile-aligned nested dissection, 1 RCM
+  int nd_leaf = 256;   // nested dissection: largest part (dofs) left undivided
   // topology results
   int ncorner = 0, nedge = 0, nface = 0, nnodc = 0, ndofc = 0;
   std::vector<int> nndfc, kdofc;
@@ -71,7 +73,7 @@ void build_weights(Solver &S);
 void build_constraints(Solver &S);
 void build_condensation_rhs(Solver &S, int isub_only = -1);
 void build_unique_interface(Solver &S);
-void build_orderings(Solver &S);
+void build_orderings(Solver &S);  // interior/interface device orderings
 
 // device steps (device.hip)
 void device_upload_entries(Solver &S, int is);

@@ -63,6 +63,10 @@ def main():
     ap.add_argument("--nsub", type=int, default=8, help="subdomains per cube edge")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ordering", type=int, default=0,
+                    help="interior ordering: 0 tile-aligned nested dissection, 1 RCM")
+    ap.add_argument("--nd-leaf", type=int, default=256,
+                    help="nested dissection: largest part (dofs) left whole")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); every subdomain of the "
@@ -83,6 +87,8 @@ def main():
         b = bddcml_amd.Bddcml()
         b.init(2, [p.nsub, 1], numbase=p.numbase,
                verbose_level=int(os.environ.get("BENCH_VERBOSE", "0")))
+        b.set_option("interior_ordering", args.ordering)
+        b.set_option("nd_leaf", args.nd_leaf)
         t0 = time.perf_counter()
         for d in subs:
             b.upload_subdomain_data(d)

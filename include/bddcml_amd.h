@@ -35,8 +35,16 @@ int bddcml_amd_residual_history(double *out, int maxn);
  * factorisation and the interface phase; event milliseconds and algorithmic
  * bytes of one batched Schur GEMV; last iteration count */
 int bddcml_amd_timings(double *out, int n);
-/* options: "verbose" (0/1/2), "plan_only" (1: setup stops after the host
- * plan, no device work; for inspecting topology without a GPU) */
+/* options (set after bddcml_init_c, which resets them):
+ *   "verbose" (0/1/2);
+ *   "plan_only" (1: setup stops after the host plan, no device work; for
+ *     inspecting topology without a GPU);
+ *   "throw_on_error" (1: errors are recorded for bddcml_amd_last_error
+ *     instead of aborting);
+ *   "interior_ordering" (0: tile-aligned nested dissection by coordinate
+ *     bisection, the default; 1: reverse Cuthill-McKee);
+ *   "nd_leaf" (largest part, in dofs, that nested dissection leaves whole;
+ *     default 256) */
 int bddcml_amd_set_option(const char *key, double value);
 /* per subdomain (isub in numbase): out[0]=ndofi out[1]=nconstr out[2]=ndofo */
 int bddcml_amd_sub_info(int isub, int *out);

@@ -92,6 +92,20 @@ struct Phase {
   }
 };
 
+// one launch of a fixed launch sequence (tile Cholesky or GEMM-sum)
+struct SeqLaunch {
+  TaskList l;
+  int kind = GEMM_NT, post = POST_NONE;
+  double alpha = 1.0, beta = -1.0;
+  bool potrf = false;
+};
+struct SeqPhase {
+  Phase ph;
+  int kind = GEMM_NT, post = POST_NONE;
+  double alpha = 1.0, beta = -1.0;
+  bool potrf = false;
+};
+
 static inline int lowidx(int n, int I, int J) { return J * n - J * (J - 1) / 2 + (I - J); }
 
 struct SubTiles {
@@ -118,8 +132,9 @@ struct Device {
   DBuf<int> err;
   std::vector<SubTiles> sub;
   // setup phases
-  std::vector<Phase> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd, c_diag, c_off, cw_step;
+  std::vector<Phase> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd;
   Phase a_schur, cinv;
+  std::vector<SeqPhase> cseq;  // coarse factorization and inverse
   DBuf<int> zero_ids, sc_perm, sc_ttab;
   DBuf<char> sc_fix;
   DBuf<ScatterSub> sc_subs;
@@ -291,27 +306,6 @@ static void dense_chol_tasks(int n, FP Ptile, FL Linv, std::vector<TaskList> &di
       off[J].tasks.push_back(u);
     }
   }
-}
-
-// W = L^{-1} (strict lower tiles, diagonal = Linv) and Kraw = W^T W for the
-// explicit inverse of a dense factor
-template <class FP, class FW>
-static void inverse_tasks(int n, FP Ptile, FW Wt, std::vector<TaskList> &wstep, TaskList &kraw) {
-  if ((int)wstep.size() < n) wstep.resize(n);
-  for (int i = 1; i < n; i++)
-    for (int j = 0; j < i; j++) {
-      TileTask t{Wt(i, j), Wt(i, i), (int)wstep[i].pairs.size(), 0};
-      for (int k = j; k < i; k++) wstep[i].pairs.push_back({Ptile(i, k), Wt(k, j)});
-      t.p1 = (int)wstep[i].pairs.size();
-      wstep[i].tasks.push_back(t);
-    }
-  for (int j = 0; j < n; j++)
-    for (int i = j; i < n; i++) {
-      TileTask t{Ptile(i, j), -1, (int)kraw.pairs.size(), 0};
-      for (int k = i; k < n; k++) kraw.pairs.push_back({Wt(k, i), Wt(k, j)});
-      t.p1 = (int)kraw.pairs.size();
-      kraw.tasks.push_back(t);
-    }
 }
 
 // Host-side bounds check of every tile id a task list hands to the kernels.
@@ -544,18 +538,59 @@ void device_setup(Solver &S) {
   if (S.verbose >= 2)
     std::printf("   device plan: %d structure classes over %d subdomains, %zu interior steps\n", ncls,
                 nsub, a_diag.size());
-  std::vector<TaskList> c_diag, c_off, cw_step;
+  // coarse problem (one dense matrix): right-looking tile Cholesky and the
+  // inverse factor W = L^{-1} by right-looking forward substitution, so that
+  // every task is a single tile product and each step is a few short
+  // launches; then Ac^{-1} = W^T W
+  std::vector<SeqLaunch> cseq;
   TaskList cinv;
   {
     const int n = D.ntc, cb = D.cbase, cl = D.clinv, cw = D.cwbase;
     auto Pc = [=](int I, int J) { return cb + lowidx(n, I, J); };
     auto Wc = [=](int I, int J) { return I == J ? cl + I : cw + lowidx(n - 1, I - 1, J); };
-    dense_chol_tasks(n, Pc, [=](int J) { return cl + J; }, c_diag, c_off);
-    inverse_tasks(n, Pc, Wc, cw_step, cinv);
+    auto one = [](TaskList &l, int c, int a, int b) {
+      const int p0 = (int)l.pairs.size();
+      l.pairs.push_back({a, b});
+      l.tasks.push_back({c, -1, p0, p0 + 1});
+    };
+    for (int k = 0; k < n; k++) {
+      SeqLaunch f;  // L_kk = chol(A_kk), Linv_k
+      f.potrf = true;
+      f.l.tasks.push_back({Pc(k, k), cl + k, 0, 0});
+      cseq.push_back(std::move(f));
+      if (k + 1 < n) {  // L_ik = A_ik Linv_k^T
+        SeqLaunch pn;
+        pn.post = POST_RIGHT_XT, pn.beta = 0.0;
+        for (int i = k + 1; i < n; i++) pn.l.tasks.push_back({Pc(i, k), cl + k, 0, 0});
+        cseq.push_back(std::move(pn));
+      }
+      if (k > 0) {  // W_kj = Linv_k B_kj
+        SeqLaunch wr;
+        wr.kind = GEMM_NN, wr.post = POST_LEFT_X, wr.beta = 0.0;
+        for (int j = 0; j < k; j++) wr.l.tasks.push_back({Wc(k, j), cl + k, 0, 0});
+        cseq.push_back(std::move(wr));
+      }
+      if (k + 1 < n) {
+        SeqLaunch tr;  // A_IK -= L_Ik L_Kk^T
+        for (int K = k + 1; K < n; K++)
+          for (int I = K; I < n; I++) one(tr.l, Pc(I, K), Pc(I, k), Pc(K, k));
+        cseq.push_back(std::move(tr));
+        SeqLaunch bu;  // B_ij -= L_ik W_kj
+        bu.kind = GEMM_NN;
+        for (int i = k + 1; i < n; i++)
+          for (int j = 0; j <= k; j++) one(bu.l, Wc(i, j), Pc(i, k), Wc(k, j));
+        cseq.push_back(std::move(bu));
+      }
+    }
+    for (int j = 0; j < n; j++)
+      for (int i = j; i < n; i++) {
+        TileTask t{Pc(i, j), -1, (int)cinv.pairs.size(), 0};
+        for (int k = i; k < n; k++) cinv.pairs.push_back({Wc(k, i), Wc(k, j)});
+        t.p1 = (int)cinv.pairs.size();
+        cinv.tasks.push_back(t);
+      }
   }
-  for (auto &l : c_diag) check_tasks(l, D.ntiles, true, "coarse Cholesky");
-  for (auto &l : c_off) check_tasks(l, D.ntiles, true, "coarse panel");
-  for (auto &l : cw_step) check_tasks(l, D.ntiles, true, "coarse inverse");
+  for (auto &q : cseq) check_tasks(q.l, D.ntiles, q.potrf || q.post != POST_NONE, "coarse factorization");
   check_tasks(cinv, D.ntiles, false, "coarse W^T W");
   upload_phases(D.a_diag, a_diag, st);
   upload_phases(D.a_off, a_off, st);
@@ -563,9 +598,13 @@ void device_setup(Solver &S) {
   upload_phases(D.p_off, p_off, st);
   upload_phases(D.z_fwd, z_fwd, st);
   upload_phases(D.z_bwd, z_bwd, st);
-  upload_phases(D.c_diag, c_diag, st);
-  upload_phases(D.c_off, c_off, st);
-  upload_phases(D.cw_step, cw_step, st);
+  D.cseq.resize(cseq.size());
+  for (size_t k = 0; k < cseq.size(); k++) {
+    SeqPhase &q = D.cseq[k];
+    q.ph.upload(cseq[k].l, st);
+    q.kind = cseq[k].kind, q.post = cseq[k].post, q.alpha = cseq[k].alpha, q.beta = cseq[k].beta;
+    q.potrf = cseq[k].potrf;
+  }
   D.a_schur.upload(a_schur, st);
   D.cinv.upload(cinv, st);
 
@@ -626,7 +665,8 @@ void device_setup(Solver &S) {
     if (bad & 2) fatal("BDDCML_DEVICE_PLAN", "invalid permutation");
     {
       const int n = D.ntc;
-      ranges[nsub] = make_int2(D.cbase, n * (n + 1) / 2);
+      // factor, inverse diagonal and W tiles of the coarse matrix
+      ranges[nsub] = make_int2(D.cbase, n * (n + 1) / 2 + n + n * (n - 1) / 2);
       for (int p = S.ndofc; p < D.NC; p++) {
         int T_ = D.cbase + lowidx(n, p / TILE, p / TILE);
         sidx[doff[nsub] + (p - S.ndofc)] = (int64_t)T_ * TILE_ELEMS + (p % TILE) * TILE + (p % TILE);
@@ -1017,14 +1057,13 @@ void device_setup(Solver &S) {
   HIPCHECK(hipEventRecord(ev[3], st));
   // coarse problem
   launch_coarse_assemble(st, pool, D.ca_dst.p, D.ca_ptr.p, D.ca_src.p, D.nca, D.Acs.p);
-  for (size_t J = 0; J < D.c_diag.size(); J++) {
-    launch_tile_potrf(st, pool, D.c_diag[J].tasks.p, D.c_diag[J].pairs.p, D.c_diag[J].ntask, D.err.p);
-    launch_tile_gemm(st, pool, D.c_off[J].tasks.p, D.c_off[J].pairs.p, D.c_off[J].ntask, GEMM_NT,
-                     POST_RIGHT_XT, 1.0, -1.0);
+  for (const SeqPhase &q : D.cseq) {
+    if (q.potrf)
+      launch_tile_potrf(st, pool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, D.err.p);
+    else
+      launch_tile_gemm(st, pool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, q.kind, q.post, q.alpha,
+                       q.beta);
   }
-  for (size_t i = 1; i < D.cw_step.size(); i++)
-    launch_tile_gemm(st, pool, D.cw_step[i].tasks.p, D.cw_step[i].pairs.p, D.cw_step[i].ntask,
-                     GEMM_NN, POST_LEFT_X, 0.0, -1.0);
   launch_tile_gemm(st, pool, D.cinv.tasks.p, D.cinv.pairs.p, D.cinv.ntask, GEMM_TN, POST_NONE, 0.0,
                    1.0);
   launch_tile_mirror(st, pool, D.mir_c.p, (int)D.mir_c.n);

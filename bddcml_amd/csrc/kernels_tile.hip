@@ -85,19 +85,56 @@ __device__ __forceinline__ void zero_acc(v4d acc[2][2]) {
     for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
 }
 
-// acc = sum over pairs of opA(A) opB(B) in the "r,q x c,q" form
+// a tile held in registers: thread tid owns double2 elements tid + NTHR*i
+struct TileRegs {
+  double2 v[8];
+};
+__device__ __forceinline__ void tile_load(const double *__restrict__ T, TileRegs &r) {
+  const double2 *T2 = reinterpret_cast<const double2 *>(T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) r.v[i] = T2[threadIdx.x + NTHR * i];
+}
+__device__ __forceinline__ void tile_store_lds(const TileRegs &r, double *lds, bool trans) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int e = 2 * (threadIdx.x + NTHR * i);
+    const int row = e & 63, col = e >> 6;
+    if (!trans) {
+      lds[col * LDL + row] = r.v[i].x;
+      lds[col * LDL + row + 1] = r.v[i].y;
+    } else {
+      lds[row * LDL + col] = r.v[i].x;
+      lds[(row + 1) * LDL + col] = r.v[i].y;
+    }
+  }
+}
+
+// acc = sum over pairs of opA(A) opB(B) in the "r,q x c,q" form.  Software
+// pipelined: the next pair's tiles are loaded into registers while the
+// matrix cores work on the current pair from LDS.
 __device__ __forceinline__ void gemm_sum(const double *__restrict__ pool, const TilePair *pairs,
                                          int p0, int p1, int base, int kind, double *As,
                                          double *Bs, v4d acc[2][2]) {
   const bool ta = (kind == GEMM_TN);
   const bool tb = (kind == GEMM_NN || kind == GEMM_TN);
   const double *pb = pool + (size_t)base * TILE_ELEMS;
+  if (p0 >= p1) return;
+  TileRegs ra, rb;
+  {
+    const TilePair pr = pairs[p0];
+    tile_load(pb + (size_t)pr.a * TILE_ELEMS, ra);
+    tile_load(pb + (size_t)pr.b * TILE_ELEMS, rb);
+  }
   for (int p = p0; p < p1; p++) {
-    TilePair pr = pairs[p];
     __syncthreads();
-    tile_to_lds(pb + (size_t)pr.a * TILE_ELEMS, As, ta);
-    tile_to_lds(pb + (size_t)pr.b * TILE_ELEMS, Bs, tb);
+    tile_store_lds(ra, As, ta);
+    tile_store_lds(rb, Bs, tb);
     __syncthreads();
+    if (p + 1 < p1) {
+      const TilePair pr = pairs[p + 1];
+      tile_load(pb + (size_t)pr.a * TILE_ELEMS, ra);
+      tile_load(pb + (size_t)pr.b * TILE_ELEMS, rb);
+    }
     mfma_accumulate(As, Bs, acc);
   }
 }
@@ -175,62 +212,139 @@ __global__ __launch_bounds__(NTHR) void k_tile_gemm(double *__restrict__ pool,
   lds_to_tile(As, C);
 }
 
+// acc += sum_{k < nk} A(r, k) B'(c, k) for one 16x16 block in the lane layout
+// of the f64 16x16x4 MFMA: lane l supplies A(l%16, q + l/16) and
+// B'(l%16, q + l/16); A(r, k) = pa[k*ska + r*sra], B'(c, k) = pb[k*skb + c*srb]
+__device__ __forceinline__ v4d mfma16(v4d acc, const double *pa, int sra, int ska,
+                                      const double *pb, int srb, int skb, int nk) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  for (int q = 0; q < nk; q += 4) {
+    const double a = pa[(q + lk) * ska + li * sra];
+    const double b = pb[(q + lk) * skb + li * srb];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// Cholesky factor of the 16x16 diagonal block at (c0, c0) by one wave in
+// registers: lane i (mod 16) holds row i, cross-lane values move by
+// shuffles; L goes back to As with its upper part zeroed.
+__device__ __forceinline__ void chol16(double *As, int c0, int &bad) {
+  const int lane = threadIdx.x & 63, i = lane & 15;
+  double a[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) a[c] = As[(c0 + c) * LDL + c0 + i];
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    double d = __shfl(a[j], j, 64);
+    if (!(d > 0.0)) {
+      bad = 1;
+      d = 1.0;
+    }
+    const double sd = sqrt(d), inv = 1.0 / sd;
+    a[j] = (i == j) ? sd : (i > j ? a[j] * inv : a[j]);
+#pragma unroll
+    for (int k = j + 1; k < 16; k++) {
+      const double lkj = __shfl(a[j], k, 64);
+      if (i >= k) a[k] -= a[j] * lkj;
+    }
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int c = 0; c < 16; c++) As[(c0 + c) * LDL + c0 + i] = (c <= i) ? a[c] : 0.0;
+  }
+}
+
+// inv(L) of the factored 16x16 diagonal block into Bs: lane i (< 16) solves
+// for column i by forward substitution, reading L from As (broadcast reads)
+__device__ __forceinline__ void trinv16(const double *As, double *Bs, int c0) {
+  const int i = threadIdx.x & 63;
+  if (i >= 16) return;
+  double x[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    double sacc = (r == i) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < r; k++) sacc -= As[(c0 + k) * LDL + c0 + r] * x[k];
+    x[r] = sacc / As[(c0 + r) * LDL + c0 + r];
+  }
+#pragma unroll
+  for (int r = 0; r < 16; r++) Bs[(c0 + i) * LDL + c0 + r] = x[r];
+}
+
+// C = C - sum A B^T over the task's pairs; C = chol(C) (lower, upper zeroed);
+// X = inv(chol(C)).  Blocked by 16: diagonal blocks in registers (chol16),
+// panel solves, trailing updates and the off-diagonal blocks of the inverse
+// (X_IJ = -inv(L_II) sum_{J<=K<I} L_IK X_KJ) on the f64 MFMA.
 __global__ __launch_bounds__(NTHR) void k_tile_potrf(double *__restrict__ pool,
                                                      const TileTask *__restrict__ tasks,
                                                      const TilePair *__restrict__ pairs,
                                                      int *__restrict__ err) {
   __shared__ double As[64 * LDL];
   __shared__ double Bs[64 * LDL];
+  __shared__ double Ts[4][16 * 17];
   __shared__ int bad;
   const TileTask t = tasks[blockIdx.x];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   v4d acc[2][2];
   zero_acc(acc);
   gemm_sum(pool, pairs, t.p0, t.p1, t.base, GEMM_NT, As, Bs, acc);
   double *C = pool + (size_t)t.c * TILE_ELEMS;
-  form_e(C, 1.0, -1.0, acc, As);
+  form_e(C, 1.0, -1.0, acc, As);  // ends with a barrier
+  for (int e = tid; e < 64 * LDL; e += NTHR) Bs[e] = 0.0;
   if (tid == 0) bad = 0;
   __syncthreads();
-  // right-looking unblocked Cholesky of the lower triangle in LDS
-  for (int j = 0; j < 64; j++) {
-    if (tid == 0) {
-      double d = As[j * LDL + j];
-      if (!(d > 0.0)) {
-        bad = 1;
-        d = 1.0;
-      }
-      As[j * LDL + j] = sqrt(d);
+  const int orow = lane >> 4, ocol = lane & 15;  // accumulator element (orow + 4v, ocol)
+  for (int J = 0; J < 4; J++) {
+    const int c0 = 16 * J;
+    // block column J -= L(:, 0:c0) L(J rows, 0:c0)^T, one row block per wave
+    if (J > 0 && wave < 4 - J) {
+      const int R = J + wave;
+      v4d u = (v4d){0.0, 0.0, 0.0, 0.0};
+      u = mfma16(u, As + 16 * R, 1, LDL, As + c0, 1, LDL, c0);
+#pragma unroll
+      for (int v = 0; v < 4; v++) As[(c0 + ocol) * LDL + 16 * R + orow + 4 * v] -= u[v];
     }
     __syncthreads();
-    double djj = As[j * LDL + j];
-    if (tid > j && tid < 64) As[j * LDL + tid] /= djj;
+    if (wave == 0) {
+      int b = 0;
+      chol16(As, c0, b);
+      if (b && lane == 0) bad = 1;
+    }
     __syncthreads();
-    for (int e = tid; e < 4096; e += NTHR) {
-      int r = e & 63, c = e >> 6;
-      if (c > j && r >= c) As[c * LDL + r] -= As[j * LDL + r] * As[j * LDL + c];
+    if (wave == 0) trinv16(As, Bs, c0);
+    __syncthreads();
+    // panel: L(R, J) = A(R, J) inv(L_JJ)^T
+    if (wave < 3 - J) {
+      const int R = J + 1 + wave;
+      v4d u = (v4d){0.0, 0.0, 0.0, 0.0};
+      u = mfma16(u, As + c0 * LDL + 16 * R, 1, LDL, Bs + c0 * LDL + c0, 1, LDL, 16);
+#pragma unroll
+      for (int v = 0; v < 4; v++) As[(c0 + ocol) * LDL + 16 * R + orow + 4 * v] = u[v];
     }
     __syncthreads();
   }
-  // zero the strict upper triangle
+  // zero the strict upper triangle of L
   for (int e = tid; e < 4096; e += NTHR) {
     int r = e & 63, c = e >> 6;
     if (r < c) As[c * LDL + r] = 0.0;
   }
-  __syncthreads();
-  // inverse of the lower factor by a row sweep over [L | I]: after step k
-  // row k of X = L^{-1} is final and eliminated from the rows below
-  for (int e = tid; e < 4096; e += NTHR) {
-    int r = e & 63, c = e >> 6;
-    Bs[c * LDL + r] = (r == c) ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  for (int k = 0; k < 64; k++) {
-    const double inv = 1.0 / As[k * LDL + k];
-    if (tid <= k) Bs[tid * LDL + k] *= inv;
+  // off-diagonal blocks of inv(L), one block row at a time
+  for (int I = 1; I < 4; I++) {
+    const int Jb = wave;
+    v4d u = (v4d){0.0, 0.0, 0.0, 0.0};
+    if (Jb < I) {
+      for (int K = Jb; K < I; K++)
+        u = mfma16(u, As + (16 * K) * LDL + 16 * I, 1, LDL, Bs + (16 * Jb) * LDL + 16 * K, LDL, 1, 16);
+#pragma unroll
+      for (int v = 0; v < 4; v++) Ts[wave][ocol * 17 + orow + 4 * v] = u[v];
+    }
     __syncthreads();
-    for (int e = tid; e < 4096; e += NTHR) {
-      int i = e & 63, c = e >> 6;
-      if (i > k && c <= k) Bs[c * LDL + i] -= As[k * LDL + i] * Bs[c * LDL + k];
+    if (Jb < I) {
+      u = (v4d){0.0, 0.0, 0.0, 0.0};
+      u = mfma16(u, Bs + (16 * I) * LDL + 16 * I, 1, LDL, Ts[wave], 17, 1, 16);
+#pragma unroll
+      for (int v = 0; v < 4; v++) Bs[(16 * Jb + ocol) * LDL + 16 * I + orow + 4 * v] = -u[v];
     }
     __syncthreads();
   }

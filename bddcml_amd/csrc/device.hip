@@ -1030,8 +1030,7 @@ void device_setup(Solver &S) {
                      POST_RIGHT_XT, 1.0, -1.0);
   }
   HIPCHECK(hipEventRecord(ev[1], st));
-  launch_tile_gemm(st, pool, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask, GEMM_NT,
-                   POST_NONE, 1.0, -1.0);
+  launch_schur_gemm(st, pool, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask);
   HIPCHECK(hipEventRecord(ev[2], st));
   launch_tile_mirror(st, pool, D.mir_s.p, (int)D.mir_s.n);
   // augmented interface matrices

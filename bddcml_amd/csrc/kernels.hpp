@@ -20,6 +20,9 @@ enum GemmPost { POST_NONE = 0, POST_RIGHT_XT = 1, POST_LEFT_X = 2, POST_LEFT_XT 
 
 void launch_tile_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
                       int ntasks, int kind, int post, double alpha, double beta);
+// C = C - sum A B^T as its own kernel symbol (Schur-complement block update)
+void launch_schur_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+                       int ntasks);
 // C = C - sum A B^T ; C = chol(C) (lower, upper zeroed) ; X = inv(C)
 void launch_tile_potrf(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
                        int ntasks, int *err);

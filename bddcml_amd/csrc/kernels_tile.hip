@@ -173,13 +173,11 @@ __device__ __forceinline__ void acc_to_lds(v4d acc[2][2], double *As) {
       }
 }
 
-__global__ __launch_bounds__(NTHR) void k_tile_gemm(double *__restrict__ pool,
-                                                    const TileTask *__restrict__ tasks,
-                                                    const TilePair *__restrict__ pairs, int kind,
-                                                    int post, double alpha, double beta) {
+__device__ __forceinline__ void tile_gemm_task(double *__restrict__ pool, const TileTask t,
+                                               const TilePair *__restrict__ pairs, int kind,
+                                               int post, double alpha, double beta) {
   __shared__ double As[64 * LDL];
   __shared__ double Bs[64 * LDL];
-  const TileTask t = tasks[blockIdx.x];
   v4d acc[2][2];
   zero_acc(acc);
   gemm_sum(pool, pairs, t.p0, t.p1, t.base, kind, As, Bs, acc);
@@ -210,6 +208,21 @@ __global__ __launch_bounds__(NTHR) void k_tile_gemm(double *__restrict__ pool,
   acc_to_lds(acc, As);
   __syncthreads();
   lds_to_tile(As, C);
+}
+
+__global__ __launch_bounds__(NTHR) void k_tile_gemm(double *__restrict__ pool,
+                                                    const TileTask *__restrict__ tasks,
+                                                    const TilePair *__restrict__ pairs, int kind,
+                                                    int post, double alpha, double beta) {
+  tile_gemm_task(pool, tasks[blockIdx.x], pairs, kind, post, alpha, beta);
+}
+
+// the Schur-complement block update S = A22 - sum W^T W in a launch of its
+// own symbol, so that profiles separate it from the factorization steps
+__global__ __launch_bounds__(NTHR) void k_schur_gemm(double *__restrict__ pool,
+                                                     const TileTask *__restrict__ tasks,
+                                                     const TilePair *__restrict__ pairs) {
+  tile_gemm_task(pool, tasks[blockIdx.x], pairs, GEMM_NT, POST_NONE, 1.0, -1.0);
 }
 
 // acc += sum_{k < nk} A(r, k) B'(c, k) for one 16x16 block in the lane layout
@@ -430,6 +443,11 @@ void launch_tile_gemm(hipStream_t st, double *pool, const TileTask *tasks, const
   if (ntasks <= 0) return;
   hipLaunchKernelGGL(k_tile_gemm, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs, kind, post,
                      alpha, beta);
+}
+void launch_schur_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+                       int ntasks) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_schur_gemm, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs);
 }
 void launch_tile_potrf(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
                        int ntasks, int *err) {

@@ -123,7 +123,7 @@ def main():
         f"S GEMV {tm['ev_gemv_ms']:.3f} ms ({gemv_gbs:.0f} GB/s)")
     roofline = dict(bound="mfma", achieved=round(schur_tf, 3), peak=PEAK_FP64_TFLOPS,
                     unit="TFLOP/s", frac=round(schur_tf / PEAK_FP64_TFLOPS, 4), traffic=None,
-                    kernel="k_tile_gemm (Schur-block launch)")
+                    kernel="k_schur_gemm")
     line = {
         "metric": METRIC, "value": value, "unit": "dofs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,

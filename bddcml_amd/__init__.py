@@ -35,6 +35,7 @@ EXTRA_SYMBOLS = [
     "bddcml_amd_residual_history", "bddcml_amd_timings", "bddcml_amd_set_option",
     "bddcml_amd_sub_info", "bddcml_amd_sub_maps", "bddcml_amd_sub_matrix",
     "bddcml_amd_sub_coarse", "bddcml_amd_last_error", "bddcml_amd_selftest_tile",
+    "bddcml_amd_selftest_copy",
 ]
 
 
@@ -50,6 +51,7 @@ def lib():
         dp = C.POINTER(C.c_double)
         L.bddcml_amd_selftest_tile.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, C.c_double,
                                                C.c_double, dp]
+        L.bddcml_amd_selftest_copy.argtypes = [C.c_longlong, dp]
         _lib = L
     return _lib
 
@@ -89,6 +91,16 @@ def selftest_tile(op, A, B, C_, X=None, alpha=1.0, beta=1.0):
                                Cm.ctypes.data_as(dp), Xm.ctypes.data_as(dp), alpha, beta,
                                out.ctypes.data_as(dp))
     return out[:4096].reshape(64, 64).T.copy(), out[4096:].reshape(64, 64).T.copy()
+
+
+def selftest_copy(nbytes):
+    """Time a plain device copy of nbytes (reference point for HBM counters);
+    returns milliseconds."""
+    L = _Checked(lib())
+    L.bddcml_amd_set_option(b"throw_on_error", 1.0)
+    out = np.zeros(1)
+    L.bddcml_amd_selftest_copy(int(nbytes), out.ctypes.data_as(C.POINTER(C.c_double)))
+    return float(out[0])
 
 
 def _ip(v):
@@ -243,12 +255,12 @@ class Bddcml:
         return out
 
     def timings(self):
-        out = np.zeros(18)
-        self.L.bddcml_amd_timings(out.ctypes.data_as(C.POINTER(C.c_double)), 18)
+        out = np.zeros(20)
+        self.L.bddcml_amd_timings(out.ctypes.data_as(C.POINTER(C.c_double)), 20)
         keys = ("topology", "plan", "factor", "krylov", "postprocess", "setup_total",
                 "solve_total", "rhs", "ev_interior_ms", "ev_schur_ms", "ev_interface_ms",
                 "ev_coarse_ms", "schur_flops", "interior_flops", "interface_flops",
-                "ev_gemv_ms", "gemv_bytes", "iterations")
+                "ev_gemv_ms", "gemv_bytes", "iterations", "ev_kapply_ms", "kapply_bytes")
         return dict(zip(keys, out.tolist()))
 
     def sub_info(self, isub):

@@ -457,12 +457,12 @@ catch (const Error &e) {
 }
 int bddcml_amd_timings(double *out, int n) try {
   Solver &S = solver();
-  double v[18] = {S.t.topology,        S.t.plan,           S.t.factor,          S.t.krylov,
+  double v[20] = {S.t.topology,        S.t.plan,           S.t.factor,          S.t.krylov,
                   S.t.postprocess,     S.t.setup_total,    S.t.solve_total,     S.t.rhs,
                   S.t.ev_interior_ms,  S.t.ev_schur_ms,    S.t.ev_interface_ms, S.t.ev_coarse_ms,
                   S.t.schur_flops,     S.t.interior_flops, S.t.interface_flops, S.t.ev_gemv_ms,
-                  S.t.gemv_bytes,      (double)S.last_iters};
-  for (int k = 0; k < std::min(n, 18); k++) out[k] = v[k];
+                  S.t.gemv_bytes,      (double)S.last_iters, S.t.ev_kapply_ms, S.t.kapply_bytes};
+  for (int k = 0; k < std::min(n, 20); k++) out[k] = v[k];
   return 0;
 }
 catch (const Error &e) {
@@ -564,6 +564,16 @@ int bddcml_amd_last_error(char *buf, int len) {
 int bddcml_amd_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
                              const double *X, double alpha, double beta, double *out) try {
   device_selftest_tile(op, npairs, A, B, C, X, alpha, beta, out);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e);
+  return -100;
+}
+
+int bddcml_amd_selftest_copy(long long nbytes, double *out) try {
+  if (nbytes <= 0 || nbytes % 16 != 0) fatal("BDDCML_SELFTEST", "copy size must be a positive multiple of 16");
+  out[0] = device_selftest_copy(nbytes);
   return 0;
 }
 catch (const Error &e) {

@@ -1153,6 +1153,27 @@ void device_selftest_tile(int op, int npairs, const double *A, const double *B, 
                        hipMemcpyDeviceToHost));
 }
 
+double device_selftest_copy(long long nbytes) {
+  DBuf<double2> a, b;
+  const int64_t n = nbytes / 16;
+  a.alloc(n);
+  b.alloc(n);
+  HIPCHECK(hipMemset(a.p, 0, n * sizeof(double2)));
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  launch_copy_calibration(nullptr, a.p, b.p, n);  // warm
+  HIPCHECK(hipEventRecord(e0, nullptr));
+  launch_copy_calibration(nullptr, a.p, b.p, n);
+  HIPCHECK(hipEventRecord(e1, nullptr));
+  HIPCHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms;
+}
+
 void device_sub_matrix(Solver &S, int is, int which, double *out) {
   Device &D = *S.dev;
   const Sub &s = S.subs[is];
@@ -1212,17 +1233,40 @@ static void op_schur(Solver &S, const double *x, double *y, bool timed = false) 
   launch_assemble(D.st, S.nig, D.aptr.p, D.asrc.p, nullptr, D.yl.p, y);
 }
 
-static void op_pc(Solver &S, const double *r, double *z) {
+static void op_pc(Solver &S, const double *r, double *z, bool timed = false) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   // coarse correction uc = Ac^{-1} sum_i R_ci^T Phi_i^T W_i r_i
   launch_phit(st, D.desc.p, S.nsub, D.Phi.p, D.map.p, D.w.p, r, D.rcs.p, D.maxnc);
   launch_coarse_gather(st, S.ndofc, D.cgptr.p, D.cgsrc.p, D.rcs.p, D.rc.p);
   launch_gemv(st, D.pool.p, D.crows.p, D.ncrows, D.cloff.p, D.cNI.p, nullptr, nullptr, D.rc.p,
               D.uc.p, D.NC);
   // zl_i = K_i W_i r_i + Phi_i uc(indrowc), then z = sum_i R_i^T W_i zl_i
+  if (timed) {
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, st));
+  }
   launch_kapply(st, D.pool.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p, D.Z.p, D.cmap.p,
                 D.uc.p, D.zl.p, D.maxNI, D.maxnc);
+  if (timed) {
+    HIPCHECK(hipEventRecord(e1, st));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    S.t.ev_kapply_ms = ms;
+    // packed S' factor read by both sweeps, Z and Phi columns, map (4 B),
+    // weight, gathered input and output (8 B each) per local interface entry
+    double tiles = 0, zb = 0;
+    for (const SubDesc &d : D.hdesc) {
+      tiles += 0.5 * d.nti * (d.nti + 1.0);
+      zb += 2.0 * d.NI * d.nc;
+    }
+    S.t.kapply_bytes = 2.0 * tiles * TILE_ELEMS * 8.0 + 8.0 * zb + 28.0 * (double)D.LV;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
   launch_assemble(st, S.nig, D.aptr.p, D.asrc.p, D.w.p, D.zl.p, z);
 }
 
@@ -1391,7 +1435,7 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, in
         break;
       }
       lastres = relres;
-      op_pc(S, D.r.p, D.z.p);
+      op_pc(S, D.r.p, D.z.p, iter == 2);
       dots(S, {{D.r.p, D.z.p}}, sc);
       double rmpold = rmp;
       rmp = sc[0];

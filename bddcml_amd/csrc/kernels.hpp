@@ -151,5 +151,7 @@ void launch_bicg_p(hipStream_t st, int n, double b, double w, const double *r, c
 void launch_sub_scaled(hipStream_t st, int n, const double *x, double a, const double *y,
                        double *z);
 void launch_mask(hipStream_t st, int n, const char *mask, double *v);
+// dst = src over n double2 elements (HBM copy reference)
+void launch_copy_calibration(hipStream_t st, const double2 *src, double2 *dst, int64_t n);
 
 }  // namespace bddc

@@ -628,4 +628,22 @@ void launch_mask(hipStream_t st, int n, const char *mask, double *v) {
   hipLaunchKernelGGL(k_mask, dim3((n + 255) / 256), dim3(256), 0, st, n, mask, v);
 }
 
+// four double2 per thread, all loads issued before the stores
+__global__ __launch_bounds__(256) void k_copy_calibration(const double2 *__restrict__ src,
+                                                          double2 *__restrict__ dst, int64_t n) {
+  const int64_t i0 = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+  double2 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (i0 + 256 * k < n) v[k] = src[i0 + 256 * k];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (i0 + 256 * k < n) dst[i0 + 256 * k] = v[k];
+}
+void launch_copy_calibration(hipStream_t st, const double2 *src, double2 *dst, int64_t n) {
+  if (n <= 0) return;
+  const int64_t nb = (n + 1023) / 1024;
+  hipLaunchKernelGGL(k_copy_calibration, dim3((unsigned)nb), dim3(256), 0, st, src, dst, n);
+}
+
 }  // namespace bddc

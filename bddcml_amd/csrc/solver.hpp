@@ -21,7 +21,8 @@ struct Timings {
   // device event times (ms) and algorithmic work of the measured launches
   double ev_interior_ms = 0, ev_schur_ms = 0, ev_interface_ms = 0, ev_coarse_ms = 0;
   double schur_flops = 0, interior_flops = 0, interface_flops = 0;
-  double ev_gemv_ms = 0, gemv_bytes = 0;  // one batched S GEMV launch
+  double ev_gemv_ms = 0, gemv_bytes = 0;      // one batched S GEMV launch
+  double ev_kapply_ms = 0, kapply_bytes = 0;  // one local preconditioner launch
   int iterations = 0;
 };
 
@@ -87,6 +88,7 @@ void device_apply_pc(Solver &S, const double *r, double *z);
 void device_reduced_rhs_download(Solver &S, double *g);
 void device_sub_matrix(Solver &S, int is, int which, double *out);
 void device_sub_coarse(Solver &S, int is, double *phi, double *ac);
+double device_selftest_copy(long long nbytes);  // milliseconds
 void device_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
                           const double *X, double alpha, double beta, double *out);
 void device_release(Solver &S);

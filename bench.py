@@ -54,6 +54,24 @@ def cpu_baseline():
                        f"({p.ndof} dofs, {it} iterations, {dt:.2f} s, OpenMP over subdomains)")
 
 
+def schur_traffic(args):
+    """HBM bytes per k_schur_gemm launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of the default workload (null for other
+    workloads or when the summary is absent)."""
+    if (args.nel, args.nsub, args.ordering, args.nd_leaf) != (16, 8, 0, 256):
+        return None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "k_schur_gemm_128cube_pmc.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch k_schur_gemm ="):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +94,14 @@ def main():
 
     import bddcml_amd
     from bddcml_amd.cube import CubeProblem
+
+    # plain HBM copy of known size: the reference for the HBM byte counters
+    # of a profile of this run, and the bandwidth a copy reaches on this GPU
+    copy_bytes = 1 << 30
+    copy_ms = bddcml_amd.selftest_copy(copy_bytes)
+    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+    log(f"k_copy_calibration: {copy_bytes} bytes read + written in {copy_ms:.3f} ms "
+        f"({copy_gbs:.0f} GB/s)")
 
     p = CubeProblem(args.nel, args.nsub, rhs="random", seed=2)
     t0 = time.perf_counter()
@@ -117,12 +143,16 @@ def main():
     schur_tf = tm["schur_flops"] / (tm["ev_schur_ms"] * 1e-3) / 1e12
     gemv_gbs = tm["gemv_bytes"] / (tm["ev_gemv_ms"] * 1e-3) / 1e9 if tm["ev_gemv_ms"] > 0 else 0.0
     interior_tf = tm["interior_flops"] / (tm["ev_interior_ms"] * 1e-3) / 1e12
+    kapply_gbs = tm["kapply_bytes"] / (tm["ev_kapply_ms"] * 1e-3) / 1e9 if tm["ev_kapply_ms"] > 0 else 0.0
+    log(f"local preconditioner apply {tm['ev_kapply_ms']:.3f} ms ({kapply_gbs:.0f} GB/s of "
+        f"{tm['kapply_bytes'] / 1e9:.2f} GB)")
     log(f"interior factorisation {tm['ev_interior_ms']:.1f} ms ({interior_tf:.1f} TFLOP/s), "
         f"Schur block GEMM {tm['ev_schur_ms']:.1f} ms ({schur_tf:.1f} TFLOP/s), "
         f"interface {tm['ev_interface_ms']:.1f} ms, coarse {tm['ev_coarse_ms']:.1f} ms, "
         f"S GEMV {tm['ev_gemv_ms']:.3f} ms ({gemv_gbs:.0f} GB/s)")
     roofline = dict(bound="mfma", achieved=round(schur_tf, 3), peak=PEAK_FP64_TFLOPS,
-                    unit="TFLOP/s", frac=round(schur_tf / PEAK_FP64_TFLOPS, 4), traffic=None,
+                    unit="TFLOP/s", frac=round(schur_tf / PEAK_FP64_TFLOPS, 4),
+                    traffic=schur_traffic(args),
                     kernel="k_schur_gemm")
     line = {
         "metric": METRIC, "value": value, "unit": "dofs/s", "n_gpus": 1,
@@ -133,7 +163,8 @@ def main():
                                f"{args.nel}^3 elements, 2 levels, PCG to {args.tol:g}",
                    "dofs": p.ndof, "subdomains": p.nsub, "iterations": results[-1][0],
                    "condition_number": round(results[-1][1], 4), "precision": "fp64",
-                   "s_gemv_GBps": round(gemv_gbs, 1)},
+                   "s_gemv_GBps": round(gemv_gbs, 1), "hbm_copy_GBps": round(copy_gbs, 1),
+                   "pc_apply_GBps": round(kapply_gbs, 1)},
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:

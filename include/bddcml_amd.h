@@ -28,12 +28,13 @@ int bddcml_amd_reduced_rhs(double *g);
 int bddcml_amd_interface_solution(double *u);
 /* relative residual history of the last solve; returns its length */
 int bddcml_amd_residual_history(double *out, int maxn);
-/* up to 18 values: wall seconds of topology plan factor krylov postprocess
+/* up to 20 values: wall seconds of topology plan factor krylov postprocess
  * setup_total solve_total rhs; device event milliseconds of the interior
  * factorisation, the Schur-block GEMM launch, the interface phase, the
  * coarse phase; algorithmic flops of the Schur-block launch, the interior
  * factorisation and the interface phase; event milliseconds and algorithmic
- * bytes of one batched Schur GEMV; last iteration count */
+ * bytes of one batched Schur GEMV; last iteration count; event milliseconds
+ * and algorithmic bytes of one local preconditioner launch */
 int bddcml_amd_timings(double *out, int n);
 /* options (set after bddcml_init_c, which resets them):
  *   "verbose" (0/1/2);
@@ -60,11 +61,16 @@ int bddcml_amd_sub_coarse(int isub, double *phi, double *ac);
  * clears it */
 int bddcml_amd_last_error(char *buf, int len);
 /* runs one batched tile task on the device on 64x64 column-major tiles:
- * op 0 NT, 1 NN, 2 TN product sums (+10 post C X^T, +20 post X C),
+ * op 0 NT, 1 NN, 2 TN product sums (+10 post C X^T, +20 post X C,
+ * +30 post X^T C),
  * op 3 Cholesky of C - sum A B^T with inverse; out = result tile, and for
  * op 3 the inverse in out[4096..8191] */
 int bddcml_amd_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
                              const double *X, double alpha, double beta, double *out);
+/* plain device copy of nbytes (multiple of 16) through kernel
+ * k_copy_calibration, timed with events: the reference point for HBM
+ * counters and bandwidth; out[0] = milliseconds of the copy */
+int bddcml_amd_selftest_copy(long long nbytes, double *out);
 
 #ifdef __cplusplus
 }

@@ -218,34 +218,41 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
     S.use_arith = *use_arithmetic_constraints_int != 0;
     S.weights_type = *weights_type;
   }
-  {
-    double tp = wall_time();
-    auto phase = [&](const char *name) {
-      double now = wall_time();
-      if (S.verbose >= 2) std::printf(" BDDCML_SETUP_PRECONDITIONER: %-22s %8.4f s\n", name, now - tp);
-      tp = now;
-    };
-    build_topology(S);
-    phase("topology");
-    apply_boundary_conditions(S);
-    phase("boundary conditions");
-    build_weights(S);
-    phase("weights");
-    build_constraints(S);
-    phase("constraints");
-    build_condensation_rhs(S);
-    phase("condensation rhs");
-    build_unique_interface(S);
-    phase("unique interface");
-    build_orderings(S);
-    phase("orderings");
-  }
+  double tp = wall_time();
+  auto phase = [&](const char *name) {
+    double now = wall_time();
+    if (S.verbose >= 2) std::printf(" BDDCML_SETUP_PRECONDITIONER: %-22s %8.4f s\n", name, now - tp);
+    tp = now;
+  };
+  // what the interior factorization needs: interface split, constraint
+  // counts (tile layout), orderings
+  build_topology(S);
+  phase("topology");
+  build_constraints(S);
+  phase("constraints");
+  build_orderings(S);
+  phase("orderings");
   S.t.topology += wall_time() - t0;
+  if (!S.plan_only) {
+    device_setup_begin(S);  // returns with the interior factorization running
+    tp = wall_time();
+  }
+  // host work that overlaps the device factorization
+  double th = wall_time();
+  apply_boundary_conditions(S);
+  phase("boundary conditions");
+  build_weights(S);
+  phase("weights");
+  build_condensation_rhs(S);
+  phase("condensation rhs");
+  build_unique_interface(S);
+  phase("unique interface");
+  S.t.topology += wall_time() - th;
   if (S.plan_only) {
     S.is_setup = true;
     return;
   }
-  device_setup(S);
+  device_setup_finish(S);
   double t1 = wall_time();
   device_reduced_rhs(S);
   S.t.rhs += wall_time() - t1;

@@ -126,7 +126,10 @@ int TilePattern::find(int I, int J) const {
 }
 
 struct Device {
-  hipStream_t st = nullptr;
+  hipStream_t st = nullptr;  // compute stream
+  hipStream_t up = nullptr;  // uploads of the second setup half
+  hipEvent_t ev[5] = {};     // setup phase boundaries (timing)
+  hipEvent_t ev_up = nullptr;
   DBuf<double> pool;
   int64_t ntiles = 0;
   DBuf<int> err;
@@ -210,8 +213,15 @@ void device_release_input(Solver &S) { S.din.reset(); }
 
 void device_release(Solver &S) {
   if (S.dev) {
-    if (S.dev->st) (void)hipStreamDestroy(S.dev->st);
-    S.dev->st = nullptr;
+    Device &D = *S.dev;
+    if (D.st) (void)hipStreamSynchronize(D.st);
+    if (D.up) (void)hipStreamSynchronize(D.up);
+    for (hipEvent_t &e : D.ev)
+      if (e) (void)hipEventDestroy(e), e = nullptr;
+    if (D.ev_up) (void)hipEventDestroy(D.ev_up), D.ev_up = nullptr;
+    if (D.st) (void)hipStreamDestroy(D.st);
+    if (D.up) (void)hipStreamDestroy(D.up);
+    D.st = D.up = nullptr;
     S.dev.reset();
   }
 }
@@ -326,11 +336,19 @@ static void upload_phases(std::vector<Phase> &dst, const std::vector<TaskList> &
 }
 
 // ------------------------------------------------------------------ setup
-void device_setup(Solver &S) {
+// Setup runs in two halves.  device_setup_begin needs only the interface
+// split, the constraint counts and the orderings: it builds the tile layout
+// and the task lists and launches the interior factorization and the Schur
+// complements without waiting for them.  The host then prepares weights,
+// Dirichlet data and the condensation right-hand side while the device
+// factors; device_setup_finish builds the interface and coarse maps
+// (uploaded on a second stream) and launches the rest.
+void device_setup_begin(Solver &S) {
   device_release(S);
   S.dev.reset(new Device());
   Device &D = *S.dev;
   HIPCHECK(hipStreamCreateWithFlags(&D.st, hipStreamNonBlocking));
+  HIPCHECK(hipStreamCreateWithFlags(&D.up, hipStreamNonBlocking));
   hipStream_t st = D.st;
   const int nsub = S.nsub;
   double t0 = wall_time();
@@ -727,6 +745,67 @@ void device_setup(Solver &S) {
   D.n_sp = (int)spsub.size();
 
   ph("mirror and update jobs");
+  D.pool.alloc((size_t)D.ntiles * TILE_ELEMS);
+  D.err.alloc(1);
+  D.err.zero(st);
+  ph("pool allocation");
+  if (S.verbose >= 2)
+    std::printf("   device plan: %lld tiles (%.1f GB), %zu scatter entries\n", (long long)D.ntiles,
+                D.ntiles * 32768.0 / 1e9, (size_t)D.sc_idx.n);
+
+  // algorithmic work: 2*64^3 flops per tile product, 64^3/3 per tile Cholesky
+  // plus 64^3/3 for its inverse, 2*64^3 per post multiply
+  {
+    const double g = 2.0 * 64 * 64 * 64, c = 64.0 * 64 * 64 / 3.0;
+    // pair lists are shared between subdomains: count the pairs of each task
+    auto products = [](const TaskList &l) {
+      double n = 0;
+      for (const TileTask &t : l.tasks) n += t.p1 - t.p0;
+      return n;
+    };
+    auto work = [&](const std::vector<TaskList> &v, double per_task) {
+      double f = 0;
+      for (auto &l : v) f += g * products(l) + per_task * l.tasks.size();
+      return f;
+    };
+    S.t.schur_flops = g * products(a_schur);
+    S.t.interior_flops = work(a_diag, 2 * c) + work(a_off, g) + S.t.schur_flops;
+    S.t.interface_flops = work(p_diag, 2 * c) + work(p_off, g) + work(z_fwd, g) + work(z_bwd, g);
+  }
+  // ---- numerics, first half: Dirichlet elimination into the tiles,
+  // interior factorization, Schur complements
+  double *pool = D.pool.p;
+  hipEvent_t *ev = D.ev;
+  for (int k = 0; k < 5; k++) HIPCHECK(hipEventCreate(&ev[k]));
+  HIPCHECK(hipEventCreateWithFlags(&D.ev_up, hipEventDisableTiming));
+  HIPCHECK(hipEventRecord(ev[0], st));
+  launch_zero_ranges(st, pool, D.zero_rng.p, (int)D.zero_rng.n);
+  launch_scatter_sub(st, pool, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p, D.err.p);
+  launch_scatter(st, pool, D.sc_idx.p, D.sc_val.p, (int64_t)D.sc_idx.n);
+  for (size_t J = 0; J < D.a_diag.size(); J++) {
+    launch_tile_potrf(st, pool, D.a_diag[J].tasks.p, D.a_diag[J].pairs.p, D.a_diag[J].ntask, D.err.p);
+    launch_tile_gemm(st, pool, D.a_off[J].tasks.p, D.a_off[J].pairs.p, D.a_off[J].ntask, GEMM_NT,
+                     POST_RIGHT_XT, 1.0, -1.0);
+  }
+  HIPCHECK(hipEventRecord(ev[1], st));
+  launch_schur_gemm(st, pool, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask);
+  HIPCHECK(hipEventRecord(ev[2], st));
+  launch_tile_mirror(st, pool, D.mir_s.p, (int)D.mir_s.n);
+  HIPCHECK(hipGetLastError());
+  S.t.plan += wall_time() - t0;
+}
+
+void device_setup_finish(Solver &S) {
+  Device &D = *S.dev;
+  hipStream_t st = D.up;  // uploads overlap the factorization running on D.st
+  const int nsub = S.nsub;
+  double t0 = wall_time();
+  double tph = t0;
+  auto ph = [&](const char *name) {
+    double now = wall_time();
+    if (S.verbose >= 2) std::printf("   device plan: %-24s %8.4f s\n", name, now - tph);
+    tph = now;
+  };
   // ---- interface descriptors and maps: offsets first, then every
   // subdomain fills its own ranges in parallel
   D.hdesc.resize(nsub);
@@ -982,58 +1061,17 @@ void device_setup(Solver &S) {
     b->alloc(std::max(S.nig, 1));
   D.vec.alloc(D.VT);
   D.fixmask.upload(std::vector<char>(S.ig_fixed.begin(), S.ig_fixed.end()), st);
-  D.err.alloc(1);
-  D.err.zero(st);
   D.rc.zero(st);
   D.uc.zero(st);
   ph("coarse lists, vectors");
-  D.pool.alloc((size_t)D.ntiles * TILE_ELEMS);
-  ph("pool allocation");
-  HIPCHECK(hipStreamSynchronize(st));
-  ph("upload sync");
-  if (S.verbose >= 2)
-    std::printf("   device plan: %lld tiles (%.1f GB), %zu scatter entries\n", (long long)D.ntiles,
-                D.ntiles * 32768.0 / 1e9, (size_t)D.sc_idx.n);
+  HIPCHECK(hipEventRecord(D.ev_up, D.up));
   S.t.plan += wall_time() - t0;
-
-  // algorithmic work: 2*64^3 flops per tile product, 64^3/3 per tile Cholesky
-  // plus 64^3/3 for its inverse, 2*64^3 per post multiply
-  {
-    const double g = 2.0 * 64 * 64 * 64, c = 64.0 * 64 * 64 / 3.0;
-    // pair lists are shared between subdomains: count the pairs of each task
-    auto products = [](const TaskList &l) {
-      double n = 0;
-      for (const TileTask &t : l.tasks) n += t.p1 - t.p0;
-      return n;
-    };
-    auto work = [&](const std::vector<TaskList> &v, double per_task) {
-      double f = 0;
-      for (auto &l : v) f += g * products(l) + per_task * l.tasks.size();
-      return f;
-    };
-    S.t.schur_flops = g * products(a_schur);
-    S.t.interior_flops = work(a_diag, 2 * c) + work(a_off, g) + S.t.schur_flops;
-    S.t.interface_flops = work(p_diag, 2 * c) + work(p_off, g) + work(z_fwd, g) + work(z_bwd, g);
-  }
-  // ---- numerics
+  // ---- numerics, second half: augmented interface matrices, Z, Phi, coarse
   t0 = wall_time();
+  st = D.st;
+  HIPCHECK(hipStreamWaitEvent(st, D.ev_up, 0));
   double *pool = D.pool.p;
-  hipEvent_t ev[6];
-  for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
-  HIPCHECK(hipEventRecord(ev[0], st));
-  launch_zero_ranges(st, pool, D.zero_rng.p, (int)D.zero_rng.n);
-  launch_scatter_sub(st, pool, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p, D.err.p);
-  launch_scatter(st, pool, D.sc_idx.p, D.sc_val.p, (int64_t)D.sc_idx.n);
-  for (size_t J = 0; J < D.a_diag.size(); J++) {
-    launch_tile_potrf(st, pool, D.a_diag[J].tasks.p, D.a_diag[J].pairs.p, D.a_diag[J].ntask, D.err.p);
-    launch_tile_gemm(st, pool, D.a_off[J].tasks.p, D.a_off[J].pairs.p, D.a_off[J].ntask, GEMM_NT,
-                     POST_RIGHT_XT, 1.0, -1.0);
-  }
-  HIPCHECK(hipEventRecord(ev[1], st));
-  launch_schur_gemm(st, pool, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask);
-  HIPCHECK(hipEventRecord(ev[2], st));
-  launch_tile_mirror(st, pool, D.mir_s.p, (int)D.mir_s.n);
-  // augmented interface matrices
+  hipEvent_t *ev = D.ev;
   launch_sprime(st, pool, D.desc.p, D.sp_sub.p, D.sp_ij.p, D.sp_dst.p, D.n_sp, D.cid.p, D.csize.p);
   for (size_t J = 0; J < D.p_diag.size(); J++) {
     launch_tile_potrf(st, pool, D.p_diag[J].tasks.p, D.p_diag[J].pairs.p, D.p_diag[J].ntask, D.err.p);
@@ -1079,7 +1117,6 @@ void device_setup(Solver &S) {
     S.t.ev_interface_ms = ms;
     HIPCHECK(hipEventElapsedTime(&ms, ev[3], ev[4]));
     S.t.ev_coarse_ms = ms;
-    for (auto &e : ev) (void)hipEventDestroy(e);
   }
   int herr = 0;
   HIPCHECK(hipMemcpy(&herr, D.err.p, sizeof(int), hipMemcpyDeviceToHost));

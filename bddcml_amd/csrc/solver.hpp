@@ -79,7 +79,11 @@ void build_orderings(Solver &S);  // interior/interface device orderings
 // device steps (device.hip)
 void device_upload_entries(Solver &S, int is);
 void device_release_input(Solver &S);
-void device_setup(Solver &S);
+// setup in two halves: begin builds the tile plan and launches the interior
+// factorization; finish (after weights, Dirichlet data, unique interface)
+// builds the interface/coarse maps, launches the rest and waits
+void device_setup_begin(Solver &S);
+void device_setup_finish(Solver &S);
 void device_reduced_rhs(Solver &S);
 void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, int *num_iter,
                   int *reason, double *cond);

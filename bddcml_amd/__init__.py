@@ -165,6 +165,29 @@ class Bddcml:
             D(dof_data if len(dof_data) else empty), _ip(len(dof_data)),
             _ip(d["find_components"]), _ip(d["use_dual_mesh_graph"]), _ip(d["neighbouring"]))
 
+    def change_subdomain_data(self, d: dict):
+        """New Dirichlet values, right-hand side and initial guess of one
+        subdomain (same fixed dofs), as bddcml_change_subdomain_data_c."""
+        keep = []
+
+        def I(x):
+            a, p = _iarr(x)
+            keep.append(a)
+            return p
+
+        def D(x):
+            a, p = _darr(x)
+            keep.append(a)
+            return p
+
+        n = len(d["ifix"])
+        self.L.bddcml_change_subdomain_data_c(
+            _ip(d["isub"]), I(d["ifix"]), _ip(n), D(d["fixv"]), _ip(n), D(d["rhs"]), _ip(n),
+            _ip(d["is_rhs_complete"]), D(d["sol"]), _ip(n))
+
+    def setup_new_data(self):
+        self.L.bddcml_setup_new_data_c()
+
     def setup_preconditioner(self, matrixtype=1, use_defaults=0, parallel_division=1,
                              use_corner_constraints=1, use_arithmetic_constraints=1,
                              use_adaptive_constraints=0, use_user_constraints=0, weights_type=0):

@@ -365,8 +365,20 @@ catch (const Error &e) {
   record_error(e);
 }
 
+// reference src/module_levels.f90:852-903: new eliminated Dirichlet values
+// and reduced right-hand side; the factorizations are kept
 void bddcml_setup_new_data_c() try {
-  fatal("LEVELS_SETUP_NEW_DATA", "re-solving with new data is not supported yet");
+  const char *R = "LEVELS_SETUP_NEW_DATA";
+  Solver &S = solver();
+  if (!S.is_setup) fatal(R, "Cannot change data at this moment, level is not ready.");
+  double t0 = wall_time();
+  refresh_boundary_conditions(S);
+  build_condensation_rhs(S);
+  if (!S.plan_only) {
+    device_upload_rhs(S);
+    device_reduced_rhs(S);
+  }
+  S.t.rhs += wall_time() - t0;
 }
 catch (const Error &e) {
   record_error(e);

@@ -174,6 +174,12 @@ struct Device {
   DBuf<double> vec;
   int64_t VT = 0;
   DBuf<int64_t> rhs_src;
+  // local dof order <-> device order of every subdomain
+  std::vector<OrderDesc> hodesc;
+  DBuf<OrderDesc> odesc;
+  int maxndofs = 0;
+  int64_t ndofs_total = 0;
+  DBuf<double> fmod, sol;
   // vectors
   DBuf<double> yl, zl, rcs, rc, uc, partial, dots;
   DBuf<double> u, r, p, z, ap, rs, v, y, s, t, g, xin, xout;
@@ -934,6 +940,17 @@ void device_setup_finish(Solver &S) {
     int64_t vt = 0;
     for (int is = 0; is < nsub; is++) voff[is] = vt, vt += S.subs[is].NO + S.subs[is].NI;
     D.VT = vt;
+    D.hodesc.resize(nsub);
+    {
+      int64_t po = 0;
+      for (int is = 0; is < nsub; is++) {
+        D.hodesc[is] = OrderDesc{po, voff[is], S.subs[is].ndofs, 0};
+        po += S.subs[is].ndofs;
+        D.maxndofs = std::max(D.maxndofs, S.subs[is].ndofs);
+      }
+      D.ndofs_total = po;
+    }
+    D.odesc.upload(D.hodesc, st);
     std::vector<int64_t> vpos(lv);
 #pragma omp parallel for schedule(static)
     for (int is = 0; is < nsub; is++)
@@ -1063,6 +1080,14 @@ void device_setup_finish(Solver &S) {
   D.fixmask.upload(std::vector<char>(S.ig_fixed.begin(), S.ig_fixed.end()), st);
   D.rc.zero(st);
   D.uc.zero(st);
+  {
+    std::vector<double> fm(D.ndofs_total);
+#pragma omp parallel for schedule(static)
+    for (int is = 0; is < nsub; is++)
+      std::copy(S.subs[is].fmod.begin(), S.subs[is].fmod.end(), fm.begin() + D.hodesc[is].poff);
+    D.fmod.upload(fm, st);
+    D.sol.alloc(D.ndofs_total);
+  }
   ph("coarse lists, vectors");
   HIPCHECK(hipEventRecord(D.ev_up, D.up));
   S.t.plan += wall_time() - t0;
@@ -1130,16 +1155,25 @@ void device_setup_finish(Solver &S) {
 void device_reduced_rhs(Solver &S) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
-  std::vector<double> hv(D.VT, 0.0);
-  for (int is = 0; is < S.nsub; is++) {
-    const Sub &s = S.subs[is];
-    int64_t vo = D.htrsv[is].voff;
-    for (int d = 0; d < s.ndofs; d++) hv[vo + s.perm[d]] = s.fmod[d];
-  }
-  HIPCHECK(hipMemcpyAsync(D.vec.p, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  // condensation rhs into the device-ordered subdomain vectors (padding 0),
+  // forward sweep over the interior factor, sum of the interface parts
+  HIPCHECK(hipMemsetAsync(D.vec.p, 0, D.VT * sizeof(double), st));
+  launch_to_device_order(st, D.odesc.p, S.nsub, D.sc_perm.p, D.fmod.p, D.vec.p, D.maxndofs);
   launch_trsv_fwd(st, D.pool.p, D.trsv.p, S.nsub, D.rptr.p, D.rk.p, D.rt.p, D.vec.p, D.maxnt);
   launch_assemble(st, S.nig, D.aptr.p, D.rhs_src.p, nullptr, D.vec.p, D.g.p);
   HIPCHECK(hipStreamSynchronize(st));
+}
+
+// new condensation rhs of every subdomain (after new right-hand sides or
+// Dirichlet values) uploaded for the next device_reduced_rhs
+void device_upload_rhs(Solver &S) {
+  Device &D = *S.dev;
+  std::vector<double> fm(D.ndofs_total);
+#pragma omp parallel for schedule(static)
+  for (int is = 0; is < S.nsub; is++)
+    std::copy(S.subs[is].fmod.begin(), S.subs[is].fmod.end(), fm.begin() + D.hodesc[is].poff);
+  HIPCHECK(hipMemcpyAsync(D.fmod.p, fm.data(), fm.size() * sizeof(double), hipMemcpyHostToDevice, D.st));
+  HIPCHECK(hipStreamSynchronize(D.st));
 }
 
 void device_reduced_rhs_download(Solver &S, double *g) {
@@ -1369,30 +1403,21 @@ static double tridiag_max_eig(const std::vector<double> &d, const std::vector<do
 static void postprocess(Solver &S) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
-  // interface part of the device-ordered vectors <- u
-  std::vector<double> hu(S.nig);
-  HIPCHECK(hipMemcpyAsync(hu.data(), D.u.p, S.nig * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHECK(hipStreamSynchronize(st));
-  std::vector<double> hv(D.VT);
-  HIPCHECK(hipMemcpy(hv.data(), D.vec.p, D.VT * sizeof(double), hipMemcpyDeviceToHost));
-  for (int is = 0; is < S.nsub; is++) {
-    const Sub &s = S.subs[is];
-    int64_t vo = D.htrsv[is].voff;
-    for (int k = 0; k < s.ni; k++) {
-      int kr = s.dev_ifc[k];
-      hv[vo + s.NO + k] = hu[S.gpos[s.isvgvn[s.ifc_dof[kr]]]];
-    }
-  }
-  HIPCHECK(hipMemcpyAsync(D.vec.p, hv.data(), D.VT * sizeof(double), hipMemcpyHostToDevice, st));
+  // interface values of every subdomain copy <- u, backward sweep for the
+  // interior (u_o = A11^{-1}(f_o - A12 u_G)), back to local dof order
+  launch_ifc_scatter(st, S.nig, D.aptr.p, D.rhs_src.p, D.u.p, D.vec.p);
   launch_trsv_bwd(st, D.pool.p, D.trsv.p, S.nsub, D.tcptr.p, D.tck.p, D.tct.p, D.vec.p, D.maxnt);
-  HIPCHECK(hipMemcpyAsync(hv.data(), D.vec.p, D.VT * sizeof(double), hipMemcpyDeviceToHost, st));
+  launch_from_device_order(st, D.odesc.p, S.nsub, D.sc_perm.p, D.vec.p, D.sol.p, D.maxndofs);
+  std::vector<double> hs(D.ndofs_total);
+  S.uig.resize(S.nig);
+  HIPCHECK(hipMemcpyAsync(S.uig.data(), D.u.p, S.nig * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(hs.data(), D.sol.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
+#pragma omp parallel for schedule(static)
   for (int is = 0; is < S.nsub; is++) {
     Sub &s = S.subs[is];
-    int64_t vo = D.htrsv[is].voff;
-    for (int d = 0; d < s.ndofs; d++) s.sol[d] = hv[vo + s.perm[d]];
+    s.sol.assign(hs.begin() + D.hodesc[is].poff, hs.begin() + D.hodesc[is].poff + s.ndofs);
   }
-  S.uig = hu;
 }
 
 static void initial_guess(Solver &S) {

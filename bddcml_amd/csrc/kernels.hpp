@@ -102,6 +102,22 @@ void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int 
 // largest dynamic LDS allocation a kernel may request on the current device
 size_t max_dynamic_lds();
 
+// ---- subdomain vectors between local dof order and device order ---------
+struct OrderDesc {
+  int64_t poff;  // offset of the subdomain's dofs in perm and in local-order arrays
+  int64_t voff;  // offset of its device-ordered vector
+  int ndofs, pad;
+};
+// vec[voff + perm[poff + i]] = src[poff + i]
+void launch_to_device_order(hipStream_t st, const OrderDesc *d, int nsub, const int *perm,
+                            const double *src, double *vec, int maxndofs);
+// dst[poff + i] = vec[voff + perm[poff + i]]
+void launch_from_device_order(hipStream_t st, const OrderDesc *d, int nsub, const int *perm,
+                              const double *vec, double *dst, int maxndofs);
+// every local copy of unique interface dof g: vec[pos[q]] = u[g], q in ptr[g]..ptr[g+1]
+void launch_ifc_scatter(hipStream_t st, int n, const int *ptr, const int64_t *pos, const double *u,
+                        double *vec);
+
 // ---- iteration kernels ---------------------------------------------------
 struct GemvRow {
   int tile0;      // first tile of the row (row-major full storage)

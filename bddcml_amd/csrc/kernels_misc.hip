@@ -628,6 +628,44 @@ void launch_mask(hipStream_t st, int n, const char *mask, double *v) {
   hipLaunchKernelGGL(k_mask, dim3((n + 255) / 256), dim3(256), 0, st, n, mask, v);
 }
 
+// ------------------------------------------- local order <-> device order
+__global__ void k_to_device_order(const OrderDesc *__restrict__ ds, const int *__restrict__ perm,
+                                  const double *__restrict__ src, double *__restrict__ vec) {
+  const OrderDesc d = ds[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.ndofs) vec[d.voff + perm[d.poff + i]] = src[d.poff + i];
+}
+__global__ void k_from_device_order(const OrderDesc *__restrict__ ds, const int *__restrict__ perm,
+                                    const double *__restrict__ vec, double *__restrict__ dst) {
+  const OrderDesc d = ds[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.ndofs) dst[d.poff + i] = vec[d.voff + perm[d.poff + i]];
+}
+__global__ void k_ifc_scatter(int n, const int *__restrict__ ptr, const int64_t *__restrict__ pos,
+                              const double *__restrict__ u, double *__restrict__ vec) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const double v = u[g];
+  for (int q = ptr[g]; q < ptr[g + 1]; q++) vec[pos[q]] = v;
+}
+void launch_to_device_order(hipStream_t st, const OrderDesc *d, int nsub, const int *perm,
+                            const double *src, double *vec, int maxndofs) {
+  if (nsub <= 0 || maxndofs <= 0) return;
+  hipLaunchKernelGGL(k_to_device_order, dim3((maxndofs + 255) / 256, nsub), dim3(256), 0, st, d,
+                     perm, src, vec);
+}
+void launch_from_device_order(hipStream_t st, const OrderDesc *d, int nsub, const int *perm,
+                              const double *vec, double *dst, int maxndofs) {
+  if (nsub <= 0 || maxndofs <= 0) return;
+  hipLaunchKernelGGL(k_from_device_order, dim3((maxndofs + 255) / 256, nsub), dim3(256), 0, st, d,
+                     perm, vec, dst);
+}
+void launch_ifc_scatter(hipStream_t st, int n, const int *ptr, const int64_t *pos, const double *u,
+                        double *vec) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_ifc_scatter, dim3((n + 255) / 256), dim3(256), 0, st, n, ptr, pos, u, vec);
+}
+
 // four double2 per thread, all loads issued before the stores
 __global__ __launch_bounds__(256) void k_copy_calibration(const double2 *__restrict__ src,
                                                           double2 *__restrict__ dst, int64_t n) {

@@ -378,31 +378,41 @@ static void exchange_sum(Solver &S, std::vector<std::vector<double>> &v) {
   }
 }
 
-static void eliminate_bc(Sub &s) {
+// bc = eliminated Dirichlet values from the stored couplings: -A(i, fixed) g
+// at free dofs, a_ii g_i (diagonal kept, 1 if tiny) at fixed dofs
+static void bc_from_stored(Sub &s) {
   s.bc.assign(s.ndofs, 0.0);
-  if (!s.has_fix) return;
-  std::vector<char> diag_seen(s.ndofs, 0);
-  size_t nnz = s.av.size();
-  for (size_t k = 0; k < nnz; k++) {
-    int i = s.ai[k], j = s.aj[k];
-    double a = s.av[k];
+  for (size_t q = 0; q < s.bc_k.size(); q++) {
+    const size_t k = s.bc_k[q];
+    const int i = s.ai[k], j = s.aj[k];
+    double a = s.bc_a[q];
     if (i != j) {
-      if (s.ifix[i] && s.fixv[i] != 0.0 && a != 0.0 && !s.ifix[j]) s.bc[j] -= a * s.fixv[i];
-      if (s.ifix[j] && s.fixv[j] != 0.0 && a != 0.0 && !s.ifix[i]) s.bc[i] -= a * s.fixv[j];
-      if (s.ifix[i] || s.ifix[j]) s.av[k] = 0.0;
-    } else if (s.ifix[i]) {
-      if (std::fabs(a) < 1e-12) s.av[k] = a = 1.0;
-      if (s.fixv[i] != 0.0 && a != 0.0) s.bc[i] += a * s.fixv[i];
-      diag_seen[i] = 1;
+      if (s.ifix[i] && !s.ifix[j]) s.bc[j] -= a * s.fixv[i];
+      if (s.ifix[j] && !s.ifix[i]) s.bc[i] -= a * s.fixv[j];
+    } else {
+      if (std::fabs(a) < 1e-12) a = 1.0;
+      s.bc[i] += a * s.fixv[i];
     }
   }
+}
+
+// Dirichlet elimination keeping the diagonal (src/module_sm.f90:431-600):
+// couplings to fixed dofs are zeroed after their values are stored
+static void eliminate_bc(Sub &s) {
+  s.bc.assign(s.ndofs, 0.0);
+  s.bc_k.clear();
+  s.bc_a.clear();
+  if (!s.has_fix) return;
+  // every fixed dof needs a diagonal entry
+  std::vector<char> diag_seen(s.ndofs, 0);
+  for (size_t k = 0; k < s.av.size(); k++)
+    if (s.ai[k] == s.aj[k]) diag_seen[s.ai[k]] = 1;
   bool added = false;
   for (int i = 0; i < s.ndofs; i++)
     if (s.ifix[i] && !diag_seen[i]) {
       s.ai.push_back(i);
       s.aj.push_back(i);
       s.av.push_back(1.0);
-      s.bc[i] += s.fixv[i];
       added = true;
     }
   if (added) {
@@ -418,6 +428,38 @@ static void eliminate_bc(Sub &s) {
     s.aj.swap(aj);
     s.av.swap(av);
   }
+  for (size_t k = 0; k < s.av.size(); k++) {
+    const int i = s.ai[k], j = s.aj[k];
+    if (!s.ifix[i] && !s.ifix[j]) continue;
+    s.bc_k.push_back(k);
+    s.bc_a.push_back(s.av[k]);
+    if (i != j) s.av[k] = 0.0;
+    else if (std::fabs(s.av[k]) < 1e-12) s.av[k] = 1.0;
+  }
+  bc_from_stored(s);
+}
+
+// sum of the eliminated values over the subdomains sharing an interface dof
+static void exchange_bc(Solver &S) {
+  std::vector<std::vector<double>> b(S.nsub);
+  for (int is = 0; is < S.nsub; is++) {
+    Sub &s = S.subs[is];
+    b[is].resize(s.ifc_dof.size());
+    for (size_t k = 0; k < s.ifc_dof.size(); k++) b[is][k] = s.bc[s.ifc_dof[k]];
+  }
+  exchange_sum(S, b);
+  for (int is = 0; is < S.nsub; is++) {
+    Sub &s = S.subs[is];
+    for (size_t k = 0; k < s.ifc_dof.size(); k++) s.bc[s.ifc_dof[k]] = b[is][k];
+  }
+}
+
+// new Dirichlet values after bddcml_change_subdomain_data (the reference
+// recomputes bc from its stored fixed-rows matrix, module_dd.f90:1081-1256)
+void refresh_boundary_conditions(Solver &S) {
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++) bc_from_stored(S.subs[is]);
+  if (S.bc_global) exchange_bc(S);
 }
 
 void apply_boundary_conditions(Solver &S) {
@@ -425,19 +467,7 @@ void apply_boundary_conditions(Solver &S) {
   for (auto &s : S.subs) S.bc_global = S.bc_global || s.has_fix;
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < S.nsub; is++) eliminate_bc(S.subs[is]);
-  if (S.bc_global) {
-    std::vector<std::vector<double>> b(S.nsub);
-    for (int is = 0; is < S.nsub; is++) {
-      Sub &s = S.subs[is];
-      b[is].resize(s.ifc_dof.size());
-      for (size_t k = 0; k < s.ifc_dof.size(); k++) b[is][k] = s.bc[s.ifc_dof[k]];
-    }
-    exchange_sum(S, b);
-    for (int is = 0; is < S.nsub; is++) {
-      Sub &s = S.subs[is];
-      for (size_t k = 0; k < s.ifc_dof.size(); k++) s.bc[s.ifc_dof[k]] = b[is][k];
-    }
-  }
+  if (S.bc_global) exchange_bc(S);
 }
 
 void build_weights(Solver &S) {

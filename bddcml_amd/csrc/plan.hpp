@@ -71,6 +71,10 @@ struct Sub {
   std::vector<int> indrowc;     // global coarse dof of each row
 
   std::vector<double> bc;   // eliminated Dirichlet values, ndofs
+  // triplet entries touching fixed dofs with their values before the
+  // elimination, kept to eliminate new Dirichlet values later
+  std::vector<size_t> bc_k;
+  std::vector<double> bc_a;
   std::vector<double> w;    // interface weights (reference order)
   std::vector<double> fmod; // right-hand side used for condensation, ndofs
 

@@ -70,6 +70,7 @@ void reset_solver();
 // host-side setup steps (plan.cpp)
 void build_topology(Solver &S);
 void apply_boundary_conditions(Solver &S);
+void refresh_boundary_conditions(Solver &S);  // new fixv, same fixed dofs
 void build_weights(Solver &S);
 void build_constraints(Solver &S);
 void build_condensation_rhs(Solver &S, int isub_only = -1);
@@ -85,6 +86,7 @@ void device_release_input(Solver &S);
 void device_setup_begin(Solver &S);
 void device_setup_finish(Solver &S);
 void device_reduced_rhs(Solver &S);
+void device_upload_rhs(Solver &S);  // re-upload the condensation rhs of all subdomains
 void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, int *num_iter,
                   int *reason, double *cond);
 void device_apply_schur(Solver &S, const double *x, double *y);

@@ -99,11 +99,14 @@ class CubeProblem:
           'random' -> uniform[-1,1] per global dof from splitmix64(seed ^ dof).
     coef: optional callable mapping global element index array -> per element
           coefficient multiplying the element matrix.
+    dirichlet: 'zero' (homogeneous, as the reference example) or 'random'
+          -> fixed values uniform[-1,1] per global dof from
+          splitmix64((seed + 1000) ^ dof).
     """
 
     def __init__(self, nel_sub: int, nsub_edge: int, kind: str = "poisson",
                  rhs: str = "const", seed: int = 2, numbase: int = 1,
-                 coef=None, E: float = 1.0, nu: float = 0.3):
+                 coef=None, E: float = 1.0, nu: float = 0.3, dirichlet: str = "zero"):
         self.nel_sub = nel_sub
         self.nsub_edge = nsub_edge
         self.kind = kind
@@ -111,6 +114,7 @@ class CubeProblem:
         self.seed = seed
         self.numbase = numbase
         self.coef = coef
+        self.dirichlet = dirichlet
         self.ndim = 3
         self.meshdim = 3
         self.ndf = 1 if kind == "poisson" else 3
@@ -158,8 +162,10 @@ class CubeProblem:
         nb = self.nel
         onb = (ig == 0) | (ig == nb) | (jg == 0) | (jg == nb) | (kg == 0) | (kg == nb)
         ifix = np.repeat(onb.astype(np.int32), ndf)
-        fixv = np.zeros(ndofs)
         isvgvn = (np.repeat(isngn * ndf, ndf) + np.tile(np.arange(ndf), nnods))
+        fixv = np.zeros(ndofs)
+        if self.dirichlet == "random":
+            fixv = np.where(ifix > 0, 2.0 * uniform01(self.seed + 1000, isvgvn) - 1.0, 0.0)
         # elements
         ek, ej, ei = np.meshgrid(np.arange(nes), np.arange(nes), np.arange(nes), indexing="ij")
         ei, ej, ek = ei.ravel(), ej.ravel(), ek.ravel()
@@ -215,6 +221,7 @@ class CubeProblem:
         n = self.ndof
         A = np.zeros((n, n))
         f = np.zeros(n)
+        g = np.zeros(n)
         fixed = np.zeros(n, dtype=bool)
         b = self.numbase
         for d in self.subdomains():
@@ -227,9 +234,13 @@ class CubeProblem:
             np.add.at(A, (j[off], i[off]), a[off])
             f[gi] = d["rhs"]
             fixed[gi[d["ifix"] > 0]] = True
+            g[gi] = d["fixv"]
+        # Dirichlet values move to the right-hand side; fixed rows keep the
+        # diagonal, u_fixed = g
         dg = np.diag(A).copy()
+        f -= A[:, fixed] @ g[fixed]
         A[fixed, :] = 0.0
         A[:, fixed] = 0.0
         A[fixed, fixed] = dg[fixed]
-        f[fixed] = 0.0
+        f[fixed] = dg[fixed] * g[fixed]
         return A, f, fixed

@@ -87,7 +87,10 @@ void bddcml_change_subdomain_data_c(int *isub, int *ifix, int *lifix, double *fi
                                     double *rhs, int *lrhs, int *is_rhs_complete, double *sol,
                                     int *lsol);
 
-/* reference src/bddcml_interface_c.h:57, src/module_bddcml.f90:721-728 */
+/* reference src/bddcml_interface_c.h:57, src/module_bddcml.f90:721-728.
+ * After bddcml_change_subdomain_data_c: new eliminated Dirichlet values and
+ * reduced right-hand side on the kept factorization
+ * (src/module_levels.f90:852-903). */
 void bddcml_setup_new_data_c();
 
 /* reference src/bddcml_interface_c.h:59, src/module_bddcml.f90:731-758 */

@@ -96,3 +96,33 @@ def test_solve_matches_oracle(oracle_lib, nel, ns, kind, wt, method, rhs):
         assert np.abs(ls - ls_ref).max() <= 1e-10 * np.abs(u_ref).max()
     o.close()
     b.finalize()
+
+
+@pytest.mark.parametrize("nel,ns,kind", [(3, 3, "poisson"), (2, 2, "elasticity")])
+def test_new_data_reuses_factorization(oracle_lib, nel, ns, kind):
+    """bddcml_change_subdomain_data + bddcml_setup_new_data (module_levels.f90:
+    785-903): a second right-hand side with new, non-zero Dirichlet values on
+    the same factorization gives the solution of that problem solved from
+    scratch by the oracle, and the first solve still matched its own."""
+    _say(f"case new data nel={nel} ns={ns} kind={kind}")
+    p1 = CubeProblem(nel, ns, kind=kind, rhs="random", seed=3)
+    p2 = CubeProblem(nel, ns, kind=kind, rhs="random", seed=4, dirichlet="random")
+    b, (it1, r1, c1) = bddcml_amd.solve_cube(p1, tol=1e-10, maxit=300)
+    o1, (it1_o, _, _) = _oracle(oracle_lib, p1)
+    assert r1 == 0 and abs(it1 - it1_o) <= 1
+    assert _relmax(b.interface_solution(), o1.interface_solution()) < 1e-10
+    o1.close()
+    for d in p2.subdomains():
+        b.change_subdomain_data(d)
+    b.setup_new_data()
+    it2, r2, c2 = b.solve(0, 1e-10, 300, 50)
+    o2, (it2_o, r2_o, c2_o) = _oracle(oracle_lib, p2)
+    assert r2 == r2_o == 0
+    assert abs(it2 - it2_o) <= 1
+    u_ref = o2.interface_solution()
+    assert _relmax(b.interface_solution(), u_ref) < 1e-10
+    for d in p2.subdomains():
+        ls = b.download_local_solution(d["isub"], d["ndofs"])
+        assert np.abs(ls - o2.local_solution(d["isub"])).max() <= 1e-10 * np.abs(u_ref).max()
+    o2.close()
+    b.finalize()

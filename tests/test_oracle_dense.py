@@ -184,3 +184,23 @@ def test_elasticity_small(oracle_lib):
     assert reason == 0
     assert np.abs(o.interface_solution() - u[o.interface_dofs()]).max() <= 1e-9 * np.abs(u).max()
     o.close()
+
+
+@pytest.mark.parametrize("nel,ns", [(2, 2), (3, 2)])
+def test_nonzero_dirichlet_matches_direct_solve(oracle_lib, nel, ns):
+    """Dirichlet values moved to the right-hand side (module_sm.f90:431-600,
+    module_dd.f90:1081-1256): the whole solution, fixed dofs included,
+    against the dense solve with u_fixed = g."""
+    p = CubeProblem(nel, ns, rhs="random", seed=11, dirichlet="random")
+    o, (it, reason, cond) = oracle_lib.solve_cube(p, tol=1e-12, maxit=200)
+    A, f, fixed = p.global_dense()
+    u = np.linalg.solve(A, f)
+    assert reason == 0
+    scale = np.abs(u).max()
+    assert np.abs(o.interface_solution() - u[o.interface_dofs()]).max() <= 1e-10 * scale
+    for d in p.subdomains():
+        ls = o.local_solution(d["isub"])
+        assert np.abs(ls - u[d["isvgvn"] - 1]).max() <= 1e-10 * scale
+        fx = d["ifix"] > 0
+        assert np.abs(ls[fx] - d["fixv"][fx]).max() <= 1e-12
+    o.close()

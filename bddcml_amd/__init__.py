@@ -204,6 +204,18 @@ class Bddcml:
                               C.byref(it), C.byref(reason), C.byref(cond))
         return it.value, reason.value, cond.value
 
+    def download_local_reactions(self, isub, ndofs):
+        out = np.zeros(ndofs)
+        self.L.bddcml_download_local_reactions_c(_ip(isub), out.ctypes.data_as(
+            C.POINTER(C.c_double)), _ip(ndofs))
+        return out
+
+    def download_global_reactions(self, ndof):
+        out = np.zeros(ndof)
+        self.L.bddcml_download_global_reactions_c(out.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  _ip(ndof))
+        return out
+
     def download_local_solution(self, isub, ndofs):
         out = np.zeros(ndofs)
         _, p = _darr(out)

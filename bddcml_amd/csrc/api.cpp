@@ -294,6 +294,7 @@ void bddcml_solve_c(int *comm_all, int *method, double *tol, int *maxit, int *nd
   S.last_reason = *converged_reason;
   S.last_cond = *condition_number;
   S.solved = true;
+  S.reactions_ready = false;
 }
 catch (const Error &e) {
   record_error(e);
@@ -319,16 +320,35 @@ catch (const Error &e) {
   record_error(e);
 }
 
-// reactions at Dirichlet dofs: r = A u - f with the original (uneliminated)
-// matrix is not kept by this library
-void bddcml_download_local_reactions_c(int *, double *, int *) try {
-  fatal("BDDCML_DOWNLOAD_LOCAL_REACTIONS", "reactions are not supported by this library");
+// reactions of the last solve, computed on the first request after it
+// (the reference computes them in levels_postprocess_solution)
+static void ensure_reactions(Solver &S, const char *R) {
+  if (!S.solved) fatal(R, "Reactions are not computed: no solution yet");
+  if (!S.reactions_ready) {
+    build_reactions(S);
+    S.reactions_ready = true;
+  }
+}
+
+void bddcml_download_local_reactions_c(int *isub, double *reas, int *lreas) try {
+  const char *R = "LEVELS_DD_DOWNLOAD_REACTIONS";
+  Solver &S = solver();
+  Sub &s = local_sub(S, *isub, R);
+  if (*lreas != s.ndofs) fatal("DD_DOWNLOAD_REACTIONS", "Array REA dimension mismatch.");
+  ensure_reactions(S, R);
+  std::copy(s.rea.begin(), s.rea.end(), reas);
 }
 catch (const Error &e) {
   record_error(e);
 }
-void bddcml_download_global_reactions_c(double *, int *) try {
-  fatal("BDDCML_DOWNLOAD_GLOBAL_REACTIONS", "reactions are not supported by this library");
+void bddcml_download_global_reactions_c(double *rea, int *lrea) try {
+  const char *R = "LEVELS_GET_GLOBAL_REACTIONS";
+  Solver &S = solver();
+  if (*lrea != S.ndof_g) fatal(R, "Space for global reactions not properly allocated on root.");
+  ensure_reactions(S, R);
+  std::fill(rea, rea + S.ndof_g, 0.0);
+  for (auto &s : S.subs)
+    for (int d = 0; d < s.ndofs; d++) rea[s.isvgvn[d]] = s.rea[d];
 }
 catch (const Error &e) {
   record_error(e);

@@ -454,6 +454,46 @@ static void exchange_bc(Solver &S) {
   }
 }
 
+// Reactions (src/module_dd.f90:1258-1380): rea = A_f u over the kept
+// couplings to fixed dofs (upper triangle applied symmetrically), summed
+// over the subdomains sharing an interface dof, minus the right-hand side at
+// fixed dofs (before the sum when the rhs is not a complete restriction,
+// after it when it is).
+void build_reactions(Solver &S) {
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++) {
+    Sub &s = S.subs[is];
+    s.rea.assign(s.ndofs, 0.0);
+    if (!s.has_fix) continue;
+    for (size_t q = 0; q < s.bc_k.size(); q++) {
+      const size_t k = s.bc_k[q];
+      const int i = s.ai[k], j = s.aj[k];
+      const double a = s.bc_a[q];
+      s.rea[i] += a * s.sol[j];
+      if (i != j) s.rea[j] += a * s.sol[i];
+    }
+    if (!s.rhs_complete)
+      for (int d = 0; d < s.ndofs; d++)
+        if (s.ifix[d]) s.rea[d] -= s.rhs[d];
+  }
+  if (S.bc_global) {
+    std::vector<std::vector<double>> b(S.nsub);
+    for (int is = 0; is < S.nsub; is++) {
+      const Sub &s = S.subs[is];
+      b[is].resize(s.ifc_dof.size());
+      for (size_t k = 0; k < s.ifc_dof.size(); k++) b[is][k] = s.has_fix ? s.rea[s.ifc_dof[k]] : 0.0;
+    }
+    exchange_sum(S, b);
+    for (int is = 0; is < S.nsub; is++) {
+      Sub &s = S.subs[is];
+      for (size_t k = 0; k < s.ifc_dof.size(); k++) s.rea[s.ifc_dof[k]] = b[is][k];
+      if (s.has_fix && s.rhs_complete)
+        for (int d = 0; d < s.ndofs; d++)
+          if (s.ifix[d]) s.rea[d] -= s.rhs[d];
+    }
+  }
+}
+
 // new Dirichlet values after bddcml_change_subdomain_data (the reference
 // recomputes bc from its stored fixed-rows matrix, module_dd.f90:1081-1256)
 void refresh_boundary_conditions(Solver &S) {

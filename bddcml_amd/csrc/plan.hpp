@@ -75,6 +75,7 @@ struct Sub {
   // elimination, kept to eliminate new Dirichlet values later
   std::vector<size_t> bc_k;
   std::vector<double> bc_a;
+  std::vector<double> rea;  // reactions of the last solve (ndofs)
   std::vector<double> w;    // interface weights (reference order)
   std::vector<double> fmod; // right-hand side used for condensation, ndofs
 

@@ -51,7 +51,7 @@ struct Solver {
   std::vector<int> gpos;   // global dof -> unique position or -1
   std::vector<char> ig_fixed;
   // state
-  bool is_setup = false, solved = false, plan_only = false;
+  bool is_setup = false, solved = false, plan_only = false, reactions_ready = false;
   std::vector<double> uig;  // interface solution
   std::unique_ptr<Device> dev;
   std::unique_ptr<DeviceInput> din;
@@ -71,6 +71,7 @@ void reset_solver();
 void build_topology(Solver &S);
 void apply_boundary_conditions(Solver &S);
 void refresh_boundary_conditions(Solver &S);  // new fixv, same fixed dofs
+void build_reactions(Solver &S);              // from the solution of the last solve
 void build_weights(Solver &S);
 void build_constraints(Solver &S);
 void build_condensation_rhs(Solver &S, int isub_only = -1);

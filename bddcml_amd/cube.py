@@ -215,9 +215,11 @@ class CubeProblem:
             yield self.subdomain(s)
 
     # -- global assembly (tests only need this for small cubes) --------------
-    def global_dense(self):
+    def global_dense(self, eliminate: bool = True):
         """Global matrix (full, after Dirichlet elimination that keeps the
-        original diagonal) and right-hand side, dense."""
+        original diagonal) and right-hand side, dense.  With eliminate=False:
+        the assembled matrix and right-hand side as uploaded, and the fixed
+        values."""
         n = self.ndof
         A = np.zeros((n, n))
         f = np.zeros(n)
@@ -235,6 +237,8 @@ class CubeProblem:
             f[gi] = d["rhs"]
             fixed[gi[d["ifix"] > 0]] = True
             g[gi] = d["fixv"]
+        if not eliminate:
+            return A, f, fixed, g
         # Dirichlet values move to the right-hand side; fixed rows keep the
         # diagonal, u_fixed = g
         dg = np.diag(A).copy()

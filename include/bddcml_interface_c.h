@@ -70,11 +70,14 @@ void bddcml_download_local_solution_c(int *isub, double *sols, int *lsols);
 /* reference src/bddcml_interface_c.h:44, src/module_bddcml.f90:592-605 */
 void bddcml_download_global_solution_c(double *sol, int *lsol);
 
-/* reference src/bddcml_interface_c.h:46, src/module_bddcml.f90:608-625:
- * reactions r = A u - f restricted to the subdomain (values at fixed dofs). */
+/* reference src/bddcml_interface_c.h:46, src/module_bddcml.f90:608-625;
+ * computation src/module_dd.f90:1258-1380: rea = A_f u (entries coupling to
+ * fixed dofs, as uploaded), summed over subdomains at the interface, minus
+ * the right-hand side at fixed dofs. */
 void bddcml_download_local_reactions_c(int *isub, double *reas, int *lreas);
 
-/* reference src/bddcml_interface_c.h:48, src/module_bddcml.f90:628-641 */
+/* reference src/bddcml_interface_c.h:48, src/module_bddcml.f90:628-641,
+ * src/module_levels.f90:4785-4900: rea(isvgvn) = local reactions */
 void bddcml_download_global_reactions_c(double *rea, int *lrea);
 
 /* reference src/bddcml_interface_c.h:50, src/module_bddcml.f90:644-679.

@@ -126,3 +126,28 @@ def test_new_data_reuses_factorization(oracle_lib, nel, ns, kind):
         assert np.abs(ls - o2.local_solution(d["isub"])).max() <= 1e-10 * np.abs(u_ref).max()
     o2.close()
     b.finalize()
+
+
+@pytest.mark.parametrize("kind", ["poisson", "elasticity"])
+def test_reactions_and_global_solution(kind):
+    """Reactions (module_dd.f90:1258-1380) against the assembled matrix as
+    uploaded: at fixed dofs (A u - f)_i, at free dofs the couplings to fixed
+    dofs sum_{j fixed} A_ij u_j; the global solution download equals the
+    dense solve."""
+    _say(f"case reactions kind={kind}")
+    p = CubeProblem(3, 2, kind=kind, rhs="random", seed=5, dirichlet="random")
+    b, (it, r, c) = bddcml_amd.solve_cube(p, tol=1e-12, maxit=300)
+    assert r == 0
+    A, f, fixed, g = p.global_dense(eliminate=False)
+    Ae, fe, _ = p.global_dense()
+    u = np.linalg.solve(Ae, fe)
+    ug = b.download_global_solution(p.ndof)
+    assert np.abs(ug - u).max() <= 1e-9 * np.abs(u).max()
+    rea_ref = np.where(fixed, A @ u - f, A[:, fixed] @ u[fixed])
+    scale = np.abs(rea_ref).max()
+    rg = b.download_global_reactions(p.ndof)
+    assert np.abs(rg - rea_ref).max() <= 1e-9 * scale
+    for d in p.subdomains():
+        rl = b.download_local_reactions(d["isub"], d["ndofs"])
+        assert np.abs(rl - rea_ref[d["isvgvn"] - 1]).max() <= 1e-9 * scale
+    b.finalize()

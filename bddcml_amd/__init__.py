@@ -334,12 +334,15 @@ class Bddcml:
 
 
 def solve_cube(problem, use_corner=1, use_arith=1, weights_type=0, method=0, tol=1e-8,
-               maxit=500, ndecrmax=50, verbose=0):
+               maxit=500, ndecrmax=50, verbose=0, options=None):
     """Drive the drop-in interface on a CubeProblem the way the reference
     harness does (examples/poisson_on_cube.f90:373-560); returns the Bddcml
     handle and (num_iter, converged_reason, condition_number)."""
     b = Bddcml()
     b.init(2, [problem.nsub, 1], numbase=problem.numbase, verbose_level=verbose)
+    for k, v in (options or {}).items():
+        if b.set_option(k, v) != 0:
+            raise ValueError(f"option {k}={v} rejected")
     for d in problem.subdomains():
         b.upload_subdomain_data(d)
     b.setup_preconditioner(1, 0, 1, use_corner, use_arith, 0, 0, weights_type)

@@ -521,6 +521,11 @@ int bddcml_amd_set_option(const char *key, double value) try {
     S.plan_only = value != 0.0;
     return 0;
   }
+  if (!std::strcmp(key, "precision")) {
+    if (value != 32.0 && value != 64.0) return -2;
+    S.precision = (int)value;
+    return 0;
+  }
   if (!std::strcmp(key, "interior_ordering")) {
     if (value != 0.0 && value != 1.0) return -2;
     S.ordering = (int)value;

@@ -113,6 +113,9 @@ struct SubTiles {
   TilePattern pat;   // A pattern (lower), tile ids absolute
   std::vector<std::vector<uint64_t>> col;  // column bitsets
   int base = 0, span = 0;  // first tile id and tile count of the subdomain
+  // factor space (S' factor, its inverse diagonal tiles, Z): the FP64 pool,
+  // or the FP32 pool in the mixed-precision mode
+  int fbase = 0, fspan = 0;
   int linvA = 0, sfull = 0, pbase = 0, linvP = 0, ybase = 0, ncT = 0;
   int P(int I, int J) const { return pbase + lowidx(nti, I, J); }
   int Y(int I, int cT) const { return ybase + I * ncT + cT; }
@@ -132,6 +135,9 @@ struct Device {
   hipEvent_t ev_up = nullptr;
   DBuf<double> pool;
   int64_t ntiles = 0;
+  bool mixed = false;     // S' factor, its inverse diagonal tiles and Z in FP32
+  DBuf<float> pool32;     // factor space of the mixed-precision mode
+  int64_t ntiles32 = 0;
   DBuf<int> err;
   std::vector<SubTiles> sub;
   // setup phases
@@ -260,7 +266,7 @@ static void symbolic(const Sub &s, SubTiles &T) {
   }
 }
 
-static void assign_tiles(SubTiles &T, int nc, int &next) {
+static void assign_tiles(SubTiles &T, int nc, int &next, int &next32, bool mixed) {
   const int nt = T.nto + T.nti;
   T.base = next;
   T.pat.nt = nt;
@@ -280,14 +286,23 @@ static void assign_tiles(SubTiles &T, int nc, int &next) {
   next += T.nto;
   T.sfull = next;
   next += T.nti * T.nti;
-  T.pbase = next;
-  next += T.nti * (T.nti + 1) / 2;
-  T.linvP = next;
-  next += T.nti;
+  int &nf = mixed ? next32 : next;
+  const int f0 = nf;
+  T.pbase = nf;
+  nf += T.nti * (T.nti + 1) / 2;
+  T.linvP = nf;
+  nf += T.nti;
   T.ncT = (nc + TILE - 1) / TILE;
-  T.ybase = next;
-  next += T.nti * T.ncT;
+  T.ybase = nf;
+  nf += T.nti * T.ncT;
   T.span = next - T.base;
+  if (mixed) {
+    T.fbase = f0;
+    T.fspan = nf - f0;
+  } else {
+    T.fbase = T.base;
+    T.fspan = T.span;
+  }
 }
 
 // Structural identity of two subdomains' tile layouts: same tile pattern,
@@ -295,13 +310,14 @@ static void assign_tiles(SubTiles &T, int nc, int &next) {
 static uint64_t structure_hash(const SubTiles &T) {
   uint64_t h = 1469598103934665603ull;
   auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
-  mix(T.nto), mix(T.nti), mix(T.ncT), mix(T.span);
+  mix(T.nto), mix(T.nti), mix(T.ncT), mix(T.span), mix(T.fspan);
   for (int v : T.pat.colptr) mix((uint64_t)v);
   for (int v : T.pat.rowidx) mix((uint64_t)v);
   return h;
 }
 static bool same_structure(const SubTiles &a, const SubTiles &b) {
   return a.nto == b.nto && a.nti == b.nti && a.ncT == b.ncT && a.span == b.span &&
+         a.fspan == b.fspan &&
          a.pat.colptr == b.pat.colptr && a.pat.rowidx == b.pat.rowidx;
 }
 
@@ -369,8 +385,10 @@ void device_setup_begin(Solver &S) {
   D.sub.resize(nsub);
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < nsub; is++) symbolic(S.subs[is], D.sub[is]);
-  int next = 0;
-  for (int is = 0; is < nsub; is++) assign_tiles(D.sub[is], S.subs[is].nconstr, next);
+  int next = 0, next32 = 0;
+  D.mixed = S.precision == 32;
+  for (int is = 0; is < nsub; is++) assign_tiles(D.sub[is], S.subs[is].nconstr, next, next32, D.mixed);
+  D.ntiles32 = next32;
   D.NC = std::max(TILE, (S.ndofc + TILE - 1) / TILE * TILE);
   D.ntc = D.NC / TILE;
   D.cbase = next;
@@ -495,30 +513,34 @@ void device_setup_begin(Solver &S) {
       }
     // make every tile id relative to the subdomain's first tile, checking
     // that it stays inside the subdomain's tiles
-    auto inside = [&](int id) { return id >= T.base && id < T.base + T.span; };
-    auto relative = [&](TaskList &L) {
+    auto relative = [&](TaskList &L, int b0, int nb) {
+      auto inside = [&](int id) { return id >= b0 && id < b0 + nb; };
       for (TileTask &t : L.tasks) {
         if (!inside(t.c) || (t.x != -1 && !inside(t.x)) || t.p0 < 0 || t.p1 < t.p0 ||
             t.p1 > (int)L.pairs.size())
           bad_ids |= 1;
-        t.c -= T.base;
-        if (t.x >= 0) t.x -= T.base;
+        t.c -= b0;
+        if (t.x >= 0) t.x -= b0;
       }
       for (TilePair &p : L.pairs) {
         if (!inside(p.a) || !inside(p.b)) bad_ids |= 1;
-        p.a -= T.base;
-        p.b -= T.base;
+        p.a -= b0;
+        p.b -= b0;
       }
     };
-    for (auto *v : {&Q.a_diag, &Q.a_off, &Q.p_diag, &Q.p_off, &Q.z_fwd, &Q.z_bwd})
-      for (TaskList &L : *v) relative(L);
-    relative(Q.a_schur);
+    for (auto *v : {&Q.a_diag, &Q.a_off})
+      for (TaskList &L : *v) relative(L, T.base, T.span);
+    relative(Q.a_schur, T.base, T.span);
+    for (auto *v : {&Q.p_diag, &Q.p_off, &Q.z_fwd, &Q.z_bwd})
+      for (TaskList &L : *v) relative(L, T.fbase, T.fspan);
   }
   if (bad_ids) fatal("BDDCML_DEVICE_PLAN", "tile task outside its subdomain's tiles");
   for (int is = 0; is < nsub; is++)
-    if ((int64_t)D.sub[is].base + D.sub[is].span > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "tile range");
+    if ((int64_t)D.sub[is].base + D.sub[is].span > D.ntiles ||
+        (int64_t)D.sub[is].fbase + D.sub[is].fspan > (D.mixed ? D.ntiles32 : D.ntiles))
+      fatal("BDDCML_DEVICE_PLAN", "tile range");
   // one step of every subdomain: shared pairs, per-subdomain tasks
-  auto materialize = [&](auto get) {
+  auto materialize = [&](auto get, bool factor) {
     TaskList out;
     std::vector<size_t> poff(ncls + 1, 0), toff(nsub + 1, 0);
     for (int ic = 0; ic < ncls; ic++) {
@@ -537,27 +559,33 @@ void device_setup_begin(Solver &S) {
     for (int is = 0; is < nsub; is++) {
       const TaskList *L = get(sts[cls[is]]);
       if (!L) continue;
-      const int b = D.sub[is].base, po = (int)poff[cls[is]];
+      const int b = factor ? D.sub[is].fbase : D.sub[is].base, po = (int)poff[cls[is]];
       TileTask *o = out.tasks.data() + toff[is];
       for (const TileTask &t : L->tasks)
         *o++ = TileTask{t.c + b, t.x >= 0 ? t.x + b : -1, t.p0 + po, t.p1 + po, b};
     }
     return out;
   };
-  auto merge_steps = [&](std::vector<TaskList> SubTasks::*member) {
+  auto merge_steps = [&](std::vector<TaskList> SubTasks::*member, bool factor) {
     size_t nsteps = 0;
     for (auto &Q : sts) nsteps = std::max(nsteps, (Q.*member).size());
     std::vector<TaskList> out(nsteps);
     for (size_t k = 0; k < nsteps; k++)
-      out[k] = materialize([&](const SubTasks &Q) -> const TaskList * {
-        return k < (Q.*member).size() ? &(Q.*member)[k] : nullptr;
-      });
+      out[k] = materialize(
+          [&](const SubTasks &Q) -> const TaskList * {
+            return k < (Q.*member).size() ? &(Q.*member)[k] : nullptr;
+          },
+          factor);
     return out;
   };
-  std::vector<TaskList> a_diag = merge_steps(&SubTasks::a_diag), a_off = merge_steps(&SubTasks::a_off),
-                        p_diag = merge_steps(&SubTasks::p_diag), p_off = merge_steps(&SubTasks::p_off),
-                        z_fwd = merge_steps(&SubTasks::z_fwd), z_bwd = merge_steps(&SubTasks::z_bwd);
-  TaskList a_schur = materialize([](const SubTasks &Q) -> const TaskList * { return &Q.a_schur; });
+  std::vector<TaskList> a_diag = merge_steps(&SubTasks::a_diag, false),
+                        a_off = merge_steps(&SubTasks::a_off, false),
+                        p_diag = merge_steps(&SubTasks::p_diag, true),
+                        p_off = merge_steps(&SubTasks::p_off, true),
+                        z_fwd = merge_steps(&SubTasks::z_fwd, true),
+                        z_bwd = merge_steps(&SubTasks::z_bwd, true);
+  TaskList a_schur =
+      materialize([](const SubTasks &Q) -> const TaskList * { return &Q.a_schur; }, false);
   sts.clear();
   if (S.verbose >= 2)
     std::printf("   device plan: %d structure classes over %d subdomains, %zu interior steps\n", ncls,
@@ -752,12 +780,14 @@ void device_setup_begin(Solver &S) {
 
   ph("mirror and update jobs");
   D.pool.alloc((size_t)D.ntiles * TILE_ELEMS);
+  if (D.mixed) D.pool32.alloc((size_t)std::max<int64_t>(D.ntiles32, 1) * TILE_ELEMS);
   D.err.alloc(1);
   D.err.zero(st);
   ph("pool allocation");
   if (S.verbose >= 2)
-    std::printf("   device plan: %lld tiles (%.1f GB), %zu scatter entries\n", (long long)D.ntiles,
-                D.ntiles * 32768.0 / 1e9, (size_t)D.sc_idx.n);
+    std::printf("   device plan: %lld FP64 tiles (%.1f GB), %lld FP32 tiles (%.1f GB), %zu scatter entries\n",
+                (long long)D.ntiles, D.ntiles * 32768.0 / 1e9, (long long)D.ntiles32,
+                D.ntiles32 * 16384.0 / 1e9, (size_t)D.sc_idx.n);
 
   // algorithmic work: 2*64^3 flops per tile product, 64^3/3 per tile Cholesky
   // plus 64^3/3 for its inverse, 2*64^3 per post multiply
@@ -1097,21 +1127,27 @@ void device_setup_finish(Solver &S) {
   HIPCHECK(hipStreamWaitEvent(st, D.ev_up, 0));
   double *pool = D.pool.p;
   hipEvent_t *ev = D.ev;
-  launch_sprime(st, pool, D.desc.p, D.sp_sub.p, D.sp_ij.p, D.sp_dst.p, D.n_sp, D.cid.p, D.csize.p);
-  for (size_t J = 0; J < D.p_diag.size(); J++) {
-    launch_tile_potrf(st, pool, D.p_diag[J].tasks.p, D.p_diag[J].pairs.p, D.p_diag[J].ntask, D.err.p);
-    launch_tile_gemm(st, pool, D.p_off[J].tasks.p, D.p_off[J].pairs.p, D.p_off[J].ntask, GEMM_NT,
-                     POST_RIGHT_XT, 1.0, -1.0);
-  }
-  // Z = S'^{-1} C^T through the factor, G = C Z, Phi = Z G^{-1}
-  launch_ct_init(st, pool, D.desc.p, nsub, D.cid.p, D.maxnti);
-  for (size_t i = 0; i < D.z_fwd.size(); i++)
-    launch_tile_gemm(st, pool, D.z_fwd[i].tasks.p, D.z_fwd[i].pairs.p, D.z_fwd[i].ntask, GEMM_NN,
-                     POST_LEFT_X, 1.0, -1.0);
-  for (size_t i = 0; i < D.z_bwd.size(); i++)
-    launch_tile_gemm(st, pool, D.z_bwd[i].tasks.p, D.z_bwd[i].pairs.p, D.z_bwd[i].ntask, GEMM_TN,
-                     POST_LEFT_XT, 1.0, -1.0);
-  launch_y_to_z(st, pool, D.desc.p, nsub, D.Z.p, D.maxNI);
+  // S' factor and Z = S'^{-1} C^T in the factor space (FP32 when mixed);
+  // then G = C Z, Phi = Z G^{-1} in FP64
+  auto factor_phase = [&](auto *fp) {
+    launch_sprime(st, pool, fp, D.desc.p, D.sp_sub.p, D.sp_ij.p, D.sp_dst.p, D.n_sp, D.cid.p,
+                  D.csize.p);
+    for (size_t J = 0; J < D.p_diag.size(); J++) {
+      launch_tile_potrf(st, fp, D.p_diag[J].tasks.p, D.p_diag[J].pairs.p, D.p_diag[J].ntask, D.err.p);
+      launch_tile_gemm(st, fp, D.p_off[J].tasks.p, D.p_off[J].pairs.p, D.p_off[J].ntask, GEMM_NT,
+                       POST_RIGHT_XT, 1.0, -1.0);
+    }
+    launch_ct_init(st, fp, D.desc.p, nsub, D.cid.p, D.maxnti);
+    for (size_t i = 0; i < D.z_fwd.size(); i++)
+      launch_tile_gemm(st, fp, D.z_fwd[i].tasks.p, D.z_fwd[i].pairs.p, D.z_fwd[i].ntask, GEMM_NN,
+                       POST_LEFT_X, 1.0, -1.0);
+    for (size_t i = 0; i < D.z_bwd.size(); i++)
+      launch_tile_gemm(st, fp, D.z_bwd[i].tasks.p, D.z_bwd[i].pairs.p, D.z_bwd[i].ntask, GEMM_TN,
+                       POST_LEFT_XT, 1.0, -1.0);
+    launch_y_to_z(st, fp, D.desc.p, nsub, D.Z.p, D.maxNI);
+  };
+  if (D.mixed) factor_phase(D.pool32.p);
+  else factor_phase(pool);
   launch_g(st, D.desc.p, nsub, D.cptr.p, D.cdof.p, D.Z.p, D.G.p);
   launch_ginv(st, D.desc.p, nsub, D.G.p, nullptr, D.err.p);
   launch_phi(st, D.desc.p, nsub, D.Z.p, D.G.p, D.Phi.p);
@@ -1198,6 +1234,14 @@ void device_selftest_tile(int op, int npairs, const double *A, const double *B, 
     std::copy(B + (size_t)p * TILE_ELEMS, B + (size_t)(p + 1) * TILE_ELEMS,
               h.begin() + (size_t)(3 + 2 * p) * TILE_ELEMS);
   }
+  const bool f32 = op >= 100;
+  if (f32) op -= 100;
+  DBuf<float> pool32;
+  if (f32) {
+    std::vector<float> hf(h.begin(), h.end());
+    pool32.alloc(hf.size());
+    HIPCHECK(hipMemcpy(pool32.p, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
   HIPCHECK(hipMemcpy(pool.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
   TaskList l;
   l.tasks.push_back({0, 1, 0, npairs});
@@ -1210,14 +1254,22 @@ void device_selftest_tile(int op, int npairs, const double *A, const double *B, 
   err.alloc(1);
   err.zero(nullptr);
   if (op == 3) {
-    launch_tile_potrf(nullptr, pool.p, t.p, pr.p, 1, err.p);
+    if (f32) launch_tile_potrf(nullptr, pool32.p, t.p, pr.p, 1, err.p);
+    else launch_tile_potrf(nullptr, pool.p, t.p, pr.p, 1, err.p);
   } else {
     int kind = op % 10, post = op / 10;
     if (kind < 0 || kind > 2 || post < 0 || post > 3) fatal("BDDCML_SELFTEST", "bad op");
-    launch_tile_gemm(nullptr, pool.p, t.p, pr.p, 1, kind, post, alpha, beta);
+    if (f32) launch_tile_gemm(nullptr, pool32.p, t.p, pr.p, 1, kind, post, alpha, beta);
+    else launch_tile_gemm(nullptr, pool.p, t.p, pr.p, 1, kind, post, alpha, beta);
   }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipDeviceSynchronize());
+  if (f32) {
+    std::vector<float> hf(2 * TILE_ELEMS);
+    HIPCHECK(hipMemcpy(hf.data(), pool32.p, hf.size() * sizeof(float), hipMemcpyDeviceToHost));
+    std::copy(hf.begin(), hf.begin() + (op == 3 ? 2 : 1) * TILE_ELEMS, out);
+    return;
+  }
   HIPCHECK(hipMemcpy(out, pool.p, TILE_ELEMS * sizeof(double), hipMemcpyDeviceToHost));
   if (op == 3)
     HIPCHECK(hipMemcpy(out + TILE_ELEMS, pool.p + TILE_ELEMS, TILE_ELEMS * sizeof(double),
@@ -1319,8 +1371,12 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed = false) {
     HIPCHECK(hipEventCreate(&e1));
     HIPCHECK(hipEventRecord(e0, st));
   }
-  launch_kapply(st, D.pool.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p, D.Z.p, D.cmap.p,
-                D.uc.p, D.zl.p, D.maxNI, D.maxnc);
+  if (D.mixed)
+    launch_kapply(st, (const float *)D.pool32.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p,
+                  D.Z.p, D.cmap.p, D.uc.p, D.zl.p, D.maxNI, D.maxnc);
+  else
+    launch_kapply(st, (const double *)D.pool.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p,
+                  D.Z.p, D.cmap.p, D.uc.p, D.zl.p, D.maxNI, D.maxnc);
   if (timed) {
     HIPCHECK(hipEventRecord(e1, st));
     HIPCHECK(hipEventSynchronize(e1));
@@ -1334,7 +1390,8 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed = false) {
       tiles += 0.5 * d.nti * (d.nti + 1.0);
       zb += 2.0 * d.NI * d.nc;
     }
-    S.t.kapply_bytes = 2.0 * tiles * TILE_ELEMS * 8.0 + 8.0 * zb + 28.0 * (double)D.LV;
+    const double fe = D.mixed ? 4.0 : 8.0;  // factor element size
+    S.t.kapply_bytes = 2.0 * tiles * TILE_ELEMS * fe + 8.0 * zb + 28.0 * (double)D.LV;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }

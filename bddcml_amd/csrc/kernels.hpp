@@ -18,13 +18,16 @@ enum GemmKind { GEMM_NT = 0, GEMM_NN = 1, GEMM_TN = 2 };
 // post operations on E after the sum
 enum GemmPost { POST_NONE = 0, POST_RIGHT_XT = 1, POST_LEFT_X = 2, POST_LEFT_XT = 3 };
 
-void launch_tile_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+// T = double or float (tile pool element type)
+template <typename T>
+void launch_tile_gemm(hipStream_t st, T *pool, const TileTask *tasks, const TilePair *pairs,
                       int ntasks, int kind, int post, double alpha, double beta);
 // C = C - sum A B^T as its own kernel symbol (Schur-complement block update)
 void launch_schur_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
                        int ntasks);
 // C = C - sum A B^T ; C = chol(C) (lower, upper zeroed) ; X = inv(C)
-void launch_tile_potrf(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+template <typename T>
+void launch_tile_potrf(hipStream_t st, T *pool, const TileTask *tasks, const TilePair *pairs,
                        int ntasks, int *err);
 // dst tiles (full storage) from lower tiles: for each job (src, dst, transpose)
 struct MirrorJob {
@@ -52,9 +55,9 @@ void launch_scatter(hipStream_t st, double *pool, const int64_t *idx, const doub
 struct SubDesc {
   int ni, NI, nti, nc;    // interface dofs, padded, tile count, constraints
   int sfull;              // first tile of S (full, row-major tile order)
-  int pbase;              // first tile of the packed lower factor of S'
-  int plinv;              // first of the nti inverse diagonal factor tiles
-  int ybase;              // first tile of Z = S'^{-1} C^T (nti x ncT, row-major)
+  int pbase;              // first tile of the packed lower factor of S' (factor pool)
+  int plinv;              // first of the nti inverse diagonal factor tiles (factor pool)
+  int ybase;              // first tile of Z = S'^{-1} C^T (nti x ncT, row-major; factor pool)
   int ncT;                // constraint tile columns, ceil(nc/64)
   int64_t loff;           // offset into local interface vectors
   int64_t zoff;           // offset into Z / Phi (NI x nc column major)
@@ -62,16 +65,20 @@ struct SubDesc {
   int coff;               // offset into constraint arrays (row pointers)
   double rho;             // augmented-Lagrangian scale
 };
-// S' = S + sum_c (rho/|D_c|) 1_c 1_c^T written into lower tiles
-void launch_sprime(hipStream_t st, double *pool, const SubDesc *subs, const int *jobs_sub,
-                   const int *jobs_ij, const int *jobs_dst, int njobs, const int *cid,
-                   const int *csize);
+// S' = S + sum_c (rho/|D_c|) 1_c 1_c^T into the lower tiles of the factor
+// pool ppool (element type T; FP32 in the mixed-precision mode)
+template <typename T>
+void launch_sprime(hipStream_t st, const double *pool, T *ppool, const SubDesc *subs,
+                   const int *jobs_sub, const int *jobs_ij, const int *jobs_dst, int njobs,
+                   const int *cid, const int *csize);
 // right-hand sides C^T of the multi-RHS solve S' Z = C^T, written as the
 // nti x ncT tile block at ybase: B(i, c) = 1 if interface dof i is in D_c
-void launch_ct_init(hipStream_t st, double *pool, const SubDesc *subs, int nsub, const int *cid,
+template <typename T>
+void launch_ct_init(hipStream_t st, T *pool, const SubDesc *subs, int nsub, const int *cid,
                     int maxnti);
 // Z (column major NI x nc at zoff) from the solved tile block
-void launch_y_to_z(hipStream_t st, const double *pool, const SubDesc *subs, int nsub, double *Z,
+template <typename T>
+void launch_y_to_z(hipStream_t st, const T *pool, const SubDesc *subs, int nsub, double *Z,
                    int maxNI);
 // G = C Z, Ginv = G^{-1} (in place Gauss-Jordan, SPD), Phi = Z Ginv,
 // Acs = Ginv - diag(rho/|D_c|)
@@ -141,10 +148,10 @@ void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
 // Local part of the BDDC preconditioner on every subdomain, one workgroup
 // each: x = w .* r(map); z = S'^{-1} x through the packed factor of S'
 // (forward and backward tile sweeps); zl = z + Phi (uc(indrowc) - Z^T x).
-void launch_kapply(hipStream_t st, const double *pool, const SubDesc *subs, int nsub,
-                   const int *map, const double *w, const double *r, const double *Phi,
-                   const double *Z, const int *cmap, const double *uc, double *zl, int maxNI,
-                   int maxnc);
+template <typename T>
+void launch_kapply(hipStream_t st, const T *pool, const SubDesc *subs, int nsub, const int *map,
+                   const double *w, const double *r, const double *Phi, const double *Z,
+                   const int *cmap, const double *uc, double *zl, int maxNI, int maxnc);
 
 // deterministic reductions and vector updates on unique vectors
 constexpr int DOT_BLOCKS = 240;

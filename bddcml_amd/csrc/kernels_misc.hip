@@ -8,57 +8,67 @@
 namespace bddc {
 
 // --------------------------------------------------------- S' = S + rho C^T C
-__global__ void k_sprime(double *__restrict__ pool, const SubDesc *__restrict__ subs,
-                         const int *__restrict__ jsub, const int *__restrict__ jij,
-                         const int *__restrict__ jdst, const int *__restrict__ cid,
-                         const int *__restrict__ csize) {
+template <typename T>
+__global__ void k_sprime(const double *__restrict__ pool, T *__restrict__ ppool,
+                         const SubDesc *__restrict__ subs, const int *__restrict__ jsub,
+                         const int *__restrict__ jij, const int *__restrict__ jdst,
+                         const int *__restrict__ cid, const int *__restrict__ csize) {
   const int job = blockIdx.x;
   const SubDesc d = subs[jsub[job]];
   const int I = jij[job] >> 16, J = jij[job] & 0xffff;
   const double *src = pool + (size_t)(d.sfull + I * d.nti + J) * TILE_ELEMS;
-  double *dst = pool + (size_t)jdst[job] * TILE_ELEMS;
+  T *dst = ppool + (size_t)jdst[job] * TILE_ELEMS;
   const int *ci = cid + d.loff;
   for (int e = threadIdx.x; e < TILE_ELEMS; e += blockDim.x) {
     int r = I * 64 + (e & 63), c = J * 64 + (e >> 6);
     double v = src[e];
     int a = ci[r], b = ci[c];
     if (a >= 0 && a == b) v += d.rho / (double)csize[a];
-    dst[e] = v;
+    dst[e] = (T)v;
   }
 }
 
-void launch_sprime(hipStream_t st, double *pool, const SubDesc *subs, const int *jobs_sub,
-                   const int *jobs_ij, const int *jobs_dst, int njobs, const int *cid,
-                   const int *csize) {
+template <typename T>
+void launch_sprime(hipStream_t st, const double *pool, T *ppool, const SubDesc *subs,
+                   const int *jobs_sub, const int *jobs_ij, const int *jobs_dst, int njobs,
+                   const int *cid, const int *csize) {
   if (njobs <= 0) return;
-  hipLaunchKernelGGL(k_sprime, dim3(njobs), dim3(256), 0, st, pool, subs, jobs_sub, jobs_ij,
-                     jobs_dst, cid, csize);
+  hipLaunchKernelGGL(k_sprime<T>, dim3(njobs), dim3(256), 0, st, pool, ppool, subs, jobs_sub,
+                     jobs_ij, jobs_dst, cid, csize);
 }
+template void launch_sprime<double>(hipStream_t, const double *, double *, const SubDesc *,
+                                    const int *, const int *, const int *, int, const int *,
+                                    const int *);
+template void launch_sprime<float>(hipStream_t, const double *, float *, const SubDesc *,
+                                   const int *, const int *, const int *, int, const int *,
+                                   const int *);
 
 // --------------------------------------------- C^T blocks and Z = S'^{-1} C^T
-__global__ void k_ct_init(double *__restrict__ pool, const SubDesc *__restrict__ subs,
+template <typename T>
+__global__ void k_ct_init(T *__restrict__ pool, const SubDesc *__restrict__ subs,
                           const int *__restrict__ cid) {
   const SubDesc d = subs[blockIdx.y];
   const int I = blockIdx.x;
   if (I >= d.nti) return;
   const int *ci = cid + d.loff + I * 64;
   for (int cT = 0; cT < d.ncT; cT++) {
-    double *T = pool + (size_t)(d.ybase + I * d.ncT + cT) * TILE_ELEMS;
+    T *Y = pool + (size_t)(d.ybase + I * d.ncT + cT) * TILE_ELEMS;
     for (int e = threadIdx.x; e < TILE_ELEMS; e += blockDim.x) {
       const int r = e & 63, c = cT * 64 + (e >> 6);
-      T[e] = (c < d.nc && ci[r] == d.coff + c) ? 1.0 : 0.0;
+      Y[e] = (c < d.nc && ci[r] == d.coff + c) ? (T)1 : (T)0;
     }
   }
 }
 
-__global__ void k_y_to_z(const double *__restrict__ pool, const SubDesc *__restrict__ subs,
+template <typename T>
+__global__ void k_y_to_z(const T *__restrict__ pool, const SubDesc *__restrict__ subs,
                          double *__restrict__ Z) {
   const SubDesc d = subs[blockIdx.y];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.NI) return;
-  const double *T = pool + (size_t)(d.ybase + (i >> 6) * d.ncT) * TILE_ELEMS + (i & 63);
+  const T *Y = pool + (size_t)(d.ybase + (i >> 6) * d.ncT) * TILE_ELEMS + (i & 63);
   for (int c = 0; c < d.nc; c++)
-    Z[d.zoff + (size_t)c * d.NI + i] = T[(size_t)(c >> 6) * TILE_ELEMS + (c & 63) * 64];
+    Z[d.zoff + (size_t)c * d.NI + i] = (double)Y[(size_t)(c >> 6) * TILE_ELEMS + (c & 63) * 64];
 }
 
 __global__ void k_g(const SubDesc *__restrict__ subs, const int *__restrict__ cptr,
@@ -144,16 +154,22 @@ __global__ void k_coarse_assemble(double *__restrict__ pool, const int64_t *__re
   pool[dst[k]] = s;
 }
 
-void launch_ct_init(hipStream_t st, double *pool, const SubDesc *subs, int nsub, const int *cid,
+template <typename T>
+void launch_ct_init(hipStream_t st, T *pool, const SubDesc *subs, int nsub, const int *cid,
                     int maxnti) {
   if (nsub <= 0 || maxnti <= 0) return;
-  hipLaunchKernelGGL(k_ct_init, dim3(maxnti, nsub), dim3(256), 0, st, pool, subs, cid);
+  hipLaunchKernelGGL(k_ct_init<T>, dim3(maxnti, nsub), dim3(256), 0, st, pool, subs, cid);
 }
-void launch_y_to_z(hipStream_t st, const double *pool, const SubDesc *subs, int nsub, double *Z,
+template <typename T>
+void launch_y_to_z(hipStream_t st, const T *pool, const SubDesc *subs, int nsub, double *Z,
                    int maxNI) {
   if (nsub <= 0 || maxNI <= 0) return;
-  hipLaunchKernelGGL(k_y_to_z, dim3((maxNI + 255) / 256, nsub), dim3(256), 0, st, pool, subs, Z);
+  hipLaunchKernelGGL(k_y_to_z<T>, dim3((maxNI + 255) / 256, nsub), dim3(256), 0, st, pool, subs, Z);
 }
+template void launch_ct_init<double>(hipStream_t, double *, const SubDesc *, int, const int *, int);
+template void launch_ct_init<float>(hipStream_t, float *, const SubDesc *, int, const int *, int);
+template void launch_y_to_z<double>(hipStream_t, const double *, const SubDesc *, int, double *, int);
+template void launch_y_to_z<float>(hipStream_t, const float *, const SubDesc *, int, double *, int);
 void launch_g(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const int *cdof,
               const double *Z, double *G) {
   if (nsub <= 0) return;
@@ -412,8 +428,9 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+template <typename T>
 __global__ __launch_bounds__(KA_THR) void k_kapply(
-    const double *__restrict__ pool, const SubDesc *__restrict__ subs, const int *__restrict__ map,
+    const T *__restrict__ pool, const SubDesc *__restrict__ subs, const int *__restrict__ map,
     const double *__restrict__ w, const double *__restrict__ rin, const double *__restrict__ Phi,
     const double *__restrict__ Z, const int *__restrict__ cmap, const double *__restrict__ uc,
     double *__restrict__ zl) {
@@ -442,10 +459,10 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
     double s = 0.0;
 #pragma unroll 2
     for (int k = 0; k < I; k++) {
-      const double *T = pool + (size_t)(d.pbase + packed_lower(n, I, k)) * TILE_ELEMS + l;
+      const T *L = pool + (size_t)(d.pbase + packed_lower(n, I, k)) * TILE_ELEMS + l;
       const double *x = xs + k * 64;
 #pragma unroll
-      for (int m = 0; m < 4; m++) s += T[(g + 16 * m) * 64] * x[g + 16 * m];
+      for (int m = 0; m < 4; m++) s += (double)L[(g + 16 * m) * 64] * x[g + 16 * m];
     }
     red[g * 64 + l] = s;
     __syncthreads();
@@ -457,10 +474,10 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
     }
     __syncthreads();
     {
-      const double *L = pool + (size_t)(d.plinv + I) * TILE_ELEMS + l;
+      const T *L = pool + (size_t)(d.plinv + I) * TILE_ELEMS + l;
       double u = 0.0;
 #pragma unroll
-      for (int m = 0; m < 4; m++) u += L[(g + 16 * m) * 64] * t[g + 16 * m];
+      for (int m = 0; m < 4; m++) u += (double)L[(g + 16 * m) * 64] * t[g + 16 * m];
       red[g * 64 + l] = u;
     }
     __syncthreads();
@@ -478,10 +495,10 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
     double a[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 2
     for (int k = I + 1; k < n; k++) {
-      const double *T = pool + (size_t)(d.pbase + packed_lower(n, k, I)) * TILE_ELEMS + g * 64 + l;
+      const T *L = pool + (size_t)(d.pbase + packed_lower(n, k, I)) * TILE_ELEMS + g * 64 + l;
       const double xv = xs[k * 64 + l];
 #pragma unroll
-      for (int m = 0; m < 4; m++) a[m] += T[m * 16 * 64] * xv;
+      for (int m = 0; m < 4; m++) a[m] += (double)L[m * 16 * 64] * xv;
     }
 #pragma unroll
     for (int m = 0; m < 4; m++) a[m] = wave_sum(a[m]);
@@ -490,10 +507,10 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
       for (int m = 0; m < 4; m++) t[g + 16 * m] = xs[I * 64 + g + 16 * m] - a[m];
     }
     __syncthreads();
-    const double *L = pool + (size_t)(d.plinv + I) * TILE_ELEMS + g * 64 + l;
+    const T *L = pool + (size_t)(d.plinv + I) * TILE_ELEMS + g * 64 + l;
     const double tv = t[l];
 #pragma unroll
-    for (int m = 0; m < 4; m++) a[m] = wave_sum(L[m * 16 * 64] * tv);
+    for (int m = 0; m < 4; m++) a[m] = wave_sum((double)L[m * 16 * 64] * tv);
     if (l == 0) {
 #pragma unroll
       for (int m = 0; m < 4; m++) xs[I * 64 + g + 16 * m] = a[m];
@@ -530,16 +547,23 @@ void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
   if (n <= 0) return;
   hipLaunchKernelGGL(k_coarse_gather, dim3((n + 255) / 256), dim3(256), 0, st, n, ptr, src, rcs, rc);
 }
-void launch_kapply(hipStream_t st, const double *pool, const SubDesc *subs, int nsub,
-                   const int *map, const double *w, const double *r, const double *Phi,
-                   const double *Z, const int *cmap, const double *uc, double *zl, int maxNI,
-                   int maxnc) {
+template <typename T>
+void launch_kapply(hipStream_t st, const T *pool, const SubDesc *subs, int nsub, const int *map,
+                   const double *w, const double *r, const double *Phi, const double *Z,
+                   const int *cmap, const double *uc, double *zl, int maxNI, int maxnc) {
   if (nsub <= 0) return;
   size_t sh = (size_t)(maxNI + 1024 + 64 + maxnc) * sizeof(double);
-  lds_request((const void *)k_kapply, sh, "preconditioner apply");
-  hipLaunchKernelGGL(k_kapply, dim3(nsub), dim3(KA_THR), sh, st, pool, subs, map, w, r, Phi, Z,
+  lds_request((const void *)k_kapply<T>, sh, "preconditioner apply");
+  hipLaunchKernelGGL(k_kapply<T>, dim3(nsub), dim3(KA_THR), sh, st, pool, subs, map, w, r, Phi, Z,
                      cmap, uc, zl);
 }
+template void launch_kapply<double>(hipStream_t, const double *, const SubDesc *, int, const int *,
+                                    const double *, const double *, const double *,
+                                    const double *, const int *, const double *, double *, int,
+                                    int);
+template void launch_kapply<float>(hipStream_t, const float *, const SubDesc *, int, const int *,
+                                   const double *, const double *, const double *, const double *,
+                                   const int *, const double *, double *, int, int);
 
 // ------------------------------------------------------------ vector ops
 __global__ __launch_bounds__(256) void k_dots(int n, DotArgs a, double *__restrict__ partial) {

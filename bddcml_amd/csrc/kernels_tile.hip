@@ -1,7 +1,8 @@
-// kernels_tile.hip -- batched 64x64 FP64 tile algebra on CDNA4 matrix cores.
+// kernels_tile.hip -- batched 64x64 tile algebra on CDNA4 matrix cores, FP64
+// (and FP32 for the preconditioner factors of the mixed-precision mode).
 //
 // One workgroup (4 waves) owns one output tile.  Each wave computes a 32x32
-// quadrant as 2x2 blocks of v_mfma_f64_16x16x4 accumulators; operand tiles
+// quadrant as 2x2 blocks of 16x16x4 MFMA accumulators; operand tiles
 // are staged through LDS with a padded leading dimension.  These kernels
 // carry every dense factorisation of the path: the partial Cholesky of the
 // subdomain matrices (Schur complements), the Cholesky / inverse of the
@@ -10,116 +11,147 @@
 
 namespace bddc {
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-constexpr int LDL = 65;  // LDS leading dimension (doubles)
+constexpr int LDL = 65;  // LDS leading dimension (elements)
 constexpr int NTHR = 256;
 
-// lds[q*LDL + r] = T(r,q)   (direct)      element (r,q) of the tile
-// lds[q*LDL + r] = T(q,r)   (transposed)
-__device__ __forceinline__ void tile_to_lds(const double *__restrict__ T, double *lds, bool trans) {
-  const int tid = threadIdx.x;
-  const double2 *T2 = reinterpret_cast<const double2 *>(T);
+// Element-type traits of the 16x16x4 matrix-core instruction.  Lane l
+// supplies A(l%16, l/16) and B(l/16, l%16); register v of lane l holds
+// D(row(l, v), l%16), with row = l/16 + 4v for f64 and 4(l/16) + v for f32
+// (both measured on gfx950).
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  typedef double2 vec;  // 16-byte global access
+  static constexpr int VE = 2;
+  __device__ static __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <> struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef float4 vec;
+  static constexpr int VE = 4;
+  __device__ static __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+template <typename T> __device__ __forceinline__ T vget(const typename Mfma<T>::vec &v, int i);
+template <> __device__ __forceinline__ double vget<double>(const double2 &v, int i) {
+  return i == 0 ? v.x : v.y;
+}
+template <> __device__ __forceinline__ float vget<float>(const float4 &v, int i) {
+  return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+template <typename T> __device__ __forceinline__ void vset(typename Mfma<T>::vec &v, int i, T x);
+template <> __device__ __forceinline__ void vset<double>(double2 &v, int i, double x) {
+  if (i == 0) v.x = x;
+  else v.y = x;
+}
+template <> __device__ __forceinline__ void vset<float>(float4 &v, int i, float x) {
+  if (i == 0) v.x = x;
+  else if (i == 1) v.y = x;
+  else if (i == 2) v.z = x;
+  else v.w = x;
+}
+
+// a tile held in registers: thread tid owns vector elements tid + NTHR*i
+template <typename T> struct TileRegs {
+  static constexpr int N = TILE_ELEMS / Mfma<T>::VE / NTHR;
+  typename Mfma<T>::vec v[N];
+};
+template <typename T>
+__device__ __forceinline__ void tile_load(const T *__restrict__ t, TileRegs<T> &r) {
+  const typename Mfma<T>::vec *V = reinterpret_cast<const typename Mfma<T>::vec *>(t);
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    int e2 = tid + NTHR * i;  // double2 index, elements 2*e2 and 2*e2+1
-    double2 v = T2[e2];
-    int e = 2 * e2;
-    int row = e & 63, col = e >> 6;
-    if (!trans) {
-      lds[col * LDL + row] = v.x;
-      lds[col * LDL + row + 1] = v.y;
-    } else {
-      lds[row * LDL + col] = v.x;
-      lds[(row + 1) * LDL + col] = v.y;
+  for (int i = 0; i < TileRegs<T>::N; i++) r.v[i] = V[threadIdx.x + NTHR * i];
+}
+// lds[q*LDL + r] = T(r,q) (direct) or T(q,r) (transposed)
+template <typename T>
+__device__ __forceinline__ void tile_store_lds(const TileRegs<T> &r, T *lds, bool trans) {
+  constexpr int VE = Mfma<T>::VE;
+#pragma unroll
+  for (int i = 0; i < TileRegs<T>::N; i++) {
+    const int e = VE * (threadIdx.x + NTHR * i);
+    const int row = e & 63, col = e >> 6;
+#pragma unroll
+    for (int j = 0; j < VE; j++) {
+      const T x = vget<T>(r.v[i], j);
+      if (!trans) lds[col * LDL + row + j] = x;
+      else lds[(row + j) * LDL + col] = x;
     }
   }
 }
-
-// global tile (col major) from lds holding element (r,c) at c*LDL + r
-__device__ __forceinline__ void lds_to_tile(const double *lds, double *__restrict__ T) {
-  const int tid = threadIdx.x;
-  double2 *T2 = reinterpret_cast<double2 *>(T);
+template <typename T>
+__device__ __forceinline__ void tile_to_lds(const T *__restrict__ t, T *lds, bool trans) {
+  TileRegs<T> r;
+  tile_load(t, r);
+  tile_store_lds(r, lds, trans);
+}
+// global tile (column major) from lds holding element (r,c) at c*LDL + r
+template <typename T> __device__ __forceinline__ void lds_to_tile(const T *lds, T *__restrict__ t) {
+  constexpr int VE = Mfma<T>::VE;
+  typename Mfma<T>::vec *V = reinterpret_cast<typename Mfma<T>::vec *>(t);
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    int e2 = tid + NTHR * i;
-    int e = 2 * e2;
-    int row = e & 63, col = e >> 6;
-    double2 v;
-    v.x = lds[col * LDL + row];
-    v.y = lds[col * LDL + row + 1];
-    T2[e2] = v;
+  for (int i = 0; i < TileRegs<T>::N; i++) {
+    const int e2 = threadIdx.x + NTHR * i, e = VE * e2;
+    const int row = e & 63, col = e >> 6;
+    typename Mfma<T>::vec v;
+#pragma unroll
+    for (int j = 0; j < VE; j++) vset<T>(v, j, lds[col * LDL + row + j]);
+    V[e2] = v;
   }
 }
 
-// acc(r,c) += sum_q As[q*LDL + r] * Bs[q*LDL + c] over this wave's quadrant
-__device__ __forceinline__ void mfma_accumulate(const double *As, const double *Bs, v4d acc[2][2]) {
+// acc(r,c) += sum_q As[q*LDL + r] * Bs[q*LDL + c] over this wave's 32x32
+// quadrant, as 2x2 blocks of 16x16
+template <typename T>
+__device__ __forceinline__ void mfma_accumulate(const T *As, const T *Bs,
+                                                typename Mfma<T>::v4 acc[2][2]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int R0 = (wave & 1) * 32, C0 = (wave >> 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
 #pragma unroll 4
   for (int q = 0; q < 64; q += 4) {
-    const double *ar = As + (q + lk) * LDL + R0 + li;
-    const double *br = Bs + (q + lk) * LDL + C0 + li;
-    double a0 = ar[0], a1 = ar[16];
-    double b0 = br[0], b1 = br[16];
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    const T *ar = As + (q + lk) * LDL + R0 + li;
+    const T *br = Bs + (q + lk) * LDL + C0 + li;
+    const T a0 = ar[0], a1 = ar[16];
+    const T b0 = br[0], b1 = br[16];
+    acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+    acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+    acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+    acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
   }
 }
 
-// element coordinates of accumulator (bi,bj,v) of this lane: for the f64
-// 16x16x4 MFMA, register v of lane l holds row (l/16) + 4v, column l%16 of
-// the 16x16 block (measured on gfx950)
-__device__ __forceinline__ void acc_coord(int bi, int bj, int v, int &r, int &c) {
+// tile coordinates of accumulator (bi, bj, v) of this lane
+template <typename T> __device__ __forceinline__ void acc_coord(int bi, int bj, int v, int &r, int &c) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  r = (wave & 1) * 32 + bi * 16 + (lane >> 4) + 4 * v;
+  r = (wave & 1) * 32 + bi * 16 + Mfma<T>::row(lane, v);
   c = (wave >> 1) * 32 + bj * 16 + (lane & 15);
 }
 
-__device__ __forceinline__ void zero_acc(v4d acc[2][2]) {
+template <typename T> __device__ __forceinline__ void zero_acc(typename Mfma<T>::v4 acc[2][2]) {
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-}
-
-// a tile held in registers: thread tid owns double2 elements tid + NTHR*i
-struct TileRegs {
-  double2 v[8];
-};
-__device__ __forceinline__ void tile_load(const double *__restrict__ T, TileRegs &r) {
-  const double2 *T2 = reinterpret_cast<const double2 *>(T);
-#pragma unroll
-  for (int i = 0; i < 8; i++) r.v[i] = T2[threadIdx.x + NTHR * i];
-}
-__device__ __forceinline__ void tile_store_lds(const TileRegs &r, double *lds, bool trans) {
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    const int e = 2 * (threadIdx.x + NTHR * i);
-    const int row = e & 63, col = e >> 6;
-    if (!trans) {
-      lds[col * LDL + row] = r.v[i].x;
-      lds[col * LDL + row + 1] = r.v[i].y;
-    } else {
-      lds[row * LDL + col] = r.v[i].x;
-      lds[(row + 1) * LDL + col] = r.v[i].y;
-    }
-  }
+    for (int j = 0; j < 2; j++) acc[i][j] = (typename Mfma<T>::v4){0, 0, 0, 0};
 }
 
 // acc = sum over pairs of opA(A) opB(B) in the "r,q x c,q" form.  Software
 // pipelined: the next pair's tiles are loaded into registers while the
 // matrix cores work on the current pair from LDS.
-__device__ __forceinline__ void gemm_sum(const double *__restrict__ pool, const TilePair *pairs,
-                                         int p0, int p1, int base, int kind, double *As,
-                                         double *Bs, v4d acc[2][2]) {
+template <typename T>
+__device__ __forceinline__ void gemm_sum(const T *__restrict__ pool, const TilePair *pairs, int p0,
+                                         int p1, int base, int kind, T *As, T *Bs,
+                                         typename Mfma<T>::v4 acc[2][2]) {
   const bool ta = (kind == GEMM_TN);
   const bool tb = (kind == GEMM_NN || kind == GEMM_TN);
-  const double *pb = pool + (size_t)base * TILE_ELEMS;
+  const T *pb = pool + (size_t)base * TILE_ELEMS;
   if (p0 >= p1) return;
-  TileRegs ra, rb;
+  TileRegs<T> ra, rb;
   {
     const TilePair pr = pairs[p0];
     tile_load(pb + (size_t)pr.a * TILE_ELEMS, ra);
@@ -135,13 +167,14 @@ __device__ __forceinline__ void gemm_sum(const double *__restrict__ pool, const 
       tile_load(pb + (size_t)pr.a * TILE_ELEMS, ra);
       tile_load(pb + (size_t)pr.b * TILE_ELEMS, rb);
     }
-    mfma_accumulate(As, Bs, acc);
+    mfma_accumulate<T>(As, Bs, acc);
   }
 }
 
 // E = alpha*C + beta*acc, left in As as element (r,c) at c*LDL + r
-__device__ __forceinline__ void form_e(const double *__restrict__ C, double alpha, double beta,
-                                       v4d acc[2][2], double *As) {
+template <typename T>
+__device__ __forceinline__ void form_e(const T *__restrict__ C, double alpha, double beta,
+                                       typename Mfma<T>::v4 acc[2][2], T *As) {
   __syncthreads();
   if (alpha != 0.0) tile_to_lds(C, As, false);
   __syncthreads();
@@ -152,15 +185,16 @@ __device__ __forceinline__ void form_e(const double *__restrict__ C, double alph
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         int r, c;
-        acc_coord(bi, bj, v, r, c);
-        double e = beta * acc[bi][bj][v];
-        if (alpha != 0.0) e += alpha * As[c * LDL + r];
+        acc_coord<T>(bi, bj, v, r, c);
+        T e = (T)beta * acc[bi][bj][v];
+        if (alpha != 0.0) e += (T)alpha * As[c * LDL + r];
         As[c * LDL + r] = e;
       }
   __syncthreads();
 }
 
-__device__ __forceinline__ void acc_to_lds(v4d acc[2][2], double *As) {
+template <typename T>
+__device__ __forceinline__ void acc_to_lds(typename Mfma<T>::v4 acc[2][2], T *As) {
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
@@ -168,26 +202,27 @@ __device__ __forceinline__ void acc_to_lds(v4d acc[2][2], double *As) {
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         int r, c;
-        acc_coord(bi, bj, v, r, c);
+        acc_coord<T>(bi, bj, v, r, c);
         As[c * LDL + r] = acc[bi][bj][v];
       }
 }
 
-__device__ __forceinline__ void tile_gemm_task(double *__restrict__ pool, const TileTask t,
+template <typename T>
+__device__ __forceinline__ void tile_gemm_task(T *__restrict__ pool, const TileTask t,
                                                const TilePair *__restrict__ pairs, int kind,
                                                int post, double alpha, double beta) {
-  __shared__ double As[64 * LDL];
-  __shared__ double Bs[64 * LDL];
-  v4d acc[2][2];
-  zero_acc(acc);
-  gemm_sum(pool, pairs, t.p0, t.p1, t.base, kind, As, Bs, acc);
-  double *C = pool + (size_t)t.c * TILE_ELEMS;
-  form_e(C, alpha, beta, acc, As);  // E in As
+  __shared__ T As[64 * LDL];
+  __shared__ T Bs[64 * LDL];
+  typename Mfma<T>::v4 acc[2][2];
+  zero_acc<T>(acc);
+  gemm_sum<T>(pool, pairs, t.p0, t.p1, t.base, kind, As, Bs, acc);
+  T *C = pool + (size_t)t.c * TILE_ELEMS;
+  form_e<T>(C, alpha, beta, acc, As);  // E in As
   if (post == POST_NONE) {
     lds_to_tile(As, C);
     return;
   }
-  const double *X = pool + (size_t)t.x * TILE_ELEMS;
+  const T *X = pool + (size_t)t.x * TILE_ELEMS;
   if (post == POST_RIGHT_XT) {
     // C_new(r,c) = sum_q E(r,q) X(c,q): A' = E (already As), B'(c,q) = X(c,q)
     tile_to_lds(X, Bs, false);
@@ -202,19 +237,20 @@ __device__ __forceinline__ void tile_gemm_task(double *__restrict__ pool, const 
     tile_to_lds(X, As, post == POST_LEFT_XT);
   }
   __syncthreads();
-  zero_acc(acc);
-  mfma_accumulate(As, Bs, acc);
+  zero_acc<T>(acc);
+  mfma_accumulate<T>(As, Bs, acc);
   __syncthreads();
-  acc_to_lds(acc, As);
+  acc_to_lds<T>(acc, As);
   __syncthreads();
   lds_to_tile(As, C);
 }
 
-__global__ __launch_bounds__(NTHR) void k_tile_gemm(double *__restrict__ pool,
+template <typename T>
+__global__ __launch_bounds__(NTHR) void k_tile_gemm(T *__restrict__ pool,
                                                     const TileTask *__restrict__ tasks,
                                                     const TilePair *__restrict__ pairs, int kind,
                                                     int post, double alpha, double beta) {
-  tile_gemm_task(pool, tasks[blockIdx.x], pairs, kind, post, alpha, beta);
+  tile_gemm_task<T>(pool, tasks[blockIdx.x], pairs, kind, post, alpha, beta);
 }
 
 // the Schur-complement block update S = A22 - sum W^T W in a launch of its
@@ -222,19 +258,21 @@ __global__ __launch_bounds__(NTHR) void k_tile_gemm(double *__restrict__ pool,
 __global__ __launch_bounds__(NTHR) void k_schur_gemm(double *__restrict__ pool,
                                                      const TileTask *__restrict__ tasks,
                                                      const TilePair *__restrict__ pairs) {
-  tile_gemm_task(pool, tasks[blockIdx.x], pairs, GEMM_NT, POST_NONE, 1.0, -1.0);
+  tile_gemm_task<double>(pool, tasks[blockIdx.x], pairs, GEMM_NT, POST_NONE, 1.0, -1.0);
 }
 
 // acc += sum_{k < nk} A(r, k) B'(c, k) for one 16x16 block in the lane layout
-// of the f64 16x16x4 MFMA: lane l supplies A(l%16, q + l/16) and
-// B'(l%16, q + l/16); A(r, k) = pa[k*ska + r*sra], B'(c, k) = pb[k*skb + c*srb]
-__device__ __forceinline__ v4d mfma16(v4d acc, const double *pa, int sra, int ska,
-                                      const double *pb, int srb, int skb, int nk) {
+// of the 16x16x4 MFMA: lane l supplies A(l%16, q + l/16) and B'(l%16, q + l/16);
+// A(r, k) = pa[k*ska + r*sra], B'(c, k) = pb[k*skb + c*srb]
+template <typename T>
+__device__ __forceinline__ typename Mfma<T>::v4 mfma16(typename Mfma<T>::v4 acc, const T *pa,
+                                                       int sra, int ska, const T *pb, int srb,
+                                                       int skb, int nk) {
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
   for (int q = 0; q < nk; q += 4) {
-    const double a = pa[(q + lk) * ska + li * sra];
-    const double b = pb[(q + lk) * skb + li * srb];
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    const T a = pa[(q + lk) * ska + li * sra];
+    const T b = pb[(q + lk) * skb + li * srb];
+    acc = Mfma<T>::op(a, b, acc);
   }
   return acc;
 }
@@ -242,41 +280,41 @@ __device__ __forceinline__ v4d mfma16(v4d acc, const double *pa, int sra, int sk
 // Cholesky factor of the 16x16 diagonal block at (c0, c0) by one wave in
 // registers: lane i (mod 16) holds row i, cross-lane values move by
 // shuffles; L goes back to As with its upper part zeroed.
-__device__ __forceinline__ void chol16(double *As, int c0, int &bad) {
+template <typename T> __device__ __forceinline__ void chol16(T *As, int c0, int &bad) {
   const int lane = threadIdx.x & 63, i = lane & 15;
-  double a[16];
+  T a[16];
 #pragma unroll
   for (int c = 0; c < 16; c++) a[c] = As[(c0 + c) * LDL + c0 + i];
 #pragma unroll
   for (int j = 0; j < 16; j++) {
-    double d = __shfl(a[j], j, 64);
-    if (!(d > 0.0)) {
+    T d = __shfl(a[j], j, 64);
+    if (!(d > (T)0)) {
       bad = 1;
-      d = 1.0;
+      d = (T)1;
     }
-    const double sd = sqrt(d), inv = 1.0 / sd;
+    const T sd = sqrt(d), inv = (T)1 / sd;
     a[j] = (i == j) ? sd : (i > j ? a[j] * inv : a[j]);
 #pragma unroll
     for (int k = j + 1; k < 16; k++) {
-      const double lkj = __shfl(a[j], k, 64);
+      const T lkj = __shfl(a[j], k, 64);
       if (i >= k) a[k] -= a[j] * lkj;
     }
   }
   if (lane < 16) {
 #pragma unroll
-    for (int c = 0; c < 16; c++) As[(c0 + c) * LDL + c0 + i] = (c <= i) ? a[c] : 0.0;
+    for (int c = 0; c < 16; c++) As[(c0 + c) * LDL + c0 + i] = (c <= i) ? a[c] : (T)0;
   }
 }
 
 // inv(L) of the factored 16x16 diagonal block into Bs: lane i (< 16) solves
 // for column i by forward substitution, reading L from As (broadcast reads)
-__device__ __forceinline__ void trinv16(const double *As, double *Bs, int c0) {
+template <typename T> __device__ __forceinline__ void trinv16(const T *As, T *Bs, int c0) {
   const int i = threadIdx.x & 63;
   if (i >= 16) return;
-  double x[16];
+  T x[16];
 #pragma unroll
   for (int r = 0; r < 16; r++) {
-    double sacc = (r == i) ? 1.0 : 0.0;
+    T sacc = (r == i) ? (T)1 : (T)0;
 #pragma unroll
     for (int k = 0; k < r; k++) sacc -= As[(c0 + k) * LDL + c0 + r] * x[k];
     x[r] = sacc / As[(c0 + r) * LDL + c0 + r];
@@ -288,76 +326,78 @@ __device__ __forceinline__ void trinv16(const double *As, double *Bs, int c0) {
 // C = C - sum A B^T over the task's pairs; C = chol(C) (lower, upper zeroed);
 // X = inv(chol(C)).  Blocked by 16: diagonal blocks in registers (chol16),
 // panel solves, trailing updates and the off-diagonal blocks of the inverse
-// (X_IJ = -inv(L_II) sum_{J<=K<I} L_IK X_KJ) on the f64 MFMA.
-__global__ __launch_bounds__(NTHR) void k_tile_potrf(double *__restrict__ pool,
+// (X_IJ = -inv(L_II) sum_{J<=K<I} L_IK X_KJ) on the matrix cores.
+template <typename T>
+__global__ __launch_bounds__(NTHR) void k_tile_potrf(T *__restrict__ pool,
                                                      const TileTask *__restrict__ tasks,
                                                      const TilePair *__restrict__ pairs,
                                                      int *__restrict__ err) {
-  __shared__ double As[64 * LDL];
-  __shared__ double Bs[64 * LDL];
-  __shared__ double Ts[4][16 * 17];
+  typedef typename Mfma<T>::v4 v4;
+  __shared__ T As[64 * LDL];
+  __shared__ T Bs[64 * LDL];
+  __shared__ T Ts[4][16 * 17];
   __shared__ int bad;
   const TileTask t = tasks[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  v4d acc[2][2];
-  zero_acc(acc);
-  gemm_sum(pool, pairs, t.p0, t.p1, t.base, GEMM_NT, As, Bs, acc);
-  double *C = pool + (size_t)t.c * TILE_ELEMS;
-  form_e(C, 1.0, -1.0, acc, As);  // ends with a barrier
-  for (int e = tid; e < 64 * LDL; e += NTHR) Bs[e] = 0.0;
+  v4 acc[2][2];
+  zero_acc<T>(acc);
+  gemm_sum<T>(pool, pairs, t.p0, t.p1, t.base, GEMM_NT, As, Bs, acc);
+  T *C = pool + (size_t)t.c * TILE_ELEMS;
+  form_e<T>(C, 1.0, -1.0, acc, As);  // ends with a barrier
+  for (int e = tid; e < 64 * LDL; e += NTHR) Bs[e] = (T)0;
   if (tid == 0) bad = 0;
   __syncthreads();
-  const int orow = lane >> 4, ocol = lane & 15;  // accumulator element (orow + 4v, ocol)
+  const int ocol = lane & 15;  // accumulator element (Mfma<T>::row(lane, v), ocol)
   for (int J = 0; J < 4; J++) {
     const int c0 = 16 * J;
     // block column J -= L(:, 0:c0) L(J rows, 0:c0)^T, one row block per wave
     if (J > 0 && wave < 4 - J) {
       const int R = J + wave;
-      v4d u = (v4d){0.0, 0.0, 0.0, 0.0};
-      u = mfma16(u, As + 16 * R, 1, LDL, As + c0, 1, LDL, c0);
+      v4 u = (v4){0, 0, 0, 0};
+      u = mfma16<T>(u, As + 16 * R, 1, LDL, As + c0, 1, LDL, c0);
 #pragma unroll
-      for (int v = 0; v < 4; v++) As[(c0 + ocol) * LDL + 16 * R + orow + 4 * v] -= u[v];
+      for (int v = 0; v < 4; v++) As[(c0 + ocol) * LDL + 16 * R + Mfma<T>::row(lane, v)] -= u[v];
     }
     __syncthreads();
     if (wave == 0) {
       int b = 0;
-      chol16(As, c0, b);
+      chol16<T>(As, c0, b);
       if (b && lane == 0) bad = 1;
     }
     __syncthreads();
-    if (wave == 0) trinv16(As, Bs, c0);
+    if (wave == 0) trinv16<T>(As, Bs, c0);
     __syncthreads();
     // panel: L(R, J) = A(R, J) inv(L_JJ)^T
     if (wave < 3 - J) {
       const int R = J + 1 + wave;
-      v4d u = (v4d){0.0, 0.0, 0.0, 0.0};
-      u = mfma16(u, As + c0 * LDL + 16 * R, 1, LDL, Bs + c0 * LDL + c0, 1, LDL, 16);
+      v4 u = (v4){0, 0, 0, 0};
+      u = mfma16<T>(u, As + c0 * LDL + 16 * R, 1, LDL, Bs + c0 * LDL + c0, 1, LDL, 16);
 #pragma unroll
-      for (int v = 0; v < 4; v++) As[(c0 + ocol) * LDL + 16 * R + orow + 4 * v] = u[v];
+      for (int v = 0; v < 4; v++) As[(c0 + ocol) * LDL + 16 * R + Mfma<T>::row(lane, v)] = u[v];
     }
     __syncthreads();
   }
   // zero the strict upper triangle of L
   for (int e = tid; e < 4096; e += NTHR) {
     int r = e & 63, c = e >> 6;
-    if (r < c) As[c * LDL + r] = 0.0;
+    if (r < c) As[c * LDL + r] = (T)0;
   }
   // off-diagonal blocks of inv(L), one block row at a time
   for (int I = 1; I < 4; I++) {
     const int Jb = wave;
-    v4d u = (v4d){0.0, 0.0, 0.0, 0.0};
+    v4 u = (v4){0, 0, 0, 0};
     if (Jb < I) {
       for (int K = Jb; K < I; K++)
-        u = mfma16(u, As + (16 * K) * LDL + 16 * I, 1, LDL, Bs + (16 * Jb) * LDL + 16 * K, LDL, 1, 16);
+        u = mfma16<T>(u, As + (16 * K) * LDL + 16 * I, 1, LDL, Bs + (16 * Jb) * LDL + 16 * K, LDL, 1, 16);
 #pragma unroll
-      for (int v = 0; v < 4; v++) Ts[wave][ocol * 17 + orow + 4 * v] = u[v];
+      for (int v = 0; v < 4; v++) Ts[wave][ocol * 17 + Mfma<T>::row(lane, v)] = u[v];
     }
     __syncthreads();
     if (Jb < I) {
-      u = (v4d){0.0, 0.0, 0.0, 0.0};
-      u = mfma16(u, Bs + (16 * I) * LDL + 16 * I, 1, LDL, Ts[wave], 17, 1, 16);
+      u = (v4){0, 0, 0, 0};
+      u = mfma16<T>(u, Bs + (16 * I) * LDL + 16 * I, 1, LDL, Ts[wave], 17, 1, 16);
 #pragma unroll
-      for (int v = 0; v < 4; v++) Bs[(16 * Jb + ocol) * LDL + 16 * I + orow + 4 * v] = -u[v];
+      for (int v = 0; v < 4; v++) Bs[(16 * Jb + ocol) * LDL + 16 * I + Mfma<T>::row(lane, v)] = -u[v];
     }
     __syncthreads();
   }
@@ -438,22 +478,32 @@ void launch_zero_ranges(hipStream_t st, double *pool, const int2 *ranges, int nr
   hipLaunchKernelGGL(k_zero_ranges, dim3(64, nranges), dim3(256), 0, st, pool, ranges);
 }
 
-void launch_tile_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+template <typename T>
+void launch_tile_gemm(hipStream_t st, T *pool, const TileTask *tasks, const TilePair *pairs,
                       int ntasks, int kind, int post, double alpha, double beta) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_tile_gemm, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs, kind, post,
-                     alpha, beta);
+  hipLaunchKernelGGL(k_tile_gemm<T>, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs, kind,
+                     post, alpha, beta);
 }
+template void launch_tile_gemm<double>(hipStream_t, double *, const TileTask *, const TilePair *,
+                                       int, int, int, double, double);
+template void launch_tile_gemm<float>(hipStream_t, float *, const TileTask *, const TilePair *, int,
+                                      int, int, double, double);
 void launch_schur_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
                        int ntasks) {
   if (ntasks <= 0) return;
   hipLaunchKernelGGL(k_schur_gemm, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs);
 }
-void launch_tile_potrf(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+template <typename T>
+void launch_tile_potrf(hipStream_t st, T *pool, const TileTask *tasks, const TilePair *pairs,
                        int ntasks, int *err) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_tile_potrf, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs, err);
+  hipLaunchKernelGGL(k_tile_potrf<T>, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs, err);
 }
+template void launch_tile_potrf<double>(hipStream_t, double *, const TileTask *, const TilePair *,
+                                        int, int *);
+template void launch_tile_potrf<float>(hipStream_t, float *, const TileTask *, const TilePair *, int,
+                                       int *);
 void launch_tile_mirror(hipStream_t st, double *pool, const MirrorJob *jobs, int njobs) {
   if (njobs <= 0) return;
   hipLaunchKernelGGL(k_tile_mirror, dim3(njobs), dim3(NTHR), 0, st, pool, jobs);

@@ -83,6 +83,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ordering", type=int, default=0,
                     help="interior ordering: 0 tile-aligned nested dissection, 1 RCM")
+    ap.add_argument("--precision", type=int, default=64, choices=(32, 64),
+                    help="64: FP64 throughout; 32: FP32 factor of the augmented interface "
+                         "matrices (mixed precision)")
     ap.add_argument("--nd-leaf", type=int, default=256,
                     help="nested dissection: largest part (dofs) left whole")
     args = ap.parse_args()
@@ -115,6 +118,7 @@ def main():
                verbose_level=int(os.environ.get("BENCH_VERBOSE", "0")))
         b.set_option("interior_ordering", args.ordering)
         b.set_option("nd_leaf", args.nd_leaf)
+        b.set_option("precision", args.precision)
         t0 = time.perf_counter()
         for d in subs:
             b.upload_subdomain_data(d)
@@ -157,12 +161,15 @@ def main():
     line = {
         "metric": METRIC, "value": value, "unit": "dofs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == 64 else "f64+f32",
         "data": "synthetic",
         "config": {"workload": f"Q1 Poisson {p.nel}^3 unit cube, {p.nsub} subdomains of "
                                f"{args.nel}^3 elements, 2 levels, PCG to {args.tol:g}",
                    "dofs": p.ndof, "subdomains": p.nsub, "iterations": results[-1][0],
-                   "condition_number": round(results[-1][1], 4), "precision": "fp64",
+                   "condition_number": round(results[-1][1], 4),
+                   "precision": "fp64" if args.precision == 64 else
+                                "mixed: FP32 interface factor, FP64 Schur operator/Krylov/coarse",
                    "s_gemv_GBps": round(gemv_gbs, 1), "hbm_copy_GBps": round(copy_gbs, 1),
                    "pc_apply_GBps": round(kapply_gbs, 1)},
         "roofline": roofline,

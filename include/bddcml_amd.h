@@ -45,7 +45,11 @@ int bddcml_amd_timings(double *out, int n);
  *   "interior_ordering" (0: tile-aligned nested dissection by coordinate
  *     bisection, the default; 1: reverse Cuthill-McKee);
  *   "nd_leaf" (largest part, in dofs, that nested dissection leaves whole;
- *     default 256) */
+ *     default 256);
+ *   "precision" (64: FP64 throughout, the default; 32: the augmented
+ *     interface factor S', its triangular sweeps and Z = S'^{-1} C^T in FP32,
+ *     while the Schur operator, the Krylov method and the coarse problem
+ *     stay FP64) */
 int bddcml_amd_set_option(const char *key, double value);
 /* per subdomain (isub in numbase): out[0]=ndofi out[1]=nconstr out[2]=ndofo */
 int bddcml_amd_sub_info(int isub, int *out);
@@ -62,7 +66,7 @@ int bddcml_amd_sub_coarse(int isub, double *phi, double *ac);
 int bddcml_amd_last_error(char *buf, int len);
 /* runs one batched tile task on the device on 64x64 column-major tiles:
  * op 0 NT, 1 NN, 2 TN product sums (+10 post C X^T, +20 post X C,
- * +30 post X^T C),
+ * +30 post X^T C), op + 100: the same in FP32,
  * op 3 Cholesky of C - sum A B^T with inverse; out = result tile, and for
  * op 3 the inverse in out[4096..8191] */
 int bddcml_amd_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,

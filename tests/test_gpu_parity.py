@@ -151,3 +151,25 @@ def test_reactions_and_global_solution(kind):
         rl = b.download_local_reactions(d["isub"], d["ndofs"])
         assert np.abs(rl - rea_ref[d["isvgvn"] - 1]).max() <= 1e-9 * scale
     b.finalize()
+
+
+@pytest.mark.parametrize("nel,ns,kind,wt", [(4, 3, "poisson", 0), (3, 3, "poisson", 1),
+                                            (3, 2, "elasticity", 0)])
+def test_mixed_precision_solve(oracle_lib, nel, ns, kind, wt):
+    """Mixed mode (FP32 factor of S' and Z, FP64 Schur operator, Krylov and
+    coarse problem): iteration count within +-2 of the FP64 oracle and the
+    solution within 1e-8 relative of it at tolerance 1e-10."""
+    _say(f"case mixed nel={nel} ns={ns} kind={kind} wt={wt}")
+    p = CubeProblem(nel, ns, kind=kind, rhs="random", seed=6)
+    o, (it_o, r_o, c_o) = _oracle(oracle_lib, p, wt, tol=1e-10)
+    b, (it, r, c) = bddcml_amd.solve_cube(p, weights_type=wt, tol=1e-10, maxit=300,
+                                          options={"precision": 32})
+    assert r == r_o == 0
+    assert abs(it - it_o) <= 2
+    u_ref = o.interface_solution()
+    assert _relmax(b.interface_solution(), u_ref) < 1e-8
+    for d in p.subdomains():
+        ls = b.download_local_solution(d["isub"], d["ndofs"])
+        assert np.abs(ls - o.local_solution(d["isub"])).max() <= 1e-8 * np.abs(u_ref).max()
+    o.close()
+    b.finalize()

@@ -83,9 +83,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ordering", type=int, default=0,
                     help="interior ordering: 0 tile-aligned nested dissection, 1 RCM")
-    ap.add_argument("--precision", type=int, default=64, choices=(32, 64),
+    ap.add_argument("--precision", default="both", choices=("both", "64", "32"),
                     help="64: FP64 throughout; 32: FP32 factor of the augmented interface "
-                         "matrices (mixed precision)")
+                         "matrices (mixed precision); both: FP64 is the reported value and "
+                         "the mixed run is added to the line")
     ap.add_argument("--nd-leaf", type=int, default=256,
                     help="nested dissection: largest part (dofs) left whole")
     args = ap.parse_args()
@@ -111,39 +112,46 @@ def main():
     subs = list(p.subdomains())
     log(f"generated {p.nsub} subdomains, {p.ndof} dofs in {time.perf_counter() - t0:.1f} s")
 
-    step_times, results, tims = [], [], []
-    for step in range(args.warmup + args.steps):
-        b = bddcml_amd.Bddcml()
-        b.init(2, [p.nsub, 1], numbase=p.numbase,
-               verbose_level=int(os.environ.get("BENCH_VERBOSE", "0")))
-        b.set_option("interior_ordering", args.ordering)
-        b.set_option("nd_leaf", args.nd_leaf)
-        b.set_option("precision", args.precision)
-        t0 = time.perf_counter()
-        for d in subs:
-            b.upload_subdomain_data(d)
-        t_up = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        b.setup_preconditioner(1, 0, 1, 1, 1, 0, 0, 0)
-        t1 = time.perf_counter()
-        it, reason, cond = b.solve(0, args.tol, 500, 50)
-        t2 = time.perf_counter()
-        tm = b.timings()
-        hist = b.residual_history()
-        b.finalize()
-        if reason != 0:
-            raise SystemExit(f"PCG did not converge: reason {reason}")
-        log(f"step {step}: upload {t_up:.2f} s setup {t1 - t0:.3f} s solve {t2 - t1:.3f} s "
-            f"iters {it} cond {cond:.4f} final relres {hist[-1]:.3e}")
-        log("  " + json.dumps({k: round(v, 4) for k, v in tm.items()}))
-        if step >= args.warmup:
-            step_times.append(t2 - t0)
-            results.append((it, cond))
-            tims.append(tm)
+    def run(precision):
+        """warmup + timed steps of one precision mode; each step is a fresh
+        init + upload (untimed) and setup_preconditioner + solve (timed)"""
+        step_times, results, tims = [], [], []
+        for step in range(args.warmup + args.steps):
+            b = bddcml_amd.Bddcml()
+            b.init(2, [p.nsub, 1], numbase=p.numbase,
+                   verbose_level=int(os.environ.get("BENCH_VERBOSE", "0")))
+            b.set_option("interior_ordering", args.ordering)
+            b.set_option("nd_leaf", args.nd_leaf)
+            b.set_option("precision", precision)
+            t0 = time.perf_counter()
+            for d in subs:
+                b.upload_subdomain_data(d)
+            t_up = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            b.setup_preconditioner(1, 0, 1, 1, 1, 0, 0, 0)
+            t1 = time.perf_counter()
+            it, reason, cond = b.solve(0, args.tol, 500, 50)
+            t2 = time.perf_counter()
+            tm = b.timings()
+            hist = b.residual_history()
+            b.finalize()
+            if reason != 0:
+                raise SystemExit(f"PCG did not converge: reason {reason}")
+            log(f"[fp{precision}] step {step}: upload {t_up:.2f} s setup {t1 - t0:.3f} s "
+                f"solve {t2 - t1:.3f} s iters {it} cond {cond:.4f} final relres {hist[-1]:.3e}")
+            log("  " + json.dumps({k: round(v, 4) for k, v in tm.items()}))
+            if step >= args.warmup:
+                step_times.append(t2 - t0)
+                results.append((it, cond, hist[-1]))
+                tims.append(tm)
+        ms = 1000.0 * float(np.mean(step_times))
+        return ms, results[-1], tims[-1]
 
-    ms = 1000.0 * float(np.mean(step_times))
+    modes = [64, 32] if args.precision == "both" else [int(args.precision)]
+    runs = {m: run(m) for m in modes}
+    head = modes[0]
+    ms, (iters, cond, relres), tm = runs[head]
     value = p.ndof / (ms / 1000.0)
-    tm = tims[-1]
     schur_tf = tm["schur_flops"] / (tm["ev_schur_ms"] * 1e-3) / 1e12
     gemv_gbs = tm["gemv_bytes"] / (tm["ev_gemv_ms"] * 1e-3) / 1e9 if tm["ev_gemv_ms"] > 0 else 0.0
     interior_tf = tm["interior_flops"] / (tm["ev_interior_ms"] * 1e-3) / 1e12
@@ -162,18 +170,28 @@ def main():
         "metric": METRIC, "value": value, "unit": "dofs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == 64 else "f64+f32",
+        "dtype": "f64" if head == 64 else "f64+f32",
         "data": "synthetic",
         "config": {"workload": f"Q1 Poisson {p.nel}^3 unit cube, {p.nsub} subdomains of "
                                f"{args.nel}^3 elements, 2 levels, PCG to {args.tol:g}",
-                   "dofs": p.ndof, "subdomains": p.nsub, "iterations": results[-1][0],
-                   "condition_number": round(results[-1][1], 4),
-                   "precision": "fp64" if args.precision == 64 else
+                   "dofs": p.ndof, "subdomains": p.nsub, "iterations": iters,
+                   "condition_number": round(cond, 4), "final_relres": relres,
+                   "precision": "fp64" if head == 64 else
                                 "mixed: FP32 interface factor, FP64 Schur operator/Krylov/coarse",
                    "s_gemv_GBps": round(gemv_gbs, 1), "hbm_copy_GBps": round(copy_gbs, 1),
                    "pc_apply_GBps": round(kapply_gbs, 1)},
         "roofline": roofline,
     }
+    if head == 64 and 32 in runs:
+        ms32, (it32, c32, rr32), tm32 = runs[32]
+        line["mixed"] = {"value": p.ndof / (ms32 / 1000.0), "unit": "dofs/s", "ms_per_step": ms32,
+                         "dtype": "f64+f32", "iterations": it32, "final_relres": rr32,
+                         "interface_phase_ms": round(tm32["ev_interface_ms"], 2),
+                         "pc_apply_ms": round(tm32["ev_kapply_ms"], 3),
+                         "note": "FP32 factor of S' = S + C^T W C, its sweeps and Z; "
+                                 "Schur operator, Krylov and coarse problem FP64"}
+        log(f"mixed precision: {ms32:.1f} ms/step ({p.ndof / (ms32 / 1000.0):.3e} dofs/s), "
+            f"{it32} iterations, final relres {rr32:.3e}")
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline()

@@ -171,8 +171,6 @@ struct Device {
   DBuf<int64_t> ca_dst, ca_src;
   DBuf<int> ca_ptr;
   int nca = 0;
-  DBuf<int64_t> cloff;
-  DBuf<int> cNI;
   // triangular sweeps
   std::vector<TrsvDesc> htrsv;
   DBuf<TrsvDesc> trsv;
@@ -921,6 +919,14 @@ void device_setup_finish(Solver &S) {
   if (two_constraints)
     fatal("BDDCML_SETUP_PRECONDITIONER", "a dof belongs to two constraints; only corner and "
                                          "arithmetic constraints are supported");
+  // limits of the kernels: G^{-1} in place with one LDS row buffer (k_ginv),
+  // the whole interface vector of a subdomain in LDS (k_kapply), the coarse
+  // matrix as full tiles
+  if (D.maxnc > 1024) fatal("BDDCML_SETUP_PRECONDITIONER", "more than 1024 constraints per subdomain");
+  if ((size_t)(D.maxNI + 1024 + 64 + D.maxnc) * sizeof(double) > max_dynamic_lds())
+    fatal("BDDCML_SETUP_PRECONDITIONER", "subdomain interface too large for the preconditioner apply");
+  if ((int64_t)D.ntc * D.ntc > ((int64_t)1 << 24))
+    fatal("BDDCML_SETUP_PRECONDITIONER", "coarse problem too large for its dense factorization");
   D.LV = lv;
   D.desc.upload(D.hdesc, st);
   D.loff.upload(loff, st);
@@ -950,8 +956,6 @@ void device_setup_finish(Solver &S) {
   D.crows.upload(cr, st);
   D.nsrows = (int)sr.size();
   D.ncrows = (int)cr.size();
-  D.cloff.upload(std::vector<int64_t>{0}, st);
-  D.cNI.upload(std::vector<int>{D.NC}, st);
   ph("interface maps");
   // assembly CSR: unique interface dof -> local flat indices (subdomain order)
   {
@@ -1363,8 +1367,7 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed = false) {
   // coarse correction uc = Ac^{-1} sum_i R_ci^T Phi_i^T W_i r_i
   launch_phit(st, D.desc.p, S.nsub, D.Phi.p, D.map.p, D.w.p, r, D.rcs.p, D.maxnc);
   launch_coarse_gather(st, S.ndofc, D.cgptr.p, D.cgsrc.p, D.rcs.p, D.rc.p);
-  launch_gemv(st, D.pool.p, D.crows.p, D.ncrows, D.cloff.p, D.cNI.p, nullptr, nullptr, D.rc.p,
-              D.uc.p, D.NC);
+  launch_gemv_plain(st, D.pool.p, D.crows.p, D.ncrows, D.rc.p, D.uc.p);
   // zl_i = K_i W_i r_i + Phi_i uc(indrowc), then z = sum_i R_i^T W_i zl_i
   if (timed) {
     HIPCHECK(hipEventCreate(&e0));

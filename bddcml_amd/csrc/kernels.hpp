@@ -137,6 +137,9 @@ struct GemvRow {
 void launch_gemv(hipStream_t st, const double *pool, const GemvRow *rows, int nrows,
                  const int64_t *loff, const int *NI, const int *map, const double *w,
                  const double *xin, double *yout, int maxNI);
+// y[I*64 + r] = sum_J T(I,J) x_J with x read directly (coarse inverse)
+void launch_gemv_plain(hipStream_t st, const double *pool, const GemvRow *rows, int nrows,
+                       const double *xin, double *yout);
 // unique y[g] = sum_{q} (w? w[src]:1) * (ylocal[src] + (phiu? phiu[src]:0))
 void launch_assemble(hipStream_t st, int n, const int *ptr, const int64_t *src, const double *w,
                      const double *ylocal, double *y);

@@ -372,6 +372,38 @@ __global__ __launch_bounds__(256) void k_gemv(const double *__restrict__ pool,
     yout[lo + row.I * 64 + r] = (red[r] + red[64 + r]) + (red[128 + r] + red[192 + r]);
 }
 
+// dense GEMV over full-storage tile rows with the input read from global
+// memory (no staging; for the coarse inverse, whose input is one short
+// vector shared by every row)
+__global__ __launch_bounds__(256) void k_gemv_plain(const double *__restrict__ pool,
+                                                    const GemvRow *__restrict__ rows,
+                                                    const double *__restrict__ xin,
+                                                    double *__restrict__ yout) {
+  __shared__ double red[256];
+  const GemvRow row = rows[blockIdx.x];
+  const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
+  double s0 = 0.0, s1 = 0.0;
+  for (int J = g; J < row.nt; J += 4) {
+    const double *T = pool + (size_t)(row.tile0 + J) * TILE_ELEMS + r;
+    const double *x = xin + J * 64;
+#pragma unroll 8
+    for (int c = 0; c < 64; c += 2) {
+      s0 += T[c * 64] * x[c];
+      s1 += T[(c + 1) * 64] * x[c + 1];
+    }
+  }
+  red[threadIdx.x] = s0 + s1;
+  __syncthreads();
+  if (threadIdx.x < 64)
+    yout[row.I * 64 + r] = (red[r] + red[64 + r]) + (red[128 + r] + red[192 + r]);
+}
+
+void launch_gemv_plain(hipStream_t st, const double *pool, const GemvRow *rows, int nrows,
+                       const double *xin, double *yout) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_gemv_plain, dim3(nrows), dim3(256), 0, st, pool, rows, xin, yout);
+}
+
 __global__ void k_assemble(int n, const int *__restrict__ ptr, const int64_t *__restrict__ src,
                            const double *__restrict__ w, const double *__restrict__ yl,
                            double *__restrict__ y) {
@@ -529,8 +561,10 @@ void launch_gemv(hipStream_t st, const double *pool, const GemvRow *rows, int nr
                  const int64_t *loff, const int *NI, const int *map, const double *w,
                  const double *xin, double *yout, int maxNI) {
   if (nrows <= 0) return;
-  hipLaunchKernelGGL(k_gemv, dim3(nrows), dim3(256), (size_t)maxNI * sizeof(double), st, pool,
-                     rows, loff, NI, map, w, xin, yout);
+  const size_t sh = (size_t)maxNI * sizeof(double);
+  lds_request((const void *)k_gemv, sh + 256 * sizeof(double), "interface GEMV");
+  hipLaunchKernelGGL(k_gemv, dim3(nrows), dim3(256), sh, st, pool, rows, loff, NI, map, w, xin,
+                     yout);
 }
 void launch_assemble(hipStream_t st, int n, const int *ptr, const int64_t *src, const double *w,
                      const double *ylocal, double *y) {

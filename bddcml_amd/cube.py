@@ -140,19 +140,24 @@ class CubeProblem:
         n = self.nel + 1
         return ig + jg * n + kg * n * n
 
+    def box(self, isub0: int):
+        """Element box (x0, y0, z0, nx, ny, nz) of subdomain isub0."""
+        nes = self.nel_sub
+        sx, sy, sz = self.sub_indices(isub0)
+        return sx * nes, sy * nes, sz * nes, nes, nes, nes
+
     def subdomain(self, isub0: int) -> dict:
         """Arguments of bddcml_upload_subdomain_data_c for subdomain isub0
         (0-based; the returned isub carries numbase)."""
         b = self.numbase
-        nes = self.nel_sub
-        nn = nes + 1
-        sx, sy, sz = self.sub_indices(isub0)
-        kk, jj, ii = np.meshgrid(np.arange(nn), np.arange(nn), np.arange(nn), indexing="ij")
-        ig = (sx * nes + ii).ravel()
-        jg = (sy * nes + jj).ravel()
-        kg = (sz * nes + kk).ravel()
+        x0, y0, z0, nx, ny, nz = self.box(isub0)
+        mx, my, mz = nx + 1, ny + 1, nz + 1
+        kk, jj, ii = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+        ig = (x0 + ii).ravel()
+        jg = (y0 + jj).ravel()
+        kg = (z0 + kk).ravel()
         isngn = self.global_node(ig, jg, kg)
-        nnods = nn ** 3
+        nnods = mx * my * mz
         ndf = self.ndf
         ndofs = nnods * ndf
         xyz = np.empty((nnods, 3))
@@ -167,16 +172,16 @@ class CubeProblem:
         if self.dirichlet == "random":
             fixv = np.where(ifix > 0, 2.0 * uniform01(self.seed + 1000, isvgvn) - 1.0, 0.0)
         # elements
-        ek, ej, ei = np.meshgrid(np.arange(nes), np.arange(nes), np.arange(nes), indexing="ij")
+        ek, ej, ei = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
         ei, ej, ek = ei.ravel(), ej.ravel(), ek.ravel()
-        n1 = ei + ej * nn + ek * nn * nn
+        n1 = ei + ej * mx + ek * mx * my
         n2 = n1 + 1
-        n3 = n2 + nn
+        n3 = n2 + mx
         n4 = n3 - 1
-        inet = np.stack([n1, n2, n3, n4, n1 + nn * nn, n2 + nn * nn, n3 + nn * nn,
-                         n4 + nn * nn], axis=1)
+        pl = mx * my
+        inet = np.stack([n1, n2, n3, n4, n1 + pl, n2 + pl, n3 + pl, n4 + pl], axis=1)
         nelems = inet.shape[0]
-        isegn = (sx * nes + ei) + (sy * nes + ej) * nb + (sz * nes + ek) * nb * nb
+        isegn = (x0 + ei) + (y0 + ej) * nb + (z0 + ek) * nb * nb
         # element dofs (node-major)
         edofs = (inet[:, :, None] * ndf + np.arange(ndf)[None, None, :]).reshape(nelems, -1)
         ii_, jj_ = self._iu, self._ju
@@ -248,3 +253,65 @@ class CubeProblem:
         A[fixed, fixed] = dg[fixed]
         f[fixed] = dg[fixed] * g[fixed]
         return A, f, fixed
+
+
+class BoxProblem(CubeProblem):
+    """Unit cube of n^3 Q1 hexahedra split into nsub boxes of uneven size by
+    seeded random bisection (the box with the most elements is cut across its
+    longest extent at a fraction uniform in [0.25, 0.75]), with `layers` stiff
+    slabs: each is `thick` elements across, normal to a random axis at a
+    random position, and multiplies the element stiffness by `jump`.  The
+    default is 3D linear elasticity with every boundary dof fixed."""
+
+    def __init__(self, n: int, nsub: int, seed: int = 3, kind: str = "elasticity",
+                 layers: int = 27, jump: float = 1e4, thick: int = 0, rhs: str = "random",
+                 numbase: int = 1, dirichlet: str = "zero", E: float = 1.0, nu: float = 0.3):
+        self.layer_seed = seed
+        self.nlayers = layers
+        self.jump = jump
+        self.thick = thick if thick > 0 else max(1, n // 48)
+        super().__init__(n, 1, kind=kind, rhs=rhs, seed=seed, numbase=numbase,
+                         coef=self._coef if layers > 0 else None, E=E, nu=nu,
+                         dirichlet=dirichlet)
+        self.nsub = nsub
+        self.boxes = self._split(n, nsub, seed)
+        self.nsub = len(self.boxes)
+
+    @staticmethod
+    def _split(n, nsub, seed):
+        boxes = [(0, 0, 0, n, n, n)]
+        k = 0
+        while len(boxes) < nsub:
+            i = max(range(len(boxes)), key=lambda j: boxes[j][3] * boxes[j][4] * boxes[j][5])
+            x0, y0, z0, nx, ny, nz = boxes[i]
+            dims = [nx, ny, nz]
+            ax = int(np.argmax(dims))
+            L = dims[ax]
+            if L < 2:
+                break
+            u = float(uniform01(seed * 7919 + 17, k))
+            k += 1
+            cut = min(L - 1, max(1, int(round((0.25 + 0.5 * u) * L))))
+            lo = list(boxes[i])
+            hi = list(boxes[i])
+            lo[3 + ax] = cut
+            hi[ax] += cut
+            hi[3 + ax] = L - cut
+            boxes[i] = tuple(lo)
+            boxes.append(tuple(hi))
+        return boxes
+
+    def _coef(self, e):
+        n = self.nel
+        e = np.asarray(e)
+        idx = (e % n, (e // n) % n, e // (n * n))
+        c = np.ones(e.shape)
+        for l in range(self.nlayers):
+            ax = int(3 * uniform01(self.layer_seed * 104729 + 3, 2 * l))
+            pos = int(n * uniform01(self.layer_seed * 104729 + 3, 2 * l + 1))
+            inl = (idx[ax] >= pos) & (idx[ax] < pos + self.thick)
+            c[inl] = self.jump
+        return c
+
+    def box(self, isub0: int):
+        return self.boxes[isub0]

@@ -58,7 +58,7 @@ def schur_traffic(args):
     """HBM bytes per k_schur_gemm launch from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of the default workload (null for other
     workloads or when the summary is absent)."""
-    if (args.nel, args.nsub, args.ordering, args.nd_leaf) != (16, 8, 0, 256):
+    if (args.workload, args.nel, args.nsub, args.ordering, args.nd_leaf) != ("cube", 16, 8, 0, 256):
         return None
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "k_schur_gemm_128cube_pmc.md")
@@ -77,6 +77,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cube", choices=("cube", "boxes"),
+                    help="cube: Q1 Poisson on a cube split into equal cubes (the reported "
+                         "workload); boxes: Q1 elasticity with stiff layers on uneven "
+                         "subdomains from seeded random box splits")
+    ap.add_argument("--box-n", type=int, default=96, help="boxes: elements per cube edge")
+    ap.add_argument("--box-nsub", type=int, default=400, help="boxes: number of subdomains")
     ap.add_argument("--nel", type=int, default=16, help="elements per subdomain edge")
     ap.add_argument("--nsub", type=int, default=8, help="subdomains per cube edge")
     ap.add_argument("--tol", type=float, default=1e-8)
@@ -107,10 +113,22 @@ def main():
     log(f"k_copy_calibration: {copy_bytes} bytes read + written in {copy_ms:.3f} ms "
         f"({copy_gbs:.0f} GB/s)")
 
-    p = CubeProblem(args.nel, args.nsub, rhs="random", seed=2)
     t0 = time.perf_counter()
+    if args.workload == "cube":
+        p = CubeProblem(args.nel, args.nsub, rhs="random", seed=2)
+        workload = (f"Q1 Poisson {p.nel}^3 unit cube, {p.nsub} subdomains of {args.nel}^3 "
+                    f"elements, 2 levels, PCG to {args.tol:g}")
+    else:
+        from bddcml_amd.cube import BoxProblem
+        p = BoxProblem(args.box_n, args.box_nsub, seed=3, kind="elasticity", layers=27,
+                       jump=1e4)
+        workload = (f"Q1 elasticity (E=1, nu=0.3) {p.nel}^3 unit cube, 27 seeded layers E=1e4, "
+                    f"{p.nsub} subdomains from seeded random box splits, 2 levels, PCG to "
+                    f"{args.tol:g}")
     subs = list(p.subdomains())
-    log(f"generated {p.nsub} subdomains, {p.ndof} dofs in {time.perf_counter() - t0:.1f} s")
+    sizes = [d["ndofs"] for d in subs]
+    log(f"generated {p.nsub} subdomains ({min(sizes)}..{max(sizes)} dofs), {p.ndof} dofs in "
+        f"{time.perf_counter() - t0:.1f} s")
 
     def run(precision):
         """warmup + timed steps of one precision mode; each step is a fresh
@@ -130,7 +148,7 @@ def main():
             t0 = time.perf_counter()
             b.setup_preconditioner(1, 0, 1, 1, 1, 0, 0, 0)
             t1 = time.perf_counter()
-            it, reason, cond = b.solve(0, args.tol, 500, 50)
+            it, reason, cond = b.solve(0, args.tol, 2000, 200)
             t2 = time.perf_counter()
             tm = b.timings()
             hist = b.residual_history()
@@ -172,9 +190,9 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if head == 64 else "f64+f32",
         "data": "synthetic",
-        "config": {"workload": f"Q1 Poisson {p.nel}^3 unit cube, {p.nsub} subdomains of "
-                               f"{args.nel}^3 elements, 2 levels, PCG to {args.tol:g}",
+        "config": {"workload": workload,
                    "dofs": p.ndof, "subdomains": p.nsub, "iterations": iters,
+                   "subdomain_dofs": [min(sizes), max(sizes)],
                    "condition_number": round(cond, 4), "final_relres": relres,
                    "precision": "fp64" if head == 64 else
                                 "mixed: FP32 interface factor, FP64 Schur operator/Krylov/coarse",
@@ -192,7 +210,7 @@ def main():
                                  "Schur operator, Krylov and coarse problem FP64"}
         log(f"mixed precision: {ms32:.1f} ms/step ({p.ndof / (ms32 / 1000.0):.3e} dofs/s), "
             f"{it32} iterations, final relres {rr32:.3e}")
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == "cube":
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline()
         log(f"cpu baseline took {time.perf_counter() - t0:.1f} s")

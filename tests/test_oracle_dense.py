@@ -204,3 +204,20 @@ def test_nonzero_dirichlet_matches_direct_solve(oracle_lib, nel, ns):
         fx = d["ifix"] > 0
         assert np.abs(ls[fx] - d["fixv"][fx]).max() <= 1e-12
     o.close()
+
+
+@pytest.mark.parametrize("n,ns,kind", [(6, 5, "elasticity"), (6, 7, "poisson")])
+def test_uneven_boxes_with_stiff_layers(oracle_lib, n, ns, kind):
+    """Subdomains of uneven size from random box splits, with coefficient
+    jumps (stiff slabs), against the dense solve."""
+    from bddcml_amd.cube import BoxProblem
+    p = BoxProblem(n, ns, seed=3, kind=kind, layers=3, jump=100.0, thick=1)
+    o, (it, reason, cond) = oracle_lib.solve_cube(p, tol=1e-12, maxit=500)
+    A, f, fixed = p.global_dense()
+    u = np.linalg.solve(A, f)
+    assert reason == 0
+    scale = np.abs(u).max()
+    assert np.abs(o.interface_solution() - u[o.interface_dofs()]).max() <= 1e-10 * scale
+    for d in p.subdomains():
+        assert np.abs(o.local_solution(d["isub"]) - u[d["isvgvn"] - 1]).max() <= 1e-10 * scale
+    o.close()

@@ -116,7 +116,7 @@ struct SubTiles {
   // factor space (S' factor, its inverse diagonal tiles, Z): the FP64 pool,
   // or the FP32 pool in the mixed-precision mode
   int fbase = 0, fspan = 0;
-  int linvA = 0, sfull = 0, pbase = 0, linvP = 0, ybase = 0, ncT = 0;
+  int linvA = 0, sfull = 0, pbase = 0, linvP = 0, ybase = 0, ncT = 0, gtbase = 0, phibase = 0;
   int P(int I, int J) const { return pbase + lowidx(nti, I, J); }
   int Y(int I, int cT) const { return ybase + I * ncT + cT; }
 };
@@ -142,7 +142,7 @@ struct Device {
   std::vector<SubTiles> sub;
   // setup phases
   std::vector<Phase> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd;
-  Phase a_schur, cinv;
+  Phase a_schur, cinv, phi;
   std::vector<SeqPhase> cseq;  // coarse factorization and inverse
   DBuf<int> zero_ids, sc_perm, sc_ttab;
   DBuf<char> sc_fix;
@@ -159,7 +159,7 @@ struct Device {
   DBuf<int64_t> loff;
   DBuf<int> NIs, map, cid, csize, cptr, cdof, cmap;
   DBuf<double> w, Z, Phi, G, Acs;
-  int maxNI = 0, maxnc = 0, maxnti = 0, maxnt = 1;
+  int maxNI = 0, maxnc = 0, maxnti = 0, maxnt = 1, maxncT = 0;
   int64_t LV = 0;
   DBuf<GemvRow> srows, crows;
   int nsrows = 0, ncrows = 0;
@@ -292,6 +292,10 @@ static void assign_tiles(SubTiles &T, int nc, int &next, int &next32, bool mixed
   nf += T.nti;
   T.ncT = (nc + TILE - 1) / TILE;
   T.ybase = nf;
+  nf += T.nti * T.ncT;
+  T.gtbase = nf;
+  nf += T.ncT * T.ncT;
+  T.phibase = nf;
   nf += T.nti * T.ncT;
   T.span = next - T.base;
   if (mixed) {
@@ -427,7 +431,7 @@ void device_setup_begin(Solver &S) {
   const int ncls = (int)reps.size();
   struct SubTasks {
     std::vector<TaskList> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd;
-    TaskList a_schur;
+    TaskList a_schur, phi;
   };
   std::vector<SubTasks> sts(ncls);
   int bad_ids = 0;
@@ -509,6 +513,15 @@ void device_setup_begin(Solver &S) {
         u.p1 = (int)B.pairs.size();
         B.tasks.push_back(u);
       }
+    // Phi = Z G^{-1} on tiles: task (I, CT) sums Z(I, KT) Ginv(KT, CT)
+    for (int I = 0; I < n; I++)
+      for (int CT = 0; CT < T.ncT; CT++) {
+        TileTask t{T.phibase + I * T.ncT + CT, -1, (int)Q.phi.pairs.size(), 0};
+        for (int KT = 0; KT < T.ncT; KT++)
+          Q.phi.pairs.push_back({T.Y(I, KT), T.gtbase + KT * T.ncT + CT});
+        t.p1 = (int)Q.phi.pairs.size();
+        Q.phi.tasks.push_back(t);
+      }
     // make every tile id relative to the subdomain's first tile, checking
     // that it stays inside the subdomain's tiles
     auto relative = [&](TaskList &L, int b0, int nb) {
@@ -531,6 +544,7 @@ void device_setup_begin(Solver &S) {
     relative(Q.a_schur, T.base, T.span);
     for (auto *v : {&Q.p_diag, &Q.p_off, &Q.z_fwd, &Q.z_bwd})
       for (TaskList &L : *v) relative(L, T.fbase, T.fspan);
+    relative(Q.phi, T.fbase, T.fspan);
   }
   if (bad_ids) fatal("BDDCML_DEVICE_PLAN", "tile task outside its subdomain's tiles");
   for (int is = 0; is < nsub; is++)
@@ -584,6 +598,7 @@ void device_setup_begin(Solver &S) {
                         z_bwd = merge_steps(&SubTasks::z_bwd, true);
   TaskList a_schur =
       materialize([](const SubTasks &Q) -> const TaskList * { return &Q.a_schur; }, false);
+  TaskList phi = materialize([](const SubTasks &Q) -> const TaskList * { return &Q.phi; }, true);
   sts.clear();
   if (S.verbose >= 2)
     std::printf("   device plan: %d structure classes over %d subdomains, %zu interior steps\n", ncls,
@@ -656,6 +671,7 @@ void device_setup_begin(Solver &S) {
     q.potrf = cseq[k].potrf;
   }
   D.a_schur.upload(a_schur, st);
+  D.phi.upload(phi, st);
   D.cinv.upload(cinv, st);
 
   ph("task lists");
@@ -804,7 +820,8 @@ void device_setup_begin(Solver &S) {
     };
     S.t.schur_flops = g * products(a_schur);
     S.t.interior_flops = work(a_diag, 2 * c) + work(a_off, g) + S.t.schur_flops;
-    S.t.interface_flops = work(p_diag, 2 * c) + work(p_off, g) + work(z_fwd, g) + work(z_bwd, g);
+    S.t.interface_flops =
+        work(p_diag, 2 * c) + work(p_off, g) + work(z_fwd, g) + work(z_bwd, g) + g * products(phi);
   }
   // ---- numerics, first half: Dirichlet elimination into the tiles,
   // interior factorization, Schur complements
@@ -859,6 +876,8 @@ void device_setup_finish(Solver &S) {
     d.pbase = T.pbase;
     d.plinv = T.linvP;
     d.ybase = T.ybase;
+    d.gtbase = T.gtbase;
+    d.phibase = T.phibase;
     d.ncT = T.ncT;
     d.loff = lv;
     d.zoff = zo;
@@ -876,6 +895,7 @@ void device_setup_finish(Solver &S) {
     D.maxNI = std::max(D.maxNI, s.NI);
     D.maxnc = std::max(D.maxnc, s.nconstr);
     D.maxnti = std::max(D.maxnti, s.nti);
+    D.maxncT = std::max(D.maxncT, T.ncT);
   }
   std::vector<int> map(lv), cidv(lv), csz(co), cptrv(co + 1, 0), cdofv(cdoff[nsub]), cmapv(co);
   std::vector<double> wv(lv);
@@ -1148,13 +1168,21 @@ void device_setup_finish(Solver &S) {
     for (size_t i = 0; i < D.z_bwd.size(); i++)
       launch_tile_gemm(st, fp, D.z_bwd[i].tasks.p, D.z_bwd[i].pairs.p, D.z_bwd[i].ntask, GEMM_TN,
                        POST_LEFT_XT, 1.0, -1.0);
-    launch_y_to_z(st, fp, D.desc.p, nsub, D.Z.p, D.maxNI);
+    launch_y_to_z(st, fp, D.desc.p, nsub, D.Z.p, D.maxNI, 0);
+  };
+  // Phi = Z G^{-1} on the tile GEMM, in the factor space
+  auto phi_phase = [&](auto *fp) {
+    launch_ginv_to_tiles(st, fp, D.desc.p, nsub, D.G.p, D.maxncT);
+    launch_tile_gemm(st, fp, D.phi.tasks.p, D.phi.pairs.p, D.phi.ntask, GEMM_NN, POST_NONE, 0.0,
+                     1.0);
+    launch_y_to_z(st, fp, D.desc.p, nsub, D.Phi.p, D.maxNI, 1);
   };
   if (D.mixed) factor_phase(D.pool32.p);
   else factor_phase(pool);
   launch_g(st, D.desc.p, nsub, D.cptr.p, D.cdof.p, D.Z.p, D.G.p);
   launch_ginv(st, D.desc.p, nsub, D.G.p, nullptr, D.err.p);
-  launch_phi(st, D.desc.p, nsub, D.Z.p, D.G.p, D.Phi.p);
+  if (D.mixed) phi_phase(D.pool32.p);
+  else phi_phase(pool);
   launch_acs(st, D.desc.p, nsub, D.cptr.p, D.G.p, D.Acs.p);
   HIPCHECK(hipEventRecord(ev[3], st));
   // coarse problem

@@ -59,6 +59,8 @@ struct SubDesc {
   int plinv;              // first of the nti inverse diagonal factor tiles (factor pool)
   int ybase;              // first tile of Z = S'^{-1} C^T (nti x ncT, row-major; factor pool)
   int ncT;                // constraint tile columns, ceil(nc/64)
+  int gtbase;             // G^{-1} as ncT x ncT tiles (factor pool)
+  int phibase;            // Phi = Z G^{-1} as nti x ncT tiles (factor pool)
   int64_t loff;           // offset into local interface vectors
   int64_t zoff;           // offset into Z / Phi (NI x nc column major)
   int64_t goff;           // offset into G / Ginv (nc x nc)
@@ -76,17 +78,21 @@ void launch_sprime(hipStream_t st, const double *pool, T *ppool, const SubDesc *
 template <typename T>
 void launch_ct_init(hipStream_t st, T *pool, const SubDesc *subs, int nsub, const int *cid,
                     int maxnti);
-// Z (column major NI x nc at zoff) from the solved tile block
+// Z (which 0) or Phi (which 1), column major NI x nc at zoff, from the
+// tile block at ybase or phibase
 template <typename T>
 void launch_y_to_z(hipStream_t st, const T *pool, const SubDesc *subs, int nsub, double *Z,
-                   int maxNI);
-// G = C Z, Ginv = G^{-1} (in place Gauss-Jordan, SPD), Phi = Z Ginv,
-// Acs = Ginv - diag(rho/|D_c|)
+                   int maxNI, int which);
+// G^{-1} into the ncT x ncT tile block at gtbase (zero padded), the right
+// operand of Phi = Z G^{-1} on the tile GEMM
+template <typename T>
+void launch_ginv_to_tiles(hipStream_t st, T *pool, const SubDesc *subs, int nsub, const double *G,
+                          int maxncT);
+// G = C Z, Ginv = G^{-1} (in place Gauss-Jordan, SPD), Acs = Ginv - diag(rho/|D_c|)
 void launch_g(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const int *cdof,
               const double *Z, double *G);
 void launch_ginv(hipStream_t st, const SubDesc *subs, int nsub, double *G, double *work, int *err);
-void launch_phi(hipStream_t st, const SubDesc *subs, int nsub, const double *Z, const double *G,
-                double *Phi);
+
 void launch_acs(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const double *G,
                 double *Acs);
 // coarse assembly: dst[k] = sum_{q in ptr[k]..ptr[k+1]} Acs[src[q]] into pool

@@ -62,11 +62,12 @@ __global__ void k_ct_init(T *__restrict__ pool, const SubDesc *__restrict__ subs
 
 template <typename T>
 __global__ void k_y_to_z(const T *__restrict__ pool, const SubDesc *__restrict__ subs,
-                         double *__restrict__ Z) {
+                         double *__restrict__ Z, int which) {
   const SubDesc d = subs[blockIdx.y];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.NI) return;
-  const T *Y = pool + (size_t)(d.ybase + (i >> 6) * d.ncT) * TILE_ELEMS + (i & 63);
+  const int b0 = which == 0 ? d.ybase : d.phibase;
+  const T *Y = pool + (size_t)(b0 + (i >> 6) * d.ncT) * TILE_ELEMS + (i & 63);
   for (int c = 0; c < d.nc; c++)
     Z[d.zoff + (size_t)c * d.NI + i] = (double)Y[(size_t)(c >> 6) * TILE_ELEMS + (c & 63) * 64];
 }
@@ -119,19 +120,6 @@ __global__ void k_ginv(const SubDesc *__restrict__ subs, double *__restrict__ G,
   }
 }
 
-__global__ void k_phi(const SubDesc *__restrict__ subs, const double *__restrict__ Z,
-                      const double *__restrict__ G, double *__restrict__ Phi) {
-  const SubDesc d = subs[blockIdx.y];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.NI) return;
-  const int nc = d.nc;
-  for (int c = 0; c < nc; c++) {
-    double s = 0.0;
-    for (int k = 0; k < nc; k++) s += Z[d.zoff + (size_t)k * d.NI + i] * G[d.goff + (size_t)c * nc + k];
-    Phi[d.zoff + (size_t)c * d.NI + i] = s;
-  }
-}
-
 __global__ void k_acs(const SubDesc *__restrict__ subs, const int *__restrict__ cptr,
                       const double *__restrict__ G, double *__restrict__ Acs) {
   const SubDesc d = subs[blockIdx.x];
@@ -162,14 +150,43 @@ void launch_ct_init(hipStream_t st, T *pool, const SubDesc *subs, int nsub, cons
 }
 template <typename T>
 void launch_y_to_z(hipStream_t st, const T *pool, const SubDesc *subs, int nsub, double *Z,
-                   int maxNI) {
+                   int maxNI, int which) {
   if (nsub <= 0 || maxNI <= 0) return;
-  hipLaunchKernelGGL(k_y_to_z<T>, dim3((maxNI + 255) / 256, nsub), dim3(256), 0, st, pool, subs, Z);
+  hipLaunchKernelGGL(k_y_to_z<T>, dim3((maxNI + 255) / 256, nsub), dim3(256), 0, st, pool, subs, Z,
+                     which);
 }
+
+// G^{-1} (nc x nc) into the ncT x ncT tile block at gtbase, zero padded
+template <typename T>
+__global__ void k_ginv_to_tiles(T *__restrict__ pool, const SubDesc *__restrict__ subs,
+                                const double *__restrict__ G) {
+  const SubDesc d = subs[blockIdx.y];
+  const int t = blockIdx.x;
+  if (t >= d.ncT * d.ncT) return;
+  const int KT = t / d.ncT, CT = t % d.ncT;
+  T *dst = pool + (size_t)(d.gtbase + t) * TILE_ELEMS;
+  for (int e = threadIdx.x; e < TILE_ELEMS; e += blockDim.x) {
+    const int k = KT * 64 + (e & 63), c = CT * 64 + (e >> 6);
+    dst[e] = (k < d.nc && c < d.nc) ? (T)G[d.goff + (size_t)c * d.nc + k] : (T)0;
+  }
+}
+template <typename T>
+void launch_ginv_to_tiles(hipStream_t st, T *pool, const SubDesc *subs, int nsub, const double *G,
+                          int maxncT) {
+  if (nsub <= 0 || maxncT <= 0) return;
+  hipLaunchKernelGGL(k_ginv_to_tiles<T>, dim3(maxncT * maxncT, nsub), dim3(256), 0, st, pool, subs,
+                     G);
+}
+template void launch_ginv_to_tiles<double>(hipStream_t, double *, const SubDesc *, int,
+                                           const double *, int);
+template void launch_ginv_to_tiles<float>(hipStream_t, float *, const SubDesc *, int, const double *,
+                                          int);
 template void launch_ct_init<double>(hipStream_t, double *, const SubDesc *, int, const int *, int);
 template void launch_ct_init<float>(hipStream_t, float *, const SubDesc *, int, const int *, int);
-template void launch_y_to_z<double>(hipStream_t, const double *, const SubDesc *, int, double *, int);
-template void launch_y_to_z<float>(hipStream_t, const float *, const SubDesc *, int, double *, int);
+template void launch_y_to_z<double>(hipStream_t, const double *, const SubDesc *, int, double *, int,
+                                    int);
+template void launch_y_to_z<float>(hipStream_t, const float *, const SubDesc *, int, double *, int,
+                                   int);
 void launch_g(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const int *cdof,
               const double *Z, double *G) {
   if (nsub <= 0) return;
@@ -179,11 +196,6 @@ void launch_ginv(hipStream_t st, const SubDesc *subs, int nsub, double *G, doubl
   (void)work;
   if (nsub <= 0) return;
   hipLaunchKernelGGL(k_ginv, dim3(nsub), dim3(256), 0, st, subs, G, err);
-}
-void launch_phi(hipStream_t st, const SubDesc *subs, int nsub, const double *Z, const double *G,
-                double *Phi) {
-  if (nsub <= 0) return;
-  hipLaunchKernelGGL(k_phi, dim3(8192 / 256, nsub), dim3(256), 0, st, subs, Z, G, Phi);
 }
 void launch_acs(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const double *G,
                 double *Acs) {

@@ -83,6 +83,8 @@ def main():
                          "subdomains from seeded random box splits")
     ap.add_argument("--box-n", type=int, default=96, help="boxes: elements per cube edge")
     ap.add_argument("--box-nsub", type=int, default=400, help="boxes: number of subdomains")
+    ap.add_argument("--weights", type=int, default=0, choices=(0, 1, 3),
+                    help="interface weights: 0 cardinality, 1 diagonal stiffness, 3 unit")
     ap.add_argument("--nel", type=int, default=16, help="elements per subdomain edge")
     ap.add_argument("--nsub", type=int, default=8, help="subdomains per cube edge")
     ap.add_argument("--tol", type=float, default=1e-8)
@@ -123,8 +125,8 @@ def main():
         p = BoxProblem(args.box_n, args.box_nsub, seed=3, kind="elasticity", layers=27,
                        jump=1e4)
         workload = (f"Q1 elasticity (E=1, nu=0.3) {p.nel}^3 unit cube, 27 seeded layers E=1e4, "
-                    f"{p.nsub} subdomains from seeded random box splits, 2 levels, PCG to "
-                    f"{args.tol:g}")
+                    f"{p.nsub} subdomains from seeded random box splits, 2 levels, weights "
+                    f"type {args.weights}, PCG to {args.tol:g}")
     subs = list(p.subdomains())
     sizes = [d["ndofs"] for d in subs]
     log(f"generated {p.nsub} subdomains ({min(sizes)}..{max(sizes)} dofs), {p.ndof} dofs in "
@@ -146,7 +148,7 @@ def main():
                 b.upload_subdomain_data(d)
             t_up = time.perf_counter() - t0
             t0 = time.perf_counter()
-            b.setup_preconditioner(1, 0, 1, 1, 1, 0, 0, 0)
+            b.setup_preconditioner(1, 0, 1, 1, 1, 0, 0, args.weights)
             t1 = time.perf_counter()
             it, reason, cond = b.solve(0, args.tol, 2000, 200)
             t2 = time.perf_counter()

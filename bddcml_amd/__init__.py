@@ -35,7 +35,7 @@ EXTRA_SYMBOLS = [
     "bddcml_amd_residual_history", "bddcml_amd_timings", "bddcml_amd_set_option",
     "bddcml_amd_sub_info", "bddcml_amd_sub_maps", "bddcml_amd_sub_matrix",
     "bddcml_amd_sub_coarse", "bddcml_amd_last_error", "bddcml_amd_selftest_tile",
-    "bddcml_amd_selftest_copy",
+    "bddcml_amd_selftest_copy", "bddcml_amd_element_gradient",
 ]
 
 
@@ -52,6 +52,7 @@ def lib():
         L.bddcml_amd_selftest_tile.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, C.c_double,
                                                C.c_double, dp]
         L.bddcml_amd_selftest_copy.argtypes = [C.c_longlong, dp]
+        L.bddcml_amd_element_gradient.argtypes = [dp, C.c_int, C.c_int, dp, dp, dp]
         _lib = L
     return _lib
 

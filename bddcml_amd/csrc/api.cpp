@@ -85,8 +85,6 @@ void bddcml_upload_subdomain_data_c(
     int *luser_constraints1, int *luser_constraints2, double *element_data,
     int *lelement_data1, int *lelement_data2, double *dof_data, int *ldof_data,
     int *find_components_int, int *use_dual_mesh_graph_int, int *neighbouring) try {
-  (void)isegn;
-  (void)lisegn;
   (void)user_constraints;
   (void)element_data;
   (void)lelement_data1;
@@ -122,6 +120,10 @@ void bddcml_upload_subdomain_data_c(
   s.inet.resize(*linet);
   for (int k = 0; k < *linet; k++) s.inet[k] = inet[k] - b;
   s.nnet.assign(nnet, nnet + *nelems);
+  if (*lisegn == *nelems) {
+    s.isegn.resize(*nelems);
+    for (int k = 0; k < *nelems; k++) s.isegn[k] = isegn[k] - b;
+  }
   s.nndf.assign(nndf, nndf + *nnods);
   s.isngn.resize(*nnods);
   for (int k = 0; k < *nnods; k++) s.isngn[k] = isngn[k] - b;
@@ -608,6 +610,18 @@ int bddcml_amd_last_error(char *buf, int len) {
 int bddcml_amd_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
                              const double *X, double alpha, double beta, double *out) try {
   device_selftest_tile(op, npairs, A, B, C, X, alpha, beta, out);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e);
+  return -100;
+}
+
+int bddcml_amd_element_gradient(const double *ae, int nke, int per_element, const double *u,
+                                const double *lam, double *grad) try {
+  Solver &S = solver();
+  if (!S.is_setup || S.plan_only) fatal("BDDCML_AMD_ELEMENT_GRADIENT", "solver not set up");
+  device_element_gradient(S, ae, nke, per_element != 0, u, lam, grad);
   return 0;
 }
 catch (const Error &e) {

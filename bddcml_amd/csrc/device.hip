@@ -1329,6 +1329,46 @@ double device_selftest_copy(long long nbytes) {
   return ms;
 }
 
+void device_element_gradient(Solver &S, const double *ae, int nke, bool per_element,
+                             const double *u, const double *lam, double *grad) {
+  const char *R = "BDDCML_AMD_ELEMENT_GRADIENT";
+  Device &D = *S.dev;
+  // element dof lists in global numbering and global element ids
+  std::vector<int> edof, eid;
+  for (const Sub &s : S.subs) {
+    if ((int)s.isegn.size() != s.nelems) fatal(R, "global element numbers (isegn) were not uploaded");
+    int k = 0;
+    for (int e = 0; e < s.nelems; e++) {
+      int nd = 0;
+      for (int q = 0; q < s.nnet[e]; q++) {
+        const int l = s.inet[k + q];
+        for (int d = s.kdof[l]; d < s.kdof[l + 1]; d++, nd++) edof.push_back(s.isvgvn[d]);
+      }
+      k += s.nnet[e];
+      if (nd != nke) fatal(R, "element dof count differs from nke");
+      if (s.isegn[e] < 0 || s.isegn[e] >= S.nelem_g) fatal(R, "element number out of range");
+      eid.push_back(s.isegn[e]);
+    }
+  }
+  for (int d : edof)
+    if (d < 0 || d >= S.ndof_g) fatal(R, "dof number out of range");
+  const int nel = (int)eid.size();
+  const size_t nae = per_element ? (size_t)S.nelem_g * nke * nke : (size_t)nke * nke;
+  DBuf<int> dedof, deid;
+  DBuf<double> dae, du, dl, dg;
+  dedof.upload(edof, D.st);
+  deid.upload(eid, D.st);
+  dae.upload(std::vector<double>(ae, ae + nae), D.st);
+  du.upload(std::vector<double>(u, u + S.ndof_g), D.st);
+  dl.upload(std::vector<double>(lam, lam + S.ndof_g), D.st);
+  dg.alloc(S.nelem_g);
+  HIPCHECK(hipMemsetAsync(dg.p, 0, S.nelem_g * sizeof(double), D.st));
+  launch_element_gradient(D.st, nel, nke, dedof.p, deid.p, dae.p, per_element, du.p, dl.p, dg.p);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(grad, dg.p, S.nelem_g * sizeof(double), hipMemcpyDeviceToHost, D.st));
+  HIPCHECK(hipStreamSynchronize(D.st));
+}
+
 void device_sub_matrix(Solver &S, int is, int which, double *out) {
   Device &D = *S.dev;
   const Sub &s = S.subs[is];

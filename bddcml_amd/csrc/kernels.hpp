@@ -183,6 +183,11 @@ void launch_bicg_p(hipStream_t st, int n, double b, double w, const double *r, c
 void launch_sub_scaled(hipStream_t st, int n, const double *x, double a, const double *y,
                        double *z);
 void launch_mask(hipStream_t st, int n, const char *mask, double *v);
+// grad[eid[e]] = -sum_ij lam[dof_ei] A_e(i,j) u[dof_ej], A_e = ae (+ e*nke^2
+// when per_element); dofs of element e at edof[e*nke ...]
+void launch_element_gradient(hipStream_t st, int nel, int nke, const int *edof, const int *eid,
+                             const double *ae, bool per_element, const double *u,
+                             const double *lam, double *grad);
 // dst = src over n double2 elements (HBM copy reference)
 void launch_copy_calibration(hipStream_t st, const double2 *src, double2 *dst, int64_t n);
 

@@ -698,6 +698,34 @@ void launch_mask(hipStream_t st, int n, const char *mask, double *v) {
   hipLaunchKernelGGL(k_mask, dim3((n + 255) / 256), dim3(256), 0, st, n, mask, v);
 }
 
+// ------------------------------------------------ coefficient gradient
+// one thread per element: g_e = -lam_e^T A_e u_e
+__global__ void k_element_gradient(int nel, int nke, const int *__restrict__ edof,
+                                   const int *__restrict__ eid, const double *__restrict__ ae,
+                                   int per_element, const double *__restrict__ u,
+                                   const double *__restrict__ lam, double *__restrict__ grad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  const int *dof = edof + (size_t)e * nke;
+  const double *A = ae + (per_element ? (size_t)eid[e] * nke * nke : 0);
+  double g = 0.0;
+  for (int i = 0; i < nke; i++) {
+    const double li = lam[dof[i]];
+    if (li == 0.0) continue;
+    double s = 0.0;
+    for (int j = 0; j < nke; j++) s += A[i * nke + j] * u[dof[j]];
+    g += li * s;
+  }
+  grad[eid[e]] = -g;
+}
+void launch_element_gradient(hipStream_t st, int nel, int nke, const int *edof, const int *eid,
+                             const double *ae, bool per_element, const double *u,
+                             const double *lam, double *grad) {
+  if (nel <= 0) return;
+  hipLaunchKernelGGL(k_element_gradient, dim3((nel + 255) / 256), dim3(256), 0, st, nel, nke, edof,
+                     eid, ae, per_element ? 1 : 0, u, lam, grad);
+}
+
 // ------------------------------------------- local order <-> device order
 __global__ void k_to_device_order(const OrderDesc *__restrict__ ds, const int *__restrict__ perm,
                                   const double *__restrict__ src, double *__restrict__ vec) {

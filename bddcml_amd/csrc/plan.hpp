@@ -32,6 +32,7 @@ struct Sub {
   int isub = -1;
   int nelems = 0, nnods = 0, ndofs = 0;
   std::vector<int> inet, nnet, nndf, isngn, isvgvn;
+  std::vector<int> isegn;   // global element ids (when uploaded)
   std::vector<double> xyz;  // nnods x ndim, column major
   std::vector<int> ifix;
   std::vector<double> fixv, rhs, sol;

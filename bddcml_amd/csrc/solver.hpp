@@ -96,6 +96,9 @@ void device_reduced_rhs_download(Solver &S, double *g);
 void device_sub_matrix(Solver &S, int is, int which, double *out);
 void device_sub_coarse(Solver &S, int is, double *phi, double *ac);
 double device_selftest_copy(long long nbytes);  // milliseconds
+// grad[isegn(e)] = -lam_e^T A_e u_e over all elements (global u, lam)
+void device_element_gradient(Solver &S, const double *ae, int nke, bool per_element,
+                             const double *u, const double *lam, double *grad);
 void device_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
                           const double *X, double alpha, double beta, double *out);
 void device_release(Solver &S);

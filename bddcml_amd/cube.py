@@ -34,6 +34,16 @@ def uniform01(seed: int, index):
     return (z >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
 
 
+def lognormal(seed: int, index, mu: float = 0.0, sigma: float = 1.0):
+    """exp(mu + sigma * z), z standard normal by Box-Muller from the pair of
+    uniforms (2i, 2i+1) of the seed's stream."""
+    i = np.asarray(index, dtype=np.uint64)
+    u1 = 1.0 - uniform01(seed, np.uint64(2) * i)          # (0, 1]
+    u2 = uniform01(seed, np.uint64(2) * i + np.uint64(1))
+    z = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    return np.exp(mu + sigma * z)
+
+
 def q1_laplace_reference_element() -> np.ndarray:
     """8x8 stiffness of -Laplace on the unit hexahedron with trilinear shape
     functions, node order of the reference harness (poisson_on_cube.f90:146-167):

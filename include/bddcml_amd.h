@@ -71,6 +71,19 @@ int bddcml_amd_last_error(char *buf, int len);
  * op 3 the inverse in out[4096..8191] */
 int bddcml_amd_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
                              const double *X, double alpha, double beta, double *out);
+/* Element coefficient gradient for the adjoint of A(k) u = f with
+ * A = sum_e k_e A_e (no counterpart in the reference; for gradient-based
+ * callers): grad[isegn(e)] = -lam_e^T A_e u_e for every element e of every
+ * subdomain, where u is the solution, lam the adjoint solution (A lam =
+ * dJ/du, homogeneous Dirichlet values; solved with
+ * bddcml_change_subdomain_data_c + bddcml_setup_new_data_c + bddcml_solve_c
+ * on the same factorization), both global vectors of ndof entries.  A_e is
+ * the nke x nke element matrix per unit coefficient (row major, element dofs
+ * in node order, nndf dofs per node), one block shared by all elements
+ * (per_element = 0) or one per global element (per_element = 1).  grad has
+ * nelem entries.  Needs isegn at upload. */
+int bddcml_amd_element_gradient(const double *ae, int nke, int per_element, const double *u,
+                                const double *lam, double *grad);
 /* plain device copy of nbytes (multiple of 16) through kernel
  * k_copy_calibration, timed with events: the reference point for HBM
  * counters and bandwidth; out[0] = milliseconds of the copy */

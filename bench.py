@@ -85,22 +85,21 @@ def gradient_workload(args):
     k = lognormal(5, np.arange(p0.nelem))
     ut = 2.0 * uniform01(6, np.arange(p0.ndof, dtype=np.uint64)) - 1.0
     make = lambda kk: CubeProblem(nel, ns, rhs="random", seed=5, coef=lambda e: kk[e])
-    sol = AdjointSolver(make, p0.kref, tol=1e-10)
+    sol = AdjointSolver(make, p0.kref, tol=1e-12)
     times = []
     for step in range(args.warmup + args.steps):
-        sol.forward(k)  # warm the generator path, then time the device work only
-        b = sol.b
-        t0 = time.perf_counter()
         u = sol.forward(k)
+        tm = sol.b.timings()
+        tf = tm["setup_total"] + tm["solve_total"]  # setup_preconditioner + solve
         t1 = time.perf_counter()
         lam = sol.adjoint(u - ut)
         t2 = time.perf_counter()
         g = sol.gradient(u, lam)
         t3 = time.perf_counter()
-        log(f"step {step}: forward (init+upload+setup+solve) {t1 - t0:.3f} s, adjoint "
-            f"{t2 - t1:.3f} s, gradient {t3 - t2:.4f} s, iterations {sol.iterations[-2:]}")
+        log(f"step {step}: forward setup+solve {tf:.3f} s, adjoint (new data + solve) "
+            f"{t2 - t1:.3f} s, gradient kernel {t3 - t2:.4f} s, iterations {sol.iterations[-2:]}")
         if step >= args.warmup:
-            times.append((t1 - t0, t2 - t1, t3 - t2))
+            times.append((tf, t2 - t1, t3 - t2))
     J = lambda uu: 0.5 * np.sum((uu - ut) ** 2)
     rng_e = (uniform01(7, np.arange(50, dtype=np.uint64)) * p0.nelem).astype(int)
     errs = []
@@ -112,19 +111,20 @@ def gradient_workload(args):
         errs.append(abs(fd - g[e]) / np.abs(g).max())
     sol.close()
     tf, ta, tg = (float(np.mean([t[i] for t in times])) for i in range(3))
-    ms = 1000.0 * (ta + tg)
+    ms = 1000.0 * (tf + ta + tg)
     line = {
-        "metric": METRIC, "value": p0.ndof / (ta + tg), "unit": "dofs/s", "n_gpus": 1,
+        "metric": METRIC, "value": p0.ndof / (tf + ta + tg), "unit": "dofs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "adjoint gradient: -div(k grad u)=f, 64^3 Q1, 512 subdomains of "
                                "8^3, log-normal k (seed 5), J=0.5|u-u_t|^2 (seed 6)",
                    "dofs": p0.ndof, "elements": p0.nelem,
-                   "forward_s_incl_upload": round(tf, 4), "adjoint_ms": round(1000 * ta, 2),
+                   "forward_ms": round(1000 * tf, 2), "adjoint_ms": round(1000 * ta, 2),
                    "gradient_kernel_ms": round(1000 * tg, 3),
                    "fd_check_elements": 50, "fd_max_rel_error": float(max(errs))},
-        "note": "value: adjoint solve + element gradient on the forward factorization",
+        "note": "value: dofs / (forward setup+solve + adjoint solve on the same factorization "
+                "+ element gradient kernel)",
     }
     print(json.dumps(line), flush=True)
 

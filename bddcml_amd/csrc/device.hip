@@ -81,14 +81,45 @@ struct DBuf {
   }
 };
 
+// A device array inside an Arena.
+template <class T> struct View {
+  T *p = nullptr;
+  size_t n = 0;
+};
+
+// Many small plan arrays gathered into one host buffer and moved to the
+// device with one allocation and one copy.  Views get their device
+// addresses at commit; they must not move between add and commit.
+struct Arena {
+  std::vector<char> host;
+  DBuf<char> dev;
+  std::vector<std::pair<size_t, void **>> fix;
+  template <class T> void add(View<T> &v, const std::vector<T> &src) {
+    const size_t off = (host.size() + 255) / 256 * 256;
+    host.resize(off + src.size() * sizeof(T));
+    if (!src.empty()) std::memcpy(host.data() + off, src.data(), src.size() * sizeof(T));
+    v.n = src.size();
+    fix.push_back({off, reinterpret_cast<void **>(&v.p)});
+  }
+  void commit(hipStream_t st) {
+    dev.alloc(std::max<size_t>(host.size(), 1));
+    if (!host.empty())
+      HIPCHECK(hipMemcpyAsync(dev.p, host.data(), host.size(), hipMemcpyHostToDevice, st));
+    for (auto &f : fix) *f.second = dev.p + f.first;
+    fix.clear();
+    HIPCHECK(hipStreamSynchronize(st));  // the host buffer may go away
+    std::vector<char>().swap(host);
+  }
+};
+
 struct Phase {
-  DBuf<TileTask> tasks;
-  DBuf<TilePair> pairs;
+  View<TileTask> tasks;
+  View<TilePair> pairs;
   int ntask = 0;
-  void upload(const TaskList &l, hipStream_t st) {
+  void upload(const TaskList &l, Arena &a) {
     ntask = (int)l.tasks.size();
-    tasks.upload(l.tasks, st);
-    pairs.upload(l.pairs, st);
+    a.add(tasks, l.tasks);
+    a.add(pairs, l.pairs);
   }
 };
 
@@ -144,14 +175,15 @@ struct Device {
   std::vector<Phase> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd;
   Phase a_schur, cinv, phi;
   std::vector<SeqPhase> cseq;  // coarse factorization and inverse
-  DBuf<int> zero_ids, sc_perm, sc_ttab;
-  DBuf<char> sc_fix;
-  DBuf<ScatterSub> sc_subs;
-  DBuf<int2> zero_rng;
-  DBuf<int64_t> sc_idx;
-  DBuf<double> sc_val;
-  DBuf<MirrorJob> mir_s, mir_c;
-  DBuf<int> sp_sub, sp_ij, sp_dst;
+  Arena plan;  // task lists, scatter tables and job lists of the first half
+  View<int> sc_perm, sc_ttab;
+  View<char> sc_fix;
+  View<ScatterSub> sc_subs;
+  View<int2> zero_rng;
+  View<int64_t> sc_idx;
+  View<double> sc_val;
+  View<MirrorJob> mir_s, mir_c;
+  View<int> sp_sub, sp_ij, sp_dst;
   int n_sp = 0;
   // interface data
   std::vector<SubDesc> hdesc;
@@ -353,10 +385,9 @@ static void check_tasks(const TaskList &l, int64_t ntiles, bool uses_x, const ch
     if (bad(p.a) || bad(p.b)) fatal("BDDCML_DEVICE_PLAN", std::string("invalid tile pair in ") + what);
 }
 
-static void upload_phases(std::vector<Phase> &dst, const std::vector<TaskList> &src,
-                          hipStream_t st) {
+static void upload_phases(std::vector<Phase> &dst, const std::vector<TaskList> &src, Arena &a) {
   dst.resize(src.size());
-  for (size_t k = 0; k < src.size(); k++) dst[k].upload(src[k], st);
+  for (size_t k = 0; k < src.size(); k++) dst[k].upload(src[k], a);
 }
 
 // ------------------------------------------------------------------ setup
@@ -385,8 +416,16 @@ void device_setup_begin(Solver &S) {
 
   // ---- symbolic patterns and tile ids
   D.sub.resize(nsub);
+  // symbolic factorization once per class of identical subdomains
+  if ((int)S.ord_cls.size() != nsub) fatal("BDDCML_DEVICE_PLAN", "orderings missing");
 #pragma omp parallel for schedule(dynamic)
-  for (int is = 0; is < nsub; is++) symbolic(S.subs[is], D.sub[is]);
+  for (int k = 0; k < (int)S.ord_reps.size(); k++)
+    symbolic(S.subs[S.ord_reps[k]], D.sub[S.ord_reps[k]]);
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < nsub; is++) {
+    const int r = S.ord_reps[S.ord_cls[is]];
+    if (r != is) D.sub[is] = D.sub[r];
+  }
   int next = 0, next32 = 0;
   D.mixed = S.precision == 32;
   for (int is = 0; is < nsub; is++) assign_tiles(D.sub[is], S.subs[is].nconstr, next, next32, D.mixed);
@@ -429,6 +468,7 @@ void device_setup_begin(Solver &S) {
     }
   }
   const int ncls = (int)reps.size();
+  if (S.verbose >= 3) ph("  structure classes");
   struct SubTasks {
     std::vector<TaskList> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd;
     TaskList a_schur, phi;
@@ -547,6 +587,7 @@ void device_setup_begin(Solver &S) {
     relative(Q.phi, T.fbase, T.fspan);
   }
   if (bad_ids) fatal("BDDCML_DEVICE_PLAN", "tile task outside its subdomain's tiles");
+  if (S.verbose >= 3) ph("  class task lists");
   for (int is = 0; is < nsub; is++)
     if ((int64_t)D.sub[is].base + D.sub[is].span > D.ntiles ||
         (int64_t)D.sub[is].fbase + D.sub[is].fspan > (D.mixed ? D.ntiles32 : D.ntiles))
@@ -600,6 +641,7 @@ void device_setup_begin(Solver &S) {
       materialize([](const SubTasks &Q) -> const TaskList * { return &Q.a_schur; }, false);
   TaskList phi = materialize([](const SubTasks &Q) -> const TaskList * { return &Q.phi; }, true);
   sts.clear();
+  if (S.verbose >= 3) ph("  materialized");
   if (S.verbose >= 2)
     std::printf("   device plan: %d structure classes over %d subdomains, %zu interior steps\n", ncls,
                 nsub, a_diag.size());
@@ -657,22 +699,23 @@ void device_setup_begin(Solver &S) {
   }
   for (auto &q : cseq) check_tasks(q.l, D.ntiles, q.potrf || q.post != POST_NONE, "coarse factorization");
   check_tasks(cinv, D.ntiles, false, "coarse W^T W");
-  upload_phases(D.a_diag, a_diag, st);
-  upload_phases(D.a_off, a_off, st);
-  upload_phases(D.p_diag, p_diag, st);
-  upload_phases(D.p_off, p_off, st);
-  upload_phases(D.z_fwd, z_fwd, st);
-  upload_phases(D.z_bwd, z_bwd, st);
+  if (S.verbose >= 3) ph("  coarse sequence");
+  upload_phases(D.a_diag, a_diag, D.plan);
+  upload_phases(D.a_off, a_off, D.plan);
+  upload_phases(D.p_diag, p_diag, D.plan);
+  upload_phases(D.p_off, p_off, D.plan);
+  upload_phases(D.z_fwd, z_fwd, D.plan);
+  upload_phases(D.z_bwd, z_bwd, D.plan);
   D.cseq.resize(cseq.size());
   for (size_t k = 0; k < cseq.size(); k++) {
     SeqPhase &q = D.cseq[k];
-    q.ph.upload(cseq[k].l, st);
+    q.ph.upload(cseq[k].l, D.plan);
     q.kind = cseq[k].kind, q.post = cseq[k].post, q.alpha = cseq[k].alpha, q.beta = cseq[k].beta;
     q.potrf = cseq[k].potrf;
   }
-  D.a_schur.upload(a_schur, st);
-  D.phi.upload(phi, st);
-  D.cinv.upload(cinv, st);
+  D.a_schur.upload(a_schur, D.plan);
+  D.phi.upload(phi, D.plan);
+  D.cinv.upload(cinv, D.plan);
 
   ph("task lists");
   // ---- matrix entries: the assembled triplets are already on the device
@@ -744,13 +787,13 @@ void device_setup_begin(Solver &S) {
       if (k < 0 || k >= D.ntiles * (int64_t)TILE_ELEMS) fatal("BDDCML_DEVICE_PLAN", "scatter index out of range");
     for (int v : ttab)
       if (v >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid tile table entry");
-    D.sc_perm.upload(perm, st);
-    D.sc_fix.upload(fix, st);
-    D.sc_ttab.upload(ttab, st);
-    D.sc_subs.upload(ss, st);
-    D.zero_rng.upload(ranges, st);
-    D.sc_idx.upload(sidx, st);
-    D.sc_val.upload(sval, st);
+    D.plan.add(D.sc_perm, perm);
+    D.plan.add(D.sc_fix, fix);
+    D.plan.add(D.sc_ttab, ttab);
+    D.plan.add(D.sc_subs, ss);
+    D.plan.add(D.zero_rng, ranges);
+    D.plan.add(D.sc_idx, sidx);
+    D.plan.add(D.sc_val, sval);
   }
 
   ph("scatter entries");
@@ -785,14 +828,15 @@ void device_setup_begin(Solver &S) {
         fatal("BDDCML_DEVICE_PLAN", "invalid mirror job");
   for (int d : spdst)
     if (d < 0 || d >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid augmented tile");
-  D.mir_s.upload(ms, st);
-  D.mir_c.upload(mc, st);
-  D.sp_sub.upload(spsub, st);
-  D.sp_ij.upload(spij, st);
-  D.sp_dst.upload(spdst, st);
+  D.plan.add(D.mir_s, ms);
+  D.plan.add(D.mir_c, mc);
+  D.plan.add(D.sp_sub, spsub);
+  D.plan.add(D.sp_ij, spij);
+  D.plan.add(D.sp_dst, spdst);
   D.n_sp = (int)spsub.size();
 
   ph("mirror and update jobs");
+  D.plan.commit(st);
   D.pool.alloc((size_t)D.ntiles * TILE_ELEMS);
   if (D.mixed) D.pool32.alloc((size_t)std::max<int64_t>(D.ntiles32, 1) * TILE_ELEMS);
   D.err.alloc(1);

@@ -17,6 +17,7 @@
 #include <map>
 #include <numeric>
 #include <queue>
+#include <unordered_map>
 
 #include "solver.hpp"
 
@@ -326,7 +327,73 @@ static void globs_second_pass(Solver &S, int is) {
   });
 }
 
+// ------------------------------------------------------ structure classes
+// Subdomains whose local structure is identical (translated copies on a
+// regular decomposition) share per-subdomain analyses.  hash(i) digests the
+// integer structure; same(r, i) decides.  Candidates are compared with the
+// first subdomain of their hash bucket in parallel; the rare mismatches are
+// resolved serially.
+template <class H, class E>
+static void structure_classes(int n, H hash, E same, std::vector<int> &cls, std::vector<int> &reps) {
+  std::vector<uint64_t> h(n);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) h[i] = hash(i);
+  std::unordered_map<uint64_t, int> first;
+  std::vector<int> cand(n);
+  for (int i = 0; i < n; i++) cand[i] = first.emplace(h[i], i).first->second;
+  std::vector<char> ok(n, 0);
+#pragma omp parallel for schedule(dynamic)
+  for (int i = 0; i < n; i++) ok[i] = cand[i] == i || same(cand[i], i);
+  cls.assign(n, -1);
+  reps.clear();
+  for (int i = 0; i < n; i++) {
+    if (ok[i] && cls[cand[i]] >= 0) {
+      cls[i] = cls[cand[i]];
+      continue;
+    }
+    if (ok[i] && cand[i] == i) {
+      cls[i] = (int)reps.size();
+      reps.push_back(i);
+      continue;
+    }
+    for (int r : reps)  // mismatch inside a hash bucket
+      if (h[r] == h[i] && same(r, i)) {
+        cls[i] = cls[r];
+        break;
+      }
+    if (cls[i] < 0) {
+      cls[i] = (int)reps.size();
+      reps.push_back(i);
+    }
+  }
+}
+
+struct Fnv {
+  uint64_t h = 1469598103934665603ull;
+  void add(uint64_t v) { h = (h ^ v) * 1099511628211ull; }
+  template <class T> void add(const std::vector<T> &v) {
+    add((uint64_t)v.size());
+    for (const T &x : v) add((uint64_t)x);
+  }
+};
+
+// relative node coordinates equal up to a tolerance of the extent
+static bool same_shape(const Sub &a, const Sub &b, int ndim) {
+  if (a.xyz.size() != b.xyz.size()) return false;
+  const int n = a.nnods;
+  for (int d = 0; d < ndim; d++) {
+    const double *x = a.xyz.data() + (size_t)d * n, *y = b.xyz.data() + (size_t)d * n;
+    double lo = x[0], hi = x[0];
+    for (int i = 0; i < n; i++) lo = std::min(lo, x[i]), hi = std::max(hi, x[i]);
+    const double tol = 1e-9 * std::max(hi - lo, 1e-300);
+    for (int i = 0; i < n; i++)
+      if (std::fabs((x[i] - x[0]) - (y[i] - y[0])) > tol) return false;
+  }
+  return true;
+}
+
 void build_topology(Solver &S) {
+  const double t_n2s = wall_time();
   int ng = S.nnod_g;
   S.n2s_ptr.assign(ng + 1, 0);
   for (auto &s : S.subs)
@@ -344,10 +411,31 @@ void build_topology(Solver &S) {
     if (S.verbose >= 2) std::printf("   topology: %-20s %8.4f s\n", name, now - tp);
     tp = now;
   };
+  if (S.verbose >= 2) std::printf("   topology: %-20s %8.4f s\n", "node to subdomains", tp - t_n2s);
+  // nodal components depend only on the element connectivity: once per class
+  std::vector<int> ccls, creps;
+  structure_classes(
+      S.nsub,
+      [&](int i) {
+        const Sub &s = S.subs[i];
+        Fnv f;
+        f.add((uint64_t)s.nnods), f.add((uint64_t)s.find_components), f.add((uint64_t)s.use_dual);
+        f.add((uint64_t)s.neighbouring), f.add(s.nnet), f.add(s.inet);
+        return f.h;
+      },
+      [&](int a, int b) {
+        const Sub &x = S.subs[a], &y = S.subs[b];
+        return x.nnods == y.nnods && x.find_components == y.find_components &&
+               x.use_dual == y.use_dual && x.neighbouring == y.neighbouring && x.nnet == y.nnet &&
+               x.inet == y.inet;
+      },
+      ccls, creps);
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < (int)creps.size(); k++) nodal_components(S.subs[creps[k]]);
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < S.nsub; is++) {
     Sub &s = S.subs[is];
-    nodal_components(s);
+    if (creps[ccls[is]] != is) s.ncomp = S.subs[creps[ccls[is]]].ncomp;
     s.g2l.resize(s.nnods);
     for (int l = 0; l < s.nnods; l++) s.g2l[l] = {s.isngn[l], l};
     std::sort(s.g2l.begin(), s.g2l.end());
@@ -886,9 +974,39 @@ static void order_subdomain(Sub &s, int method, int leaf_dofs, int ndim) {
   }
 }
 
+// Orderings (and the symbolic factorization built on them) depend on the
+// matrix pattern, the interior/interface split and the node coordinates up
+// to a translation: computed once per class of identical subdomains.
 void build_orderings(Solver &S) {
+  structure_classes(
+      S.nsub,
+      [&](int i) {
+        const Sub &s = S.subs[i];
+        Fnv f;
+        f.add((uint64_t)s.ndofs), f.add((uint64_t)s.nnods), f.add(s.kdof);
+        f.add(s.ai), f.add(s.aj), f.add(s.int_dof), f.add(s.ifc_dof);
+        return f.h;
+      },
+      [&](int a, int b) {
+        const Sub &x = S.subs[a], &y = S.subs[b];
+        return x.ndofs == y.ndofs && x.nnods == y.nnods && x.kdof == y.kdof && x.ai == y.ai &&
+               x.aj == y.aj && x.int_dof == y.int_dof && x.ifc_dof == y.ifc_dof &&
+               same_shape(x, y, S.ndim);
+      },
+      S.ord_cls, S.ord_reps);
 #pragma omp parallel for schedule(dynamic)
-  for (int is = 0; is < S.nsub; is++) order_subdomain(S.subs[is], S.ordering, S.nd_leaf, S.ndim);
+  for (int k = 0; k < (int)S.ord_reps.size(); k++)
+    order_subdomain(S.subs[S.ord_reps[k]], S.ordering, S.nd_leaf, S.ndim);
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++) {
+    const Sub &r = S.subs[S.ord_reps[S.ord_cls[is]]];
+    if (&r == &S.subs[is]) continue;
+    Sub &s = S.subs[is];
+    s.no = r.no, s.ni = r.ni, s.NO = r.NO, s.NI = r.NI, s.nto = r.nto, s.nti = r.nti;
+    s.perm = r.perm;
+    s.dev_ifc = r.dev_ifc;
+    s.pad = r.pad;
+  }
 }
 
 }  // namespace bddc

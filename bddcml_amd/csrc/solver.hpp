@@ -39,6 +39,8 @@ struct Solver {
   int precision = 64;  // 64: FP64 everywhere; 32: FP32 preconditioner factors
   int ordering = 0;    // interior ordering: 0 tile-aligned nested dissection, 1 RCM
   int nd_leaf = 256;   // nested dissection: largest part (dofs) left undivided
+  // classes of subdomains with identical structure (ordering, symbolic)
+  std::vector<int> ord_cls, ord_reps;
   // topology results
   int ncorner = 0, nedge = 0, nface = 0, nnodc = 0, ndofc = 0;
   std::vector<int> nndfc, kdofc;

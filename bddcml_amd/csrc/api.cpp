@@ -129,6 +129,10 @@ void bddcml_upload_subdomain_data_c(
   for (int k = 0; k < *nnods; k++) s.isngn[k] = isngn[k] - b;
   s.isvgvn.resize(*ndofs);
   for (int k = 0; k < *ndofs; k++) s.isvgvn[k] = isvgvn[k] - b;
+  for (int g : s.isngn)
+    if (g < 0 || g >= *nnod) fatal("BDDCML_UPLOAD_SUBDOMAIN_DATA", "global node index out of range");
+  for (int g : s.isvgvn)
+    if (g < 0 || g >= *ndof) fatal("BDDCML_UPLOAD_SUBDOMAIN_DATA", "global dof index out of range");
   s.xyz.assign(xyz, xyz + (size_t)*nnods * *ndim);
   s.ifix.assign(ifix, ifix + *ndofs);
   s.fixv.assign(fixv, fixv + *ndofs);

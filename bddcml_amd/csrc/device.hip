@@ -25,7 +25,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <utility>
 #include <numeric>
+#include <thread>
 #include <unordered_map>
 
 #include "kernels.hpp"
@@ -94,7 +97,9 @@ struct Arena {
   std::vector<char> host;
   DBuf<char> dev;
   std::vector<std::pair<size_t, void **>> fix;
+  int verbose = 0;
   template <class T> void add(View<T> &v, const std::vector<T> &src) {
+    if (verbose >= 4) std::printf("   device plan: arena %10zu bytes\n", src.size() * sizeof(T));
     const size_t off = (host.size() + 255) / 256 * 256;
     host.resize(off + src.size() * sizeof(T));
     if (!src.empty()) std::memcpy(host.data() + off, src.data(), src.size() * sizeof(T));
@@ -147,7 +152,9 @@ struct SubTiles {
   // factor space (S' factor, its inverse diagonal tiles, Z): the FP64 pool,
   // or the FP32 pool in the mixed-precision mode
   int fbase = 0, fspan = 0;
-  int linvA = 0, sfull = 0, pbase = 0, linvP = 0, ybase = 0, ncT = 0, gtbase = 0, phibase = 0;
+  // slow: tile (nto, nto), the first of S's lower tiles; the interface tile
+  // columns are dense, so tile (nto + I, nto + J) is slow + lowidx(nti, I, J)
+  int linvA = 0, slow = 0, pbase = 0, linvP = 0, ybase = 0, ncT = 0, gtbase = 0, phibase = 0;
   int P(int I, int J) const { return pbase + lowidx(nti, I, J); }
   int Y(int I, int cT) const { return ybase + I * ncT + cT; }
 };
@@ -176,13 +183,13 @@ struct Device {
   Phase a_schur, cinv, phi;
   std::vector<SeqPhase> cseq;  // coarse factorization and inverse
   Arena plan;  // task lists, scatter tables and job lists of the first half
-  View<int> sc_perm, sc_ttab;
+  View<int> sc_perm, sc_ttab, sc_pads;  // shared per class of identically ordered subdomains
+  std::vector<int> perm_off;            // offset of each subdomain's permutation in sc_perm
   View<char> sc_fix;
   View<ScatterSub> sc_subs;
   View<int2> zero_rng;
-  View<int64_t> sc_idx;
-  View<double> sc_val;
-  View<MirrorJob> mir_s, mir_c;
+  View<int64_t> sc_idx;                 // padding diagonals of the coarse matrix
+  View<MirrorJob> mir_c;
   View<int> sp_sub, sp_ij, sp_dst;
   int n_sp = 0;
   // interface data
@@ -193,8 +200,11 @@ struct Device {
   DBuf<double> w, Z, Phi, G, Acs;
   int maxNI = 0, maxnc = 0, maxnti = 0, maxnt = 1, maxncT = 0;
   int64_t LV = 0;
-  DBuf<GemvRow> srows, crows;
-  int nsrows = 0, ncrows = 0;
+  DBuf<SymvCol> scols;
+  DBuf<SymvRow> srows;
+  DBuf<double> spart;  // partial slots of the symmetric S GEMV
+  DBuf<GemvRow> crows;
+  int nscols = 0, nsrows = 0, ncrows = 0;
   DBuf<int> aptr;
   DBuf<int64_t> asrc;
   DBuf<int> cgptr, cgsrc;
@@ -314,8 +324,7 @@ static void assign_tiles(SubTiles &T, int nc, int &next, int &next32, bool mixed
   }
   T.linvA = next;
   next += T.nto;
-  T.sfull = next;
-  next += T.nti * T.nti;
+  T.slow = T.pat.find(T.nto, T.nto);
   int &nf = mixed ? next32 : next;
   const int f0 = nf;
   T.pbase = nf;
@@ -415,6 +424,8 @@ void device_setup_begin(Solver &S) {
   };
 
   // ---- symbolic patterns and tile ids
+  D.plan.verbose = S.verbose;
+  D.plan.host.reserve(32u << 20);
   D.sub.resize(nsub);
   // symbolic factorization once per class of identical subdomains
   if ((int)S.ord_cls.size() != nsub) fatal("BDDCML_DEVICE_PLAN", "orderings missing");
@@ -443,6 +454,33 @@ void device_setup_begin(Solver &S) {
   D.ntiles = next;
 
   ph("symbolic");
+  // ---- the tile pools are allocated on a helper thread while the host
+  // builds the task lists and tables below (a large hipMalloc takes
+  // milliseconds); joined before the first launch, also on error paths.
+  struct PoolAlloc {
+    std::thread th;
+    std::exception_ptr err;
+    ~PoolAlloc() {
+      if (th.joinable()) th.join();
+    }
+    void wait() {
+      if (th.joinable()) th.join();
+      if (err) std::rethrow_exception(std::exchange(err, nullptr));
+    }
+  } pool_alloc;
+  {
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    pool_alloc.th = std::thread([&D, &pool_alloc, dev] {
+      try {
+        HIPCHECK(hipSetDevice(dev));
+        D.pool.alloc((size_t)D.ntiles * TILE_ELEMS);
+        if (D.mixed) D.pool32.alloc((size_t)std::max<int64_t>(D.ntiles32, 1) * TILE_ELEMS);
+      } catch (...) {
+        pool_alloc.err = std::current_exception();
+      }
+    });
+  }
   // ---- task lists.  Subdomains of identical tile structure form a class
   // and share one list of tile pairs, kept relative to the subdomain's first
   // tile; the tasks (output tile, inverse tile, pair range, tile base) are
@@ -720,94 +758,135 @@ void device_setup_begin(Solver &S) {
   ph("task lists");
   // ---- matrix entries: the assembled triplets are already on the device
   // (uploaded with each subdomain); here only the device permutation, the
-  // Dirichlet flags and the tile tables go up.  Padding diagonals (1.0) of
-  // subdomain and coarse matrices come from a short host list.
+  // Dirichlet flags and the tile tables go up.  The permutation, the packed
+  // lower tile table (tile ids relative to the subdomain's first tile) and
+  // the padding diagonals (1.0) are shared by each class of identically
+  // ordered subdomains; the Dirichlet flags are per subdomain.  Padding
+  // diagonals of the coarse matrix come from a short host list.
   if (!S.din)
     for (int is = 0; is < nsub; is++) device_upload_entries(S, is);
   {
-    std::vector<size_t> poff(nsub + 1, 0), toff(nsub + 1, 0), doff(nsub + 1, 0);
-    for (int is = 0; is < nsub; is++) {
-      const Sub &s = S.subs[is];
-      const SubTiles &T = D.sub[is];
+    const std::vector<int> &ocls = S.ord_cls, &oreps = S.ord_reps;
+    const int nrep = (int)oreps.size();
+    std::vector<size_t> foff(nsub + 1, 0), poff(nrep + 1, 0), toff(nrep + 1, 0), doff(nrep + 1, 0);
+    for (int is = 0; is < nsub; is++) foff[is + 1] = foff[is] + S.subs[is].ndofs;
+    for (int k = 0; k < nrep; k++) {
+      const Sub &s = S.subs[oreps[k]];
+      const SubTiles &T = D.sub[oreps[k]];
       const size_t nt = T.nto + T.nti;
-      poff[is + 1] = poff[is] + s.ndofs;
-      toff[is + 1] = toff[is] + nt * (nt + 1) / 2;
-      doff[is + 1] = doff[is] + s.pad.size() + (s.NI - s.ni);
+      poff[k + 1] = poff[k] + s.ndofs;
+      toff[k + 1] = toff[k] + nt * (nt + 1) / 2;
+      doff[k + 1] = doff[k] + s.pad.size() + (s.NI - s.ni);
     }
     const int npad_c = D.NC - S.ndofc;
-    std::vector<int> perm(poff[nsub]), ttab(toff[nsub], -1);
-    std::vector<char> fix(poff[nsub]);
+    std::vector<int> perm(poff[nrep]), ttab(toff[nrep], -1), pads(doff[nrep]);
+    std::vector<char> fix(foff[nsub]);
     std::vector<ScatterSub> ss(nsub);
     std::vector<int2> ranges(nsub + 1);
-    std::vector<int64_t> sidx(doff[nsub] + npad_c);
-    std::vector<double> sval(doff[nsub] + npad_c, 1.0);
+    std::vector<int64_t> sidx(npad_c);
+    if (S.verbose >= 3) ph("  scatter allocation");
     int bad = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad)
-    for (int is = 0; is < nsub; is++) {
-      const Sub &s = S.subs[is];
-      const SubTiles &T = D.sub[is];
+    for (int k = 0; k < nrep; k++) {
+      const Sub &s = S.subs[oreps[k]];
+      const SubTiles &T = D.sub[oreps[k]];
       const int nt = T.nto + T.nti;
-      ss[is] = ScatterSub{S.din->ai[is].p, S.din->aj[is].p, S.din->av[is].p,
-                          (int)S.din->ai[is].n, (int)poff[is], (int)toff[is]};
-      if ((int64_t)S.din->ai[is].n != (int64_t)s.ai.size()) bad |= 1;
+      if ((int64_t)T.pat.tile.size() * TILE_ELEMS > INT32_MAX) bad |= 4;
       for (int d = 0; d < s.ndofs; d++) {
-        perm[poff[is] + d] = s.perm[d];
-        fix[poff[is] + d] = s.ifix[d] != 0;
+        perm[poff[k] + d] = s.perm[d];
         if (s.perm[d] < 0 || s.perm[d] >= s.NO + s.NI) bad |= 2;
       }
-      int *tt = ttab.data() + toff[is];
+      int *tt = ttab.data() + toff[k];
       for (int J = 0; J < nt; J++)
         for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) {
           int I = T.pat.rowidx[q];
-          tt[(size_t)I * (I + 1) / 2 + J] = T.pat.tile[q];
+          tt[(size_t)I * (I + 1) / 2 + J] = T.pat.tile[q] - T.base;
         }
-      ranges[is] = make_int2(T.pat.tile.front(), (int)T.pat.tile.size());
-      int64_t *si = sidx.data() + doff[is];
+      int *pd = pads.data() + doff[k];
       auto put_diag = [&](int p) {
-        int tid = T.pat.find(p / TILE, p / TILE);
-        *si++ = (int64_t)tid * TILE_ELEMS + (p % TILE) * TILE + (p % TILE);
+        int tid = T.pat.find(p / TILE, p / TILE) - T.base;
+        *pd++ = tid * TILE_ELEMS + (p % TILE) * TILE + (p % TILE);
       };
       for (int p : s.pad) put_diag(p);
       for (int p = s.NO + s.ni; p < s.NO + s.NI; p++) put_diag(p);
     }
+    D.perm_off.assign(nsub, 0);
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
+    for (int is = 0; is < nsub; is++) {
+      const Sub &s = S.subs[is];
+      const SubTiles &T = D.sub[is];
+      const int c = ocls[is];
+      ss[is] = ScatterSub{S.din->ai[is].p, S.din->aj[is].p, S.din->av[is].p,
+                          (int)S.din->ai[is].n, (int)foff[is], (int)poff[c], (int)toff[c],
+                          (int)doff[c], (int)(doff[c + 1] - doff[c]), T.base};
+      D.perm_off[is] = (int)poff[c];
+      if ((int64_t)S.din->ai[is].n != (int64_t)s.ai.size()) bad |= 1;
+      if (T.base != T.pat.tile.front() || s.ndofs != S.subs[oreps[c]].ndofs) bad |= 4;
+      if ((int64_t)T.base + (int64_t)T.pat.tile.size() > D.ntiles) bad |= 4;
+      for (int d = 0; d < s.ndofs; d++) fix[foff[is] + d] = s.ifix[d] != 0;
+      ranges[is] = make_int2(T.pat.tile.front(), (int)T.pat.tile.size());
+    }
+    if (S.verbose >= 3) ph("  scatter tables");
     if (bad & 1) fatal("BDDCML_DEVICE_PLAN", "device triplets out of date");
     if (bad & 2) fatal("BDDCML_DEVICE_PLAN", "invalid permutation");
+    if (bad & 4) fatal("BDDCML_DEVICE_PLAN", "invalid subdomain tile range");
     {
       const int n = D.ntc;
       // factor, inverse diagonal and W tiles of the coarse matrix
       ranges[nsub] = make_int2(D.cbase, n * (n + 1) / 2 + n + n * (n - 1) / 2);
       for (int p = S.ndofc; p < D.NC; p++) {
         int T_ = D.cbase + lowidx(n, p / TILE, p / TILE);
-        sidx[doff[nsub] + (p - S.ndofc)] = (int64_t)T_ * TILE_ELEMS + (p % TILE) * TILE + (p % TILE);
+        sidx[p - S.ndofc] = (int64_t)T_ * TILE_ELEMS + (p % TILE) * TILE + (p % TILE);
       }
     }
     for (auto &r : ranges)
       if (r.x < 0 || (int64_t)r.x + r.y > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid tile range to clear");
     for (int64_t k : sidx)
       if (k < 0 || k >= D.ntiles * (int64_t)TILE_ELEMS) fatal("BDDCML_DEVICE_PLAN", "scatter index out of range");
-    for (int v : ttab)
-      if (v >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid tile table entry");
+    for (int k = 0; k < nrep; k++) {
+      const int span = (int)D.sub[oreps[k]].pat.tile.size();
+      for (size_t q = toff[k]; q < toff[k + 1]; q++)
+        if (ttab[q] >= span) fatal("BDDCML_DEVICE_PLAN", "invalid tile table entry");
+      for (size_t q = doff[k]; q < doff[k + 1]; q++)
+        if (pads[q] < 0 || pads[q] >= span * TILE_ELEMS) fatal("BDDCML_DEVICE_PLAN", "invalid padding entry");
+    }
+    if (S.verbose >= 3) ph("  scatter checks");
     D.plan.add(D.sc_perm, perm);
     D.plan.add(D.sc_fix, fix);
     D.plan.add(D.sc_ttab, ttab);
+    D.plan.add(D.sc_pads, pads);
     D.plan.add(D.sc_subs, ss);
     D.plan.add(D.zero_rng, ranges);
     D.plan.add(D.sc_idx, sidx);
-    D.plan.add(D.sc_val, sval);
   }
 
   ph("scatter entries");
   // ---- mirror jobs, S' jobs
-  std::vector<MirrorJob> ms, mc;
+  // S stays in its lower tiles (read by the symmetric GEMV and by S'); the
+  // coarse matrix is mirrored to full storage for its GEMV
+  std::vector<MirrorJob> mc;
   std::vector<int> spsub, spij, spdst;
+  int bad_s = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad_s)
+  for (int k = 0; k < (int)S.ord_reps.size(); k++) {
+    const SubTiles &T = D.sub[S.ord_reps[k]];
+    for (int J = 0; J < T.nti; J++)
+      for (int I = J; I < T.nti; I++)
+        if (T.pat.find(T.nto + I, T.nto + J) != T.slow + lowidx(T.nti, I, J)) bad_s = 1;
+  }
   for (int is = 0; is < nsub; is++) {
     const SubTiles &T = D.sub[is];
-    const int nto = T.nto, n = T.nti;
+    const SubTiles &R = D.sub[S.ord_reps[S.ord_cls[is]]];
+    if (T.slow - T.base != R.slow - R.base || T.slow < 0 ||
+        (int64_t)T.slow + (int64_t)T.nti * (T.nti + 1) / 2 > D.ntiles)
+      bad_s = 1;
+  }
+  if (bad_s) fatal("BDDCML_DEVICE_PLAN", "interface tiles of S are not packed by columns");
+  for (int is = 0; is < nsub; is++) {
+    const SubTiles &T = D.sub[is];
+    const int n = T.nti;
     for (int J = 0; J < n; J++)
       for (int I = J; I < n; I++) {
-        int src = T.pat.find(nto + I, nto + J);
-        ms.push_back({src, T.sfull + I * n + J, 0});
-        if (I > J) ms.push_back({src, T.sfull + J * n + I, 1});
         spsub.push_back(is);
         spij.push_back((I << 16) | J);
         spdst.push_back(T.P(I, J));
@@ -822,13 +901,11 @@ void device_setup_begin(Solver &S) {
         if (I > J) mc.push_back({src, D.cfull + J * n + I, 1});
       }
   }
-  for (auto *jobs : {&ms, &mc})
-    for (const MirrorJob &j : *jobs)
-      if (j.src < 0 || j.src >= D.ntiles || j.dst < 0 || j.dst >= D.ntiles)
-        fatal("BDDCML_DEVICE_PLAN", "invalid mirror job");
-  for (int d : spdst)
-    if (d < 0 || d >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid augmented tile");
-  D.plan.add(D.mir_s, ms);
+  for (const MirrorJob &j : mc)
+    if (j.src < 0 || j.src >= D.ntiles || j.dst < 0 || j.dst >= D.ntiles)
+      fatal("BDDCML_DEVICE_PLAN", "invalid mirror job");
+  for (int d : spdst)  // tiles of S' live in the factor pool
+    if (d < 0 || d >= (D.mixed ? D.ntiles32 : D.ntiles)) fatal("BDDCML_DEVICE_PLAN", "invalid augmented tile");
   D.plan.add(D.mir_c, mc);
   D.plan.add(D.sp_sub, spsub);
   D.plan.add(D.sp_ij, spij);
@@ -836,16 +913,20 @@ void device_setup_begin(Solver &S) {
   D.n_sp = (int)spsub.size();
 
   ph("mirror and update jobs");
+  const size_t plan_bytes = D.plan.host.size();
   D.plan.commit(st);
-  D.pool.alloc((size_t)D.ntiles * TILE_ELEMS);
-  if (D.mixed) D.pool32.alloc((size_t)std::max<int64_t>(D.ntiles32, 1) * TILE_ELEMS);
+  if (S.verbose >= 3) {
+    std::printf("   device plan: %.1f MB of plan arrays\n", plan_bytes / 1e6);
+    ph("  plan upload");
+  }
+  pool_alloc.wait();
   D.err.alloc(1);
   D.err.zero(st);
   ph("pool allocation");
   if (S.verbose >= 2)
-    std::printf("   device plan: %lld FP64 tiles (%.1f GB), %lld FP32 tiles (%.1f GB), %zu scatter entries\n",
+    std::printf("   device plan: %lld FP64 tiles (%.1f GB), %lld FP32 tiles (%.1f GB), %zu padding slots\n",
                 (long long)D.ntiles, D.ntiles * 32768.0 / 1e9, (long long)D.ntiles32,
-                D.ntiles32 * 16384.0 / 1e9, (size_t)D.sc_idx.n);
+                D.ntiles32 * 16384.0 / 1e9, (size_t)(D.sc_pads.n + D.sc_idx.n));
 
   // algorithmic work: 2*64^3 flops per tile product, 64^3/3 per tile Cholesky
   // plus 64^3/3 for its inverse, 2*64^3 per post multiply
@@ -875,8 +956,9 @@ void device_setup_begin(Solver &S) {
   HIPCHECK(hipEventCreateWithFlags(&D.ev_up, hipEventDisableTiming));
   HIPCHECK(hipEventRecord(ev[0], st));
   launch_zero_ranges(st, pool, D.zero_rng.p, (int)D.zero_rng.n);
-  launch_scatter_sub(st, pool, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p, D.err.p);
-  launch_scatter(st, pool, D.sc_idx.p, D.sc_val.p, (int64_t)D.sc_idx.n);
+  launch_scatter_sub(st, pool, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p,
+                     D.sc_pads.p, D.err.p);
+  launch_scatter_ones(st, pool, D.sc_idx.p, (int64_t)D.sc_idx.n);
   for (size_t J = 0; J < D.a_diag.size(); J++) {
     launch_tile_potrf(st, pool, D.a_diag[J].tasks.p, D.a_diag[J].pairs.p, D.a_diag[J].ntask, D.err.p);
     launch_tile_gemm(st, pool, D.a_off[J].tasks.p, D.a_off[J].pairs.p, D.a_off[J].ntask, GEMM_NT,
@@ -885,7 +967,6 @@ void device_setup_begin(Solver &S) {
   HIPCHECK(hipEventRecord(ev[1], st));
   launch_schur_gemm(st, pool, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask);
   HIPCHECK(hipEventRecord(ev[2], st));
-  launch_tile_mirror(st, pool, D.mir_s.p, (int)D.mir_s.n);
   HIPCHECK(hipGetLastError());
   S.t.plan += wall_time() - t0;
 }
@@ -916,7 +997,7 @@ void device_setup_finish(Solver &S) {
     d.NI = s.NI;
     d.nti = s.nti;
     d.nc = s.nconstr;
-    d.sfull = T.sfull;
+    d.slow = T.slow;
     d.pbase = T.pbase;
     d.plinv = T.linvP;
     d.ybase = T.ybase;
@@ -1006,19 +1087,42 @@ void device_setup_finish(Solver &S) {
   D.Phi.alloc(std::max<int64_t>(zo, 1));
   D.G.alloc(std::max<int64_t>(go, 1));
   D.Acs.alloc(std::max<int64_t>(go, 1));
-  // gemv rows
-  std::vector<GemvRow> sr, cr;
-  for (int is = 0; is < nsub; is++) {
-    const SubTiles &T = D.sub[is];
-    for (int I = 0; I < T.nti; I++) sr.push_back({T.sfull + I * T.nti, T.nti, is, I});
+  // symmetric S GEMV: tile columns, longest first, and reduce rows; coarse
+  // GEMV rows
+  std::vector<SymvCol> sc;
+  std::vector<SymvRow> sr;
+  std::vector<GemvRow> cr;
+  {
+    int64_t ps = 0;
+    for (int is = 0; is < nsub; is++) {
+      const SubTiles &T = D.sub[is];
+      for (int I = 0; I < T.nti; I++) sr.push_back({is, I, (int)ps, T.nti});
+      ps += (int64_t)T.nti * (T.nti + 1) / 2;
+    }
+    if (ps >= INT32_MAX) fatal("BDDCML_DEVICE_PLAN", "too many interface tiles");
+    for (int J = 0; J < D.maxnti; J++)
+      for (int is = 0; is < nsub; is++) {
+        const SubTiles &T = D.sub[is];
+        if (J < T.nti)
+          sc.push_back({T.slow + lowidx(T.nti, J, J), T.nti - J, is, J,
+                        0});  // pslot set below
+      }
+    std::vector<int> psub(nsub);
+    for (const SymvRow &r : sr) psub[r.mat] = r.psub;
+    for (SymvCol &c : sc) c.pslot = psub[c.mat] + lowidx(D.sub[c.mat].nti, c.J, c.J);
+    std::stable_sort(sc.begin(), sc.end(), [](const SymvCol &a, const SymvCol &b) { return a.nt > b.nt; });
+    for (const SymvCol &c : sc)
+      if (c.tile0 < 0 || (int64_t)c.tile0 + c.nt > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid GEMV column");
+    D.spart.alloc((size_t)std::max<int64_t>(ps, 1) * 64);
   }
   for (int I = 0; I < D.ntc; I++) cr.push_back({D.cfull + I * D.ntc, D.ntc, 0, I});
-  for (auto *rows : {&sr, &cr})
-    for (const GemvRow &g : *rows)
-      if (g.tile0 < 0 || (int64_t)g.tile0 + g.nt > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid GEMV row");
+  for (const GemvRow &g : cr)
+    if (g.tile0 < 0 || (int64_t)g.tile0 + g.nt > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid GEMV row");
+  D.scols.upload(sc, st);
   D.srows.upload(sr, st);
   D.crows.upload(cr, st);
   D.nsrows = (int)sr.size();
+  D.nscols = (int)sc.size();
   D.ncrows = (int)cr.size();
   ph("interface maps");
   // assembly CSR: unique interface dof -> local flat indices (subdomain order)
@@ -1042,7 +1146,7 @@ void device_setup_finish(Solver &S) {
     {
       int64_t po = 0;
       for (int is = 0; is < nsub; is++) {
-        D.hodesc[is] = OrderDesc{po, voff[is], S.subs[is].ndofs, 0};
+        D.hodesc[is] = OrderDesc{po, voff[is], S.subs[is].ndofs, D.perm_off[is]};
         po += S.subs[is].ndofs;
         D.maxndofs = std::max(D.maxndofs, S.subs[is].ndofs);
       }
@@ -1418,15 +1522,16 @@ void device_sub_matrix(Solver &S, int is, int which, double *out) {
   const Sub &s = S.subs[is];
   const SubTiles &T = D.sub[is];
   if (which != 'S') fatal("BDDCML_AMD_SUB_MATRIX", "only the Schur complement is stored explicitly");
-  const int n = T.nti, first = T.sfull;
-  std::vector<double> tiles((size_t)n * n * TILE_ELEMS);
-  HIPCHECK(hipMemcpy(tiles.data(), D.pool.p + (size_t)first * TILE_ELEMS,
+  const int n = T.nti;
+  std::vector<double> tiles((size_t)n * (n + 1) / 2 * TILE_ELEMS);
+  HIPCHECK(hipMemcpy(tiles.data(), D.pool.p + (size_t)T.slow * TILE_ELEMS,
                      tiles.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int a = 0; a < s.ni; a++)
     for (int b = 0; b < s.ni; b++) {
-      size_t t = (size_t)(a / TILE) * n + b / TILE;
+      const int p = std::max(a, b), q = std::min(a, b);  // lower triangle entry (p, q)
+      const size_t t = lowidx(n, p / TILE, q / TILE);
       out[(size_t)s.dev_ifc[a] * s.ni + s.dev_ifc[b]] =
-          tiles[t * TILE_ELEMS + (b % TILE) * TILE + a % TILE];
+          tiles[t * TILE_ELEMS + (q % TILE) * TILE + p % TILE];
     }
 }
 
@@ -1454,8 +1559,8 @@ static void op_schur(Solver &S, const double *x, double *y, bool timed = false) 
     HIPCHECK(hipEventCreate(&e1));
     HIPCHECK(hipEventRecord(e0, D.st));
   }
-  launch_gemv(D.st, D.pool.p, D.srows.p, D.nsrows, D.loff.p, D.NIs.p, D.map.p, nullptr, x, D.yl.p,
-              D.maxNI);
+  launch_symv(D.st, D.pool.p, D.scols.p, D.nscols, D.srows.p, D.nsrows, D.loff.p, D.map.p, x,
+              D.spart.p, D.yl.p);
   if (timed) {
     HIPCHECK(hipEventRecord(e1, D.st));
     HIPCHECK(hipEventSynchronize(e1));
@@ -1463,8 +1568,8 @@ static void op_schur(Solver &S, const double *x, double *y, bool timed = false) 
     HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
     S.t.ev_gemv_ms = ms;
     double tiles = 0;
-    for (const SubTiles &T : D.sub) tiles += (double)T.nti * T.nti;
-    // matrix tiles + gathered input + written output (8 bytes each)
+    for (const SubTiles &T : D.sub) tiles += (double)T.nti * (T.nti + 1) / 2;
+    // lower matrix tiles + gathered input + written output (8 bytes each)
     S.t.gemv_bytes = tiles * TILE_ELEMS * 8.0 + 16.0 * (double)D.LV;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -42,19 +42,24 @@ struct ScatterSub {
   const int *ai, *aj;
   const double *av;
   int nnz;
-  int poff;  // offset of the subdomain in perm / fix (dofs)
-  int toff;  // offset of its packed lower tile table
+  int foff;    // offset of the subdomain's Dirichlet flags (dofs)
+  int poff;    // offset of its class's permutation (dofs)
+  int toff;    // offset of its class's packed lower tile table (tile ids relative to base)
+  int padoff;  // offset of its class's padding diagonals (element index relative to base)
+  int npad;
+  int base;    // first tile of the subdomain
 };
 void launch_scatter_sub(hipStream_t st, double *pool, const ScatterSub *subs, int nsub,
-                        const int *perm, const char *fix, const int *ttab, int *err);
-void launch_scatter(hipStream_t st, double *pool, const int64_t *idx, const double *val,
-                    int64_t n);
+                        const int *perm, const char *fix, const int *ttab, const int *pads,
+                        int *err);
+// pool[idx[k]] = 1.0
+void launch_scatter_ones(hipStream_t st, double *pool, const int64_t *idx, int64_t n);
 
 // ---- interface preconditioner pieces ------------------------------------
 // Descriptor of one subdomain's dense interface matrices.
 struct SubDesc {
   int ni, NI, nti, nc;    // interface dofs, padded, tile count, constraints
-  int sfull;              // first tile of S (full, row-major tile order)
+  int slow;               // first tile of S (lower tiles, packed by columns)
   int pbase;              // first tile of the packed lower factor of S' (factor pool)
   int plinv;              // first of the nti inverse diagonal factor tiles (factor pool)
   int ybase;              // first tile of Z = S'^{-1} C^T (nti x ncT, row-major; factor pool)
@@ -117,14 +122,15 @@ size_t max_dynamic_lds();
 
 // ---- subdomain vectors between local dof order and device order ---------
 struct OrderDesc {
-  int64_t poff;  // offset of the subdomain's dofs in perm and in local-order arrays
+  int64_t poff;  // offset of the subdomain's dofs in local-order arrays
   int64_t voff;  // offset of its device-ordered vector
-  int ndofs, pad;
+  int ndofs;
+  int permoff;   // offset of its permutation in perm
 };
-// vec[voff + perm[poff + i]] = src[poff + i]
+// vec[voff + perm[permoff + i]] = src[poff + i]
 void launch_to_device_order(hipStream_t st, const OrderDesc *d, int nsub, const int *perm,
                             const double *src, double *vec, int maxndofs);
-// dst[poff + i] = vec[voff + perm[poff + i]]
+// dst[poff + i] = vec[voff + perm[permoff + i]]
 void launch_from_device_order(hipStream_t st, const OrderDesc *d, int nsub, const int *perm,
                               const double *vec, double *dst, int maxndofs);
 // every local copy of unique interface dof g: vec[pos[q]] = u[g], q in ptr[g]..ptr[g+1]
@@ -138,11 +144,26 @@ struct GemvRow {
   int mat;        // matrix index (subdomain) for gather
   int I;          // tile row
 };
-// y_local[rowoff + I*64 + r] = sum_J T(I,J) x_J, x_J gathered:
-// x[k] = (w ? w[loff+k] : 1) * (map ? xin[map[loff+k]] : xin[k])
-void launch_gemv(hipStream_t st, const double *pool, const GemvRow *rows, int nrows,
-                 const int64_t *loff, const int *NI, const int *map, const double *w,
-                 const double *xin, double *yout, int maxNI);
+// Symmetric tile GEMV over the packed lower tiles of each subdomain's S:
+// y_local[loff + I*64 + r] = sum_J S(I,J) x_J with x_J[r] = xin[map[loff + J*64 + r]]
+// (0 where map < 0).  One workgroup per tile column (SymvCol) writes partial
+// slots (I, J) of `part` (64 doubles each, packed like the tiles); the reduce
+// sums each row's slots in a fixed order.
+struct SymvCol {
+  int tile0;  // tile (J, J); the column's tiles (I, J), I = J..nti-1, follow
+  int nt;     // nti - J
+  int mat;    // subdomain
+  int J;
+  int pslot;  // partial slot of (J, J)
+};
+struct SymvRow {
+  int mat, I;
+  int psub;   // partial slot of the subdomain's (0, 0)
+  int nti;
+};
+void launch_symv(hipStream_t st, const double *pool, const SymvCol *cols, int ncols,
+                 const SymvRow *rows, int nrows, const int64_t *loff, const int *map,
+                 const double *xin, double *part, double *yout);
 // y[I*64 + r] = sum_J T(I,J) x_J with x read directly (coarse inverse)
 void launch_gemv_plain(hipStream_t st, const double *pool, const GemvRow *rows, int nrows,
                        const double *xin, double *yout);

@@ -16,7 +16,9 @@ __global__ void k_sprime(const double *__restrict__ pool, T *__restrict__ ppool,
   const int job = blockIdx.x;
   const SubDesc d = subs[jsub[job]];
   const int I = jij[job] >> 16, J = jij[job] & 0xffff;
-  const double *src = pool + (size_t)(d.sfull + I * d.nti + J) * TILE_ELEMS;
+  // jobs cover I >= J: tile (I, J) of the packed lower S
+  const double *src =
+      pool + (size_t)(d.slow + J * d.nti - J * (J - 1) / 2 + (I - J)) * TILE_ELEMS;
   T *dst = ppool + (size_t)jdst[job] * TILE_ELEMS;
   const int *ci = cid + d.loff;
   for (int e = threadIdx.x; e < TILE_ELEMS; e += blockDim.x) {
@@ -342,46 +344,99 @@ void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int 
 }
 
 // -------------------------------------------------------- iteration kernels
-__global__ __launch_bounds__(256) void k_gemv(const double *__restrict__ pool,
-                                              const GemvRow *__restrict__ rows,
+// y = S x for symmetric S held as its lower tiles.  Workgroup = tile column
+// J of one subdomain; wave w reads columns 16w..16w+15 of each tile (lane =
+// row, one coalesced 512-byte column per load), the next tile prefetched.
+// Slot (I, J), I > J, receives S_IJ x_J; slot (J, J) receives S_JJ x_J plus
+// sum_{I>J} S_IJ^T x_I (lane products summed over rows by butterflies).
+__global__ __launch_bounds__(256) void k_symv(const double *__restrict__ pool,
+                                              const SymvCol *__restrict__ cols,
                                               const int64_t *__restrict__ loff,
-                                              const int *__restrict__ NIs,
                                               const int *__restrict__ map,
-                                              const double *__restrict__ w,
                                               const double *__restrict__ xin,
-                                              double *__restrict__ yout) {
-  extern __shared__ double xs[];
-  __shared__ double red[256];
-  const GemvRow row = rows[blockIdx.x];
-  const int64_t lo = loff[row.mat];
-  const int NI = NIs[row.mat];
-  for (int k = threadIdx.x; k < NI; k += 256) {
-    double x;
-    if (map) {
-      int g = map[lo + k];
-      x = g >= 0 ? xin[g] : 0.0;
-    } else {
-      x = xin[k];
-    }
-    if (w) x *= w[lo + k];
-    xs[k] = x;
+                                              double *__restrict__ part) {
+  __shared__ double xj[64];
+  __shared__ double red[2][4][64];
+  __shared__ double dg[64];
+  const SymvCol col = cols[blockIdx.x];
+  const int64_t lo = loff[col.mat];
+  const int tid = threadIdx.x, w = tid >> 6, r = tid & 63;
+  const int *mp = map + lo;
+  if (tid < 64) {
+    const int g = mp[col.J * 64 + tid];
+    xj[tid] = g >= 0 ? xin[g] : 0.0;
   }
   __syncthreads();
-  const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
-  double s0 = 0.0, s1 = 0.0;
-  for (int J = g; J < row.nt; J += 4) {
-    const double *T = pool + (size_t)(row.tile0 + J) * TILE_ELEMS + r;
-    const double *x = xs + J * 64;
-#pragma unroll 8
-    for (int c = 0; c < 64; c += 2) {
-      s0 += T[c * 64] * x[c];
-      s1 += T[(c + 1) * 64] * x[c + 1];
+  double xc[16], acc[16], cur[16], nxt[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) xc[c] = xj[16 * w + c], acc[c] = 0.0;
+  const double *T0 = pool + (size_t)col.tile0 * TILE_ELEMS + (16 * w) * 64 + r;
+#pragma unroll
+  for (int c = 0; c < 16; c++) cur[c] = T0[c * 64];
+  double xi = 0.0;  // x_I[r] of the current tile (I > J)
+  for (int k = 0; k < col.nt; k++) {
+    double xn = 0.0;
+    if (k + 1 < col.nt) {
+      const double *Tn = T0 + (size_t)(k + 1) * TILE_ELEMS;
+#pragma unroll
+      for (int c = 0; c < 16; c++) nxt[c] = Tn[c * 64];
+      const int g = mp[(col.J + k + 1) * 64 + r];
+      xn = g >= 0 ? xin[g] : 0.0;
     }
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 16; c += 2) s0 += cur[c] * xc[c], s1 += cur[c + 1] * xc[c + 1];
+    if (k > 0) {
+#pragma unroll
+      for (int c = 0; c < 16; c++) acc[c] += cur[c] * xi;
+    }
+    red[k & 1][w][r] = s0 + s1;
+    __syncthreads();
+    if (w == 0) {
+      const double v = (red[k & 1][0][r] + red[k & 1][1][r]) + (red[k & 1][2][r] + red[k & 1][3][r]);
+      if (k == 0)
+        dg[r] = v;
+      else
+        part[(size_t)(col.pslot + k) * 64 + r] = v;
+    }
+#pragma unroll
+    for (int c = 0; c < 16; c++) cur[c] = nxt[c];
+    xi = xn;
   }
-  red[threadIdx.x] = s0 + s1;
+  double mine = 0.0;
+#pragma unroll
+  for (int c = 0; c < 16; c++) {
+    double v = acc[c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (r == c) mine = v;
+  }
   __syncthreads();
-  if (threadIdx.x < 64)
-    yout[lo + row.I * 64 + r] = (red[r] + red[64 + r]) + (red[128 + r] + red[192 + r]);
+  if (r < 16) part[(size_t)col.pslot * 64 + 16 * w + r] = dg[16 * w + r] + mine;
+}
+
+// y_local[loff + I*64 + r] = sum_{J <= I} slot(I, J)[r], J ascending; one
+// wave per row
+__global__ __launch_bounds__(256) void k_symv_reduce(const SymvRow *__restrict__ rows, int nrows,
+                                                     const int64_t *__restrict__ loff,
+                                                     const double *__restrict__ part,
+                                                     double *__restrict__ yout) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), r = threadIdx.x & 63;
+  if (row >= nrows) return;
+  const SymvRow R = rows[row];
+  double s = 0.0;
+  for (int J = 0; J <= R.I; J++)
+    s += part[(size_t)(R.psub + J * R.nti - J * (J - 1) / 2 + (R.I - J)) * 64 + r];
+  yout[loff[R.mat] + R.I * 64 + r] = s;
+}
+
+void launch_symv(hipStream_t st, const double *pool, const SymvCol *cols, int ncols,
+                 const SymvRow *rows, int nrows, const int64_t *loff, const int *map,
+                 const double *xin, double *part, double *yout) {
+  if (ncols <= 0 || nrows <= 0) return;
+  hipLaunchKernelGGL(k_symv, dim3(ncols), dim3(256), 0, st, pool, cols, loff, map, xin, part);
+  hipLaunchKernelGGL(k_symv_reduce, dim3((nrows + 3) / 4), dim3(256), 0, st, rows, nrows, loff,
+                     part, yout);
 }
 
 // dense GEMV over full-storage tile rows with the input read from global
@@ -569,15 +624,6 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
   }
 }
 
-void launch_gemv(hipStream_t st, const double *pool, const GemvRow *rows, int nrows,
-                 const int64_t *loff, const int *NI, const int *map, const double *w,
-                 const double *xin, double *yout, int maxNI) {
-  if (nrows <= 0) return;
-  const size_t sh = (size_t)maxNI * sizeof(double);
-  lds_request((const void *)k_gemv, sh + 256 * sizeof(double), "interface GEMV");
-  hipLaunchKernelGGL(k_gemv, dim3(nrows), dim3(256), sh, st, pool, rows, loff, NI, map, w, xin,
-                     yout);
-}
 void launch_assemble(hipStream_t st, int n, const int *ptr, const int64_t *src, const double *w,
                      const double *ylocal, double *y) {
   if (n <= 0) return;
@@ -731,13 +777,13 @@ __global__ void k_to_device_order(const OrderDesc *__restrict__ ds, const int *_
                                   const double *__restrict__ src, double *__restrict__ vec) {
   const OrderDesc d = ds[blockIdx.y];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < d.ndofs) vec[d.voff + perm[d.poff + i]] = src[d.poff + i];
+  if (i < d.ndofs) vec[d.voff + perm[d.permoff + i]] = src[d.poff + i];
 }
 __global__ void k_from_device_order(const OrderDesc *__restrict__ ds, const int *__restrict__ perm,
                                     const double *__restrict__ vec, double *__restrict__ dst) {
   const OrderDesc d = ds[blockIdx.y];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < d.ndofs) dst[d.poff + i] = vec[d.voff + perm[d.poff + i]];
+  if (i < d.ndofs) dst[d.poff + i] = vec[d.voff + perm[d.permoff + i]];
 }
 __global__ void k_ifc_scatter(int n, const int *__restrict__ ptr, const int64_t *__restrict__ pos,
                               const double *__restrict__ u, double *__restrict__ vec) {

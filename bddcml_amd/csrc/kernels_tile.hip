@@ -420,24 +420,31 @@ __global__ void k_zero_tiles(double *__restrict__ pool, const int *__restrict__ 
   for (int e = threadIdx.x; e < TILE_ELEMS / 2; e += blockDim.x) T[e] = make_double2(0.0, 0.0);
 }
 
-__global__ void k_scatter(double *__restrict__ pool, const int64_t *__restrict__ idx,
-                          const double *__restrict__ val, int64_t n) {
+__global__ void k_scatter_ones(double *__restrict__ pool, const int64_t *__restrict__ idx,
+                               int64_t n) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) pool[idx[k]] = val[k];
+  if (k < n) pool[idx[k]] = 1.0;
 }
 
 // Assembled subdomain triplets (upper triangle, local dof indices) into the
 // tile pool: Dirichlet elimination keeping the diagonal
 // (src/module_sm.f90:431-600), device permutation, both triangles inside
-// diagonal tiles.
+// diagonal tiles; then 1.0 on the diagonal of every padding slot (slots no
+// triplet reaches).
 __global__ void k_scatter_sub(double *__restrict__ pool, const ScatterSub *__restrict__ subs,
                               const int *__restrict__ perm, const char *__restrict__ fix,
-                              const int *__restrict__ ttab, int *__restrict__ err) {
+                              const int *__restrict__ ttab, const int *__restrict__ pads,
+                              int *__restrict__ err) {
   const ScatterSub d = subs[blockIdx.y];
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < d.nnz; e += gridDim.x * blockDim.x) {
+  double *P = pool + (size_t)d.base * TILE_ELEMS;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < d.nnz + d.npad; e += gridDim.x * blockDim.x) {
+    if (e >= d.nnz) {
+      P[pads[d.padoff + e - d.nnz]] = 1.0;
+      continue;
+    }
     int i = d.ai[e], j = d.aj[e];
     double v = d.av[e];
-    const bool fi = fix[d.poff + i] != 0, fj = fix[d.poff + j] != 0;
+    const bool fi = fix[d.foff + i] != 0, fj = fix[d.foff + j] != 0;
     if (i != j) {
       if (fi || fj) v = 0.0;
     } else if (fi && fabs(v) < 1e-12) {
@@ -455,8 +462,8 @@ __global__ void k_scatter_sub(double *__restrict__ pool, const ScatterSub *__res
       atomicOr(err, 4);
       continue;
     }
-    pool[(size_t)t * TILE_ELEMS + (b & 63) * 64 + (a & 63)] = v;
-    if (I == J && a != b) pool[(size_t)t * TILE_ELEMS + (a & 63) * 64 + (b & 63)] = v;
+    P[(size_t)t * TILE_ELEMS + (b & 63) * 64 + (a & 63)] = v;
+    if (I == J && a != b) P[(size_t)t * TILE_ELEMS + (a & 63) * 64 + (b & 63)] = v;
   }
 }
 
@@ -469,9 +476,11 @@ __global__ void k_zero_ranges(double *__restrict__ pool, const int2 *__restrict_
 }
 
 void launch_scatter_sub(hipStream_t st, double *pool, const ScatterSub *subs, int nsub,
-                        const int *perm, const char *fix, const int *ttab, int *err) {
+                        const int *perm, const char *fix, const int *ttab, const int *pads,
+                        int *err) {
   if (nsub <= 0) return;
-  hipLaunchKernelGGL(k_scatter_sub, dim3(64, nsub), dim3(256), 0, st, pool, subs, perm, fix, ttab, err);
+  hipLaunchKernelGGL(k_scatter_sub, dim3(64, nsub), dim3(256), 0, st, pool, subs, perm, fix, ttab,
+                     pads, err);
 }
 void launch_zero_ranges(hipStream_t st, double *pool, const int2 *ranges, int nranges) {
   if (nranges <= 0) return;
@@ -512,11 +521,10 @@ void launch_zero_tiles(hipStream_t st, double *pool, const int *ids, int n) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_zero_tiles, dim3(n), dim3(256), 0, st, pool, ids);
 }
-void launch_scatter(hipStream_t st, double *pool, const int64_t *idx, const double *val,
-                    int64_t n) {
+void launch_scatter_ones(hipStream_t st, double *pool, const int64_t *idx, int64_t n) {
   if (n <= 0) return;
   int64_t nb = (n + 255) / 256;
-  hipLaunchKernelGGL(k_scatter, dim3((unsigned)nb), dim3(256), 0, st, pool, idx, val, n);
+  hipLaunchKernelGGL(k_scatter_ones, dim3((unsigned)nb), dim3(256), 0, st, pool, idx, n);
 }
 
 }  // namespace bddc

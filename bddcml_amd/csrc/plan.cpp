@@ -394,16 +394,39 @@ static bool same_shape(const Sub &a, const Sub &b, int ndim) {
 
 void build_topology(Solver &S) {
   const double t_n2s = wall_time();
-  int ng = S.nnod_g;
+  // node -> subdomains (CSR, subdomains ascending per node): counts and
+  // slots by relaxed atomics, a blocked prefix sum, then a sort of each
+  // node's short list
+  const int ng = S.nnod_g;
   S.n2s_ptr.assign(ng + 1, 0);
-  for (auto &s : S.subs)
-    for (int g : s.isngn) S.n2s_ptr[g + 1]++;
-  for (int k = 0; k < ng; k++) S.n2s_ptr[k + 1] += S.n2s_ptr[k];
-  S.n2s_idx.assign(S.n2s_ptr[ng], 0);
+  int *cnt = S.n2s_ptr.data();
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int is = 0; is < S.nsub; is++)
+    for (int g : S.subs[is].isngn) __atomic_fetch_add(&cnt[g + 1], 1, __ATOMIC_RELAXED);
   {
-    std::vector<int> fill(ng, 0);
+    const int nb = 64, blk = (ng + nb) / nb;
+    std::vector<int> part(nb + 1, 0);
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < nb; b++) {
+      int acc = 0;
+      for (int k = b * blk; k < std::min(ng + 1, (b + 1) * blk); k++) acc += cnt[k], cnt[k] = acc;
+      part[b + 1] = acc;
+    }
+    for (int b = 0; b < nb; b++) part[b + 1] += part[b];
+#pragma omp parallel for schedule(static)
+    for (int b = 1; b < nb; b++)
+      for (int k = b * blk; k < std::min(ng + 1, (b + 1) * blk); k++) cnt[k] += part[b];
+  }
+  S.n2s_idx.resize(S.n2s_ptr[ng]);
+  {
+    std::vector<int> fill(S.n2s_ptr.begin(), S.n2s_ptr.end() - 1);
+    int *f = fill.data(), *idx = S.n2s_idx.data();
+#pragma omp parallel for schedule(dynamic, 4)
     for (int is = 0; is < S.nsub; is++)
-      for (int g : S.subs[is].isngn) S.n2s_idx[S.n2s_ptr[g] + fill[g]++] = is;
+      for (int g : S.subs[is].isngn) idx[__atomic_fetch_add(&f[g], 1, __ATOMIC_RELAXED)] = is;
+#pragma omp parallel for schedule(static, 4096)
+    for (int g = 0; g < ng; g++)
+      if (cnt[g + 1] - cnt[g] > 1) std::sort(idx + cnt[g], idx + cnt[g + 1]);
   }
   double tp = wall_time();
   auto phase = [&](const char *name) {

@@ -48,6 +48,7 @@ void bddcml_init_c(int *nl, int *nsublev, int *lnsublev, int *nsub_loc_1, int *c
   (void)comm_init;
   (void)just_direct_solve_int;
   reset_solver();
+  set_task_order(1);
   Solver &S = solver();
   if (*nl < 2) fatal("LEVELS_INIT", "Number of levels must be at least 2");
   if (*nl != 2) fatal("LEVELS_INIT", "This library implements two levels");
@@ -230,12 +231,10 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
     if (S.verbose >= 2) std::printf(" BDDCML_SETUP_PRECONDITIONER: %-22s %8.4f s\n", name, now - tp);
     tp = now;
   };
-  // what the interior factorization needs: interface split, constraint
-  // counts (tile layout), orderings
+  // what the interior factorization needs: the interior / interface split
+  // and the orderings; globs and constraints follow while the device factors
   build_topology(S);
   phase("topology");
-  build_constraints(S);
-  phase("constraints");
   build_orderings(S);
   phase("orderings");
   S.t.topology += wall_time() - t0;
@@ -245,6 +244,10 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
   }
   // host work that overlaps the device factorization
   double th = wall_time();
+  build_globs(S);
+  phase("globs");
+  build_constraints(S);
+  phase("constraints");
   apply_boundary_conditions(S);
   phase("boundary conditions");
   build_weights(S);
@@ -521,6 +524,10 @@ int bddcml_amd_set_option(const char *key, double value) try {
   }
   if (!std::strcmp(key, "verbose")) {
     S.verbose = (int)value;
+    return 0;
+  }
+  if (!std::strcmp(key, "task_order")) {
+    bddc::set_task_order(value != 0.0);
     return 0;
   }
   if (!std::strcmp(key, "plan_only")) {

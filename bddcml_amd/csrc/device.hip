@@ -27,6 +27,7 @@
 #include <cstring>
 #include <exception>
 #include <utility>
+#include <map>
 #include <numeric>
 #include <thread>
 #include <unordered_map>
@@ -149,8 +150,9 @@ struct SubTiles {
   TilePattern pat;   // A pattern (lower), tile ids absolute
   std::vector<std::vector<uint64_t>> col;  // column bitsets
   int base = 0, span = 0;  // first tile id and tile count of the subdomain
-  // factor space (S' factor, its inverse diagonal tiles, Z): the FP64 pool,
-  // or the FP32 pool in the mixed-precision mode
+  // factor space (S' factor, its inverse diagonal tiles, Z, G^{-1}, Phi):
+  // a pool of its own (FP64, or FP32 in the mixed-precision mode), laid out
+  // per subdomain by FactorLayout from fbase
   int fbase = 0, fspan = 0;
   // slow: tile (nto, nto), the first of S's lower tiles; the interface tile
   // columns are dense, so tile (nto + I, nto + J) is slow + lowidx(nti, I, J)
@@ -171,11 +173,15 @@ struct Device {
   hipStream_t up = nullptr;  // uploads of the second setup half
   hipEvent_t ev[5] = {};     // setup phase boundaries (timing)
   hipEvent_t ev_up = nullptr;
-  DBuf<double> pool;
+  DBuf<double> pool;      // subdomain tiles: pattern of A (factor, S), inverse diagonal tiles
   int64_t ntiles = 0;
   bool mixed = false;     // S' factor, its inverse diagonal tiles and Z in FP32
-  DBuf<float> pool32;     // factor space of the mixed-precision mode
-  int64_t ntiles32 = 0;
+  DBuf<double> fpool;     // factor space, FP64 mode
+  DBuf<float> pool32;     // factor space, mixed-precision mode
+  int64_t ntilesf = 0;
+  DBuf<double> cpool;     // coarse matrix: factor, inverse diagonal, W, full inverse
+  int64_t ntilesc = 0;
+  Arena plan2;            // task lists and job lists of the second half
   DBuf<int> err;
   std::vector<SubTiles> sub;
   // setup phases
@@ -188,7 +194,8 @@ struct Device {
   View<char> sc_fix;
   View<ScatterSub> sc_subs;
   View<int2> zero_rng;
-  View<int64_t> sc_idx;                 // padding diagonals of the coarse matrix
+  View<int2> c_zero;                    // coarse tiles cleared before assembly
+  View<int64_t> c_pads;                 // padding diagonals of the coarse matrix
   View<MirrorJob> mir_c;
   View<int> sp_sub, sp_ij, sp_dst;
   int n_sp = 0;
@@ -306,7 +313,7 @@ static void symbolic(const Sub &s, SubTiles &T) {
   }
 }
 
-static void assign_tiles(SubTiles &T, int nc, int &next, int &next32, bool mixed) {
+static void assign_tiles(SubTiles &T, int &next) {
   const int nt = T.nto + T.nti;
   T.base = next;
   T.pat.nt = nt;
@@ -325,43 +332,35 @@ static void assign_tiles(SubTiles &T, int nc, int &next, int &next32, bool mixed
   T.linvA = next;
   next += T.nto;
   T.slow = T.pat.find(T.nto, T.nto);
-  int &nf = mixed ? next32 : next;
-  const int f0 = nf;
-  T.pbase = nf;
-  nf += T.nti * (T.nti + 1) / 2;
-  T.linvP = nf;
-  nf += T.nti;
-  T.ncT = (nc + TILE - 1) / TILE;
-  T.ybase = nf;
-  nf += T.nti * T.ncT;
-  T.gtbase = nf;
-  nf += T.ncT * T.ncT;
-  T.phibase = nf;
-  nf += T.nti * T.ncT;
   T.span = next - T.base;
-  if (mixed) {
-    T.fbase = f0;
-    T.fspan = nf - f0;
-  } else {
-    T.fbase = T.base;
-    T.fspan = T.span;
-  }
 }
 
-// Structural identity of two subdomains' tile layouts: same tile pattern,
-// interface and constraint tile counts, hence the same relative tile ids.
-static uint64_t structure_hash(const SubTiles &T) {
-  uint64_t h = 1469598103934665603ull;
-  auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
-  mix(T.nto), mix(T.nti), mix(T.ncT), mix(T.span), mix(T.fspan);
-  for (int v : T.pat.colptr) mix((uint64_t)v);
-  for (int v : T.pat.rowidx) mix((uint64_t)v);
-  return h;
-}
-static bool same_structure(const SubTiles &a, const SubTiles &b) {
-  return a.nto == b.nto && a.nti == b.nti && a.ncT == b.ncT && a.span == b.span &&
-         a.fspan == b.fspan &&
-         a.pat.colptr == b.pat.colptr && a.pat.rowidx == b.pat.rowidx;
+// Factor space of one subdomain relative to its first tile: P (S' factor,
+// packed lower), LinvP (inverse diagonal tiles), Y (Z = S'^{-1} C^T, nti x
+// ncT), Gt (G^{-1}, ncT x ncT), Phi (nti x ncT)
+struct FactorLayout {
+  int p, linv, y, gt, phi, span;
+  FactorLayout(int nti, int ncT) {
+    p = 0;
+    linv = nti * (nti + 1) / 2;
+    y = linv + nti;
+    gt = y + nti * ncT;
+    phi = gt + ncT * ncT;
+    span = phi + nti * ncT;
+  }
+};
+
+static void assign_factor(SubTiles &T, int nc, int64_t &nf) {
+  T.ncT = (nc + TILE - 1) / TILE;
+  const FactorLayout L(T.nti, T.ncT);
+  T.fbase = (int)nf;
+  T.fspan = L.span;
+  T.pbase = T.fbase + L.p;
+  T.linvP = T.fbase + L.linv;
+  T.ybase = T.fbase + L.y;
+  T.gtbase = T.fbase + L.gt;
+  T.phibase = T.fbase + L.phi;
+  nf += L.span;
 }
 
 // tasks of a dense lower Cholesky on tiles P(I,J) with inverse tiles Linv(J)
@@ -399,14 +398,101 @@ static void upload_phases(std::vector<Phase> &dst, const std::vector<TaskList> &
   for (size_t k = 0; k < src.size(); k++) dst[k].upload(src[k], a);
 }
 
+// Makes every tile id of a class list relative to the subdomain's first
+// tile b0; false if an id leaves [b0, b0 + nb) or a pair range is invalid.
+static bool relative_ids(TaskList &L, int b0, int nb) {
+  auto inside = [&](int id) { return id >= b0 && id < b0 + nb; };
+  bool ok = true;
+  for (TileTask &t : L.tasks) {
+    if (!inside(t.c) || (t.x != -1 && !inside(t.x)) || t.p0 < 0 || t.p1 < t.p0 ||
+        t.p1 > (int)L.pairs.size())
+      ok = false;
+    t.c -= b0;
+    if (t.x >= 0) t.x -= b0;
+  }
+  for (TilePair &p : L.pairs) {
+    if (!inside(p.a) || !inside(p.b)) ok = false;
+    p.a -= b0;
+    p.b -= b0;
+  }
+  return ok;
+}
+
+// One launch over every subdomain from the class lists (tile ids relative
+// to the subdomain's base): the pairs once per class, the tasks per
+// subdomain with its base added, subdomain-major.
+template <class Q, class Get, class Base>
+static TaskList materialize(const std::vector<Q> &sts, Get get, const std::vector<int> &cls,
+                            Base base_of) {
+  const int ncls = (int)sts.size(), nsub = (int)cls.size();
+  TaskList out;
+  std::vector<size_t> poff(ncls + 1, 0), toff(nsub + 1, 0);
+  for (int ic = 0; ic < ncls; ic++) {
+    const TaskList *L = get(sts[ic]);
+    poff[ic + 1] = poff[ic] + (L ? L->pairs.size() : 0);
+  }
+  for (int is = 0; is < nsub; is++) {
+    const TaskList *L = get(sts[cls[is]]);
+    toff[is + 1] = toff[is] + (L ? L->tasks.size() : 0);
+  }
+  out.pairs.resize(poff[ncls]);
+  out.tasks.resize(toff[nsub]);
+  for (int ic = 0; ic < ncls; ic++)
+    if (const TaskList *L = get(sts[ic]))
+      std::copy(L->pairs.begin(), L->pairs.end(), out.pairs.begin() + poff[ic]);
+#pragma omp parallel for schedule(static)
+  for (int is = 0; is < nsub; is++) {
+    const TaskList *L = get(sts[cls[is]]);
+    if (!L) continue;
+    const int b = base_of(is), po = (int)poff[cls[is]];
+    TileTask *o = out.tasks.data() + toff[is];
+    for (const TileTask &t : L->tasks)
+      *o++ = TileTask{t.c + b, t.x >= 0 ? t.x + b : -1, t.p0 + po, t.p1 + po, b};
+  }
+  return out;
+}
+
+// the same for every step of a multi-step member (steps of different
+// classes aligned by index)
+template <class Q, class Base>
+static std::vector<TaskList> merge_steps(const std::vector<Q> &sts, std::vector<TaskList> Q::*member,
+                                         const std::vector<int> &cls, Base base_of) {
+  size_t nsteps = 0;
+  for (auto &q : sts) nsteps = std::max(nsteps, (q.*member).size());
+  std::vector<TaskList> out(nsteps);
+  for (size_t k = 0; k < nsteps; k++)
+    out[k] = materialize(
+        sts,
+        [&](const Q &q) -> const TaskList * { return k < (q.*member).size() ? &(q.*member)[k] : nullptr; },
+        cls, base_of);
+  return out;
+}
+
+// algorithmic work: 2*64^3 flops per tile product, 64^3/3 per tile Cholesky
+// plus 64^3/3 for its inverse, 2*64^3 per post multiply; pair lists are
+// shared between subdomains, so the pairs of each task are counted
+static const double kTileGemm = 2.0 * 64 * 64 * 64, kTileChol = 64.0 * 64 * 64 / 3.0;
+static double tile_products(const TaskList &l) {
+  double n = 0;
+  for (const TileTask &t : l.tasks) n += t.p1 - t.p0;
+  return n;
+}
+static double tile_work(const std::vector<TaskList> &v, double per_task) {
+  double f = 0;
+  for (auto &l : v) f += kTileGemm * tile_products(l) + per_task * l.tasks.size();
+  return f;
+}
+
 // ------------------------------------------------------------------ setup
 // Setup runs in two halves.  device_setup_begin needs only the interface
-// split, the constraint counts and the orderings: it builds the tile layout
-// and the task lists and launches the interior factorization and the Schur
-// complements without waiting for them.  The host then prepares weights,
-// Dirichlet data and the condensation right-hand side while the device
-// factors; device_setup_finish builds the interface and coarse maps
-// (uploaded on a second stream) and launches the rest.
+// split and the orderings: it builds the subdomain tile layout and the task
+// lists of the interior factorization and the Schur complements, and
+// launches them without waiting.  The host then classifies globs, builds
+// the constraints, weights, Dirichlet data and the condensation right-hand
+// side while the device factors; device_setup_finish lays out the factor
+// space and the coarse tiles (pools of their own, sized by the constraints),
+// builds their task lists and the interface and coarse maps (uploaded on a
+// second stream) and launches the rest.
 void device_setup_begin(Solver &S) {
   device_release(S);
   S.dev.reset(new Device());
@@ -437,26 +523,15 @@ void device_setup_begin(Solver &S) {
     const int r = S.ord_reps[S.ord_cls[is]];
     if (r != is) D.sub[is] = D.sub[r];
   }
-  int next = 0, next32 = 0;
+  int next = 0;
   D.mixed = S.precision == 32;
-  for (int is = 0; is < nsub; is++) assign_tiles(D.sub[is], S.subs[is].nconstr, next, next32, D.mixed);
-  D.ntiles32 = next32;
-  D.NC = std::max(TILE, (S.ndofc + TILE - 1) / TILE * TILE);
-  D.ntc = D.NC / TILE;
-  D.cbase = next;
-  next += D.ntc * (D.ntc + 1) / 2;
-  D.clinv = next;
-  next += D.ntc;
-  D.cwbase = next;
-  next += D.ntc * (D.ntc - 1) / 2;
-  D.cfull = next;
-  next += D.ntc * D.ntc;
+  for (int is = 0; is < nsub; is++) assign_tiles(D.sub[is], next);
   D.ntiles = next;
 
   ph("symbolic");
-  // ---- the tile pools are allocated on a helper thread while the host
-  // builds the task lists and tables below (a large hipMalloc takes
-  // milliseconds); joined before the first launch, also on error paths.
+  // ---- the tile pool is allocated on a helper thread while the host builds
+  // the task lists and tables below (a large hipMalloc takes milliseconds);
+  // joined before the first launch, also on error paths.
   struct PoolAlloc {
     std::thread th;
     std::exception_ptr err;
@@ -475,41 +550,21 @@ void device_setup_begin(Solver &S) {
       try {
         HIPCHECK(hipSetDevice(dev));
         D.pool.alloc((size_t)D.ntiles * TILE_ELEMS);
-        if (D.mixed) D.pool32.alloc((size_t)std::max<int64_t>(D.ntiles32, 1) * TILE_ELEMS);
       } catch (...) {
         pool_alloc.err = std::current_exception();
       }
     });
   }
-  // ---- task lists.  Subdomains of identical tile structure form a class
-  // and share one list of tile pairs, kept relative to the subdomain's first
+  // ---- task lists of the interior factorization and the Schur complements.
+  // Identically ordered subdomains have identical tile patterns: each class
+  // shares one list of tile pairs, kept relative to the subdomain's first
   // tile; the tasks (output tile, inverse tile, pair range, tile base) are
   // materialised per subdomain, subdomain-major within each step.
-  std::vector<int> cls(nsub, -1), reps;
-  {
-    std::vector<uint64_t> h(nsub);
-#pragma omp parallel for schedule(static)
-    for (int is = 0; is < nsub; is++) h[is] = structure_hash(D.sub[is]);
-    std::unordered_map<uint64_t, std::vector<int>> byhash;
-    for (int is = 0; is < nsub; is++) {
-      auto &cands = byhash[h[is]];
-      for (int r : cands)
-        if (same_structure(D.sub[r], D.sub[is])) {
-          cls[is] = cls[r];
-          break;
-        }
-      if (cls[is] < 0) {
-        cls[is] = (int)reps.size();
-        reps.push_back(is);
-        cands.push_back(is);
-      }
-    }
-  }
+  const std::vector<int> &cls = S.ord_cls, &reps = S.ord_reps;
   const int ncls = (int)reps.size();
-  if (S.verbose >= 3) ph("  structure classes");
   struct SubTasks {
-    std::vector<TaskList> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd;
-    TaskList a_schur, phi;
+    std::vector<TaskList> a_diag, a_off;
+    TaskList a_schur;
   };
   std::vector<SubTasks> sts(ncls);
   int bad_ids = 0;
@@ -568,192 +623,31 @@ void device_setup_begin(Solver &S) {
         t.p1 = (int)Q.a_schur.pairs.size();
         if (t.p1 > t.p0) Q.a_schur.tasks.push_back(t);
       }
-    const SubTiles *Tp = &T;
-    dense_chol_tasks(
-        T.nti, [Tp](int I, int J) { return Tp->P(I, J); }, [Tp](int J) { return Tp->linvP + J; },
-        Q.p_diag, Q.p_off);
-    // Z = S'^{-1} C^T by tile sweeps over the factor: forward steps
-    // Y_I = Linv_I (Y_I - sum_{k<I} L_Ik Y_k), backward steps
-    // Y_I = Linv_I^T (Y_I - sum_{k>I} L_kI^T Y_k), one task per tile column
-    const int n = T.nti;
-    Q.z_fwd.resize(T.ncT ? n : 0);
-    Q.z_bwd.resize(T.ncT ? n : 0);
-    for (int cT = 0; cT < T.ncT; cT++)
-      for (int I = 0; I < n; I++) {
-        TaskList &F = Q.z_fwd[I];
-        TileTask t{T.Y(I, cT), T.linvP + I, (int)F.pairs.size(), 0};
-        for (int k = 0; k < I; k++) F.pairs.push_back({T.P(I, k), T.Y(k, cT)});
-        t.p1 = (int)F.pairs.size();
-        F.tasks.push_back(t);
-        TaskList &B = Q.z_bwd[n - 1 - I];
-        TileTask u{T.Y(I, cT), T.linvP + I, (int)B.pairs.size(), 0};
-        for (int k = I + 1; k < n; k++) B.pairs.push_back({T.P(k, I), T.Y(k, cT)});
-        u.p1 = (int)B.pairs.size();
-        B.tasks.push_back(u);
-      }
-    // Phi = Z G^{-1} on tiles: task (I, CT) sums Z(I, KT) Ginv(KT, CT)
-    for (int I = 0; I < n; I++)
-      for (int CT = 0; CT < T.ncT; CT++) {
-        TileTask t{T.phibase + I * T.ncT + CT, -1, (int)Q.phi.pairs.size(), 0};
-        for (int KT = 0; KT < T.ncT; KT++)
-          Q.phi.pairs.push_back({T.Y(I, KT), T.gtbase + KT * T.ncT + CT});
-        t.p1 = (int)Q.phi.pairs.size();
-        Q.phi.tasks.push_back(t);
-      }
-    // make every tile id relative to the subdomain's first tile, checking
-    // that it stays inside the subdomain's tiles
-    auto relative = [&](TaskList &L, int b0, int nb) {
-      auto inside = [&](int id) { return id >= b0 && id < b0 + nb; };
-      for (TileTask &t : L.tasks) {
-        if (!inside(t.c) || (t.x != -1 && !inside(t.x)) || t.p0 < 0 || t.p1 < t.p0 ||
-            t.p1 > (int)L.pairs.size())
-          bad_ids |= 1;
-        t.c -= b0;
-        if (t.x >= 0) t.x -= b0;
-      }
-      for (TilePair &p : L.pairs) {
-        if (!inside(p.a) || !inside(p.b)) bad_ids |= 1;
-        p.a -= b0;
-        p.b -= b0;
-      }
-    };
     for (auto *v : {&Q.a_diag, &Q.a_off})
-      for (TaskList &L : *v) relative(L, T.base, T.span);
-    relative(Q.a_schur, T.base, T.span);
-    for (auto *v : {&Q.p_diag, &Q.p_off, &Q.z_fwd, &Q.z_bwd})
-      for (TaskList &L : *v) relative(L, T.fbase, T.fspan);
-    relative(Q.phi, T.fbase, T.fspan);
+      for (TaskList &L : *v)
+        if (!relative_ids(L, T.base, T.span)) bad_ids |= 1;
+    if (!relative_ids(Q.a_schur, T.base, T.span)) bad_ids |= 1;
   }
   if (bad_ids) fatal("BDDCML_DEVICE_PLAN", "tile task outside its subdomain's tiles");
   if (S.verbose >= 3) ph("  class task lists");
-  for (int is = 0; is < nsub; is++)
-    if ((int64_t)D.sub[is].base + D.sub[is].span > D.ntiles ||
-        (int64_t)D.sub[is].fbase + D.sub[is].fspan > (D.mixed ? D.ntiles32 : D.ntiles))
+  for (int is = 0; is < nsub; is++) {
+    const SubTiles &T = D.sub[is], &R = D.sub[reps[cls[is]]];
+    if ((int64_t)T.base + T.span > D.ntiles || T.span != R.span)
       fatal("BDDCML_DEVICE_PLAN", "tile range");
-  // one step of every subdomain: shared pairs, per-subdomain tasks
-  auto materialize = [&](auto get, bool factor) {
-    TaskList out;
-    std::vector<size_t> poff(ncls + 1, 0), toff(nsub + 1, 0);
-    for (int ic = 0; ic < ncls; ic++) {
-      const TaskList *L = get(sts[ic]);
-      poff[ic + 1] = poff[ic] + (L ? L->pairs.size() : 0);
-    }
-    for (int is = 0; is < nsub; is++) {
-      const TaskList *L = get(sts[cls[is]]);
-      toff[is + 1] = toff[is] + (L ? L->tasks.size() : 0);
-    }
-    out.pairs.resize(poff[ncls]);
-    out.tasks.resize(toff[nsub]);
-    for (int ic = 0; ic < ncls; ic++)
-      if (const TaskList *L = get(sts[ic])) std::copy(L->pairs.begin(), L->pairs.end(), out.pairs.begin() + poff[ic]);
-#pragma omp parallel for schedule(static)
-    for (int is = 0; is < nsub; is++) {
-      const TaskList *L = get(sts[cls[is]]);
-      if (!L) continue;
-      const int b = factor ? D.sub[is].fbase : D.sub[is].base, po = (int)poff[cls[is]];
-      TileTask *o = out.tasks.data() + toff[is];
-      for (const TileTask &t : L->tasks)
-        *o++ = TileTask{t.c + b, t.x >= 0 ? t.x + b : -1, t.p0 + po, t.p1 + po, b};
-    }
-    return out;
-  };
-  auto merge_steps = [&](std::vector<TaskList> SubTasks::*member, bool factor) {
-    size_t nsteps = 0;
-    for (auto &Q : sts) nsteps = std::max(nsteps, (Q.*member).size());
-    std::vector<TaskList> out(nsteps);
-    for (size_t k = 0; k < nsteps; k++)
-      out[k] = materialize(
-          [&](const SubTasks &Q) -> const TaskList * {
-            return k < (Q.*member).size() ? &(Q.*member)[k] : nullptr;
-          },
-          factor);
-    return out;
-  };
-  std::vector<TaskList> a_diag = merge_steps(&SubTasks::a_diag, false),
-                        a_off = merge_steps(&SubTasks::a_off, false),
-                        p_diag = merge_steps(&SubTasks::p_diag, true),
-                        p_off = merge_steps(&SubTasks::p_off, true),
-                        z_fwd = merge_steps(&SubTasks::z_fwd, true),
-                        z_bwd = merge_steps(&SubTasks::z_bwd, true);
-  TaskList a_schur =
-      materialize([](const SubTasks &Q) -> const TaskList * { return &Q.a_schur; }, false);
-  TaskList phi = materialize([](const SubTasks &Q) -> const TaskList * { return &Q.phi; }, true);
+  }
+  auto base_of = [&](int is) { return D.sub[is].base; };
+  std::vector<TaskList> a_diag = merge_steps(sts, &SubTasks::a_diag, cls, base_of),
+                        a_off = merge_steps(sts, &SubTasks::a_off, cls, base_of);
+  TaskList a_schur = materialize(
+      sts, [](const SubTasks &Q) -> const TaskList * { return &Q.a_schur; }, cls, base_of);
   sts.clear();
   if (S.verbose >= 3) ph("  materialized");
   if (S.verbose >= 2)
     std::printf("   device plan: %d structure classes over %d subdomains, %zu interior steps\n", ncls,
                 nsub, a_diag.size());
-  // coarse problem (one dense matrix): right-looking tile Cholesky and the
-  // inverse factor W = L^{-1} by right-looking forward substitution, so that
-  // every task is a single tile product and each step is a few short
-  // launches; then Ac^{-1} = W^T W
-  std::vector<SeqLaunch> cseq;
-  TaskList cinv;
-  {
-    const int n = D.ntc, cb = D.cbase, cl = D.clinv, cw = D.cwbase;
-    auto Pc = [=](int I, int J) { return cb + lowidx(n, I, J); };
-    auto Wc = [=](int I, int J) { return I == J ? cl + I : cw + lowidx(n - 1, I - 1, J); };
-    auto one = [](TaskList &l, int c, int a, int b) {
-      const int p0 = (int)l.pairs.size();
-      l.pairs.push_back({a, b});
-      l.tasks.push_back({c, -1, p0, p0 + 1});
-    };
-    for (int k = 0; k < n; k++) {
-      SeqLaunch f;  // L_kk = chol(A_kk), Linv_k
-      f.potrf = true;
-      f.l.tasks.push_back({Pc(k, k), cl + k, 0, 0});
-      cseq.push_back(std::move(f));
-      if (k + 1 < n) {  // L_ik = A_ik Linv_k^T
-        SeqLaunch pn;
-        pn.post = POST_RIGHT_XT, pn.beta = 0.0;
-        for (int i = k + 1; i < n; i++) pn.l.tasks.push_back({Pc(i, k), cl + k, 0, 0});
-        cseq.push_back(std::move(pn));
-      }
-      if (k > 0) {  // W_kj = Linv_k B_kj
-        SeqLaunch wr;
-        wr.kind = GEMM_NN, wr.post = POST_LEFT_X, wr.beta = 0.0;
-        for (int j = 0; j < k; j++) wr.l.tasks.push_back({Wc(k, j), cl + k, 0, 0});
-        cseq.push_back(std::move(wr));
-      }
-      if (k + 1 < n) {
-        SeqLaunch tr;  // A_IK -= L_Ik L_Kk^T
-        for (int K = k + 1; K < n; K++)
-          for (int I = K; I < n; I++) one(tr.l, Pc(I, K), Pc(I, k), Pc(K, k));
-        cseq.push_back(std::move(tr));
-        SeqLaunch bu;  // B_ij -= L_ik W_kj
-        bu.kind = GEMM_NN;
-        for (int i = k + 1; i < n; i++)
-          for (int j = 0; j <= k; j++) one(bu.l, Wc(i, j), Pc(i, k), Wc(k, j));
-        cseq.push_back(std::move(bu));
-      }
-    }
-    for (int j = 0; j < n; j++)
-      for (int i = j; i < n; i++) {
-        TileTask t{Pc(i, j), -1, (int)cinv.pairs.size(), 0};
-        for (int k = i; k < n; k++) cinv.pairs.push_back({Wc(k, i), Wc(k, j)});
-        t.p1 = (int)cinv.pairs.size();
-        cinv.tasks.push_back(t);
-      }
-  }
-  for (auto &q : cseq) check_tasks(q.l, D.ntiles, q.potrf || q.post != POST_NONE, "coarse factorization");
-  check_tasks(cinv, D.ntiles, false, "coarse W^T W");
-  if (S.verbose >= 3) ph("  coarse sequence");
   upload_phases(D.a_diag, a_diag, D.plan);
   upload_phases(D.a_off, a_off, D.plan);
-  upload_phases(D.p_diag, p_diag, D.plan);
-  upload_phases(D.p_off, p_off, D.plan);
-  upload_phases(D.z_fwd, z_fwd, D.plan);
-  upload_phases(D.z_bwd, z_bwd, D.plan);
-  D.cseq.resize(cseq.size());
-  for (size_t k = 0; k < cseq.size(); k++) {
-    SeqPhase &q = D.cseq[k];
-    q.ph.upload(cseq[k].l, D.plan);
-    q.kind = cseq[k].kind, q.post = cseq[k].post, q.alpha = cseq[k].alpha, q.beta = cseq[k].beta;
-    q.potrf = cseq[k].potrf;
-  }
   D.a_schur.upload(a_schur, D.plan);
-  D.phi.upload(phi, D.plan);
-  D.cinv.upload(cinv, D.plan);
 
   ph("task lists");
   // ---- matrix entries: the assembled triplets are already on the device
@@ -761,8 +655,7 @@ void device_setup_begin(Solver &S) {
   // Dirichlet flags and the tile tables go up.  The permutation, the packed
   // lower tile table (tile ids relative to the subdomain's first tile) and
   // the padding diagonals (1.0) are shared by each class of identically
-  // ordered subdomains; the Dirichlet flags are per subdomain.  Padding
-  // diagonals of the coarse matrix come from a short host list.
+  // ordered subdomains; the Dirichlet flags are per subdomain.
   if (!S.din)
     for (int is = 0; is < nsub; is++) device_upload_entries(S, is);
   {
@@ -778,12 +671,10 @@ void device_setup_begin(Solver &S) {
       toff[k + 1] = toff[k] + nt * (nt + 1) / 2;
       doff[k + 1] = doff[k] + s.pad.size() + (s.NI - s.ni);
     }
-    const int npad_c = D.NC - S.ndofc;
     std::vector<int> perm(poff[nrep]), ttab(toff[nrep], -1), pads(doff[nrep]);
     std::vector<char> fix(foff[nsub]);
     std::vector<ScatterSub> ss(nsub);
-    std::vector<int2> ranges(nsub + 1);
-    std::vector<int64_t> sidx(npad_c);
+    std::vector<int2> ranges(nsub);
     if (S.verbose >= 3) ph("  scatter allocation");
     int bad = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad)
@@ -830,19 +721,8 @@ void device_setup_begin(Solver &S) {
     if (bad & 1) fatal("BDDCML_DEVICE_PLAN", "device triplets out of date");
     if (bad & 2) fatal("BDDCML_DEVICE_PLAN", "invalid permutation");
     if (bad & 4) fatal("BDDCML_DEVICE_PLAN", "invalid subdomain tile range");
-    {
-      const int n = D.ntc;
-      // factor, inverse diagonal and W tiles of the coarse matrix
-      ranges[nsub] = make_int2(D.cbase, n * (n + 1) / 2 + n + n * (n - 1) / 2);
-      for (int p = S.ndofc; p < D.NC; p++) {
-        int T_ = D.cbase + lowidx(n, p / TILE, p / TILE);
-        sidx[p - S.ndofc] = (int64_t)T_ * TILE_ELEMS + (p % TILE) * TILE + (p % TILE);
-      }
-    }
     for (auto &r : ranges)
       if (r.x < 0 || (int64_t)r.x + r.y > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid tile range to clear");
-    for (int64_t k : sidx)
-      if (k < 0 || k >= D.ntiles * (int64_t)TILE_ELEMS) fatal("BDDCML_DEVICE_PLAN", "scatter index out of range");
     for (int k = 0; k < nrep; k++) {
       const int span = (int)D.sub[oreps[k]].pat.tile.size();
       for (size_t q = toff[k]; q < toff[k + 1]; q++)
@@ -857,62 +737,27 @@ void device_setup_begin(Solver &S) {
     D.plan.add(D.sc_pads, pads);
     D.plan.add(D.sc_subs, ss);
     D.plan.add(D.zero_rng, ranges);
-    D.plan.add(D.sc_idx, sidx);
   }
 
   ph("scatter entries");
-  // ---- mirror jobs, S' jobs
-  // S stays in its lower tiles (read by the symmetric GEMV and by S'); the
-  // coarse matrix is mirrored to full storage for its GEMV
-  std::vector<MirrorJob> mc;
-  std::vector<int> spsub, spij, spdst;
+  // S stays in its lower tiles, read by the symmetric GEMV and by S'
   int bad_s = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad_s)
-  for (int k = 0; k < (int)S.ord_reps.size(); k++) {
-    const SubTiles &T = D.sub[S.ord_reps[k]];
+  for (int k = 0; k < ncls; k++) {
+    const SubTiles &T = D.sub[reps[k]];
     for (int J = 0; J < T.nti; J++)
       for (int I = J; I < T.nti; I++)
         if (T.pat.find(T.nto + I, T.nto + J) != T.slow + lowidx(T.nti, I, J)) bad_s = 1;
   }
   for (int is = 0; is < nsub; is++) {
     const SubTiles &T = D.sub[is];
-    const SubTiles &R = D.sub[S.ord_reps[S.ord_cls[is]]];
+    const SubTiles &R = D.sub[reps[cls[is]]];
     if (T.slow - T.base != R.slow - R.base || T.slow < 0 ||
         (int64_t)T.slow + (int64_t)T.nti * (T.nti + 1) / 2 > D.ntiles)
       bad_s = 1;
   }
   if (bad_s) fatal("BDDCML_DEVICE_PLAN", "interface tiles of S are not packed by columns");
-  for (int is = 0; is < nsub; is++) {
-    const SubTiles &T = D.sub[is];
-    const int n = T.nti;
-    for (int J = 0; J < n; J++)
-      for (int I = J; I < n; I++) {
-        spsub.push_back(is);
-        spij.push_back((I << 16) | J);
-        spdst.push_back(T.P(I, J));
-      }
-  }
-  {
-    const int n = D.ntc;
-    for (int J = 0; J < n; J++)
-      for (int I = J; I < n; I++) {
-        int src = D.cbase + lowidx(n, I, J);
-        mc.push_back({src, D.cfull + I * n + J, 0});
-        if (I > J) mc.push_back({src, D.cfull + J * n + I, 1});
-      }
-  }
-  for (const MirrorJob &j : mc)
-    if (j.src < 0 || j.src >= D.ntiles || j.dst < 0 || j.dst >= D.ntiles)
-      fatal("BDDCML_DEVICE_PLAN", "invalid mirror job");
-  for (int d : spdst)  // tiles of S' live in the factor pool
-    if (d < 0 || d >= (D.mixed ? D.ntiles32 : D.ntiles)) fatal("BDDCML_DEVICE_PLAN", "invalid augmented tile");
-  D.plan.add(D.mir_c, mc);
-  D.plan.add(D.sp_sub, spsub);
-  D.plan.add(D.sp_ij, spij);
-  D.plan.add(D.sp_dst, spdst);
-  D.n_sp = (int)spsub.size();
-
-  ph("mirror and update jobs");
+  ph("packed S check");
   const size_t plan_bytes = D.plan.host.size();
   D.plan.commit(st);
   if (S.verbose >= 3) {
@@ -924,30 +769,11 @@ void device_setup_begin(Solver &S) {
   D.err.zero(st);
   ph("pool allocation");
   if (S.verbose >= 2)
-    std::printf("   device plan: %lld FP64 tiles (%.1f GB), %lld FP32 tiles (%.1f GB), %zu padding slots\n",
-                (long long)D.ntiles, D.ntiles * 32768.0 / 1e9, (long long)D.ntiles32,
-                D.ntiles32 * 16384.0 / 1e9, (size_t)(D.sc_pads.n + D.sc_idx.n));
+    std::printf("   device plan: %lld subdomain tiles (%.1f GB), %zu padding slots\n",
+                (long long)D.ntiles, D.ntiles * 32768.0 / 1e9, (size_t)D.sc_pads.n);
 
-  // algorithmic work: 2*64^3 flops per tile product, 64^3/3 per tile Cholesky
-  // plus 64^3/3 for its inverse, 2*64^3 per post multiply
-  {
-    const double g = 2.0 * 64 * 64 * 64, c = 64.0 * 64 * 64 / 3.0;
-    // pair lists are shared between subdomains: count the pairs of each task
-    auto products = [](const TaskList &l) {
-      double n = 0;
-      for (const TileTask &t : l.tasks) n += t.p1 - t.p0;
-      return n;
-    };
-    auto work = [&](const std::vector<TaskList> &v, double per_task) {
-      double f = 0;
-      for (auto &l : v) f += g * products(l) + per_task * l.tasks.size();
-      return f;
-    };
-    S.t.schur_flops = g * products(a_schur);
-    S.t.interior_flops = work(a_diag, 2 * c) + work(a_off, g) + S.t.schur_flops;
-    S.t.interface_flops =
-        work(p_diag, 2 * c) + work(p_off, g) + work(z_fwd, g) + work(z_bwd, g) + g * products(phi);
-  }
+  S.t.schur_flops = kTileGemm * tile_products(a_schur);
+  S.t.interior_flops = tile_work(a_diag, 2 * kTileChol) + tile_work(a_off, kTileGemm) + S.t.schur_flops;
   // ---- numerics, first half: Dirichlet elimination into the tiles,
   // interior factorization, Schur complements
   double *pool = D.pool.p;
@@ -958,7 +784,6 @@ void device_setup_begin(Solver &S) {
   launch_zero_ranges(st, pool, D.zero_rng.p, (int)D.zero_rng.n);
   launch_scatter_sub(st, pool, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p,
                      D.sc_pads.p, D.err.p);
-  launch_scatter_ones(st, pool, D.sc_idx.p, (int64_t)D.sc_idx.n);
   for (size_t J = 0; J < D.a_diag.size(); J++) {
     launch_tile_potrf(st, pool, D.a_diag[J].tasks.p, D.a_diag[J].pairs.p, D.a_diag[J].ntask, D.err.p);
     launch_tile_gemm(st, pool, D.a_off[J].tasks.p, D.a_off[J].pairs.p, D.a_off[J].ntask, GEMM_NT,
@@ -982,6 +807,223 @@ void device_setup_finish(Solver &S) {
     if (S.verbose >= 2) std::printf("   device plan: %-24s %8.4f s\n", name, now - tph);
     tph = now;
   };
+  // ---- second half of the plan, now that the constraints are known: the
+  // factor space (its own pool, FP64 or FP32 in the mixed-precision mode),
+  // the coarse tiles (FP64 pool), the task lists of the S' factorization, Z
+  // and Phi, the coarse factorization, the S' assembly jobs and the coarse
+  // mirror
+  {
+    int64_t nf = 0;
+    for (int is = 0; is < nsub; is++) assign_factor(D.sub[is], S.subs[is].nconstr, nf);
+    if (nf >= INT32_MAX) fatal("BDDCML_DEVICE_PLAN", "too many factor tiles");
+    D.ntilesf = nf;
+    D.NC = std::max(TILE, (S.ndofc + TILE - 1) / TILE * TILE);
+    D.ntc = D.NC / TILE;
+    const int n = D.ntc;
+    int next = 0;
+    D.cbase = next;
+    next += n * (n + 1) / 2;
+    D.clinv = next;
+    next += n;
+    D.cwbase = next;
+    next += n * (n - 1) / 2;
+    D.cfull = next;
+    next += n * n;
+    D.ntilesc = next;
+    if (D.mixed)
+      D.pool32.alloc((size_t)std::max<int64_t>(nf, 1) * TILE_ELEMS);
+    else
+      D.fpool.alloc((size_t)std::max<int64_t>(nf, 1) * TILE_ELEMS);
+    D.cpool.alloc((size_t)D.ntilesc * TILE_ELEMS);
+  }
+  {
+    // the factor-space lists depend only on (nti, ncT): one class per pair
+    std::map<std::pair<int, int>, int> key;
+    std::vector<int> fcls(nsub);
+    std::vector<std::pair<int, int>> fkeys;
+    for (int is = 0; is < nsub; is++) {
+      auto k = std::make_pair(D.sub[is].nti, D.sub[is].ncT);
+      auto it = key.emplace(k, (int)fkeys.size()).first;
+      if (it->second == (int)fkeys.size()) fkeys.push_back(k);
+      fcls[is] = it->second;
+    }
+    struct FactorTasks {
+      std::vector<TaskList> p_diag, p_off, z_fwd, z_bwd;
+      TaskList phi;
+    };
+    std::vector<FactorTasks> fts(fkeys.size());
+    int bad_ids = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad_ids)
+    for (int k = 0; k < (int)fkeys.size(); k++) {
+      const int n = fkeys[k].first, ncT = fkeys[k].second;
+      const FactorLayout L(n, ncT);
+      FactorTasks &Q = fts[k];
+      auto Pt = [&](int I, int J) { return L.p + lowidx(n, I, J); };
+      auto Yt = [&](int I, int cT) { return L.y + I * ncT + cT; };
+      dense_chol_tasks(n, Pt, [&](int J) { return L.linv + J; }, Q.p_diag, Q.p_off);
+      // Z = S'^{-1} C^T by tile sweeps over the factor: forward steps
+      // Y_I = Linv_I (Y_I - sum_{k<I} L_Ik Y_k), backward steps
+      // Y_I = Linv_I^T (Y_I - sum_{k>I} L_kI^T Y_k), one task per tile column
+      Q.z_fwd.resize(ncT ? n : 0);
+      Q.z_bwd.resize(ncT ? n : 0);
+      for (int cT = 0; cT < ncT; cT++)
+        for (int I = 0; I < n; I++) {
+          TaskList &F = Q.z_fwd[I];
+          TileTask t{Yt(I, cT), L.linv + I, (int)F.pairs.size(), 0};
+          for (int q = 0; q < I; q++) F.pairs.push_back({Pt(I, q), Yt(q, cT)});
+          t.p1 = (int)F.pairs.size();
+          F.tasks.push_back(t);
+          TaskList &B = Q.z_bwd[n - 1 - I];
+          TileTask u{Yt(I, cT), L.linv + I, (int)B.pairs.size(), 0};
+          for (int q = I + 1; q < n; q++) B.pairs.push_back({Pt(q, I), Yt(q, cT)});
+          u.p1 = (int)B.pairs.size();
+          B.tasks.push_back(u);
+        }
+      // Phi = Z G^{-1} on tiles: task (I, CT) sums Z(I, KT) Ginv(KT, CT)
+      for (int I = 0; I < n; I++)
+        for (int CT = 0; CT < ncT; CT++) {
+          TileTask t{L.phi + I * ncT + CT, -1, (int)Q.phi.pairs.size(), 0};
+          for (int KT = 0; KT < ncT; KT++) Q.phi.pairs.push_back({Yt(I, KT), L.gt + KT * ncT + CT});
+          t.p1 = (int)Q.phi.pairs.size();
+          Q.phi.tasks.push_back(t);
+        }
+      for (auto *v : {&Q.p_diag, &Q.p_off, &Q.z_fwd, &Q.z_bwd})
+        for (TaskList &T : *v)
+          if (!relative_ids(T, 0, L.span)) bad_ids |= 1;
+      if (!relative_ids(Q.phi, 0, L.span)) bad_ids |= 1;
+    }
+    if (bad_ids) fatal("BDDCML_DEVICE_PLAN", "tile task outside its subdomain's factor tiles");
+    for (int is = 0; is < nsub; is++)
+      if (D.sub[is].fbase < 0 || (int64_t)D.sub[is].fbase + D.sub[is].fspan > D.ntilesf)
+        fatal("BDDCML_DEVICE_PLAN", "factor tile range");
+    auto fbase_of = [&](int is) { return D.sub[is].fbase; };
+    std::vector<TaskList> p_diag = merge_steps(fts, &FactorTasks::p_diag, fcls, fbase_of),
+                          p_off = merge_steps(fts, &FactorTasks::p_off, fcls, fbase_of),
+                          z_fwd = merge_steps(fts, &FactorTasks::z_fwd, fcls, fbase_of),
+                          z_bwd = merge_steps(fts, &FactorTasks::z_bwd, fcls, fbase_of);
+    TaskList phi = materialize(
+        fts, [](const FactorTasks &Q) -> const TaskList * { return &Q.phi; }, fcls, fbase_of);
+    S.t.interface_flops = tile_work(p_diag, 2 * kTileChol) + tile_work(p_off, kTileGemm) +
+                          tile_work(z_fwd, kTileGemm) + tile_work(z_bwd, kTileGemm) +
+                          kTileGemm * tile_products(phi);
+    D.plan2.verbose = S.verbose;
+    upload_phases(D.p_diag, p_diag, D.plan2);
+    upload_phases(D.p_off, p_off, D.plan2);
+    upload_phases(D.z_fwd, z_fwd, D.plan2);
+    upload_phases(D.z_bwd, z_bwd, D.plan2);
+    D.phi.upload(phi, D.plan2);
+  }
+  {
+    // coarse problem (one dense matrix, tiles in the coarse pool):
+    // right-looking tile Cholesky and the inverse factor W = L^{-1} by
+    // right-looking forward substitution, so that every task is a single tile
+    // product and each step is a few short launches; then Ac^{-1} = W^T W
+    std::vector<SeqLaunch> cseq;
+    TaskList cinv;
+    const int n = D.ntc, cb = D.cbase, cl = D.clinv, cw = D.cwbase;
+    auto Pc = [=](int I, int J) { return cb + lowidx(n, I, J); };
+    auto Wc = [=](int I, int J) { return I == J ? cl + I : cw + lowidx(n - 1, I - 1, J); };
+    auto one = [](TaskList &l, int c, int a, int b) {
+      const int p0 = (int)l.pairs.size();
+      l.pairs.push_back({a, b});
+      l.tasks.push_back({c, -1, p0, p0 + 1});
+    };
+    for (int k = 0; k < n; k++) {
+      SeqLaunch f;  // L_kk = chol(A_kk), Linv_k
+      f.potrf = true;
+      f.l.tasks.push_back({Pc(k, k), cl + k, 0, 0});
+      cseq.push_back(std::move(f));
+      if (k + 1 < n) {  // L_ik = A_ik Linv_k^T
+        SeqLaunch pn;
+        pn.post = POST_RIGHT_XT, pn.beta = 0.0;
+        for (int i = k + 1; i < n; i++) pn.l.tasks.push_back({Pc(i, k), cl + k, 0, 0});
+        cseq.push_back(std::move(pn));
+      }
+      if (k > 0) {  // W_kj = Linv_k B_kj
+        SeqLaunch wr;
+        wr.kind = GEMM_NN, wr.post = POST_LEFT_X, wr.beta = 0.0;
+        for (int j = 0; j < k; j++) wr.l.tasks.push_back({Wc(k, j), cl + k, 0, 0});
+        cseq.push_back(std::move(wr));
+      }
+      if (k + 1 < n) {
+        SeqLaunch tr;  // A_IK -= L_Ik L_Kk^T
+        for (int K = k + 1; K < n; K++)
+          for (int I = K; I < n; I++) one(tr.l, Pc(I, K), Pc(I, k), Pc(K, k));
+        cseq.push_back(std::move(tr));
+        SeqLaunch bu;  // B_ij -= L_ik W_kj
+        bu.kind = GEMM_NN;
+        for (int i = k + 1; i < n; i++)
+          for (int j = 0; j <= k; j++) one(bu.l, Wc(i, j), Pc(i, k), Wc(k, j));
+        cseq.push_back(std::move(bu));
+      }
+    }
+    for (int j = 0; j < n; j++)
+      for (int i = j; i < n; i++) {
+        TileTask t{Pc(i, j), -1, (int)cinv.pairs.size(), 0};
+        for (int k = i; k < n; k++) cinv.pairs.push_back({Wc(k, i), Wc(k, j)});
+        t.p1 = (int)cinv.pairs.size();
+        cinv.tasks.push_back(t);
+      }
+    for (auto &q : cseq) check_tasks(q.l, D.ntilesc, q.potrf || q.post != POST_NONE, "coarse factorization");
+    check_tasks(cinv, D.ntilesc, false, "coarse W^T W");
+    D.cseq.resize(cseq.size());
+    for (size_t k = 0; k < cseq.size(); k++) {
+      SeqPhase &q = D.cseq[k];
+      q.ph.upload(cseq[k].l, D.plan2);
+      q.kind = cseq[k].kind, q.post = cseq[k].post, q.alpha = cseq[k].alpha, q.beta = cseq[k].beta;
+      q.potrf = cseq[k].potrf;
+    }
+    D.cinv.upload(cinv, D.plan2);
+    // full-storage mirror of Ac^{-1} for its GEMV; the factor, inverse
+    // diagonal and W tiles start from zero; padding diagonals 1.0
+    std::vector<MirrorJob> mc;
+    for (int J = 0; J < n; J++)
+      for (int I = J; I < n; I++) {
+        int src = D.cbase + lowidx(n, I, J);
+        mc.push_back({src, D.cfull + I * n + J, 0});
+        if (I > J) mc.push_back({src, D.cfull + J * n + I, 1});
+      }
+    for (const MirrorJob &j : mc)
+      if (j.src < 0 || j.src >= D.ntilesc || j.dst < 0 || j.dst >= D.ntilesc)
+        fatal("BDDCML_DEVICE_PLAN", "invalid mirror job");
+    std::vector<int2> cz{make_int2(D.cbase, n * (n + 1) / 2 + n + n * (n - 1) / 2)};
+    if ((int64_t)cz[0].x + cz[0].y > D.ntilesc) fatal("BDDCML_DEVICE_PLAN", "invalid tile range to clear");
+    std::vector<int64_t> cpads;
+    for (int p = S.ndofc; p < D.NC; p++)
+      cpads.push_back((int64_t)(D.cbase + lowidx(n, p / TILE, p / TILE)) * TILE_ELEMS +
+                      (p % TILE) * TILE + (p % TILE));
+    for (int64_t k : cpads)
+      if (k < 0 || k >= D.ntilesc * (int64_t)TILE_ELEMS) fatal("BDDCML_DEVICE_PLAN", "scatter index out of range");
+    D.plan2.add(D.mir_c, mc);
+    D.plan2.add(D.c_zero, cz);
+    D.plan2.add(D.c_pads, cpads);
+  }
+  {
+    // S' = S + C^T W C: one job per lower tile, into the factor space
+    std::vector<int> spsub, spij, spdst;
+    for (int is = 0; is < nsub; is++) {
+      const SubTiles &T = D.sub[is];
+      const int n = T.nti;
+      for (int J = 0; J < n; J++)
+        for (int I = J; I < n; I++) {
+          spsub.push_back(is);
+          spij.push_back((I << 16) | J);
+          spdst.push_back(T.P(I, J));
+        }
+    }
+    for (int d : spdst)
+      if (d < 0 || d >= D.ntilesf) fatal("BDDCML_DEVICE_PLAN", "invalid augmented tile");
+    D.plan2.add(D.sp_sub, spsub);
+    D.plan2.add(D.sp_ij, spij);
+    D.plan2.add(D.sp_dst, spdst);
+    D.n_sp = (int)spsub.size();
+  }
+  D.plan2.commit(st);
+  ph("factor and coarse lists");
+  if (S.verbose >= 2)
+    std::printf("   device plan: %lld factor tiles (%.1f GB), %lld coarse tiles\n",
+                (long long)D.ntilesf, D.ntilesf * (D.mixed ? 16384.0 : 32768.0) / 1e9,
+                (long long)D.ntilesc);
   // ---- interface descriptors and maps: offsets first, then every
   // subdomain fills its own ranges in parallel
   D.hdesc.resize(nsub);
@@ -1117,7 +1159,7 @@ void device_setup_finish(Solver &S) {
   }
   for (int I = 0; I < D.ntc; I++) cr.push_back({D.cfull + I * D.ntc, D.ntc, 0, I});
   for (const GemvRow &g : cr)
-    if (g.tile0 < 0 || (int64_t)g.tile0 + g.nt > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid GEMV row");
+    if (g.tile0 < 0 || (int64_t)g.tile0 + g.nt > D.ntilesc) fatal("BDDCML_DEVICE_PLAN", "invalid GEMV row");
   D.scols.upload(sc, st);
   D.srows.upload(sr, st);
   D.crows.upload(cr, st);
@@ -1244,6 +1286,8 @@ void device_setup_finish(Solver &S) {
       for (int a = 0; a < s.nconstr; a++)
         for (int b = 0; b < s.nconstr; b++) {
           int ga = s.indrowc[a], gb = s.indrowc[b];
+          if (ga < 0 || ga >= S.ndofc || gb < 0 || gb >= S.ndofc)
+            fatal("BDDCML_DEVICE_PLAN", "coarse dof out of range");
           int I = ga / TILE, J = gb / TILE;
           if (I < J) continue;
           int64_t dst = (int64_t)(D.cbase + lowidx(n, I, J)) * TILE_ELEMS + (gb % TILE) * TILE + ga % TILE;
@@ -1325,26 +1369,31 @@ void device_setup_finish(Solver &S) {
                      1.0);
     launch_y_to_z(st, fp, D.desc.p, nsub, D.Phi.p, D.maxNI, 1);
   };
+  // coarse tiles cleared (they accumulate) and padding diagonals set early,
+  // off the critical path of the coarse phase
+  launch_zero_ranges(st, D.cpool.p, D.c_zero.p, (int)D.c_zero.n);
+  launch_scatter_ones(st, D.cpool.p, D.c_pads.p, (int64_t)D.c_pads.n);
   if (D.mixed) factor_phase(D.pool32.p);
-  else factor_phase(pool);
+  else factor_phase(D.fpool.p);
   launch_g(st, D.desc.p, nsub, D.cptr.p, D.cdof.p, D.Z.p, D.G.p);
   launch_ginv(st, D.desc.p, nsub, D.G.p, nullptr, D.err.p);
   if (D.mixed) phi_phase(D.pool32.p);
-  else phi_phase(pool);
+  else phi_phase(D.fpool.p);
   launch_acs(st, D.desc.p, nsub, D.cptr.p, D.G.p, D.Acs.p);
   HIPCHECK(hipEventRecord(ev[3], st));
   // coarse problem
-  launch_coarse_assemble(st, pool, D.ca_dst.p, D.ca_ptr.p, D.ca_src.p, D.nca, D.Acs.p);
+  double *cpool = D.cpool.p;
+  launch_coarse_assemble(st, cpool, D.ca_dst.p, D.ca_ptr.p, D.ca_src.p, D.nca, D.Acs.p);
   for (const SeqPhase &q : D.cseq) {
     if (q.potrf)
-      launch_tile_potrf(st, pool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, D.err.p);
+      launch_tile_potrf(st, cpool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, D.err.p);
     else
-      launch_tile_gemm(st, pool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, q.kind, q.post, q.alpha,
+      launch_tile_gemm(st, cpool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, q.kind, q.post, q.alpha,
                        q.beta);
   }
-  launch_tile_gemm(st, pool, D.cinv.tasks.p, D.cinv.pairs.p, D.cinv.ntask, GEMM_TN, POST_NONE, 0.0,
+  launch_tile_gemm(st, cpool, D.cinv.tasks.p, D.cinv.pairs.p, D.cinv.ntask, GEMM_TN, POST_NONE, 0.0,
                    1.0);
-  launch_tile_mirror(st, pool, D.mir_c.p, (int)D.mir_c.n);
+  launch_tile_mirror(st, cpool, D.mir_c.p, (int)D.mir_c.n);
   HIPCHECK(hipEventRecord(ev[4], st));
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(st));
@@ -1584,7 +1633,7 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed = false) {
   // coarse correction uc = Ac^{-1} sum_i R_ci^T Phi_i^T W_i r_i
   launch_phit(st, D.desc.p, S.nsub, D.Phi.p, D.map.p, D.w.p, r, D.rcs.p, D.maxnc);
   launch_coarse_gather(st, S.ndofc, D.cgptr.p, D.cgsrc.p, D.rcs.p, D.rc.p);
-  launch_gemv_plain(st, D.pool.p, D.crows.p, D.ncrows, D.rc.p, D.uc.p);
+  launch_gemv_plain(st, D.cpool.p, D.crows.p, D.ncrows, D.rc.p, D.uc.p);
   // zl_i = K_i W_i r_i + Phi_i uc(indrowc), then z = sum_i R_i^T W_i zl_i
   if (timed) {
     HIPCHECK(hipEventCreate(&e0));
@@ -1595,7 +1644,7 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed = false) {
     launch_kapply(st, (const float *)D.pool32.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p,
                   D.Z.p, D.cmap.p, D.uc.p, D.zl.p, D.maxNI, D.maxnc);
   else
-    launch_kapply(st, (const double *)D.pool.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p,
+    launch_kapply(st, (const double *)D.fpool.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p,
                   D.Z.p, D.cmap.p, D.uc.p, D.zl.p, D.maxNI, D.maxnc);
   if (timed) {
     HIPCHECK(hipEventRecord(e1, st));

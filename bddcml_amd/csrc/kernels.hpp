@@ -23,6 +23,9 @@ template <typename T>
 void launch_tile_gemm(hipStream_t st, T *pool, const TileTask *tasks, const TilePair *pairs,
                       int ntasks, int kind, int post, double alpha, double beta);
 // C = C - sum A B^T as its own kernel symbol (Schur-complement block update)
+// 1: tile-task launches hand each XCD one contiguous range of their task
+// list (default); 0: plain blockIdx order
+void set_task_order(int chunked);
 void launch_schur_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
                        int ntasks);
 // C = C - sum A B^T ; C = chol(C) (lower, upper zeroed) ; X = inv(C)

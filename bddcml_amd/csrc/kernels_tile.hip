@@ -245,20 +245,36 @@ __device__ __forceinline__ void tile_gemm_task(T *__restrict__ pool, const TileT
   lds_to_tile(As, C);
 }
 
+// Logical task of this workgroup.  Workgroups reach the 8 XCDs round-robin
+// (blockIdx % 8); with `chunked` set each XCD receives one contiguous range
+// of the task list instead, so that neighbouring tasks, which share a row of
+// operand tiles, run under the same L2.  A bijection of [0, gridDim.x).
+__device__ __forceinline__ int task_index(int chunked) {
+  const int b = blockIdx.x;
+  if (!chunked) return b;
+  const int n = gridDim.x, q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
+
+static int g_task_order = 1;
+void set_task_order(int chunked) { g_task_order = chunked != 0; }
+
 template <typename T>
 __global__ __launch_bounds__(NTHR) void k_tile_gemm(T *__restrict__ pool,
                                                     const TileTask *__restrict__ tasks,
                                                     const TilePair *__restrict__ pairs, int kind,
-                                                    int post, double alpha, double beta) {
-  tile_gemm_task<T>(pool, tasks[blockIdx.x], pairs, kind, post, alpha, beta);
+                                                    int post, double alpha, double beta,
+                                                    int chunked) {
+  tile_gemm_task<T>(pool, tasks[task_index(chunked)], pairs, kind, post, alpha, beta);
 }
 
 // the Schur-complement block update S = A22 - sum W^T W in a launch of its
 // own symbol, so that profiles separate it from the factorization steps
 __global__ __launch_bounds__(NTHR) void k_schur_gemm(double *__restrict__ pool,
                                                      const TileTask *__restrict__ tasks,
-                                                     const TilePair *__restrict__ pairs) {
-  tile_gemm_task<double>(pool, tasks[blockIdx.x], pairs, GEMM_NT, POST_NONE, 1.0, -1.0);
+                                                     const TilePair *__restrict__ pairs,
+                                                     int chunked) {
+  tile_gemm_task<double>(pool, tasks[task_index(chunked)], pairs, GEMM_NT, POST_NONE, 1.0, -1.0);
 }
 
 // acc += sum_{k < nk} A(r, k) B'(c, k) for one 16x16 block in the lane layout
@@ -331,13 +347,13 @@ template <typename T>
 __global__ __launch_bounds__(NTHR) void k_tile_potrf(T *__restrict__ pool,
                                                      const TileTask *__restrict__ tasks,
                                                      const TilePair *__restrict__ pairs,
-                                                     int *__restrict__ err) {
+                                                     int *__restrict__ err, int chunked) {
   typedef typename Mfma<T>::v4 v4;
   __shared__ T As[64 * LDL];
   __shared__ T Bs[64 * LDL];
   __shared__ T Ts[4][16 * 17];
   __shared__ int bad;
-  const TileTask t = tasks[blockIdx.x];
+  const TileTask t = tasks[task_index(chunked)];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   v4 acc[2][2];
   zero_acc<T>(acc);
@@ -492,7 +508,7 @@ void launch_tile_gemm(hipStream_t st, T *pool, const TileTask *tasks, const Tile
                       int ntasks, int kind, int post, double alpha, double beta) {
   if (ntasks <= 0) return;
   hipLaunchKernelGGL(k_tile_gemm<T>, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs, kind,
-                     post, alpha, beta);
+                     post, alpha, beta, g_task_order);
 }
 template void launch_tile_gemm<double>(hipStream_t, double *, const TileTask *, const TilePair *,
                                        int, int, int, double, double);
@@ -501,13 +517,15 @@ template void launch_tile_gemm<float>(hipStream_t, float *, const TileTask *, co
 void launch_schur_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
                        int ntasks) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_schur_gemm, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs);
+  hipLaunchKernelGGL(k_schur_gemm, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs,
+                     g_task_order);
 }
 template <typename T>
 void launch_tile_potrf(hipStream_t st, T *pool, const TileTask *tasks, const TilePair *pairs,
                        int ntasks, int *err) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_tile_potrf<T>, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs, err);
+  hipLaunchKernelGGL(k_tile_potrf<T>, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs, err,
+                     g_task_order);
 }
 template void launch_tile_potrf<double>(hipStream_t, double *, const TileTask *, const TilePair *,
                                         int, int *);

@@ -108,35 +108,17 @@ static void nodal_components(Sub &s) {
 }
 
 // -------------------------------------------------------------- topology
-static void interface_of(Solver &S, int is) {
+// interface nodes (shared with another subdomain) and the interior /
+// interface split of the local dofs
+static void split_of(Solver &S, int is) {
   Sub &s = S.subs[is];
-  // (neighbour, global node, local node) for every shared node
-  struct Sh {
-    int t, g, l;
-  };
-  std::vector<Sh> sh;
-  std::vector<char> shared(s.nnods, 0);
-  for (int l = 0; l < s.nnods; l++) {
-    int g = s.isngn[l];
-    int p0 = S.n2s_ptr[g], p1 = S.n2s_ptr[g + 1];
-    if (p1 - p0 < 2) continue;
-    shared[l] = 1;
-    for (int q = p0; q < p1; q++) {
-      int t = S.n2s_idx[q];
-      if (t != is) sh.push_back({t, g, l});
-    }
-  }
-  std::sort(sh.begin(), sh.end(), [](const Sh &a, const Sh &b) {
-    return a.t != b.t ? a.t < b.t : a.g < b.g;
-  });
   s.iin.clear();
   s.ifc_dof.clear();
   s.int_dof.clear();
   s.kdofi.clear();
-  std::vector<int> l2i(s.nnods, -1);
   for (int l = 0; l < s.nnods; l++) {
-    if (shared[l]) {
-      l2i[l] = (int)s.iin.size();
+    const int g = s.isngn[l];
+    if (S.n2s_ptr[g + 1] - S.n2s_ptr[g] >= 2) {
       s.iin.push_back(l);
       s.kdofi.push_back((int)s.ifc_dof.size());
       for (int d = s.kdof[l]; d < s.kdof[l + 1]; d++) s.ifc_dof.push_back(d);
@@ -144,6 +126,26 @@ static void interface_of(Solver &S, int is) {
       for (int d = s.kdof[l]; d < s.kdof[l + 1]; d++) s.int_dof.push_back(d);
   }
   s.kdofi.push_back((int)s.ifc_dof.size());
+}
+
+// neighbours of a subdomain and, per neighbour, the shared interface nodes
+// (by global node) with their component inside the neighbour
+static void neighbours_of(Solver &S, int is) {
+  Sub &s = S.subs[is];
+  struct Sh {
+    int t, g, l;  // neighbour, global node, interface node
+  };
+  std::vector<Sh> sh;
+  for (int i = 0; i < (int)s.iin.size(); i++) {
+    const int g = s.isngn[s.iin[i]];
+    for (int q = S.n2s_ptr[g]; q < S.n2s_ptr[g + 1]; q++) {
+      int t = S.n2s_idx[q];
+      if (t != is) sh.push_back({t, g, i});
+    }
+  }
+  std::sort(sh.begin(), sh.end(), [](const Sh &a, const Sh &b) {
+    return a.t != b.t ? a.t < b.t : a.g < b.g;
+  });
   s.adj.clear();
   s.shptr.assign(1, 0);
   s.shnode.clear();
@@ -153,7 +155,7 @@ static void interface_of(Solver &S, int is) {
       if (!s.adj.empty()) s.shptr.push_back((int)s.shnode.size());
       s.adj.push_back(sh[k].t);
     }
-    s.shnode.push_back(l2i[sh[k].l]);
+    s.shnode.push_back(sh[k].l);
     // component of the node inside the neighbour
     const Sub &u = S.subs[sh[k].t];
     int cu = 1;
@@ -429,12 +431,21 @@ void build_topology(Solver &S) {
       if (cnt[g + 1] - cnt[g] > 1) std::sort(idx + cnt[g], idx + cnt[g + 1]);
   }
   double tp = wall_time();
+  if (S.verbose >= 2) std::printf("   topology: %-20s %8.4f s\n", "node to subdomains", tp - t_n2s);
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++) split_of(S, is);
+  if (S.verbose >= 2) std::printf("   topology: %-20s %8.4f s\n", "interface split", wall_time() - tp);
+}
+
+// Components, neighbour lists and globs: needed by the constraints and the
+// interface phase only, so they run while the device factors the interiors.
+void build_globs(Solver &S) {
+  double tp = wall_time();
   auto phase = [&](const char *name) {
     double now = wall_time();
-    if (S.verbose >= 2) std::printf("   topology: %-20s %8.4f s\n", name, now - tp);
+    if (S.verbose >= 2) std::printf("   globs: %-22s %8.4f s\n", name, now - tp);
     tp = now;
   };
-  if (S.verbose >= 2) std::printf("   topology: %-20s %8.4f s\n", "node to subdomains", tp - t_n2s);
   // nodal components depend only on the element connectivity: once per class
   std::vector<int> ccls, creps;
   structure_classes(
@@ -465,8 +476,8 @@ void build_topology(Solver &S) {
   }
   phase("components");
 #pragma omp parallel for schedule(dynamic)
-  for (int is = 0; is < S.nsub; is++) interface_of(S, is);
-  phase("interface");
+  for (int is = 0; is < S.nsub; is++) neighbours_of(S, is);
+  phase("neighbours");
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < S.nsub; is++) globs_first_pass(S, is);
   phase("globs pass 1");

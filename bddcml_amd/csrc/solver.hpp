@@ -70,7 +70,8 @@ Solver &solver();
 void reset_solver();
 
 // host-side setup steps (plan.cpp)
-void build_topology(Solver &S);
+void build_topology(Solver &S);  // node to subdomains, interior / interface split
+void build_globs(Solver &S);     // components, neighbours, globs
 void apply_boundary_conditions(Solver &S);
 void refresh_boundary_conditions(Solver &S);  // new fixv, same fixed dofs
 void build_reactions(Solver &S);              // from the solution of the last solve
@@ -86,6 +87,9 @@ void device_release_input(Solver &S);
 // setup in two halves: begin builds the tile plan and launches the interior
 // factorization; finish (after weights, Dirichlet data, unique interface)
 // builds the interface/coarse maps, launches the rest and waits
+// order of tile-task launches (kernels: 1 = one contiguous range of tasks
+// per XCD, the default; 0 = blockIdx order)
+void set_task_order(int chunked);
 void device_setup_begin(Solver &S);
 void device_setup_finish(Solver &S);
 void device_reduced_rhs(Solver &S);

@@ -313,6 +313,7 @@ void bddcml_download_local_solution_c(int *isub, double *sols, int *lsols) try {
   Solver &S = solver();
   Sub &s = local_sub(S, *isub, "LEVELS_DD_DOWNLOAD_SOLUTION");
   if (*lsols != s.ndofs) fatal("LEVELS_DD_DOWNLOAD_SOLUTION", "solution size mismatch");
+  device_fetch_solution(S);
   std::copy(s.sol.begin(), s.sol.end(), sols);
 }
 catch (const Error &e) {
@@ -322,6 +323,7 @@ catch (const Error &e) {
 void bddcml_download_global_solution_c(double *sol, int *lsol) try {
   Solver &S = solver();
   if (*lsol != S.ndof_g) fatal("LEVELS_GET_GLOBAL_SOLUTION", "solution size mismatch");
+  device_fetch_solution(S);
   for (auto &s : S.subs)
     for (int d = 0; d < s.ndofs; d++) sol[s.isvgvn[d]] = s.sol[d];
 }
@@ -333,6 +335,7 @@ catch (const Error &e) {
 // (the reference computes them in levels_postprocess_solution)
 static void ensure_reactions(Solver &S, const char *R) {
   if (!S.solved) fatal(R, "Reactions are not computed: no solution yet");
+  device_fetch_solution(S);
   if (!S.reactions_ready) {
     build_reactions(S);
     S.reactions_ready = true;
@@ -376,6 +379,7 @@ void bddcml_change_subdomain_data_c(int *isub, int *ifix, int *lifix, double *fi
   const char *R = "LEVELS_CHANGE_SUBDOMAIN_DATA";
   Solver &S = solver();
   if (!S.is_setup) fatal(R, "Cannot change data at this moment, level is not ready.");
+  device_fetch_solution(S);
   Sub &s = local_sub(S, *isub, R);
   if (*lifix != s.ndofs || *lfixv != s.ndofs || *lrhs != s.ndofs || *lsol != s.ndofs)
     fatal(R, "array length mismatch");
@@ -487,6 +491,7 @@ catch (const Error &e) {
 int bddcml_amd_interface_solution(double *u) try {
   Solver &S = solver();
   if (!S.solved) return -1;
+  device_fetch_solution(S);
   std::copy(S.uig.begin(), S.uig.end(), u);
   return 0;
 }

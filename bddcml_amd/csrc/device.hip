@@ -94,27 +94,59 @@ template <class T> struct View {
 // Many small plan arrays gathered into one host buffer and moved to the
 // device with one allocation and one copy.  Views get their device
 // addresses at commit; they must not move between add and commit.
+// Pinned host staging for the plan uploads, kept for the life of the
+// process and grown to the largest plan seen (tens of MB), so that uploads
+// run at PCIe speed without pinning pages on every setup.  Used by one
+// arena at a time; every commit waits for its copy before returning.
+struct Staging {
+  char *p = nullptr;
+  size_t cap = 0, used = 0;
+  void grow(size_t n) {
+    if (n <= cap) return;
+    const size_t c = std::max(n, cap + cap / 2 + (4u << 20));
+    char *q = nullptr;
+    HIPCHECK(hipHostMalloc((void **)&q, c, hipHostMallocDefault));
+    if (used) std::memcpy(q, p, used);
+    if (p) (void)hipHostFree(p);
+    p = q;
+    cap = c;
+  }
+};
+static Staging &staging() {
+  static Staging *s = new Staging();  // never freed: outlives the HIP runtime's teardown order
+  return *s;
+}
+
+// Plan arrays of one setup half, packed into the pinned staging area and
+// sent to one device allocation with one copy; each View is fixed up to
+// its place in that allocation at commit.
 struct Arena {
-  std::vector<char> host;
   DBuf<char> dev;
   std::vector<std::pair<size_t, void **>> fix;
   int verbose = 0;
+  bool open = false;
+  size_t bytes() const { return open ? staging().used : 0; }
   template <class T> void add(View<T> &v, const std::vector<T> &src) {
     if (verbose >= 4) std::printf("   device plan: arena %10zu bytes\n", src.size() * sizeof(T));
-    const size_t off = (host.size() + 255) / 256 * 256;
-    host.resize(off + src.size() * sizeof(T));
-    if (!src.empty()) std::memcpy(host.data() + off, src.data(), src.size() * sizeof(T));
+    Staging &h = staging();
+    if (!open) h.used = 0, open = true;
+    const size_t off = (h.used + 255) / 256 * 256;
+    h.grow(off + src.size() * sizeof(T));
+    if (!src.empty()) std::memcpy(h.p + off, src.data(), src.size() * sizeof(T));
+    h.used = off + src.size() * sizeof(T);
     v.n = src.size();
     fix.push_back({off, reinterpret_cast<void **>(&v.p)});
   }
   void commit(hipStream_t st) {
-    dev.alloc(std::max<size_t>(host.size(), 1));
-    if (!host.empty())
-      HIPCHECK(hipMemcpyAsync(dev.p, host.data(), host.size(), hipMemcpyHostToDevice, st));
+    Staging &h = staging();
+    const size_t n = open ? h.used : 0;
+    dev.alloc(std::max<size_t>(n, 1));
+    if (n) HIPCHECK(hipMemcpyAsync(dev.p, h.p, n, hipMemcpyHostToDevice, st));
     for (auto &f : fix) *f.second = dev.p + f.first;
     fix.clear();
-    HIPCHECK(hipStreamSynchronize(st));  // the host buffer may go away
-    std::vector<char>().swap(host);
+    HIPCHECK(hipStreamSynchronize(st));  // the staging area is reused by the next arena
+    h.used = 0;
+    open = false;
   }
 };
 
@@ -271,6 +303,7 @@ void device_upload_entries(Solver &S, int is) {
 void device_release_input(Solver &S) { S.din.reset(); }
 
 void device_release(Solver &S) {
+  S.sol_on_device = false;
   if (S.dev) {
     Device &D = *S.dev;
     if (D.st) (void)hipStreamSynchronize(D.st);
@@ -494,6 +527,7 @@ static double tile_work(const std::vector<TaskList> &v, double per_task) {
 // builds their task lists and the interface and coarse maps (uploaded on a
 // second stream) and launches the rest.
 void device_setup_begin(Solver &S) {
+  device_fetch_solution(S);  // a new setup keeps the last solution as the initial guess
   device_release(S);
   S.dev.reset(new Device());
   Device &D = *S.dev;
@@ -511,7 +545,6 @@ void device_setup_begin(Solver &S) {
 
   // ---- symbolic patterns and tile ids
   D.plan.verbose = S.verbose;
-  D.plan.host.reserve(32u << 20);
   D.sub.resize(nsub);
   // symbolic factorization once per class of identical subdomains
   if ((int)S.ord_cls.size() != nsub) fatal("BDDCML_DEVICE_PLAN", "orderings missing");
@@ -758,7 +791,7 @@ void device_setup_begin(Solver &S) {
   }
   if (bad_s) fatal("BDDCML_DEVICE_PLAN", "interface tiles of S are not packed by columns");
   ph("packed S check");
-  const size_t plan_bytes = D.plan.host.size();
+  const size_t plan_bytes = D.plan.bytes();
   D.plan.commit(st);
   if (S.verbose >= 3) {
     std::printf("   device plan: %.1f MB of plan arrays\n", plan_bytes / 1e6);
@@ -1726,6 +1759,8 @@ static double tridiag_max_eig(const std::vector<double> &d, const std::vector<do
 }
 
 // ------------------------------------------------------------------- Krylov
+// The solve ends with the local solutions in device memory (the timed
+// result); the host copies are made on first use.
 static void postprocess(Solver &S) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
@@ -1734,20 +1769,35 @@ static void postprocess(Solver &S) {
   launch_ifc_scatter(st, S.nig, D.aptr.p, D.rhs_src.p, D.u.p, D.vec.p);
   launch_trsv_bwd(st, D.pool.p, D.trsv.p, S.nsub, D.tcptr.p, D.tck.p, D.tct.p, D.vec.p, D.maxnt);
   launch_from_device_order(st, D.odesc.p, S.nsub, D.sc_perm.p, D.vec.p, D.sol.p, D.maxndofs);
-  std::vector<double> hs(D.ndofs_total);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(st));
+  S.sol_on_device = true;
+}
+
+void device_fetch_solution(Solver &S) {
+  if (!S.sol_on_device || !S.dev) return;
+  Device &D = *S.dev;
+  hipStream_t st = D.st;
+  Staging &h = staging();
+  const size_t nb = (size_t)D.ndofs_total * sizeof(double);
+  h.used = 0;
+  h.grow(nb);
   S.uig.resize(S.nig);
   HIPCHECK(hipMemcpyAsync(S.uig.data(), D.u.p, S.nig * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHECK(hipMemcpyAsync(hs.data(), D.sol.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(h.p, D.sol.p, nb, hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
+  const double *hs = reinterpret_cast<const double *>(h.p);
 #pragma omp parallel for schedule(static)
   for (int is = 0; is < S.nsub; is++) {
     Sub &s = S.subs[is];
-    s.sol.assign(hs.begin() + D.hodesc[is].poff, hs.begin() + D.hodesc[is].poff + s.ndofs);
+    s.sol.assign(hs + D.hodesc[is].poff, hs + D.hodesc[is].poff + s.ndofs);
   }
+  S.sol_on_device = false;
 }
 
 static void initial_guess(Solver &S) {
   Device &D = *S.dev;
+  device_fetch_solution(S);
   std::vector<double> u0(S.nig, 0.0);
   std::vector<char> done(S.nig, 0);
   for (int is = 0; is < S.nsub; is++) {

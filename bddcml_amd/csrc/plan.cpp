@@ -377,6 +377,13 @@ struct Fnv {
     add((uint64_t)v.size());
     for (const T &x : v) add((uint64_t)x);
   }
+  // size and every 61st element: enough to separate classes cheaply, since
+  // candidates are confirmed by a full comparison
+  template <class T> void sample(const std::vector<T> &v) {
+    add((uint64_t)v.size());
+    for (size_t k = 0; k < v.size(); k += 61) add((uint64_t)v[k]);
+    if (!v.empty()) add((uint64_t)v.back());
+  }
 };
 
 // relative node coordinates equal up to a tolerance of the extent
@@ -454,7 +461,7 @@ void build_globs(Solver &S) {
         const Sub &s = S.subs[i];
         Fnv f;
         f.add((uint64_t)s.nnods), f.add((uint64_t)s.find_components), f.add((uint64_t)s.use_dual);
-        f.add((uint64_t)s.neighbouring), f.add(s.nnet), f.add(s.inet);
+        f.add((uint64_t)s.neighbouring), f.sample(s.nnet), f.sample(s.inet);
         return f.h;
       },
       [&](int a, int b) {
@@ -1012,13 +1019,19 @@ static void order_subdomain(Sub &s, int method, int leaf_dofs, int ndim) {
 // matrix pattern, the interior/interface split and the node coordinates up
 // to a translation: computed once per class of identical subdomains.
 void build_orderings(Solver &S) {
+  double tp = wall_time();
+  auto phase = [&](const char *name) {
+    double now = wall_time();
+    if (S.verbose >= 3) std::printf("   orderings: %-19s %8.4f s\n", name, now - tp);
+    tp = now;
+  };
   structure_classes(
       S.nsub,
       [&](int i) {
         const Sub &s = S.subs[i];
         Fnv f;
-        f.add((uint64_t)s.ndofs), f.add((uint64_t)s.nnods), f.add(s.kdof);
-        f.add(s.ai), f.add(s.aj), f.add(s.int_dof), f.add(s.ifc_dof);
+        f.add((uint64_t)s.ndofs), f.add((uint64_t)s.nnods), f.sample(s.kdof);
+        f.sample(s.ai), f.sample(s.aj), f.add(s.int_dof), f.add(s.ifc_dof);
         return f.h;
       },
       [&](int a, int b) {
@@ -1028,9 +1041,11 @@ void build_orderings(Solver &S) {
                same_shape(x, y, S.ndim);
       },
       S.ord_cls, S.ord_reps);
+  phase("classes");
 #pragma omp parallel for schedule(dynamic)
   for (int k = 0; k < (int)S.ord_reps.size(); k++)
     order_subdomain(S.subs[S.ord_reps[k]], S.ordering, S.nd_leaf, S.ndim);
+  phase("representatives");
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < S.nsub; is++) {
     const Sub &r = S.subs[S.ord_reps[S.ord_cls[is]]];
@@ -1041,6 +1056,7 @@ void build_orderings(Solver &S) {
     s.dev_ifc = r.dev_ifc;
     s.pad = r.pad;
   }
+  phase("copies");
 }
 
 }  // namespace bddc

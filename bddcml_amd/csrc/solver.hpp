@@ -54,6 +54,9 @@ struct Solver {
   std::vector<char> ig_fixed;
   // state
   bool is_setup = false, solved = false, plan_only = false, reactions_ready = false;
+  // the last solution (local vectors and interface) is in device memory
+  // only; device_fetch_solution brings it to subs[*].sol and uig
+  bool sol_on_device = false;
   std::vector<double> uig;  // interface solution
   std::unique_ptr<Device> dev;
   std::unique_ptr<DeviceInput> din;
@@ -90,6 +93,8 @@ void device_release_input(Solver &S);
 // order of tile-task launches (kernels: 1 = one contiguous range of tasks
 // per XCD, the default; 0 = blockIdx order)
 void set_task_order(int chunked);
+// copies the solution of the last solve to the host (once; no-op otherwise)
+void device_fetch_solution(Solver &S);
 void device_setup_begin(Solver &S);
 void device_setup_finish(Solver &S);
 void device_reduced_rhs(Solver &S);

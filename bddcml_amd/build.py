@@ -3,7 +3,11 @@
     python -m bddcml_amd.build
 
 Every HIP source is compiled with hipcc for gfx950 only; packed FP32 ops are
-disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
+disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).  Matrix-core
+instructions use the VGPR form for their accumulators
+(-mllvm -amdgpu-mfma-vgpr-form): with accumulator registers the compiler
+copied the loop-carried accumulators between register files on every
+unrolled step of the tile products and drained the matrix pipeline there.
 """
 from __future__ import annotations
 
@@ -19,7 +23,8 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 SOURCES = ["kernels_tile.hip", "kernels_misc.hip", "device.hip", "plan.cpp", "api.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-function"]
-HIPFLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HIPFLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def _run(cmd):

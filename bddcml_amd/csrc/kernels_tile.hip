@@ -141,8 +141,8 @@ template <typename T> __device__ __forceinline__ void zero_acc(typename Mfma<T>:
 }
 
 // acc = sum over pairs of opA(A) opB(B) in the "r,q x c,q" form.  Software
-// pipelined: the next pair's tiles are loaded into registers while the
-// matrix cores work on the current pair from LDS.
+// pipelined two pairs deep: the tiles of pairs p+1 and p+2 are in flight in
+// registers while the matrix cores work on pair p from LDS.
 template <typename T>
 __device__ __forceinline__ void gemm_sum(const T *__restrict__ pool, const TilePair *pairs, int p0,
                                          int p1, int base, int kind, T *As, T *Bs,
@@ -151,24 +151,30 @@ __device__ __forceinline__ void gemm_sum(const T *__restrict__ pool, const TileP
   const bool tb = (kind == GEMM_NN || kind == GEMM_TN);
   const T *pb = pool + (size_t)base * TILE_ELEMS;
   if (p0 >= p1) return;
-  TileRegs<T> ra, rb;
-  {
-    const TilePair pr = pairs[p0];
+  TileRegs<T> ra0, rb0, ra1, rb1;
+  auto load = [&](int p, TileRegs<T> &ra, TileRegs<T> &rb) {
+    const TilePair pr = pairs[p];
     tile_load(pb + (size_t)pr.a * TILE_ELEMS, ra);
     tile_load(pb + (size_t)pr.b * TILE_ELEMS, rb);
-  }
-  for (int p = p0; p < p1; p++) {
+  };
+  // one pair from registers: to LDS, refill the registers with pair p + 2,
+  // multiply
+  auto step = [&](int p, TileRegs<T> &ra, TileRegs<T> &rb) {
     __syncthreads();
     tile_store_lds(ra, As, ta);
     tile_store_lds(rb, Bs, tb);
     __syncthreads();
-    if (p + 1 < p1) {
-      const TilePair pr = pairs[p + 1];
-      tile_load(pb + (size_t)pr.a * TILE_ELEMS, ra);
-      tile_load(pb + (size_t)pr.b * TILE_ELEMS, rb);
-    }
+    if (p + 2 < p1) load(p + 2, ra, rb);
     mfma_accumulate<T>(As, Bs, acc);
+  };
+  load(p0, ra0, rb0);
+  if (p0 + 1 < p1) load(p0 + 1, ra1, rb1);
+  int p = p0;
+  for (; p + 1 < p1; p += 2) {
+    step(p, ra0, rb0);
+    step(p + 1, ra1, rb1);
   }
+  if (p < p1) step(p, ra0, rb0);
 }
 
 // E = alpha*C + beta*acc, left in As as element (r,c) at c*LDL + r

@@ -49,6 +49,9 @@ double wall_time() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
+// host seconds spent in device allocations and uploads (setup timers)
+static double g_alloc_s = 0.0, g_upload_s = 0.0;
+
 template <class T>
 struct DBuf {
   T *p = nullptr;
@@ -72,13 +75,17 @@ struct DBuf {
     n = 0;
   }
   void alloc(size_t cnt) {
+    const double t = wall_time();
     release();
     n = cnt;
     HIPCHECK(hipMalloc(&p, std::max<size_t>(cnt, 1) * sizeof(T)));
+    g_alloc_s += wall_time() - t;
   }
   void upload(const std::vector<T> &v, hipStream_t st) {
     alloc(v.size());
+    const double t = wall_time();
     if (!v.empty()) HIPCHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    g_upload_s += wall_time() - t;
   }
   void zero(hipStream_t st) {
     if (n) HIPCHECK(hipMemsetAsync(p, 0, n * sizeof(T), st));
@@ -233,42 +240,48 @@ struct Device {
   int n_sp = 0;
   // interface data
   std::vector<SubDesc> hdesc;
-  DBuf<SubDesc> desc;
-  DBuf<int64_t> loff;
-  DBuf<int> NIs, map, cid, csize, cptr, cdof, cmap;
-  DBuf<double> w, Z, Phi, G, Acs;
+  // uploaded with the second half's arena (plan2)
+  View<SubDesc> desc;
+  View<int64_t> loff;
+  View<int> NIs, map, cid, csize, cptr, cdof, cmap;
+  View<double> w;
+  DBuf<double> Z, Phi, G, Acs;
   int maxNI = 0, maxnc = 0, maxnti = 0, maxnt = 1, maxncT = 0;
   int64_t LV = 0;
-  DBuf<SymvCol> scols;
-  DBuf<SymvRow> srows;
+  View<SymvCol> scols;
+  View<SymvRow> srows;
   DBuf<double> spart;  // partial slots of the symmetric S GEMV
-  DBuf<GemvRow> crows;
-  int nscols = 0, nsrows = 0, ncrows = 0;
-  DBuf<int> aptr;
-  DBuf<int64_t> asrc;
-  DBuf<int> cgptr, cgsrc;
+  DBuf<double> cpart;  // partial slots of the coarse GEMV
+  int nscols = 0, nsrows = 0;
+  View<int> aptr;
+  View<int64_t> asrc;
+  View<int> cgptr, cgsrc;
   // coarse
   int ntc = 0, NC = 0, cbase = 0, clinv = 0, cwbase = 0, cfull = 0;
-  DBuf<int64_t> ca_dst, ca_src;
-  DBuf<int> ca_ptr;
+  View<int64_t> ca_dst, ca_src;
+  View<int> ca_ptr;
   int nca = 0;
   // triangular sweeps
   std::vector<TrsvDesc> htrsv;
-  DBuf<TrsvDesc> trsv;
-  DBuf<int> rptr, rk, rt, tcptr, tck, tct;
+  View<TrsvDesc> trsv;
+  View<int> rptr, rk, rt, tcptr, tck, tct;
   DBuf<double> vec;
   int64_t VT = 0;
-  DBuf<int64_t> rhs_src;
+  View<int64_t> rhs_src;
   // local dof order <-> device order of every subdomain
   std::vector<OrderDesc> hodesc;
-  DBuf<OrderDesc> odesc;
+  View<OrderDesc> odesc;
   int maxndofs = 0;
   int64_t ndofs_total = 0;
-  DBuf<double> fmod, sol;
+  View<double> fmod;  // right-hand side after Dirichlet elimination (rewritten in place)
+  DBuf<double> sol;
   // vectors
   DBuf<double> yl, zl, rcs, rc, uc, partial, dots;
+  DBuf<double> pcg;           // PCG scalars per iteration: r.z, p.Ap, r.r
+  double *hrr = nullptr;      // pinned host copies of r.r per iteration
+  size_t hrr_n = 0;
   DBuf<double> u, r, p, z, ap, rs, v, y, s, t, g, xin, xout;
-  DBuf<char> fixmask;
+  View<char> fixmask;
   int nig = 0;
 };
 
@@ -308,6 +321,7 @@ void device_release(Solver &S) {
     Device &D = *S.dev;
     if (D.st) (void)hipStreamSynchronize(D.st);
     if (D.up) (void)hipStreamSynchronize(D.up);
+    if (D.hrr) (void)hipHostFree(D.hrr), D.hrr = nullptr;
     for (hipEvent_t &e : D.ev)
       if (e) (void)hipEventDestroy(e), e = nullptr;
     if (D.ev_up) (void)hipEventDestroy(D.ev_up), D.ev_up = nullptr;
@@ -835,9 +849,12 @@ void device_setup_finish(Solver &S) {
   const int nsub = S.nsub;
   double t0 = wall_time();
   double tph = t0;
+  g_alloc_s = g_upload_s = 0.0;
   auto ph = [&](const char *name) {
     double now = wall_time();
     if (S.verbose >= 2) std::printf("   device plan: %-24s %8.4f s\n", name, now - tph);
+    if (S.verbose >= 3)
+      std::printf("   device plan:   (allocations %.4f s, uploads %.4f s so far)\n", g_alloc_s, g_upload_s);
     tph = now;
   };
   // ---- second half of the plan, now that the constraints are known: the
@@ -1051,7 +1068,6 @@ void device_setup_finish(Solver &S) {
     D.plan2.add(D.sp_dst, spdst);
     D.n_sp = (int)spsub.size();
   }
-  D.plan2.commit(st);
   ph("factor and coarse lists");
   if (S.verbose >= 2)
     std::printf("   device plan: %lld factor tiles (%.1f GB), %lld coarse tiles\n",
@@ -1148,16 +1164,16 @@ void device_setup_finish(Solver &S) {
   if ((int64_t)D.ntc * D.ntc > ((int64_t)1 << 24))
     fatal("BDDCML_SETUP_PRECONDITIONER", "coarse problem too large for its dense factorization");
   D.LV = lv;
-  D.desc.upload(D.hdesc, st);
-  D.loff.upload(loff, st);
-  D.NIs.upload(NIs, st);
-  D.map.upload(map, st);
-  D.w.upload(wv, st);
-  D.cid.upload(cidv, st);
-  D.csize.upload(csz, st);
-  D.cptr.upload(cptrv, st);
-  D.cdof.upload(cdofv, st);
-  D.cmap.upload(cmapv, st);
+  D.plan2.add(D.desc, D.hdesc);
+  D.plan2.add(D.loff, loff);
+  D.plan2.add(D.NIs, NIs);
+  D.plan2.add(D.map, map);
+  D.plan2.add(D.w, wv);
+  D.plan2.add(D.cid, cidv);
+  D.plan2.add(D.csize, csz);
+  D.plan2.add(D.cptr, cptrv);
+  D.plan2.add(D.cdof, cdofv);
+  D.plan2.add(D.cmap, cmapv);
   D.Z.alloc(std::max<int64_t>(zo, 1));
   D.Phi.alloc(std::max<int64_t>(zo, 1));
   D.G.alloc(std::max<int64_t>(go, 1));
@@ -1166,7 +1182,6 @@ void device_setup_finish(Solver &S) {
   // GEMV rows
   std::vector<SymvCol> sc;
   std::vector<SymvRow> sr;
-  std::vector<GemvRow> cr;
   {
     int64_t ps = 0;
     for (int is = 0; is < nsub; is++) {
@@ -1190,15 +1205,14 @@ void device_setup_finish(Solver &S) {
       if (c.tile0 < 0 || (int64_t)c.tile0 + c.nt > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid GEMV column");
     D.spart.alloc((size_t)std::max<int64_t>(ps, 1) * 64);
   }
-  for (int I = 0; I < D.ntc; I++) cr.push_back({D.cfull + I * D.ntc, D.ntc, 0, I});
-  for (const GemvRow &g : cr)
-    if (g.tile0 < 0 || (int64_t)g.tile0 + g.nt > D.ntilesc) fatal("BDDCML_DEVICE_PLAN", "invalid GEMV row");
-  D.scols.upload(sc, st);
-  D.srows.upload(sr, st);
-  D.crows.upload(cr, st);
+  // coarse inverse GEMV over the full-storage tiles
+  if (D.cfull < 0 || (int64_t)D.cfull + (int64_t)D.ntc * D.ntc > D.ntilesc)
+    fatal("BDDCML_DEVICE_PLAN", "invalid coarse GEMV tiles");
+  D.cpart.alloc((size_t)D.ntc * D.ntc * 64);
+  D.plan2.add(D.scols, sc);
+  D.plan2.add(D.srows, sr);
   D.nsrows = (int)sr.size();
   D.nscols = (int)sc.size();
-  D.ncrows = (int)cr.size();
   ph("interface maps");
   // assembly CSR: unique interface dof -> local flat indices (subdomain order)
   {
@@ -1210,8 +1224,8 @@ void device_setup_finish(Solver &S) {
     std::vector<int> fill(cnt.begin(), cnt.end() - 1);
     for (int64_t k = 0; k < lv; k++)
       if (map[k] >= 0) src[fill[map[k]]++] = k;
-    D.aptr.upload(cnt, st);
-    D.asrc.upload(src, st);
+    D.plan2.add(D.aptr, cnt);
+    D.plan2.add(D.asrc, src);
     // the same contributors in the device-ordered subdomain vectors
     std::vector<int64_t> voff(nsub);
     int64_t vt = 0;
@@ -1227,7 +1241,7 @@ void device_setup_finish(Solver &S) {
       }
       D.ndofs_total = po;
     }
-    D.odesc.upload(D.hodesc, st);
+    D.plan2.add(D.odesc, D.hodesc);
     std::vector<int64_t> vpos(lv);
 #pragma omp parallel for schedule(static)
     for (int is = 0; is < nsub; is++)
@@ -1235,7 +1249,7 @@ void device_setup_finish(Solver &S) {
     std::vector<int64_t> rsrc(src.size());
 #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < src.size(); q++) rsrc[q] = vpos[src[q]];
-    D.rhs_src.upload(rsrc, st);
+    D.plan2.add(D.rhs_src, rsrc);
     // triangular sweep lists of the interior factor: row lists (forward) and
     // column lists (backward) of the strictly lower factored tiles
     D.htrsv.resize(nsub);
@@ -1286,13 +1300,13 @@ void device_setup_finish(Solver &S) {
     for (int id : ctv)
       if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid sweep tile");
     D.maxnt = maxnt;
-    D.trsv.upload(D.htrsv, st);
-    D.rptr.upload(rp, st);
-    D.rk.upload(rkv, st);
-    D.rt.upload(rtv, st);
-    D.tcptr.upload(cp, st);
-    D.tck.upload(ckv, st);
-    D.tct.upload(ctv, st);
+    D.plan2.add(D.trsv, D.htrsv);
+    D.plan2.add(D.rptr, rp);
+    D.plan2.add(D.rk, rkv);
+    D.plan2.add(D.rt, rtv);
+    D.plan2.add(D.tcptr, cp);
+    D.plan2.add(D.tck, ckv);
+    D.plan2.add(D.tct, ctv);
   }
   ph("assembly and sweep lists");
   // coarse gather CSR: coarse dof -> constraint rows (subdomain order)
@@ -1303,8 +1317,8 @@ void device_setup_finish(Solver &S) {
     std::vector<int> src(cnt[S.ndofc]);
     std::vector<int> fill(cnt.begin(), cnt.end() - 1);
     for (size_t k = 0; k < cmapv.size(); k++) src[fill[cmapv[k]]++] = (int)k;
-    D.cgptr.upload(cnt, st);
-    D.cgsrc.upload(src, st);
+    D.plan2.add(D.cgptr, cnt);
+    D.plan2.add(D.cgsrc, src);
   }
   // coarse assembly CSR over the lower tiles of the coarse matrix
   {
@@ -1338,9 +1352,9 @@ void device_setup_finish(Solver &S) {
       src.push_back(ent[k].src);
     }
     ptr.push_back((int)ent.size());
-    D.ca_dst.upload(dst, st);
-    D.ca_src.upload(src, st);
-    D.ca_ptr.upload(ptr, st);
+    D.plan2.add(D.ca_dst, dst);
+    D.plan2.add(D.ca_src, src);
+    D.plan2.add(D.ca_ptr, ptr);
     D.nca = (int)dst.size();
   }
   // vectors
@@ -1356,7 +1370,7 @@ void device_setup_finish(Solver &S) {
                           &D.xin, &D.xout})
     b->alloc(std::max(S.nig, 1));
   D.vec.alloc(D.VT);
-  D.fixmask.upload(std::vector<char>(S.ig_fixed.begin(), S.ig_fixed.end()), st);
+  D.plan2.add(D.fixmask, std::vector<char>(S.ig_fixed.begin(), S.ig_fixed.end()));
   D.rc.zero(st);
   D.uc.zero(st);
   {
@@ -1364,9 +1378,10 @@ void device_setup_finish(Solver &S) {
 #pragma omp parallel for schedule(static)
     for (int is = 0; is < nsub; is++)
       std::copy(S.subs[is].fmod.begin(), S.subs[is].fmod.end(), fm.begin() + D.hodesc[is].poff);
-    D.fmod.upload(fm, st);
+    D.plan2.add(D.fmod, fm);
     D.sol.alloc(D.ndofs_total);
   }
+  D.plan2.commit(st);  // every array of the second half in one copy
   ph("coarse lists, vectors");
   HIPCHECK(hipEventRecord(D.ev_up, D.up));
   S.t.plan += wall_time() - t0;
@@ -1666,7 +1681,7 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed = false) {
   // coarse correction uc = Ac^{-1} sum_i R_ci^T Phi_i^T W_i r_i
   launch_phit(st, D.desc.p, S.nsub, D.Phi.p, D.map.p, D.w.p, r, D.rcs.p, D.maxnc);
   launch_coarse_gather(st, S.ndofc, D.cgptr.p, D.cgsrc.p, D.rcs.p, D.rc.p);
-  launch_gemv_plain(st, D.cpool.p, D.crows.p, D.ncrows, D.rc.p, D.uc.p);
+  launch_dense_gemv(st, D.cpool.p, D.cfull, D.ntc, D.rc.p, D.cpart.p, D.uc.p);
   // zl_i = K_i W_i r_i + Phi_i uc(indrowc), then z = sum_i R_i^T W_i zl_i
   if (timed) {
     HIPCHECK(hipEventCreate(&e0));
@@ -1698,6 +1713,15 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed = false) {
     (void)hipEventDestroy(e1);
   }
   launch_assemble(st, S.nig, D.aptr.p, D.asrc.p, D.w.p, D.zl.p, z);
+}
+
+// dot products into device memory (no host synchronization)
+static void dots_dev(Solver &S, const double *a, const double *b, double *dst) {
+  DotArgs args{};
+  args.ndot = 1;
+  args.a[0] = a;
+  args.b[0] = b;
+  launch_dots(S.dev->st, S.nig, args, S.dev->partial.p, dst);
 }
 
 static void dots(Solver &S, std::initializer_list<std::pair<const double *, const double *>> prs,
@@ -1836,21 +1860,37 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, in
     double normres0 = std::sqrt(sc[0]);
     if (normres0 == 0.0) goto finish;
     op_pc(S, D.r.p, D.p.p);
-    dots(S, {{D.r.p, D.p.p}}, sc);
-    double rmp = sc[0];
-    std::vector<double> diag(maxit + 2, 0.0), sub(maxit + 2, 0.0);
-    int ndecr = 0, nw = 0;
+    // scalars stay on the device: rz[k] = r.z, pap[k] = p.Ap, rr[k] = r.r
+    // of iteration k; the host reads only rr[k], and while it waits the
+    // device already applies the preconditioner for the next direction
+    // (work that is simply unused when the residual test stops)
+    const size_t m = (size_t)maxit + 2;
+    D.pcg.alloc(3 * m);
+    double *rz = D.pcg.p, *pap = D.pcg.p + m, *rr = D.pcg.p + 2 * m;
+    if (D.hrr_n < m) {
+      if (D.hrr) (void)hipHostFree(D.hrr), D.hrr = nullptr;
+      HIPCHECK(hipHostMalloc((void **)&D.hrr, m * sizeof(double), hipHostMallocDefault));
+      D.hrr_n = m;
+    }
+    dots_dev(S, D.r.p, D.p.p, rz);
+    hipEvent_t evr = nullptr;
+    HIPCHECK(hipEventCreateWithFlags(&evr, hipEventDisableTiming));
+    int ndecr = 0, nw = 0, last = 0;
     double lastres = 1.0;
     for (int iter = 1; iter <= maxit; iter++) {
       op_schur(S, D.p.p, D.ap.p, iter == 2);
-      dots(S, {{D.p.p, D.ap.p}}, sc);
-      double pap = sc[0];
-      if (pap <= 0.0 && S.verbose > 0)
-        std::fprintf(stderr, "WARNING in KRYLOV_BDDCPCG: System matrix not positive definite!\n");
-      double alpha = rmp / pap;
-      launch_axpy2(st, n, alpha, D.p.p, D.u.p, -alpha, D.ap.p, D.r.p);
-      dots(S, {{D.r.p, D.r.p}}, sc);
-      double relres = std::sqrt(sc[0]) / normres0;
+      dots_dev(S, D.p.p, D.ap.p, pap + iter);
+      launch_pcg_update(st, n, rz, pap, iter, D.p.p, D.ap.p, D.u.p, D.r.p);
+      dots_dev(S, D.r.p, D.r.p, rr + iter);
+      HIPCHECK(hipMemcpyAsync(D.hrr + iter, rr + iter, sizeof(double), hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipEventRecord(evr, st));
+      // next direction, computed ahead of the residual test
+      op_pc(S, D.r.p, D.z.p, iter == 2);
+      dots_dev(S, D.r.p, D.z.p, rz + iter);
+      launch_pcg_dir(st, n, rz, iter, D.z.p, D.p.p);
+      HIPCHECK(hipEventSynchronize(evr));
+      last = iter;
+      double relres = std::sqrt(D.hrr[iter]) / normres0;
       S.hist.push_back(relres);
       if (S.verbose > 0) std::printf(" KRYLOV_BDDCPCG: iteration: %d relative residual: %.6e\n", iter, relres);
       if (relres < tol) {
@@ -1873,17 +1913,29 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, in
         break;
       }
       lastres = relres;
-      op_pc(S, D.r.p, D.z.p, iter == 2);
-      dots(S, {{D.r.p, D.z.p}}, sc);
-      double rmpold = rmp;
-      rmp = sc[0];
-      if (rmp <= 0.0 && S.verbose > 0)
-        std::fprintf(stderr, "WARNING in KRYLOV_BDDCPCG: Preconditioner not positive definite!\n");
-      double beta = rmp / rmpold;
-      launch_xpay(st, n, beta, D.z.p, D.p.p);
-      diag[iter - 1] += 1.0 / alpha;
-      diag[iter] = beta / alpha;
-      sub[iter - 1] = beta >= 0.0 ? -std::sqrt(beta) / alpha : 0.0;
+    }
+    (void)hipEventDestroy(evr);
+    // Lanczos tridiagonal of the finished iterations for the condition
+    // number estimate (src/module_krylov.f90)
+    std::vector<double> hrz(m, 0.0), hpap(m, 0.0);
+    HIPCHECK(hipMemcpyAsync(hrz.data(), rz, m * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(hpap.data(), pap, m * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    std::vector<double> diag(maxit + 2, 0.0), sub(maxit + 2, 0.0);
+    bool pap_neg = false, rz_neg = false;
+    for (int k = 1; k <= last; k++) {
+      pap_neg |= hpap[k] <= 0.0;
+      if (k <= nw) rz_neg |= hrz[k] <= 0.0;
+    }
+    if (S.verbose > 0 && pap_neg)
+      std::fprintf(stderr, "WARNING in KRYLOV_BDDCPCG: System matrix not positive definite!\n");
+    if (S.verbose > 0 && rz_neg)
+      std::fprintf(stderr, "WARNING in KRYLOV_BDDCPCG: Preconditioner not positive definite!\n");
+    for (int k = 1; k <= nw; k++) {
+      const double alpha = hrz[k - 1] / hpap[k], beta = hrz[k] / hrz[k - 1];
+      diag[k - 1] += 1.0 / alpha;
+      diag[k] = beta / alpha;
+      sub[k - 1] = beta >= 0.0 ? -std::sqrt(beta) / alpha : 0.0;
     }
     *cond = nw > 1 ? tridiag_max_eig(diag, sub, nw) : 1.0;
   } else {

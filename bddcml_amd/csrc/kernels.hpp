@@ -141,12 +141,6 @@ void launch_ifc_scatter(hipStream_t st, int n, const int *ptr, const int64_t *po
                         double *vec);
 
 // ---- iteration kernels ---------------------------------------------------
-struct GemvRow {
-  int tile0;      // first tile of the row (row-major full storage)
-  int nt;         // tiles in the row
-  int mat;        // matrix index (subdomain) for gather
-  int I;          // tile row
-};
 // Symmetric tile GEMV over the packed lower tiles of each subdomain's S:
 // y_local[loff + I*64 + r] = sum_J S(I,J) x_J with x_J[r] = xin[map[loff + J*64 + r]]
 // (0 where map < 0).  One workgroup per tile column (SymvCol) writes partial
@@ -167,9 +161,10 @@ struct SymvRow {
 void launch_symv(hipStream_t st, const double *pool, const SymvCol *cols, int ncols,
                  const SymvRow *rows, int nrows, const int64_t *loff, const int *map,
                  const double *xin, double *part, double *yout);
-// y[I*64 + r] = sum_J T(I,J) x_J with x read directly (coarse inverse)
-void launch_gemv_plain(hipStream_t st, const double *pool, const GemvRow *rows, int nrows,
-                       const double *xin, double *yout);
+// y = A x for a dense nt x nt tile matrix in full storage (tile (I, J) at
+// base + I*nt + J); part holds nt*nt*64 doubles (coarse inverse)
+void launch_dense_gemv(hipStream_t st, const double *pool, int base, int nt, const double *x,
+                       double *part, double *y);
 // unique y[g] = sum_{q} (w? w[src]:1) * (ylocal[src] + (phiu? phiu[src]:0))
 void launch_assemble(hipStream_t st, int n, const int *ptr, const int64_t *src, const double *w,
                      const double *ylocal, double *y);
@@ -200,6 +195,12 @@ void launch_axpy2(hipStream_t st, int n, double a, const double *x, double *y, d
                   const double *x2, double *y2);
 // y = x + b*y
 void launch_xpay(hipStream_t st, int n, double b, const double *x, double *y);
+// PCG steps with the scalars in device memory (iteration k >= 1):
+// u += alpha p, r -= alpha ap with alpha = rz[k-1] / pap[k]
+void launch_pcg_update(hipStream_t st, int n, const double *rz, const double *pap, int k,
+                       const double *p, const double *ap, double *u, double *r);
+// p = z + beta p with beta = rz[k] / rz[k-1]
+void launch_pcg_dir(hipStream_t st, int n, const double *rz, int k, const double *z, double *p);
 // y = x + b*(y - w*v)   (BiCGSTAB direction update)
 void launch_bicg_p(hipStream_t st, int n, double b, double w, const double *r, const double *v,
                    double *p);

@@ -439,36 +439,41 @@ void launch_symv(hipStream_t st, const double *pool, const SymvCol *cols, int nc
                      part, yout);
 }
 
-// dense GEMV over full-storage tile rows with the input read from global
-// memory (no staging; for the coarse inverse, whose input is one short
-// vector shared by every row)
-__global__ __launch_bounds__(256) void k_gemv_plain(const double *__restrict__ pool,
-                                                    const GemvRow *__restrict__ rows,
-                                                    const double *__restrict__ xin,
-                                                    double *__restrict__ yout) {
-  __shared__ double red[256];
-  const GemvRow row = rows[blockIdx.x];
-  const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
+// y = A x for a dense matrix of nt x nt tiles in full storage (tile (I, J)
+// at base + I*nt + J), x read from global memory: one workgroup per tile
+// writes the tile's product to partial slot (J, I); the reduce sums the
+// slots of each row in a fixed order.  For the coarse inverse, whose tiles
+// are few but large in total.
+__global__ __launch_bounds__(256) void k_dense_gemv(const double *__restrict__ pool, int base,
+                                                    int nt, const double *__restrict__ x,
+                                                    double *__restrict__ part) {
+  __shared__ double red[4][64];
+  const int I = blockIdx.x / nt, J = blockIdx.x % nt;
+  const int w = threadIdx.x >> 6, r = threadIdx.x & 63;
+  const double *T = pool + (size_t)(base + I * nt + J) * TILE_ELEMS + r;
+  const double *xj = x + J * 64;
   double s0 = 0.0, s1 = 0.0;
-  for (int J = g; J < row.nt; J += 4) {
-    const double *T = pool + (size_t)(row.tile0 + J) * TILE_ELEMS + r;
-    const double *x = xin + J * 64;
-#pragma unroll 8
-    for (int c = 0; c < 64; c += 2) {
-      s0 += T[c * 64] * x[c];
-      s1 += T[(c + 1) * 64] * x[c + 1];
-    }
+#pragma unroll
+  for (int c = 16 * w; c < 16 * w + 16; c += 2) {
+    s0 += T[c * 64] * xj[c];
+    s1 += T[(c + 1) * 64] * xj[c + 1];
   }
-  red[threadIdx.x] = s0 + s1;
+  red[w][r] = s0 + s1;
   __syncthreads();
-  if (threadIdx.x < 64)
-    yout[row.I * 64 + r] = (red[r] + red[64 + r]) + (red[128 + r] + red[192 + r]);
+  if (w == 0) part[((size_t)J * nt + I) * 64 + r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
 }
-
-void launch_gemv_plain(hipStream_t st, const double *pool, const GemvRow *rows, int nrows,
-                       const double *xin, double *yout) {
-  if (nrows <= 0) return;
-  hipLaunchKernelGGL(k_gemv_plain, dim3(nrows), dim3(256), 0, st, pool, rows, xin, yout);
+__global__ void k_dense_gemv_reduce(int nt, const double *__restrict__ part, double *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt * 64) return;
+  double s = 0.0;
+  for (int J = 0; J < nt; J++) s += part[(size_t)J * nt * 64 + i];
+  y[i] = s;
+}
+void launch_dense_gemv(hipStream_t st, const double *pool, int base, int nt, const double *x,
+                       double *part, double *y) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_dense_gemv, dim3(nt * nt), dim3(256), 0, st, pool, base, nt, x, part);
+  hipLaunchKernelGGL(k_dense_gemv_reduce, dim3((nt * 64 + 255) / 256), dim3(256), 0, st, nt, part, y);
 }
 
 __global__ void k_assemble(int n, const int *__restrict__ ptr, const int64_t *__restrict__ src,
@@ -714,6 +719,32 @@ __global__ void k_xpay(int n, double b, const double *__restrict__ x, double *__
 void launch_xpay(hipStream_t st, int n, double b, const double *x, double *y) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_xpay, dim3((n + 255) / 256), dim3(256), 0, st, n, b, x, y);
+}
+__global__ void k_pcg_update(int n, const double *__restrict__ rz, const double *__restrict__ pap,
+                             int k, const double *__restrict__ p, const double *__restrict__ ap,
+                             double *__restrict__ u, double *__restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double alpha = rz[k - 1] / pap[k];
+  u[i] += alpha * p[i];
+  r[i] -= alpha * ap[i];
+}
+void launch_pcg_update(hipStream_t st, int n, const double *rz, const double *pap, int k,
+                       const double *p, const double *ap, double *u, double *r) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pcg_update, dim3((n + 255) / 256), dim3(256), 0, st, n, rz, pap, k, p, ap,
+                     u, r);
+}
+__global__ void k_pcg_dir(int n, const double *__restrict__ rz, int k,
+                          const double *__restrict__ z, double *__restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double beta = rz[k] / rz[k - 1];
+  p[i] = z[i] + beta * p[i];
+}
+void launch_pcg_dir(hipStream_t st, int n, const double *rz, int k, const double *z, double *p) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pcg_dir, dim3((n + 255) / 256), dim3(256), 0, st, n, rz, k, z, p);
 }
 __global__ void k_bicg_p(int n, double b, double w, const double *__restrict__ r,
                          const double *__restrict__ v, double *__restrict__ p) {

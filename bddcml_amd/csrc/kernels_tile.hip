@@ -299,17 +299,30 @@ __device__ __forceinline__ typename Mfma<T>::v4 mfma16(typename Mfma<T>::v4 acc,
   return acc;
 }
 
-// Cholesky factor of the 16x16 diagonal block at (c0, c0) by one wave in
-// registers: lane i (mod 16) holds row i, cross-lane values move by
-// shuffles; L goes back to As with its upper part zeroed.
-template <typename T> __device__ __forceinline__ void chol16(T *As, int c0, int &bad) {
+// value of x in lane l (uniform l) for every lane of the wave
+__device__ __forceinline__ double lane_bcast(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float lane_bcast(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// Cholesky factor L of the 16x16 diagonal block at (c0, c0) and its inverse,
+// by one wave in registers: lane i (mod 16) holds row i of the block, then
+// column i of inv(L); every cross-lane value is a broadcast from one lane
+// (lane reads into scalar registers).  L goes back to As (upper part
+// zeroed), inv(L) to the same block of Bs.
+template <typename T> __device__ __forceinline__ void chol_inv16(T *As, T *Bs, int c0, int &bad) {
   const int lane = threadIdx.x & 63, i = lane & 15;
   T a[16];
 #pragma unroll
   for (int c = 0; c < 16; c++) a[c] = As[(c0 + c) * LDL + c0 + i];
 #pragma unroll
   for (int j = 0; j < 16; j++) {
-    T d = __shfl(a[j], j, 64);
+    T d = lane_bcast(a[j], j);
     if (!(d > (T)0)) {
       bad = 1;
       d = (T)1;
@@ -318,35 +331,32 @@ template <typename T> __device__ __forceinline__ void chol16(T *As, int c0, int 
     a[j] = (i == j) ? sd : (i > j ? a[j] * inv : a[j]);
 #pragma unroll
     for (int k = j + 1; k < 16; k++) {
-      const T lkj = __shfl(a[j], k, 64);
+      const T lkj = lane_bcast(a[j], k);
       if (i >= k) a[k] -= a[j] * lkj;
     }
   }
-  if (lane < 16) {
 #pragma unroll
-    for (int c = 0; c < 16; c++) As[(c0 + c) * LDL + c0 + i] = (c <= i) ? a[c] : (T)0;
-  }
-}
-
-// inv(L) of the factored 16x16 diagonal block into Bs: lane i (< 16) solves
-// for column i by forward substitution, reading L from As (broadcast reads)
-template <typename T> __device__ __forceinline__ void trinv16(const T *As, T *Bs, int c0) {
-  const int i = threadIdx.x & 63;
-  if (i >= 16) return;
+  for (int c = 0; c < 16; c++)
+    if (c > i) a[c] = (T)0;
+  // column i of inv(L) by forward substitution; L(r, k) is a[k] of lane r
   T x[16];
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     T sacc = (r == i) ? (T)1 : (T)0;
 #pragma unroll
-    for (int k = 0; k < r; k++) sacc -= As[(c0 + k) * LDL + c0 + r] * x[k];
-    x[r] = sacc / As[(c0 + r) * LDL + c0 + r];
+    for (int k = 0; k < r; k++) sacc -= lane_bcast(a[k], r) * x[k];
+    x[r] = sacc / lane_bcast(a[r], r);
   }
+  if (lane < 16) {
 #pragma unroll
-  for (int r = 0; r < 16; r++) Bs[(c0 + i) * LDL + c0 + r] = x[r];
+    for (int c = 0; c < 16; c++) As[(c0 + c) * LDL + c0 + i] = a[c];
+#pragma unroll
+    for (int r = 0; r < 16; r++) Bs[(c0 + i) * LDL + c0 + r] = x[r];
+  }
 }
 
 // C = C - sum A B^T over the task's pairs; C = chol(C) (lower, upper zeroed);
-// X = inv(chol(C)).  Blocked by 16: diagonal blocks in registers (chol16),
+// X = inv(chol(C)).  Blocked by 16: diagonal blocks in registers (chol_inv16),
 // panel solves, trailing updates and the off-diagonal blocks of the inverse
 // (X_IJ = -inv(L_II) sum_{J<=K<I} L_IK X_KJ) on the matrix cores.
 template <typename T>
@@ -383,11 +393,9 @@ __global__ __launch_bounds__(NTHR) void k_tile_potrf(T *__restrict__ pool,
     __syncthreads();
     if (wave == 0) {
       int b = 0;
-      chol16<T>(As, c0, b);
+      chol_inv16<T>(As, Bs, c0, b);
       if (b && lane == 0) bad = 1;
     }
-    __syncthreads();
-    if (wave == 0) trinv16<T>(As, Bs, c0);
     __syncthreads();
     // panel: L(R, J) = A(R, J) inv(L_JJ)^T
     if (wave < 3 - J) {

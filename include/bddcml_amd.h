@@ -49,7 +49,9 @@ int bddcml_amd_timings(double *out, int n);
  *   "precision" (64: FP64 throughout, the default; 32: the augmented
  *     interface factor S', its triangular sweeps and Z = S'^{-1} C^T in FP32,
  *     while the Schur operator, the Krylov method and the coarse problem
- *     stay FP64) */
+ *     stay FP64);
+ *   "task_order" (1: each XCD runs one contiguous range of a tile-task
+ *     launch, the default; 0: plain workgroup order) */
 int bddcml_amd_set_option(const char *key, double value);
 /* per subdomain (isub in numbase): out[0]=ndofi out[1]=nconstr out[2]=ndofo */
 int bddcml_amd_sub_info(int isub, int *out);

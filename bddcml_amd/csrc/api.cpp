@@ -248,14 +248,14 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
   phase("globs");
   build_constraints(S);
   phase("constraints");
+  build_unique_interface(S);  // the exchange sums below run over it
+  phase("unique interface");
   apply_boundary_conditions(S);
   phase("boundary conditions");
   build_weights(S);
   phase("weights");
   build_condensation_rhs(S);
   phase("condensation rhs");
-  build_unique_interface(S);
-  phase("unique interface");
   S.t.topology += wall_time() - th;
   if (S.plan_only) {
     S.is_setup = true;

@@ -495,15 +495,16 @@ void build_globs(Solver &S) {
 
 // ------------------------------------------------------------ BC, weights
 // Sum interface values over all subdomains sharing a dof.
+// Sums over the copies of each interface dof in subdomain order, through
+// the unique-interface CSR (build_unique_interface): parallel over dofs and
+// identical to a serial pass over the subdomains.
 static void exchange_sum(Solver &S, std::vector<std::vector<double>> &v) {
-  std::vector<double> acc(S.ndof_g, 0.0);
-  for (int is = 0; is < S.nsub; is++) {
-    const Sub &s = S.subs[is];
-    for (size_t k = 0; k < s.ifc_dof.size(); k++) acc[s.isvgvn[s.ifc_dof[k]]] += v[is][k];
-  }
-  for (int is = 0; is < S.nsub; is++) {
-    const Sub &s = S.subs[is];
-    for (size_t k = 0; k < s.ifc_dof.size(); k++) v[is][k] = acc[s.isvgvn[s.ifc_dof[k]]];
+  if ((int)S.ig_ptr.size() != S.nig + 1) fatal("DD_EXCHANGE", "interface numbering missing");
+#pragma omp parallel for schedule(static, 1024)
+  for (int u = 0; u < S.nig; u++) {
+    double acc = 0.0;
+    for (int q = S.ig_ptr[u]; q < S.ig_ptr[u + 1]; q++) acc += v[S.ig_sub[q]][S.ig_k[q]];
+    for (int q = S.ig_ptr[u]; q < S.ig_ptr[u + 1]; q++) v[S.ig_sub[q]][S.ig_k[q]] = acc;
   }
 }
 
@@ -571,12 +572,14 @@ static void eliminate_bc(Sub &s) {
 // sum of the eliminated values over the subdomains sharing an interface dof
 static void exchange_bc(Solver &S) {
   std::vector<std::vector<double>> b(S.nsub);
+#pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < S.nsub; is++) {
     Sub &s = S.subs[is];
     b[is].resize(s.ifc_dof.size());
     for (size_t k = 0; k < s.ifc_dof.size(); k++) b[is][k] = s.bc[s.ifc_dof[k]];
   }
   exchange_sum(S, b);
+#pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < S.nsub; is++) {
     Sub &s = S.subs[is];
     for (size_t k = 0; k < s.ifc_dof.size(); k++) s.bc[s.ifc_dof[k]] = b[is][k];
@@ -640,7 +643,12 @@ void apply_boundary_conditions(Solver &S) {
 }
 
 void build_weights(Solver &S) {
+  if (S.weights_type != 0 && S.weights_type != 1 && S.weights_type != 3)
+    fatal("DD_GET_MY_COEFFICIENTS_FOR_WEIGHTS",
+          "Type of weights not supported: " + std::to_string(S.weights_type));
   std::vector<std::vector<double>> rho(S.nsub), tot(S.nsub);
+  int bad_data = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad_data)
   for (int is = 0; is < S.nsub; is++) {
     Sub &s = S.subs[is];
     size_t ni = s.ifc_dof.size();
@@ -654,26 +662,30 @@ void build_weights(Solver &S) {
         if (rho[is][k] == 0.0) rho[is][k] = 1.0;
       }
     } else if (S.weights_type == 3) {
-      if ((int)s.dof_data.size() != s.ndofs)
-        fatal("DD_GET_MY_COEFFICIENTS_FOR_WEIGHTS", "dof_data must have ndofs entries for weights type 3");
+      if ((int)s.dof_data.size() != s.ndofs) {
+        bad_data = 1;
+        continue;
+      }
       for (size_t k = 0; k < ni; k++) rho[is][k] = s.dof_data[s.ifc_dof[k]];
-    } else if (S.weights_type != 0) {
-      fatal("DD_GET_MY_COEFFICIENTS_FOR_WEIGHTS",
-            "Type of weights not supported: " + std::to_string(S.weights_type));
     }
     tot[is] = rho[is];
   }
+  if (bad_data)
+    fatal("DD_GET_MY_COEFFICIENTS_FOR_WEIGHTS", "dof_data must have ndofs entries for weights type 3");
   exchange_sum(S, tot);
+  int zero_sub = S.nsub;
+#pragma omp parallel for schedule(dynamic) reduction(min : zero_sub)
   for (int is = 0; is < S.nsub; is++) {
     Sub &s = S.subs[is];
     s.w.resize(s.ifc_dof.size());
     for (size_t k = 0; k < s.w.size(); k++) {
-      if (tot[is][k] == 0.0)
-        fatal("DD_WEIGHTS_PREPARE", "Zero in denominator in computation of weights for subdomain: " +
-                                        std::to_string(s.isub + S.numbase));
+      if (tot[is][k] == 0.0) zero_sub = std::min(zero_sub, is);
       s.w[k] = rho[is][k] / tot[is][k];
     }
   }
+  if (zero_sub < S.nsub)
+    fatal("DD_WEIGHTS_PREPARE", "Zero in denominator in computation of weights for subdomain: " +
+                                    std::to_string(S.subs[zero_sub].isub + S.numbase));
 }
 
 // ------------------------------------------------------------ constraints
@@ -771,6 +783,7 @@ void build_constraints(Solver &S) {
 // restriction of a global rhs, Dirichlet values eliminated
 // (src/module_dd.f90:7068-7087, src/module_sm.f90:602-628)
 void build_condensation_rhs(Solver &S, int isub_only) {
+#pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < S.nsub; is++) {
     if (isub_only >= 0 && is != isub_only) continue;
     Sub &s = S.subs[is];
@@ -788,20 +801,88 @@ void build_condensation_rhs(Solver &S, int isub_only) {
   }
 }
 
+// Unique interface: sorted global dofs of every subdomain's interface, the
+// position of each global dof in it, fixed flags, and the CSR of the copies
+// of each unique dof (subdomain, interface index), subdomains ascending.
 void build_unique_interface(Solver &S) {
-  std::vector<char> on(S.ndof_g, 0), fixed(S.ndof_g, 0);
-  for (auto &s : S.subs)
+  const int ng = S.ndof_g;
+  std::vector<char> on(ng, 0), fixed(ng, 0);
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++) {
+    const Sub &s = S.subs[is];
     for (int d : s.ifc_dof) {
-      on[s.isvgvn[d]] = 1;
-      if (s.has_fix && s.ifix[d]) fixed[s.isvgvn[d]] = 1;
+      __atomic_store_n(&on[s.isvgvn[d]], (char)1, __ATOMIC_RELAXED);
+      if (s.has_fix && s.ifix[d]) __atomic_store_n(&fixed[s.isvgvn[d]], (char)1, __ATOMIC_RELAXED);
     }
-  S.igdof.clear();
-  S.gpos.assign(S.ndof_g, -1);
-  for (int g = 0; g < S.ndof_g; g++)
-    if (on[g]) S.gpos[g] = (int)S.igdof.size(), S.igdof.push_back(g);
-  S.nig = (int)S.igdof.size();
-  S.ig_fixed.resize(S.nig);
-  for (int k = 0; k < S.nig; k++) S.ig_fixed[k] = fixed[S.igdof[k]];
+  }
+  S.gpos.assign(ng, -1);
+  {
+    const int nb = 64, blk = (ng + nb - 1) / nb;
+    std::vector<int> part(nb + 1, 0);
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < nb; b++) {
+      int c = 0;
+      for (int g = b * blk; g < std::min(ng, (b + 1) * blk); g++) c += on[g];
+      part[b + 1] = c;
+    }
+    for (int b = 0; b < nb; b++) part[b + 1] += part[b];
+    S.nig = part[nb];
+    S.igdof.resize(S.nig);
+    S.ig_fixed.resize(S.nig);
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < nb; b++) {
+      int u = part[b];
+      for (int g = b * blk; g < std::min(ng, (b + 1) * blk); g++)
+        if (on[g]) S.gpos[g] = u, S.igdof[u] = g, S.ig_fixed[u] = fixed[g], u++;
+    }
+  }
+  // copies: counts, then each subdomain fills its slots; a copy's rank among
+  // the copies of its dof is the number of lower-numbered subdomains that
+  // hold the dof
+  std::vector<int> cnt(S.nig + 1, 0);
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < S.nsub; is++)
+    for (int d : S.subs[is].ifc_dof) __atomic_fetch_add(&cnt[S.gpos[S.subs[is].isvgvn[d]] + 1], 1, __ATOMIC_RELAXED);
+  for (int u = 0; u < S.nig; u++) cnt[u + 1] += cnt[u];
+  S.ig_ptr = cnt;
+  S.ig_sub.assign(cnt[S.nig], -1);
+  S.ig_k.assign(cnt[S.nig], -1);
+  // rank through the node-to-subdomain lists (every holder of a node holds
+  // all of its dofs); checked below, with a serial fill as the fallback
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
+  for (int is = 0; is < S.nsub; is++) {
+    const Sub &s = S.subs[is];
+    for (size_t i = 0; i < s.iin.size(); i++) {
+      const int g = s.isngn[s.iin[i]];
+      int rank = -1;
+      for (int q = S.n2s_ptr[g]; q < S.n2s_ptr[g + 1]; q++)
+        if (S.n2s_idx[q] == is) rank = q - S.n2s_ptr[g];
+      for (int k = s.kdofi[i]; k < s.kdofi[i + 1]; k++) {
+        const int u = S.gpos[s.isvgvn[s.ifc_dof[k]]];
+        const int q = S.ig_ptr[u] + rank;
+        if (rank < 0 || q >= S.ig_ptr[u + 1]) {
+          bad = 1;
+          continue;
+        }
+        S.ig_sub[q] = is;
+        S.ig_k[q] = k;
+      }
+    }
+  }
+  for (size_t q = 0; q < S.ig_sub.size() && !bad; q++)
+    if (S.ig_sub[q] < 0) bad = 1;
+  if (bad) {
+    std::vector<int> fill(S.ig_ptr.begin(), S.ig_ptr.end() - 1);
+    for (int is = 0; is < S.nsub; is++) {
+      const Sub &s = S.subs[is];
+      for (size_t k = 0; k < s.ifc_dof.size(); k++) {
+        const int q = fill[S.gpos[s.isvgvn[s.ifc_dof[k]]]]++;
+        S.ig_sub[q] = is;
+        S.ig_k[q] = (int)k;
+      }
+    }
+  }
 }
 
 // ------------------------------------------------------------- orderings

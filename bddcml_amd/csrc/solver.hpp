@@ -52,6 +52,9 @@ struct Solver {
   std::vector<int> igdof;  // sorted global dofs
   std::vector<int> gpos;   // global dof -> unique position or -1
   std::vector<char> ig_fixed;
+  // every copy of each unique interface dof: (subdomain, index in its
+  // interface list), subdomains ascending (CSR over unique positions)
+  std::vector<int> ig_ptr, ig_sub, ig_k;
   // state
   bool is_setup = false, solved = false, plan_only = false, reactions_ready = false;
   // the last solution (local vectors and interface) is in device memory

@@ -232,7 +232,7 @@ struct Device {
   std::vector<int> perm_off;            // offset of each subdomain's permutation in sc_perm
   View<char> sc_fix;
   View<ScatterSub> sc_subs;
-  View<int2> zero_rng;
+  View<int> clear_ids;  // subdomain tiles cleared before the scatter
   View<int2> c_zero;                    // coarse tiles cleared before assembly
   View<int64_t> c_pads;                 // padding diagonals of the coarse matrix
   View<MirrorJob> mir_c;
@@ -494,7 +494,7 @@ static TaskList materialize(const std::vector<Q> &sts, Get get, const std::vecto
     const int b = base_of(is), po = (int)poff[cls[is]];
     TileTask *o = out.tasks.data() + toff[is];
     for (const TileTask &t : L->tasks)
-      *o++ = TileTask{t.c + b, t.x >= 0 ? t.x + b : -1, t.p0 + po, t.p1 + po, b};
+      *o++ = TileTask{t.c + b, t.x >= 0 ? t.x + b : -1, t.p0 + po, t.p1 + po, b, t.fresh};
   }
   return out;
 }
@@ -612,6 +612,7 @@ void device_setup_begin(Solver &S) {
   struct SubTasks {
     std::vector<TaskList> a_diag, a_off;
     TaskList a_schur;
+    std::vector<int> clear;  // pattern tiles (relative) that start as zeros
   };
   std::vector<SubTasks> sts(ncls);
   int bad_ids = 0;
@@ -670,6 +671,40 @@ void device_setup_begin(Solver &S) {
         t.p1 = (int)Q.a_schur.pairs.size();
         if (t.p1 > t.p0) Q.a_schur.tasks.push_back(t);
       }
+    // Tiles without matrix entries whose task writes them whole are
+    // "fresh": the task starts from zero and the tile is never cleared.
+    // Entry tiles (and the padding diagonals) are cleared and scattered;
+    // tiles no task writes are cleared.
+    {
+      const Sub &s = S.subs[reps[ic]];
+      const int npat = (int)T.pat.tile.size();
+      std::vector<char> entry(npat, 0), written(npat, 0);
+      auto mark = [&](int a, int b) {
+        const int I = std::max(a, b) / TILE, J = std::min(a, b) / TILE;
+        const int q = T.pat.find(I, J) - T.base;
+        if (q < 0 || q >= npat) bad_ids |= 1;
+        else entry[q] = 1;
+      };
+      for (size_t k = 0; k < s.ai.size(); k++) mark(s.perm[s.ai[k]], s.perm[s.aj[k]]);
+      for (int p : s.pad) mark(p, p);
+      for (int p = s.NO + s.ni; p < s.NO + s.NI; p++) mark(p, p);
+      auto visit = [&](TaskList &L) {
+        for (TileTask &t : L.tasks) {
+          const int q = t.c - T.base;
+          if (q < 0 || q >= npat) {
+            bad_ids |= 1;
+            continue;
+          }
+          written[q] = 1;
+          t.fresh = entry[q] ? 0 : 1;
+        }
+      };
+      for (auto *v : {&Q.a_diag, &Q.a_off})
+        for (TaskList &L : *v) visit(L);
+      visit(Q.a_schur);
+      for (int q = 0; q < npat; q++)
+        if (entry[q] || !written[q]) Q.clear.push_back(q);
+    }
     for (auto *v : {&Q.a_diag, &Q.a_off})
       for (TaskList &L : *v)
         if (!relative_ids(L, T.base, T.span)) bad_ids |= 1;
@@ -687,6 +722,20 @@ void device_setup_begin(Solver &S) {
                         a_off = merge_steps(sts, &SubTasks::a_off, cls, base_of);
   TaskList a_schur = materialize(
       sts, [](const SubTasks &Q) -> const TaskList * { return &Q.a_schur; }, cls, base_of);
+  {
+    // tiles to clear before the scatter, per subdomain
+    std::vector<size_t> off(nsub + 1, 0);
+    for (int is = 0; is < nsub; is++) off[is + 1] = off[is] + sts[cls[is]].clear.size();
+    std::vector<int> ids(off[nsub]);
+#pragma omp parallel for schedule(static)
+    for (int is = 0; is < nsub; is++) {
+      const std::vector<int> &c = sts[cls[is]].clear;
+      for (size_t k = 0; k < c.size(); k++) ids[off[is] + k] = D.sub[is].base + c[k];
+    }
+    for (int id : ids)
+      if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid tile to clear");
+    D.plan.add(D.clear_ids, ids);
+  }
   sts.clear();
   if (S.verbose >= 3) ph("  materialized");
   if (S.verbose >= 2)
@@ -721,7 +770,6 @@ void device_setup_begin(Solver &S) {
     std::vector<int> perm(poff[nrep]), ttab(toff[nrep], -1), pads(doff[nrep]);
     std::vector<char> fix(foff[nsub]);
     std::vector<ScatterSub> ss(nsub);
-    std::vector<int2> ranges(nsub);
     if (S.verbose >= 3) ph("  scatter allocation");
     int bad = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad)
@@ -762,14 +810,11 @@ void device_setup_begin(Solver &S) {
       if (T.base != T.pat.tile.front() || s.ndofs != S.subs[oreps[c]].ndofs) bad |= 4;
       if ((int64_t)T.base + (int64_t)T.pat.tile.size() > D.ntiles) bad |= 4;
       for (int d = 0; d < s.ndofs; d++) fix[foff[is] + d] = s.ifix[d] != 0;
-      ranges[is] = make_int2(T.pat.tile.front(), (int)T.pat.tile.size());
     }
     if (S.verbose >= 3) ph("  scatter tables");
     if (bad & 1) fatal("BDDCML_DEVICE_PLAN", "device triplets out of date");
     if (bad & 2) fatal("BDDCML_DEVICE_PLAN", "invalid permutation");
     if (bad & 4) fatal("BDDCML_DEVICE_PLAN", "invalid subdomain tile range");
-    for (auto &r : ranges)
-      if (r.x < 0 || (int64_t)r.x + r.y > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid tile range to clear");
     for (int k = 0; k < nrep; k++) {
       const int span = (int)D.sub[oreps[k]].pat.tile.size();
       for (size_t q = toff[k]; q < toff[k + 1]; q++)
@@ -783,7 +828,6 @@ void device_setup_begin(Solver &S) {
     D.plan.add(D.sc_ttab, ttab);
     D.plan.add(D.sc_pads, pads);
     D.plan.add(D.sc_subs, ss);
-    D.plan.add(D.zero_rng, ranges);
   }
 
   ph("scatter entries");
@@ -828,7 +872,7 @@ void device_setup_begin(Solver &S) {
   for (int k = 0; k < 5; k++) HIPCHECK(hipEventCreate(&ev[k]));
   HIPCHECK(hipEventCreateWithFlags(&D.ev_up, hipEventDisableTiming));
   HIPCHECK(hipEventRecord(ev[0], st));
-  launch_zero_ranges(st, pool, D.zero_rng.p, (int)D.zero_rng.n);
+  launch_zero_tiles(st, pool, D.clear_ids.p, (int)D.clear_ids.n);
   launch_scatter_sub(st, pool, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p,
                      D.sc_pads.p, D.err.p);
   for (size_t J = 0; J < D.a_diag.size(); J++) {

@@ -223,7 +223,7 @@ __device__ __forceinline__ void tile_gemm_task(T *__restrict__ pool, const TileT
   zero_acc<T>(acc);
   gemm_sum<T>(pool, pairs, t.p0, t.p1, t.base, kind, As, Bs, acc);
   T *C = pool + (size_t)t.c * TILE_ELEMS;
-  form_e<T>(C, alpha, beta, acc, As);  // E in As
+  form_e<T>(C, t.fresh ? 0.0 : alpha, beta, acc, As);  // E in As
   if (post == POST_NONE) {
     lds_to_tile(As, C);
     return;
@@ -375,7 +375,7 @@ __global__ __launch_bounds__(NTHR) void k_tile_potrf(T *__restrict__ pool,
   zero_acc<T>(acc);
   gemm_sum<T>(pool, pairs, t.p0, t.p1, t.base, GEMM_NT, As, Bs, acc);
   T *C = pool + (size_t)t.c * TILE_ELEMS;
-  form_e<T>(C, 1.0, -1.0, acc, As);  // ends with a barrier
+  form_e<T>(C, t.fresh ? 0.0 : 1.0, -1.0, acc, As);  // ends with a barrier
   for (int e = tid; e < 64 * LDL; e += NTHR) Bs[e] = (T)0;
   if (tid == 0) bad = 0;
   __syncthreads();

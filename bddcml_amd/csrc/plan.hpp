@@ -93,10 +93,12 @@ struct Sub {
 // post multiply by tile X, optional Cholesky.  Pair tile ids are relative to
 // base, so subdomains of identical structure share one pair list.
 struct TileTask {
-  int c;         // output tile id
-  int x;         // post-multiply / inverse tile id (-1 none)
-  int p0, p1;    // pair range
-  int base = 0;  // added to the tile ids of the pairs
+  int c;          // output tile id
+  int x;          // post-multiply / inverse tile id (-1 none)
+  int p0, p1;     // pair range
+  int base = 0;   // added to the tile ids of the pairs
+  int fresh = 0;  // 1: the output tile holds no matrix entries; the task
+                  // starts from zero and never reads it (it is not cleared)
 };
 struct TilePair {
   int a, b;

@@ -35,7 +35,7 @@ EXTRA_SYMBOLS = [
     "bddcml_amd_residual_history", "bddcml_amd_timings", "bddcml_amd_set_option",
     "bddcml_amd_sub_info", "bddcml_amd_sub_maps", "bddcml_amd_sub_matrix",
     "bddcml_amd_sub_coarse", "bddcml_amd_last_error", "bddcml_amd_selftest_tile",
-    "bddcml_amd_selftest_copy", "bddcml_amd_element_gradient",
+    "bddcml_amd_selftest_copy", "bddcml_amd_element_gradient", "bddcml_amd_recycling_basis",
 ]
 
 
@@ -283,6 +283,12 @@ class Bddcml:
         if self.L.bddcml_amd_interface_solution(u.ctypes.data_as(C.POINTER(C.c_double))):
             raise RuntimeError("no solution")
         return u
+
+    def recycling_basis(self):
+        """(number of stored recycling directions, ||V^T W - I||_F)"""
+        n, orth = C.c_int(), C.c_double()
+        self.L.bddcml_amd_recycling_basis(C.byref(n), C.byref(orth))
+        return n.value, orth.value
 
     def residual_history(self):
         n = self.L.bddcml_amd_residual_history(None, 0)

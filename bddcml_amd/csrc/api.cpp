@@ -279,25 +279,25 @@ void bddcml_solve_c(int *comm_all, int *method, double *tol, int *maxit, int *nd
                     int *recycling_int, int *max_number_of_stored_vectors, int *num_iter,
                     int *converged_reason, double *condition_number) try {
   (void)comm_all;
-  (void)max_number_of_stored_vectors;
   Solver &S = solver();
   if (!S.is_setup) fatal("BDDCML_SOLVE", "preconditioner not set up");
+  // defaults of src/module_bddcml.f90:515-520
   int m = 1;
   double tl = 1e-6;
-  int mi = 1000, nd = 30;
+  int mi = 1000, nd = 30, maxvec = 100;
+  bool rec = false;
   if (*method != -1) {
     m = *method;
     tl = *tol;
     mi = *maxit;
     nd = *ndecrmax;
-    if (*recycling_int) {
-      // Krylov recycling is not implemented: the solve proceeds without it
-      if (S.verbose > 0) std::fprintf(stderr, "WARNING in BDDCML_SOLVE: recycling ignored\n");
-    }
+    rec = *recycling_int != 0;
+    maxvec = *max_number_of_stored_vectors;
   }
   if (m != 0 && m != 1) fatal("BDDCML_SOLVE", "unknown iterative method " + std::to_string(m));
+  if (rec && maxvec < 0) fatal("BDDCML_SOLVE", "negative number of stored vectors");
   double t0 = wall_time();
-  device_solve(S, m, tl, mi, nd, num_iter, converged_reason, condition_number);
+  device_solve(S, m, tl, mi, nd, rec, maxvec, num_iter, converged_reason, condition_number);
   S.t.solve_total = wall_time() - t0;
   S.last_iters = *num_iter;
   S.last_reason = *converged_reason;
@@ -488,6 +488,16 @@ int bddcml_amd_reduced_rhs(double *g) try {
 catch (const Error &e) {
   record_error(e); return -100;
 }
+int bddcml_amd_recycling_basis(int *nvec, double *orth) try {
+  Solver &S = solver();
+  if (!S.is_setup || S.plan_only || !S.dev) return -1;
+  device_recycling_check(S, nvec, orth);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e); return -100;
+}
+
 int bddcml_amd_interface_solution(double *u) try {
   Solver &S = solver();
   if (!S.solved) return -1;

@@ -278,6 +278,8 @@ struct Device {
   // vectors
   DBuf<double> yl, zl, rcs, rc, uc, partial, dots;
   DBuf<double> pcg;           // PCG scalars per iteration: r.z, p.Ap, r.r
+  DBuf<double> recV, recW;    // recycling basis V and W = S V (columns of length nig)
+  DBuf<double> recc, recpart; // its projection coefficients and partial sums
   double *hrr = nullptr;      // pinned host copies of r.r per iteration
   size_t hrr_n = 0;
   DBuf<double> u, r, p, z, ap, rs, v, y, s, t, g, xin, xout;
@@ -317,6 +319,7 @@ void device_release_input(Solver &S) { S.din.reset(); }
 
 void device_release(Solver &S) {
   S.sol_on_device = false;
+  S.rec_n = S.rec_max = 0;  // stored directions belong to the released operator
   if (S.dev) {
     Device &D = *S.dev;
     if (D.st) (void)hipStreamSynchronize(D.st);
@@ -1783,6 +1786,26 @@ static void dots(Solver &S, std::initializer_list<std::pair<const double *, cons
   HIPCHECK(hipStreamSynchronize(D.st));
 }
 
+void device_recycling_check(Solver &S, int *nvec, double *orth) {
+  *nvec = S.rec_n;
+  *orth = 0.0;
+  if (!S.dev || S.rec_n == 0) return;
+  Device &D = *S.dev;
+  const int n = S.nig, m = S.rec_n;
+  std::vector<double> c(m);
+  double f = 0.0;
+  for (int j = 0; j < m; j++) {
+    launch_multidot(D.st, n, D.recV.p, n, m, D.recW.p + (size_t)j * n, D.recpart.p, D.recc.p);
+    HIPCHECK(hipMemcpyAsync(c.data(), D.recc.p, m * sizeof(double), hipMemcpyDeviceToHost, D.st));
+    HIPCHECK(hipStreamSynchronize(D.st));
+    for (int i = 0; i < m; i++) {
+      const double e = c[i] - (i == j ? 1.0 : 0.0);
+      f += e * e;
+    }
+  }
+  *orth = std::sqrt(f);
+}
+
 void device_apply_schur(Solver &S, const double *x, double *y) {
   Device &D = *S.dev;
   HIPCHECK(hipMemcpy(D.xin.p, x, S.nig * sizeof(double), hipMemcpyHostToDevice));
@@ -1880,8 +1903,8 @@ static void initial_guess(Solver &S) {
   HIPCHECK(hipMemcpyAsync(D.u.p, u0.data(), S.nig * sizeof(double), hipMemcpyHostToDevice, D.st));
 }
 
-void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, int *num_iter,
-                  int *reason, double *cond) {
+void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bool recycling,
+                  int maxvec, int *num_iter, int *reason, double *cond) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
   const int n = S.nig;
@@ -1903,6 +1926,36 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, in
     dots(S, {{D.r.p, D.r.p}}, sc);
     double normres0 = std::sqrt(sc[0]);
     if (normres0 == 0.0) goto finish;
+    // Krylov recycling (src/module_krylov.f90:186-216, 308-455, 547-613,
+    // 663-726, 1859-2036): the kept directions V (A-orthonormal, W = S V)
+    // correct the initial guess by V V^T r0, every new direction is
+    // A-orthogonalized against them by two classical Gram-Schmidt passes,
+    // and new directions are stored while there is room
+    const bool rec = recycling && maxvec > 0 && method == 0;
+    if (rec && S.rec_max == 0) {
+      S.rec_max = maxvec;
+      S.rec_n = 0;
+      D.recV.alloc((size_t)n * maxvec);
+      D.recW.alloc((size_t)n * maxvec);
+      D.recc.alloc(maxvec);
+      D.recpart.alloc((size_t)DOT_BLOCKS * maxvec);
+    }
+    if (rec && S.rec_n > 0) {
+      launch_multidot(st, n, D.recV.p, n, S.rec_n, D.r.p, D.recpart.p, D.recc.p);
+      HIPCHECK(hipMemsetAsync(D.z.p, 0, n * sizeof(double), st));
+      launch_basis_update(st, n, D.recV.p, n, S.rec_n, D.recc.p, 1.0, D.z.p);
+      launch_axpy2(st, n, 1.0, D.z.p, D.u.p, 0.0, nullptr, nullptr);
+      op_schur(S, D.z.p, D.ap.p);
+      launch_axpy2(st, n, -1.0, D.ap.p, D.r.p, 0.0, nullptr, nullptr);
+      launch_mask(st, n, D.fixmask.p, D.r.p);
+      dots(S, {{D.r.p, D.r.p}}, sc);
+      if (std::sqrt(sc[0]) / normres0 < tol) {
+        *num_iter = 0;
+        *reason = 0;
+        *cond = 1.0;
+        goto finish;
+      }
+    }
     op_pc(S, D.r.p, D.p.p);
     // scalars stay on the device: rz[k] = r.z, pap[k] = p.Ap, rr[k] = r.r
     // of iteration k; the host reads only rr[k], and while it waits the
@@ -1922,8 +1975,18 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, in
     int ndecr = 0, nw = 0, last = 0;
     double lastres = 1.0;
     for (int iter = 1; iter <= maxit; iter++) {
+      if (rec && S.rec_n > 0)
+        for (int pass = 0; pass < 2; pass++) {
+          launch_multidot(st, n, D.recW.p, n, S.rec_n, D.p.p, D.recpart.p, D.recc.p);
+          launch_basis_update(st, n, D.recV.p, n, S.rec_n, D.recc.p, -1.0, D.p.p);
+        }
       op_schur(S, D.p.p, D.ap.p, iter == 2);
       dots_dev(S, D.p.p, D.ap.p, pap + iter);
+      if (rec && S.rec_n < S.rec_max) {
+        launch_basis_store(st, n, pap + iter, D.p.p, D.ap.p, D.recV.p + (size_t)S.rec_n * n,
+                           D.recW.p + (size_t)S.rec_n * n);
+        S.rec_n++;
+      }
       launch_pcg_update(st, n, rz, pap, iter, D.p.p, D.ap.p, D.u.p, D.r.p);
       dots_dev(S, D.r.p, D.r.p, rr + iter);
       HIPCHECK(hipMemcpyAsync(D.hrr + iter, rr + iter, sizeof(double), hipMemcpyDeviceToHost, st));

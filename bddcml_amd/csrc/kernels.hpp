@@ -201,6 +201,17 @@ void launch_pcg_update(hipStream_t st, int n, const double *rz, const double *pa
                        const double *p, const double *ap, double *u, double *r);
 // p = z + beta p with beta = rz[k] / rz[k-1]
 void launch_pcg_dir(hipStream_t st, int n, const double *rz, int k, const double *z, double *p);
+// Krylov recycling basis (columns of length n, leading dimension ld):
+// out[j] = V(:, j) . x for j < m (deterministic two-stage sums; partial
+// holds DOT_BLOCKS * m doubles)
+void launch_multidot(hipStream_t st, int n, const double *V, int64_t ld, int m, const double *x,
+                     double *partial, double *out);
+// y += sign * V(:, 0:m) c, c in device memory
+void launch_basis_update(hipStream_t st, int n, const double *V, int64_t ld, int m,
+                         const double *c, double sign, double *y);
+// V(:, j) = p / sqrt(*pap), W(:, j) = ap / sqrt(*pap)
+void launch_basis_store(hipStream_t st, int n, const double *pap, const double *p,
+                        const double *ap, double *vcol, double *wcol);
 // y = x + b*(y - w*v)   (BiCGSTAB direction update)
 void launch_bicg_p(hipStream_t st, int n, double b, double w, const double *r, const double *v,
                    double *p);

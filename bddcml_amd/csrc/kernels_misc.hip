@@ -746,6 +746,71 @@ void launch_pcg_dir(hipStream_t st, int n, const double *rz, int k, const double
   if (n <= 0) return;
   hipLaunchKernelGGL(k_pcg_dir, dim3((n + 255) / 256), dim3(256), 0, st, n, rz, k, z, p);
 }
+
+// ------------------------------------------------ Krylov recycling basis
+__global__ __launch_bounds__(256) void k_multidot(int n, const double *__restrict__ V, int64_t ld,
+                                                  const double *__restrict__ x,
+                                                  double *__restrict__ partial) {
+  __shared__ double red[256];
+  const double *v = V + (size_t)blockIdx.y * ld;
+  double s = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += v[i] * x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(256) void k_multidot_final(int nb, const double *__restrict__ partial,
+                                                        double *__restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += partial[(size_t)blockIdx.x * nb + b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+void launch_multidot(hipStream_t st, int n, const double *V, int64_t ld, int m, const double *x,
+                     double *partial, double *out) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(k_multidot, dim3(DOT_BLOCKS, m), dim3(256), 0, st, n, V, ld, x, partial);
+  hipLaunchKernelGGL(k_multidot_final, dim3(m), dim3(256), 0, st, DOT_BLOCKS, partial, out);
+}
+__global__ void k_basis_update(int n, const double *__restrict__ V, int64_t ld, int m,
+                               const double *__restrict__ c, double sign, double *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < m; j++) s += V[(size_t)j * ld + i] * c[j];
+  y[i] += sign * s;
+}
+void launch_basis_update(hipStream_t st, int n, const double *V, int64_t ld, int m,
+                         const double *c, double sign, double *y) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(k_basis_update, dim3((n + 255) / 256), dim3(256), 0, st, n, V, ld, m, c, sign,
+                     y);
+}
+__global__ void k_basis_store(int n, const double *__restrict__ pap, const double *__restrict__ p,
+                              const double *__restrict__ ap, double *__restrict__ vcol,
+                              double *__restrict__ wcol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double sc = 1.0 / sqrt(*pap);
+  vcol[i] = sc * p[i];
+  wcol[i] = sc * ap[i];
+}
+void launch_basis_store(hipStream_t st, int n, const double *pap, const double *p,
+                        const double *ap, double *vcol, double *wcol) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_basis_store, dim3((n + 255) / 256), dim3(256), 0, st, n, pap, p, ap, vcol,
+                     wcol);
+}
 __global__ void k_bicg_p(int n, double b, double w, const double *__restrict__ r,
                          const double *__restrict__ v, double *__restrict__ p) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;

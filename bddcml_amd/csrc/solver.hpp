@@ -60,6 +60,9 @@ struct Solver {
   // the last solution (local vectors and interface) is in device memory
   // only; device_fetch_solution brings it to subs[*].sol and uig
   bool sol_on_device = false;
+  // Krylov recycling: A-orthonormal directions kept across solves on the
+  // same factorization (count in use, capacity fixed by the first request)
+  int rec_n = 0, rec_max = 0;
   std::vector<double> uig;  // interface solution
   std::unique_ptr<Device> dev;
   std::unique_ptr<DeviceInput> din;
@@ -102,8 +105,10 @@ void device_setup_begin(Solver &S);
 void device_setup_finish(Solver &S);
 void device_reduced_rhs(Solver &S);
 void device_upload_rhs(Solver &S);  // re-upload the condensation rhs of all subdomains
-void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, int *num_iter,
-                  int *reason, double *cond);
+void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bool recycling,
+                  int maxvec, int *num_iter, int *reason, double *cond);
+// stored recycling directions and ||V^T W - I||_F of the basis
+void device_recycling_check(Solver &S, int *n, double *orth);
 void device_apply_schur(Solver &S, const double *x, double *y);
 void device_apply_pc(Solver &S, const double *r, double *z);
 void device_reduced_rhs_download(Solver &S, double *g);

@@ -86,6 +86,12 @@ int bddcml_amd_selftest_tile(int op, int npairs, const double *A, const double *
  * nelem entries.  Needs isegn at upload. */
 int bddcml_amd_element_gradient(const double *ae, int nke, int per_element, const double *u,
                                 const double *lam, double *grad);
+/* Krylov recycling (bddcml_solve_c with recycling_int = 1, PCG): number
+ * of directions kept for later solves on the same factorization and
+ * ||V^T W - I||_F of the stored basis (V A-orthonormal, W = S V), the
+ * invariant of src/module_krylov.f90:346-367; the basis is dropped by a new
+ * setup or bddcml_finalize_c */
+int bddcml_amd_recycling_basis(int *nvec, double *orth);
 /* plain device copy of nbytes (multiple of 16) through kernel
  * k_copy_calibration, timed with events: the reference point for HBM
  * counters and bandwidth; out[0] = milliseconds of the copy */

@@ -436,7 +436,8 @@ static void dense_chol_tasks(int n, FP Ptile, FL Linv, std::vector<TaskList> &di
 static void check_tasks(const TaskList &l, int64_t ntiles, bool uses_x, const char *what) {
   auto bad = [&](int id) { return id < 0 || id >= ntiles; };
   for (const TileTask &t : l.tasks) {
-    if (bad(t.c) || (uses_x && bad(t.x)) || t.p0 < 0 || t.p1 < t.p0 || t.p1 > (int)l.pairs.size())
+    const bool need_x = uses_x || (t.mode >= 0 && ((t.mode >> 2) & 3) != 0);
+    if (bad(t.c) || (need_x && bad(t.x)) || t.p0 < 0 || t.p1 < t.p0 || t.p1 > (int)l.pairs.size())
       fatal("BDDCML_DEVICE_PLAN", std::string("invalid task in ") + what);
   }
   for (const TilePair &p : l.pairs)
@@ -497,7 +498,7 @@ static TaskList materialize(const std::vector<Q> &sts, Get get, const std::vecto
     const int b = base_of(is), po = (int)poff[cls[is]];
     TileTask *o = out.tasks.data() + toff[is];
     for (const TileTask &t : L->tasks)
-      *o++ = TileTask{t.c + b, t.x >= 0 ? t.x + b : -1, t.p0 + po, t.p1 + po, b, t.fresh};
+      *o++ = TileTask{t.c + b, t.x >= 0 ? t.x + b : -1, t.p0 + po, t.p1 + po, b, t.fresh, t.mode};
   }
   return out;
 }
@@ -1020,39 +1021,40 @@ void device_setup_finish(Solver &S) {
     const int n = D.ntc, cb = D.cbase, cl = D.clinv, cw = D.cwbase;
     auto Pc = [=](int I, int J) { return cb + lowidx(n, I, J); };
     auto Wc = [=](int I, int J) { return I == J ? cl + I : cw + lowidx(n - 1, I - 1, J); };
-    auto one = [](TaskList &l, int c, int a, int b) {
+    // per column k three launches: the diagonal factor; the panel
+    // L_ik = A_ik Linv_k^T together with the W row W_kj = Linv_k B_kj; the
+    // trailing updates A_IK -= L_Ik L_Kk^T together with B_ij -= L_ik W_kj
+    // (tasks of one launch carry their own product kind)
+    const int panel = task_mode(GEMM_NT, POST_RIGHT_XT, false),
+              wrow = task_mode(GEMM_NN, POST_LEFT_X, false),
+              trail = task_mode(GEMM_NT, POST_NONE, true), bupd = task_mode(GEMM_NN, POST_NONE, true);
+    auto one = [](TaskList &l, int c, int a, int b, int mode) {
       const int p0 = (int)l.pairs.size();
       l.pairs.push_back({a, b});
-      l.tasks.push_back({c, -1, p0, p0 + 1});
+      TileTask t{c, -1, p0, p0 + 1};
+      t.mode = mode;
+      l.tasks.push_back(t);
+    };
+    auto post = [](TaskList &l, int c, int x, int mode) {
+      TileTask t{c, x, 0, 0};
+      t.mode = mode;
+      l.tasks.push_back(t);
     };
     for (int k = 0; k < n; k++) {
       SeqLaunch f;  // L_kk = chol(A_kk), Linv_k
       f.potrf = true;
       f.l.tasks.push_back({Pc(k, k), cl + k, 0, 0});
       cseq.push_back(std::move(f));
-      if (k + 1 < n) {  // L_ik = A_ik Linv_k^T
-        SeqLaunch pn;
-        pn.post = POST_RIGHT_XT, pn.beta = 0.0;
-        for (int i = k + 1; i < n; i++) pn.l.tasks.push_back({Pc(i, k), cl + k, 0, 0});
-        cseq.push_back(std::move(pn));
-      }
-      if (k > 0) {  // W_kj = Linv_k B_kj
-        SeqLaunch wr;
-        wr.kind = GEMM_NN, wr.post = POST_LEFT_X, wr.beta = 0.0;
-        for (int j = 0; j < k; j++) wr.l.tasks.push_back({Wc(k, j), cl + k, 0, 0});
-        cseq.push_back(std::move(wr));
-      }
-      if (k + 1 < n) {
-        SeqLaunch tr;  // A_IK -= L_Ik L_Kk^T
-        for (int K = k + 1; K < n; K++)
-          for (int I = K; I < n; I++) one(tr.l, Pc(I, K), Pc(I, k), Pc(K, k));
-        cseq.push_back(std::move(tr));
-        SeqLaunch bu;  // B_ij -= L_ik W_kj
-        bu.kind = GEMM_NN;
-        for (int i = k + 1; i < n; i++)
-          for (int j = 0; j <= k; j++) one(bu.l, Wc(i, j), Pc(i, k), Wc(k, j));
-        cseq.push_back(std::move(bu));
-      }
+      SeqLaunch pw;
+      for (int i = k + 1; i < n; i++) post(pw.l, Pc(i, k), cl + k, panel);
+      for (int j = 0; j < k; j++) post(pw.l, Wc(k, j), cl + k, wrow);
+      if (!pw.l.tasks.empty()) cseq.push_back(std::move(pw));
+      SeqLaunch up;
+      for (int K = k + 1; K < n; K++)
+        for (int I = K; I < n; I++) one(up.l, Pc(I, K), Pc(I, k), Pc(K, k), trail);
+      for (int i = k + 1; i < n; i++)
+        for (int j = 0; j <= k; j++) one(up.l, Wc(i, j), Pc(i, k), Wc(k, j), bupd);
+      if (!up.l.tasks.empty()) cseq.push_back(std::move(up));
     }
     for (int j = 0; j < n; j++)
       for (int i = j; i < n; i++) {

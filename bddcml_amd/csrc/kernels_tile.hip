@@ -271,7 +271,11 @@ __global__ __launch_bounds__(NTHR) void k_tile_gemm(T *__restrict__ pool,
                                                     const TilePair *__restrict__ pairs, int kind,
                                                     int post, double alpha, double beta,
                                                     int chunked) {
-  tile_gemm_task<T>(pool, tasks[task_index(chunked)], pairs, kind, post, alpha, beta);
+  const TileTask t = tasks[task_index(chunked)];
+  if (t.mode >= 0)
+    tile_gemm_task<T>(pool, t, pairs, t.mode & 3, (t.mode >> 2) & 3, 1.0, (t.mode & 16) ? -1.0 : 0.0);
+  else
+    tile_gemm_task<T>(pool, t, pairs, kind, post, alpha, beta);
 }
 
 // the Schur-complement block update S = A22 - sum W^T W in a launch of its

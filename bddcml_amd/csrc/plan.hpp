@@ -99,7 +99,12 @@ struct TileTask {
   int base = 0;   // added to the tile ids of the pairs
   int fresh = 0;  // 1: the output tile holds no matrix entries; the task
                   // starts from zero and never reads it (it is not cleared)
+  int mode = -1;  // >= 0: this task's product kind, post step and beta
+                  // instead of the launch's (task_mode), so that tasks of
+                  // different kinds share a launch
 };
+// per-task mode: kind | post << 2 | (beta == -1) << 4; alpha is 1
+constexpr int task_mode(int kind, int post, bool minus) { return kind | post << 2 | (minus ? 16 : 0); }
 struct TilePair {
   int a, b;
 };

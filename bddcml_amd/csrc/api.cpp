@@ -51,11 +51,14 @@ void bddcml_init_c(int *nl, int *nsublev, int *lnsublev, int *nsub_loc_1, int *c
   set_task_order(1);
   Solver &S = solver();
   if (*nl < 2) fatal("LEVELS_INIT", "Number of levels must be at least 2");
-  if (*nl != 2) fatal("LEVELS_INIT", "This library implements two levels");
+  if (*nl > 3) fatal("LEVELS_INIT", "This library implements two and three levels");
   if (*lnsublev < *nl) fatal("LEVELS_INIT", "lnsublev smaller than number of levels");
   if (nsublev[*nl - 1] != 1) fatal("LEVELS_INIT", "Number of subdomains at last level must be 1.");
+  if (*nl == 3 && (nsublev[1] < 1 || nsublev[1] > nsublev[0]))
+    fatal("LEVELS_INIT", "Number of subdomains on level 2 must be between 1 and that of level 1");
   S.nl = *nl;
   S.nsub = nsublev[0];
+  S.nsub2 = *nl == 3 ? nsublev[1] : 0;
   S.numbase = *numbase;
   S.verbose = *verbose_level;
   S.subs.assign(S.nsub, Sub());
@@ -262,6 +265,12 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
     return;
   }
   device_setup_finish(S);
+  // three levels: the second level from the coarse nodes and matrices
+  // (with one second-level subdomain the coarse problem is solved directly)
+  if (S.nl == 3 && S.nsub2 >= 2) {
+    setup_second_level(S);
+    phase("second level");
+  }
   double t1 = wall_time();
   device_reduced_rhs(S);
   S.t.rhs += wall_time() - t1;
@@ -439,6 +448,24 @@ catch (const Error &e) {
 }
 
 // ------------------------------------------------------------- extra ABI
+int bddcml_amd_level2_counts(int *out) try {
+  Solver &S = solver();
+  if (!S.is_setup || !S.lev2) return -1;
+  const Solver &T = *S.lev2;
+  out[0] = T.nsub;
+  out[1] = T.ndof_g;
+  out[2] = T.ncorner;
+  out[3] = T.nedge;
+  out[4] = T.nface;
+  out[5] = T.ndofc;
+  out[6] = T.nig;
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e);
+  return -1;
+}
+
 int bddcml_amd_counts(int *out) try {
   Solver &S = solver();
   if (!S.is_setup) return -1;

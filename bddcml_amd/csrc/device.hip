@@ -209,6 +209,7 @@ int TilePattern::find(int I, int J) const {
 
 struct Device {
   hipStream_t st = nullptr;  // compute stream
+  bool borrowed_st = false;  // st belongs to the level above (second level)
   hipStream_t up = nullptr;  // uploads of the second setup half
   hipEvent_t ev[5] = {};     // setup phase boundaries (timing)
   hipEvent_t ev_up = nullptr;
@@ -285,7 +286,18 @@ struct Device {
   DBuf<double> u, r, p, z, ap, rs, v, y, s, t, g, xin, xout;
   View<char> fixmask;
   int nig = 0;
+  // three levels: no dense coarse matrix here; maps to the second level's
+  // device-ordered vectors (source coarse dof and weight of each entry, and
+  // one entry of every coarse dof)
+  bool dense_coarse = true;
+  DBuf<int> l2src, l2pos;
+  DBuf<double> l2w;
+  int64_t l2n = 0;
 };
+
+// a second level replaces the dense coarse solve (with one second-level
+// subdomain the coarse problem stays dense: the same coarse solve)
+static bool has_level2(const Solver &S) { return S.nl == 3 && S.nsub2 >= 2; }
 
 // Assembled subdomain triplets kept in HBM from upload time on.
 struct DeviceInput {
@@ -318,6 +330,10 @@ void device_upload_entries(Solver &S, int is) {
 void device_release_input(Solver &S) { S.din.reset(); }
 
 void device_release(Solver &S) {
+  if (S.lev2) {  // borrows this level's stream: released first
+    device_release(*S.lev2);
+    S.lev2.reset();
+  }
   S.sol_on_device = false;
   S.rec_n = S.rec_max = 0;  // stored directions belong to the released operator
   if (S.dev) {
@@ -328,7 +344,7 @@ void device_release(Solver &S) {
     for (hipEvent_t &e : D.ev)
       if (e) (void)hipEventDestroy(e), e = nullptr;
     if (D.ev_up) (void)hipEventDestroy(D.ev_up), D.ev_up = nullptr;
-    if (D.st) (void)hipStreamDestroy(D.st);
+    if (D.st && !D.borrowed_st) (void)hipStreamDestroy(D.st);
     if (D.up) (void)hipStreamDestroy(D.up);
     D.st = D.up = nullptr;
     S.dev.reset();
@@ -915,8 +931,9 @@ void device_setup_finish(Solver &S) {
     for (int is = 0; is < nsub; is++) assign_factor(D.sub[is], S.subs[is].nconstr, nf);
     if (nf >= INT32_MAX) fatal("BDDCML_DEVICE_PLAN", "too many factor tiles");
     D.ntilesf = nf;
+    D.dense_coarse = !has_level2(S);
     D.NC = std::max(TILE, (S.ndofc + TILE - 1) / TILE * TILE);
-    D.ntc = D.NC / TILE;
+    D.ntc = D.dense_coarse ? D.NC / TILE : 0;
     const int n = D.ntc;
     int next = 0;
     D.cbase = next;
@@ -932,7 +949,7 @@ void device_setup_finish(Solver &S) {
       D.pool32.alloc((size_t)std::max<int64_t>(nf, 1) * TILE_ELEMS);
     else
       D.fpool.alloc((size_t)std::max<int64_t>(nf, 1) * TILE_ELEMS);
-    D.cpool.alloc((size_t)D.ntilesc * TILE_ELEMS);
+    if (D.dense_coarse) D.cpool.alloc((size_t)D.ntilesc * TILE_ELEMS);
   }
   {
     // the factor-space lists depend only on (nti, ncT): one class per pair
@@ -1011,7 +1028,7 @@ void device_setup_finish(Solver &S) {
     upload_phases(D.z_bwd, z_bwd, D.plan2);
     D.phi.upload(phi, D.plan2);
   }
-  {
+  if (D.dense_coarse) {
     // coarse problem (one dense matrix, tiles in the coarse pool):
     // right-looking tile Cholesky and the inverse factor W = L^{-1} by
     // right-looking forward substitution, so that every task is a single tile
@@ -1211,7 +1228,8 @@ void device_setup_finish(Solver &S) {
   if ((size_t)(D.maxNI + 1024 + 64 + D.maxnc) * sizeof(double) > max_dynamic_lds())
     fatal("BDDCML_SETUP_PRECONDITIONER", "subdomain interface too large for the preconditioner apply");
   if ((int64_t)D.ntc * D.ntc > ((int64_t)1 << 24))
-    fatal("BDDCML_SETUP_PRECONDITIONER", "coarse problem too large for its dense factorization");
+    fatal("BDDCML_SETUP_PRECONDITIONER", "coarse problem too large for its dense factorization; "
+                                         "use three levels");
   D.LV = lv;
   D.plan2.add(D.desc, D.hdesc);
   D.plan2.add(D.loff, loff);
@@ -1255,9 +1273,11 @@ void device_setup_finish(Solver &S) {
     D.spart.alloc((size_t)std::max<int64_t>(ps, 1) * 64);
   }
   // coarse inverse GEMV over the full-storage tiles
-  if (D.cfull < 0 || (int64_t)D.cfull + (int64_t)D.ntc * D.ntc > D.ntilesc)
-    fatal("BDDCML_DEVICE_PLAN", "invalid coarse GEMV tiles");
-  D.cpart.alloc((size_t)D.ntc * D.ntc * 64);
+  if (D.dense_coarse) {
+    if (D.cfull < 0 || (int64_t)D.cfull + (int64_t)D.ntc * D.ntc > D.ntilesc)
+      fatal("BDDCML_DEVICE_PLAN", "invalid coarse GEMV tiles");
+    D.cpart.alloc((size_t)D.ntc * D.ntc * 64);
+  }
   D.plan2.add(D.scols, sc);
   D.plan2.add(D.srows, sr);
   D.nsrows = (int)sr.size();
@@ -1370,7 +1390,7 @@ void device_setup_finish(Solver &S) {
     D.plan2.add(D.cgsrc, src);
   }
   // coarse assembly CSR over the lower tiles of the coarse matrix
-  {
+  if (D.dense_coarse) {
     struct E {
       int64_t dst, src;
     };
@@ -1468,8 +1488,10 @@ void device_setup_finish(Solver &S) {
   };
   // coarse tiles cleared (they accumulate) and padding diagonals set early,
   // off the critical path of the coarse phase
-  launch_zero_ranges(st, D.cpool.p, D.c_zero.p, (int)D.c_zero.n);
-  launch_scatter_ones(st, D.cpool.p, D.c_pads.p, (int64_t)D.c_pads.n);
+  if (D.dense_coarse) {
+    launch_zero_ranges(st, D.cpool.p, D.c_zero.p, (int)D.c_zero.n);
+    launch_scatter_ones(st, D.cpool.p, D.c_pads.p, (int64_t)D.c_pads.n);
+  }
   if (D.mixed) factor_phase(D.pool32.p);
   else factor_phase(D.fpool.p);
   launch_g(st, D.desc.p, nsub, D.cptr.p, D.cdof.p, D.Z.p, D.G.p);
@@ -1478,19 +1500,21 @@ void device_setup_finish(Solver &S) {
   else phi_phase(D.fpool.p);
   launch_acs(st, D.desc.p, nsub, D.cptr.p, D.G.p, D.Acs.p);
   HIPCHECK(hipEventRecord(ev[3], st));
-  // coarse problem
-  double *cpool = D.cpool.p;
-  launch_coarse_assemble(st, cpool, D.ca_dst.p, D.ca_ptr.p, D.ca_src.p, D.nca, D.Acs.p);
-  for (const SeqPhase &q : D.cseq) {
-    if (q.potrf)
-      launch_tile_potrf(st, cpool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, D.err.p);
-    else
-      launch_tile_gemm(st, cpool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, q.kind, q.post, q.alpha,
-                       q.beta);
+  // coarse problem (with three levels the second level is set up instead)
+  if (D.dense_coarse) {
+    double *cpool = D.cpool.p;
+    launch_coarse_assemble(st, cpool, D.ca_dst.p, D.ca_ptr.p, D.ca_src.p, D.nca, D.Acs.p);
+    for (const SeqPhase &q : D.cseq) {
+      if (q.potrf)
+        launch_tile_potrf(st, cpool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, D.err.p);
+      else
+        launch_tile_gemm(st, cpool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, q.kind, q.post,
+                         q.alpha, q.beta);
+    }
+    launch_tile_gemm(st, cpool, D.cinv.tasks.p, D.cinv.pairs.p, D.cinv.ntask, GEMM_TN, POST_NONE,
+                     0.0, 1.0);
+    launch_tile_mirror(st, cpool, D.mir_c.p, (int)D.mir_c.n);
   }
-  launch_tile_gemm(st, cpool, D.cinv.tasks.p, D.cinv.pairs.p, D.cinv.ntask, GEMM_TN, POST_NONE, 0.0,
-                   1.0);
-  launch_tile_mirror(st, cpool, D.mir_c.p, (int)D.mir_c.n);
   HIPCHECK(hipEventRecord(ev[4], st));
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(st));
@@ -1696,6 +1720,58 @@ void device_sub_coarse(Solver &S, int is, double *phi, double *ac) {
   for (size_t k = 0; k < A.size(); k++) ac[k] = A[k];
 }
 
+void device_coarse_matrices(Solver &S, std::vector<std::vector<double>> &acs) {
+  Device &D = *S.dev;
+  std::vector<double> all(D.Acs.n);
+  HIPCHECK(hipMemcpy(all.data(), D.Acs.p, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+  acs.resize(S.nsub);
+  for (int is = 0; is < S.nsub; is++) {
+    const int64_t o = D.hdesc[is].goff, m = (int64_t)S.subs[is].nconstr * S.subs[is].nconstr;
+    if (o < 0 || o + m > (int64_t)all.size()) fatal("LEVELS_PREPARE_STANDARD_LEVEL", "coarse matrix range");
+    acs[is].assign(all.begin() + o, all.begin() + o + m);
+  }
+}
+
+// Entry p of the second level's device-ordered subdomain vectors takes the
+// level-1 coarse residual at its coarse dof, weighted at the interface (the
+// residual is a complete right-hand side: src/module_levels.f90:3914-3926,
+// src/module_dd.f90:7068-7087); pads take 0.  Each coarse dof reads its
+// solution back from one of its copies (all copies agree after the
+// interface assembly).  The second level then runs on this level's stream.
+void device_link_levels(Solver &S) {
+  const char *R = "LEVELS_PREPARE_STANDARD_LEVEL";
+  Solver &T = *S.lev2;
+  Device &D = *S.dev, &E = *T.dev;
+  const int64_t n = E.VT;
+  if (n >= INT32_MAX) fatal(R, "second level too large");
+  std::vector<int> src(n, -1), pos(std::max(S.ndofc, 1), -1);
+  std::vector<double> w(n, 0.0);
+  for (int j = 0; j < T.nsub; j++) {
+    const Sub &t = T.subs[j];
+    std::vector<double> wd(t.ndofs, 1.0);
+    for (size_t k = 0; k < t.ifc_dof.size(); k++) wd[t.ifc_dof[k]] = t.w[k];
+    for (int d = 0; d < t.ndofs; d++) {
+      const int64_t p = E.hodesc[j].voff + t.perm[d];
+      const int g = t.isvgvn[d];
+      if (p < 0 || p >= n || src[p] != -1 || g < 0 || g >= S.ndofc) fatal(R, "invalid second-level map");
+      src[p] = g;
+      w[p] = wd[d];
+      if (pos[g] < 0) pos[g] = (int)p;
+    }
+  }
+  for (int g = 0; g < S.ndofc; g++)
+    if (pos[g] < 0) fatal(R, "coarse dof missing on the second level");
+  D.l2n = n;
+  D.l2src.upload(src, D.st);
+  D.l2w.upload(w, D.st);
+  D.l2pos.upload(pos, D.st);
+  HIPCHECK(hipStreamSynchronize(D.st));
+  HIPCHECK(hipStreamSynchronize(E.st));
+  if (!E.borrowed_st) HIPCHECK(hipStreamDestroy(E.st));
+  E.st = D.st;
+  E.borrowed_st = true;
+}
+
 // --------------------------------------------------------------- operators
 static void op_schur(Solver &S, const double *x, double *y, bool timed = false) {
   Device &D = *S.dev;
@@ -1723,14 +1799,38 @@ static void op_schur(Solver &S, const double *x, double *y, bool timed = false) 
   launch_assemble(D.st, S.nig, D.aptr.p, D.asrc.p, nullptr, D.yl.p, y);
 }
 
-static void op_pc(Solver &S, const double *r, double *z, bool timed = false) {
+static void op_pc(Solver &S, const double *r, double *z, bool timed = false);
+
+// uc = one application of the second level's BDDC to the coarse residual rc
+// (src/module_levels.f90:3849-4026, 4064-4249): its subdomain right-hand
+// sides from rc, interior forward sweep and reduced interface residual,
+// preconditioner, interface values back, interior backward sweep; then the
+// coarse dofs read back.  All on this level's stream.
+static void level2_apply(Solver &S, const double *rc, double *uc) {
+  Solver &T = *S.lev2;
+  Device &D = *S.dev, &E = *T.dev;
+  hipStream_t st = D.st;
+  launch_level_fill(st, D.l2n, D.l2src.p, D.l2w.p, rc, E.vec.p);
+  launch_trsv_fwd(st, E.pool.p, E.trsv.p, T.nsub, E.rptr.p, E.rk.p, E.rt.p, E.vec.p, E.maxnt);
+  launch_assemble(st, T.nig, E.aptr.p, E.rhs_src.p, nullptr, E.vec.p, E.g.p);
+  op_pc(T, E.g.p, E.u.p);
+  launch_ifc_scatter(st, T.nig, E.aptr.p, E.rhs_src.p, E.u.p, E.vec.p);
+  launch_trsv_bwd(st, E.pool.p, E.trsv.p, T.nsub, E.tcptr.p, E.tck.p, E.tct.p, E.vec.p, E.maxnt);
+  launch_level_gather(st, S.ndofc, D.l2pos.p, E.vec.p, uc);
+}
+
+static void op_pc(Solver &S, const double *r, double *z, bool timed) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  // coarse correction uc = Ac^{-1} sum_i R_ci^T Phi_i^T W_i r_i
+  // coarse correction uc = Ac^{-1} sum_i R_ci^T Phi_i^T W_i r_i (three
+  // levels: the second level's BDDC in place of Ac^{-1})
   launch_phit(st, D.desc.p, S.nsub, D.Phi.p, D.map.p, D.w.p, r, D.rcs.p, D.maxnc);
   launch_coarse_gather(st, S.ndofc, D.cgptr.p, D.cgsrc.p, D.rcs.p, D.rc.p);
-  launch_dense_gemv(st, D.cpool.p, D.cfull, D.ntc, D.rc.p, D.cpart.p, D.uc.p);
+  if (S.lev2)
+    level2_apply(S, D.rc.p, D.uc.p);
+  else
+    launch_dense_gemv(st, D.cpool.p, D.cfull, D.ntc, D.rc.p, D.cpart.p, D.uc.p);
   // zl_i = K_i W_i r_i + Phi_i uc(indrowc), then z = sum_i R_i^T W_i zl_i
   if (timed) {
     HIPCHECK(hipEventCreate(&e0));

@@ -137,6 +137,11 @@ void launch_to_device_order(hipStream_t st, const OrderDesc *d, int nsub, const 
 void launch_from_device_order(hipStream_t st, const OrderDesc *d, int nsub, const int *perm,
                               const double *vec, double *dst, int maxndofs);
 // every local copy of unique interface dof g: vec[pos[q]] = u[g], q in ptr[g]..ptr[g+1]
+// three levels: vec[p] = w[p] * rc[src[p]] (0 where src < 0), p < n; and
+// uc[g] = vec[pos[g]], g < ndofc
+void launch_level_fill(hipStream_t st, int64_t n, const int *src, const double *w, const double *rc,
+                       double *vec);
+void launch_level_gather(hipStream_t st, int ndofc, const int *pos, const double *vec, double *uc);
 void launch_ifc_scatter(hipStream_t st, int n, const int *ptr, const int64_t *pos, const double *u,
                         double *vec);
 

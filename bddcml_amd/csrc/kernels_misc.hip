@@ -900,6 +900,28 @@ void launch_from_device_order(hipStream_t st, const OrderDesc *d, int nsub, cons
   hipLaunchKernelGGL(k_from_device_order, dim3((maxndofs + 255) / 256, nsub), dim3(256), 0, st, d,
                      perm, vec, dst);
 }
+__global__ void k_level_fill(int64_t n, const int *__restrict__ src, const double *__restrict__ w,
+                             const double *__restrict__ rc, double *__restrict__ vec) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int g = src[p];
+  vec[p] = g >= 0 ? w[p] * rc[g] : 0.0;
+}
+__global__ void k_level_gather(int n, const int *__restrict__ pos, const double *__restrict__ vec,
+                               double *__restrict__ uc) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) uc[g] = vec[pos[g]];
+}
+void launch_level_fill(hipStream_t st, int64_t n, const int *src, const double *w, const double *rc,
+                       double *vec) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_level_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, src, w, rc,
+                     vec);
+}
+void launch_level_gather(hipStream_t st, int ndofc, const int *pos, const double *vec, double *uc) {
+  if (ndofc <= 0) return;
+  hipLaunchKernelGGL(k_level_gather, dim3((ndofc + 255) / 256), dim3(256), 0, st, ndofc, pos, vec, uc);
+}
 void launch_ifc_scatter(hipStream_t st, int n, const int *ptr, const int64_t *pos, const double *u,
                         double *vec) {
   if (n <= 0) return;

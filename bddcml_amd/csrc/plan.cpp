@@ -747,6 +747,7 @@ void build_constraints(Solver &S) {
     s.cidx.clear();
     s.cval.clear();
     s.indrowc.clear();
+    s.cnodes.clear();
     std::vector<int> g2i;  // interface node of a global node (linear search on small sets)
     for (int g : s.corners) {
       int in = -1;
@@ -756,6 +757,7 @@ void build_constraints(Solver &S) {
           break;
         }
       int id = find(0, g);
+      s.cnodes.push_back(id);
       for (int d = 0; d < S.nndfc[id]; d++) {
         s.cidx.push_back(s.kdofi[in] + d);
         s.cval.push_back(1.0);
@@ -765,6 +767,7 @@ void build_constraints(Solver &S) {
     }
     for (auto &gl : s.globs) {
       int id = find(gl.type == 2 ? 1 : 2, gl.key);
+      s.cnodes.push_back(id);
       for (int d = 0; d < S.nndfc[id]; d++) {
         for (int in : gl.nodes)
           if (d < s.kdofi[in + 1] - s.kdofi[in]) {

@@ -70,6 +70,7 @@ struct Sub {
   std::vector<int> cptr, cidx;  // CSR, cidx = interface dof index (reference order)
   std::vector<double> cval;
   std::vector<int> indrowc;     // global coarse dof of each row
+  std::vector<int> cnodes;      // global coarse node of each local one (corners, then globs)
 
   std::vector<double> bc;   // eliminated Dirichlet values, ndofs
   // triplet entries touching fixed dofs with their values before the

@@ -23,12 +23,14 @@ struct Timings {
   double schur_flops = 0, interior_flops = 0, interface_flops = 0;
   double ev_gemv_ms = 0, gemv_bytes = 0;      // one batched S GEMV launch
   double ev_kapply_ms = 0, kapply_bytes = 0;  // one local preconditioner launch
+  double level2 = 0;  // second level: mesh, host setup and device setup (s)
   int iterations = 0;
 };
 
 struct Solver {
   // init
   int nl = 0, nsub = 0, numbase = 1, verbose = 0;
+  int nsub2 = 0;  // subdomains of the second level (three levels)
   bool initialised = false;
   // global sizes
   int nelem_g = 0, nnod_g = 0, ndof_g = 0, ndim = 3, meshdim = 3;
@@ -44,6 +46,7 @@ struct Solver {
   // topology results
   int ncorner = 0, nedge = 0, nface = 0, nnodc = 0, ndofc = 0;
   std::vector<int> nndfc, kdofc;
+  std::vector<double> xyzc;  // coarse node coordinates, nnodc x ndim column major
   bool bc_global = false;
   // node -> subdomains
   std::vector<int> n2s_ptr, n2s_idx;
@@ -66,6 +69,10 @@ struct Solver {
   std::vector<double> uig;  // interface solution
   std::unique_ptr<Device> dev;
   std::unique_ptr<DeviceInput> din;
+  // three levels: the second level, whose elements are this level's
+  // subdomains and whose nodes are its coarse nodes; one application of its
+  // BDDC replaces the coarse solve (src/module_levels.f90:3849-4249)
+  std::unique_ptr<Solver> lev2;
   Timings t;
   int last_iters = 0, last_reason = 0;
   double last_cond = 0;
@@ -89,6 +96,10 @@ void build_constraints(Solver &S);
 void build_condensation_rhs(Solver &S, int isub_only = -1);
 void build_unique_interface(Solver &S);
 void build_orderings(Solver &S);  // interior/interface device orderings
+// three levels (levels.cpp): coarse node coordinates, and the second level
+// built from this one's coarse nodes and coarse matrices and set up
+void build_coarse_nodes(Solver &S);
+void setup_second_level(Solver &S);
 
 // device steps (device.hip)
 void device_upload_entries(Solver &S, int is);
@@ -104,6 +115,10 @@ void device_fetch_solution(Solver &S);
 void device_setup_begin(Solver &S);
 void device_setup_finish(Solver &S);
 void device_reduced_rhs(Solver &S);
+// coarse matrices Ac_i of every subdomain (nconstr^2 each, row major) from HBM
+void device_coarse_matrices(Solver &S, std::vector<std::vector<double>> &acs);
+// maps between a level and its second level (which then shares its stream)
+void device_link_levels(Solver &S);
 void device_upload_rhs(Solver &S);  // re-upload the condensation rhs of all subdomains
 void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bool recycling,
                   int maxvec, int *num_iter, int *reason, double *cond);

@@ -27,8 +27,15 @@
  * hand-counted structured-cube values; invariants (partition of unity of
  * weights, S*1 = 0 on floating subdomains, lambda_min(MS) >= 1) are tested too.
  *
+ *   coarse nodes / level mesh . src/module_dd.f90:4869-5016 (cnode coordinates),
+ *                              src/module_levels.f90:1484-1530 (next level's mesh)
+ *   inexact coarse solve ..... src/module_levels.f90:3849-4249 (orc_apply_level)
+ *
  * Restrictions: matrixtype 1 (SPD) only; weights types 0, 1 and 3;
- * corner and arithmetic (edge/face average) constraints; two levels.
+ * corner and arithmetic (edge/face average) constraints.  Three levels are
+ * composed by the caller (oracle/multilevel.py): a second instance holds the
+ * next level's mesh and replaces the coarse solve through
+ * orc_set_coarse_solver.
  */
 #include "bddc_oracle.h"
 
@@ -80,6 +87,7 @@ typedef struct {
   int ng, *gtype, *gptr, *gnod, *gkey;
   /* coarse nodes and constraints */
   int nconstr;
+  int ncn, *cnid; /* global coarse node of each local one (corners, then globs) */
   double *C; /* nconstr x ndofi row major */
   int *indrowc;
   double *wi;
@@ -111,6 +119,9 @@ typedef struct {
   double *uig;                /* last interface solution (unique) */
   double *hist;
   int nhist;
+  /* coarse solve replaced by the next level's inexact one (three levels) */
+  orc_coarse_fn cfn;
+  void *cctx;
 } oprob;
 
 /* ------------------------------------------------------------------ utils */
@@ -162,7 +173,7 @@ static void free_sub(osub *s) {
                   s->icgn, s->gtype, s->gptr, s->gnod, s->gkey, s->C,
                   s->indrowc, s->wi, s->bc, s->sky_first, s->sky_off, s->sky,
                   s->S, s->aug, s->augpiv, s->phi, s->acs, s->g, s->sols,
-                  s->gdofi};
+                  s->gdofi, s->cnid};
   for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
 }
 
@@ -1062,7 +1073,9 @@ static int setup_constraints(oprob *P, int use_arith) {
       }
       r += nd;
     }
-    free(cid);
+    free(s->cnid);
+    s->ncn = ncn;
+    s->cnid = cid;
   }
   free(all);
   return 0;
@@ -1451,7 +1464,13 @@ static void op_pc(oprob *P, double **r, double **z) {
     memcpy(z[is], aux, sizeof(double) * ni);
     free(aux);
   }
-  if (nc > 0) chol_solve(P->Ac, nc, rc);
+  if (nc > 0 && P->cfn) {
+    double *uc = dmalloc(nc);
+    P->cfn(P->cctx, rc, uc);
+    memcpy(rc, uc, sizeof(double) * nc);
+    free(uc);
+  } else if (nc > 0)
+    chol_solve(P->Ac, nc, rc);
   for (int is = 0; is < P->nsub; is++) {
     osub *s = &P->s[is];
     int ni = s->ndofi, m = s->nconstr;
@@ -1909,4 +1928,113 @@ int orc_residual_history(void *h, double *out, int maxn) {
   int n = ONE_MIN(maxn, P->nhist);
   if (out) memcpy(out, P->hist, sizeof(double) * n);
   return P->nhist;
+}
+
+/* ---------------------------------------------------------- three levels */
+typedef struct {
+  int g, l;
+} gnl;
+static int cmp_gnl(const void *a, const void *b) {
+  const gnl *x = (const gnl *)a, *y = (const gnl *)b;
+  return (x->g > y->g) - (x->g < y->g);
+}
+
+void orc_coarse_nodes(void *h, int *nndfc, double *xyzc) {
+  oprob *P = (oprob *)h;
+  const int nc = P->nnodc, nd = P->ndim;
+  char *done = (char *)xcalloc(nc > 0 ? nc : 1, 1);
+  for (int k = 0; k < nc; k++) nndfc[k] = P->nndfc[k];
+  for (int is = 0; is < P->nsub; is++) {
+    const osub *s = &P->s[is];
+    for (int c = 0; c < s->ncn; c++) {
+      int id = s->cnid[c];
+      if (done[id]) continue;
+      done[id] = 1;
+      if (c < s->ncorner) {
+        int ln = -1;
+        for (int k = 0; k < s->nnodi; k++)
+          if (s->isngn[s->iin[k]] == s->icgn[c]) ln = s->iin[k];
+        for (int d = 0; d < nd; d++) xyzc[id + (size_t)nc * d] = s->xyz[ln + (size_t)s->nnods * d];
+      } else {
+        int q = c - s->ncorner, m = s->gptr[q + 1] - s->gptr[q];
+        gnl *v = (gnl *)xcalloc(m, sizeof(gnl));
+        for (int k = 0; k < m; k++) {
+          v[k].l = s->iin[s->gnod[s->gptr[q] + k]];
+          v[k].g = s->isngn[v[k].l];
+        }
+        qsort(v, m, sizeof(gnl), cmp_gnl);
+        for (int d = 0; d < nd; d++) {
+          double sum = 0.0;
+          for (int k = 0; k < m; k++) sum += s->xyz[v[k].l + (size_t)s->nnods * d];
+          xyzc[id + (size_t)nc * d] = sum / m;
+        }
+        free(v);
+      }
+    }
+  }
+  free(done);
+}
+
+int orc_sub_cnodes(void *h, int isub, int *out) {
+  oprob *P = (oprob *)h;
+  const osub *s = &P->s[isub - P->numbase];
+  if (out) memcpy(out, s->cnid, sizeof(int) * s->ncn);
+  return s->ncn;
+}
+
+void orc_set_coarse_solver(void *h, orc_coarse_fn fn, void *ctx) {
+  oprob *P = (oprob *)h;
+  P->cfn = fn;
+  P->cctx = ctx;
+}
+
+int orc_apply_level(void *h, const double *f, double *x) {
+  oprob *P = (oprob *)h;
+  if (P->bc_global) return -1;
+  double **g = alloc_rep(P), **z = alloc_rep(P);
+  double **fs = (double **)xcalloc(P->nsub, sizeof(double *));
+  /* right-hand side of each subdomain, weighted at the interface (it is
+   * complete), interior solve and reduced interface part */
+  for (int is = 0; is < P->nsub; is++) {
+    osub *s = &P->s[is];
+    int *o2l = imalloc(s->ndofs), *i2l = imalloc(s->ndofs);
+    sub_maps(s, o2l, i2l);
+    fs[is] = dmalloc(s->ndofs);
+    for (int k = 0; k < s->ndofs; k++) fs[is][k] = f[s->isvgvn[k]];
+    double *fo = dmalloc(s->ndofo + 1), *t = dmalloc(s->ndofi + 1);
+    for (int k = 0; k < s->ndofo; k++) fo[k] = fs[is][s->iovsvn[k]];
+    if (s->ndofo > 0) {
+      sky_solve(s, fo);
+      sub_a21(s, o2l, i2l, fo, t);
+    }
+    for (int k = 0; k < s->ndofi; k++) g[is][k] = s->wi[k] * fs[is][s->iivsvn[k]] - t[k];
+    free(fo);
+    free(t);
+    free(o2l);
+    free(i2l);
+  }
+  exchange_sum(P, g);
+  op_pc(P, g, z);
+  /* interior recovery from the assembled interface values */
+  for (int is = 0; is < P->nsub; is++) {
+    osub *s = &P->s[is];
+    int *o2l = imalloc(s->ndofs), *i2l = imalloc(s->ndofs);
+    sub_maps(s, o2l, i2l);
+    if (s->ndofo > 0) {
+      double *xo = dmalloc(s->ndofo);
+      sub_a12(s, o2l, i2l, z[is], xo);
+      for (int k = 0; k < s->ndofo; k++) xo[k] = fs[is][s->iovsvn[k]] - xo[k];
+      sky_solve(s, xo);
+      for (int k = 0; k < s->ndofo; k++) x[s->isvgvn[s->iovsvn[k]]] = xo[k];
+      free(xo);
+    }
+    for (int k = 0; k < s->ndofi; k++) x[s->isvgvn[s->iivsvn[k]]] = z[is][k];
+    free(fs[is]);
+    free(o2l);
+    free(i2l);
+  }
+  free(fs);
+  free_rep(P, g);
+  free_rep(P, z);
+  return 0;
 }

@@ -17,6 +17,7 @@
 extern "C" {
 #endif
 
+typedef void (*orc_coarse_fn)(void *ctx, const double *rc, double *uc);
 void *orc_create(int nsub, int nnod_global, int ndof_global, int ndim,
                  int meshdim, int numbase);
 void orc_destroy(void *h);
@@ -75,6 +76,24 @@ void orc_reduced_rhs(void *h, double *g);
 void orc_interface_solution(void *h, double *u);
 /* residual history of the last solve: n values (relres per iteration) */
 int orc_residual_history(void *h, double *out, int maxn);
+
+/* ---- three levels ------------------------------------------------------ */
+/* coarse nodes of the setup: nndfc (nnodc) and their coordinates xyzc
+ * (nnodc x ndim, column major): a corner's node, the mean of a glob's nodes
+ * summed in ascending global node order (src/module_dd.f90:4941-4992) */
+void orc_coarse_nodes(void *h, int *nndfc, double *xyzc);
+/* global coarse node of each coarse node of subdomain isub (corners, then
+ * globs); returns their number (out may be NULL to query it) */
+int orc_sub_cnodes(void *h, int isub, int *out);
+/* uc = (coarse solve)(rc) replaces the Cholesky solve of the coarse problem
+ * in the preconditioner (fn NULL restores it) */
+void orc_set_coarse_solver(void *h, orc_coarse_fn fn, void *ctx);
+/* one application of this level's BDDC to a whole-level vector, as the
+ * reference's levels >= 2 do (src/module_levels.f90:3849-4026, 4064-4249):
+ * f (ndof_global, a complete right-hand side) -> x (ndof_global):
+ * x_G = M (f_G - A21 A11^{-1} f_o) assembled, x_o = A11^{-1}(f_o - A12 x_G).
+ * Returns 0, or -1 where Dirichlet data would be needed. */
+int orc_apply_level(void *h, const double *f, double *x);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,16 @@ def _declare(L):
     L.orc_interface_solution.argtypes = [_v, _dp]
     L.orc_residual_history.restype = _i
     L.orc_residual_history.argtypes = [_v, C.c_void_p, _i]
+    L.orc_coarse_nodes.argtypes = [_v, _ip, _dp]
+    L.orc_sub_cnodes.restype = _i
+    L.orc_sub_cnodes.argtypes = [_v, _i, C.c_void_p]
+    L.orc_set_coarse_solver.argtypes = [_v, COARSE_FN, _v]
+    L.orc_apply_level.restype = _i
+    L.orc_apply_level.argtypes = [_v, _dp, _dp]
+
+
+# uc = f(ctx, rc) in place of the coarse Cholesky solve (three levels)
+COARSE_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
 
 
 def _ia(x):
@@ -217,6 +227,27 @@ class Oracle:
         if self.L.orc_download_local_solution(self.h, isub, out, n):
             raise RuntimeError("no solution")
         return out
+
+    def coarse_nodes(self):
+        """(nndfc, xyzc) of the coarse nodes, xyzc of shape (nnodc, 3)."""
+        c = self.counts()
+        n = c["ncorner"] + c["nedge"] + c["nface"]
+        nndf = np.zeros(max(n, 1), dtype=np.int32)
+        xyz = np.zeros(max(n, 1) * 3)
+        self.L.orc_coarse_nodes(self.h, nndf, xyz)
+        return nndf[:n], xyz[:n * 3].reshape(3, n).T.copy() if n else np.zeros((0, 3))
+
+    def sub_cnodes(self, isub):
+        n = self.L.orc_sub_cnodes(self.h, isub, None)
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        self.L.orc_sub_cnodes(self.h, isub, out.ctypes.data)
+        return out[:n]
+
+    def apply_level(self, f):
+        x = np.zeros_like(_da(f))
+        if self.L.orc_apply_level(self.h, _da(f), x):
+            raise RuntimeError("orc_apply_level needs a level without Dirichlet data")
+        return x
 
     def dotprod_subdomain(self, isub, v1, v2):
         out = C.c_double()

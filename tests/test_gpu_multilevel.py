@@ -1,0 +1,96 @@
+"""Three-level BDDC on the device against the three-level oracle.
+
+Level 2's elements are level 1's subdomains, its nodes level 1's coarse
+nodes, its element matrices the coarse matrices Ac_i; one application of
+its BDDC replaces the coarse solve (src/module_levels.f90:1297-1530,
+3849-4249).  The oracle composes two CPU instances the same way
+(oracle/multilevel.py).  Checked per case:
+- the second level's mesh and globs: subdomain count, corners, edges, faces,
+  coarse dofs and interface dofs equal to the oracle's (integers, exact);
+- FP64: the preconditioner M r within 1e-10 relative of the oracle's, the
+  interface solution within 1e-10 relative with PCG iteration counts within
+  +-1 and the Lanczos lambda_max within 1e-6 relative;
+- mixed precision (FP32 factors of level 1): interface solution within 1e-8
+  of the FP64 oracle, iterations within +-1 (the layered boxes, whose
+  stiff slabs make S' ill-conditioned and PCG's count sensitive to the
+  factor's precision: 15 % + 2, FP64 5 % + 1, as for two levels);
+- one second-level subdomain: identical to two levels (the coarse problem is
+  then solved directly).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import bddcml_amd
+from bddcml_amd.cube import BoxProblem, CubeProblem
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _relmax(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+CASES = [
+    ("cube-poisson", lambda: CubeProblem(4, 4, kind="poisson", rhs="random", seed=3), 8, 0),
+    ("cube-poisson-16", lambda: CubeProblem(3, 4, kind="poisson", rhs="random", seed=4), 16, 1),
+    ("cube-elasticity", lambda: CubeProblem(3, 3, kind="elasticity", rhs="random", seed=5), 4, 0),
+    ("boxes-elasticity", lambda: BoxProblem(12, 24, seed=3, layers=3, thick=1), 5, 1),
+]
+
+
+@pytest.mark.parametrize("name,make,nsub2,wt", CASES, ids=[c[0] for c in CASES])
+def test_three_levels_match_oracle(oracle_lib, name, make, nsub2, wt):
+    from oracle.multilevel import solve_three_levels
+    _say(f"case three_levels {name} nsub2={nsub2} wt={wt}")
+    p = make()
+    tol = 1e-10
+    t, (it_o, r_o, cond_o) = solve_three_levels(p, nsub2, weights_type=wt, tol=tol, maxit=400)
+    assert r_o == 0
+    u_o = t.o1.interface_solution()
+    c2 = t.o2.counts()
+    b, (it, r, cond) = bddcml_amd.solve_cube(p, weights_type=wt, tol=tol, maxit=400, nsub2=nsub2)
+    _say(f"  oracle {it_o} iterations cond {cond_o:.6g}; device {it} iterations cond {cond:.6g}")
+    lc = b.level2_counts()
+    assert lc is not None
+    assert lc["nsub"] == nsub2 and lc["ndof"] == t.ndofc
+    for k in ("ncorner", "nedge", "nface", "ndofc", "nig"):
+        assert lc[k] == c2[k], k
+    layered = isinstance(p, BoxProblem)
+    assert r == 0
+    assert abs(it - it_o) <= (max(1, 0.05 * it_o) if layered else 1)
+    assert abs(cond - cond_o) <= 1e-6 * cond_o
+    assert _relmax(b.interface_solution(), u_o) < 1e-10
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(len(u_o))
+    assert _relmax(b.apply_pc(x), t.o1.apply_pc(x)) < 1e-10
+    b.finalize()
+
+    _say(f"case three_levels {name} mixed precision")
+    b, (it, r, _) = bddcml_amd.solve_cube(p, weights_type=wt, tol=tol, maxit=400, nsub2=nsub2,
+                                          options={"precision": 32})
+    _say(f"  mixed: {it} iterations")
+    assert r == 0
+    assert (it <= 1.15 * it_o + 2) if layered else abs(it - it_o) <= 1
+    assert _relmax(b.interface_solution(), u_o) < 1e-8
+    b.finalize()
+    t.close()
+
+
+def test_one_second_level_subdomain_is_two_levels(oracle_lib):
+    _say("case three_levels one second-level subdomain")
+    p = CubeProblem(3, 3, kind="poisson", rhs="random", seed=6)
+    b2, res2 = bddcml_amd.solve_cube(p, tol=1e-10, maxit=300)
+    u2 = b2.interface_solution()
+    b2.finalize()
+    b3, res3 = bddcml_amd.solve_cube(p, tol=1e-10, maxit=300, nsub2=1)
+    assert b3.level2_counts() is None
+    assert res3[0] == res2[0] and res3[1] == 0
+    assert _relmax(b3.interface_solution(), u2) < 1e-13
+    b3.finalize()

@@ -547,12 +547,13 @@ catch (const Error &e) {
 }
 int bddcml_amd_timings(double *out, int n) try {
   Solver &S = solver();
-  double v[20] = {S.t.topology,        S.t.plan,           S.t.factor,          S.t.krylov,
+  double v[21] = {S.t.topology,        S.t.plan,           S.t.factor,          S.t.krylov,
                   S.t.postprocess,     S.t.setup_total,    S.t.solve_total,     S.t.rhs,
                   S.t.ev_interior_ms,  S.t.ev_schur_ms,    S.t.ev_interface_ms, S.t.ev_coarse_ms,
                   S.t.schur_flops,     S.t.interior_flops, S.t.interface_flops, S.t.ev_gemv_ms,
-                  S.t.gemv_bytes,      (double)S.last_iters, S.t.ev_kapply_ms, S.t.kapply_bytes};
-  for (int k = 0; k < std::min(n, 20); k++) out[k] = v[k];
+                  S.t.gemv_bytes,      (double)S.last_iters, S.t.ev_kapply_ms, S.t.kapply_bytes,
+                  S.t.level2};
+  for (int k = 0; k < std::min(n, 21); k++) out[k] = v[k];
   return 0;
 }
 catch (const Error &e) {

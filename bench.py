@@ -58,7 +58,8 @@ def schur_traffic(args):
     """HBM bytes per k_schur_gemm launch from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of the default workload (null for other
     workloads or when the summary is absent)."""
-    if (args.workload, args.nel, args.nsub, args.ordering, args.nd_leaf) != ("cube", 16, 8, 0, 256):
+    if (args.workload, args.nel, args.nsub, args.ordering, args.nd_leaf, args.levels) != (
+            "cube", 16, 8, 0, 256, 2):
         return None
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "k_schur_gemm_128cube_pmc.md")
@@ -145,6 +146,9 @@ def main():
     ap.add_argument("--nel", type=int, default=16, help="elements per subdomain edge")
     ap.add_argument("--nsub", type=int, default=8, help="subdomains per cube edge")
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--levels", type=int, default=2, choices=(2, 3),
+                    help="3: a second level of --nsub2 subdomains replaces the coarse solve")
+    ap.add_argument("--nsub2", type=int, default=8, help="subdomains of the second level")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ordering", type=int, default=0,
                     help="interior ordering: 0 tile-aligned nested dissection, 1 RCM")
@@ -174,17 +178,19 @@ def main():
 
     if args.workload == "gradient":
         return gradient_workload(args)
+    levels_txt = ("2 levels" if args.levels == 2 else
+                  f"3 levels ({args.nsub2} subdomains on level 2)")
     t0 = time.perf_counter()
     if args.workload == "cube":
         p = CubeProblem(args.nel, args.nsub, rhs="random", seed=2)
         workload = (f"Q1 Poisson {p.nel}^3 unit cube, {p.nsub} subdomains of {args.nel}^3 "
-                    f"elements, 2 levels, PCG to {args.tol:g}")
+                    f"elements, {levels_txt}, PCG to {args.tol:g}")
     else:
         from bddcml_amd.cube import BoxProblem
         p = BoxProblem(args.box_n, args.box_nsub, seed=3, kind="elasticity", layers=27,
                        jump=1e4)
         workload = (f"Q1 elasticity (E=1, nu=0.3) {p.nel}^3 unit cube, 27 seeded layers E=1e4, "
-                    f"{p.nsub} subdomains from seeded random box splits, 2 levels, weights "
+                    f"{p.nsub} subdomains from seeded random box splits, {levels_txt}, weights "
                     f"type {args.weights}, PCG to {args.tol:g}")
     subs = list(p.subdomains())
     sizes = [d["ndofs"] for d in subs]
@@ -197,7 +203,8 @@ def main():
         step_times, results, tims = [], [], []
         for step in range(args.warmup + args.steps):
             b = bddcml_amd.Bddcml()
-            b.init(2, [p.nsub, 1], numbase=p.numbase,
+            nsublev = [p.nsub, 1] if args.levels == 2 else [p.nsub, args.nsub2, 1]
+            b.init(args.levels, nsublev, numbase=p.numbase,
                    verbose_level=int(os.environ.get("BENCH_VERBOSE", "0")))
             b.set_option("interior_ordering", args.ordering)
             b.set_option("nd_leaf", args.nd_leaf)

@@ -217,16 +217,25 @@ void launch_coarse_assemble(hipStream_t st, double *pool, const int64_t *dst, co
 // Forward:  v_I = Linv_I (b_I - sum_k L_Ik v_k)      for the nto factored rows,
 //           v_I = b_I - sum_k L_Ik v_k               for the nti trailing rows
 // Backward: v_I = Linv_I^T (y_I - sum_k L_kI^T v_k)  for the nto factored rows
+// NW waves (64 lanes each) per workgroup: 4 when the subdomains fill the
+// chip, 16 when there are few of them (a second level) so that one
+// workgroup keeps more loads in flight.
+template <int NW>
 __device__ __forceinline__ double row_reduce(double s, double *red) {
   const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
   red[g * 64 + r] = s;
   __syncthreads();
   double v = 0.0;
-  if (threadIdx.x < 64) v = (red[r] + red[64 + r]) + (red[128 + r] + red[192 + r]);
+  if (threadIdx.x < 64) {
+#pragma unroll
+    for (int q = 0; q < NW; q += 4)
+      v += (red[q * 64 + r] + red[(q + 1) * 64 + r]) + (red[(q + 2) * 64 + r] + red[(q + 3) * 64 + r]);
+  }
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_trsv_fwd(const double *__restrict__ pool,
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_trsv_fwd(const double *__restrict__ pool,
                                                   const TrsvDesc *__restrict__ ds,
                                                   const int *__restrict__ rptr,
                                                   const int *__restrict__ rk,
@@ -236,8 +245,8 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const double *__restrict__ poo
   const TrsvDesc d = ds[blockIdx.x];
   const int nt = d.nto + d.nti;
   double *v = smem;              // nt*64
-  double *red = smem + nt * 64;  // 256
-  double *t = red + 256;         // 64
+  double *red = smem + nt * 64;  // NW*64
+  double *t = red + NW * 64;     // 64
   double *gv = vec + d.voff;
   const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
   for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
@@ -248,17 +257,17 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const double *__restrict__ poo
       const double *T = pool + (size_t)rt[q] * TILE_ELEMS + r;
       const double *x = v + rk[q] * 64;
 #pragma unroll 4
-      for (int c = g; c < 64; c += 4) s += T[c * 64] * x[c];
+      for (int c = g; c < 64; c += NW) s += T[c * 64] * x[c];
     }
-    double tv = row_reduce(s, red);
+    double tv = row_reduce<NW>(s, red);
     if (threadIdx.x < 64) t[r] = v[I * 64 + r] - tv;
     __syncthreads();
     if (I < d.nto) {
       const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r;
       double u = 0.0;
 #pragma unroll 4
-      for (int c = g; c < 64; c += 4) u += L[c * 64] * t[c];
-      double w = row_reduce(u, red);
+      for (int c = g; c < 64; c += NW) u += L[c * 64] * t[c];
+      double w = row_reduce<NW>(u, red);
       if (threadIdx.x < 64) v[I * 64 + r] = w;
     } else if (threadIdx.x < 64) {
       v[I * 64 + r] = t[r];
@@ -268,18 +277,20 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const double *__restrict__ poo
   for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) gv[k] = v[k];
 }
 
-__global__ __launch_bounds__(256) void k_trsv_bwd(const double *__restrict__ pool,
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_trsv_bwd(const double *__restrict__ pool,
                                                   const TrsvDesc *__restrict__ ds,
                                                   const int *__restrict__ cptr,
                                                   const int *__restrict__ ck,
                                                   const int *__restrict__ ct,
                                                   double *__restrict__ vec) {
+  constexpr int CW = 64 / NW;  // tile columns per wave
   extern __shared__ double smem[];
   const TrsvDesc d = ds[blockIdx.x];
   const int nt = d.nto + d.nti;
   double *v = smem;
   double *red = smem + nt * 64;
-  double *t = red + 256;
+  double *t = red + NW * 64;
   double *gv = vec + d.voff;
   const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
   for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
@@ -288,19 +299,19 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(const double *__restrict__ poo
     double s = 0.0;
     for (int q = cptr[d.cptr_off + I]; q < cptr[d.cptr_off + I + 1]; q++) {
       // (L_kI^T v_k)[r] = sum_c L_kI(c, r) v_k[c], column r of the tile is contiguous
-      const double *T = pool + (size_t)ct[q] * TILE_ELEMS + r * 64 + g * 16;
-      const double *x = v + ck[q] * 64 + g * 16;
+      const double *T = pool + (size_t)ct[q] * TILE_ELEMS + r * 64 + g * CW;
+      const double *x = v + ck[q] * 64 + g * CW;
 #pragma unroll
-      for (int c = 0; c < 16; c++) s += T[c] * x[c];
+      for (int c = 0; c < CW; c++) s += T[c] * x[c];
     }
-    double tv = row_reduce(s, red);
+    double tv = row_reduce<NW>(s, red);
     if (threadIdx.x < 64) t[r] = v[I * 64 + r] - tv;
     __syncthreads();
-    const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r * 64 + g * 16;
+    const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r * 64 + g * CW;
     double u = 0.0;
 #pragma unroll
-    for (int c = 0; c < 16; c++) u += L[c] * t[g * 16 + c];
-    double w = row_reduce(u, red);
+    for (int c = 0; c < CW; c++) u += L[c] * t[g * CW + c];
+    double w = row_reduce<NW>(u, red);
     if (threadIdx.x < 64) v[I * 64 + r] = w;
     __syncthreads();
   }
@@ -328,19 +339,33 @@ static void lds_request(const void *kernel, size_t bytes, const char *what) {
   }
 }
 
+// 16 waves per workgroup when fewer subdomains than half the CUs
+static constexpr int kTrsvFewSubs = 128;
 void launch_trsv_fwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
                      const int *rptr, const int *rk, const int *rt, double *v, int maxnt) {
   if (nsub <= 0) return;
-  size_t sh = (size_t)(maxnt * 64 + 256 + 64) * sizeof(double);
-  lds_request((const void *)k_trsv_fwd, sh, "triangular sweep");
-  hipLaunchKernelGGL(k_trsv_fwd, dim3(nsub), dim3(256), sh, st, pool, d, rptr, rk, rt, v);
+  if (nsub < kTrsvFewSubs) {
+    size_t sh = (size_t)(maxnt * 64 + 16 * 64 + 64) * sizeof(double);
+    lds_request((const void *)k_trsv_fwd<16>, sh, "triangular sweep");
+    hipLaunchKernelGGL(k_trsv_fwd<16>, dim3(nsub), dim3(1024), sh, st, pool, d, rptr, rk, rt, v);
+  } else {
+    size_t sh = (size_t)(maxnt * 64 + 4 * 64 + 64) * sizeof(double);
+    lds_request((const void *)k_trsv_fwd<4>, sh, "triangular sweep");
+    hipLaunchKernelGGL(k_trsv_fwd<4>, dim3(nsub), dim3(256), sh, st, pool, d, rptr, rk, rt, v);
+  }
 }
 void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
                      const int *cptr, const int *ck, const int *ct, double *v, int maxnt) {
   if (nsub <= 0) return;
-  size_t sh = (size_t)(maxnt * 64 + 256 + 64) * sizeof(double);
-  lds_request((const void *)k_trsv_bwd, sh, "triangular sweep");
-  hipLaunchKernelGGL(k_trsv_bwd, dim3(nsub), dim3(256), sh, st, pool, d, cptr, ck, ct, v);
+  if (nsub < kTrsvFewSubs) {
+    size_t sh = (size_t)(maxnt * 64 + 16 * 64 + 64) * sizeof(double);
+    lds_request((const void *)k_trsv_bwd<16>, sh, "triangular sweep");
+    hipLaunchKernelGGL(k_trsv_bwd<16>, dim3(nsub), dim3(1024), sh, st, pool, d, cptr, ck, ct, v);
+  } else {
+    size_t sh = (size_t)(maxnt * 64 + 4 * 64 + 64) * sizeof(double);
+    lds_request((const void *)k_trsv_bwd<4>, sh, "triangular sweep");
+    hipLaunchKernelGGL(k_trsv_bwd<4>, dim3(nsub), dim3(256), sh, st, pool, d, cptr, ck, ct, v);
+  }
 }
 
 // -------------------------------------------------------- iteration kernels

@@ -252,21 +252,28 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_fwd(const double *__restrict__
   for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
   __syncthreads();
   for (int I = 0; I < nt; I++) {
+    // Linv_I entries first (independent of the row sum)
+    double lv[64 / NW];
+    if (I < d.nto) {
+      const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r;
+#pragma unroll
+      for (int m = 0; m < 64 / NW; m++) lv[m] = L[(g + NW * m) * 64];
+    }
     double s = 0.0;
+#pragma unroll 2
     for (int q = rptr[d.rptr_off + I]; q < rptr[d.rptr_off + I + 1]; q++) {
       const double *T = pool + (size_t)rt[q] * TILE_ELEMS + r;
       const double *x = v + rk[q] * 64;
-#pragma unroll 4
+#pragma unroll
       for (int c = g; c < 64; c += NW) s += T[c * 64] * x[c];
     }
     double tv = row_reduce<NW>(s, red);
     if (threadIdx.x < 64) t[r] = v[I * 64 + r] - tv;
     __syncthreads();
     if (I < d.nto) {
-      const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r;
       double u = 0.0;
-#pragma unroll 4
-      for (int c = g; c < 64; c += NW) u += L[c * 64] * t[c];
+#pragma unroll
+      for (int m = 0; m < 64 / NW; m++) u += lv[m] * t[g + NW * m];
       double w = row_reduce<NW>(u, red);
       if (threadIdx.x < 64) v[I * 64 + r] = w;
     } else if (threadIdx.x < 64) {
@@ -583,10 +590,18 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
     s = wave_sum(s);
     if (l == 0) cc[c] = uc[cmap[d.coff + c]] - s;
   }
-  // forward sweep x_I <- Linv_I (x_I - sum_{k<I} L_Ik x_k); lane = tile row
+  // forward sweep x_I <- Linv_I (x_I - sum_{k<I} L_Ik x_k); lane = tile row.
+  // The Linv_I entries are loaded before the row sum (they do not depend on
+  // it), and four tiles of the row are in flight at a time.
   for (int I = 0; I < n; I++) {
+    double lv[4];
+    {
+      const T *L = pool + (size_t)(d.plinv + I) * TILE_ELEMS + l;
+#pragma unroll
+      for (int m = 0; m < 4; m++) lv[m] = (double)L[(g + 16 * m) * 64];
+    }
     double s = 0.0;
-#pragma unroll 2
+#pragma unroll 4
     for (int k = 0; k < I; k++) {
       const T *L = pool + (size_t)(d.pbase + packed_lower(n, I, k)) * TILE_ELEMS + l;
       const double *x = xs + k * 64;
@@ -603,10 +618,9 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
     }
     __syncthreads();
     {
-      const T *L = pool + (size_t)(d.plinv + I) * TILE_ELEMS + l;
       double u = 0.0;
 #pragma unroll
-      for (int m = 0; m < 4; m++) u += (double)L[(g + 16 * m) * 64] * t[g + 16 * m];
+      for (int m = 0; m < 4; m++) u += lv[m] * t[g + 16 * m];
       red[g * 64 + l] = u;
     }
     __syncthreads();
@@ -621,8 +635,14 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
   // backward sweep x_I <- Linv_I^T (x_I - sum_{k>I} L_kI^T x_k); lane = tile
   // row of L_kI (contiguous), wave g owns outputs g, g+16, g+32, g+48
   for (int I = n - 1; I >= 0; I--) {
+    double lv[4];
+    {
+      const T *L = pool + (size_t)(d.plinv + I) * TILE_ELEMS + g * 64 + l;
+#pragma unroll
+      for (int m = 0; m < 4; m++) lv[m] = (double)L[m * 16 * 64];
+    }
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 2
+#pragma unroll 4
     for (int k = I + 1; k < n; k++) {
       const T *L = pool + (size_t)(d.pbase + packed_lower(n, k, I)) * TILE_ELEMS + g * 64 + l;
       const double xv = xs[k * 64 + l];
@@ -636,10 +656,9 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
       for (int m = 0; m < 4; m++) t[g + 16 * m] = xs[I * 64 + g + 16 * m] - a[m];
     }
     __syncthreads();
-    const T *L = pool + (size_t)(d.plinv + I) * TILE_ELEMS + g * 64 + l;
     const double tv = t[l];
 #pragma unroll
-    for (int m = 0; m < 4; m++) a[m] = wave_sum((double)L[m * 16 * 64] * tv);
+    for (int m = 0; m < 4; m++) a[m] = wave_sum(lv[m] * tv);
     if (l == 0) {
 #pragma unroll
       for (int m = 0; m < 4; m++) xs[I * 64 + g + 16 * m] = a[m];

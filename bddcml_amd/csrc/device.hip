@@ -248,6 +248,7 @@ struct Device {
   View<double> w;
   DBuf<double> Z, Phi, G, Acs;
   int maxNI = 0, maxnc = 0, maxnti = 0, maxnt = 1, maxncT = 0;
+  bool kapply_deep = false;  // form of the local preconditioner launch
   int64_t LV = 0;
   View<SymvCol> scols;
   View<SymvRow> srows;
@@ -1231,6 +1232,20 @@ void device_setup_finish(Solver &S) {
     fatal("BDDCML_SETUP_PRECONDITIONER", "coarse problem too large for its dense factorization; "
                                          "use three levels");
   D.LV = lv;
+  {
+    // the deep form when subdomains are few or the largest sweep is well
+    // above the average one (its workgroup is then the tail of the launch)
+    double mx = 0.0, sum = 0.0;
+    for (const SubDesc &d : D.hdesc) {
+      const double wk = (double)d.nti * d.nti;
+      mx = std::max(mx, wk);
+      sum += wk;
+    }
+    D.kapply_deep = nsub < 128 || (sum > 0.0 && mx > 2.0 * sum / nsub);
+    if (S.verbose >= 2)
+      std::printf("   device plan: local preconditioner %s form (largest sweep %.2fx the mean)\n",
+                  D.kapply_deep ? "deep" : "shallow", sum > 0.0 ? mx * nsub / sum : 0.0);
+  }
   D.plan2.add(D.desc, D.hdesc);
   D.plan2.add(D.loff, loff);
   D.plan2.add(D.NIs, NIs);
@@ -1839,10 +1854,10 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed) {
   }
   if (D.mixed)
     launch_kapply(st, (const float *)D.pool32.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p,
-                  D.Z.p, D.cmap.p, D.uc.p, D.zl.p, D.maxNI, D.maxnc);
+                  D.Z.p, D.cmap.p, D.uc.p, D.zl.p, D.maxNI, D.maxnc, D.kapply_deep);
   else
     launch_kapply(st, (const double *)D.fpool.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p,
-                  D.Z.p, D.cmap.p, D.uc.p, D.zl.p, D.maxNI, D.maxnc);
+                  D.Z.p, D.cmap.p, D.uc.p, D.zl.p, D.maxNI, D.maxnc, D.kapply_deep);
   if (timed) {
     HIPCHECK(hipEventRecord(e1, st));
     HIPCHECK(hipEventSynchronize(e1));

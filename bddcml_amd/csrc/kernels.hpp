@@ -181,10 +181,12 @@ void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
 // Local part of the BDDC preconditioner on every subdomain, one workgroup
 // each: x = w .* r(map); z = S'^{-1} x through the packed factor of S'
 // (forward and backward tile sweeps); zl = z + Phi (uc(indrowc) - Z^T x).
+// deep: more loads in flight per workgroup at lower occupancy, for few or
+// uneven subdomains (same arithmetic, same results).
 template <typename T>
 void launch_kapply(hipStream_t st, const T *pool, const SubDesc *subs, int nsub, const int *map,
                    const double *w, const double *r, const double *Phi, const double *Z,
-                   const int *cmap, const double *uc, double *zl, int maxNI, int maxnc);
+                   const int *cmap, const double *uc, double *zl, int maxNI, int maxnc, bool deep);
 
 // deterministic reductions and vector updates on unique vectors
 constexpr int DOT_BLOCKS = 240;

@@ -564,7 +564,13 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <typename T>
+// DEEP: four tiles of a row in flight and the inverse diagonal tile loaded
+// ahead of the row sum (more loads per workgroup, 96 VGPRs: one workgroup
+// per CU) -- for few or uneven subdomains, where the largest subdomain's
+// sweep is the tail.  Otherwise two tiles in flight (56 VGPRs, two
+// workgroups per CU), for many subdomains of similar size, where the
+// launch is HBM-bound as a whole.
+template <typename T, bool DEEP>
 __global__ __launch_bounds__(KA_THR) void k_kapply(
     const T *__restrict__ pool, const SubDesc *__restrict__ subs, const int *__restrict__ map,
     const double *__restrict__ w, const double *__restrict__ rin, const double *__restrict__ Phi,
@@ -594,19 +600,29 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
   // The Linv_I entries are loaded before the row sum (they do not depend on
   // it), and four tiles of the row are in flight at a time.
   for (int I = 0; I < n; I++) {
+    const T *Li = pool + (size_t)(d.plinv + I) * TILE_ELEMS + l;
     double lv[4];
-    {
-      const T *L = pool + (size_t)(d.plinv + I) * TILE_ELEMS + l;
+    if (DEEP) {
 #pragma unroll
-      for (int m = 0; m < 4; m++) lv[m] = (double)L[(g + 16 * m) * 64];
+      for (int m = 0; m < 4; m++) lv[m] = (double)Li[(g + 16 * m) * 64];
     }
     double s = 0.0;
+    if (DEEP) {
 #pragma unroll 4
-    for (int k = 0; k < I; k++) {
-      const T *L = pool + (size_t)(d.pbase + packed_lower(n, I, k)) * TILE_ELEMS + l;
-      const double *x = xs + k * 64;
+      for (int k = 0; k < I; k++) {
+        const T *L = pool + (size_t)(d.pbase + packed_lower(n, I, k)) * TILE_ELEMS + l;
+        const double *x = xs + k * 64;
 #pragma unroll
-      for (int m = 0; m < 4; m++) s += (double)L[(g + 16 * m) * 64] * x[g + 16 * m];
+        for (int m = 0; m < 4; m++) s += (double)L[(g + 16 * m) * 64] * x[g + 16 * m];
+      }
+    } else {
+#pragma unroll 2
+      for (int k = 0; k < I; k++) {
+        const T *L = pool + (size_t)(d.pbase + packed_lower(n, I, k)) * TILE_ELEMS + l;
+        const double *x = xs + k * 64;
+#pragma unroll
+        for (int m = 0; m < 4; m++) s += (double)L[(g + 16 * m) * 64] * x[g + 16 * m];
+      }
     }
     red[g * 64 + l] = s;
     __syncthreads();
@@ -619,6 +635,10 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
     __syncthreads();
     {
       double u = 0.0;
+      if (!DEEP) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) lv[m] = (double)Li[(g + 16 * m) * 64];
+      }
 #pragma unroll
       for (int m = 0; m < 4; m++) u += lv[m] * t[g + 16 * m];
       red[g * 64 + l] = u;
@@ -635,19 +655,29 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
   // backward sweep x_I <- Linv_I^T (x_I - sum_{k>I} L_kI^T x_k); lane = tile
   // row of L_kI (contiguous), wave g owns outputs g, g+16, g+32, g+48
   for (int I = n - 1; I >= 0; I--) {
+    const T *Li = pool + (size_t)(d.plinv + I) * TILE_ELEMS + g * 64 + l;
     double lv[4];
-    {
-      const T *L = pool + (size_t)(d.plinv + I) * TILE_ELEMS + g * 64 + l;
+    if (DEEP) {
 #pragma unroll
-      for (int m = 0; m < 4; m++) lv[m] = (double)L[m * 16 * 64];
+      for (int m = 0; m < 4; m++) lv[m] = (double)Li[m * 16 * 64];
     }
     double a[4] = {0.0, 0.0, 0.0, 0.0};
+    if (DEEP) {
 #pragma unroll 4
-    for (int k = I + 1; k < n; k++) {
-      const T *L = pool + (size_t)(d.pbase + packed_lower(n, k, I)) * TILE_ELEMS + g * 64 + l;
-      const double xv = xs[k * 64 + l];
+      for (int k = I + 1; k < n; k++) {
+        const T *L = pool + (size_t)(d.pbase + packed_lower(n, k, I)) * TILE_ELEMS + g * 64 + l;
+        const double xv = xs[k * 64 + l];
 #pragma unroll
-      for (int m = 0; m < 4; m++) a[m] += (double)L[m * 16 * 64] * xv;
+        for (int m = 0; m < 4; m++) a[m] += (double)L[m * 16 * 64] * xv;
+      }
+    } else {
+#pragma unroll 2
+      for (int k = I + 1; k < n; k++) {
+        const T *L = pool + (size_t)(d.pbase + packed_lower(n, k, I)) * TILE_ELEMS + g * 64 + l;
+        const double xv = xs[k * 64 + l];
+#pragma unroll
+        for (int m = 0; m < 4; m++) a[m] += (double)L[m * 16 * 64] * xv;
+      }
     }
 #pragma unroll
     for (int m = 0; m < 4; m++) a[m] = wave_sum(a[m]);
@@ -657,6 +687,10 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
     }
     __syncthreads();
     const double tv = t[l];
+    if (!DEEP) {
+#pragma unroll
+      for (int m = 0; m < 4; m++) lv[m] = (double)Li[m * 16 * 64];
+    }
 #pragma unroll
     for (int m = 0; m < 4; m++) a[m] = wave_sum(lv[m] * tv);
     if (l == 0) {
@@ -691,20 +725,26 @@ void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
 template <typename T>
 void launch_kapply(hipStream_t st, const T *pool, const SubDesc *subs, int nsub, const int *map,
                    const double *w, const double *r, const double *Phi, const double *Z,
-                   const int *cmap, const double *uc, double *zl, int maxNI, int maxnc) {
+                   const int *cmap, const double *uc, double *zl, int maxNI, int maxnc, bool deep) {
   if (nsub <= 0) return;
   size_t sh = (size_t)(maxNI + 1024 + 64 + maxnc) * sizeof(double);
-  lds_request((const void *)k_kapply<T>, sh, "preconditioner apply");
-  hipLaunchKernelGGL(k_kapply<T>, dim3(nsub), dim3(KA_THR), sh, st, pool, subs, map, w, r, Phi, Z,
-                     cmap, uc, zl);
+  if (deep) {
+    lds_request((const void *)k_kapply<T, true>, sh, "preconditioner apply");
+    hipLaunchKernelGGL((k_kapply<T, true>), dim3(nsub), dim3(KA_THR), sh, st, pool, subs, map, w, r,
+                       Phi, Z, cmap, uc, zl);
+  } else {
+    lds_request((const void *)k_kapply<T, false>, sh, "preconditioner apply");
+    hipLaunchKernelGGL((k_kapply<T, false>), dim3(nsub), dim3(KA_THR), sh, st, pool, subs, map, w, r,
+                       Phi, Z, cmap, uc, zl);
+  }
 }
 template void launch_kapply<double>(hipStream_t, const double *, const SubDesc *, int, const int *,
                                     const double *, const double *, const double *,
                                     const double *, const int *, const double *, double *, int,
-                                    int);
+                                    int, bool);
 template void launch_kapply<float>(hipStream_t, const float *, const SubDesc *, int, const int *,
                                    const double *, const double *, const double *, const double *,
-                                   const int *, const double *, double *, int, int);
+                                   const int *, const double *, double *, int, int, bool);
 
 // ------------------------------------------------------------ vector ops
 __global__ __launch_bounds__(256) void k_dots(int n, DotArgs a, double *__restrict__ partial) {

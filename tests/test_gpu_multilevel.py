@@ -94,3 +94,44 @@ def test_one_second_level_subdomain_is_two_levels(oracle_lib):
     assert res3[0] == res2[0] and res3[1] == 0
     assert _relmax(b3.interface_solution(), u2) < 1e-13
     b3.finalize()
+
+
+def test_three_levels_bicgstab_and_new_data(oracle_lib):
+    """BiCGSTAB with three levels, then a new right-hand side on the kept
+    factorization (change_subdomain_data + setup_new_data, level 2 untouched),
+    both against the three-level oracle: iterations +-1, interface and local
+    solutions within 1e-10."""
+    from oracle.multilevel import solve_three_levels
+    _say("case three_levels bicgstab and new data")
+    p = CubeProblem(3, 4, kind="poisson", rhs="random", seed=11)
+    q = CubeProblem(3, 4, kind="poisson", rhs="random", seed=12)
+    tol = 1e-10
+    t, (it_o, r_o, _) = solve_three_levels(p, 8, method=1, tol=tol, maxit=300)
+    assert r_o == 0
+    u_o = t.o1.interface_solution()
+    t.close()
+    b = bddcml_amd.Bddcml()
+    b.init(3, [p.nsub, 8, 1], numbase=p.numbase)
+    for d in p.subdomains():
+        b.upload_subdomain_data(d)
+    b.setup_preconditioner(1, 0, 1, 1, 1, 0, 0, 0)
+    it, r, _ = b.solve(1, tol, 300, 50)
+    _say(f"  BiCGSTAB: device {it}, oracle {it_o} iterations")
+    assert r == 0 and abs(it - it_o) <= 1
+    assert _relmax(b.interface_solution(), u_o) < 1e-10
+
+    t, (it_o, r_o, _) = solve_three_levels(q, 8, method=0, tol=tol, maxit=300)
+    assert r_o == 0
+    for d in q.subdomains():
+        b.change_subdomain_data(d)
+    b.setup_new_data()
+    it, r, _ = b.solve(0, tol, 300, 50)
+    _say(f"  new right-hand side, PCG: device {it}, oracle {it_o} iterations")
+    assert r == 0 and abs(it - it_o) <= 1
+    u_o = t.o1.interface_solution()
+    assert _relmax(b.interface_solution(), u_o) < 1e-10
+    for d in q.subdomains():
+        ls = b.download_local_solution(d["isub"], d["ndofs"])
+        assert np.abs(ls - t.o1.local_solution(d["isub"])).max() <= 1e-10 * np.abs(u_o).max()
+    t.close()
+    b.finalize()

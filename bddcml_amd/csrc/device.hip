@@ -1840,7 +1840,7 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // coarse correction uc = Ac^{-1} sum_i R_ci^T Phi_i^T W_i r_i (three
   // levels: the second level's BDDC in place of Ac^{-1})
-  launch_phit(st, D.desc.p, S.nsub, D.Phi.p, D.map.p, D.w.p, r, D.rcs.p, D.maxnc);
+  launch_phit(st, D.desc.p, S.nsub, D.Phi.p, D.map.p, D.w.p, r, D.rcs.p, D.maxnc, D.maxNI);
   launch_coarse_gather(st, S.ndofc, D.cgptr.p, D.cgsrc.p, D.rcs.p, D.rc.p);
   if (S.lev2)
     level2_apply(S, D.rc.p, D.uc.p);

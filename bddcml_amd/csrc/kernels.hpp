@@ -175,7 +175,8 @@ void launch_assemble(hipStream_t st, int n, const int *ptr, const int64_t *src, 
                      const double *ylocal, double *y);
 // rc_s = Phi_s^T (w .* x_gathered)  -> coarse residual assembled by ptr/src
 void launch_phit(hipStream_t st, const SubDesc *subs, int nsub, const double *Phi,
-                 const int *map, const double *w, const double *x, double *rcs, int maxnc);
+                 const int *map, const double *w, const double *x, double *rcs, int maxnc,
+                 int maxNI);
 void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
                           const double *rcs, double *rc);
 // Local part of the BDDC preconditioner on every subdomain, one workgroup

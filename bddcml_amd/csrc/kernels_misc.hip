@@ -252,28 +252,21 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_fwd(const double *__restrict__
   for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
   __syncthreads();
   for (int I = 0; I < nt; I++) {
-    // Linv_I entries first (independent of the row sum)
-    double lv[64 / NW];
-    if (I < d.nto) {
-      const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r;
-#pragma unroll
-      for (int m = 0; m < 64 / NW; m++) lv[m] = L[(g + NW * m) * 64];
-    }
     double s = 0.0;
-#pragma unroll 2
     for (int q = rptr[d.rptr_off + I]; q < rptr[d.rptr_off + I + 1]; q++) {
       const double *T = pool + (size_t)rt[q] * TILE_ELEMS + r;
       const double *x = v + rk[q] * 64;
-#pragma unroll
+#pragma unroll 4
       for (int c = g; c < 64; c += NW) s += T[c * 64] * x[c];
     }
     double tv = row_reduce<NW>(s, red);
     if (threadIdx.x < 64) t[r] = v[I * 64 + r] - tv;
     __syncthreads();
     if (I < d.nto) {
+      const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r;
       double u = 0.0;
-#pragma unroll
-      for (int m = 0; m < 64 / NW; m++) u += lv[m] * t[g + NW * m];
+#pragma unroll 4
+      for (int c = g; c < 64; c += NW) u += L[c * 64] * t[c];
       double w = row_reduce<NW>(u, red);
       if (threadIdx.x < 64) v[I * 64 + r] = w;
     } else if (threadIdx.x < 64) {
@@ -521,27 +514,6 @@ __global__ void k_assemble(int n, const int *__restrict__ ptr, const int64_t *__
   y[g] = s;
 }
 
-__global__ void k_phit(const SubDesc *__restrict__ subs, const double *__restrict__ Phi,
-                       const int *__restrict__ map, const double *__restrict__ w,
-                       const double *__restrict__ x, double *__restrict__ rcs) {
-  __shared__ double red[256];
-  const SubDesc d = subs[blockIdx.y];
-  const int c = blockIdx.x;
-  if (c >= d.nc) return;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < d.NI; i += 256) {
-    int g = map[d.loff + i];
-    if (g >= 0) s += Phi[d.zoff + (size_t)c * d.NI + i] * w[d.loff + i] * x[g];
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) rcs[d.coff + c] = red[0];
-}
-
 __global__ void k_coarse_gather(int n, const int *__restrict__ ptr, const int *__restrict__ src,
                                 const double *__restrict__ rcs, double *__restrict__ rc) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -562,6 +534,36 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// rcs = Phi_i^T (w .* r(map)) per subdomain: the weighted interface vector is
+// gathered into LDS once per block, then each wave reduces PHIT_COLS / 4 of
+// the block's PHIT_COLS columns of Phi_i (contiguous, lane-strided).
+constexpr int PHIT_COLS = 16;
+__global__ __launch_bounds__(256) void k_phit(const SubDesc *__restrict__ subs,
+                                              const double *__restrict__ Phi,
+                                              const int *__restrict__ map,
+                                              const double *__restrict__ w,
+                                              const double *__restrict__ x,
+                                              double *__restrict__ rcs) {
+  extern __shared__ double xw[];
+  const SubDesc d = subs[blockIdx.y];
+  const int c0 = blockIdx.x * PHIT_COLS;
+  if (c0 >= d.nc) return;
+  const int NI = d.NI;
+  for (int i = threadIdx.x; i < NI; i += 256) {
+    const int g = map[d.loff + i];
+    xw[i] = g >= 0 ? w[d.loff + i] * x[g] : 0.0;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int c = c0 + wave; c < min(c0 + PHIT_COLS, d.nc); c += 4) {
+    const double *pc = Phi + d.zoff + (size_t)c * NI;
+    double s = 0.0;
+    for (int i = lane; i < NI; i += 64) s += pc[i] * xw[i];
+    s = wave_sum(s);
+    if (lane == 0) rcs[d.coff + c] = s;
+  }
 }
 
 // DEEP: four tiles of a row in flight and the inverse diagonal tile loaded
@@ -713,9 +715,12 @@ void launch_assemble(hipStream_t st, int n, const int *ptr, const int64_t *src, 
   hipLaunchKernelGGL(k_assemble, dim3((n + 255) / 256), dim3(256), 0, st, n, ptr, src, w, ylocal, y);
 }
 void launch_phit(hipStream_t st, const SubDesc *subs, int nsub, const double *Phi, const int *map,
-                 const double *w, const double *x, double *rcs, int maxnc) {
+                 const double *w, const double *x, double *rcs, int maxnc, int maxNI) {
   if (nsub <= 0 || maxnc <= 0) return;
-  hipLaunchKernelGGL(k_phit, dim3(maxnc, nsub), dim3(256), 0, st, subs, Phi, map, w, x, rcs);
+  const size_t sh = (size_t)std::max(maxNI, 1) * sizeof(double);
+  lds_request((const void *)k_phit, sh, "coarse residual");
+  hipLaunchKernelGGL(k_phit, dim3((maxnc + PHIT_COLS - 1) / PHIT_COLS, nsub), dim3(256), sh, st, subs,
+                     Phi, map, w, x, rcs);
 }
 void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
                           const double *rcs, double *rc) {

@@ -261,6 +261,7 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
   phase("condensation rhs");
   S.t.topology += wall_time() - th;
   if (S.plan_only) {
+    if (S.nl == 3 && S.nsub2 >= 2) setup_second_level(S);  // its host plan
     S.is_setup = true;
     return;
   }

@@ -115,8 +115,10 @@ static std::vector<int> divide_subdomains(const Solver &S, int nsub2) {
 }
 
 // The second level's subdomain data (0-based, as uploaded), with the
-// assembled upper triangle of its matrix from the coarse matrices.
-static void level2_subdomain(const Solver &S, const std::vector<int> &elems,
+// assembled upper triangle of its matrix from the coarse matrices; false if
+// a coarse dof of an element is missing from the subdomain (inconsistent
+// coarse numbering).
+static bool level2_subdomain(const Solver &S, const std::vector<int> &elems,
                              const std::vector<std::vector<double>> &acs, int j, Sub &t) {
   t = Sub();
   t.isub = j;
@@ -165,8 +167,7 @@ static void level2_subdomain(const Solver &S, const std::vector<int> &elems,
     std::vector<int> loc(nc);
     for (int a = 0; a < nc; a++) {
       auto it = std::lower_bound(t.isvgvn.begin(), t.isvgvn.end(), s.indrowc[a]);
-      if (it == t.isvgvn.end() || *it != s.indrowc[a])
-        fatal("LEVELS_PREPARE_STANDARD_LEVEL", "coarse dof outside the second-level subdomain");
+      if (it == t.isvgvn.end() || *it != s.indrowc[a]) return false;
       loc[a] = int(it - t.isvgvn.begin());
     }
     const std::vector<double> &A = acs[e];
@@ -182,6 +183,7 @@ static void level2_subdomain(const Solver &S, const std::vector<int> &elems,
       t.ai.push_back(x.i), t.aj.push_back(x.j), t.av.push_back(x.a);
   }
   t.uploaded = true;
+  return true;
 }
 
 void setup_second_level(Solver &S) {
@@ -191,7 +193,18 @@ void setup_second_level(Solver &S) {
   if (S.nsub2 > S.nsub) fatal(R, "more subdomains on the second level than on the first");
   build_coarse_nodes(S);
   std::vector<std::vector<double>> acs;
-  device_coarse_matrices(S, acs);
+  if (S.plan_only) {
+    // host plan only (no device): the second level's topology, division,
+    // globs and corners do not depend on the values; identity blocks
+    acs.resize(S.nsub);
+    for (int is = 0; is < S.nsub; is++) {
+      const int nc = S.subs[is].nconstr;
+      acs[is].assign((size_t)nc * nc, 0.0);
+      for (int a = 0; a < nc; a++) acs[is][(size_t)a * nc + a] = 1.0;
+    }
+  } else {
+    device_coarse_matrices(S, acs);
+  }
   const std::vector<int> part = divide_subdomains(S, S.nsub2);
   std::vector<std::vector<int>> elems(S.nsub2);
   for (int is = 0; is < S.nsub; is++) elems[part[is]].push_back(is);
@@ -216,23 +229,30 @@ void setup_second_level(Solver &S) {
   T.precision = 64;  // the coarse levels stay in FP64
   T.ordering = S.ordering;
   T.nd_leaf = S.nd_leaf;
+  T.plan_only = S.plan_only;
   T.subs.resize(T.nsub);
-#pragma omp parallel for schedule(dynamic)
-  for (int j = 0; j < T.nsub; j++) level2_subdomain(S, elems[j], acs, j, T.subs[j]);
-  for (int j = 0; j < T.nsub; j++) device_upload_entries(T, j);
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
+  for (int j = 0; j < T.nsub; j++)
+    if (!level2_subdomain(S, elems[j], acs, j, T.subs[j])) bad |= 1;
+  if (bad) fatal(R, "coarse dof outside the second-level subdomain");
+  if (!T.plan_only)
+    for (int j = 0; j < T.nsub; j++) device_upload_entries(T, j);
   // the pipeline of bddcml_setup_preconditioner on the second level
   build_topology(T);
   build_orderings(T);
-  device_setup_begin(T);
+  if (!T.plan_only) device_setup_begin(T);
   build_globs(T);
   build_constraints(T);
   build_unique_interface(T);
   apply_boundary_conditions(T);
   build_weights(T);
   build_condensation_rhs(T);
-  device_setup_finish(T);
+  if (!T.plan_only) {
+    device_setup_finish(T);
+    device_link_levels(S);
+  }
   T.is_setup = true;
-  device_link_levels(S);
   S.t.level2 = wall_time() - t0;
   if (S.verbose > 0)
     std::printf(" LEVELS_PREPARE_STANDARD_LEVEL: level 2: %d subdomains, %d dofs, ndofc %d, interface %d, %.4f s\n",

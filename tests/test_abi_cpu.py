@@ -97,3 +97,74 @@ def test_host_plan_matches_oracle(oracle_lib, nel, ns, kind, wt):
         _, ind = o.sub_constraints(isub)
         assert sorted(prod["subs"][s]["c"]) == sorted(ind.tolist())
     o.close()
+
+
+def _plan3_in_subprocess(nel, ns, kind, nsub2):
+    """The product's host plan with three levels (plan_only: the second level
+    is planned on identity element blocks, its topology does not depend on
+    the values); returns both levels' counts."""
+    code = f"""
+import sys, json
+sys.path.insert(0, {ROOT!r})
+import bddcml_amd
+from bddcml_amd.cube import CubeProblem
+p = CubeProblem({nel}, {ns}, kind={kind!r}, rhs='random', seed=3)
+b = bddcml_amd.Bddcml()
+b.init(3, [p.nsub, {nsub2}, 1], numbase=p.numbase)
+b.set_option('plan_only', 1)
+for d in p.subdomains():
+    b.upload_subdomain_data(d)
+b.setup_preconditioner(1, 0, 1, 1, 1, 0, 0, 0)
+print(json.dumps(dict(l1=b.counts(), l2=b.level2_counts())))
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    import json
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+@pytest.mark.parametrize("nel,ns,kind,nsub2", [(3, 4, "poisson", 8), (2, 4, "poisson", 16),
+                                               (2, 4, "poisson", 5), (2, 3, "elasticity", 4)])
+def test_second_level_plan_matches_oracle(oracle_lib, nel, ns, kind, nsub2):
+    """Second-level mesh, division (recursive bisection), globs and corners of
+    the product against the three-level oracle's (counts, exact)."""
+    from oracle.multilevel import ThreeLevels
+    p = CubeProblem(nel, ns, kind=kind, rhs="random", seed=3)
+    t = ThreeLevels(p.subdomains(), p.nsub, p.nnod, p.ndof, nsub2, p.numbase)
+    prod = _plan3_in_subprocess(nel, ns, kind, nsub2)
+    o2 = t.o2.counts()
+    assert prod["l2"] is not None
+    assert prod["l2"]["nsub"] == nsub2 and prod["l2"]["ndof"] == t.ndofc
+    for k in ("ncorner", "nedge", "nface", "ndofc", "nig"):
+        assert prod["l2"][k] == o2[k], k
+    t.close()
+
+
+def test_level_arguments_rejected():
+    """More than three levels, a last level of more than one subdomain, and a
+    second level larger than the first are rejected at init (reported, not
+    aborted, with throw_on_error)."""
+    code = f"""
+import sys
+sys.path.insert(0, {ROOT!r})
+import ctypes as C
+import bddcml_amd
+L = bddcml_amd.lib()
+L.bddcml_amd_set_option(b'throw_on_error', 1.0)
+def init(nl, lev):
+    arr = (C.c_int * len(lev))(*lev)
+    ip = lambda v: C.byref(C.c_int(v))
+    L.bddcml_init_c(ip(nl), arr, ip(len(lev)), C.byref(C.c_int(-1)), ip(0), ip(0), ip(1), ip(0))
+    buf = C.create_string_buffer(512)
+    L.bddcml_amd_last_error(buf, 512)
+    return buf.value.decode()
+print(repr(init(4, [8, 4, 2, 1])))
+print(repr(init(3, [8, 2, 2])))
+print(repr(init(3, [8, 16, 1])))
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()[-3:]
+    assert "two and three levels" in lines[0]
+    assert "last level must be 1" in lines[1]
+    assert "level 2" in lines[2]

@@ -217,6 +217,12 @@ void launch_coarse_assemble(hipStream_t st, double *pool, const int64_t *dst, co
 // Forward:  v_I = Linv_I (b_I - sum_k L_Ik v_k)      for the nto factored rows,
 //           v_I = b_I - sum_k L_Ik v_k               for the nti trailing rows
 // Backward: v_I = Linv_I^T (y_I - sum_k L_kI^T v_k)  for the nto factored rows
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // NW waves (64 lanes each) per workgroup: 4 when the subdomains fill the
 // chip, 16 when there are few of them (a second level) so that one
 // workgroup keeps more loads in flight.
@@ -295,6 +301,36 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_bwd(const double *__restrict__
   const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
   for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
   __syncthreads();
+  if constexpr (NW == 16) {
+    // lane = row c of the contiguous column: wave g produces the outputs
+    // g + 16 m (m < 4) by wave reductions, every load a 512-byte column
+    const int l = r;
+    for (int I = d.nto - 1; I >= 0; I--) {
+      double a[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int q = cptr[d.cptr_off + I]; q < cptr[d.cptr_off + I + 1]; q++) {
+        const double *T = pool + (size_t)ct[q] * TILE_ELEMS + l;
+        const double xv = v[ck[q] * 64 + l];
+#pragma unroll
+        for (int m = 0; m < 4; m++) a[m] += T[(g + 16 * m) * 64] * xv;
+      }
+#pragma unroll
+      for (int m = 0; m < 4; m++) a[m] = wave_sum(a[m]);
+      if (l == 0) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) t[g + 16 * m] = v[I * 64 + g + 16 * m] - a[m];
+      }
+      __syncthreads();
+      const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + l;
+      const double tv = t[l];
+#pragma unroll
+      for (int m = 0; m < 4; m++) a[m] = wave_sum(L[(g + 16 * m) * 64] * tv);
+      if (l == 0) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) v[I * 64 + g + 16 * m] = a[m];
+      }
+      __syncthreads();
+    }
+  } else {
   for (int I = d.nto - 1; I >= 0; I--) {
     double s = 0.0;
     for (int q = cptr[d.cptr_off + I]; q < cptr[d.cptr_off + I + 1]; q++) {
@@ -314,6 +350,7 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_bwd(const double *__restrict__
     double w = row_reduce<NW>(u, red);
     if (threadIdx.x < 64) v[I * 64 + r] = w;
     __syncthreads();
+  }
   }
   for (int k = threadIdx.x; k < d.nto * 64; k += blockDim.x) gv[k] = v[k];
 }
@@ -530,11 +567,6 @@ __device__ __forceinline__ int packed_lower(int n, int I, int J) {
   return J * n - J * (J - 1) / 2 + (I - J);
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 // rcs = Phi_i^T (w .* r(map)) per subdomain: the weighted interface vector is
 // gathered into LDS once per block, then each wave reduces PHIT_COLS / 4 of

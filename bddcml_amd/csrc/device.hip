@@ -222,6 +222,7 @@ struct Device {
   DBuf<double> cpool;     // coarse matrix: factor, inverse diagonal, W, full inverse
   int64_t ntilesc = 0;
   Arena plan2;            // task lists and job lists of the second half
+  Arena plan3;            // coarse factorization lists (built during the interface phase)
   DBuf<int> err;
   std::vector<SubTiles> sub;
   // setup phases
@@ -1029,92 +1030,6 @@ void device_setup_finish(Solver &S) {
     upload_phases(D.z_bwd, z_bwd, D.plan2);
     D.phi.upload(phi, D.plan2);
   }
-  if (D.dense_coarse) {
-    // coarse problem (one dense matrix, tiles in the coarse pool):
-    // right-looking tile Cholesky and the inverse factor W = L^{-1} by
-    // right-looking forward substitution, so that every task is a single tile
-    // product and each step is a few short launches; then Ac^{-1} = W^T W
-    std::vector<SeqLaunch> cseq;
-    TaskList cinv;
-    const int n = D.ntc, cb = D.cbase, cl = D.clinv, cw = D.cwbase;
-    auto Pc = [=](int I, int J) { return cb + lowidx(n, I, J); };
-    auto Wc = [=](int I, int J) { return I == J ? cl + I : cw + lowidx(n - 1, I - 1, J); };
-    // per column k three launches: the diagonal factor; the panel
-    // L_ik = A_ik Linv_k^T together with the W row W_kj = Linv_k B_kj; the
-    // trailing updates A_IK -= L_Ik L_Kk^T together with B_ij -= L_ik W_kj
-    // (tasks of one launch carry their own product kind)
-    const int panel = task_mode(GEMM_NT, POST_RIGHT_XT, false),
-              wrow = task_mode(GEMM_NN, POST_LEFT_X, false),
-              trail = task_mode(GEMM_NT, POST_NONE, true), bupd = task_mode(GEMM_NN, POST_NONE, true);
-    auto one = [](TaskList &l, int c, int a, int b, int mode) {
-      const int p0 = (int)l.pairs.size();
-      l.pairs.push_back({a, b});
-      TileTask t{c, -1, p0, p0 + 1};
-      t.mode = mode;
-      l.tasks.push_back(t);
-    };
-    auto post = [](TaskList &l, int c, int x, int mode) {
-      TileTask t{c, x, 0, 0};
-      t.mode = mode;
-      l.tasks.push_back(t);
-    };
-    for (int k = 0; k < n; k++) {
-      SeqLaunch f;  // L_kk = chol(A_kk), Linv_k
-      f.potrf = true;
-      f.l.tasks.push_back({Pc(k, k), cl + k, 0, 0});
-      cseq.push_back(std::move(f));
-      SeqLaunch pw;
-      for (int i = k + 1; i < n; i++) post(pw.l, Pc(i, k), cl + k, panel);
-      for (int j = 0; j < k; j++) post(pw.l, Wc(k, j), cl + k, wrow);
-      if (!pw.l.tasks.empty()) cseq.push_back(std::move(pw));
-      SeqLaunch up;
-      for (int K = k + 1; K < n; K++)
-        for (int I = K; I < n; I++) one(up.l, Pc(I, K), Pc(I, k), Pc(K, k), trail);
-      for (int i = k + 1; i < n; i++)
-        for (int j = 0; j <= k; j++) one(up.l, Wc(i, j), Pc(i, k), Wc(k, j), bupd);
-      if (!up.l.tasks.empty()) cseq.push_back(std::move(up));
-    }
-    for (int j = 0; j < n; j++)
-      for (int i = j; i < n; i++) {
-        TileTask t{Pc(i, j), -1, (int)cinv.pairs.size(), 0};
-        for (int k = i; k < n; k++) cinv.pairs.push_back({Wc(k, i), Wc(k, j)});
-        t.p1 = (int)cinv.pairs.size();
-        cinv.tasks.push_back(t);
-      }
-    for (auto &q : cseq) check_tasks(q.l, D.ntilesc, q.potrf || q.post != POST_NONE, "coarse factorization");
-    check_tasks(cinv, D.ntilesc, false, "coarse W^T W");
-    D.cseq.resize(cseq.size());
-    for (size_t k = 0; k < cseq.size(); k++) {
-      SeqPhase &q = D.cseq[k];
-      q.ph.upload(cseq[k].l, D.plan2);
-      q.kind = cseq[k].kind, q.post = cseq[k].post, q.alpha = cseq[k].alpha, q.beta = cseq[k].beta;
-      q.potrf = cseq[k].potrf;
-    }
-    D.cinv.upload(cinv, D.plan2);
-    // full-storage mirror of Ac^{-1} for its GEMV; the factor, inverse
-    // diagonal and W tiles start from zero; padding diagonals 1.0
-    std::vector<MirrorJob> mc;
-    for (int J = 0; J < n; J++)
-      for (int I = J; I < n; I++) {
-        int src = D.cbase + lowidx(n, I, J);
-        mc.push_back({src, D.cfull + I * n + J, 0});
-        if (I > J) mc.push_back({src, D.cfull + J * n + I, 1});
-      }
-    for (const MirrorJob &j : mc)
-      if (j.src < 0 || j.src >= D.ntilesc || j.dst < 0 || j.dst >= D.ntilesc)
-        fatal("BDDCML_DEVICE_PLAN", "invalid mirror job");
-    std::vector<int2> cz{make_int2(D.cbase, n * (n + 1) / 2 + n + n * (n - 1) / 2)};
-    if ((int64_t)cz[0].x + cz[0].y > D.ntilesc) fatal("BDDCML_DEVICE_PLAN", "invalid tile range to clear");
-    std::vector<int64_t> cpads;
-    for (int p = S.ndofc; p < D.NC; p++)
-      cpads.push_back((int64_t)(D.cbase + lowidx(n, p / TILE, p / TILE)) * TILE_ELEMS +
-                      (p % TILE) * TILE + (p % TILE));
-    for (int64_t k : cpads)
-      if (k < 0 || k >= D.ntilesc * (int64_t)TILE_ELEMS) fatal("BDDCML_DEVICE_PLAN", "scatter index out of range");
-    D.plan2.add(D.mir_c, mc);
-    D.plan2.add(D.c_zero, cz);
-    D.plan2.add(D.c_pads, cpads);
-  }
   {
     // S' = S + C^T W C: one job per lower tile, into the factor space
     std::vector<int> spsub, spij, spdst;
@@ -1404,43 +1319,6 @@ void device_setup_finish(Solver &S) {
     D.plan2.add(D.cgptr, cnt);
     D.plan2.add(D.cgsrc, src);
   }
-  // coarse assembly CSR over the lower tiles of the coarse matrix
-  if (D.dense_coarse) {
-    struct E {
-      int64_t dst, src;
-    };
-    std::vector<E> ent;
-    const int n = D.ntc;
-    for (int is = 0; is < nsub; is++) {
-      const Sub &s = S.subs[is];
-      const SubDesc &d = D.hdesc[is];
-      for (int a = 0; a < s.nconstr; a++)
-        for (int b = 0; b < s.nconstr; b++) {
-          int ga = s.indrowc[a], gb = s.indrowc[b];
-          if (ga < 0 || ga >= S.ndofc || gb < 0 || gb >= S.ndofc)
-            fatal("BDDCML_DEVICE_PLAN", "coarse dof out of range");
-          int I = ga / TILE, J = gb / TILE;
-          if (I < J) continue;
-          int64_t dst = (int64_t)(D.cbase + lowidx(n, I, J)) * TILE_ELEMS + (gb % TILE) * TILE + ga % TILE;
-          ent.push_back({dst, d.goff + (int64_t)a * s.nconstr + b});
-        }
-    }
-    std::stable_sort(ent.begin(), ent.end(), [](const E &a, const E &b) { return a.dst < b.dst; });
-    std::vector<int64_t> dst, src;
-    std::vector<int> ptr(1, 0);
-    for (size_t k = 0; k < ent.size(); k++) {
-      if (dst.empty() || dst.back() != ent[k].dst) {
-        if (!dst.empty()) ptr.push_back((int)k);
-        dst.push_back(ent[k].dst);
-      }
-      src.push_back(ent[k].src);
-    }
-    ptr.push_back((int)ent.size());
-    D.plan2.add(D.ca_dst, dst);
-    D.plan2.add(D.ca_src, src);
-    D.plan2.add(D.ca_ptr, ptr);
-    D.nca = (int)dst.size();
-  }
   // vectors
   D.nig = S.nig;
   D.yl.alloc(lv);
@@ -1501,12 +1379,6 @@ void device_setup_finish(Solver &S) {
                      1.0);
     launch_y_to_z(st, fp, D.desc.p, nsub, D.Phi.p, D.maxNI, 1);
   };
-  // coarse tiles cleared (they accumulate) and padding diagonals set early,
-  // off the critical path of the coarse phase
-  if (D.dense_coarse) {
-    launch_zero_ranges(st, D.cpool.p, D.c_zero.p, (int)D.c_zero.n);
-    launch_scatter_ones(st, D.cpool.p, D.c_pads.p, (int64_t)D.c_pads.n);
-  }
   if (D.mixed) factor_phase(D.pool32.p);
   else factor_phase(D.fpool.p);
   launch_g(st, D.desc.p, nsub, D.cptr.p, D.cdof.p, D.Z.p, D.G.p);
@@ -1515,8 +1387,140 @@ void device_setup_finish(Solver &S) {
   else phi_phase(D.fpool.p);
   launch_acs(st, D.desc.p, nsub, D.cptr.p, D.G.p, D.Acs.p);
   HIPCHECK(hipEventRecord(ev[3], st));
-  // coarse problem (with three levels the second level is set up instead)
+  // coarse problem (with three levels the second level is set up instead):
+  // its task lists and assembly map are built on the host while the device
+  // runs the interface phase launched above, and uploaded on the upload
+  // stream (synchronized there, so the compute stream is not waited on)
   if (D.dense_coarse) {
+    const double tc = wall_time();
+    {
+      // coarse problem (one dense matrix, tiles in the coarse pool):
+      // right-looking tile Cholesky and the inverse factor W = L^{-1} by
+      // right-looking forward substitution, so that every task is a single tile
+      // product and each step is a few short launches; then Ac^{-1} = W^T W
+      std::vector<SeqLaunch> cseq;
+      TaskList cinv;
+      const int n = D.ntc, cb = D.cbase, cl = D.clinv, cw = D.cwbase;
+      auto Pc = [=](int I, int J) { return cb + lowidx(n, I, J); };
+      auto Wc = [=](int I, int J) { return I == J ? cl + I : cw + lowidx(n - 1, I - 1, J); };
+      // per column k three launches: the diagonal factor; the panel
+      // L_ik = A_ik Linv_k^T together with the W row W_kj = Linv_k B_kj; the
+      // trailing updates A_IK -= L_Ik L_Kk^T together with B_ij -= L_ik W_kj
+      // (tasks of one launch carry their own product kind)
+      const int panel = task_mode(GEMM_NT, POST_RIGHT_XT, false),
+                wrow = task_mode(GEMM_NN, POST_LEFT_X, false),
+                trail = task_mode(GEMM_NT, POST_NONE, true), bupd = task_mode(GEMM_NN, POST_NONE, true);
+      auto one = [](TaskList &l, int c, int a, int b, int mode) {
+        const int p0 = (int)l.pairs.size();
+        l.pairs.push_back({a, b});
+        TileTask t{c, -1, p0, p0 + 1};
+        t.mode = mode;
+        l.tasks.push_back(t);
+      };
+      auto post = [](TaskList &l, int c, int x, int mode) {
+        TileTask t{c, x, 0, 0};
+        t.mode = mode;
+        l.tasks.push_back(t);
+      };
+      for (int k = 0; k < n; k++) {
+        SeqLaunch f;  // L_kk = chol(A_kk), Linv_k
+        f.potrf = true;
+        f.l.tasks.push_back({Pc(k, k), cl + k, 0, 0});
+        cseq.push_back(std::move(f));
+        SeqLaunch pw;
+        for (int i = k + 1; i < n; i++) post(pw.l, Pc(i, k), cl + k, panel);
+        for (int j = 0; j < k; j++) post(pw.l, Wc(k, j), cl + k, wrow);
+        if (!pw.l.tasks.empty()) cseq.push_back(std::move(pw));
+        SeqLaunch up;
+        for (int K = k + 1; K < n; K++)
+          for (int I = K; I < n; I++) one(up.l, Pc(I, K), Pc(I, k), Pc(K, k), trail);
+        for (int i = k + 1; i < n; i++)
+          for (int j = 0; j <= k; j++) one(up.l, Wc(i, j), Pc(i, k), Wc(k, j), bupd);
+        if (!up.l.tasks.empty()) cseq.push_back(std::move(up));
+      }
+      for (int j = 0; j < n; j++)
+        for (int i = j; i < n; i++) {
+          TileTask t{Pc(i, j), -1, (int)cinv.pairs.size(), 0};
+          for (int k = i; k < n; k++) cinv.pairs.push_back({Wc(k, i), Wc(k, j)});
+          t.p1 = (int)cinv.pairs.size();
+          cinv.tasks.push_back(t);
+        }
+      for (auto &q : cseq) check_tasks(q.l, D.ntilesc, q.potrf || q.post != POST_NONE, "coarse factorization");
+      check_tasks(cinv, D.ntilesc, false, "coarse W^T W");
+      D.cseq.resize(cseq.size());
+      for (size_t k = 0; k < cseq.size(); k++) {
+        SeqPhase &q = D.cseq[k];
+        q.ph.upload(cseq[k].l, D.plan3);
+        q.kind = cseq[k].kind, q.post = cseq[k].post, q.alpha = cseq[k].alpha, q.beta = cseq[k].beta;
+        q.potrf = cseq[k].potrf;
+      }
+      D.cinv.upload(cinv, D.plan3);
+      // full-storage mirror of Ac^{-1} for its GEMV; the factor, inverse
+      // diagonal and W tiles start from zero; padding diagonals 1.0
+      std::vector<MirrorJob> mc;
+      for (int J = 0; J < n; J++)
+        for (int I = J; I < n; I++) {
+          int src = D.cbase + lowidx(n, I, J);
+          mc.push_back({src, D.cfull + I * n + J, 0});
+          if (I > J) mc.push_back({src, D.cfull + J * n + I, 1});
+        }
+      for (const MirrorJob &j : mc)
+        if (j.src < 0 || j.src >= D.ntilesc || j.dst < 0 || j.dst >= D.ntilesc)
+          fatal("BDDCML_DEVICE_PLAN", "invalid mirror job");
+      std::vector<int2> cz{make_int2(D.cbase, n * (n + 1) / 2 + n + n * (n - 1) / 2)};
+      if ((int64_t)cz[0].x + cz[0].y > D.ntilesc) fatal("BDDCML_DEVICE_PLAN", "invalid tile range to clear");
+      std::vector<int64_t> cpads;
+      for (int p = S.ndofc; p < D.NC; p++)
+        cpads.push_back((int64_t)(D.cbase + lowidx(n, p / TILE, p / TILE)) * TILE_ELEMS +
+                        (p % TILE) * TILE + (p % TILE));
+      for (int64_t k : cpads)
+        if (k < 0 || k >= D.ntilesc * (int64_t)TILE_ELEMS) fatal("BDDCML_DEVICE_PLAN", "scatter index out of range");
+      D.plan3.add(D.mir_c, mc);
+      D.plan3.add(D.c_zero, cz);
+      D.plan3.add(D.c_pads, cpads);
+    }
+    // coarse assembly CSR over the lower tiles of the coarse matrix
+    {
+      struct E {
+        int64_t dst, src;
+      };
+      std::vector<E> ent;
+      const int n = D.ntc;
+      for (int is = 0; is < nsub; is++) {
+        const Sub &s = S.subs[is];
+        const SubDesc &d = D.hdesc[is];
+        for (int a = 0; a < s.nconstr; a++)
+          for (int b = 0; b < s.nconstr; b++) {
+            int ga = s.indrowc[a], gb = s.indrowc[b];
+            if (ga < 0 || ga >= S.ndofc || gb < 0 || gb >= S.ndofc)
+              fatal("BDDCML_DEVICE_PLAN", "coarse dof out of range");
+            int I = ga / TILE, J = gb / TILE;
+            if (I < J) continue;
+            int64_t dst = (int64_t)(D.cbase + lowidx(n, I, J)) * TILE_ELEMS + (gb % TILE) * TILE + ga % TILE;
+            ent.push_back({dst, d.goff + (int64_t)a * s.nconstr + b});
+          }
+      }
+      std::stable_sort(ent.begin(), ent.end(), [](const E &a, const E &b) { return a.dst < b.dst; });
+      std::vector<int64_t> dst, src;
+      std::vector<int> ptr(1, 0);
+      for (size_t k = 0; k < ent.size(); k++) {
+        if (dst.empty() || dst.back() != ent[k].dst) {
+          if (!dst.empty()) ptr.push_back((int)k);
+          dst.push_back(ent[k].dst);
+        }
+        src.push_back(ent[k].src);
+      }
+      ptr.push_back((int)ent.size());
+      D.plan3.add(D.ca_dst, dst);
+      D.plan3.add(D.ca_src, src);
+      D.plan3.add(D.ca_ptr, ptr);
+      D.nca = (int)dst.size();
+    }
+    D.plan3.commit(D.up);
+    S.t.plan += wall_time() - tc;
+    // coarse tiles cleared (they accumulate), padding diagonals 1
+    launch_zero_ranges(st, D.cpool.p, D.c_zero.p, (int)D.c_zero.n);
+    launch_scatter_ones(st, D.cpool.p, D.c_pads.p, (int64_t)D.c_pads.n);
     double *cpool = D.cpool.p;
     launch_coarse_assemble(st, cpool, D.ca_dst.p, D.ca_ptr.p, D.ca_src.p, D.nca, D.Acs.p);
     for (const SeqPhase &q : D.cseq) {

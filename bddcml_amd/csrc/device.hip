@@ -213,6 +213,8 @@ struct Device {
   hipStream_t up = nullptr;  // uploads of the second setup half
   hipEvent_t ev[5] = {};     // setup phase boundaries (timing)
   hipEvent_t ev_up = nullptr;
+  hipEvent_t ev_rhs = nullptr;  // reduced right-hand side done (upload stream)
+  bool rhs_pending = false;     // it was launched there by the setup
   DBuf<double> pool;      // subdomain tiles: pattern of A (factor, S), inverse diagonal tiles
   int64_t ntiles = 0;
   bool mixed = false;     // S' factor, its inverse diagonal tiles and Z in FP32
@@ -346,6 +348,7 @@ void device_release(Solver &S) {
     for (hipEvent_t &e : D.ev)
       if (e) (void)hipEventDestroy(e), e = nullptr;
     if (D.ev_up) (void)hipEventDestroy(D.ev_up), D.ev_up = nullptr;
+    if (D.ev_rhs) (void)hipEventDestroy(D.ev_rhs), D.ev_rhs = nullptr;
     if (D.st && !D.borrowed_st) (void)hipStreamDestroy(D.st);
     if (D.up) (void)hipStreamDestroy(D.up);
     D.st = D.up = nullptr;
@@ -893,6 +896,7 @@ void device_setup_begin(Solver &S) {
   hipEvent_t *ev = D.ev;
   for (int k = 0; k < 5; k++) HIPCHECK(hipEventCreate(&ev[k]));
   HIPCHECK(hipEventCreateWithFlags(&D.ev_up, hipEventDisableTiming));
+  HIPCHECK(hipEventCreateWithFlags(&D.ev_rhs, hipEventDisableTiming));
   HIPCHECK(hipEventRecord(ev[0], st));
   launch_zero_tiles(st, pool, D.clear_ids.p, (int)D.clear_ids.n);
   launch_scatter_sub(st, pool, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p,
@@ -907,6 +911,16 @@ void device_setup_begin(Solver &S) {
   HIPCHECK(hipEventRecord(ev[2], st));
   HIPCHECK(hipGetLastError());
   S.t.plan += wall_time() - t0;
+}
+
+// condensation rhs into the device-ordered subdomain vectors (padding 0),
+// forward sweep over the interior factor, sum of the interface parts into g
+static void reduced_rhs_launches(Solver &S, hipStream_t st) {
+  Device &D = *S.dev;
+  HIPCHECK(hipMemsetAsync(D.vec.p, 0, D.VT * sizeof(double), st));
+  launch_to_device_order(st, D.odesc.p, S.nsub, D.sc_perm.p, D.fmod.p, D.vec.p, D.maxndofs);
+  launch_trsv_fwd(st, D.pool.p, D.trsv.p, S.nsub, D.rptr.p, D.rk.p, D.rt.p, D.vec.p, D.maxnt);
+  launch_assemble(st, S.nig, D.aptr.p, D.rhs_src.p, nullptr, D.vec.p, D.g.p);
 }
 
 void device_setup_finish(Solver &S) {
@@ -1346,6 +1360,12 @@ void device_setup_finish(Solver &S) {
   D.plan2.commit(st);  // every array of the second half in one copy
   ph("coarse lists, vectors");
   HIPCHECK(hipEventRecord(D.ev_up, D.up));
+  // the reduced right-hand side needs only the interior factor (event 2):
+  // on the upload stream, concurrent with the interface phase below
+  HIPCHECK(hipStreamWaitEvent(D.up, D.ev[2], 0));
+  reduced_rhs_launches(S, D.up);
+  HIPCHECK(hipEventRecord(D.ev_rhs, D.up));
+  D.rhs_pending = true;
   S.t.plan += wall_time() - t0;
   // ---- numerics, second half: augmented interface matrices, Z, Phi, coarse
   t0 = wall_time();
@@ -1559,14 +1579,13 @@ void device_setup_finish(Solver &S) {
 // ------------------------------------------------------------- right-hand side
 void device_reduced_rhs(Solver &S) {
   Device &D = *S.dev;
-  hipStream_t st = D.st;
-  // condensation rhs into the device-ordered subdomain vectors (padding 0),
-  // forward sweep over the interior factor, sum of the interface parts
-  HIPCHECK(hipMemsetAsync(D.vec.p, 0, D.VT * sizeof(double), st));
-  launch_to_device_order(st, D.odesc.p, S.nsub, D.sc_perm.p, D.fmod.p, D.vec.p, D.maxndofs);
-  launch_trsv_fwd(st, D.pool.p, D.trsv.p, S.nsub, D.rptr.p, D.rk.p, D.rt.p, D.vec.p, D.maxnt);
-  launch_assemble(st, S.nig, D.aptr.p, D.rhs_src.p, nullptr, D.vec.p, D.g.p);
-  HIPCHECK(hipStreamSynchronize(st));
+  if (D.rhs_pending) {  // launched by the setup on the upload stream
+    HIPCHECK(hipStreamWaitEvent(D.st, D.ev_rhs, 0));
+    D.rhs_pending = false;
+  } else {
+    reduced_rhs_launches(S, D.st);
+  }
+  HIPCHECK(hipStreamSynchronize(D.st));
 }
 
 // new condensation rhs of every subdomain (after new right-hand sides or
@@ -1786,6 +1805,8 @@ void device_link_levels(Solver &S) {
   D.l2pos.upload(pos, D.st);
   HIPCHECK(hipStreamSynchronize(D.st));
   HIPCHECK(hipStreamSynchronize(E.st));
+  HIPCHECK(hipStreamSynchronize(E.up));  // its own reduced rhs (unused) is done
+  E.rhs_pending = false;
   if (!E.borrowed_st) HIPCHECK(hipStreamDestroy(E.st));
   E.st = D.st;
   E.borrowed_st = true;

@@ -578,6 +578,10 @@ int bddcml_amd_set_option(const char *key, double value) try {
     S.plan_only = value != 0.0;
     return 0;
   }
+  if (!std::strcmp(key, "explicit_inverse")) {
+    S.explicit_inverse = value != 0.0;
+    return 0;
+  }
   if (!std::strcmp(key, "precision")) {
     if (value != 32.0 && value != 64.0) return -2;
     S.precision = (int)value;

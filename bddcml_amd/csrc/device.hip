@@ -228,8 +228,9 @@ struct Device {
   DBuf<int> err;
   std::vector<SubTiles> sub;
   // setup phases
-  std::vector<Phase> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd;
-  Phase a_schur, cinv, phi;
+  std::vector<Phase> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd, w_rows;
+  Phase a_schur, cinv, phi, kinv;
+  bool explicit_inv = false;  // S'^{-1} formed (W = L^{-1}, W^T W) and applied by a GEMV
   std::vector<SeqPhase> cseq;  // coarse factorization and inverse
   Arena plan;  // task lists, scatter tables and job lists of the first half
   View<int> sc_perm, sc_ttab, sc_pads;  // shared per class of identically ordered subdomains
@@ -254,6 +255,7 @@ struct Device {
   bool kapply_deep = false;  // form of the local preconditioner launch
   int64_t LV = 0;
   View<SymvCol> scols;
+  View<SymvCol> pcols;  // the same columns over the explicit S'^{-1} tiles
   View<SymvRow> srows;
   DBuf<double> spart;  // partial slots of the symmetric S GEMV
   DBuf<double> cpart;  // partial slots of the coarse GEMV
@@ -408,22 +410,27 @@ static void assign_tiles(SubTiles &T, int &next) {
 
 // Factor space of one subdomain relative to its first tile: P (S' factor,
 // packed lower), LinvP (inverse diagonal tiles), Y (Z = S'^{-1} C^T, nti x
-// ncT), Gt (G^{-1}, ncT x ncT), Phi (nti x ncT)
+// ncT), Gt (G^{-1}, ncT x ncT), Phi (nti x ncT); with the explicit inverse,
+// W (the strictly lower tiles of L^{-1}, packed by columns), after which P
+// holds S'^{-1} = W^T W (lower tiles)
 struct FactorLayout {
-  int p, linv, y, gt, phi, span;
-  FactorLayout(int nti, int ncT) {
+  int p, linv, y, gt, phi, w, span;
+  FactorLayout(int nti, int ncT, bool explicit_inv) {
     p = 0;
     linv = nti * (nti + 1) / 2;
     y = linv + nti;
     gt = y + nti * ncT;
     phi = gt + ncT * ncT;
-    span = phi + nti * ncT;
+    w = phi + nti * ncT;
+    span = w + (explicit_inv ? nti * (nti - 1) / 2 : 0);
   }
+  // strictly lower tile (I, J), I > J, of W
+  int W(int n, int I, int J) const { return w + J * (n - 1) - J * (J - 1) / 2 + (I - J - 1); }
 };
 
-static void assign_factor(SubTiles &T, int nc, int64_t &nf) {
+static void assign_factor(SubTiles &T, int nc, int64_t &nf, bool explicit_inv) {
   T.ncT = (nc + TILE - 1) / TILE;
-  const FactorLayout L(T.nti, T.ncT);
+  const FactorLayout L(T.nti, T.ncT, explicit_inv);
   T.fbase = (int)nf;
   T.fspan = L.span;
   T.pbase = T.fbase + L.p;
@@ -944,7 +951,8 @@ void device_setup_finish(Solver &S) {
   // mirror
   {
     int64_t nf = 0;
-    for (int is = 0; is < nsub; is++) assign_factor(D.sub[is], S.subs[is].nconstr, nf);
+    D.explicit_inv = S.explicit_inverse != 0;
+    for (int is = 0; is < nsub; is++) assign_factor(D.sub[is], S.subs[is].nconstr, nf, D.explicit_inv);
     if (nf >= INT32_MAX) fatal("BDDCML_DEVICE_PLAN", "too many factor tiles");
     D.ntilesf = nf;
     D.dense_coarse = !has_level2(S);
@@ -979,15 +987,15 @@ void device_setup_finish(Solver &S) {
       fcls[is] = it->second;
     }
     struct FactorTasks {
-      std::vector<TaskList> p_diag, p_off, z_fwd, z_bwd;
-      TaskList phi;
+      std::vector<TaskList> p_diag, p_off, z_fwd, z_bwd, w_rows;
+      TaskList phi, kinv;
     };
     std::vector<FactorTasks> fts(fkeys.size());
     int bad_ids = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad_ids)
     for (int k = 0; k < (int)fkeys.size(); k++) {
       const int n = fkeys[k].first, ncT = fkeys[k].second;
-      const FactorLayout L(n, ncT);
+      const FactorLayout L(n, ncT, D.explicit_inv);
       FactorTasks &Q = fts[k];
       auto Pt = [&](int I, int J) { return L.p + lowidx(n, I, J); };
       auto Yt = [&](int I, int cT) { return L.y + I * ncT + cT; };
@@ -1018,10 +1026,36 @@ void device_setup_finish(Solver &S) {
           t.p1 = (int)Q.phi.pairs.size();
           Q.phi.tasks.push_back(t);
         }
-      for (auto *v : {&Q.p_diag, &Q.p_off, &Q.z_fwd, &Q.z_bwd})
+      if (D.explicit_inv && n > 0) {
+        // W = L^{-1} by rows: W_IJ = -Linv_I sum_{J <= K < I} L_IK W_KJ with
+        // W_JJ = Linv_J (one launch per row I, tasks start from zero);
+        // then S'^{-1} = W^T W into the P tiles: (I, J) = sum_{K >= I} W_KI^T W_KJ
+        auto Wf = [&](int K, int J) { return K == J ? L.linv + J : L.W(n, K, J); };
+        Q.w_rows.resize(n);
+        for (int I = 1; I < n; I++) {
+          TaskList &R = Q.w_rows[I];
+          for (int J = 0; J < I; J++) {
+            TileTask t{L.W(n, I, J), L.linv + I, (int)R.pairs.size(), 0};
+            for (int K = J; K < I; K++) R.pairs.push_back({Pt(I, K), Wf(K, J)});
+            t.p1 = (int)R.pairs.size();
+            t.fresh = 1;
+            R.tasks.push_back(t);
+          }
+        }
+        for (int J = 0; J < n; J++)
+          for (int I = J; I < n; I++) {
+            TileTask t{Pt(I, J), -1, (int)Q.kinv.pairs.size(), 0};
+            for (int K = I; K < n; K++) Q.kinv.pairs.push_back({Wf(K, I), Wf(K, J)});
+            t.p1 = (int)Q.kinv.pairs.size();
+            t.fresh = 1;
+            Q.kinv.tasks.push_back(t);
+          }
+      }
+      for (auto *v : {&Q.p_diag, &Q.p_off, &Q.z_fwd, &Q.z_bwd, &Q.w_rows})
         for (TaskList &T : *v)
           if (!relative_ids(T, 0, L.span)) bad_ids |= 1;
       if (!relative_ids(Q.phi, 0, L.span)) bad_ids |= 1;
+      if (!relative_ids(Q.kinv, 0, L.span)) bad_ids |= 1;
     }
     if (bad_ids) fatal("BDDCML_DEVICE_PLAN", "tile task outside its subdomain's factor tiles");
     for (int is = 0; is < nsub; is++)
@@ -1034,15 +1068,21 @@ void device_setup_finish(Solver &S) {
                           z_bwd = merge_steps(fts, &FactorTasks::z_bwd, fcls, fbase_of);
     TaskList phi = materialize(
         fts, [](const FactorTasks &Q) -> const TaskList * { return &Q.phi; }, fcls, fbase_of);
+    std::vector<TaskList> w_rows = merge_steps(fts, &FactorTasks::w_rows, fcls, fbase_of);
+    TaskList kinv = materialize(
+        fts, [](const FactorTasks &Q) -> const TaskList * { return &Q.kinv; }, fcls, fbase_of);
     S.t.interface_flops = tile_work(p_diag, 2 * kTileChol) + tile_work(p_off, kTileGemm) +
                           tile_work(z_fwd, kTileGemm) + tile_work(z_bwd, kTileGemm) +
-                          kTileGemm * tile_products(phi);
+                          kTileGemm * tile_products(phi) + tile_work(w_rows, kTileGemm) +
+                          kTileGemm * tile_products(kinv);
     D.plan2.verbose = S.verbose;
     upload_phases(D.p_diag, p_diag, D.plan2);
     upload_phases(D.p_off, p_off, D.plan2);
     upload_phases(D.z_fwd, z_fwd, D.plan2);
     upload_phases(D.z_bwd, z_bwd, D.plan2);
     D.phi.upload(phi, D.plan2);
+    upload_phases(D.w_rows, w_rows, D.plan2);
+    D.kinv.upload(kinv, D.plan2);
   }
   {
     // S' = S + C^T W C: one job per lower tile, into the factor space
@@ -1214,6 +1254,14 @@ void device_setup_finish(Solver &S) {
     std::stable_sort(sc.begin(), sc.end(), [](const SymvCol &a, const SymvCol &b) { return a.nt > b.nt; });
     for (const SymvCol &c : sc)
       if (c.tile0 < 0 || (int64_t)c.tile0 + c.nt > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid GEMV column");
+    if (D.explicit_inv) {
+      std::vector<SymvCol> pc(sc);
+      for (SymvCol &c : pc) {
+        c.tile0 = D.sub[c.mat].pbase + lowidx(D.sub[c.mat].nti, c.J, c.J);
+        if (c.tile0 < 0 || (int64_t)c.tile0 + c.nt > D.ntilesf) fatal("BDDCML_DEVICE_PLAN", "invalid GEMV column");
+      }
+      D.plan2.add(D.pcols, pc);
+    }
     D.spart.alloc((size_t)std::max<int64_t>(ps, 1) * 64);
   }
   // coarse inverse GEMV over the full-storage tiles
@@ -1405,6 +1453,19 @@ void device_setup_finish(Solver &S) {
   launch_ginv(st, D.desc.p, nsub, D.G.p, nullptr, D.err.p);
   if (D.mixed) phi_phase(D.pool32.p);
   else phi_phase(D.fpool.p);
+  // explicit inverse: W = L^{-1} row by row, then S'^{-1} = W^T W over the
+  // factor tiles (the factor itself is not needed any more)
+  auto inverse_phase = [&](auto *fp) {
+    for (size_t i = 0; i < D.w_rows.size(); i++)
+      launch_tile_gemm(st, fp, D.w_rows[i].tasks.p, D.w_rows[i].pairs.p, D.w_rows[i].ntask, GEMM_NN,
+                       POST_LEFT_X, 1.0, -1.0);
+    launch_tile_gemm(st, fp, D.kinv.tasks.p, D.kinv.pairs.p, D.kinv.ntask, GEMM_TN, POST_NONE, 1.0,
+                     1.0);
+  };
+  if (D.explicit_inv) {
+    if (D.mixed) inverse_phase(D.pool32.p);
+    else inverse_phase(D.fpool.p);
+  }
   launch_acs(st, D.desc.p, nsub, D.cptr.p, D.G.p, D.Acs.p);
   HIPCHECK(hipEventRecord(ev[3], st));
   // coarse problem (with three levels the second level is set up instead):
@@ -1821,8 +1882,8 @@ static void op_schur(Solver &S, const double *x, double *y, bool timed = false) 
     HIPCHECK(hipEventCreate(&e1));
     HIPCHECK(hipEventRecord(e0, D.st));
   }
-  launch_symv(D.st, D.pool.p, D.scols.p, D.nscols, D.srows.p, D.nsrows, D.loff.p, D.map.p, x,
-              D.spart.p, D.yl.p);
+  launch_symv(D.st, (const double *)D.pool.p, D.scols.p, D.nscols, D.srows.p, D.nsrows, D.loff.p,
+              D.map.p, nullptr, x, D.spart.p, D.yl.p);
   if (timed) {
     HIPCHECK(hipEventRecord(e1, D.st));
     HIPCHECK(hipEventSynchronize(e1));
@@ -1877,12 +1938,23 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed) {
     HIPCHECK(hipEventCreate(&e1));
     HIPCHECK(hipEventRecord(e0, st));
   }
+  // explicit inverse: yl = S'^{-1} (w .* r) by the symmetric tile GEMV
+  const double *ysym = nullptr;
+  if (D.explicit_inv) {
+    if (D.mixed)
+      launch_symv(st, (const float *)D.pool32.p, D.pcols.p, D.nscols, D.srows.p, D.nsrows,
+                  D.loff.p, D.map.p, D.w.p, r, D.spart.p, D.yl.p);
+    else
+      launch_symv(st, (const double *)D.fpool.p, D.pcols.p, D.nscols, D.srows.p, D.nsrows,
+                  D.loff.p, D.map.p, D.w.p, r, D.spart.p, D.yl.p);
+    ysym = D.yl.p;
+  }
   if (D.mixed)
     launch_kapply(st, (const float *)D.pool32.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p,
-                  D.Z.p, D.cmap.p, D.uc.p, D.zl.p, D.maxNI, D.maxnc, D.kapply_deep);
+                  D.Z.p, D.cmap.p, D.uc.p, ysym, D.zl.p, D.maxNI, D.maxnc, D.kapply_deep);
   else
     launch_kapply(st, (const double *)D.fpool.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p,
-                  D.Z.p, D.cmap.p, D.uc.p, D.zl.p, D.maxNI, D.maxnc, D.kapply_deep);
+                  D.Z.p, D.cmap.p, D.uc.p, ysym, D.zl.p, D.maxNI, D.maxnc, D.kapply_deep);
   if (timed) {
     HIPCHECK(hipEventRecord(e1, st));
     HIPCHECK(hipEventSynchronize(e1));
@@ -1897,7 +1969,9 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed) {
       zb += 2.0 * d.NI * d.nc;
     }
     const double fe = D.mixed ? 4.0 : 8.0;  // factor element size
-    S.t.kapply_bytes = 2.0 * tiles * TILE_ELEMS * fe + 8.0 * zb + 28.0 * (double)D.LV;
+    // the explicit inverse is read once (symmetric GEMV), the factor twice
+    const double passes = D.explicit_inv ? 1.0 : 2.0;
+    S.t.kapply_bytes = passes * tiles * TILE_ELEMS * fe + 8.0 * zb + 28.0 * (double)D.LV;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }

@@ -163,9 +163,10 @@ struct SymvRow {
   int psub;   // partial slot of the subdomain's (0, 0)
   int nti;
 };
-void launch_symv(hipStream_t st, const double *pool, const SymvCol *cols, int ncols,
+template <typename T>
+void launch_symv(hipStream_t st, const T *pool, const SymvCol *cols, int ncols,
                  const SymvRow *rows, int nrows, const int64_t *loff, const int *map,
-                 const double *xin, double *part, double *yout);
+                 const double *wts, const double *xin, double *part, double *yout);
 // y = A x for a dense nt x nt tile matrix in full storage (tile (I, J) at
 // base + I*nt + J); part holds nt*nt*64 doubles (coarse inverse)
 void launch_dense_gemv(hipStream_t st, const double *pool, int base, int nt, const double *x,
@@ -183,11 +184,13 @@ void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
 // each: x = w .* r(map); z = S'^{-1} x through the packed factor of S'
 // (forward and backward tile sweeps); zl = z + Phi (uc(indrowc) - Z^T x).
 // deep: more loads in flight per workgroup at lower occupancy, for few or
-// uneven subdomains (same arithmetic, same results).
+// uneven subdomains (same arithmetic, same results).  ysym (may be null):
+// S'^{-1} x already applied by an explicit inverse; the sweeps are skipped.
 template <typename T>
 void launch_kapply(hipStream_t st, const T *pool, const SubDesc *subs, int nsub, const int *map,
                    const double *w, const double *r, const double *Phi, const double *Z,
-                   const int *cmap, const double *uc, double *zl, int maxNI, int maxnc, bool deep);
+                   const int *cmap, const double *uc, const double *ysym, double *zl, int maxNI,
+                   int maxnc, bool deep);
 
 // deterministic reductions and vector updates on unique vectors
 constexpr int DOT_BLOCKS = 240;

@@ -411,10 +411,15 @@ void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int 
 // row, one coalesced 512-byte column per load), the next tile prefetched.
 // Slot (I, J), I > J, receives S_IJ x_J; slot (J, J) receives S_JJ x_J plus
 // sum_{I>J} S_IJ^T x_I (lane products summed over rows by butterflies).
-__global__ __launch_bounds__(256) void k_symv(const double *__restrict__ pool,
+// T: element type of the tiles (double: S; float or double: an explicit
+// S'^{-1} in the factor space).  wts (may be null): per local entry weight
+// applied to the gathered input.
+template <typename T>
+__global__ __launch_bounds__(256) void k_symv(const T *__restrict__ pool,
                                               const SymvCol *__restrict__ cols,
                                               const int64_t *__restrict__ loff,
                                               const int *__restrict__ map,
+                                              const double *__restrict__ wts,
                                               const double *__restrict__ xin,
                                               double *__restrict__ part) {
   __shared__ double xj[64];
@@ -426,24 +431,24 @@ __global__ __launch_bounds__(256) void k_symv(const double *__restrict__ pool,
   const int *mp = map + lo;
   if (tid < 64) {
     const int g = mp[col.J * 64 + tid];
-    xj[tid] = g >= 0 ? xin[g] : 0.0;
+    xj[tid] = g >= 0 ? (wts ? wts[lo + col.J * 64 + tid] * xin[g] : xin[g]) : 0.0;
   }
   __syncthreads();
   double xc[16], acc[16], cur[16], nxt[16];
 #pragma unroll
   for (int c = 0; c < 16; c++) xc[c] = xj[16 * w + c], acc[c] = 0.0;
-  const double *T0 = pool + (size_t)col.tile0 * TILE_ELEMS + (16 * w) * 64 + r;
+  const T *T0 = pool + (size_t)col.tile0 * TILE_ELEMS + (16 * w) * 64 + r;
 #pragma unroll
-  for (int c = 0; c < 16; c++) cur[c] = T0[c * 64];
+  for (int c = 0; c < 16; c++) cur[c] = (double)T0[c * 64];
   double xi = 0.0;  // x_I[r] of the current tile (I > J)
   for (int k = 0; k < col.nt; k++) {
     double xn = 0.0;
     if (k + 1 < col.nt) {
-      const double *Tn = T0 + (size_t)(k + 1) * TILE_ELEMS;
+      const T *Tn = T0 + (size_t)(k + 1) * TILE_ELEMS;
 #pragma unroll
-      for (int c = 0; c < 16; c++) nxt[c] = Tn[c * 64];
+      for (int c = 0; c < 16; c++) nxt[c] = (double)Tn[c * 64];
       const int g = mp[(col.J + k + 1) * 64 + r];
-      xn = g >= 0 ? xin[g] : 0.0;
+      xn = g >= 0 ? (wts ? wts[lo + (col.J + k + 1) * 64 + r] * xin[g] : xin[g]) : 0.0;
     }
     double s0 = 0.0, s1 = 0.0;
 #pragma unroll
@@ -492,14 +497,22 @@ __global__ __launch_bounds__(256) void k_symv_reduce(const SymvRow *__restrict__
   yout[loff[R.mat] + R.I * 64 + r] = s;
 }
 
-void launch_symv(hipStream_t st, const double *pool, const SymvCol *cols, int ncols,
+template <typename T>
+void launch_symv(hipStream_t st, const T *pool, const SymvCol *cols, int ncols,
                  const SymvRow *rows, int nrows, const int64_t *loff, const int *map,
-                 const double *xin, double *part, double *yout) {
+                 const double *wts, const double *xin, double *part, double *yout) {
   if (ncols <= 0 || nrows <= 0) return;
-  hipLaunchKernelGGL(k_symv, dim3(ncols), dim3(256), 0, st, pool, cols, loff, map, xin, part);
+  hipLaunchKernelGGL(k_symv<T>, dim3(ncols), dim3(256), 0, st, pool, cols, loff, map, wts, xin,
+                     part);
   hipLaunchKernelGGL(k_symv_reduce, dim3((nrows + 3) / 4), dim3(256), 0, st, rows, nrows, loff,
                      part, yout);
 }
+template void launch_symv<double>(hipStream_t, const double *, const SymvCol *, int,
+                                  const SymvRow *, int, const int64_t *, const int *,
+                                  const double *, const double *, double *, double *);
+template void launch_symv<float>(hipStream_t, const float *, const SymvCol *, int, const SymvRow *,
+                                 int, const int64_t *, const int *, const double *, const double *,
+                                 double *, double *);
 
 // y = A x for a dense matrix of nt x nt tiles in full storage (tile (I, J)
 // at base + I*nt + J), x read from global memory: one workgroup per tile
@@ -609,7 +622,7 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
     const T *__restrict__ pool, const SubDesc *__restrict__ subs, const int *__restrict__ map,
     const double *__restrict__ w, const double *__restrict__ rin, const double *__restrict__ Phi,
     const double *__restrict__ Z, const int *__restrict__ cmap, const double *__restrict__ uc,
-    double *__restrict__ zl) {
+    const double *__restrict__ ysym, double *__restrict__ zl) {
   extern __shared__ double smem[];
   const SubDesc d = subs[blockIdx.x];
   const int NI = d.NI, nc = d.nc, n = d.nti;
@@ -630,6 +643,12 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
     s = wave_sum(s);
     if (l == 0) cc[c] = uc[cmap[d.coff + c]] - s;
   }
+  if (ysym) {
+    // S'^{-1} x already applied (explicit inverse, symmetric GEMV): no sweeps
+    __syncthreads();
+    for (int i = tid; i < NI; i += KA_THR) xs[i] = ysym[d.loff + i];
+    __syncthreads();
+  } else {
   // forward sweep x_I <- Linv_I (x_I - sum_{k<I} L_Ik x_k); lane = tile row.
   // The Linv_I entries are loaded before the row sum (they do not depend on
   // it), and four tiles of the row are in flight at a time.
@@ -733,6 +752,7 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
     }
     __syncthreads();
   }
+  }
   __syncthreads();
   for (int i = tid; i < NI; i += KA_THR) {
     double s = xs[i];
@@ -762,26 +782,28 @@ void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
 template <typename T>
 void launch_kapply(hipStream_t st, const T *pool, const SubDesc *subs, int nsub, const int *map,
                    const double *w, const double *r, const double *Phi, const double *Z,
-                   const int *cmap, const double *uc, double *zl, int maxNI, int maxnc, bool deep) {
+                   const int *cmap, const double *uc, const double *ysym, double *zl, int maxNI,
+                   int maxnc, bool deep) {
   if (nsub <= 0) return;
   size_t sh = (size_t)(maxNI + 1024 + 64 + maxnc) * sizeof(double);
-  if (deep) {
+  if (deep && !ysym) {
     lds_request((const void *)k_kapply<T, true>, sh, "preconditioner apply");
     hipLaunchKernelGGL((k_kapply<T, true>), dim3(nsub), dim3(KA_THR), sh, st, pool, subs, map, w, r,
-                       Phi, Z, cmap, uc, zl);
+                       Phi, Z, cmap, uc, ysym, zl);
   } else {
     lds_request((const void *)k_kapply<T, false>, sh, "preconditioner apply");
     hipLaunchKernelGGL((k_kapply<T, false>), dim3(nsub), dim3(KA_THR), sh, st, pool, subs, map, w, r,
-                       Phi, Z, cmap, uc, zl);
+                       Phi, Z, cmap, uc, ysym, zl);
   }
 }
 template void launch_kapply<double>(hipStream_t, const double *, const SubDesc *, int, const int *,
                                     const double *, const double *, const double *,
-                                    const double *, const int *, const double *, double *, int,
-                                    int, bool);
+                                    const double *, const int *, const double *, const double *,
+                                    double *, int, int, bool);
 template void launch_kapply<float>(hipStream_t, const float *, const SubDesc *, int, const int *,
                                    const double *, const double *, const double *, const double *,
-                                   const int *, const double *, double *, int, int, bool);
+                                   const int *, const double *, const double *, double *, int, int,
+                                   bool);
 
 // ------------------------------------------------------------ vector ops
 __global__ __launch_bounds__(256) void k_dots(int n, DotArgs a, double *__restrict__ partial) {

@@ -229,6 +229,7 @@ void setup_second_level(Solver &S) {
   T.precision = 64;  // the coarse levels stay in FP64
   T.ordering = S.ordering;
   T.nd_leaf = S.nd_leaf;
+  T.explicit_inverse = S.explicit_inverse;
   T.plan_only = S.plan_only;
   T.subs.resize(T.nsub);
   int bad = 0;

@@ -58,7 +58,11 @@ int bddcml_amd_timings(double *out, int n);
  *     while the Schur operator, the Krylov method and the coarse problem
  *     stay FP64);
  *   "task_order" (1: each XCD runs one contiguous range of a tile-task
- *     launch, the default; 0: plain workgroup order) */
+ *     launch, the default; 0: plain workgroup order);
+ *   "explicit_inverse" (1: S'^{-1} is formed at setup, W = L^{-1} and
+ *     W^T W on the tile GEMM, and each preconditioner apply reads it once in
+ *     a symmetric GEMV instead of sweeping the factor twice; more setup work,
+ *     for problems with hundreds of iterations; default 0) */
 int bddcml_amd_set_option(const char *key, double value);
 /* per subdomain (isub in numbase): out[0]=ndofi out[1]=nconstr out[2]=ndofo */
 int bddcml_amd_sub_info(int isub, int *out);

@@ -216,3 +216,54 @@ def test_uneven_boxes(oracle_lib, n, ns, kind, precision):
         assert np.abs(ls - o.local_solution(d["isub"])).max() <= tol_u * np.abs(u_ref).max()
     o.close()
     b.finalize()
+
+
+@pytest.mark.parametrize("kind,precision", [("cube-poisson", 64), ("cube-elasticity", 64),
+                                            ("boxes", 64), ("cube-poisson", 32), ("boxes", 32),
+                                            ("three-levels", 64)])
+def test_explicit_inverse(oracle_lib, kind, precision):
+    """Local preconditioner through an explicit S'^{-1} = W^T W (W = L^{-1}
+    formed by tile tasks, applied by the symmetric tile GEMV) against the
+    oracle's LU solve of the augmented system: M r within 1e-9 relative in
+    FP64 (the inverse is formed, not solved, so its rounding is cond(S')
+    times that of the sweeps), solution within 1e-10 (FP64) / 1e-8 (FP32
+    factor); iterations +-1 (layered boxes: 5 % + 1 in FP64, 15 % + 2 with
+    the FP32 factor)."""
+    from bddcml_amd.cube import BoxProblem
+    _say(f"case explicit_inverse {kind} precision={precision}")
+    nsub2 = 0
+    if kind == "cube-poisson":
+        p = CubeProblem(4, 3, kind="poisson", rhs="random", seed=21)
+    elif kind == "cube-elasticity":
+        p = CubeProblem(3, 2, kind="elasticity", rhs="random", seed=22)
+    elif kind == "boxes":
+        p = BoxProblem(12, 20, seed=3, kind="elasticity", layers=3, jump=1e3, thick=1)
+    else:
+        p = CubeProblem(3, 4, kind="poisson", rhs="random", seed=23)
+        nsub2 = 8
+    tol = 1e-12
+    if nsub2:
+        from oracle.multilevel import solve_three_levels
+        t, (it_o, r_o, _) = solve_three_levels(p, nsub2, tol=tol, maxit=400)
+        o = t.o1
+    else:
+        o, (it_o, r_o, _) = _oracle(oracle_lib, p, tol=tol)
+    b, (it, r, _) = bddcml_amd.solve_cube(p, tol=tol, maxit=400, nsub2=nsub2,
+                                          options={"precision": precision, "explicit_inverse": 1})
+    _say(f"  iterations {it}, oracle {it_o}")
+    assert r == r_o == 0
+    layered = isinstance(p, BoxProblem)
+    u_ref = o.interface_solution()
+    if precision == 64:
+        x = np.random.default_rng(4).standard_normal(len(u_ref))
+        assert _relmax(b.apply_pc(x), o.apply_pc(x)) < 1e-9
+        assert abs(it - it_o) <= (max(1, 0.05 * it_o) if layered else 1)
+        assert _relmax(b.interface_solution(), u_ref) < 1e-10
+    else:
+        assert (it <= 1.15 * it_o + 2) if layered else abs(it - it_o) <= 1
+        assert _relmax(b.interface_solution(), u_ref) < 1e-8
+    b.finalize()
+    if nsub2:
+        t.close()
+    else:
+        o.close()

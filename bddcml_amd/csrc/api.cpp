@@ -579,7 +579,8 @@ int bddcml_amd_set_option(const char *key, double value) try {
     return 0;
   }
   if (!std::strcmp(key, "explicit_inverse")) {
-    S.explicit_inverse = value != 0.0;
+    if (value != 0.0 && value != 1.0 && value != 2.0) return -1;
+    S.explicit_inverse = (int)value;
     return 0;
   }
   if (!std::strcmp(key, "precision")) {

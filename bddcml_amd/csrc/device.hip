@@ -230,7 +230,12 @@ struct Device {
   // setup phases
   std::vector<Phase> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd, w_rows;
   Phase a_schur, cinv, phi, kinv;
-  bool explicit_inv = false;  // S'^{-1} formed (W = L^{-1}, W^T W) and applied by a GEMV
+  // explicit inverse of S': planned (W tiles in the factor space, task
+  // lists, GEMV columns), formed (W = L^{-1}, W^T W: the apply is a GEMV),
+  // automatic (formed once the preconditioner has been applied
+  // kInverseAfterApplies times on this setup)
+  bool inv_planned = false, explicit_inv = false, inv_auto = false;
+  int pc_count = 0;
   std::vector<SeqPhase> cseq;  // coarse factorization and inverse
   Arena plan;  // task lists, scatter tables and job lists of the first half
   View<int> sc_perm, sc_ttab, sc_pads;  // shared per class of identically ordered subdomains
@@ -920,6 +925,25 @@ void device_setup_begin(Solver &S) {
   S.t.plan += wall_time() - t0;
 }
 
+// Explicit inverse of S' on the compute stream: W = L^{-1} row by row, then
+// S'^{-1} = W^T W over the factor tiles (the factor is not needed after
+// that: later applies use the GEMV).  Launches queued behind work already on
+// the stream, which still sees the factor.
+static void form_inverse(Solver &S) {
+  Device &D = *S.dev;
+  hipStream_t st = D.st;
+  auto run = [&](auto *fp) {
+    for (size_t i = 0; i < D.w_rows.size(); i++)
+      launch_tile_gemm(st, fp, D.w_rows[i].tasks.p, D.w_rows[i].pairs.p, D.w_rows[i].ntask, GEMM_NN,
+                       POST_LEFT_X, 1.0, -1.0);
+    launch_tile_gemm(st, fp, D.kinv.tasks.p, D.kinv.pairs.p, D.kinv.ntask, GEMM_TN, POST_NONE, 1.0,
+                     1.0);
+  };
+  if (D.mixed) run(D.pool32.p);
+  else run(D.fpool.p);
+  D.explicit_inv = true;
+}
+
 // condensation rhs into the device-ordered subdomain vectors (padding 0),
 // forward sweep over the interior factor, sum of the interface parts into g
 static void reduced_rhs_launches(Solver &S, hipStream_t st) {
@@ -951,8 +975,10 @@ void device_setup_finish(Solver &S) {
   // mirror
   {
     int64_t nf = 0;
-    D.explicit_inv = S.explicit_inverse != 0;
-    for (int is = 0; is < nsub; is++) assign_factor(D.sub[is], S.subs[is].nconstr, nf, D.explicit_inv);
+    D.inv_planned = S.explicit_inverse != 0;
+    D.inv_auto = S.explicit_inverse == 2;
+    D.explicit_inv = false;
+    for (int is = 0; is < nsub; is++) assign_factor(D.sub[is], S.subs[is].nconstr, nf, D.inv_planned);
     if (nf >= INT32_MAX) fatal("BDDCML_DEVICE_PLAN", "too many factor tiles");
     D.ntilesf = nf;
     D.dense_coarse = !has_level2(S);
@@ -995,7 +1021,7 @@ void device_setup_finish(Solver &S) {
 #pragma omp parallel for schedule(dynamic) reduction(| : bad_ids)
     for (int k = 0; k < (int)fkeys.size(); k++) {
       const int n = fkeys[k].first, ncT = fkeys[k].second;
-      const FactorLayout L(n, ncT, D.explicit_inv);
+      const FactorLayout L(n, ncT, D.inv_planned);
       FactorTasks &Q = fts[k];
       auto Pt = [&](int I, int J) { return L.p + lowidx(n, I, J); };
       auto Yt = [&](int I, int cT) { return L.y + I * ncT + cT; };
@@ -1026,7 +1052,7 @@ void device_setup_finish(Solver &S) {
           t.p1 = (int)Q.phi.pairs.size();
           Q.phi.tasks.push_back(t);
         }
-      if (D.explicit_inv && n > 0) {
+      if (D.inv_planned && n > 0) {
         // W = L^{-1} by rows: W_IJ = -Linv_I sum_{J <= K < I} L_IK W_KJ with
         // W_JJ = Linv_J (one launch per row I, tasks start from zero);
         // then S'^{-1} = W^T W into the P tiles: (I, J) = sum_{K >= I} W_KI^T W_KJ
@@ -1254,7 +1280,7 @@ void device_setup_finish(Solver &S) {
     std::stable_sort(sc.begin(), sc.end(), [](const SymvCol &a, const SymvCol &b) { return a.nt > b.nt; });
     for (const SymvCol &c : sc)
       if (c.tile0 < 0 || (int64_t)c.tile0 + c.nt > D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid GEMV column");
-    if (D.explicit_inv) {
+    if (D.inv_planned) {
       std::vector<SymvCol> pc(sc);
       for (SymvCol &c : pc) {
         c.tile0 = D.sub[c.mat].pbase + lowidx(D.sub[c.mat].nti, c.J, c.J);
@@ -1453,19 +1479,7 @@ void device_setup_finish(Solver &S) {
   launch_ginv(st, D.desc.p, nsub, D.G.p, nullptr, D.err.p);
   if (D.mixed) phi_phase(D.pool32.p);
   else phi_phase(D.fpool.p);
-  // explicit inverse: W = L^{-1} row by row, then S'^{-1} = W^T W over the
-  // factor tiles (the factor itself is not needed any more)
-  auto inverse_phase = [&](auto *fp) {
-    for (size_t i = 0; i < D.w_rows.size(); i++)
-      launch_tile_gemm(st, fp, D.w_rows[i].tasks.p, D.w_rows[i].pairs.p, D.w_rows[i].ntask, GEMM_NN,
-                       POST_LEFT_X, 1.0, -1.0);
-    launch_tile_gemm(st, fp, D.kinv.tasks.p, D.kinv.pairs.p, D.kinv.ntask, GEMM_TN, POST_NONE, 1.0,
-                     1.0);
-  };
-  if (D.explicit_inv) {
-    if (D.mixed) inverse_phase(D.pool32.p);
-    else inverse_phase(D.fpool.p);
-  }
+  if (D.inv_planned && !D.inv_auto) form_inverse(S);
   launch_acs(st, D.desc.p, nsub, D.cptr.p, D.G.p, D.Acs.p);
   HIPCHECK(hipEventRecord(ev[3], st));
   // coarse problem (with three levels the second level is set up instead):
@@ -1920,10 +1934,15 @@ static void level2_apply(Solver &S, const double *rc, double *uc) {
   launch_level_gather(st, S.ndofc, D.l2pos.p, E.vec.p, uc);
 }
 
+// applies after which the automatic mode forms the explicit inverse: about
+// the number whose saving (roughly half an apply each) pays for forming it
+constexpr int kInverseAfterApplies = 40;
+
 static void op_pc(Solver &S, const double *r, double *z, bool timed) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (D.inv_auto && !D.explicit_inv && ++D.pc_count > kInverseAfterApplies) form_inverse(S);
   // coarse correction uc = Ac^{-1} sum_i R_ci^T Phi_i^T W_i r_i (three
   // levels: the second level's BDDC in place of Ac^{-1})
   launch_phit(st, D.desc.p, S.nsub, D.Phi.p, D.map.p, D.w.p, r, D.rcs.p, D.maxnc, D.maxNI);

@@ -41,9 +41,11 @@ struct Solver {
   int precision = 64;  // 64: FP64 everywhere; 32: FP32 preconditioner factors
   int ordering = 0;    // interior ordering: 0 tile-aligned nested dissection, 1 RCM
   int nd_leaf = 256;   // nested dissection: largest part (dofs) left undivided
-  // local preconditioner through an explicit S'^{-1} (formed at setup, applied
-  // by a symmetric tile GEMV) instead of two sweeps over the factor of S'
-  int explicit_inverse = 0;
+  // local preconditioner through an explicit S'^{-1} (applied by a symmetric
+  // tile GEMV) instead of two sweeps over the factor of S': 0 never, 1 formed
+  // at setup, 2 formed once the preconditioner has been applied 40 times on
+  // the same setup (the default: easy problems never pay for it)
+  int explicit_inverse = 2;
   // classes of subdomains with identical structure (ordering, symbolic)
   std::vector<int> ord_cls, ord_reps;
   // topology results

@@ -59,7 +59,7 @@ def schur_traffic(args):
     FETCH_SIZE / WRITE_SIZE summary of the default workload (null for other
     workloads or when the summary is absent)."""
     if (args.workload, args.nel, args.nsub, args.ordering, args.nd_leaf, args.levels,
-            args.explicit_inverse) != ("cube", 16, 8, 0, 256, 2, False):
+            args.explicit_inverse) != ("cube", 16, 8, 0, 256, 2, 2):
         return None
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "k_schur_gemm_128cube_pmc.md")
@@ -149,9 +149,9 @@ def main():
     ap.add_argument("--levels", type=int, default=2, choices=(2, 3),
                     help="3: a second level of --nsub2 subdomains replaces the coarse solve")
     ap.add_argument("--nsub2", type=int, default=8, help="subdomains of the second level")
-    ap.add_argument("--explicit-inverse", action="store_true",
-                    help="local preconditioner through an explicit inverse of S' (pays off "
-                         "for many iterations)")
+    ap.add_argument("--explicit-inverse", type=int, default=2, choices=(0, 1, 2),
+                    help="explicit inverse of S' for the local preconditioner: 0 never, 1 at "
+                         "setup, 2 after 40 applies (the library's default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ordering", type=int, default=0,
                     help="interior ordering: 0 tile-aligned nested dissection, 1 RCM")
@@ -183,8 +183,9 @@ def main():
         return gradient_workload(args)
     levels_txt = ("2 levels" if args.levels == 2 else
                   f"3 levels ({args.nsub2} subdomains on level 2)")
-    if args.explicit_inverse:
-        levels_txt += ", explicit inverse of S'"
+    if args.explicit_inverse != 2:
+        levels_txt += (", explicit inverse of S' at setup" if args.explicit_inverse == 1 else
+                       ", no explicit inverse of S'")
     t0 = time.perf_counter()
     if args.workload == "cube":
         p = CubeProblem(args.nel, args.nsub, rhs="random", seed=2)
@@ -214,7 +215,7 @@ def main():
             b.set_option("interior_ordering", args.ordering)
             b.set_option("nd_leaf", args.nd_leaf)
             b.set_option("precision", precision)
-            b.set_option("explicit_inverse", 1 if args.explicit_inverse else 0)
+            b.set_option("explicit_inverse", args.explicit_inverse)
             t0 = time.perf_counter()
             for d in subs:
                 b.upload_subdomain_data(d)

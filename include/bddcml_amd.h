@@ -59,10 +59,11 @@ int bddcml_amd_timings(double *out, int n);
  *     stay FP64);
  *   "task_order" (1: each XCD runs one contiguous range of a tile-task
  *     launch, the default; 0: plain workgroup order);
- *   "explicit_inverse" (1: S'^{-1} is formed at setup, W = L^{-1} and
- *     W^T W on the tile GEMM, and each preconditioner apply reads it once in
- *     a symmetric GEMV instead of sweeping the factor twice; more setup work,
- *     for problems with hundreds of iterations; default 0) */
+ *   "explicit_inverse" (S'^{-1} formed, W = L^{-1} and W^T W on the tile
+ *     GEMM, so that each preconditioner apply reads it once in a symmetric
+ *     GEMV instead of sweeping the factor twice: 0 never; 1 at setup; 2, the
+ *     default, once the preconditioner has been applied 40 times on the same
+ *     setup, where the saved apply time has about paid for forming it) */
 int bddcml_amd_set_option(const char *key, double value);
 /* per subdomain (isub in numbase): out[0]=ndofi out[1]=nconstr out[2]=ndofo */
 int bddcml_amd_sub_info(int isub, int *out);

@@ -658,6 +658,13 @@ void device_setup_begin(Solver &S) {
     const SubTiles &T = D.sub[reps[ic]];
     SubTasks &Q = sts[ic];
     const int nto = T.nto, nt = T.nto + T.nti;
+    // tile id of (I, J), I >= J, or -1: a dense table (the lookups below run
+    // once per tile pair)
+    std::vector<int> tid((size_t)nt * nt, -1);
+    for (int J = 0; J < nt; J++)
+      for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++)
+        tid[(size_t)T.pat.rowidx[q] * nt + J] = T.pat.tile[q];
+    auto find = [&](int I, int J) { return tid[(size_t)I * nt + J]; };
     // row sets restricted to factored columns
     std::vector<std::vector<int>> rowk(nt);
     for (int J = 0; J < nto; J++)
@@ -681,14 +688,14 @@ void device_setup_begin(Solver &S) {
         if (a[x] < b[y]) x++;
         else if (b[y] < a[x]) y++;
         else {
-          L.pairs.push_back({T.pat.find(I, a[x]), T.pat.find(J, b[y])});
+          L.pairs.push_back({find(I, a[x]), find(J, b[y])});
           x++, y++;
         }
       }
     };
     for (int J = 0; J < nto; J++) {
       TaskList &Dg = Q.a_diag[lvl[J]], &Of = Q.a_off[lvl[J]];
-      TileTask t{T.pat.find(J, J), T.linvA + J, (int)Dg.pairs.size(), 0};
+      TileTask t{find(J, J), T.linvA + J, (int)Dg.pairs.size(), 0};
       common(J, J, J, Dg);
       t.p1 = (int)Dg.pairs.size();
       Dg.tasks.push_back(t);
@@ -703,7 +710,7 @@ void device_setup_begin(Solver &S) {
     }
     for (int J = nto; J < nt; J++)
       for (int I = J; I < nt; I++) {
-        TileTask t{T.pat.find(I, J), -1, (int)Q.a_schur.pairs.size(), 0};
+        TileTask t{find(I, J), -1, (int)Q.a_schur.pairs.size(), 0};
         common(I, J, nto, Q.a_schur);
         t.p1 = (int)Q.a_schur.pairs.size();
         if (t.p1 > t.p0) Q.a_schur.tasks.push_back(t);
@@ -718,7 +725,7 @@ void device_setup_begin(Solver &S) {
       std::vector<char> entry(npat, 0), written(npat, 0);
       auto mark = [&](int a, int b) {
         const int I = std::max(a, b) / TILE, J = std::min(a, b) / TILE;
-        const int q = T.pat.find(I, J) - T.base;
+        const int q = find(I, J) - T.base;
         if (q < 0 || q >= npat) bad_ids |= 1;
         else entry[q] = 1;
       };

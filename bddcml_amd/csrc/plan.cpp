@@ -168,8 +168,6 @@ static void neighbours_of(Solver &S, int is) {
   if (!s.adj.empty()) s.shptr.push_back((int)s.shnode.size());
 }
 
-static bool same_key(const Sub &s, int a, int b) { return s.nodekey[a] == s.nodekey[b]; }
-
 // glob classification and face-triangle corner selection
 // (src/module_dd.f90:9130-9300 and 10017-10327)
 static void globs_first_pass(Solver &S, int is) {

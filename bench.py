@@ -33,12 +33,13 @@ def log(msg):
 
 def cpu_baseline():
     """Oracle (CPU FP64 restatement) on a bounded sample of the same
-    workload: 3x3x3 subdomains of 16^3 elements (48^3 mesh, 117,649 dofs),
-    the subdomain size of the 128^3 case, solved to 1e-8."""
+    workload: 5x5x5 subdomains of 16^3 elements (80^3 mesh, 531,441 dofs),
+    the subdomain size of the 128^3 case, solved to 1e-8 (10-20 s of CPU
+    work on the host cores)."""
     from oracle import oracle as orc
     from bddcml_amd.cube import CubeProblem
     orc.build()
-    p = CubeProblem(16, 3, rhs="random", seed=2)
+    p = CubeProblem(16, 5, rhs="random", seed=2)
     subs = list(p.subdomains())
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     o = orc.Oracle(p.nsub, p.nnod, p.ndof, 3, 3, p.numbase)
@@ -50,7 +51,7 @@ def cpu_baseline():
     dt = time.perf_counter() - t0
     o.close()
     return dict(value=p.ndof / dt, unit="dofs/s", cores=threads, kind="port",
-                sample=f"oracle setup+PCG to 1e-8 on 48^3 Q1 Poisson, 27 subdomains of 16^3 "
+                sample=f"oracle setup+PCG to 1e-8 on 80^3 Q1 Poisson, 125 subdomains of 16^3 "
                        f"({p.ndof} dofs, {it} iterations, {dt:.2f} s, OpenMP over subdomains)")
 
 

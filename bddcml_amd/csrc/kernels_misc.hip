@@ -8,11 +8,26 @@
 namespace bddc {
 
 // --------------------------------------------------------- S' = S + rho C^T C
+__device__ __forceinline__ void store2(float *p, double x, double y) {
+  *reinterpret_cast<float2 *>(p) = make_float2((float)x, (float)y);
+}
+__device__ __forceinline__ void store2(double *p, double x, double y) {
+  *reinterpret_cast<double2 *>(p) = make_double2(x, y);
+}
+
+// One job per lower tile (I, J): thread t handles rows 2(t%32), 2(t%32)+1
+// of the columns t/32 + 8k (16-byte loads); the constraint ids of the rows
+// are loop invariants, those of the columns are staged in LDS.
 template <typename T>
-__global__ void k_sprime(const double *__restrict__ pool, T *__restrict__ ppool,
-                         const SubDesc *__restrict__ subs, const int *__restrict__ jsub,
-                         const int *__restrict__ jij, const int *__restrict__ jdst,
-                         const int *__restrict__ cid, const int *__restrict__ csize) {
+__global__ __launch_bounds__(256) void k_sprime(const double *__restrict__ pool,
+                                                T *__restrict__ ppool,
+                                                const SubDesc *__restrict__ subs,
+                                                const int *__restrict__ jsub,
+                                                const int *__restrict__ jij,
+                                                const int *__restrict__ jdst,
+                                                const int *__restrict__ cid,
+                                                const int *__restrict__ csize) {
+  __shared__ int ccol[64];
   const int job = blockIdx.x;
   const SubDesc d = subs[jsub[job]];
   const int I = jij[job] >> 16, J = jij[job] & 0xffff;
@@ -21,12 +36,23 @@ __global__ void k_sprime(const double *__restrict__ pool, T *__restrict__ ppool,
       pool + (size_t)(d.slow + J * d.nti - J * (J - 1) / 2 + (I - J)) * TILE_ELEMS;
   T *dst = ppool + (size_t)jdst[job] * TILE_ELEMS;
   const int *ci = cid + d.loff;
-  for (int e = threadIdx.x; e < TILE_ELEMS; e += blockDim.x) {
-    int r = I * 64 + (e & 63), c = J * 64 + (e >> 6);
-    double v = src[e];
-    int a = ci[r], b = ci[c];
-    if (a >= 0 && a == b) v += d.rho / (double)csize[a];
-    dst[e] = (T)v;
+  const int tid = threadIdx.x;
+  if (tid < 64) ccol[tid] = ci[J * 64 + tid];
+  __syncthreads();
+  const int r0 = 2 * (tid & 31), c0 = tid >> 5;
+  const int a0 = ci[I * 64 + r0], a1 = ci[I * 64 + r0 + 1];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int c = c0 + 8 * k;
+    const double2 v = *reinterpret_cast<const double2 *>(src + c * 64 + r0);
+    double x = v.x, y = v.y;
+    const int b = ccol[c];
+    if (b >= 0 && (a0 == b || a1 == b)) {
+      const double add = d.rho / (double)csize[b];
+      if (a0 == b) x += add;
+      if (a1 == b) y += add;
+    }
+    store2(dst + c * 64 + r0, x, y);
   }
 }
 

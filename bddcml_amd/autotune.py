@@ -1,0 +1,119 @@
+"""Autotuner for the launch-plan options of one problem.
+
+The device path has a few knobs that change how the work is cut into
+launches, not what is computed: the largest part that nested dissection
+leaves whole (``nd_leaf``: the tile structure of the interior factors and
+the number of elimination levels, hence of launches), the interior ordering,
+the XCD task order of the tile launches, and when the explicit inverse of the
+local operator is formed.  Which setting is fastest depends on the mesh, the
+subdomain sizes and the iteration count, so this module times setup + solve
+for a set of candidate option sets on the caller's own subdomains and ranks
+them.  Every candidate solves the same problem; the solutions must agree
+(they differ only by rounding), which the tuner checks.
+
+    from bddcml_amd.autotune import autotune
+    ranking = autotune(subdomains, nsub, candidates=None)   # default grid
+    best_options = ranking[0]["options"]
+
+``python -m bddcml_amd.autotune --nel 16 --nsub 4`` tunes a cube problem.
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import time
+
+import numpy as np
+
+from . import Bddcml
+
+# option sets tried by default: nested-dissection leaf size x task order x
+# explicit-inverse mode (the ordering stays nested dissection, RCM is slower
+# for every mesh measured)
+DEFAULT_GRID = {
+    "nd_leaf": (128, 256, 512),
+    "task_order": (1, 0),
+    "explicit_inverse": (2, 0),
+}
+
+
+def candidate_sets(grid: dict | None = None) -> list[dict]:
+    """Cartesian product of a {option: values} grid as a list of dicts."""
+    grid = grid or DEFAULT_GRID
+    keys = list(grid)
+    return [dict(zip(keys, vals)) for vals in itertools.product(*(grid[k] for k in keys))]
+
+
+def _run(subdomains, nsub, numbase, options, levels, nsub2, setup_args, solve_args):
+    b = Bddcml()
+    b.init(levels, [nsub, 1] if levels == 2 else [nsub, nsub2, 1], numbase=numbase)
+    for k, v in options.items():
+        if b.set_option(k, v) != 0:
+            b.finalize()
+            raise ValueError(f"option {k}={v} rejected")
+    for d in subdomains:
+        b.upload_subdomain_data(d)
+    t0 = time.perf_counter()
+    b.setup_preconditioner(*setup_args)
+    t1 = time.perf_counter()
+    it, reason, cond = b.solve(*solve_args)
+    t2 = time.perf_counter()
+    u = b.interface_solution()
+    b.finalize()
+    return dict(setup_s=t1 - t0, solve_s=t2 - t1, total_s=t2 - t0, iterations=it,
+                reason=reason), u
+
+
+def autotune(subdomains, nsub, numbase=1, candidates=None, repeats=2, levels=2, nsub2=0,
+             setup_args=(1, 0, 1, 1, 1, 0, 0, 0), solve_args=(0, 1e-8, 1000, 100),
+             agree=1e-6, verbose=False):
+    """Times setup + solve for every candidate option set (best of `repeats`
+    runs each, after one untimed run of the first candidate that pays the
+    one-time costs) and returns the candidates sorted by time:
+    [{"options": {...}, "total_s", "setup_s", "solve_s", "iterations"}, ...].
+    Raises if a candidate does not converge or its interface solution differs
+    from the first candidate's by more than `agree` (relative)."""
+    subdomains = list(subdomains)
+    candidates = candidates if candidates is not None else candidate_sets()
+    if not candidates:
+        raise ValueError("no candidates")
+    _run(subdomains, nsub, numbase, candidates[0], levels, nsub2, setup_args, solve_args)
+    out, u_ref = [], None
+    for opts in candidates:
+        best = None
+        for _ in range(max(1, repeats)):
+            r, u = _run(subdomains, nsub, numbase, opts, levels, nsub2, setup_args, solve_args)
+            if r["reason"] != 0:
+                raise RuntimeError(f"options {opts}: the solve did not converge ({r['reason']})")
+            if u_ref is None:
+                u_ref = u
+            err = np.abs(u - u_ref).max() / max(np.abs(u_ref).max(), 1e-300)
+            if err > agree:
+                raise RuntimeError(f"options {opts}: solution differs by {err:.2e}")
+            if best is None or r["total_s"] < best["total_s"]:
+                best = r
+        best = dict(options=dict(opts), **{k: best[k] for k in ("total_s", "setup_s", "solve_s",
+                                                                 "iterations")})
+        if verbose:
+            print(json.dumps(best), flush=True)
+        out.append(best)
+    out.sort(key=lambda r: r["total_s"])
+    return out
+
+
+def main():
+    from .cube import CubeProblem
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--nel", type=int, default=16, help="elements per subdomain edge")
+    ap.add_argument("--nsub", type=int, default=4, help="subdomains per cube edge")
+    ap.add_argument("--kind", default="poisson", choices=("poisson", "elasticity"))
+    ap.add_argument("--repeats", type=int, default=2)
+    args = ap.parse_args()
+    p = CubeProblem(args.nel, args.nsub, kind=args.kind, rhs="random", seed=2)
+    ranking = autotune(p.subdomains(), p.nsub, p.numbase, repeats=args.repeats, verbose=True)
+    print("best:", json.dumps(ranking[0]))
+
+
+if __name__ == "__main__":
+    main()

@@ -168,3 +168,14 @@ print(repr(init(3, [8, 16, 1])))
     assert "two and three levels" in lines[0]
     assert "last level must be 1" in lines[1]
     assert "level 2" in lines[2]
+
+
+def test_autotune_candidate_grid():
+    from bddcml_amd.autotune import DEFAULT_GRID, candidate_sets
+    c = candidate_sets()
+    n = 1
+    for v in DEFAULT_GRID.values():
+        n *= len(v)
+    assert len(c) == n and len({tuple(sorted(d.items())) for d in c}) == n
+    assert candidate_sets({"nd_leaf": (64,), "task_order": (0, 1)}) == [
+        {"nd_leaf": 64, "task_order": 0}, {"nd_leaf": 64, "task_order": 1}]

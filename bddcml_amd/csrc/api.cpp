@@ -49,6 +49,7 @@ void bddcml_init_c(int *nl, int *nsublev, int *lnsublev, int *nsub_loc_1, int *c
   (void)just_direct_solve_int;
   reset_solver();
   set_task_order(1);
+  set_trsv_global(false);
   Solver &S = solver();
   if (*nl < 2) fatal("LEVELS_INIT", "Number of levels must be at least 2");
   if (*nl > 3) fatal("LEVELS_INIT", "This library implements two and three levels");
@@ -568,6 +569,10 @@ int bddcml_amd_set_option(const char *key, double value) try {
   }
   if (!std::strcmp(key, "verbose")) {
     S.verbose = (int)value;
+    return 0;
+  }
+  if (!std::strcmp(key, "sweep_in_global")) {
+    set_trsv_global(value != 0.0);
     return 0;
   }
   if (!std::strcmp(key, "task_order")) {

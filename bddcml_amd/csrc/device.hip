@@ -1387,8 +1387,8 @@ void device_setup_finish(Solver &S) {
           rkv[k] = J, rtv[k] = T.pat.tile[q];
         }
     }
-    if ((size_t)(maxnt * 64 + 320) * sizeof(double) > max_dynamic_lds())
-      fatal("BDDCML_SETUP_PRECONDITIONER", "subdomain larger than the triangular sweep supports");
+    // (subdomains whose vector does not fit in LDS are swept with it in
+    // global memory, see launch_trsv_fwd)
     for (int id : rtv)
       if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid sweep tile");
     for (int id : ctv)

@@ -266,22 +266,26 @@ __device__ __forceinline__ double row_reduce(double s, double *red) {
   return v;
 }
 
-template <int NW>
+// GV: the subdomain vector stays in global memory (L2-cached) instead of
+// LDS, for subdomains whose vector does not fit there; rows written by one
+// wave are published to the others with a workgroup fence before the barrier.
+template <int NW, bool GV>
 __global__ __launch_bounds__(64 * NW) void k_trsv_fwd(const double *__restrict__ pool,
                                                   const TrsvDesc *__restrict__ ds,
                                                   const int *__restrict__ rptr,
                                                   const int *__restrict__ rk,
                                                   const int *__restrict__ rt,
-                                                  double *__restrict__ vec) {
+                                                  double *vec) {
   extern __shared__ double smem[];
   const TrsvDesc d = ds[blockIdx.x];
   const int nt = d.nto + d.nti;
-  double *v = smem;              // nt*64
-  double *red = smem + nt * 64;  // NW*64
-  double *t = red + NW * 64;     // 64
   double *gv = vec + d.voff;
+  double *v = GV ? gv : smem;                    // nt*64
+  double *red = GV ? smem : smem + nt * 64;      // NW*64
+  double *t = red + NW * 64;                     // 64
   const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
-  for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
+  if (!GV)
+    for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
   __syncthreads();
   for (int I = 0; I < nt; I++) {
     double s = 0.0;
@@ -304,28 +308,31 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_fwd(const double *__restrict__
     } else if (threadIdx.x < 64) {
       v[I * 64 + r] = t[r];
     }
+    if (GV) __threadfence_block();
     __syncthreads();
   }
-  for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) gv[k] = v[k];
+  if (!GV)
+    for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) gv[k] = v[k];
 }
 
-template <int NW>
+template <int NW, bool GV>
 __global__ __launch_bounds__(64 * NW) void k_trsv_bwd(const double *__restrict__ pool,
                                                   const TrsvDesc *__restrict__ ds,
                                                   const int *__restrict__ cptr,
                                                   const int *__restrict__ ck,
                                                   const int *__restrict__ ct,
-                                                  double *__restrict__ vec) {
+                                                  double *vec) {
   constexpr int CW = 64 / NW;  // tile columns per wave
   extern __shared__ double smem[];
   const TrsvDesc d = ds[blockIdx.x];
   const int nt = d.nto + d.nti;
-  double *v = smem;
-  double *red = smem + nt * 64;
-  double *t = red + NW * 64;
   double *gv = vec + d.voff;
+  double *v = GV ? gv : smem;
+  double *red = GV ? smem : smem + nt * 64;
+  double *t = red + NW * 64;
   const int g = threadIdx.x >> 6, r = threadIdx.x & 63;
-  for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
+  if (!GV)
+    for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) v[k] = gv[k];
   __syncthreads();
   if constexpr (NW == 16) {
     // lane = row c of the contiguous column: wave g produces the outputs
@@ -354,6 +361,7 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_bwd(const double *__restrict__
 #pragma unroll
         for (int m = 0; m < 4; m++) v[I * 64 + g + 16 * m] = a[m];
       }
+      if (GV) __threadfence_block();
       __syncthreads();
     }
   } else {
@@ -375,10 +383,12 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_bwd(const double *__restrict__
     for (int c = 0; c < CW; c++) u += L[c] * t[g * CW + c];
     double w = row_reduce<NW>(u, red);
     if (threadIdx.x < 64) v[I * 64 + r] = w;
+    if (GV) __threadfence_block();
     __syncthreads();
   }
   }
-  for (int k = threadIdx.x; k < d.nto * 64; k += blockDim.x) gv[k] = v[k];
+  if (!GV)
+    for (int k = threadIdx.x; k < d.nto * 64; k += blockDim.x) gv[k] = v[k];
 }
 
 size_t max_dynamic_lds() {
@@ -402,33 +412,43 @@ static void lds_request(const void *kernel, size_t bytes, const char *what) {
   }
 }
 
-// 16 waves per workgroup when fewer subdomains than half the CUs
+// 16 waves per workgroup when fewer subdomains than half the CUs; the
+// subdomain vector in LDS when it fits, else in global memory
 static constexpr int kTrsvFewSubs = 128;
+static bool g_trsv_global = false;  // force the global-memory form (testing)
+void set_trsv_global(bool on) { g_trsv_global = on; }
+template <int NW, class K1, class K2>
+static void trsv_launch(hipStream_t st, int nsub, int maxnt, K1 lds_kernel, K2 gv_kernel,
+                        const double *pool, const TrsvDesc *d, const int *p, const int *k,
+                        const int *t, double *v) {
+  const size_t sh = (size_t)(maxnt * 64 + NW * 64 + 64) * sizeof(double);
+  if (sh <= max_dynamic_lds() && !g_trsv_global) {
+    lds_request((const void *)lds_kernel, sh, "triangular sweep");
+    hipLaunchKernelGGL(lds_kernel, dim3(nsub), dim3(64 * NW), sh, st, pool, d, p, k, t, v);
+  } else {
+    const size_t sg = (size_t)(NW * 64 + 64) * sizeof(double);
+    hipLaunchKernelGGL(gv_kernel, dim3(nsub), dim3(64 * NW), sg, st, pool, d, p, k, t, v);
+  }
+}
 void launch_trsv_fwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
                      const int *rptr, const int *rk, const int *rt, double *v, int maxnt) {
   if (nsub <= 0) return;
-  if (nsub < kTrsvFewSubs) {
-    size_t sh = (size_t)(maxnt * 64 + 16 * 64 + 64) * sizeof(double);
-    lds_request((const void *)k_trsv_fwd<16>, sh, "triangular sweep");
-    hipLaunchKernelGGL(k_trsv_fwd<16>, dim3(nsub), dim3(1024), sh, st, pool, d, rptr, rk, rt, v);
-  } else {
-    size_t sh = (size_t)(maxnt * 64 + 4 * 64 + 64) * sizeof(double);
-    lds_request((const void *)k_trsv_fwd<4>, sh, "triangular sweep");
-    hipLaunchKernelGGL(k_trsv_fwd<4>, dim3(nsub), dim3(256), sh, st, pool, d, rptr, rk, rt, v);
-  }
+  if (nsub < kTrsvFewSubs)
+    trsv_launch<16>(st, nsub, maxnt, k_trsv_fwd<16, false>, k_trsv_fwd<16, true>, pool, d, rptr,
+                    rk, rt, v);
+  else
+    trsv_launch<4>(st, nsub, maxnt, k_trsv_fwd<4, false>, k_trsv_fwd<4, true>, pool, d, rptr, rk,
+                   rt, v);
 }
 void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
                      const int *cptr, const int *ck, const int *ct, double *v, int maxnt) {
   if (nsub <= 0) return;
-  if (nsub < kTrsvFewSubs) {
-    size_t sh = (size_t)(maxnt * 64 + 16 * 64 + 64) * sizeof(double);
-    lds_request((const void *)k_trsv_bwd<16>, sh, "triangular sweep");
-    hipLaunchKernelGGL(k_trsv_bwd<16>, dim3(nsub), dim3(1024), sh, st, pool, d, cptr, ck, ct, v);
-  } else {
-    size_t sh = (size_t)(maxnt * 64 + 4 * 64 + 64) * sizeof(double);
-    lds_request((const void *)k_trsv_bwd<4>, sh, "triangular sweep");
-    hipLaunchKernelGGL(k_trsv_bwd<4>, dim3(nsub), dim3(256), sh, st, pool, d, cptr, ck, ct, v);
-  }
+  if (nsub < kTrsvFewSubs)
+    trsv_launch<16>(st, nsub, maxnt, k_trsv_bwd<16, false>, k_trsv_bwd<16, true>, pool, d, cptr,
+                    ck, ct, v);
+  else
+    trsv_launch<4>(st, nsub, maxnt, k_trsv_bwd<4, false>, k_trsv_bwd<4, true>, pool, d, cptr, ck,
+                   ct, v);
 }
 
 // -------------------------------------------------------- iteration kernels

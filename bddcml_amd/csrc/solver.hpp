@@ -115,6 +115,9 @@ void device_release_input(Solver &S);
 // order of tile-task launches (kernels: 1 = one contiguous range of tasks
 // per XCD, the default; 0 = blockIdx order)
 void set_task_order(int chunked);
+// interior sweeps always with the subdomain vector in global memory (testing;
+// by default only subdomains whose vector does not fit in LDS use that form)
+void set_trsv_global(bool on);
 // copies the solution of the last solve to the host (once; no-op otherwise)
 void device_fetch_solution(Solver &S);
 void device_setup_begin(Solver &S);

@@ -267,3 +267,25 @@ def test_explicit_inverse(oracle_lib, kind, precision):
         t.close()
     else:
         o.close()
+
+
+@pytest.mark.parametrize("nel,ns,kind,nsub", [(4, 3, "poisson", 0), (3, 2, "elasticity", 0),
+                                              (4, 8, "poisson", 512)])
+def test_sweeps_with_vector_in_global_memory(oracle_lib, nel, ns, kind, nsub):
+    """The interior sweeps in the form with the subdomain vector in global
+    memory (used automatically for subdomains above ~20,000 dofs, forced
+    here on small ones, 16- and 4-wave variants): reduced RHS and local
+    solutions against the oracle."""
+    _say(f"case sweeps in global memory nel={nel} ns={ns} kind={kind}")
+    p = CubeProblem(nel, ns, kind=kind, rhs="random", seed=51)
+    o, (it_o, r_o, _) = _oracle(oracle_lib, p, tol=1e-10)
+    b, (it, r, _) = bddcml_amd.solve_cube(p, tol=1e-10, maxit=300, options={"sweep_in_global": 1})
+    assert r == r_o == 0 and abs(it - it_o) <= 1
+    assert _relmax(b.reduced_rhs(), o.reduced_rhs()) < 1e-12
+    u_ref = o.interface_solution()
+    assert _relmax(b.interface_solution(), u_ref) < 1e-10
+    for d in p.subdomains():
+        ls = b.download_local_solution(d["isub"], d["ndofs"])
+        assert np.abs(ls - o.local_solution(d["isub"])).max() <= 1e-10 * np.abs(u_ref).max()
+    o.close()
+    b.finalize()

@@ -1228,8 +1228,9 @@ void device_setup_finish(Solver &S) {
   // the whole interface vector of a subdomain in LDS (k_kapply), the coarse
   // matrix as full tiles
   if (D.maxnc > 1024) fatal("BDDCML_SETUP_PRECONDITIONER", "more than 1024 constraints per subdomain");
-  if ((size_t)(D.maxNI + 1024 + 64 + D.maxnc) * sizeof(double) > max_dynamic_lds())
-    fatal("BDDCML_SETUP_PRECONDITIONER", "subdomain interface too large for the preconditioner apply");
+  // (interfaces too large for LDS keep their working vector in global memory)
+  if ((size_t)(1024 + 64 + D.maxnc) * sizeof(double) > max_dynamic_lds())
+    fatal("BDDCML_SETUP_PRECONDITIONER", "too many constraints per subdomain for the preconditioner apply");
   if ((int64_t)D.ntc * D.ntc > ((int64_t)1 << 24))
     fatal("BDDCML_SETUP_PRECONDITIONER", "coarse problem too large for its dense factorization; "
                                          "use three levels");

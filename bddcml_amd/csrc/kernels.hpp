@@ -116,8 +116,9 @@ struct TrsvDesc {
   int64_t voff;          // offset into the device-ordered subdomain vector
 };
 // maxnt = largest nto+nti of the batch (sizes the vector held in LDS)
-// the interior sweeps keep the subdomain vector in LDS when it fits and in
-// global memory otherwise; on = always in global memory (for testing)
+// the interior sweeps, the local preconditioner and the coarse residual keep
+// their per-subdomain vector in LDS when it fits and in global memory
+// otherwise; on = always in global memory (for testing)
 void set_trsv_global(bool on);
 void launch_trsv_fwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
                      const int *rptr, const int *rk, const int *rt, double *v, int maxnt);

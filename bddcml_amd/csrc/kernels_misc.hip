@@ -415,7 +415,8 @@ static void lds_request(const void *kernel, size_t bytes, const char *what) {
 // 16 waves per workgroup when fewer subdomains than half the CUs; the
 // subdomain vector in LDS when it fits, else in global memory
 static constexpr int kTrsvFewSubs = 128;
-static bool g_trsv_global = false;  // force the global-memory form (testing)
+// force the global-memory forms of the per-subdomain kernels (testing)
+static bool g_trsv_global = false;
 void set_trsv_global(bool on) { g_trsv_global = on; }
 template <int NW, class K1, class K2>
 static void trsv_launch(hipStream_t st, int nsub, int maxnt, K1 lds_kernel, K2 gv_kernel,
@@ -628,9 +629,12 @@ __device__ __forceinline__ int packed_lower(int n, int I, int J) {
 
 
 // rcs = Phi_i^T (w .* r(map)) per subdomain: the weighted interface vector is
-// gathered into LDS once per block, then each wave reduces PHIT_COLS / 4 of
-// the block's PHIT_COLS columns of Phi_i (contiguous, lane-strided).
+// gathered into LDS once per block (STAGE; otherwise, for interfaces too
+// large for LDS, per column from global memory), then each wave reduces
+// PHIT_COLS / 4 of the block's PHIT_COLS columns of Phi_i (contiguous,
+// lane-strided).
 constexpr int PHIT_COLS = 16;
+template <bool STAGE>
 __global__ __launch_bounds__(256) void k_phit(const SubDesc *__restrict__ subs,
                                               const double *__restrict__ Phi,
                                               const int *__restrict__ map,
@@ -642,16 +646,27 @@ __global__ __launch_bounds__(256) void k_phit(const SubDesc *__restrict__ subs,
   const int c0 = blockIdx.x * PHIT_COLS;
   if (c0 >= d.nc) return;
   const int NI = d.NI;
-  for (int i = threadIdx.x; i < NI; i += 256) {
-    const int g = map[d.loff + i];
-    xw[i] = g >= 0 ? w[d.loff + i] * x[g] : 0.0;
+  if (STAGE) {
+    for (int i = threadIdx.x; i < NI; i += 256) {
+      const int g = map[d.loff + i];
+      xw[i] = g >= 0 ? w[d.loff + i] * x[g] : 0.0;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int c = c0 + wave; c < min(c0 + PHIT_COLS, d.nc); c += 4) {
     const double *pc = Phi + d.zoff + (size_t)c * NI;
     double s = 0.0;
-    for (int i = lane; i < NI; i += 64) s += pc[i] * xw[i];
+    for (int i = lane; i < NI; i += 64) {
+      double xv;
+      if (STAGE) {
+        xv = xw[i];
+      } else {
+        const int g = map[d.loff + i];
+        xv = g >= 0 ? w[d.loff + i] * x[g] : 0.0;
+      }
+      s += pc[i] * xv;
+    }
     s = wave_sum(s);
     if (lane == 0) rcs[d.coff + c] = s;
   }
@@ -663,17 +678,27 @@ __global__ __launch_bounds__(256) void k_phit(const SubDesc *__restrict__ subs,
 // sweep is the tail.  Otherwise two tiles in flight (56 VGPRs, two
 // workgroups per CU), for many subdomains of similar size, where the
 // launch is HBM-bound as a whole.
-template <typename T, bool DEEP>
+// barrier that also publishes global-memory writes to the other waves of
+// the workgroup when the working vector lives there
+template <bool GV>
+__device__ __forceinline__ void publish_sync() {
+  if (GV) __threadfence_block();
+  __syncthreads();
+}
+
+template <typename T, bool DEEP, bool GV>
 __global__ __launch_bounds__(KA_THR) void k_kapply(
     const T *__restrict__ pool, const SubDesc *__restrict__ subs, const int *__restrict__ map,
     const double *__restrict__ w, const double *__restrict__ rin, const double *__restrict__ Phi,
     const double *__restrict__ Z, const int *__restrict__ cmap, const double *__restrict__ uc,
-    const double *__restrict__ ysym, double *__restrict__ zl) {
+    const double *__restrict__ ysym, double *zl) {
   extern __shared__ double smem[];
   const SubDesc d = subs[blockIdx.x];
   const int NI = d.NI, nc = d.nc, n = d.nti;
-  double *xs = smem;        // NI: the subdomain interface vector
-  double *red = xs + NI;    // 16 x 64 partial sums
+  // the subdomain interface vector: in LDS, or (GV, interfaces too large for
+  // LDS) in the subdomain's own range of the output, updated in place
+  double *xs = GV ? zl + d.loff : smem;
+  double *red = GV ? smem : xs + NI;  // 16 x 64 partial sums
   double *t = red + 1024;   // 64
   double *cc = t + 64;      // nc: uc(indrowc) - Z^T x
   const int tid = threadIdx.x, g = tid >> 6, l = tid & 63;
@@ -681,7 +706,7 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
     const int m = map[d.loff + k];
     xs[k] = m >= 0 ? w[d.loff + k] * rin[m] : 0.0;
   }
-  __syncthreads();
+  publish_sync<GV>();
   for (int c = g; c < nc; c += 16) {
     const double *zc = Z + d.zoff + (size_t)c * NI;
     double s = 0.0;
@@ -691,9 +716,9 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
   }
   if (ysym) {
     // S'^{-1} x already applied (explicit inverse, symmetric GEMV): no sweeps
-    __syncthreads();
+    publish_sync<GV>();
     for (int i = tid; i < NI; i += KA_THR) xs[i] = ysym[d.loff + i];
-    __syncthreads();
+    publish_sync<GV>();
   } else {
   // forward sweep x_I <- Linv_I (x_I - sum_{k<I} L_Ik x_k); lane = tile row.
   // The Linv_I entries are loaded before the row sum (they do not depend on
@@ -724,14 +749,14 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
       }
     }
     red[g * 64 + l] = s;
-    __syncthreads();
+    publish_sync<GV>();
     if (tid < 64) {
       double v = 0.0;
 #pragma unroll
       for (int q = 0; q < 16; q++) v += red[q * 64 + l];
       t[l] = xs[I * 64 + l] - v;
     }
-    __syncthreads();
+    publish_sync<GV>();
     {
       double u = 0.0;
       if (!DEEP) {
@@ -742,14 +767,14 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
       for (int m = 0; m < 4; m++) u += lv[m] * t[g + 16 * m];
       red[g * 64 + l] = u;
     }
-    __syncthreads();
+    publish_sync<GV>();
     if (tid < 64) {
       double v = 0.0;
 #pragma unroll
       for (int q = 0; q < 16; q++) v += red[q * 64 + l];
       xs[I * 64 + l] = v;
     }
-    __syncthreads();
+    publish_sync<GV>();
   }
   // backward sweep x_I <- Linv_I^T (x_I - sum_{k>I} L_kI^T x_k); lane = tile
   // row of L_kI (contiguous), wave g owns outputs g, g+16, g+32, g+48
@@ -784,7 +809,7 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
 #pragma unroll
       for (int m = 0; m < 4; m++) t[g + 16 * m] = xs[I * 64 + g + 16 * m] - a[m];
     }
-    __syncthreads();
+    publish_sync<GV>();
     const double tv = t[l];
     if (!DEEP) {
 #pragma unroll
@@ -796,10 +821,10 @@ __global__ __launch_bounds__(KA_THR) void k_kapply(
 #pragma unroll
       for (int m = 0; m < 4; m++) xs[I * 64 + g + 16 * m] = a[m];
     }
-    __syncthreads();
+    publish_sync<GV>();
   }
   }
-  __syncthreads();
+  publish_sync<GV>();
   for (int i = tid; i < NI; i += KA_THR) {
     double s = xs[i];
     for (int c = 0; c < nc; c++) s += Phi[d.zoff + (size_t)c * NI + i] * cc[c];
@@ -816,9 +841,13 @@ void launch_phit(hipStream_t st, const SubDesc *subs, int nsub, const double *Ph
                  const double *w, const double *x, double *rcs, int maxnc, int maxNI) {
   if (nsub <= 0 || maxnc <= 0) return;
   const size_t sh = (size_t)std::max(maxNI, 1) * sizeof(double);
-  lds_request((const void *)k_phit, sh, "coarse residual");
-  hipLaunchKernelGGL(k_phit, dim3((maxnc + PHIT_COLS - 1) / PHIT_COLS, nsub), dim3(256), sh, st, subs,
-                     Phi, map, w, x, rcs);
+  const dim3 grid((maxnc + PHIT_COLS - 1) / PHIT_COLS, nsub);
+  if (sh <= max_dynamic_lds() && !g_trsv_global) {
+    lds_request((const void *)k_phit<true>, sh, "coarse residual");
+    hipLaunchKernelGGL(k_phit<true>, grid, dim3(256), sh, st, subs, Phi, map, w, x, rcs);
+  } else {
+    hipLaunchKernelGGL(k_phit<false>, grid, dim3(256), 0, st, subs, Phi, map, w, x, rcs);
+  }
 }
 void launch_coarse_gather(hipStream_t st, int n, const int *ptr, const int *src,
                           const double *rcs, double *rc) {
@@ -831,15 +860,20 @@ void launch_kapply(hipStream_t st, const T *pool, const SubDesc *subs, int nsub,
                    const int *cmap, const double *uc, const double *ysym, double *zl, int maxNI,
                    int maxnc, bool deep) {
   if (nsub <= 0) return;
-  size_t sh = (size_t)(maxNI + 1024 + 64 + maxnc) * sizeof(double);
-  if (deep && !ysym) {
-    lds_request((const void *)k_kapply<T, true>, sh, "preconditioner apply");
-    hipLaunchKernelGGL((k_kapply<T, true>), dim3(nsub), dim3(KA_THR), sh, st, pool, subs, map, w, r,
-                       Phi, Z, cmap, uc, ysym, zl);
+  const size_t sh = (size_t)(maxNI + 1024 + 64 + maxnc) * sizeof(double);
+  const size_t sg = (size_t)(1024 + 64 + maxnc) * sizeof(double);
+  const bool gv = sh > max_dynamic_lds() || g_trsv_global;
+  auto go = [&](auto kernel, size_t bytes) {
+    lds_request((const void *)kernel, bytes, "preconditioner apply");
+    hipLaunchKernelGGL(kernel, dim3(nsub), dim3(KA_THR), bytes, st, pool, subs, map, w, r, Phi, Z,
+                       cmap, uc, ysym, zl);
+  };
+  if (gv) {
+    if (deep && !ysym) go(k_kapply<T, true, true>, sg);
+    else go(k_kapply<T, false, true>, sg);
   } else {
-    lds_request((const void *)k_kapply<T, false>, sh, "preconditioner apply");
-    hipLaunchKernelGGL((k_kapply<T, false>), dim3(nsub), dim3(KA_THR), sh, st, pool, subs, map, w, r,
-                       Phi, Z, cmap, uc, ysym, zl);
+    if (deep && !ysym) go(k_kapply<T, true, false>, sh);
+    else go(k_kapply<T, false, false>, sh);
   }
 }
 template void launch_kapply<double>(hipStream_t, const double *, const SubDesc *, int, const int *,

@@ -59,9 +59,11 @@ int bddcml_amd_timings(double *out, int n);
  *     stay FP64);
  *   "task_order" (1: each XCD runs one contiguous range of a tile-task
  *     launch, the default; 0: plain workgroup order);
- *   "sweep_in_global" (1: the interior sweeps keep the subdomain vector in
- *     global memory, the form used automatically for subdomains whose
- *     vector does not fit in LDS (more than about 20,000 dofs); for testing);
+ *   "sweep_in_global" (1: the interior sweeps, the local preconditioner and
+ *     the coarse residual keep their per-subdomain working vector in global
+ *     memory -- the forms used automatically for subdomains whose vector does
+ *     not fit in LDS (about 20,000 dofs, or interfaces of about 19,000);
+ *     for testing);
  *   "explicit_inverse" (S'^{-1} formed, W = L^{-1} and W^T W on the tile
  *     GEMM, so that each preconditioner apply reads it once in a symmetric
  *     GEMV instead of sweeping the factor twice: 0 never; 1 at setup; 2, the

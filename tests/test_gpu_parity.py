@@ -269,19 +269,23 @@ def test_explicit_inverse(oracle_lib, kind, precision):
         o.close()
 
 
-@pytest.mark.parametrize("nel,ns,kind,nsub", [(4, 3, "poisson", 0), (3, 2, "elasticity", 0),
-                                              (4, 8, "poisson", 512)])
-def test_sweeps_with_vector_in_global_memory(oracle_lib, nel, ns, kind, nsub):
-    """The interior sweeps in the form with the subdomain vector in global
-    memory (used automatically for subdomains above ~20,000 dofs, forced
-    here on small ones, 16- and 4-wave variants): reduced RHS and local
-    solutions against the oracle."""
-    _say(f"case sweeps in global memory nel={nel} ns={ns} kind={kind}")
+@pytest.mark.parametrize("nel,ns,kind,ex", [(4, 3, "poisson", 0), (3, 2, "elasticity", 0),
+                                            (4, 8, "poisson", 0), (4, 3, "poisson", 1)])
+def test_sweeps_with_vector_in_global_memory(oracle_lib, nel, ns, kind, ex):
+    """The per-subdomain kernels in their forms with the working vector in
+    global memory (interior sweeps, 16- and 4-wave; local preconditioner;
+    coarse residual), used automatically for subdomains too large for LDS
+    and forced here on small ones: M r, reduced RHS and local solutions
+    against the oracle."""
+    _say(f"case sweeps in global memory nel={nel} ns={ns} kind={kind} explicit={ex}")
     p = CubeProblem(nel, ns, kind=kind, rhs="random", seed=51)
     o, (it_o, r_o, _) = _oracle(oracle_lib, p, tol=1e-10)
-    b, (it, r, _) = bddcml_amd.solve_cube(p, tol=1e-10, maxit=300, options={"sweep_in_global": 1})
+    b, (it, r, _) = bddcml_amd.solve_cube(p, tol=1e-10, maxit=300,
+                                          options={"sweep_in_global": 1, "explicit_inverse": ex})
     assert r == r_o == 0 and abs(it - it_o) <= 1
     assert _relmax(b.reduced_rhs(), o.reduced_rhs()) < 1e-12
+    x = np.random.default_rng(2).standard_normal(len(o.interface_solution()))
+    assert _relmax(b.apply_pc(x), o.apply_pc(x)) < 1e-10
     u_ref = o.interface_solution()
     assert _relmax(b.interface_solution(), u_ref) < 1e-10
     for d in p.subdomains():

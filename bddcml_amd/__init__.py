@@ -36,7 +36,7 @@ EXTRA_SYMBOLS = [
     "bddcml_amd_sub_info", "bddcml_amd_sub_maps", "bddcml_amd_sub_matrix",
     "bddcml_amd_sub_coarse", "bddcml_amd_last_error", "bddcml_amd_selftest_tile",
     "bddcml_amd_selftest_copy", "bddcml_amd_element_gradient", "bddcml_amd_recycling_basis",
-    "bddcml_amd_level2_counts",
+    "bddcml_amd_level2_counts", "bddcml_amd_level_counts",
 ]
 
 
@@ -255,9 +255,13 @@ class Bddcml:
                     ndofc=int(out[3]), nig=int(out[4]))
 
     def level2_counts(self):
-        """Second level of a three-level setup (None without one)."""
+        """Second level of a multilevel setup (None without one)."""
+        return self.level_counts(2)
+
+    def level_counts(self, ilevel):
+        """Level ilevel >= 2 of a multilevel setup (None without one)."""
         out = np.zeros(7, dtype=np.int32)
-        if self.L.bddcml_amd_level2_counts(out.ctypes.data_as(C.POINTER(C.c_int))):
+        if self.L.bddcml_amd_level_counts(int(ilevel), out.ctypes.data_as(C.POINTER(C.c_int))):
             return None
         return dict(nsub=int(out[0]), ndof=int(out[1]), ncorner=int(out[2]), nedge=int(out[3]),
                     nface=int(out[4]), ndofc=int(out[5]), nig=int(out[6]))
@@ -351,14 +355,16 @@ class Bddcml:
 
 
 def solve_cube(problem, use_corner=1, use_arith=1, weights_type=0, method=0, tol=1e-8,
-               maxit=500, ndecrmax=50, verbose=0, options=None, nsub2=0):
+               maxit=500, ndecrmax=50, verbose=0, options=None, nsub2=0, nsublev=None):
     """Drive the drop-in interface on a CubeProblem the way the reference
     harness does (examples/poisson_on_cube.f90:373-560); returns the Bddcml
     handle and (num_iter, converged_reason, condition_number).  nsub2 > 0
     asks for three levels with nsub2 subdomains on the second level
-    (nsublev = [nsub, nsub2, 1])."""
+    (nsublev = [nsub, nsub2, 1]); nsublev gives every level explicitly."""
     b = Bddcml()
-    if nsub2 > 0:
+    if nsublev is not None:
+        b.init(len(nsublev), list(nsublev), numbase=problem.numbase, verbose_level=verbose)
+    elif nsub2 > 0:
         b.init(3, [problem.nsub, nsub2, 1], numbase=problem.numbase, verbose_level=verbose)
     else:
         b.init(2, [problem.nsub, 1], numbase=problem.numbase, verbose_level=verbose)

@@ -46,7 +46,9 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(objdir, src + ".o")
         cmd = [HIPCC] + COMMON + HIPFLAGS + ["-c", path, "-o", obj]
         if src.endswith(".cpp"):
-            cmd = [HIPCC] + COMMON + ["-x", "c++", "-c", path, "-o", obj]
+            # host plan: geometric ties (corner selection) need unfused
+            # IEEE arithmetic on every host target
+            cmd = [HIPCC] + COMMON + ["-ffp-contract=off", "-x", "c++", "-c", path, "-o", obj]
         out = _run(cmd)
         if verbose and out:
             print(out)

@@ -52,14 +52,16 @@ void bddcml_init_c(int *nl, int *nsublev, int *lnsublev, int *nsub_loc_1, int *c
   set_trsv_global(false);
   Solver &S = solver();
   if (*nl < 2) fatal("LEVELS_INIT", "Number of levels must be at least 2");
-  if (*nl > 3) fatal("LEVELS_INIT", "This library implements two and three levels");
   if (*lnsublev < *nl) fatal("LEVELS_INIT", "lnsublev smaller than number of levels");
   if (nsublev[*nl - 1] != 1) fatal("LEVELS_INIT", "Number of subdomains at last level must be 1.");
-  if (*nl == 3 && (nsublev[1] < 1 || nsublev[1] > nsublev[0]))
-    fatal("LEVELS_INIT", "Number of subdomains on level 2 must be between 1 and that of level 1");
+  for (int l = 1; l < *nl - 1; l++)
+    if (nsublev[l] < 1 || nsublev[l] > nsublev[l - 1])
+      fatal("LEVELS_INIT", "Number of subdomains on level " + std::to_string(l + 1) +
+                               " must be between 1 and that of level " + std::to_string(l));
   S.nl = *nl;
   S.nsub = nsublev[0];
-  S.nsub2 = *nl == 3 ? nsublev[1] : 0;
+  S.nsublev.assign(nsublev, nsublev + *nl);
+  S.nsub2 = *nl >= 3 ? nsublev[1] : 0;
   S.numbase = *numbase;
   S.verbose = *verbose_level;
   S.subs.assign(S.nsub, Sub());
@@ -262,14 +264,15 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
   phase("condensation rhs");
   S.t.topology += wall_time() - th;
   if (S.plan_only) {
-    if (S.nl == 3 && S.nsub2 >= 2) setup_second_level(S);  // its host plan
+    if (S.nl >= 3 && S.nsub2 >= 2) setup_second_level(S);  // its host plan
     S.is_setup = true;
     return;
   }
   device_setup_finish(S);
-  // three levels: the second level from the coarse nodes and matrices
-  // (with one second-level subdomain the coarse problem is solved directly)
-  if (S.nl == 3 && S.nsub2 >= 2) {
+  // three or more levels: the next level from the coarse nodes and matrices,
+  // recursively (a next level of one subdomain: the coarse problem is solved
+  // directly)
+  if (S.nl >= 3 && S.nsub2 >= 2) {
     setup_second_level(S);
     phase("second level");
   }
@@ -450,6 +453,29 @@ catch (const Error &e) {
 }
 
 // ------------------------------------------------------------- extra ABI
+int bddcml_amd_level_counts(int ilevel, int *out) try {
+  Solver &S = solver();
+  if (!S.is_setup || ilevel < 2) return -1;
+  const Solver *L = &S;
+  for (int l = 1; l < ilevel; l++) {
+    L = L->lev2.get();
+    if (!L) return -1;
+  }
+  const Solver &T = *L;
+  out[0] = T.nsub;
+  out[1] = T.ndof_g;
+  out[2] = T.ncorner;
+  out[3] = T.nedge;
+  out[4] = T.nface;
+  out[5] = T.ndofc;
+  out[6] = T.nig;
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e);
+  return -1;
+}
+
 int bddcml_amd_level2_counts(int *out) try {
   Solver &S = solver();
   if (!S.is_setup || !S.lev2) return -1;

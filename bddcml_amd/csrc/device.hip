@@ -308,7 +308,7 @@ struct Device {
 
 // a second level replaces the dense coarse solve (with one second-level
 // subdomain the coarse problem stays dense: the same coarse solve)
-static bool has_level2(const Solver &S) { return S.nl == 3 && S.nsub2 >= 2; }
+static bool has_level2(const Solver &S) { return S.nl >= 3 && S.nsub2 >= 2; }
 
 // Assembled subdomain triplets kept in HBM from upload time on.
 struct DeviceInput {
@@ -1893,6 +1893,11 @@ void device_link_levels(Solver &S) {
   if (!E.borrowed_st) HIPCHECK(hipStreamDestroy(E.st));
   E.st = D.st;
   E.borrowed_st = true;
+  // the levels below it borrowed its stream: they run on this one too
+  for (Solver *c = T.lev2.get(); c && c->dev; c = c->lev2.get()) {
+    c->dev->st = D.st;
+    c->dev->borrowed_st = true;
+  }
 }
 
 // --------------------------------------------------------------- operators

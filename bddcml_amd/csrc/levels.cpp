@@ -1,4 +1,4 @@
-// levels.cpp -- three-level BDDC: the second level's mesh and its setup.
+// levels.cpp -- multilevel BDDC: the next level's mesh and its setup.
 //
 // The reference builds level l+1 from level l (src/module_levels.f90:
 // 1484-1530): its elements are the subdomains of level l, its nodes the
@@ -212,8 +212,10 @@ void setup_second_level(Solver &S) {
     if (elems[j].empty()) fatal(R, "Partition does not contain subdomain " + std::to_string(j));
   S.lev2.reset(new Solver());
   Solver &T = *S.lev2;
-  T.nl = 2;
+  T.nl = S.nl - 1;
   T.nsub = S.nsub2;
+  T.nsublev.assign(S.nsublev.begin() + 1, S.nsublev.end());
+  T.nsub2 = T.nl >= 3 ? T.nsublev[1] : 0;
   T.numbase = 0;
   T.verbose = S.verbose >= 3 ? S.verbose : 0;
   T.initialised = true;
@@ -249,15 +251,18 @@ void setup_second_level(Solver &S) {
   apply_boundary_conditions(T);
   build_weights(T);
   build_condensation_rhs(T);
-  if (!T.plan_only) {
-    device_setup_finish(T);
-    device_link_levels(S);
-  }
+  if (!T.plan_only) device_setup_finish(T);
+  // more levels: the next one is built from this one's coarse nodes (on its
+  // stream, which the link below redirects to this level's)
+  if (T.nl >= 3 && T.nsub2 >= 2) setup_second_level(T);
+  if (!T.plan_only) device_link_levels(S);
   T.is_setup = true;
   S.t.level2 = wall_time() - t0;
   if (S.verbose > 0)
-    std::printf(" LEVELS_PREPARE_STANDARD_LEVEL: level 2: %d subdomains, %d dofs, ndofc %d, interface %d, %.4f s\n",
-                T.nsub, T.ndof_g, T.ndofc, T.nig, S.t.level2);
+    std::printf(" LEVELS_PREPARE_STANDARD_LEVEL: level %d of %d: %d subdomains, %d dofs, ndofc %d, "
+                "interface %d, %.4f s\n",
+                (int)(S.nsublev.size() - T.nsublev.size()) + 1, (int)S.nsublev.size(), T.nsub, T.ndof_g,
+                T.ndofc, T.nig, S.t.level2);
 }
 
 }  // namespace bddc

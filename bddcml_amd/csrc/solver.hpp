@@ -30,7 +30,8 @@ struct Timings {
 struct Solver {
   // init
   int nl = 0, nsub = 0, numbase = 1, verbose = 0;
-  int nsub2 = 0;  // subdomains of the second level (three levels)
+  int nsub2 = 0;  // subdomains of the next level (three or more levels)
+  std::vector<int> nsublev;  // subdomains of every level, this one first
   bool initialised = false;
   // global sizes
   int nelem_g = 0, nnod_g = 0, ndof_g = 0, ndim = 3, meshdim = 3;
@@ -74,9 +75,10 @@ struct Solver {
   std::vector<double> uig;  // interface solution
   std::unique_ptr<Device> dev;
   std::unique_ptr<DeviceInput> din;
-  // three levels: the second level, whose elements are this level's
+  // three or more levels: the next level, whose elements are this level's
   // subdomains and whose nodes are its coarse nodes; one application of its
-  // BDDC replaces the coarse solve (src/module_levels.f90:3849-4249)
+  // BDDC replaces the coarse solve (src/module_levels.f90:3849-4249).  It
+  // may have a next level of its own.
   std::unique_ptr<Solver> lev2;
   Timings t;
   int last_iters = 0, last_reason = 0;

@@ -16,13 +16,16 @@ extern "C" {
 
 /* out[0]=ncorner out[1]=nedge out[2]=nface out[3]=ndofc out[4]=interface dofs */
 int bddcml_amd_counts(int *out);
-/* Three levels (bddcml_init_c with nl = 3, nsublev[1] >= 2): the second
- * level's out[0]=subdomains out[1]=dofs (the first level's coarse dofs)
+/* Three or more levels (bddcml_init_c with nl >= 3, nsublev[1] >= 2): the
+ * second level's out[0]=subdomains out[1]=dofs (the first level's coarse dofs)
  * out[2]=ncorner out[3]=nedge out[4]=nface out[5]=ndofc out[6]=interface
  * dofs; returns -1 without a second level.  Inspection only; the second
  * level follows src/module_levels.f90:1297-1530 (setup) and 3849-4249
  * (apply). */
 int bddcml_amd_level2_counts(int *out);
+/* the same for level ilevel >= 2 of a setup with more levels (-1 if that
+ * level does not exist: fewer levels, or a level of one subdomain above it) */
+int bddcml_amd_level_counts(int ilevel, int *out);
 /* sorted global dof ids (0-based) of the interface, length out[4] above */
 int bddcml_amd_interface_dofs(int *out);
 /* y = S x, assembled Schur complement on the interface (device) */

@@ -1,6 +1,6 @@
-"""Three-level BDDC composed from two CPU oracle instances.
+"""Multilevel BDDC composed from CPU oracle instances (one per level).
 
-TEST INFRASTRUCTURE ONLY: the checker for the library's three-level path.
+TEST INFRASTRUCTURE ONLY: the checker for the library's multilevel path.
 The product (bddcml_amd) never imports this module.
 
 Level 2 restated from the reference (paths relative to its root):
@@ -111,42 +111,84 @@ def level2_subdomains(o1: Oracle, nsub1: int, nsub2: int, numbase: int = 1,
     return subs, part, nnodc, ndofc
 
 
-class ThreeLevels:
-    """First level ``o1`` with its coarse solve replaced by one application
-    of the second level ``o2``'s BDDC; solve and probes through ``o1``."""
+class MultiLevels:
+    """A chain of oracle instances, one per level below the last:
+    ``levels[0]`` holds the uploaded problem, each next one the mesh built
+    from its predecessor's coarse nodes and coarse matrices, and every
+    instance's coarse solve is one application of the next one's BDDC
+    (the last one's a direct solve).  ``nsublev`` as bddcml_init: subdomains
+    per level, the last 1; a level of one subdomain solves its coarse problem
+    exactly (all its dofs are interior), as the direct solve does.  Solve and
+    probes through ``o1``."""
 
-    def __init__(self, subdomains, nsub, nnod, ndof, nsub2, numbase=1, use_corner=1,
+    def __init__(self, subdomains, nsub, nnod, ndof, nsublev, numbase=1, use_corner=1,
                  use_arith=1, weights_type=0):
         subdomains = list(subdomains)
-        self.o1 = Oracle(nsub, nnod, ndof, 3, 3, numbase)
+        if nsublev[0] != nsub or nsublev[-1] != 1:
+            raise ValueError("nsublev must start with nsub and end with 1")
+        self.levels = [Oracle(nsub, nnod, ndof, 3, 3, numbase)]
         for d in subdomains:
-            self.o1.upload(d)
-        self.o1.setup(use_corner, use_arith, weights_type)
+            self.levels[0].upload(d)
+        self.levels[0].setup(use_corner, use_arith, weights_type)
         d0 = subdomains[0]
-        self.subs2, self.part, nnodc, ndofc = level2_subdomains(
-            self.o1, nsub, nsub2, numbase, d0["find_components"], d0["use_dual_mesh_graph"])
-        self.ndofc = ndofc
-        self.o2 = Oracle(nsub2, nnodc, ndofc, 3, 3, numbase)
-        for d in self.subs2:
-            self.o2.upload(d)
-        self.o2.setup(use_corner, use_arith, weights_type)
+        self.subs = [subdomains]
+        self.parts, self.ndofcs, self._fns = [], [], []
+        for l in range(1, len(nsublev) - 1):
+            prev = self.levels[-1]
+            subs, part, nnodc, ndofc = level2_subdomains(
+                prev, nsublev[l - 1], nsublev[l], numbase, d0["find_components"],
+                d0["use_dual_mesh_graph"])
+            nxt = Oracle(nsublev[l], nnodc, ndofc, 3, 3, numbase)
+            for d in subs:
+                nxt.upload(d)
+            nxt.setup(use_corner, use_arith, weights_type)
+            self._link(prev, nxt, ndofc)
+            self.levels.append(nxt)
+            self.subs.append(subs)
+            self.parts.append(part)
+            self.ndofcs.append(ndofc)
+        self.o1 = self.levels[0]
 
+    def _link(self, prev, nxt, ndofc):
         def coarse(ctx, rc, uc):
             r = np.ctypeslib.as_array(rc, shape=(ndofc,)).copy()
-            x = self.o2.apply_level(r)
-            np.ctypeslib.as_array(uc, shape=(ndofc,))[:] = x
+            np.ctypeslib.as_array(uc, shape=(ndofc,))[:] = nxt.apply_level(r)
 
-        self._fn = COARSE_FN(coarse)  # kept alive as long as o1 may call it
-        self.o1.L.orc_set_coarse_solver(self.o1.h, self._fn, None)
+        fn = COARSE_FN(coarse)  # kept alive as long as prev may call it
+        self._fns.append(fn)
+        prev.L.orc_set_coarse_solver(prev.h, fn, None)
 
     def solve(self, method=0, tol=1e-8, maxit=500, ndecrmax=50):
         return self.o1.solve(method, tol, maxit, ndecrmax)
 
     def close(self):
-        if self.o1.h:
-            self.o1.L.orc_set_coarse_solver(self.o1.h, COARSE_FN(), None)
-        self.o1.close()
-        self.o2.close()
+        for o in self.levels:
+            if o.h:
+                o.L.orc_set_coarse_solver(o.h, COARSE_FN(), None)
+        for o in self.levels:
+            o.close()
+
+
+class ThreeLevels(MultiLevels):
+    """Three levels, nsublev = [nsub, nsub2, 1]: ``o1`` and ``o2`` are the two
+    instances, ``subs2``/``part``/``ndofc`` the second level's data."""
+
+    def __init__(self, subdomains, nsub, nnod, ndof, nsub2, numbase=1, use_corner=1,
+                 use_arith=1, weights_type=0):
+        super().__init__(subdomains, nsub, nnod, ndof, [nsub, nsub2, 1], numbase, use_corner,
+                         use_arith, weights_type)
+        self.o2 = self.levels[1]
+        self.subs2, self.part, self.ndofc = self.subs[1], self.parts[0], self.ndofcs[0]
+
+
+def solve_levels(problem, nsublev, use_corner=1, use_arith=1, weights_type=0, method=0,
+                 tol=1e-8, maxit=500, ndecrmax=50):
+    """Multilevel solve of a CubeProblem / BoxProblem with nsublev as
+    bddcml_init; returns the MultiLevels object (caller closes) and
+    (num_iter, reason, cond)."""
+    t = MultiLevels(problem.subdomains(), problem.nsub, problem.nnod, problem.ndof, nsublev,
+                    problem.numbase, use_corner, use_arith, weights_type)
+    return t, t.solve(method, tol, maxit, ndecrmax)
 
 
 def solve_three_levels(problem, nsub2, use_corner=1, use_arith=1, weights_type=0, method=0,

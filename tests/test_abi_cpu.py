@@ -99,6 +99,48 @@ def test_host_plan_matches_oracle(oracle_lib, nel, ns, kind, wt):
     o.close()
 
 
+def _planl_in_subprocess(nel, ns, kind, nsublev):
+    """The product's host plan with the levels nsublev (plan_only); returns
+    the counts of every level from the second on (None where absent)."""
+    code = f"""
+import sys, json
+sys.path.insert(0, {ROOT!r})
+import bddcml_amd
+from bddcml_amd.cube import CubeProblem
+p = CubeProblem({nel}, {ns}, kind={kind!r}, rhs='random', seed=3)
+b = bddcml_amd.Bddcml()
+b.init({len(nsublev)}, {list(nsublev)!r}, numbase=p.numbase)
+b.set_option('plan_only', 1)
+for d in p.subdomains():
+    b.upload_subdomain_data(d)
+b.setup_preconditioner(1, 0, 1, 1, 1, 0, 0, 0)
+print(json.dumps([b.level_counts(l) for l in range(2, {len(nsublev)})]))
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    import json
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+@pytest.mark.parametrize("nel,ns,kind,nsublev", [(2, 4, "poisson", [64, 8, 2, 1]),
+                                                 (2, 4, "poisson", [64, 16, 4, 2, 1]),
+                                                 (2, 3, "elasticity", [27, 8, 2, 1])])
+def test_deeper_level_plans_match_oracle(oracle_lib, nel, ns, kind, nsublev):
+    """Four and five levels: the mesh, division, globs and corners of every
+    level from the second on against the oracle chain's (counts, exact)."""
+    from oracle.multilevel import MultiLevels
+    p = CubeProblem(nel, ns, kind=kind, rhs="random", seed=3)
+    t = MultiLevels(p.subdomains(), p.nsub, p.nnod, p.ndof, nsublev, p.numbase)
+    prod = _planl_in_subprocess(nel, ns, kind, nsublev)
+    assert len(prod) == len(t.levels) - 1
+    for k, lc in enumerate(prod):
+        oc = t.levels[k + 1].counts()
+        assert lc is not None and lc["nsub"] == nsublev[k + 1] and lc["ndof"] == t.ndofcs[k]
+        for key in ("ncorner", "nedge", "nface", "ndofc", "nig"):
+            assert lc[key] == oc[key], (k + 2, key)
+    t.close()
+
+
 def _plan3_in_subprocess(nel, ns, kind, nsub2):
     """The product's host plan with three levels (plan_only: the second level
     is planned on identity element blocks, its topology does not depend on
@@ -141,9 +183,9 @@ def test_second_level_plan_matches_oracle(oracle_lib, nel, ns, kind, nsub2):
 
 
 def test_level_arguments_rejected():
-    """More than three levels, a last level of more than one subdomain, and a
-    second level larger than the first are rejected at init (reported, not
-    aborted, with throw_on_error)."""
+    """A level with more subdomains than the one before, a last level of more
+    than one subdomain, and a second level larger than the first are rejected
+    at init (reported, not aborted, with throw_on_error)."""
     code = f"""
 import sys
 sys.path.insert(0, {ROOT!r})
@@ -158,14 +200,14 @@ def init(nl, lev):
     buf = C.create_string_buffer(512)
     L.bddcml_amd_last_error(buf, 512)
     return buf.value.decode()
-print(repr(init(4, [8, 4, 2, 1])))
+print(repr(init(4, [8, 4, 8, 1])))
 print(repr(init(3, [8, 2, 2])))
 print(repr(init(3, [8, 16, 1])))
 """
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     lines = r.stdout.strip().splitlines()[-3:]
-    assert "two and three levels" in lines[0]
+    assert "level 3" in lines[0]
     assert "last level must be 1" in lines[1]
     assert "level 2" in lines[2]
 

@@ -135,3 +135,56 @@ def test_three_levels_bicgstab_and_new_data(oracle_lib):
         assert np.abs(ls - t.o1.local_solution(d["isub"])).max() <= 1e-10 * np.abs(u_o).max()
     t.close()
     b.finalize()
+
+
+DEEP_CASES = [
+    ("poisson-4-levels", lambda: CubeProblem(3, 4, kind="poisson", rhs="random", seed=13),
+     [64, 8, 2, 1]),
+    ("elasticity-4-levels", lambda: CubeProblem(2, 3, kind="elasticity", rhs="random", seed=14),
+     [27, 8, 2, 1]),
+    ("poisson-5-levels", lambda: CubeProblem(2, 4, kind="poisson", rhs="random", seed=15),
+     [64, 16, 4, 2, 1]),
+]
+
+
+@pytest.mark.parametrize("name,make,nsublev", DEEP_CASES, ids=[c[0] for c in DEEP_CASES])
+def test_deeper_levels_match_oracle(oracle_lib, name, make, nsublev):
+    """Four and five levels: every level l+1 is built from level l's coarse
+    nodes and coarse matrices and replaces its coarse solve by one BDDC
+    application, recursively (src/module_levels.f90:1297-1530, 3849-4249);
+    the oracle chains one CPU instance per level the same way.  Counts of
+    every intermediate level exact; FP64 M r and interface solution within
+    1e-10 relative, iterations +-1, lambda_max within 1e-6; FP32 local
+    factors: interface solution within 1e-8, iterations +-1."""
+    from oracle.multilevel import solve_levels
+    _say(f"case levels {name} nsublev={nsublev}")
+    p = make()
+    tol = 1e-10
+    t, (it_o, r_o, cond_o) = solve_levels(p, nsublev, tol=tol, maxit=400)
+    assert r_o == 0
+    u_o = t.o1.interface_solution()
+    b, (it, r, cond) = bddcml_amd.solve_cube(p, tol=tol, maxit=400, nsublev=nsublev)
+    _say(f"  oracle {it_o} iterations cond {cond_o:.6g}; device {it} iterations cond {cond:.6g}")
+    for lev in range(2, len(nsublev)):
+        lc, oc = b.level_counts(lev), t.levels[lev - 1].counts()
+        assert lc is not None and lc["nsub"] == nsublev[lev - 1]
+        assert lc["ndof"] == t.ndofcs[lev - 2]
+        for k in ("ncorner", "nedge", "nface", "ndofc", "nig"):
+            assert lc[k] == oc[k], (lev, k)
+    assert b.level_counts(len(nsublev)) is None
+    assert r == 0 and abs(it - it_o) <= 1
+    assert abs(cond - cond_o) <= 1e-6 * cond_o
+    assert _relmax(b.interface_solution(), u_o) < 1e-10
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(len(u_o))
+    assert _relmax(b.apply_pc(x), t.o1.apply_pc(x)) < 1e-10
+    b.finalize()
+
+    _say(f"case levels {name} mixed precision")
+    b, (it, r, _) = bddcml_amd.solve_cube(p, tol=tol, maxit=400, nsublev=nsublev,
+                                          options={"precision": 32})
+    _say(f"  mixed: {it} iterations")
+    assert r == 0 and abs(it - it_o) <= 1
+    assert _relmax(b.interface_solution(), u_o) < 1e-8
+    b.finalize()
+    t.close()

@@ -142,3 +142,40 @@ def test_several_second_level_subdomains(kind, ns, nsub2):
         sol[d["isvgvn"] - 1] = t.o1.local_solution(d["isub"])
     assert np.abs(sol - u).max() < 1e-9 * np.abs(u).max()
     t.close()
+
+
+def test_four_levels():
+    """Four levels: a third level of one subdomain solves the second level's
+    coarse problem exactly, so [64, 8, 1, 1] reproduces three levels
+    [64, 8, 1]; [64, 8, 2, 1] keeps lambda_min(M S) >= 1 and reaches the
+    direct solution."""
+    from oracle.multilevel import MultiLevels
+    p = CubeProblem(2, 4, kind="poisson", rhs="random", seed=9)
+    t3 = MultiLevels(p.subdomains(), p.nsub, p.nnod, p.ndof, [64, 8, 1], p.numbase)
+    it3, r3, c3 = t3.solve(0, 1e-11, 300, 50)
+    u3 = t3.o1.interface_solution()
+    t3.close()
+    t4 = MultiLevels(p.subdomains(), p.nsub, p.nnod, p.ndof, [64, 8, 1, 1], p.numbase)
+    it4, r4, c4 = t4.solve(0, 1e-11, 300, 50)
+    assert (it4, r4) == (it3, r3 == 0 and 0)
+    assert abs(c4 - c3) < 1e-10 * c3
+    assert np.abs(t4.o1.interface_solution() - u3).max() < 1e-12 * np.abs(u3).max()
+    t4.close()
+    t = MultiLevels(p.subdomains(), p.nsub, p.nnod, p.ndof, [64, 8, 2, 1], p.numbase)
+    assert len(t.levels) == 3
+    n = t.o1.counts()["nig"]
+    E = np.eye(n)
+    S = np.array([t.o1.apply_schur(e) for e in E]).T
+    M = np.array([t.o1.apply_pc(e) for e in E]).T
+    lam = np.sort(np.linalg.eigvals(M @ S).real)
+    assert lam[0] > 1 - 1e-9
+    it, r, cond = t.solve(0, 1e-12, 400, 50)
+    assert r == 0 and it >= it3
+    assert abs(cond - lam[-1]) < 1e-6 * lam[-1]
+    K, f, _ = p.global_dense()
+    u = np.linalg.solve(K, f)
+    sol = np.zeros(p.ndof)
+    for d in p.subdomains():
+        sol[d["isvgvn"] - 1] = t.o1.local_solution(d["isub"])
+    assert np.abs(sol - u).max() < 1e-9 * np.abs(u).max()
+    t.close()

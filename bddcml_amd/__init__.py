@@ -167,6 +167,48 @@ class Bddcml:
             D(dof_data if len(dof_data) else empty), _ip(len(dof_data)),
             _ip(d["find_components"]), _ip(d["use_dual_mesh_graph"]), _ip(d["neighbouring"]))
 
+    def upload_global_data(self, g: dict, idelm: int, neighbouring: int = 1,
+                           load_division: int = 0):
+        """The whole mesh at once (bddcml_upload_global_data_c): g holds
+        nelem, nnod, ndof, ndim, meshdim, inet, nnet, nndf, xyz (nnod x ndim),
+        ifix, fixv, rhs, sol; idelm is an open file descriptor of the ELM file
+        of element matrices (gfortran unformatted sequential records).  The
+        library divides the mesh into this level's subdomains (the file
+        partition_l1.ES in the working directory with load_division)."""
+        keep = []
+
+        def I(x):
+            a, p = _iarr(x)
+            keep.append(a)
+            return p
+
+        def D(x):
+            a, p = _darr(x)
+            keep.append(a)
+            return p
+
+        xyz = np.asfortranarray(g["xyz"], dtype=np.float64)
+        self.L.bddcml_upload_global_data_c(
+            _ip(g["nelem"]), _ip(g["nnod"]), _ip(g["ndof"]), _ip(g["ndim"]), _ip(g["meshdim"]),
+            I(g["inet"]), _ip(len(g["inet"])), I(g["nnet"]), _ip(len(g["nnet"])),
+            I(g["nndf"]), _ip(len(g["nndf"])), D(xyz.ravel(order="F")), _ip(xyz.shape[0]),
+            _ip(xyz.shape[1]), I(g["ifix"]), _ip(len(g["ifix"])), D(g["fixv"]), _ip(len(g["fixv"])),
+            D(g["rhs"]), _ip(len(g["rhs"])), D(g["sol"]), _ip(len(g["sol"])), _ip(idelm),
+            _ip(neighbouring), _ip(load_division))
+
+    def change_global_data(self, ifix, fixv, rhs, sol):
+        """New global Dirichlet values, right-hand side and initial guess of
+        a globally loaded problem, same fixed dofs
+        (bddcml_change_global_data_c); then setup_new_data."""
+        keep = []
+        ia, ip_ = _iarr(ifix)
+        fa, fp = _darr(fixv)
+        ra, rp = _darr(rhs)
+        sa, sp = _darr(sol)
+        keep += [ia, fa, ra, sa]
+        self.L.bddcml_change_global_data_c(ip_, _ip(len(ia)), fp, _ip(len(fa)), rp, _ip(len(ra)),
+                                           sp, _ip(len(sa)))
+
     def change_subdomain_data(self, d: dict):
         """New Dirichlet values, right-hand side and initial guess of one
         subdomain (same fixed dofs), as bddcml_change_subdomain_data_c."""

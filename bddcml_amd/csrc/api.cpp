@@ -72,11 +72,15 @@ catch (const Error &e) {
   record_error(e);
 }
 
-void bddcml_upload_global_data_c(int *, int *, int *, int *, int *, int *, int *, int *, int *,
-                                 int *, int *, double *, int *, int *, int *, int *, double *,
-                                 int *, double *, int *, double *, int *, int *, int *, int *) try {
-  fatal("BDDCML_UPLOAD_GLOBAL_DATA",
-        "global loading needs a mesh partitioner; upload subdomains with bddcml_upload_subdomain_data");
+void bddcml_upload_global_data_c(int *nelem, int *nnod, int *ndof, int *ndim, int *meshdim,
+                                 int *inet, int *linet, int *nnet, int *lnnet, int *nndf,
+                                 int *lnndf, double *xyz, int *lxyz1, int *lxyz2, int *ifix,
+                                 int *lifix, double *fixv, int *lfixv, double *rhs, int *lrhs,
+                                 double *sol, int *lsol, int *idelm, int *neighbouring,
+                                 int *load_division_int) try {
+  upload_global_data(*nelem, *nnod, *ndof, *ndim, *meshdim, inet, *linet, nnet, *lnnet, nndf, *lnndf,
+                     xyz, *lxyz1, *lxyz2, ifix, *lifix, fixv, *lfixv, rhs, *lrhs, sol, *lsol, *idelm,
+                     *neighbouring, *load_division_int != 0);
 }
 catch (const Error &e) {
   record_error(e);
@@ -380,8 +384,9 @@ catch (const Error &e) {
   record_error(e);
 }
 
-void bddcml_change_global_data_c(int *, int *, double *, int *, double *, int *, double *, int *) try {
-  fatal("LEVELS_CHANGE_GLOBAL_DATA", "Global loading should be used.");
+void bddcml_change_global_data_c(int *ifix, int *lifix, double *fixv, int *lfixv, double *rhs,
+                                 int *lrhs, double *sol, int *lsol) try {
+  change_global_data(ifix, *lifix, fixv, *lfixv, rhs, *lrhs, sol, *lsol);
 }
 catch (const Error &e) {
   record_error(e);

@@ -93,6 +93,13 @@ static void bisect(const std::vector<double> &cent, int nd, std::vector<int> ids
   bisect(cent, nd, std::vector<int>(ids.begin() + nl, ids.end()), p - pl, base + pl, part);
 }
 
+std::vector<int> bisect_points(const std::vector<double> &cent, int nd, int n, int nparts) {
+  std::vector<int> part(n, -1), ids(n);
+  std::iota(ids.begin(), ids.end(), 0);
+  bisect(cent, nd, ids, nparts, 0, part);
+  return part;
+}
+
 // Division of level 1's subdomains among nsub2 second-level subdomains by
 // their centroids (mean of their coarse nodes, ascending coarse node order).
 static std::vector<int> divide_subdomains(const Solver &S, int nsub2) {
@@ -108,10 +115,7 @@ static std::vector<int> divide_subdomains(const Solver &S, int nsub2) {
       cent[(size_t)is * nd + d] = sum / (double)cn.size();
     }
   }
-  std::vector<int> part(S.nsub, -1), ids(S.nsub);
-  std::iota(ids.begin(), ids.end(), 0);
-  bisect(cent, nd, ids, nsub2, 0, part);
-  return part;
+  return bisect_points(cent, nd, S.nsub, nsub2);
 }
 
 // The second level's subdomain data (0-based, as uploaded), with the

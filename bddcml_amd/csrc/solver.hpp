@@ -73,6 +73,10 @@ struct Solver {
   // same factorization (count in use, capacity fixed by the first request)
   int rec_n = 0, rec_max = 0;
   std::vector<double> uig;  // interface solution
+  // global loading: the problem was uploaded as one mesh and divided here;
+  // its IFIX is kept because new data must keep the fixed dofs
+  bool global_loaded = false;
+  std::vector<int> global_ifix;
   std::unique_ptr<Device> dev;
   std::unique_ptr<DeviceInput> din;
   // three or more levels: the next level, whose elements are this level's
@@ -107,6 +111,22 @@ void build_orderings(Solver &S);  // interior/interface device orderings
 // built from this one's coarse nodes and coarse matrices and set up
 void build_coarse_nodes(Solver &S);
 void setup_second_level(Solver &S);
+// recursive coordinate bisection of n points (cent: n x nd, row major) into
+// nparts parts: the axis of largest extent (the first on ties), points
+// ordered by (coordinate, index), the first n * floor(p/2) / p to the lower
+// half, parts numbered depth first; returns the part of every point
+std::vector<int> bisect_points(const std::vector<double> &cent, int nd, int n, int nparts);
+// global loading (global.cpp): the mesh divided into this level's
+// subdomains, each uploaded as bddcml_upload_subdomain_data_c would; new
+// global Dirichlet values / right-hand side / initial solution
+void upload_global_data(int nelem, int nnod, int ndof, int ndim, int meshdim, const int *inet,
+                        int linet, const int *nnet, int lnnet, const int *nndf, int lnndf,
+                        const double *xyz, int lxyz1, int lxyz2, const int *ifix, int lifix,
+                        const double *fixv, int lfixv, const double *rhs, int lrhs,
+                        const double *sol, int lsol, int idelm, int neighbouring,
+                        bool load_division);
+void change_global_data(const int *ifix, int lifix, const double *fixv, int lfixv, const double *rhs,
+                        int lrhs, const double *sol, int lsol);
 
 // device steps (device.hip)
 void device_upload_entries(Solver &S, int is);

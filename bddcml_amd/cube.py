@@ -229,6 +229,65 @@ class CubeProblem:
         for s in range(self.nsub):
             yield self.subdomain(s)
 
+    # -- the whole mesh at once (global loading) -----------------------------
+    def global_data(self) -> dict:
+        """Arguments of bddcml_upload_global_data_c (numbase-based indices):
+        the global mesh (elements numbered x fastest, nodes as
+        global_node), Dirichlet data, right-hand side and a zero initial
+        solution, equal to what the subdomains carry."""
+        b = self.numbase
+        ndf = self.ndf
+        nb = self.nel
+        n = nb + 1
+        ek, ej, ei = np.meshgrid(np.arange(nb), np.arange(nb), np.arange(nb), indexing="ij")
+        n1 = self.global_node(ei.ravel(), ej.ravel(), ek.ravel())
+        n2 = n1 + 1
+        n3 = n2 + n
+        n4 = n3 - 1
+        pl = n * n
+        inet = np.stack([n1, n2, n3, n4, n1 + pl, n2 + pl, n3 + pl, n4 + pl], axis=1)
+        kg, jg, ig = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+        ig, jg, kg = ig.ravel(), jg.ravel(), kg.ravel()
+        xyz = np.stack([ig * self.h, jg * self.h, kg * self.h], axis=1)
+        onb = (ig == 0) | (ig == nb) | (jg == 0) | (jg == nb) | (kg == 0) | (kg == nb)
+        ifix = np.repeat(onb.astype(np.int32), ndf)
+        gdof = np.arange(self.ndof)
+        fixv = np.zeros(self.ndof)
+        if self.dirichlet == "random":
+            fixv = np.where(ifix > 0, 2.0 * uniform01(self.seed + 1000, gdof) - 1.0, 0.0)
+        if self.rhs_kind == "const":
+            rhs = np.full(self.ndof, self.h ** 3)
+        else:
+            rhs = 2.0 * uniform01(self.seed, gdof) - 1.0
+        rhs[ifix > 0] = 0.0
+        return dict(nelem=self.nelem, nnod=self.nnod, ndof=self.ndof, ndim=3, meshdim=3,
+                    inet=(inet.ravel() + b).astype(np.int32),
+                    nnet=np.full(self.nelem, 8, dtype=np.int32),
+                    nndf=np.full(self.nnod, ndf, dtype=np.int32),
+                    xyz=np.asfortranarray(xyz), ifix=ifix, fixv=fixv, rhs=rhs,
+                    sol=np.zeros(self.ndof))
+
+    def write_elm(self, path: str):
+        """The element matrices as the reference's ELM file: one gfortran
+        unformatted sequential record per element (4-byte length markers
+        around LELM and LELM doubles), the upper triangle column by column
+        (src/module_sm.f90:170-214, 272-375)."""
+        ne = self.kref.shape[0]
+        jj, ii = np.triu_indices(ne)[::-1]  # column j, rows 0..j
+        order = np.lexsort((ii, jj))
+        ii, jj = ii[order], jj[order]
+        col = self.kref[ii, jj]
+        lelm = len(col)
+        coef = np.ones(self.nelem)
+        if self.coef is not None:
+            coef = np.asarray(self.coef(np.arange(self.nelem)), dtype=float) * coef
+        rec = np.zeros(self.nelem, dtype=[("head", "<i4"), ("lelm", "<i4"), ("a", "<f8", lelm),
+                                          ("tail", "<i4")])
+        rec["head"] = rec["tail"] = 4 + 8 * lelm
+        rec["lelm"] = lelm
+        rec["a"] = coef[:, None] * col[None, :]
+        rec.tofile(path)
+
     # -- global assembly (tests only need this for small cubes) --------------
     def global_dense(self, eliminate: bool = True):
         """Global matrix (full, after Dirichlet elimination that keeps the

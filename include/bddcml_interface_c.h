@@ -24,13 +24,20 @@ extern "C" {
 #endif
 
 /* reference src/bddcml_interface_c.h:6-7, src/module_bddcml.f90:36-101.
- * nl must be 2 (two-level BDDC). */
+ * nl >= 2 levels, nsublev[0..nl-1] non-increasing, nsublev[nl-1] = 1. */
 void bddcml_init_c(int *nl, int *nsublev, int *lnsublev, int *nsub_loc_1, int *comm_init,
                    int *verbose_level, int *numbase, int *just_direct_solve_int);
 
-/* reference src/bddcml_interface_c.h:9-13, src/module_bddcml.f90:104-183.
- * Global loading needs a mesh partitioner; this library accepts
- * per-subdomain loading only and aborts with a message here. */
+/* reference src/bddcml_interface_c.h:9-13, src/module_bddcml.f90:104-183,
+ * src/module_levels.f90:422-552.  The mesh is divided into nsublev[0]
+ * subdomains here: with load_division_int the file partition_l1.ES of the
+ * working directory, as the reference reads it; otherwise recursive
+ * coordinate bisection of the element centroids (where the reference calls
+ * METIS).  idelm: this library has no Fortran runtime, so it is the POSIX
+ * file descriptor of the ELM file (a Fortran caller passes FNUM(unit));
+ * the records are gfortran unformatted sequential ones (4-byte markers),
+ * read from the start as the reference rewinds the unit.  Element
+ * matrices in symmetric storage (matrixtype 1). */
 void bddcml_upload_global_data_c(int *nelem, int *nnod, int *ndof, int *ndim, int *meshdim,
                                  int *inet, int *linet, int *nnet, int *lnnet, int *nndf,
                                  int *lnndf, double *xyz, int *lxyz1, int *lxyz2, int *ifix,
@@ -80,8 +87,10 @@ void bddcml_download_local_reactions_c(int *isub, double *reas, int *lreas);
  * src/module_levels.f90:4785-4900: rea(isvgvn) = local reactions */
 void bddcml_download_global_reactions_c(double *rea, int *lrea);
 
-/* reference src/bddcml_interface_c.h:50, src/module_bddcml.f90:644-679.
- * Needs global loading; aborts with a message. */
+/* reference src/bddcml_interface_c.h:50, src/module_bddcml.f90:644-679,
+ * src/module_levels.f90:555-612: after global loading, new Dirichlet
+ * values, right-hand side and initial solution (IFIX unchanged); then
+ * bddcml_setup_new_data_c. */
 void bddcml_change_global_data_c(int *ifix, int *lifix, double *fixv, int *lfixv, double *rhs,
                                  int *lrhs, double *sol, int *lsol);
 

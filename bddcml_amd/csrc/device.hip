@@ -1104,10 +1104,13 @@ void device_setup_finish(Solver &S) {
     std::vector<TaskList> w_rows = merge_steps(fts, &FactorTasks::w_rows, fcls, fbase_of);
     TaskList kinv = materialize(
         fts, [](const FactorTasks &Q) -> const TaskList * { return &Q.kinv; }, fcls, fbase_of);
+    // the explicit inverse's rows and product count only where the setup
+    // forms it (option 1); the automatic mode forms it during a solve
     S.t.interface_flops = tile_work(p_diag, 2 * kTileChol) + tile_work(p_off, kTileGemm) +
                           tile_work(z_fwd, kTileGemm) + tile_work(z_bwd, kTileGemm) +
-                          kTileGemm * tile_products(phi) + tile_work(w_rows, kTileGemm) +
-                          kTileGemm * tile_products(kinv);
+                          kTileGemm * tile_products(phi);
+    if (D.inv_planned && !D.inv_auto)
+      S.t.interface_flops += tile_work(w_rows, kTileGemm) + kTileGemm * tile_products(kinv);
     D.plan2.verbose = S.verbose;
     upload_phases(D.p_diag, p_diag, D.plan2);
     upload_phases(D.p_off, p_off, D.plan2);

@@ -147,8 +147,9 @@ def main():
     ap.add_argument("--nel", type=int, default=16, help="elements per subdomain edge")
     ap.add_argument("--nsub", type=int, default=8, help="subdomains per cube edge")
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--levels", type=int, default=2, choices=(2, 3),
-                    help="3: a second level of --nsub2 subdomains replaces the coarse solve")
+    ap.add_argument("--levels", type=int, default=2, choices=(2, 3, 4, 5),
+                    help="3 or more: a second level of --nsub2 subdomains replaces the coarse "
+                         "solve, each further level a quarter of the one before (at least 2)")
     ap.add_argument("--nsub2", type=int, default=8, help="subdomains of the second level")
     ap.add_argument("--explicit-inverse", type=int, default=2, choices=(0, 1, 2),
                     help="explicit inverse of S' for the local preconditioner: 0 never, 1 at "
@@ -182,8 +183,12 @@ def main():
 
     if args.workload == "gradient":
         return gradient_workload(args)
+    nsub_next = [args.nsub2]
+    for _ in range(args.levels - 3):
+        nsub_next.append(max(2, nsub_next[-1] // 4))
     levels_txt = ("2 levels" if args.levels == 2 else
-                  f"3 levels ({args.nsub2} subdomains on level 2)")
+                  f"{args.levels} levels ({', '.join(map(str, nsub_next))} subdomains on "
+                  f"levels 2..{args.levels - 1})")
     if args.explicit_inverse != 2:
         levels_txt += (", explicit inverse of S' at setup" if args.explicit_inverse == 1 else
                        ", no explicit inverse of S'")
@@ -210,7 +215,7 @@ def main():
         step_times, results, tims = [], [], []
         for step in range(args.warmup + args.steps):
             b = bddcml_amd.Bddcml()
-            nsublev = [p.nsub, 1] if args.levels == 2 else [p.nsub, args.nsub2, 1]
+            nsublev = [p.nsub, 1] if args.levels == 2 else [p.nsub] + nsub_next + [1]
             b.init(args.levels, nsublev, numbase=p.numbase,
                    verbose_level=int(os.environ.get("BENCH_VERBOSE", "0")))
             b.set_option("interior_ordering", args.ordering)

@@ -47,7 +47,9 @@ def candidate_sets(grid: dict | None = None) -> list[dict]:
 
 def _run(subdomains, nsub, numbase, options, levels, nsub2, setup_args, solve_args):
     b = Bddcml()
-    b.init(levels, [nsub, 1] if levels == 2 else [nsub, nsub2, 1], numbase=numbase)
+    nsublev = list(levels) if isinstance(levels, (list, tuple)) else (
+        [nsub, 1] if levels == 2 else [nsub, nsub2, 1])
+    b.init(len(nsublev), nsublev, numbase=numbase)
     for k, v in options.items():
         if b.set_option(k, v) != 0:
             b.finalize()
@@ -68,7 +70,9 @@ def _run(subdomains, nsub, numbase, options, levels, nsub2, setup_args, solve_ar
 def autotune(subdomains, nsub, numbase=1, candidates=None, repeats=2, levels=2, nsub2=0,
              setup_args=(1, 0, 1, 1, 1, 0, 0, 0), solve_args=(0, 1e-8, 1000, 100),
              agree=1e-6, verbose=False):
-    """Times setup + solve for every candidate option set (best of `repeats`
+    """levels: 2, 3 (with nsub2 second-level subdomains) or the full
+    nsublev list of bddcml_init.
+    Times setup + solve for every candidate option set (best of `repeats`
     runs each, after one untimed run of the first candidate that pays the
     one-time costs) and returns the candidates sorted by time:
     [{"options": {...}, "total_s", "setup_s", "solve_s", "iterations"}, ...].

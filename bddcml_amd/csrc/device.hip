@@ -174,12 +174,14 @@ struct SeqLaunch {
   int kind = GEMM_NT, post = POST_NONE;
   double alpha = 1.0, beta = -1.0;
   bool potrf = false;
+  bool step = false;  // mixed products and factorizations (k_tile_step)
 };
 struct SeqPhase {
   Phase ph;
   int kind = GEMM_NT, post = POST_NONE;
   double alpha = 1.0, beta = -1.0;
   bool potrf = false;
+  bool step = false;
 };
 
 static inline int lowidx(int n, int I, int J) { return J * n - J * (J - 1) / 2 + (I - J); }
@@ -469,7 +471,8 @@ static void dense_chol_tasks(int n, FP Ptile, FL Linv, std::vector<TaskList> &di
 static void check_tasks(const TaskList &l, int64_t ntiles, bool uses_x, const char *what) {
   auto bad = [&](int id) { return id < 0 || id >= ntiles; };
   for (const TileTask &t : l.tasks) {
-    const bool need_x = uses_x || (t.mode >= 0 && ((t.mode >> 2) & 3) != 0);
+    const bool need_x =
+        uses_x || t.mode == kTaskPotrf || (t.mode >= 0 && ((t.mode >> 2) & 3) != 0);
     if (bad(t.c) || (need_x && bad(t.x)) || t.p0 < 0 || t.p1 < t.p0 || t.p1 > (int)l.pairs.size())
       fatal("BDDCML_DEVICE_PLAN", std::string("invalid task in ") + what);
   }
@@ -1509,10 +1512,12 @@ void device_setup_finish(Solver &S) {
       const int n = D.ntc, cb = D.cbase, cl = D.clinv, cw = D.cwbase;
       auto Pc = [=](int I, int J) { return cb + lowidx(n, I, J); };
       auto Wc = [=](int I, int J) { return I == J ? cl + I : cw + lowidx(n - 1, I - 1, J); };
-      // per column k three launches: the diagonal factor; the panel
-      // L_ik = A_ik Linv_k^T together with the W row W_kj = Linv_k B_kj; the
-      // trailing updates A_IK -= L_Ik L_Kk^T together with B_ij -= L_ik W_kj
-      // (tasks of one launch carry their own product kind)
+      // per column k two launches: the panel L_ik = A_ik Linv_k^T together
+      // with the W row W_kj = Linv_k B_kj; the trailing updates
+      // A_IK -= L_Ik L_Kk^T together with B_ij -= L_ik W_kj, where the task
+      // of the next diagonal tile also factors it (L_k+1, Linv_k+1); only
+      // the first diagonal tile has a launch of its own (tasks of one
+      // launch carry their own kind)
       const int panel = task_mode(GEMM_NT, POST_RIGHT_XT, false),
                 wrow = task_mode(GEMM_NN, POST_LEFT_X, false),
                 trail = task_mode(GEMM_NT, POST_NONE, true), bupd = task_mode(GEMM_NN, POST_NONE, true);
@@ -1529,17 +1534,26 @@ void device_setup_finish(Solver &S) {
         l.tasks.push_back(t);
       };
       for (int k = 0; k < n; k++) {
-        SeqLaunch f;  // L_kk = chol(A_kk), Linv_k
-        f.potrf = true;
-        f.l.tasks.push_back({Pc(k, k), cl + k, 0, 0});
-        cseq.push_back(std::move(f));
+        if (k == 0) {
+          SeqLaunch f;  // L_00 = chol(A_00), Linv_0
+          f.potrf = true;
+          f.l.tasks.push_back({Pc(0, 0), cl, 0, 0});
+          cseq.push_back(std::move(f));
+        }
         SeqLaunch pw;
         for (int i = k + 1; i < n; i++) post(pw.l, Pc(i, k), cl + k, panel);
         for (int j = 0; j < k; j++) post(pw.l, Wc(k, j), cl + k, wrow);
         if (!pw.l.tasks.empty()) cseq.push_back(std::move(pw));
         SeqLaunch up;
+        up.step = true;
         for (int K = k + 1; K < n; K++)
-          for (int I = K; I < n; I++) one(up.l, Pc(I, K), Pc(I, k), Pc(K, k), trail);
+          for (int I = K; I < n; I++) {
+            one(up.l, Pc(I, K), Pc(I, k), Pc(K, k), trail);
+            if (I == k + 1) {  // A_k+1,k+1 -= L L^T, then L_k+1 and Linv_k+1
+              up.l.tasks.back().x = cl + k + 1;
+              up.l.tasks.back().mode = kTaskPotrf;
+            }
+          }
         for (int i = k + 1; i < n; i++)
           for (int j = 0; j <= k; j++) one(up.l, Wc(i, j), Pc(i, k), Wc(k, j), bupd);
         if (!up.l.tasks.empty()) cseq.push_back(std::move(up));
@@ -1559,6 +1573,7 @@ void device_setup_finish(Solver &S) {
         q.ph.upload(cseq[k].l, D.plan3);
         q.kind = cseq[k].kind, q.post = cseq[k].post, q.alpha = cseq[k].alpha, q.beta = cseq[k].beta;
         q.potrf = cseq[k].potrf;
+        q.step = cseq[k].step;
       }
       D.cinv.upload(cinv, D.plan3);
       // full-storage mirror of Ac^{-1} for its GEMV; the factor, inverse
@@ -1632,6 +1647,8 @@ void device_setup_finish(Solver &S) {
     for (const SeqPhase &q : D.cseq) {
       if (q.potrf)
         launch_tile_potrf(st, cpool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, D.err.p);
+      else if (q.step)
+        launch_tile_step(st, cpool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, D.err.p);
       else
         launch_tile_gemm(st, cpool, q.ph.tasks.p, q.ph.pairs.p, q.ph.ntask, q.kind, q.post,
                          q.alpha, q.beta);

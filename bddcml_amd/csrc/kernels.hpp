@@ -32,6 +32,10 @@ void launch_schur_gemm(hipStream_t st, double *pool, const TileTask *tasks, cons
 template <typename T>
 void launch_tile_potrf(hipStream_t st, T *pool, const TileTask *tasks, const TilePair *pairs,
                        int ntasks, int *err);
+// tasks of mixed kinds in one launch: tile products with their own mode
+// (task_mode) and tile factorizations (mode kTaskPotrf, C and inverse tile x)
+void launch_tile_step(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+                      int ntasks, int *err);
 // dst tiles (full storage) from lower tiles: for each job (src, dst, transpose)
 struct MirrorJob {
   int src, dst, trans;

@@ -214,11 +214,10 @@ __device__ __forceinline__ void acc_to_lds(typename Mfma<T>::v4 acc[2][2], T *As
 }
 
 template <typename T>
-__device__ __forceinline__ void tile_gemm_task(T *__restrict__ pool, const TileTask t,
-                                               const TilePair *__restrict__ pairs, int kind,
-                                               int post, double alpha, double beta) {
-  __shared__ T As[64 * LDL];
-  __shared__ T Bs[64 * LDL];
+__device__ __forceinline__ void tile_gemm_task_lds(T *__restrict__ pool, const TileTask t,
+                                                   const TilePair *__restrict__ pairs, int kind,
+                                                   int post, double alpha, double beta, T *As,
+                                                   T *Bs) {
   typename Mfma<T>::v4 acc[2][2];
   zero_acc<T>(acc);
   gemm_sum<T>(pool, pairs, t.p0, t.p1, t.base, kind, As, Bs, acc);
@@ -249,6 +248,14 @@ __device__ __forceinline__ void tile_gemm_task(T *__restrict__ pool, const TileT
   acc_to_lds<T>(acc, As);
   __syncthreads();
   lds_to_tile(As, C);
+}
+template <typename T>
+__device__ __forceinline__ void tile_gemm_task(T *__restrict__ pool, const TileTask t,
+                                               const TilePair *__restrict__ pairs, int kind,
+                                               int post, double alpha, double beta) {
+  __shared__ T As[64 * LDL];
+  __shared__ T Bs[64 * LDL];
+  tile_gemm_task_lds<T>(pool, t, pairs, kind, post, alpha, beta, As, Bs);
 }
 
 // Logical task of this workgroup.  Workgroups reach the 8 XCDs round-robin
@@ -363,17 +370,14 @@ template <typename T> __device__ __forceinline__ void chol_inv16(T *As, T *Bs, i
 // X = inv(chol(C)).  Blocked by 16: diagonal blocks in registers (chol_inv16),
 // panel solves, trailing updates and the off-diagonal blocks of the inverse
 // (X_IJ = -inv(L_II) sum_{J<=K<I} L_IK X_KJ) on the matrix cores.
+// C = C - sum of the task's pairs A B^T, then its Cholesky factor L into C
+// and inv(L) into tile t.x; a non-positive pivot sets *err
 template <typename T>
-__global__ __launch_bounds__(NTHR) void k_tile_potrf(T *__restrict__ pool,
-                                                     const TileTask *__restrict__ tasks,
-                                                     const TilePair *__restrict__ pairs,
-                                                     int *__restrict__ err, int chunked) {
+__device__ __forceinline__ void tile_potrf_task(T *__restrict__ pool, const TileTask t,
+                                                const TilePair *__restrict__ pairs,
+                                                int *__restrict__ err, T *As, T *Bs,
+                                                T (*Ts)[16 * 17], int &bad) {
   typedef typename Mfma<T>::v4 v4;
-  __shared__ T As[64 * LDL];
-  __shared__ T Bs[64 * LDL];
-  __shared__ T Ts[4][16 * 17];
-  __shared__ int bad;
-  const TileTask t = tasks[task_index(chunked)];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   v4 acc[2][2];
   zero_acc<T>(acc);
@@ -438,6 +442,38 @@ __global__ __launch_bounds__(NTHR) void k_tile_potrf(T *__restrict__ pool,
   lds_to_tile(As, C);
   lds_to_tile(Bs, pool + (size_t)t.x * TILE_ELEMS);
   if (tid == 0 && bad) atomicOr(err, 1);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NTHR) void k_tile_potrf(T *__restrict__ pool,
+                                                     const TileTask *__restrict__ tasks,
+                                                     const TilePair *__restrict__ pairs,
+                                                     int *__restrict__ err, int chunked) {
+  __shared__ T As[64 * LDL];
+  __shared__ T Bs[64 * LDL];
+  __shared__ T Ts[4][16 * 17];
+  __shared__ int bad;
+  tile_potrf_task<T>(pool, tasks[task_index(chunked)], pairs, err, As, Bs, Ts, bad);
+}
+
+// A launch whose tasks are tile products with their own mode (task_mode) or
+// tile factorizations (mode kTaskPotrf): the trailing update of one column
+// of a right-looking Cholesky together with the factorization of the next
+// diagonal tile, whose update is among the tasks
+__global__ __launch_bounds__(NTHR) void k_tile_step(double *__restrict__ pool,
+                                                    const TileTask *__restrict__ tasks,
+                                                    const TilePair *__restrict__ pairs,
+                                                    int *__restrict__ err, int chunked) {
+  __shared__ double As[64 * LDL];
+  __shared__ double Bs[64 * LDL];
+  __shared__ double Ts[4][16 * 17];
+  __shared__ int bad;
+  const TileTask t = tasks[task_index(chunked)];
+  if (t.mode == kTaskPotrf)
+    tile_potrf_task<double>(pool, t, pairs, err, As, Bs, Ts, bad);
+  else
+    tile_gemm_task_lds<double>(pool, t, pairs, t.mode & 3, (t.mode >> 2) & 3, 1.0,
+                               (t.mode & 16) ? -1.0 : 0.0, As, Bs);
 }
 
 __global__ __launch_bounds__(NTHR) void k_tile_mirror(double *__restrict__ pool,
@@ -549,6 +585,12 @@ template void launch_tile_potrf<double>(hipStream_t, double *, const TileTask *,
                                         int, int *);
 template void launch_tile_potrf<float>(hipStream_t, float *, const TileTask *, const TilePair *, int,
                                        int *);
+void launch_tile_step(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+                      int ntasks, int *err) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_tile_step, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs, err,
+                     g_task_order);
+}
 void launch_tile_mirror(hipStream_t st, double *pool, const MirrorJob *jobs, int njobs) {
   if (njobs <= 0) return;
   hipLaunchKernelGGL(k_tile_mirror, dim3(njobs), dim3(NTHR), 0, st, pool, jobs);

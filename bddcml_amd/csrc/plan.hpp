@@ -106,6 +106,9 @@ struct TileTask {
 };
 // per-task mode: kind | post << 2 | (beta == -1) << 4; alpha is 1
 constexpr int task_mode(int kind, int post, bool minus) { return kind | post << 2 | (minus ? 16 : 0); }
+// task mode of a tile factorization in a mixed launch: C -= sum of its pairs
+// A B^T, then C = chol(C) and tile x = inv(chol(C))
+constexpr int kTaskPotrf = 64;
 struct TilePair {
   int a, b;
 };

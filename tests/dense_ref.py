@@ -89,3 +89,89 @@ def bddc_dense(orc, problem):
     coarse = sum(p["Rc"].T @ p["Phi"].T @ p["W"] @ p["R"] for p in parts)
     Mg += coarse.T @ Acinv @ coarse
     return Sg, Mg, parts, Acg
+
+
+def bicgstab_dense(S, M, g, tol, maxit=500, ndecrmax=50):
+    """BiCGSTAB (van der Vorst 1992) on dense S and M, in the step order of
+    the reference src/module_krylov.f90:1101-1455: zero initial guess,
+    residuals relative to the initial one, the stopping test and the
+    non-decrease counter after the half step (1271) and after the full step
+    (1412).  Returns (iterations, reason, relative residual after every half
+    and full step, solution)."""
+    u = np.zeros_like(g)
+    r = g - S @ u
+    normrhs = np.linalg.norm(r)
+    hist = []
+    rhoold = alpha = omega = 1.0
+    v = np.zeros_like(g)
+    p = np.zeros_like(g)
+    rs = r.copy()
+    ndecr, lastres = 0, 1.0
+    for it in range(1, maxit + 1):
+        rho = r @ rs
+        beta = rho * alpha / (rhoold * omega)
+        p = r + beta * (p - omega * v)
+        y = M @ p
+        v = S @ y
+        alpha = rho / (v @ rs)
+        u = u + alpha * y
+        s = r - alpha * v
+        relres = np.linalg.norm(s) / normrhs
+        hist.append(relres)
+        if relres < tol:
+            return it, 0, np.array(hist), u
+        if relres < lastres:
+            ndecr = 0
+        else:
+            ndecr += 1
+            if ndecr >= ndecrmax:
+                return it, -2, np.array(hist), u
+        lastres = relres
+        z = M @ s
+        t = S @ z
+        omega = (s @ t) / (t @ t)
+        u = u + omega * z
+        r = s - omega * t
+        relres = np.linalg.norm(r) / normrhs
+        hist.append(relres)
+        if relres < tol:
+            return it, 0, np.array(hist), u
+        if it == maxit:
+            return it, -1, np.array(hist), u
+        if relres < lastres:
+            ndecr = 0
+        else:
+            ndecr += 1
+            if ndecr >= ndecrmax:
+                return it, -2, np.array(hist), u
+        lastres = relres
+        rhoold = rho
+    return maxit, -1, np.array(hist), u
+
+
+def weights_closed_form(problem, orc, weights_type, dof_data=None):
+    """Interface weights w_i = rho_i / sum_j rho_j over the subdomains j
+    sharing a dof (src/module_dd.f90:11632-11641), with rho the diagonal of
+    the subdomain's own (unassembled) matrix for type 1 (zeros replaced by 1;
+    src/module_dd.f90:11881-11934, 11690-11697) or the subdomain's dof_data
+    for type 3 (src/module_dd.f90:12020-12060).  Computed from the uploaded
+    triplets with numpy; the oracle supplies only the interface dof lists.
+    Returns {isub: w in the oracle's interface order}."""
+    rho, tot = {}, np.zeros(problem.ndof)
+    b = problem.numbase
+    for d in problem.subdomains():
+        isub = d["isub"]
+        gd = orc.sub_interface_gdofs(isub)
+        l_of = {int(gg): k for k, gg in enumerate(d["isvgvn"] - b)}
+        loc = np.array([l_of[int(gg)] for gg in gd])
+        if weights_type == 1:
+            dg = np.diag(sub_dense(d))
+            r = dg[loc].copy()
+            r[r == 0.0] = 1.0
+        elif weights_type == 3:
+            r = np.asarray(dof_data[isub], dtype=float)[loc]
+        else:
+            r = np.ones(len(loc))
+        rho[isub] = r
+        np.add.at(tot, gd, r)
+    return {isub: rho[isub] / tot[orc.sub_interface_gdofs(isub)] for isub in rho}

@@ -221,3 +221,104 @@ def test_uneven_boxes_with_stiff_layers(oracle_lib, n, ns, kind):
     for d in p.subdomains():
         assert np.abs(o.local_solution(d["isub"]) - u[d["isvgvn"] - 1]).max() <= 1e-10 * scale
     o.close()
+
+
+def _dof_data(problem, seed=21):
+    """Positive per-subdomain dof data for weights of type 3: 0.5 + u with u
+    uniform[0,1) from the seed's stream at (subdomain, local dof)."""
+    from bddcml_amd.cube import uniform01
+    out = {}
+    for d in problem.subdomains():
+        idx = np.arange(d["ndofs"], dtype=np.uint64) + np.uint64(100000 * d["isub"])
+        out[d["isub"]] = 0.5 + uniform01(seed, idx)
+    return out
+
+
+def _oracle_with(oracle_lib, p, weights_type, dof_data=None, tol=1e-10, method=0):
+    o = oracle_lib.Oracle(p.nsub, p.nnod, p.ndof, 3, 3, p.numbase)
+    for d in p.subdomains():
+        o.upload(d, dof_data=None if dof_data is None else dof_data[d["isub"]])
+    o.setup(1, 1, weights_type)
+    res = o.solve(method, tol, 500, 50)
+    return o, res
+
+
+@pytest.mark.parametrize("weights_type", [1, 3])
+def test_weights_closed_form(oracle_lib, weights_type):
+    """Weights of type 1 (subdomain diagonal stiffness) and 3 (dof data)
+    against the closed form rho_i / sum_j rho_j from the uploaded triplets.
+    The stiff slabs make the subdomain diagonals differ across an interface,
+    so type 1 is not type 0: using the assembled diagonal instead of the
+    subdomain's own, or dropping the denominator's neighbour sum, fails."""
+    from bddcml_amd.cube import BoxProblem
+    from tests.dense_ref import weights_closed_form
+    p = BoxProblem(6, 5, seed=3, kind="poisson", layers=3, jump=100.0, thick=1)
+    dd = _dof_data(p) if weights_type == 3 else None
+    o, (it, reason, cond) = _oracle_with(oracle_lib, p, weights_type, dd)
+    assert reason == 0
+    ref = weights_closed_form(p, o, weights_type, dd)
+    far_from_cardinality = 0.0
+    for d in p.subdomains():
+        isub = d["isub"]
+        w = o.sub_weights(isub)
+        np.testing.assert_allclose(w, ref[isub], rtol=1e-14, atol=0)
+        w0 = weights_closed_form(p, o, 0)[isub]
+        far_from_cardinality = max(far_from_cardinality, np.abs(w - w0).max())
+    assert far_from_cardinality > 0.2
+    # the weighted problem still solves the global system
+    A, f, fixed = p.global_dense()
+    u = np.linalg.solve(A, f)
+    assert np.abs(o.interface_solution() - u[o.interface_dofs()]).max() <= 1e-8 * np.abs(u).max()
+    o.close()
+
+
+@pytest.mark.parametrize("case", ["cube3x2", "cube2x3", "boxes_w1"])
+def test_bicgstab_matches_numpy_bicgstab(oracle_lib, case):
+    """The oracle's BiCGSTAB against a numpy BiCGSTAB on the dense S and M
+    (both built independently of the oracle's solves): iteration count,
+    converged reason and every half- and full-step relative residual.  A
+    missing or misplaced half-step test changes the recorded history, and a
+    wrong omega or beta changes every value after the first step."""
+    from bddcml_amd.cube import BoxProblem
+    from tests.dense_ref import bicgstab_dense
+    if case == "cube3x2":
+        p, wt, tol = CubeProblem(3, 2, rhs="random", seed=11), 0, 1e-10
+    elif case == "cube2x3":
+        p, wt, tol = CubeProblem(2, 3, rhs="random", seed=12), 0, 1e-11
+    else:
+        p, wt, tol = BoxProblem(6, 5, seed=3, kind="poisson", layers=3, jump=100.0, thick=1), 1, 1e-10
+    o, (it, reason, cond) = _oracle_with(oracle_lib, p, wt, tol=tol, method=1)
+    assert reason == 0 and cond == -1.0
+    Sg, Mg, _, _ = bddc_dense(o, p)
+    g = o.reduced_rhs()
+    A, f, fixed = p.global_dense()
+    g = np.where(fixed[o.interface_dofs()], 0.0, g)
+    it_ref, reason_ref, hist_ref, u_ref = bicgstab_dense(Sg, Mg, g, tol)
+    hist = o.residual_history()
+    assert (it, reason) == (it_ref, reason_ref)
+    assert len(hist) == len(hist_ref)
+    # residuals agree to rounding amplified by the Krylov recurrences
+    np.testing.assert_allclose(hist, hist_ref, rtol=1e-6, atol=1e-3 * tol)
+    assert np.abs(o.interface_solution() - u_ref).max() <= 1e-9 * np.abs(u_ref).max()
+    o.close()
+
+
+def test_bicgstab_nondecrease_exit(oracle_lib):
+    """ndecrmax = 1: the first half or full step whose residual does not
+    fall below the previous one stops with converged_reason -2 at the same
+    iteration as the numpy restatement."""
+    from bddcml_amd.cube import BoxProblem
+    from tests.dense_ref import bicgstab_dense
+    p = BoxProblem(6, 5, seed=3, kind="poisson", layers=3, jump=100.0, thick=1)
+    o = oracle_lib.Oracle(p.nsub, p.nnod, p.ndof, 3, 3, p.numbase)
+    for d in p.subdomains():
+        o.upload(d)
+    o.setup(1, 1, 0)
+    it, reason, cond = o.solve(1, 1e-14, 500, 1)
+    Sg, Mg, _, _ = bddc_dense(o, p)
+    A, f, fixed = p.global_dense()
+    g = np.where(fixed[o.interface_dofs()], 0.0, o.reduced_rhs())
+    it_ref, reason_ref, hist_ref, _ = bicgstab_dense(Sg, Mg, g, 1e-14, ndecrmax=1)
+    assert (it, reason) == (it_ref, reason_ref)
+    assert reason == -2
+    o.close()

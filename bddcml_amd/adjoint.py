@@ -40,18 +40,21 @@ class AdjointSolver:
         self.b = None
         self.subs = None
         self.problem = None
+        self.k = None  # coefficients of the factorization held by self.b
         self.iterations = []
 
     def close(self):
         if self.b is not None:
             self.b.finalize()
             self.b = None
+        self.k = None
 
     def forward(self, k):
         """Set up and solve for coefficients k (one per global element);
         returns the global solution."""
         self.close()
-        p = self.make_problem(np.asarray(k, dtype=np.float64))
+        k = np.array(k, dtype=np.float64)
+        p = self.make_problem(k)
         self.problem = p
         self.subs = list(p.subdomains())
         b = Bddcml()
@@ -65,7 +68,12 @@ class AdjointSolver:
             raise RuntimeError(f"forward solve did not converge ({reason})")
         self.iterations.append(it)
         self.b = b
+        self.k = k
         return b.download_global_solution(p.ndof)
+
+    def holds(self, k):
+        """True when the kept factorization is the one of coefficients k."""
+        return self.k is not None and np.array_equal(self.k, np.asarray(k, dtype=np.float64))
 
     def adjoint(self, dJdu):
         """Solve A lam = dJ/du (homogeneous Dirichlet) on the factorization
@@ -108,13 +116,20 @@ def _torch_function():
 
         @staticmethod
         def forward(ctx, k, solver):
-            u = solver.forward(k.detach().cpu().numpy())
+            kk = k.detach().cpu().numpy().copy()
+            u = solver.forward(kk)
             ctx.solver = solver
+            ctx.k = kk
             ctx.u = u
             return torch.from_numpy(u).to(k.device)
 
         @staticmethod
         def backward(ctx, grad_u):
+            # the solver holds one factorization: when another forward ran
+            # since this one (several solves before one backward), set this
+            # one's coefficients up again before the adjoint solve
+            if not ctx.solver.holds(ctx.k):
+                ctx.solver.forward(ctx.k)
             lam = ctx.solver.adjoint(grad_u.detach().cpu().numpy())
             g = ctx.solver.gradient(ctx.u, lam)
             return torch.from_numpy(g).to(grad_u.device), None

@@ -46,9 +46,16 @@ def candidate_sets(grid: dict | None = None) -> list[dict]:
 
 
 def _run(subdomains, nsub, numbase, options, levels, nsub2, setup_args, solve_args):
+    if isinstance(levels, (list, tuple)):
+        nsublev = list(levels)
+    elif levels == 2:
+        nsublev = [nsub, 1]
+    elif levels == 3:
+        nsublev = [nsub, nsub2, 1]
+    else:
+        raise ValueError(f"levels={levels}: an integer level count must be 2 or 3; pass the full "
+                         "nsublev list (subdomains of every level, ending with 1) for more levels")
     b = Bddcml()
-    nsublev = list(levels) if isinstance(levels, (list, tuple)) else (
-        [nsub, 1] if levels == 2 else [nsub, nsub2, 1])
     b.init(len(nsublev), nsublev, numbase=numbase)
     for k, v in options.items():
         if b.set_option(k, v) != 0:

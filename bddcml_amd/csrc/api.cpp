@@ -50,6 +50,7 @@ void bddcml_init_c(int *nl, int *nsublev, int *lnsublev, int *nsub_loc_1, int *c
   reset_solver();
   set_task_order(1);
   set_trsv_global(false);
+  set_launch_forms(0);
   Solver &S = solver();
   if (*nl < 2) fatal("LEVELS_INIT", "Number of levels must be at least 2");
   if (*lnsublev < *nl) fatal("LEVELS_INIT", "lnsublev smaller than number of levels");
@@ -438,14 +439,20 @@ catch (const Error &e) {
 
 void bddcml_dotprod_subdomain_c(int *isub, double *vec1, int *lvec1, double *vec2, int *lvec2,
                                 double *dotprod) try {
+  // reference src/module_dd.f90:12736-12788: interface dofs (weighted, in
+  // interface order) first, then interior dofs
+  const char *R = "DD_DOTPROD_SUBDOMAIN_LOCAL";
   Solver &S = solver();
   Sub &s = local_sub(S, *isub, "LEVELS_DD_DOTPROD_SUBDOMAIN_LOCAL");
-  if (*lvec1 != s.ndofs || *lvec2 != s.ndofs)
-    fatal("LEVELS_DD_DOTPROD_SUBDOMAIN_LOCAL", "vector length mismatch");
-  std::vector<double> w(s.ndofs, 1.0);
-  for (size_t k = 0; k < s.ifc_dof.size(); k++) w[s.ifc_dof[k]] = s.w[k];
+  if (!S.is_setup || s.w.size() != s.ifc_dof.size()) fatal(R, "Weights not ready.");
+  if (*lvec1 != *lvec2) fatal(R, "Dimensions mismatch.");
+  if (*lvec1 != s.ndofs) fatal(R, "Dimensions mismatch with local subdomain size.");
   double sum = 0.0;
-  for (int d = 0; d < s.ndofs; d++) sum += vec1[d] * w[d] * vec2[d];
+  for (size_t k = 0; k < s.ifc_dof.size(); k++) {
+    const int d = s.ifc_dof[k];
+    sum += vec1[d] * s.w[k] * vec2[d];
+  }
+  for (int d : s.int_dof) sum += vec1[d] * vec2[d];
   *dotprod = sum;
 }
 catch (const Error &e) {
@@ -580,13 +587,13 @@ catch (const Error &e) {
 }
 int bddcml_amd_timings(double *out, int n) try {
   Solver &S = solver();
-  double v[21] = {S.t.topology,        S.t.plan,           S.t.factor,          S.t.krylov,
+  double v[24] = {S.t.topology,        S.t.plan,           S.t.factor,          S.t.krylov,
                   S.t.postprocess,     S.t.setup_total,    S.t.solve_total,     S.t.rhs,
                   S.t.ev_interior_ms,  S.t.ev_schur_ms,    S.t.ev_interface_ms, S.t.ev_coarse_ms,
                   S.t.schur_flops,     S.t.interior_flops, S.t.interface_flops, S.t.ev_gemv_ms,
                   S.t.gemv_bytes,      (double)S.last_iters, S.t.ev_kapply_ms, S.t.kapply_bytes,
-                  S.t.level2};
-  for (int k = 0; k < std::min(n, 21); k++) out[k] = v[k];
+                  S.t.level2,          S.t.ev_gemm_ms,     S.t.gemm_flops,     (double)S.t.gemm_launches};
+  for (int k = 0; k < std::min(n, 24); k++) out[k] = v[k];
   return 0;
 }
 catch (const Error &e) {
@@ -604,6 +611,11 @@ int bddcml_amd_set_option(const char *key, double value) try {
   }
   if (!std::strcmp(key, "sweep_in_global")) {
     set_trsv_global(value != 0.0);
+    return 0;
+  }
+  if (!std::strcmp(key, "launch_forms")) {
+    if (value != 0.0 && value != 1.0 && value != 2.0) return -2;
+    set_launch_forms((int)value);
     return 0;
   }
   if (!std::strcmp(key, "task_order")) {

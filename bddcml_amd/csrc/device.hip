@@ -214,6 +214,10 @@ struct Device {
   bool borrowed_st = false;  // st belongs to the level above (second level)
   hipStream_t up = nullptr;  // uploads of the second setup half
   hipEvent_t ev[5] = {};     // setup phase boundaries (timing)
+  // start and end of every GEMM launch of the interior factorization (the
+  // launches that take most of the setup's device time)
+  std::vector<hipEvent_t> ev_gemm;
+  std::vector<double> gemm_flops;  // algorithmic flops of each of them
   hipEvent_t ev_up = nullptr;
   hipEvent_t ev_rhs = nullptr;  // reduced right-hand side done (upload stream)
   bool rhs_pending = false;     // it was launched there by the setup
@@ -355,6 +359,8 @@ void device_release(Solver &S) {
     if (D.up) (void)hipStreamSynchronize(D.up);
     if (D.hrr) (void)hipHostFree(D.hrr), D.hrr = nullptr;
     for (hipEvent_t &e : D.ev)
+      if (e) (void)hipEventDestroy(e), e = nullptr;
+    for (hipEvent_t &e : D.ev_gemm)
       if (e) (void)hipEventDestroy(e), e = nullptr;
     if (D.ev_up) (void)hipEventDestroy(D.ev_up), D.ev_up = nullptr;
     if (D.ev_rhs) (void)hipEventDestroy(D.ev_rhs), D.ev_rhs = nullptr;
@@ -917,6 +923,10 @@ void device_setup_begin(Solver &S) {
   double *pool = D.pool.p;
   hipEvent_t *ev = D.ev;
   for (int k = 0; k < 5; k++) HIPCHECK(hipEventCreate(&ev[k]));
+  D.ev_gemm.assign(2 * D.a_off.size(), nullptr);
+  for (hipEvent_t &e : D.ev_gemm) HIPCHECK(hipEventCreate(&e));
+  D.gemm_flops.clear();
+  for (const TaskList &l : a_off) D.gemm_flops.push_back(kTileGemm * (tile_products(l) + l.tasks.size()));
   HIPCHECK(hipEventCreateWithFlags(&D.ev_up, hipEventDisableTiming));
   HIPCHECK(hipEventCreateWithFlags(&D.ev_rhs, hipEventDisableTiming));
   HIPCHECK(hipEventRecord(ev[0], st));
@@ -925,8 +935,10 @@ void device_setup_begin(Solver &S) {
                      D.sc_pads.p, D.err.p);
   for (size_t J = 0; J < D.a_diag.size(); J++) {
     launch_tile_potrf(st, pool, D.a_diag[J].tasks.p, D.a_diag[J].pairs.p, D.a_diag[J].ntask, D.err.p);
+    HIPCHECK(hipEventRecord(D.ev_gemm[2 * J], st));
     launch_tile_gemm(st, pool, D.a_off[J].tasks.p, D.a_off[J].pairs.p, D.a_off[J].ntask, GEMM_NT,
                      POST_RIGHT_XT, 1.0, -1.0);
+    HIPCHECK(hipEventRecord(D.ev_gemm[2 * J + 1], st));
   }
   HIPCHECK(hipEventRecord(ev[1], st));
   launch_schur_gemm(st, pool, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask);
@@ -1250,7 +1262,8 @@ void device_setup_finish(Solver &S) {
       mx = std::max(mx, wk);
       sum += wk;
     }
-    D.kapply_deep = nsub < 128 || (sum > 0.0 && mx > 2.0 * sum / nsub);
+    D.kapply_deep = launch_forms() == 0 ? nsub < 128 || (sum > 0.0 && mx > 2.0 * sum / nsub)
+                                        : launch_forms() == 2;
     if (S.verbose >= 2)
       std::printf("   device plan: local preconditioner %s form (largest sweep %.2fx the mean)\n",
                   D.kapply_deep ? "deep" : "shallow", sum > 0.0 ? mx * nsub / sum : 0.0);
@@ -1670,6 +1683,19 @@ void device_setup_finish(Solver &S) {
     S.t.ev_interface_ms = ms;
     HIPCHECK(hipEventElapsedTime(&ms, ev[3], ev[4]));
     S.t.ev_coarse_ms = ms;
+    // the interior GEMM launches, skipping empty ones
+    S.t.ev_gemm_ms = S.t.gemm_flops = 0.0;
+    S.t.gemm_launches = 0;
+    for (size_t J = 0; J < D.gemm_flops.size(); J++) {
+      if (D.gemm_flops[J] <= 0.0) continue;
+      HIPCHECK(hipEventElapsedTime(&ms, D.ev_gemm[2 * J], D.ev_gemm[2 * J + 1]));
+      S.t.ev_gemm_ms += ms;
+      S.t.gemm_flops += D.gemm_flops[J];
+      S.t.gemm_launches++;
+      if (S.verbose >= 2)
+        std::printf("   interior GEMM launch %zu: %.4e flops, %.3f ms, %.1f TFLOP/s\n", J,
+                    D.gemm_flops[J], ms, D.gemm_flops[J] / (ms * 1e9));
+    }
   }
   int herr = 0;
   HIPCHECK(hipMemcpy(&herr, D.err.p, sizeof(int), hipMemcpyDeviceToHost));
@@ -1971,11 +1997,21 @@ static void level2_apply(Solver &S, const double *rc, double *uc) {
 // the number whose saving (roughly half an apply each) pays for forming it
 constexpr int kInverseAfterApplies = 40;
 
+// automatic explicit inverse, at a solve boundary only (so that one Krylov
+// run sees one preconditioner): formed on every level whose preconditioner
+// has been applied kInverseAfterApplies times on this setup
+static void inverse_at_solve_start(Solver &S) {
+  for (Solver *L = &S; L && L->dev; L = L->lev2.get()) {
+    Device &D = *L->dev;
+    if (D.inv_auto && !D.explicit_inv && D.pc_count >= kInverseAfterApplies) form_inverse(*L);
+  }
+}
+
 static void op_pc(Solver &S, const double *r, double *z, bool timed) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (D.inv_auto && !D.explicit_inv && ++D.pc_count > kInverseAfterApplies) form_inverse(S);
+  if (D.inv_auto && !D.explicit_inv) ++D.pc_count;
   // coarse correction uc = Ac^{-1} sum_i R_ci^T Phi_i^T W_i r_i (three
   // levels: the second level's BDDC in place of Ac^{-1})
   launch_phit(st, D.desc.p, S.nsub, D.Phi.p, D.map.p, D.w.p, r, D.rcs.p, D.maxnc, D.maxNI);
@@ -2181,6 +2217,7 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bo
   *num_iter = 0;
   *reason = 0;
   *cond = (method == 0) ? 1.0 : -1.0;
+  inverse_at_solve_start(S);
   initial_guess(S);
   HIPCHECK(hipMemcpyAsync(D.r.p, D.g.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
   launch_mask(st, n, D.fixmask.p, D.r.p);

@@ -124,6 +124,10 @@ struct TrsvDesc {
 // their per-subdomain vector in LDS when it fits and in global memory
 // otherwise; on = always in global memory (for testing)
 void set_trsv_global(bool on);
+// 0: launch forms chosen by the number of subdomains; 1: the many-subdomain
+// forms; 2: the few-subdomain forms (testing)
+void set_launch_forms(int forms);
+int launch_forms();
 void launch_trsv_fwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
                      const int *rptr, const int *rk, const int *rt, double *v, int maxnt);
 void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,

@@ -418,6 +418,14 @@ static constexpr int kTrsvFewSubs = 128;
 // force the global-memory forms of the per-subdomain kernels (testing)
 static bool g_trsv_global = false;
 void set_trsv_global(bool on) { g_trsv_global = on; }
+// forms chosen by the number of subdomains: 0 by that number, 1 always the
+// many-subdomain forms, 2 always the few-subdomain forms (testing)
+static int g_launch_forms = 0;
+void set_launch_forms(int forms) { g_launch_forms = forms; }
+int launch_forms() { return g_launch_forms; }
+static bool few_subdomains(int nsub) {
+  return g_launch_forms == 0 ? nsub < kTrsvFewSubs : g_launch_forms == 2;
+}
 template <int NW, class K1, class K2>
 static void trsv_launch(hipStream_t st, int nsub, int maxnt, K1 lds_kernel, K2 gv_kernel,
                         const double *pool, const TrsvDesc *d, const int *p, const int *k,
@@ -434,7 +442,7 @@ static void trsv_launch(hipStream_t st, int nsub, int maxnt, K1 lds_kernel, K2 g
 void launch_trsv_fwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
                      const int *rptr, const int *rk, const int *rt, double *v, int maxnt) {
   if (nsub <= 0) return;
-  if (nsub < kTrsvFewSubs)
+  if (few_subdomains(nsub))
     trsv_launch<16>(st, nsub, maxnt, k_trsv_fwd<16, false>, k_trsv_fwd<16, true>, pool, d, rptr,
                     rk, rt, v);
   else
@@ -444,7 +452,7 @@ void launch_trsv_fwd(hipStream_t st, const double *pool, const TrsvDesc *d, int 
 void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
                      const int *cptr, const int *ck, const int *ct, double *v, int maxnt) {
   if (nsub <= 0) return;
-  if (nsub < kTrsvFewSubs)
+  if (few_subdomains(nsub))
     trsv_launch<16>(st, nsub, maxnt, k_trsv_bwd<16, false>, k_trsv_bwd<16, true>, pool, d, cptr,
                     ck, ct, v);
   else

@@ -23,6 +23,10 @@ struct Timings {
   double schur_flops = 0, interior_flops = 0, interface_flops = 0;
   double ev_gemv_ms = 0, gemv_bytes = 0;      // one batched S GEMV launch
   double ev_kapply_ms = 0, kapply_bytes = 0;  // one local preconditioner launch
+  // GEMM launches of the interior factorization (k_tile_gemm<double>): their
+  // summed event time, summed algorithmic flops and number
+  double ev_gemm_ms = 0, gemm_flops = 0;
+  int gemm_launches = 0;
   double level2 = 0;  // second level: mesh, host setup and device setup (s)
   int iterations = 0;
 };
@@ -140,6 +144,9 @@ void set_task_order(int chunked);
 // interior sweeps always with the subdomain vector in global memory (testing;
 // by default only subdomains whose vector does not fit in LDS use that form)
 void set_trsv_global(bool on);
+// forms of the sweeps and of the local preconditioner launch: 0 by the number
+// of subdomains, 1 the many-subdomain forms, 2 the few-subdomain forms (testing)
+void set_launch_forms(int forms);
 // copies the solution of the last solve to the host (once; no-op otherwise)
 void device_fetch_solution(Solver &S);
 void device_setup_begin(Solver &S);

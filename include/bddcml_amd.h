@@ -44,7 +44,10 @@ int bddcml_amd_residual_history(double *out, int maxn);
  * coarse phase; algorithmic flops of the Schur-block launch, the interior
  * factorisation and the interface phase; event milliseconds and algorithmic
  * bytes of one batched Schur GEMV; last iteration count; event milliseconds
- * and algorithmic bytes of one local preconditioner launch */
+ * and algorithmic bytes of one local preconditioner launch; seconds of the
+ * next levels' setup; the GEMM launches of the interior factorization
+ * (k_tile_gemm on FP64 tiles): summed event milliseconds, summed algorithmic
+ * flops (2*64^3 per tile product, post multiplies included) and their number */
 int bddcml_amd_timings(double *out, int n);
 /* options (set after bddcml_init_c, which resets them):
  *   "verbose" (0/1/2);
@@ -67,11 +70,18 @@ int bddcml_amd_timings(double *out, int n);
  *     memory -- the forms used automatically for subdomains whose vector does
  *     not fit in LDS (about 20,000 dofs, or interfaces of about 19,000);
  *     for testing);
+ *   "launch_forms" (0: the interior sweeps and the local preconditioner
+ *     launch take their few-subdomain forms (16 waves per sweep workgroup,
+ *     the deep preconditioner form) below 128 subdomains, the default; 1:
+ *     always the many-subdomain forms; 2: always the few-subdomain forms;
+ *     for testing);
  *   "explicit_inverse" (S'^{-1} formed, W = L^{-1} and W^T W on the tile
  *     GEMM, so that each preconditioner apply reads it once in a symmetric
  *     GEMV instead of sweeping the factor twice: 0 never; 1 at setup; 2, the
- *     default, once the preconditioner has been applied 40 times on the same
- *     setup, where the saved apply time has about paid for forming it) */
+ *     default, at the start of the first solve after the preconditioner has
+ *     been applied 40 times on the same setup, where the saved apply time
+ *     has about paid for forming it; a solve never changes its
+ *     preconditioner while it runs) */
 int bddcml_amd_set_option(const char *key, double value);
 /* per subdomain (isub in numbase): out[0]=ndofi out[1]=nconstr out[2]=ndofo */
 int bddcml_amd_sub_info(int isub, int *out);

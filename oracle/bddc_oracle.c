@@ -1806,13 +1806,12 @@ int orc_dotprod_subdomain(void *h, int isub, const double *v1, const double *v2,
   oprob *P = (oprob *)h;
   osub *s = &P->s[isub - P->numbase];
   if (len != s->ndofs) return -1;
-  double *w = dmalloc(s->ndofs);
-  for (int k = 0; k < s->ndofs; k++) w[k] = 1.0;
-  for (int k = 0; k < s->ndofi; k++) w[s->iivsvn[k]] = s->wi[k];
+  /* dd_dotprod_subdomain_local (module_dd.f90:12736-12788): weighted
+   * interface dofs in interface order, then the interior dofs */
   double sum = 0;
-  for (int k = 0; k < s->ndofs; k++) sum += v1[k] * w[k] * v2[k];
+  for (int k = 0; k < s->ndofi; k++) sum += v1[s->iivsvn[k]] * s->wi[k] * v2[s->iivsvn[k]];
+  for (int k = 0; k < s->ndofo; k++) sum += v1[s->iovsvn[k]] * v2[s->iovsvn[k]];
   *out = sum;
-  free(w);
   return 0;
 }
 

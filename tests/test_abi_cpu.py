@@ -221,3 +221,83 @@ def test_autotune_candidate_grid():
     assert len(c) == n and len({tuple(sorted(d.items())) for d in c}) == n
     assert candidate_sets({"nd_leaf": (64,), "task_order": (0, 1)}) == [
         {"nd_leaf": 64, "task_order": 0}, {"nd_leaf": 64, "task_order": 1}]
+
+
+def _dotprod_in_subprocess(nel, ns, wt, before_setup=False):
+    """bddcml_dotprod_subdomain_c of the product (plan_only setup, host
+    weights) on seeded vectors of every subdomain; with before_setup the call
+    is made before bddcml_setup_preconditioner_c and must fail."""
+    code = f"""
+import sys, json, numpy as np
+sys.path.insert(0, {ROOT!r})
+import bddcml_amd
+from bddcml_amd.cube import BoxProblem, uniform01
+p = BoxProblem({nel}, {ns}, seed=3, kind='poisson', layers=3, jump=100.0, thick=1)
+b = bddcml_amd.Bddcml()
+b.init(2, [p.nsub, 1], numbase=p.numbase)
+b.set_option('plan_only', 1)
+subs = list(p.subdomains())
+for d in subs:
+    b.upload_subdomain_data(d)
+def vec(d, seed):
+    return 2.0 * uniform01(seed, d['isvgvn'].astype(np.uint64)) - 1.0
+if {before_setup!r}:
+    try:
+        b.dotprod_subdomain(1, vec(subs[0], 1), vec(subs[0], 2))
+        print(json.dumps(dict(err='')))
+    except RuntimeError as e:
+        print(json.dumps(dict(err=str(e))))
+    sys.exit(0)
+b.setup_preconditioner(1, 0, 1, 1, 1, 0, 0, {wt})
+out = [b.dotprod_subdomain(d['isub'], vec(d, 1), vec(d, 2)) for d in subs]
+try:
+    b.dotprod_subdomain(1, vec(subs[0], 1), vec(subs[0], 2)[:-1])
+    mismatch = ''
+except RuntimeError as e:
+    mismatch = str(e)
+print(json.dumps(dict(dots=out, mismatch=mismatch)))
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    import json
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+@pytest.mark.parametrize("wt", [0, 1])
+def test_dotprod_subdomain_matches_oracle(oracle_lib, wt):
+    """bddcml_dotprod_subdomain_c (reference src/module_bddcml.f90:731-758,
+    src/module_dd.f90:12736-12788) against the oracle's weighted subdomain
+    dot product, weights of type 0 and 1, and the sum over subdomains of a
+    restricted global vector against the global dot product."""
+    from bddcml_amd.cube import BoxProblem, uniform01
+    p = BoxProblem(6, 5, seed=3, kind="poisson", layers=3, jump=100.0, thick=1)
+    o = oracle_lib.Oracle(p.nsub, p.nnod, p.ndof, 3, 3, p.numbase)
+    for d in p.subdomains():
+        o.upload(d)
+    o.setup(1, 1, wt)
+    prod = _dotprod_in_subprocess(6, 5, wt)
+    tot = 0.0
+    for s, d in enumerate(p.subdomains()):
+        v1 = 2.0 * uniform01(1, d["isvgvn"].astype(np.uint64)) - 1.0
+        v2 = 2.0 * uniform01(2, d["isvgvn"].astype(np.uint64)) - 1.0
+        ref = o.dotprod_subdomain(d["isub"], v1, v2)
+        assert abs(prod["dots"][s] - ref) <= 1e-14 * max(abs(ref), 1.0)
+        tot += prod["dots"][s]
+    g1 = 2.0 * uniform01(1, np.arange(p.ndof, dtype=np.uint64) + 1) - 1.0
+    g2 = 2.0 * uniform01(2, np.arange(p.ndof, dtype=np.uint64) + 1) - 1.0
+    assert abs(tot - g1 @ g2) <= 1e-12 * np.abs(g1 * g2).sum()
+    assert "Dimensions mismatch" in prod["mismatch"]
+    o.close()
+
+
+def test_dotprod_subdomain_needs_weights():
+    """Before the preconditioner setup there are no weights: the call fails
+    as the reference's does ('Weights not ready')."""
+    prod = _dotprod_in_subprocess(6, 5, 0, before_setup=True)
+    assert "Weights not ready" in prod["err"]
+
+
+def test_autotune_rejects_unsupported_level_count():
+    from bddcml_amd.autotune import autotune
+    with pytest.raises(ValueError, match="nsublev"):
+        autotune([], 8, levels=4, candidates=[{}])

@@ -76,3 +76,30 @@ def test_torch_autograd_shim():
     _, g_ref = _dense_gradient(sol.problem, k_np, ut.numpy())
     assert np.abs(k.grad.numpy() - g_ref).max() <= 1e-8 * np.abs(g_ref).max()
     sol.close()
+
+
+def test_torch_autograd_two_solves_one_backward():
+    """Two BddcSolve applies before one backward: each backward sets its own
+    coefficients up again when the solver holds the other factorization, so
+    both gradients match the dense ones."""
+    import torch
+    from bddcml_amd.adjoint import AdjointSolver, BddcSolve
+    _say("case adjoint torch autograd, two solves then one backward")
+    nel, ns = 4, 2
+    p0 = CubeProblem(nel, ns)
+    k1_np = lognormal(5, np.arange(p0.nelem))
+    k2_np = lognormal(8, np.arange(p0.nelem))
+    ut = torch.from_numpy(2.0 * uniform01(6, np.arange(p0.ndof, dtype=np.uint64)) - 1.0)
+    sol = AdjointSolver(_make(nel, ns), p0.kref, tol=1e-12)
+    k1 = torch.tensor(k1_np, dtype=torch.float64, requires_grad=True)
+    k2 = torch.tensor(k2_np, dtype=torch.float64, requires_grad=True)
+    u1 = BddcSolve.apply(k1, sol)
+    u2 = BddcSolve.apply(k2, sol)
+    J = 0.5 * ((u1 - ut) ** 2).sum() + 0.5 * ((u2 - ut) ** 2).sum()
+    J.backward()
+    _, g1 = _dense_gradient(_make(nel, ns)(k1_np), k1_np, ut.numpy())
+    _, g2 = _dense_gradient(_make(nel, ns)(k2_np), k2_np, ut.numpy())
+    assert np.abs(g1 - g2).max() > 1e-3 * np.abs(g1).max()
+    assert np.abs(k1.grad.numpy() - g1).max() <= 1e-8 * np.abs(g1).max()
+    assert np.abs(k2.grad.numpy() - g2).max() <= 1e-8 * np.abs(g2).max()
+    sol.close()

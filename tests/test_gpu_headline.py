@@ -1,0 +1,178 @@
+"""GPU parity at the subdomain shape of the 128^3 Poisson workload.
+
+The 128^3 workload splits the cube into 512 subdomains of 16^3 Q1 elements
+(4,913 dofs; an interior one has 1,538 interface dofs and 26 constraints).
+Here 27 such subdomains (a 48^3 cube, 117,649 dofs) are checked against the
+oracle operator by operator: the centre subdomain has exactly the interior
+shape of the 128^3 workload (tile nested dissection of its 3,375 interior
+dofs, 25 interface tiles).  The launch forms the library picks for 128 or
+more subdomains (4-wave interior sweeps, the shallow local preconditioner)
+and for fewer (16-wave sweeps, the deep form) are both forced through the
+option "launch_forms", so that the forms of the 512-subdomain run are the
+ones tested.  The full 128^3 run is then checked through size-independent
+properties: the residual of the global system recomputed on the host from
+the uploaded triplets, FP64 and mixed-precision solutions agreeing, and the
+residual history reaching the tolerance.
+
+Tolerances (FP64 path, relative to the largest entry of the reference):
+Schur complements 1e-12, coarse basis and coarse matrices 1e-10, S and M
+applies 1e-11 and 1e-10, reduced right-hand side 1e-12, interface and local
+solutions 1e-10, PCG iterations within +-1 and the condition estimate within
+1e-6.  Mixed precision (FP32 preconditioner factor): solutions 1e-8,
+iterations within +-1.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import bddcml_amd
+from bddcml_amd.cube import CubeProblem
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _relmax(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def cube48(oracle_lib):
+    """27 subdomains of 16^3 elements, seeded uniform[-1,1] right-hand side,
+    solved once by the oracle."""
+    _say("case headline shape: oracle on 48^3, 27 subdomains of 16^3")
+    p = CubeProblem(16, 3, rhs="random", seed=2)
+    o, res = oracle_lib.solve_cube(p, tol=TOL, maxit=300)
+    assert res[1] == 0
+    assert o.sub_sizes(14)["ndofi"] == 1538 and o.sub_sizes(14)["nconstr"] == 26
+    yield p, o, res
+    o.close()
+
+
+@pytest.mark.parametrize("forms", [1, 2], ids=["many_subdomain_forms", "few_subdomain_forms"])
+def test_headline_shape_operators_and_solve(cube48, forms):
+    p, o, (it_o, r_o, c_o) = cube48
+    _say(f"case headline shape FP64 launch_forms={forms}")
+    b, (it, r, c) = bddcml_amd.solve_cube(p, tol=TOL, maxit=300,
+                                          options={"launch_forms": forms})
+    try:
+        assert r == 0
+        assert abs(it - it_o) <= 1
+        assert abs(c - c_o) <= 1e-6 * c_o
+        # subdomain operators: the centre (interior shape), a corner and a
+        # face-centre subdomain
+        for isub in (14, 1, 5):
+            S_ref = o.sub_schur(isub)
+            assert _relmax(b.sub_matrix(isub, "S"), S_ref) < 1e-12, f"S of subdomain {isub}"
+            g, ind = b.sub_maps(isub)
+            assert np.array_equal(g, o.sub_interface_gdofs(isub))
+            _, ind_ref = o.sub_constraints(isub)
+            pos = {int(cc): k for k, cc in enumerate(ind_ref)}
+            perm = np.array([pos[int(cc)] for cc in ind])
+            phi, ac = b.sub_coarse(isub)
+            assert _relmax(phi, o.sub_phi(isub)[:, perm]) < 1e-10, f"Phi of subdomain {isub}"
+            assert _relmax(ac, o.sub_coarse(isub)[np.ix_(perm, perm)]) < 1e-10, f"Ac of {isub}"
+        ig = b.interface_dofs()
+        assert np.array_equal(ig, o.interface_dofs())
+        x = np.random.default_rng(4).standard_normal(len(ig))
+        assert _relmax(b.apply_schur(x), o.apply_schur(x)) < 1e-11
+        assert _relmax(b.apply_pc(x), o.apply_pc(x)) < 1e-10
+        assert _relmax(b.reduced_rhs(), o.reduced_rhs()) < 1e-12
+        u_ref = o.interface_solution()
+        assert _relmax(b.interface_solution(), u_ref) < 1e-10
+        for d in p.subdomains():
+            ls = b.download_local_solution(d["isub"], d["ndofs"])
+            assert np.abs(ls - o.local_solution(d["isub"])).max() <= 1e-10 * np.abs(u_ref).max()
+    finally:
+        b.finalize()
+
+
+@pytest.mark.parametrize("options", [
+    {"precision": 32, "launch_forms": 1},
+    {"precision": 32, "launch_forms": 2},
+    {"precision": 64, "launch_forms": 1, "explicit_inverse": 1},
+    {"precision": 32, "launch_forms": 1, "explicit_inverse": 1},
+], ids=["mixed_many", "mixed_few", "fp64_inverse", "mixed_inverse"])
+def test_headline_shape_variants(cube48, options):
+    p, o, (it_o, r_o, c_o) = cube48
+    _say(f"case headline shape variant {options}")
+    b, (it, r, c) = bddcml_amd.solve_cube(p, tol=TOL, maxit=300, options=options)
+    try:
+        assert r == 0
+        assert abs(it - it_o) <= 1
+        bound = 1e-8 if options["precision"] == 32 else 1e-10
+        u_ref = o.interface_solution()
+        assert _relmax(b.interface_solution(), u_ref) < bound
+        for d in p.subdomains():
+            ls = b.download_local_solution(d["isub"], d["ndofs"])
+            assert np.abs(ls - o.local_solution(d["isub"])).max() <= bound * np.abs(u_ref).max()
+        if options["precision"] == 64:
+            x = np.random.default_rng(5).standard_normal(len(u_ref))
+            assert _relmax(b.apply_pc(x), o.apply_pc(x)) < 1e-9
+    finally:
+        b.finalize()
+
+
+def _global_residual(p, subs, u):
+    """||f - A u|| / ||f|| over the free dofs of the global system, with A
+    and f summed from the uploaded subdomain triplets and right-hand sides
+    (upper triangles, applied twice off the diagonal), on the host."""
+    b = p.numbase
+    Au = np.zeros(p.ndof)
+    f = np.zeros(p.ndof)
+    fixed = np.zeros(p.ndof, dtype=bool)
+    for d in subs:
+        gi = d["isvgvn"] - b
+        i = gi[d["i_sparse"] - b]
+        j = gi[d["j_sparse"] - b]
+        a = d["a_sparse"]
+        Au += np.bincount(i, a * u[j], minlength=p.ndof)
+        off = i != j
+        Au += np.bincount(j[off], a[off] * u[i[off]], minlength=p.ndof)
+        f[gi] = d["rhs"]  # complete restriction: equal on every copy
+        fixed[gi[d["ifix"] > 0]] = True
+    r = np.where(fixed, 0.0, f - Au)
+    return np.linalg.norm(r) / np.linalg.norm(np.where(fixed, 0.0, f))
+
+
+def test_full_size_128cube_properties():
+    """128^3 Q1 Poisson, 512 subdomains of 16^3 (2,146,689 dofs), PCG to
+    1e-8 in FP64 and in mixed precision."""
+    _say("case full size 128^3: generating 512 subdomains")
+    p = CubeProblem(16, 8, rhs="random", seed=2)
+    subs = list(p.subdomains())
+    sols, its = {}, {}
+    for prec in (64, 32):
+        _say(f"case full size 128^3 precision={prec}")
+        b = bddcml_amd.Bddcml()
+        b.init(2, [p.nsub, 1], numbase=p.numbase)
+        b.set_option("precision", prec)
+        for d in subs:
+            b.upload_subdomain_data(d)
+        b.setup_preconditioner(1, 0, 1, 1, 1, 0, 0, 0)
+        it, reason, cond = b.solve(0, 1e-8, 500, 50)
+        try:
+            assert reason == 0
+            hist = b.residual_history()
+            assert hist[-1] < 1e-8 and len(hist) == it
+            # interface residual with the library's S, and the global one on
+            # the host from the uploaded triplets
+            ui = b.interface_solution()
+            g = b.reduced_rhs()
+            assert np.linalg.norm(g - b.apply_schur(ui)) / np.linalg.norm(g) < 1e-8
+            u = b.download_global_solution(p.ndof)
+            rel = _global_residual(p, subs, u)
+            _say(f"  iterations {it} cond {cond:.4f} global residual {rel:.3e}")
+            assert rel < 1e-7
+            sols[prec], its[prec] = u, it
+        finally:
+            b.finalize()
+    assert abs(its[64] - its[32]) <= 1
+    assert _relmax(sols[32], sols[64]) < 1e-8

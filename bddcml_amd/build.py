@@ -21,7 +21,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbddcml_amd.so")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
-SOURCES = ["kernels_tile.hip", "kernels_misc.hip", "device.hip", "plan.cpp", "levels.cpp", "global.cpp", "api.cpp"]
+SOURCES = ["kernels_tile.hip", "kernels_misc.hip", "device.hip", "plan.cpp", "levels.cpp", "global.cpp",
+           "api.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-function"]
 HIPFLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-mllvm", "-amdgpu-mfma-vgpr-form"]
@@ -35,6 +36,30 @@ def _run(cmd):
         sys.stderr.write(out + "\n")
         raise RuntimeError("build failed: " + " ".join(cmd))
     return out
+
+
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_LIB = os.path.join(FORTRAN_DIR, "libbddcml_amd_fortran.so")
+FC = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The Fortran interface (module module_bddcml over the C entry points):
+    its module file goes to fortran/include, its code into
+    libbddcml_amd_fortran.so, linked against libbddcml_amd.so.  A Fortran
+    caller compiles with -I bddcml_amd/fortran/include and links both."""
+    inc = os.path.join(FORTRAN_DIR, "include")
+    os.makedirs(inc, exist_ok=True)
+    obj = os.path.join(inc, "module_bddcml.o")
+    out = _run([FC, "-O2", "-fPIC", "-module-dir", inc, "-c",
+                os.path.join(FORTRAN_DIR, "module_bddcml.f90"), "-o", obj])
+    if verbose and out:
+        print(out)
+    tmp = FORTRAN_LIB + ".tmp"
+    _run([FC, "-shared", "-o", tmp, obj, "-L" + HERE, "-lbddcml_amd", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, FORTRAN_LIB)
+    os.remove(obj)
+    return FORTRAN_LIB
 
 
 def build(verbose: bool = False) -> str:
@@ -59,6 +84,7 @@ def build(verbose: bool = False) -> str:
          ["-Wl,-rpath," + rocm_llvm, "-L" + rocm_llvm])
     os.replace(tmp, LIB)
     shutil.rmtree(objdir, ignore_errors=True)
+    build_fortran(verbose)
     return LIB
 
 

@@ -51,6 +51,7 @@ void bddcml_init_c(int *nl, int *nsublev, int *lnsublev, int *nsub_loc_1, int *c
   set_task_order(1);
   set_trsv_global(false);
   set_launch_forms(0);
+  set_pcg_graph(true);
   Solver &S = solver();
   if (*nl < 2) fatal("LEVELS_INIT", "Number of levels must be at least 2");
   if (*lnsublev < *nl) fatal("LEVELS_INIT", "lnsublev smaller than number of levels");
@@ -611,6 +612,11 @@ int bddcml_amd_set_option(const char *key, double value) try {
   }
   if (!std::strcmp(key, "sweep_in_global")) {
     set_trsv_global(value != 0.0);
+    return 0;
+  }
+  if (!std::strcmp(key, "pcg_graph")) {
+    if (value != 0.0 && value != 1.0) return -2;
+    set_pcg_graph(value != 0.0);
     return 0;
   }
   if (!std::strcmp(key, "launch_forms")) {

@@ -299,7 +299,15 @@ struct Device {
   DBuf<double> recV, recW;    // recycling basis V and W = S V (columns of length nig)
   DBuf<double> recc, recpart; // its projection coefficients and partial sums
   double *hrr = nullptr;      // pinned host copies of r.r per iteration
+  double *hrr_dev = nullptr;  // the same memory as the device addresses it
   size_t hrr_n = 0;
+  // PCG iteration captured as a graph (iterations from the third on):
+  // replayed with the iteration index in kcount; rebuilt when the buffers
+  // it was captured with or any level's preconditioner form change
+  hipGraph_t pcg_graph = nullptr;
+  hipGraphExec_t pcg_exec = nullptr;
+  std::vector<const void *> pcg_key;
+  DBuf<int> kcount;
   DBuf<double> u, r, p, z, ap, rs, v, y, s, t, g, xin, xout;
   View<char> fixmask;
   int nig = 0;
@@ -358,6 +366,8 @@ void device_release(Solver &S) {
     if (D.st) (void)hipStreamSynchronize(D.st);
     if (D.up) (void)hipStreamSynchronize(D.up);
     if (D.hrr) (void)hipHostFree(D.hrr), D.hrr = nullptr;
+    if (D.pcg_exec) (void)hipGraphExecDestroy(D.pcg_exec), D.pcg_exec = nullptr;
+    if (D.pcg_graph) (void)hipGraphDestroy(D.pcg_graph), D.pcg_graph = nullptr;
     for (hipEvent_t &e : D.ev)
       if (e) (void)hipEventDestroy(e), e = nullptr;
     for (hipEvent_t &e : D.ev_gemm)
@@ -1814,6 +1824,11 @@ double device_selftest_copy(long long nbytes) {
   return ms;
 }
 
+hipStream_t device_stream(Solver &S) {
+  if (!S.dev || !S.dev->st) fatal("BDDCML_DEVICE", "solver not set up on the device");
+  return S.dev->st;
+}
+
 void device_element_gradient(Solver &S, const double *ae, int nke, bool per_element,
                              const double *u, const double *lam, double *grad) {
   const char *R = "BDDCML_AMD_ELEMENT_GRADIENT";
@@ -2207,6 +2222,84 @@ static void initial_guess(Solver &S) {
   HIPCHECK(hipMemcpyAsync(D.u.p, u0.data(), S.nig * sizeof(double), hipMemcpyHostToDevice, D.st));
 }
 
+// ---------------------------------------------------- PCG iteration graph
+static bool g_pcg_graph = true;
+void set_pcg_graph(bool on) { g_pcg_graph = on; }
+
+// preconditioner applies counted per level (the automatic explicit inverse
+// looks at them); a graph replay runs one apply on every level
+static void count_pc_applies(Solver &S, int add) {
+  for (Solver *L = &S; L && L->dev; L = L->lev2.get())
+    if (L->dev->inv_auto && !L->dev->explicit_inv) L->dev->pc_count += add;
+}
+
+// what a captured iteration depends on beyond the solver's fixed buffers
+static std::vector<const void *> pcg_graph_key(Solver &S, const double *rz) {
+  std::vector<const void *> key{rz, S.dev->hrr_dev};
+  for (Solver *L = &S; L && L->dev; L = L->lev2.get())
+    key.push_back(L->dev->explicit_inv ? (const void *)L->dev->fpool.p : nullptr);
+  return key;
+}
+
+// body of one PCG iteration k = *kcount, as the launches of the eager loop:
+// ap = S p, p.Ap, u and r updated, r.r (also to the pinned host slot), the
+// next direction z = M r, r.z, p = z + beta p, k stepped by one
+static void pcg_graph_capture(Solver &S, double *rz, double *pap, double *rr) {
+  Device &D = *S.dev;
+  hipStream_t st = D.st;
+  const int n = S.nig;
+  if (D.pcg_exec) {
+    HIPCHECK(hipGraphExecDestroy(D.pcg_exec));
+    D.pcg_exec = nullptr;
+  }
+  if (D.pcg_graph) {
+    HIPCHECK(hipGraphDestroy(D.pcg_graph));
+    D.pcg_graph = nullptr;
+  }
+  if (!D.kcount.p) D.kcount.alloc(1);
+  // op_pc counts its applies on the host when it is called; the capture is
+  // not an apply
+  std::vector<int> counts;
+  for (Solver *L = &S; L && L->dev; L = L->lev2.get()) counts.push_back(L->dev->pc_count);
+  HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+  op_schur(S, D.p.p, D.ap.p);
+  launch_dots_at(st, n, D.p.p, D.ap.p, D.partial.p, pap, D.kcount.p, nullptr);
+  launch_pcg_update_at(st, n, rz, pap, D.kcount.p, D.p.p, D.ap.p, D.u.p, D.r.p);
+  launch_dots_at(st, n, D.r.p, D.r.p, D.partial.p, rr, D.kcount.p, D.hrr_dev);
+  op_pc(S, D.r.p, D.z.p);
+  launch_dots_at(st, n, D.r.p, D.z.p, D.partial.p, rz, D.kcount.p, nullptr);
+  launch_pcg_dir_at(st, n, rz, D.kcount.p, D.z.p, D.p.p);
+  launch_count_inc(st, D.kcount.p);
+  HIPCHECK(hipStreamEndCapture(st, &D.pcg_graph));
+  HIPCHECK(hipGraphInstantiate(&D.pcg_exec, D.pcg_graph, nullptr, nullptr, 0));
+  int q = 0;
+  for (Solver *L = &S; L && L->dev; L = L->lev2.get()) L->dev->pc_count = counts[q++];
+  D.pcg_key = pcg_graph_key(S, rz);
+}
+
+// iteration k by the graph; returns once r.r of iteration k is in hrr[k]
+// (the rest of the iteration, the next direction, may still run)
+static void pcg_graph_replay(Solver &S, int k, double *rz, double *pap, double *rr) {
+  Device &D = *S.dev;
+  hipStream_t st = D.st;
+  if (!D.pcg_exec || D.pcg_key != pcg_graph_key(S, rz)) pcg_graph_capture(S, rz, pap, rr);
+  if (k == 3) launch_count_set(st, D.kcount.p, k);  // later replays step it themselves
+  volatile double *h = D.hrr + k;
+  *h = -1.0;  // r.r >= 0 replaces it
+  HIPCHECK(hipGraphLaunch(D.pcg_exec, st));
+  count_pc_applies(S, 1);
+  // wait for the value; if the stream has drained without writing it,
+  // something failed: stop instead of waiting forever
+  for (long polls = 1; *h < 0.0; polls++) {
+    if ((polls & 255) == 0) {
+      const hipError_t q = hipStreamQuery(st);
+      if (q == hipSuccess && *h < 0.0)
+        fatal("KRYLOV_BDDCPCG", "PCG iteration graph finished without its residual");
+      if (q != hipSuccess && q != hipErrorNotReady) HIPCHECK(q);
+    }
+  }
+}
+
 void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bool recycling,
                   int maxvec, int *num_iter, int *reason, double *cond) {
   Device &D = *S.dev;
@@ -2266,12 +2359,19 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bo
     // of iteration k; the host reads only rr[k], and while it waits the
     // device already applies the preconditioner for the next direction
     // (work that is simply unused when the residual test stops)
+    // (kept across solves while large enough, so that a captured iteration
+    // graph stays valid)
     const size_t m = (size_t)maxit + 2;
-    D.pcg.alloc(3 * m);
-    double *rz = D.pcg.p, *pap = D.pcg.p + m, *rr = D.pcg.p + 2 * m;
+    if (D.pcg.n < 3 * m) D.pcg.alloc(3 * m);
+    const size_t mc = D.pcg.n / 3;
+    double *rz = D.pcg.p, *pap = D.pcg.p + mc, *rr = D.pcg.p + 2 * mc;
     if (D.hrr_n < m) {
       if (D.hrr) (void)hipHostFree(D.hrr), D.hrr = nullptr;
-      HIPCHECK(hipHostMalloc((void **)&D.hrr, m * sizeof(double), hipHostMallocDefault));
+      // mapped and coherent: the graph's dot kernel writes r.r here and the
+      // host polls it
+      HIPCHECK(hipHostMalloc((void **)&D.hrr, m * sizeof(double),
+                             hipHostMallocMapped | hipHostMallocCoherent));
+      HIPCHECK(hipHostGetDevicePointer((void **)&D.hrr_dev, D.hrr, 0));
       D.hrr_n = m;
     }
     dots_dev(S, D.r.p, D.p.p, rz);
@@ -2279,7 +2379,14 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bo
     HIPCHECK(hipEventCreateWithFlags(&evr, hipEventDisableTiming));
     int ndecr = 0, nw = 0, last = 0;
     double lastres = 1.0;
+    // iterations from the third on replay one captured graph of the
+    // iteration (recycling keeps the launches: its basis grows each step)
+    const bool graph = g_pcg_graph && !rec;
     for (int iter = 1; iter <= maxit; iter++) {
+      if (graph && iter >= 3) {
+        pcg_graph_replay(S, iter, rz, pap, rr);
+        goto residual_test;
+      }
       if (rec && S.rec_n > 0)
         for (int pass = 0; pass < 2; pass++) {
           launch_multidot(st, n, D.recW.p, n, S.rec_n, D.p.p, D.recpart.p, D.recc.p);
@@ -2301,6 +2408,7 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bo
       dots_dev(S, D.r.p, D.z.p, rz + iter);
       launch_pcg_dir(st, n, rz, iter, D.z.p, D.p.p);
       HIPCHECK(hipEventSynchronize(evr));
+    residual_test:
       last = iter;
       double relres = std::sqrt(D.hrr[iter]) / normres0;
       S.hist.push_back(relres);

@@ -224,6 +224,18 @@ void launch_pcg_update(hipStream_t st, int n, const double *rz, const double *pa
                        const double *p, const double *ap, double *u, double *r);
 // p = z + beta p with beta = rz[k] / rz[k-1]
 void launch_pcg_dir(hipStream_t st, int n, const double *rz, int k, const double *z, double *p);
+// forms with the iteration index k read from device memory (*kp), for a
+// PCG iteration captured once as a graph: base[*kp] = a.b (and host[*kp],
+// pinned host memory, when host is not null); the two updates at *kp; the
+// counter set and stepped by one
+void launch_dots_at(hipStream_t st, int n, const double *a, const double *b, double *partial,
+                    double *base, const int *kp, double *host);
+void launch_pcg_update_at(hipStream_t st, int n, const double *rz, const double *pap, const int *kp,
+                          const double *p, const double *ap, double *u, double *r);
+void launch_pcg_dir_at(hipStream_t st, int n, const double *rz, const int *kp, const double *z,
+                       double *p);
+void launch_count_set(hipStream_t st, int *kp, int v);
+void launch_count_inc(hipStream_t st, int *kp);
 // Krylov recycling basis (columns of length n, leading dimension ld):
 // out[j] = V(:, j) . x for j < m (deterministic two-stage sums; partial
 // holds DOT_BLOCKS * m doubles)

@@ -931,6 +931,42 @@ void launch_dots(hipStream_t st, int n, DotArgs args, double *partial, double *o
   hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(256), 0, st, DOT_BLOCKS, args.ndot, partial, out);
 }
 
+// The same single dot product with its result stored at base[*kp] (and,
+// when host is not null, at host[*kp], pinned host memory the host polls):
+// the slot index is read on the device, so that one captured graph of the
+// PCG iteration serves every iteration.
+__global__ void k_dots_final_at(int nb, const double *__restrict__ partial,
+                                double *__restrict__ base, const int *__restrict__ kp,
+                                double *host) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += partial[b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int k = *kp;
+    base[k] = red[0];
+    if (host) {
+      host[k] = red[0];
+      __threadfence_system();
+    }
+  }
+}
+void launch_dots_at(hipStream_t st, int n, const double *a, const double *b, double *partial,
+                    double *base, const int *kp, double *host) {
+  DotArgs args{};
+  args.ndot = 1;
+  args.a[0] = a;
+  args.b[0] = b;
+  hipLaunchKernelGGL(k_dots, dim3(DOT_BLOCKS), dim3(256), 0, st, n, args, partial);
+  hipLaunchKernelGGL(k_dots_final_at, dim3(1), dim3(256), 0, st, DOT_BLOCKS, partial, base, kp,
+                     host);
+}
+
 __global__ void k_axpy2(int n, double a, const double *__restrict__ x, double *__restrict__ y,
                         double a2, const double *__restrict__ x2, double *__restrict__ y2) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -976,6 +1012,48 @@ __global__ void k_pcg_dir(int n, const double *__restrict__ rz, int k,
 void launch_pcg_dir(hipStream_t st, int n, const double *rz, int k, const double *z, double *p) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_pcg_dir, dim3((n + 255) / 256), dim3(256), 0, st, n, rz, k, z, p);
+}
+
+// iteration-index forms of the two PCG updates (index read from *kp) and
+// the counter step that ends a graph replay
+__global__ void k_pcg_update_at(int n, const double *__restrict__ rz, const double *__restrict__ pap,
+                                const int *__restrict__ kp, const double *__restrict__ p,
+                                const double *__restrict__ ap, double *__restrict__ u,
+                                double *__restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = *kp;
+  const double alpha = rz[k - 1] / pap[k];
+  u[i] += alpha * p[i];
+  r[i] -= alpha * ap[i];
+}
+void launch_pcg_update_at(hipStream_t st, int n, const double *rz, const double *pap, const int *kp,
+                          const double *p, const double *ap, double *u, double *r) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pcg_update_at, dim3((n + 255) / 256), dim3(256), 0, st, n, rz, pap, kp, p,
+                     ap, u, r);
+}
+__global__ void k_pcg_dir_at(int n, const double *__restrict__ rz, const int *__restrict__ kp,
+                             const double *__restrict__ z, double *__restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = *kp;
+  const double beta = rz[k] / rz[k - 1];
+  p[i] = z[i] + beta * p[i];
+}
+void launch_pcg_dir_at(hipStream_t st, int n, const double *rz, const int *kp, const double *z,
+                       double *p) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pcg_dir_at, dim3((n + 255) / 256), dim3(256), 0, st, n, rz, kp, z, p);
+}
+__global__ void k_count_set(int *kp, int v, int add) {
+  if (threadIdx.x == 0) *kp = add ? *kp + v : v;
+}
+void launch_count_set(hipStream_t st, int *kp, int v) {
+  hipLaunchKernelGGL(k_count_set, dim3(1), dim3(64), 0, st, kp, v, 0);
+}
+void launch_count_inc(hipStream_t st, int *kp) {
+  hipLaunchKernelGGL(k_count_set, dim3(1), dim3(64), 0, st, kp, 1, 1);
 }
 
 // ------------------------------------------------ Krylov recycling basis

@@ -147,6 +147,9 @@ void set_trsv_global(bool on);
 // forms of the sweeps and of the local preconditioner launch: 0 by the number
 // of subdomains, 1 the many-subdomain forms, 2 the few-subdomain forms (testing)
 void set_launch_forms(int forms);
+// PCG iterations from the third on as replays of one captured graph (the
+// default) or as separate launches
+void set_pcg_graph(bool on);
 // copies the solution of the last solve to the host (once; no-op otherwise)
 void device_fetch_solution(Solver &S);
 void device_setup_begin(Solver &S);

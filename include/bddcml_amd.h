@@ -70,6 +70,11 @@ int bddcml_amd_timings(double *out, int n);
  *     memory -- the forms used automatically for subdomains whose vector does
  *     not fit in LDS (about 20,000 dofs, or interfaces of about 19,000);
  *     for testing);
+ *   "pcg_graph" (1, the default: PCG iterations from the third on replay
+ *     one captured graph of the iteration, with the iteration index read
+ *     from device memory and r.r written to pinned host memory for the
+ *     stopping test; 0: every iteration as separate launches; the two give
+ *     the same arithmetic in the same order);
  *   "launch_forms" (0: the interior sweeps and the local preconditioner
  *     launch take their few-subdomain forms (16 waves per sweep workgroup,
  *     the deep preconditioner form) below 128 subdomains, the default; 1:

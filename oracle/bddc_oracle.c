@@ -1217,6 +1217,7 @@ static void sub_maps(const osub *s, int *o2l, int *i2l) {
 
 /* explicit Schur complement, aug LU, coarse basis (module_dd.f90:4175-4296,
  * 5211-5487, 5489-5800) */
+static int sub_coarse_basis(osub *s);
 static int sub_numerics(osub *s) {
   int *o2l = imalloc(s->ndofs), *i2l = imalloc(s->ndofs);
   sub_maps(s, o2l, i2l);
@@ -1244,6 +1245,19 @@ static int sub_numerics(osub *s) {
     free(x);
     free(y);
   }
+  free(o2l);
+  free(i2l);
+  return sub_coarse_basis(s);
+}
+
+/* Phi_i and Ac_i from [S C^T; C 0] (dd_prepare_aug, dd_prepare_coarse:
+ * module_dd.f90:5211-5800) for the current constraint rows C */
+static int sub_coarse_basis(osub *s) {
+  int ni = s->ndofi;
+  free(s->aug);
+  free(s->augpiv);
+  free(s->phi);
+  free(s->acs);
   int nc = s->nconstr, na = ni + nc;
   s->aug = dmalloc((size_t)na * na);
   s->augpiv = imalloc(na);
@@ -1270,8 +1284,6 @@ static int sub_numerics(osub *s) {
     for (int i = 0; i < nc; i++) s->acs[(size_t)i * nc + j] = -b[ni + i];
   }
   free(b);
-  free(o2l);
-  free(i2l);
   return 0;
 }
 
@@ -1351,6 +1363,25 @@ static void setup_reduced_rhs(oprob *P) {
   free(g);
 }
 
+/* coarse problem (module_levels.f90:3001-3169, dd_get_my_coarsem 6384-6464):
+ * the Ac_i summed through indrowc and factored */
+static int coarse_problem(oprob *P) {
+  int nc = P->ndofc;
+  free(P->Ac);
+  P->Ac = dmalloc((size_t)nc * nc);
+  for (int is = 0; is < P->nsub; is++) {
+    osub *s = &P->s[is];
+    for (int i = 0; i < s->nconstr; i++)
+      for (int j = 0; j < s->nconstr; j++)
+        P->Ac[(size_t)s->indrowc[i] * nc + s->indrowc[j]] += s->acs[(size_t)i * s->nconstr + j];
+  }
+  if (nc > 0 && chol_factor(P->Ac, nc)) {
+    fprintf(stderr, "ERROR in ORACLE: coarse matrix not positive definite\n");
+    return -1;
+  }
+  return 0;
+}
+
 int orc_setup(void *h, int use_corner, int use_arith, int weights_type) {
   oprob *P = (oprob *)h;
   for (int is = 0; is < P->nsub; is++)
@@ -1398,19 +1429,7 @@ int orc_setup(void *h, int use_corner, int use_arith, int weights_type) {
 #pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
   for (int is = 0; is < P->nsub; is++) err |= sub_numerics(&P->s[is]) ? 1 : 0;
   if (err) return -1;
-  /* coarse problem (module_levels.f90:3001-3169, dd_get_my_coarsem 6384-6464) */
-  int nc = P->ndofc;
-  P->Ac = dmalloc((size_t)nc * nc);
-  for (int is = 0; is < P->nsub; is++) {
-    osub *s = &P->s[is];
-    for (int i = 0; i < s->nconstr; i++)
-      for (int j = 0; j < s->nconstr; j++)
-        P->Ac[(size_t)s->indrowc[i] * nc + s->indrowc[j]] += s->acs[(size_t)i * s->nconstr + j];
-  }
-  if (nc > 0 && chol_factor(P->Ac, nc)) {
-    fprintf(stderr, "ERROR in ORACLE: coarse matrix not positive definite\n");
-    return -1;
-  }
+  if (coarse_problem(P)) return -1;
   setup_reduced_rhs(P);
   /* unique interface numbering for probes */
   int *isig = imalloc(P->ndof_g);
@@ -2036,4 +2055,43 @@ int orc_apply_level(void *h, const double *f, double *x) {
   free_rep(P, g);
   free_rep(P, z);
   return 0;
+}
+
+/* ------------------------------------------------- general constraint rows */
+/* Replace the constraint rows of subdomain isub by nc dense rows over its
+ * interface dofs (C: nc x ndofi row major, the oracle's interface order) with
+ * global coarse dofs indrowc (0-based), the new coarse problem having ndofc
+ * dofs.  Used for adaptive constraints (dd_load_adaptive_constraints and
+ * dd_orthogonalize_constraints, module_dd.f90:4386-4457, 4637-4802); the
+ * coarse basis follows with orc_refresh_coarse. */
+int orc_set_constraints(void *h, int isub, int nc, const double *C, const int *indrowc,
+                        int ndofc) {
+  oprob *P = (oprob *)h;
+  int is = isub - P->numbase;
+  if (is < 0 || is >= P->nsub || !P->is_setup) return -1;
+  osub *s = &P->s[is];
+  for (int r = 0; r < nc; r++)
+    if (indrowc[r] < 0 || indrowc[r] >= ndofc) return -2;
+  free(s->C);
+  free(s->indrowc);
+  s->nconstr = nc;
+  s->C = dmalloc((size_t)nc * s->ndofi);
+  memcpy(s->C, C, sizeof(double) * nc * s->ndofi);
+  s->indrowc = imalloc(nc > 0 ? nc : 1);
+  memcpy(s->indrowc, indrowc, sizeof(int) * nc);
+  P->ndofc = ndofc;
+  return 0;
+}
+
+/* Phi_i, Ac_i of every subdomain and the coarse factor for the current
+ * constraint rows (dd_prepare_c, dd_prepare_aug, dd_prepare_coarse after the
+ * adaptive constraints are loaded: module_levels.f90:2938-2945) */
+int orc_refresh_coarse(void *h) {
+  oprob *P = (oprob *)h;
+  if (!P->is_setup) return -1;
+  int err = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+  for (int is = 0; is < P->nsub; is++) err |= sub_coarse_basis(&P->s[is]) ? 1 : 0;
+  if (err) return -2;
+  return coarse_problem(P) ? -3 : 0;
 }

@@ -94,6 +94,14 @@ void orc_set_coarse_solver(void *h, orc_coarse_fn fn, void *ctx);
  * x_G = M (f_G - A21 A11^{-1} f_o) assembled, x_o = A11^{-1}(f_o - A12 x_G).
  * Returns 0, or -1 where Dirichlet data would be needed. */
 int orc_apply_level(void *h, const double *f, double *x);
+/* replace the constraint rows of one subdomain by nc dense rows over its
+ * interface dofs (row major, the oracle's interface order) with global
+ * coarse dofs indrowc (0-based) of a coarse problem of ndofc dofs; then
+ * orc_refresh_coarse rebuilds every Phi_i, Ac_i and the coarse factor
+ * (adaptive constraints, module_dd.f90:4386-4457, 4637-4802) */
+int orc_set_constraints(void *h, int isub, int nc, const double *C, const int *indrowc,
+                        int ndofc);
+int orc_refresh_coarse(void *h);
 
 #ifdef __cplusplus
 }

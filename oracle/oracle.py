@@ -76,6 +76,10 @@ def _declare(L):
     L.orc_set_coarse_solver.argtypes = [_v, COARSE_FN, _v]
     L.orc_apply_level.restype = _i
     L.orc_apply_level.argtypes = [_v, _dp, _dp]
+    L.orc_set_constraints.restype = _i
+    L.orc_set_constraints.argtypes = [_v, _i, _i, _dp, _ip, _i]
+    L.orc_refresh_coarse.restype = _i
+    L.orc_refresh_coarse.argtypes = [_v]
 
 
 # uc = f(ctx, rc) in place of the coarse Cholesky solve (three levels)
@@ -248,6 +252,24 @@ class Oracle:
         if self.L.orc_apply_level(self.h, _da(f), x):
             raise RuntimeError("orc_apply_level needs a level without Dirichlet data")
         return x
+
+    def set_constraints(self, isub, C, indrowc, ndofc):
+        """Replace subdomain isub's constraint rows by the dense rows C
+        (nc x ndofi, the oracle's interface order) with global coarse dofs
+        indrowc of a coarse problem of ndofc dofs."""
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        rc = self.L.orc_set_constraints(self.h, isub, C.shape[0], C.ravel() if C.size else
+                                        np.zeros(1), _ia(indrowc) if len(indrowc) else
+                                        np.zeros(1, dtype=np.int32), int(ndofc))
+        if rc != 0:
+            raise RuntimeError(f"orc_set_constraints failed: {rc}")
+
+    def refresh_coarse(self):
+        """Coarse basis, coarse matrices and coarse factor for the current
+        constraint rows."""
+        rc = self.L.orc_refresh_coarse(self.h)
+        if rc != 0:
+            raise RuntimeError(f"orc_refresh_coarse failed: {rc}")
 
     def dotprod_subdomain(self, isub, v1, v2):
         out = C.c_double()

@@ -37,6 +37,7 @@ EXTRA_SYMBOLS = [
     "bddcml_amd_sub_coarse", "bddcml_amd_last_error", "bddcml_amd_selftest_tile",
     "bddcml_amd_selftest_copy", "bddcml_amd_element_gradient", "bddcml_amd_recycling_basis",
     "bddcml_amd_level2_counts", "bddcml_amd_level_counts",
+    "bddcml_amd_adaptive_pairs",
 ]
 
 
@@ -307,6 +308,20 @@ class Bddcml:
             return None
         return dict(nsub=int(out[0]), ndof=int(out[1]), ncorner=int(out[2]), nedge=int(out[3]),
                     nface=int(out[4]), ndofc=int(out[5]), nig=int(out[6]))
+
+    def adaptive_pairs(self):
+        """Face pairs of an adaptive setup: list of dicts (face coarse node,
+        isub, jsub, neig, nadaptive, iterations, eigenvalues largest first)."""
+        n = self.L.bddcml_amd_adaptive_pairs(-1, None, None)
+        out = []
+        for k in range(max(n, 0)):
+            o = np.zeros(6, dtype=np.int32)
+            e = np.zeros(32)
+            self.L.bddcml_amd_adaptive_pairs(k, o.ctypes.data_as(C.POINTER(C.c_int)),
+                                             e.ctypes.data_as(C.POINTER(C.c_double)))
+            out.append(dict(face=int(o[0]), isub=int(o[1]), jsub=int(o[2]), neig=int(o[3]),
+                            nadaptive=int(o[4]), iterations=int(o[5]), eigenvalues=e[:o[3]].copy()))
+        return out
 
     def interface_dofs(self):
         out = np.zeros(self.counts()["nig"], dtype=np.int32)

@@ -1,0 +1,1002 @@
+// adapt.hip -- adaptive BDDC constraints: the face eigenproblems of every pair
+// of subdomains sharing a face, solved together by a batched LOBPCG on the
+// device, and their eigenvectors loaded as constraints on the face.
+//
+// Reference: src/module_adaptivity.f90:296-3024 (pairs, operators, LOBPCG
+// call and constraint creation), src/lobpcg_driver.c:169-304 (BLOPEX LOBPCG
+// with an absolute tolerance), src/module_dd.f90:4386-4457 (loading on the
+// face glob) and 4637-4802 (orthonormalizing each face block),
+// src/module_dd.f90:10785-11009 (pairs = faces).
+//
+// For a pair (i, j) the vectors live on both interfaces, side i then side j,
+// each in its device interface order and padded to its tile multiple (the
+// padding stays zero).  With S = diag(S_i, S_j) (the packed lower tiles of
+// the explicit Schur complements),
+//   A = P (I - E)^T S (I - E) P,   B = P S P,
+// P = I - sum_c q_c q_c^T over the shared constraint rows [C_i; -C_j]
+// normalized (their supports are disjoint for corner and arithmetic rows,
+// so this is the QR projection of the reference), (I - E) x = the weighted
+// jump on the dofs common to i and j (w_j (x_a - x_b) on the i copy a,
+// w_i (x_b - x_a) on the j copy b, 0 elsewhere), (I - E^T) its transpose.
+// The reference runs LOBPCG for the smallest eigenvalues of (-A, B); here
+// the largest of (A, B) are sought directly (the same eigenpairs).
+//
+// Each LOBPCG step runs batched over all pairs on the device: the block
+// operator applies (projection, jump, one symmetric tile product with S for
+// the A and B inputs together), the residuals and their norms, the Gram
+// matrices of [X W P] with A and B, and the block updates.  The Rayleigh-Ritz
+// step of each pair (a generalized eigenproblem of order at most 3k = 30)
+// runs on the host between them, by two Jacobi eigendecompositions: the
+// directions where the B-Gram matrix is numerically singular are dropped, so
+// converged columns and the floating null space of B need no special case.
+// The residuals are preconditioned, as in the reference
+// (lobpcg_preconditioner = 1, src/module_adaptivity.f90:2760-2905), by a
+// BDDC of the pair itself: T = P (H + Phi K^+ Phi^T) P with, per side,
+// H_s = S'_s^{-1} - Z_s G_s^{-1} Z_s^T (the solve with the subdomain's
+// constraints held at zero: S'^{-1} applied from its explicit tiles, formed
+// by the first setup) and the pair coarse problem K = sum_s R_s^T (Phi_s^T
+// S_s Phi_s) R_s over the union of the two subdomains' coarse dofs,
+// pseudo-inverted on the host (K is singular when the pair floats).  On the
+// space where the shared constraints agree this is the inverse of S, so
+// LOBPCG converges in a few iterations where a diagonal preconditioner
+// stalls on coefficient jumps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "solver.hpp"
+
+namespace bddc {
+
+#define AD_CHECK(x)                                                                       \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess)                                                                 \
+      fatal("ADAPTIVITY_SOLVE_EIGENVECTORS", std::string(#x) + " failed: " +             \
+                                                 hipGetErrorString(e_));                  \
+  } while (0)
+
+struct PairDesc {
+  int ntii, ntij;    // tile counts of S_i, S_j
+  int sloi, sloj;    // first tile of S_i, S_j in the subdomain pool
+  int NIi, n;        // padded interface size of i; n = NIi + NIj
+  int k;             // columns of the block (eigenvectors sought)
+  int active;        // still iterating
+  double tol;        // residual norm below which a column has converged
+  int64_t off, woff; // block offsets: n x kmax and n x 2 kmax per pair
+  int q0, q1;        // projection columns (qptr range)
+  int m0, m1;        // common dofs (range in the common-dof arrays)
+  int pbi, pbj;      // first tile of S'^{-1} of i, j (factor pool)
+  int nci, ncj, ncp; // constraints of i, j; coarse dofs of the pair
+  int cm0;           // pair coarse dof of each constraint (cmap range, i then j)
+  int64_t zoi, zoj;  // Z / Phi offsets of i, j (NI x nc column major)
+  int64_t kc0;       // K^+ of the pair (ncp x ncp)
+};
+
+// ------------------------------------------------------------------ kernels
+__device__ __forceinline__ double *pcol(double *b, const PairDesc &p, bool wide, int c) {
+  return b + (wide ? p.woff : p.off) + (int64_t)c * p.n;
+}
+
+// x <- x - sum_c q_c (q_c . x) on columns [c0, c0 + ncol) of every active
+// pair; one thread per projection column (disjoint supports)
+__global__ void k_ad_project(const PairDesc *__restrict__ pairs, const int *__restrict__ qptr,
+                             const int *__restrict__ qpos, const double *__restrict__ qval,
+                             double *buf, int wide, int c0) {
+  const PairDesc p = pairs[blockIdx.x];
+  const int col = blockIdx.y;
+  if (!p.active || col >= p.k) return;
+  double *x = pcol(buf, p, wide, c0 + col);
+  for (int c = p.q0 + threadIdx.x; c < p.q1; c += blockDim.x) {
+    double t = 0.0;
+    for (int e = qptr[c]; e < qptr[c + 1]; e++) t += qval[e] * x[qpos[e]];
+    for (int e = qptr[c]; e < qptr[c + 1]; e++) x[qpos[e]] -= t * qval[e];
+  }
+}
+
+// columns [d0, d0 + k) of dst = 0, then on the common dofs the jump of
+// columns [s0, s0 + k) of src: (I - E) (transpose 0) or (I - E^T) (1)
+__global__ void k_ad_jump(const PairDesc *__restrict__ pairs, const int *__restrict__ ma,
+                          const int *__restrict__ mb, const double *__restrict__ mwa,
+                          const double *__restrict__ mwb, double *src, int src_wide, int s0,
+                          double *dst, int dst_wide, int d0, int transpose) {
+  const PairDesc p = pairs[blockIdx.x];
+  const int col = blockIdx.y;
+  if (!p.active || col >= p.k) return;
+  const double *x = pcol(src, p, src_wide, s0 + col);
+  double *y = pcol(dst, p, dst_wide, d0 + col);
+  for (int i = threadIdx.x; i < p.n; i += blockDim.x) y[i] = 0.0;
+  __syncthreads();
+  for (int e = p.m0 + threadIdx.x; e < p.m1; e += blockDim.x) {
+    const int a = ma[e], b = mb[e];
+    const double wa = mwa[e], wb = mwb[e], xa = x[a], xb = x[b];
+    if (transpose) {
+      y[a] = wb * xa - wa * xb;
+      y[b] = wa * xb - wb * xa;
+    } else {
+      y[a] = wb * (xa - xb);
+      y[b] = wa * (xb - xa);
+    }
+  }
+}
+
+// columns [d0, d0 + k) of dst = columns [s0, s0 + k) of src
+__global__ void k_ad_copy(const PairDesc *__restrict__ pairs, double *src, int src_wide, int s0,
+                          double *dst, int dst_wide, int d0) {
+  const PairDesc p = pairs[blockIdx.x];
+  const int col = blockIdx.y;
+  if (!p.active || col >= p.k) return;
+  const double *x = pcol(src, p, src_wide, s0 + col);
+  double *y = pcol(dst, p, dst_wide, d0 + col);
+  for (int i = threadIdx.x; i < p.n; i += blockDim.x) y[i] = x[i];
+}
+
+// out = M in on the wide columns, M = S (inv 0, the S pool) or S'^{-1}
+// (inv 1, the factor pool, FP64 or FP32), both packed lower tiles by
+// columns; nsets 2: the k A inputs and the k B inputs, 1: the first k
+// columns only.  One workgroup per (pair, side, tile row I): out_I = sum_J
+// M_IJ in_J, M_IJ = stored lower tile (J <= I) or the transpose of tile
+// (J, I), staged in LDS
+constexpr int kAdMaxCols = 32;
+template <typename TP>
+__global__ __launch_bounds__(256) void k_ad_schur(const PairDesc *__restrict__ pairs,
+                                                  const int *__restrict__ jobs,
+                                                  const TP *__restrict__ pool, int inv,
+                                                  const double *__restrict__ in,
+                                                  double *__restrict__ out, int kmax, int nsets) {
+  __shared__ double tile[64][65];
+  __shared__ double xs[64][kAdMaxCols + 1];
+  const int job = blockIdx.x;
+  const PairDesc p = pairs[jobs[3 * job]];
+  if (!p.active) return;
+  const int side = jobs[3 * job + 1], I = jobs[3 * job + 2];
+  const int nt = side ? p.ntij : p.ntii;
+  const int slow = inv ? (side ? p.pbj : p.pbi) : (side ? p.sloj : p.sloi);
+  const int so = side ? p.NIi : 0;
+  const int ncol = nsets * p.k;
+  const int r = threadIdx.x & 63, cg = threadIdx.x >> 6;
+  double acc[kAdMaxCols / 4];
+#pragma unroll
+  for (int q = 0; q < kAdMaxCols / 4; q++) acc[q] = 0.0;
+  const double *xin = in + p.woff + so;
+  for (int J = 0; J < nt; J++) {
+    const bool low = J <= I;
+    const int I0 = low ? I : J, J0 = low ? J : I;
+    const TP *Tp = pool + (size_t)(slow + J0 * nt - J0 * (J0 - 1) / 2 + (I0 - J0)) * TILE_ELEMS;
+    for (int e = threadIdx.x; e < TILE_ELEMS; e += 256) {
+      const int er = e & 63, ec = e >> 6;  // element (er, ec) of the stored tile
+      if (low) tile[er][ec] = (double)Tp[e];
+      else tile[ec][er] = (double)Tp[e];
+    }
+    for (int e = threadIdx.x; e < 64 * ncol; e += 256) {
+      const int c = e & 63, col = e >> 6;
+      const int wc = col < p.k ? col : kmax + col - p.k;  // A inputs, then B inputs
+      xs[c][col] = xin[(int64_t)wc * p.n + J * 64 + c];
+    }
+    __syncthreads();
+    for (int c = 0; c < 64; c++) {
+      const double s = tile[r][c];
+#pragma unroll
+      for (int q = 0; q < kAdMaxCols / 4; q++) {
+        const int col = cg + 4 * q;
+        if (col < ncol) acc[q] += s * xs[c][col];
+      }
+    }
+    __syncthreads();
+  }
+  double *yo = out + p.woff + so;
+#pragma unroll
+  for (int q = 0; q < kAdMaxCols / 4; q++) {
+    const int col = cg + 4 * q;
+    if (col < ncol) yo[(int64_t)(col < p.k ? col : kmax + col - p.k) * p.n + I * 64 + r] = acc[q];
+  }
+}
+
+// R = AX - BX diag(lam) into W and its squared column norms; a column
+// whose residual is below the pair's tolerance contributes no direction
+// (W = 0)
+__global__ __launch_bounds__(256) void k_ad_resid(const PairDesc *__restrict__ pairs,
+                                                  const double *__restrict__ lam, int kmax,
+                                                  double *AX, double *BX, double *W,
+                                                  double *__restrict__ nrm2) {
+  __shared__ double red[256];
+  const PairDesc p = pairs[blockIdx.x];
+  const int col = blockIdx.y;
+  if (!p.active || col >= p.k) return;
+  const double l = lam[blockIdx.x * kmax + col];
+  const double *ax = pcol(AX, p, false, col), *bx = pcol(BX, p, false, col);
+  double *w = pcol(W, p, false, col);
+  double s = 0.0;
+  for (int i = threadIdx.x; i < p.n; i += 256) {
+    const double v = ax[i] - l * bx[i];
+    s += v * v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double nr2 = red[0];
+  const bool keep = nr2 > p.tol * p.tol;
+  for (int i = threadIdx.x; i < p.n; i += 256) w[i] = keep ? ax[i] - l * bx[i] : 0.0;
+  if (threadIdx.x == 0) nrm2[blockIdx.x * kmax + col] = nr2;
+}
+
+// Pair coarse correction of the preconditioner.  With r the first k wide
+// columns of in and y = S'^{-1} r already in out, per side s = i, j:
+//   y_s += Phi_s (R_s u - Z_s^T r_s),   u = K^+ sum_s R_s^T Phi_s^T r_s,
+// since Z_s G_s^{-1} Z_s^T r_s = Phi_s Z_s^T r_s.  R_s maps the constraints
+// of s to the pair's coarse dofs (cmap); K^+ is symmetric.  LDS: the coarse
+// vectors g, u (ncp each) and Phi^T r, Z^T r (nci + ncj each).
+__global__ __launch_bounds__(256) void k_ad_coarse_pc(const PairDesc *__restrict__ pairs,
+                                                      const double *__restrict__ Z,
+                                                      const double *__restrict__ Phi,
+                                                      const int *__restrict__ cmap,
+                                                      const double *__restrict__ Kp,
+                                                      const double *__restrict__ in,
+                                                      double *__restrict__ out) {
+  extern __shared__ double sh[];
+  const PairDesc p = pairs[blockIdx.x];
+  const int col = blockIdx.y;
+  if (!p.active || col >= p.k) return;
+  const int nc2 = p.nci + p.ncj, NIj = p.n - p.NIi;
+  double *g = sh, *u = g + p.ncp, *gp = u + p.ncp, *zr = gp + nc2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const double *x = in + p.woff + (int64_t)col * p.n;
+  // Phi^T r and Z^T r, one wave per constraint
+  for (int e = wv; e < nc2; e += 4) {
+    const int side = e >= p.nci, c = side ? e - p.nci : e;
+    const int NI = side ? NIj : p.NIi;
+    const int64_t zo = (side ? p.zoj : p.zoi) + (int64_t)c * NI;
+    const double *xs = x + (side ? p.NIi : 0);
+    double a = 0.0, b = 0.0;
+    for (int r = lane; r < NI; r += 64) {
+      a += Phi[zo + r] * xs[r];
+      b += Z[zo + r] * xs[r];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      a += __shfl_down(a, o);
+      b += __shfl_down(b, o);
+    }
+    if (lane == 0) gp[e] = a, zr[e] = b;
+  }
+  for (int q = threadIdx.x; q < p.ncp; q += 256) g[q] = 0.0;
+  __syncthreads();
+  // assemble side by side: within a side the coarse dofs are distinct
+  const int *cm = cmap + p.cm0;
+  for (int c = threadIdx.x; c < p.nci; c += 256) g[cm[c]] += gp[c];
+  __syncthreads();
+  for (int c = threadIdx.x; c < p.ncj; c += 256) g[cm[p.nci + c]] += gp[p.nci + c];
+  __syncthreads();
+  const double *K = Kp + p.kc0;
+  for (int q = threadIdx.x; q < p.ncp; q += 256) {
+    double s = 0.0;
+    for (int t = 0; t < p.ncp; t++) s += K[(int64_t)t * p.ncp + q] * g[t];
+    u[q] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nc2; e += 256) gp[e] = u[cm[e]] - zr[e];
+  __syncthreads();
+  double *y = out + p.woff + (int64_t)col * p.n;
+  for (int i = threadIdx.x; i < p.n; i += 256) {
+    const int side = i >= p.NIi, r = side ? i - p.NIi : i;
+    const int NI = side ? NIj : p.NIi, nc = side ? p.ncj : p.nci;
+    const double *ph = Phi + (side ? p.zoj : p.zoi) + r;
+    const double *w = gp + (side ? p.nci : 0);
+    double s = 0.0;
+    for (int c = 0; c < nc; c++) s += ph[(int64_t)c * NI] * w[c];
+    y[i] += s;
+  }
+}
+
+struct Blocks {
+  double *v[3], *av[3], *bv[3];  // [X W P], A times them, B times them
+};
+
+// Gram matrices GA = V^T (A V), GB = V^T (B V) of the nb blocks of V, each
+// of k columns (row major, order nb k, stride 3 kmax); one workgroup per
+// pair, one wave per entry at a time
+__global__ __launch_bounds__(256) void k_ad_gram(const PairDesc *__restrict__ pairs, Blocks b,
+                                                 int nb, int kmax, double *__restrict__ GA,
+                                                 double *__restrict__ GB) {
+  const PairDesc p = pairs[blockIdx.x];
+  if (!p.active) return;
+  const int m = nb * p.k, ld = 3 * kmax;
+  double *ga = GA + (int64_t)blockIdx.x * ld * ld, *gb = GB + (int64_t)blockIdx.x * ld * ld;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int e = wave; e < m * m; e += 4) {
+    const int a = e / m, c = e % m;
+    if (c < a) continue;
+    const double *va = pcol(b.v[a / p.k], p, false, a % p.k);
+    const double *ac = pcol(b.av[c / p.k], p, false, c % p.k);
+    const double *bc = pcol(b.bv[c / p.k], p, false, c % p.k);
+    double sa = 0.0, sb = 0.0;
+    for (int i = lane; i < p.n; i += 64) {
+      sa += va[i] * ac[i];
+      sb += va[i] * bc[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      sa += __shfl_xor(sa, o);
+      sb += __shfl_xor(sb, o);
+    }
+    if (lane == 0) {
+      ga[a * ld + c] = ga[c * ld + a] = sa;
+      gb[a * ld + c] = gb[c * ld + a] = sb;
+    }
+  }
+}
+
+// X' = V C, P' = V C without the X block (the W and P parts), and the same
+// combinations of A V and B V; C is (nb k) x k per pair, row major with
+// stride kmax; one thread per row
+__global__ __launch_bounds__(256) void k_ad_combine(const PairDesc *__restrict__ pairs, Blocks b,
+                                                    int nb, int kmax, const double *__restrict__ C,
+                                                    Blocks o) {
+  const PairDesc p = pairs[blockIdx.y];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= p.n) return;
+  if (!p.active) {  // converged: the blocks carry over unchanged
+    for (int which = 0; which < 3; which++) {
+      double *const *src = which == 0 ? b.v : (which == 1 ? b.av : b.bv);
+      double *const *dst = which == 0 ? o.v : (which == 1 ? o.av : o.bv);
+      for (int c = 0; c < p.k; c++) {
+        pcol(dst[0], p, false, c)[i] = pcol(src[0], p, false, c)[i];
+        pcol(dst[2], p, false, c)[i] = pcol(src[2], p, false, c)[i];
+      }
+    }
+    return;
+  }
+  const double *cf = C + (int64_t)blockIdx.y * 3 * kmax * kmax;
+  for (int which = 0; which < 3; which++) {
+    double *const *src = which == 0 ? b.v : (which == 1 ? b.av : b.bv);
+    double *const *dst = which == 0 ? o.v : (which == 1 ? o.av : o.bv);
+    for (int c = 0; c < p.k; c++) {
+      double sx = 0.0, sp = 0.0;
+      for (int blk = 0; blk < nb; blk++)
+        for (int j = 0; j < p.k; j++) {
+          const double v = pcol(src[blk], p, false, j)[i] * cf[(blk * p.k + j) * kmax + c];
+          sx += v;
+          if (blk > 0) sp += v;
+        }
+      pcol(dst[0], p, false, c)[i] = sx;  // new X (or AX, BX)
+      pcol(dst[2], p, false, c)[i] = sp;  // new P (or AP, BP)
+    }
+  }
+}
+
+// ------------------------------------------------------- host eigen-solves
+// Cyclic Jacobi eigendecomposition of a symmetric n x n matrix (row major,
+// overwritten): eigenvalues in w, eigenvectors in the columns of v
+static void jacobi_eig(int n, std::vector<double> &a, std::vector<double> &w, std::vector<double> &v) {
+  v.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) v[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        const double x = a[(size_t)i * n + j] * a[(size_t)i * n + j];
+        tot += x;
+        if (i != j) off += x;
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int pq = 0; pq < n; pq++)
+      for (int q = pq + 1; q < n; q++) {
+        const int pp = pq;
+        const double apq = a[(size_t)pp * n + q];
+        if (apq == 0.0) continue;
+        const double app = a[(size_t)pp * n + pp], aqq = a[(size_t)q * n + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; k++) {
+          const double akp = a[(size_t)k * n + pp], akq = a[(size_t)k * n + q];
+          a[(size_t)k * n + pp] = c * akp - s * akq;
+          a[(size_t)k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          const double apk = a[(size_t)pp * n + k], aqk = a[(size_t)q * n + k];
+          a[(size_t)pp * n + k] = c * apk - s * aqk;
+          a[(size_t)q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          const double vkp = v[(size_t)k * n + pp], vkq = v[(size_t)k * n + q];
+          v[(size_t)k * n + pp] = c * vkp - s * vkq;
+          v[(size_t)k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  w.resize(n);
+  for (int i = 0; i < n; i++) w[i] = a[(size_t)i * n + i];
+}
+
+// Rayleigh-Ritz of one pair: the k largest eigenpairs of (GA, GB) (order m,
+// leading dimension ld) on the range where GB is numerically positive;
+// coefficients into C (m x k, stride kmax) and values into lam.  Returns the
+// number of pairs found (fewer than k only if the block lost rank).
+static int rayleigh_ritz(int m, int k, int ld, const double *ga, const double *gb, int kmax,
+                         double *C, double *lam) {
+  // the Gram matrices scaled to a unit B-diagonal first, so that the drop
+  // threshold measures linear dependence, not column size
+  std::vector<double> B((size_t)m * m), A((size_t)m * m), bw, bv, w, y, dsc(m);
+  for (int i = 0; i < m; i++) {
+    const double g = gb[i * ld + i];
+    dsc[i] = g > 0.0 ? 1.0 / std::sqrt(g) : 0.0;
+  }
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < m; j++) {
+      B[(size_t)i * m + j] = 0.5 * (gb[i * ld + j] + gb[j * ld + i]) * dsc[i] * dsc[j];
+      A[(size_t)i * m + j] = 0.5 * (ga[i * ld + j] + ga[j * ld + i]) * dsc[i] * dsc[j];
+    }
+  jacobi_eig(m, B, bw, bv);
+  double bmax = 0.0;
+  for (double x : bw) bmax = std::max(bmax, x);
+  std::vector<int> keep;
+  for (int i = 0; i < m; i++)
+    if (bw[i] > 1e-8 * bmax) keep.push_back(i);
+  const int r = (int)keep.size();
+  // T = D U_keep diag(bw)^{-1/2};  M = T^T A T (on the unscaled Gram matrices)
+  std::vector<double> T((size_t)m * r), M((size_t)r * r, 0.0), AT((size_t)m * r, 0.0);
+  for (int i = 0; i < m; i++)
+    for (int c = 0; c < r; c++) T[(size_t)i * r + c] = bv[(size_t)i * m + keep[c]] / std::sqrt(bw[keep[c]]);
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < m; j++) {
+      A[(size_t)i * m + j] = 0.5 * (ga[i * ld + j] + ga[j * ld + i]);
+    }
+  for (int i = 0; i < m; i++)
+    for (int c = 0; c < r; c++) T[(size_t)i * r + c] *= dsc[i];
+  for (int i = 0; i < m; i++)
+    for (int c = 0; c < r; c++) {
+      double s = 0.0;
+      for (int j = 0; j < m; j++) s += A[(size_t)i * m + j] * T[(size_t)j * r + c];
+      AT[(size_t)i * r + c] = s;
+    }
+  for (int a = 0; a < r; a++)
+    for (int c = 0; c < r; c++) {
+      double s = 0.0;
+      for (int i = 0; i < m; i++) s += T[(size_t)i * r + a] * AT[(size_t)i * r + c];
+      M[(size_t)a * r + c] = s;
+    }
+  jacobi_eig(r, M, w, y);
+  std::vector<int> ord(r);
+  for (int i = 0; i < r; i++) ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return w[a] > w[b]; });
+  const int kk = std::min(k, r);
+  for (int c = 0; c < k; c++) {
+    lam[c] = c < kk ? w[ord[c]] : 0.0;
+    for (int i = 0; i < m; i++) {
+      double s = 0.0;
+      if (c < kk)
+        for (int a = 0; a < r; a++) s += T[(size_t)i * r + a] * y[(size_t)a * r + ord[c]];
+      C[(size_t)i * kmax + c] = s;
+    }
+  }
+  return kk;
+}
+
+// Householder QR with column pivoting of M^T (rows of M as columns), the
+// columns kept while |R_kk| >= 0.01 |R_11|, and Q's first columns as
+// orthonormal rows spanning those of M (LAPACK's DGEQP3 + DORGQR, sign
+// conventions included; src/module_dd.f90:4637-4802).  M: r x n row major.
+static std::vector<std::vector<double>> orthonormal_rows(const std::vector<std::vector<double>> &M, int n) {
+  const int r = (int)M.size();
+  std::vector<std::vector<double>> a(r, std::vector<double>(n));  // columns of M^T
+  for (int c = 0; c < r; c++) a[c] = M[c];
+  std::vector<std::vector<double>> vs;  // Householder vectors
+  std::vector<double> taus, rdiag;
+  std::vector<double> norms(r);
+  for (int c = 0; c < r; c++) {
+    double s = 0.0;
+    for (double x : a[c]) s += x * x;
+    norms[c] = s;
+  }
+  const int kmax = std::min(r, n);
+  for (int k = 0; k < kmax; k++) {
+    // pivot: the remaining column of largest norm (first on ties)
+    int piv = k;
+    double best = -1.0;
+    for (int c = k; c < r; c++) {
+      double s = 0.0;
+      for (int i = k; i < n; i++) s += a[c][i] * a[c][i];
+      norms[c] = s;
+      if (s > best) best = s, piv = c;
+    }
+    std::swap(a[k], a[piv]);
+    // reflector for a[k][k..n)
+    double alpha = a[k][k], xn = 0.0;
+    for (int i = k + 1; i < n; i++) xn += a[k][i] * a[k][i];
+    std::vector<double> v(n, 0.0);
+    double tau = 0.0, beta = alpha;
+    if (xn > 0.0) {
+      beta = -std::copysign(std::sqrt(alpha * alpha + xn), alpha);
+      tau = (beta - alpha) / beta;
+      v[k] = 1.0;
+      for (int i = k + 1; i < n; i++) v[i] = a[k][i] / (alpha - beta);
+    }
+    rdiag.push_back(beta);
+    // apply H = I - tau v v^T to the remaining columns
+    for (int c = k + 1; c < r; c++) {
+      double s = 0.0;
+      for (int i = k; i < n; i++) s += v[i] * a[c][i];
+      for (int i = k; i < n; i++) a[c][i] -= tau * v[i] * s;
+    }
+    vs.push_back(v);
+    taus.push_back(tau);
+  }
+  int nvalid = 0;
+  if (!rdiag.empty() && std::fabs(rdiag[0]) > 1e-12) {
+    for (size_t k = 0; k < rdiag.size(); k++) {
+      if (std::fabs(rdiag[k]) < 0.01 * std::fabs(rdiag[0])) break;
+      nvalid = (int)k + 1;
+    }
+  }
+  // Q e_c = H_0 H_1 ... H_{nvalid-1} e_c
+  std::vector<std::vector<double>> q(nvalid, std::vector<double>(n, 0.0));
+  for (int c = 0; c < nvalid; c++) {
+    std::vector<double> &x = q[c];
+    x[c] = 1.0;
+    for (int k = nvalid - 1; k >= 0; k--) {
+      double s = 0.0;
+      for (int i = k; i < n; i++) s += vs[k][i] * x[i];
+      for (int i = k; i < n; i++) x[i] -= taus[k] * vs[k][i] * s;
+    }
+  }
+  return q;
+}
+
+static double unit_random(uint64_t seed, uint64_t idx) {
+  uint64_t z = (seed ^ idx) + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+template <class T> struct ABuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ABuf() = default;
+  ABuf(const ABuf &) = delete;
+  ABuf &operator=(const ABuf &) = delete;
+  ~ABuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t cnt) {
+    if (p) (void)hipFree(p);
+    n = cnt;
+    AD_CHECK(hipMalloc(&p, std::max<size_t>(cnt, 1) * sizeof(T)));
+  }
+  void swap(ABuf &o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
+  void upload(const std::vector<T> &v, hipStream_t st) {
+    alloc(v.size());
+    if (!v.empty()) AD_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  }
+};
+
+// ------------------------------------------------------------------ driver
+hipStream_t device_stream(Solver &S);  // compute stream (device.hip)
+
+void adaptive_constraints(Solver &S) {
+  const char *R = "ADAPTIVITY_SOLVE_EIGENVECTORS";
+  const int nsub = S.nsub;
+  hipStream_t st = device_stream(S);
+  S.adapt_pairs.clear();
+  S.adapt_eigs.clear();
+  std::vector<int> slow, ntis;
+  const double *pool = device_schur_tiles(S, slow, ntis);
+  PairPcSource pcs;
+  device_pair_pc_source(S, pcs);
+  const int first_face = S.ncorner + S.nedge;
+  // coarse node of every constraint row, and the subdomains of every face
+  auto node_of = [&](int cdof) {
+    return int(std::upper_bound(S.kdofc.begin(), S.kdofc.end(), cdof) - S.kdofc.begin()) - 1;
+  };
+  std::map<int, std::vector<int>> face_subs;
+  for (int is = 0; is < nsub; is++)
+    for (int id : S.subs[is].cnodes)
+      if (id >= first_face) face_subs[id].push_back(is);
+  // per subdomain: device position of each reference interface index, and
+  // the weight coefficients
+  std::vector<std::vector<int>> ref2dev(nsub);
+  std::vector<std::vector<double>> rho(nsub);
+  for (int is = 0; is < nsub; is++) {
+    const Sub &s = S.subs[is];
+    ref2dev[is].assign(s.ni, 0);
+    for (int k = 0; k < s.ni; k++) ref2dev[is][s.dev_ifc[k]] = k;
+    if (!weight_coefficients(S, is, rho[is])) fatal(R, "dof_data missing for weights type 3");
+  }
+  // ---- pairs: projection columns, common dofs, block widths
+  struct HostPair {
+    int face, i, j, k, neig;
+    double tol;
+  };
+  std::vector<HostPair> hp;
+  std::vector<PairDesc> pd;
+  std::vector<int> qptr{0}, qpos, ma, mb, cmap;
+  std::vector<double> qval, mwa, mwb;
+  int64_t off = 0, woff = 0;
+  int kmax = 0;
+  for (auto &f : face_subs) {
+    if (f.second.size() != 2) continue;
+    const int i = f.second[0], j = f.second[1];
+    const Sub &si = S.subs[i], &sj = S.subs[j];
+    PairDesc d{};
+    d.ntii = ntis[i], d.ntij = ntis[j], d.sloi = slow[i], d.sloj = slow[j];
+    d.NIi = si.NI;
+    d.n = si.NI + sj.NI;
+    // common dofs by global dof
+    std::map<int, int> gj;
+    for (int r = 0; r < sj.ni; r++) gj[sj.isvgvn[sj.ifc_dof[r]]] = r;
+    d.m0 = (int)ma.size();
+    for (int r = 0; r < si.ni; r++) {
+      auto it = gj.find(si.isvgvn[si.ifc_dof[r]]);
+      if (it == gj.end()) continue;
+      const double den = rho[i][r] + rho[j][it->second];
+      ma.push_back(ref2dev[i][r]);
+      mb.push_back(si.NI + ref2dev[j][it->second]);
+      mwa.push_back(rho[i][r] / den);
+      mwb.push_back(rho[j][it->second] / den);
+    }
+    d.m1 = (int)ma.size();
+    const int ndofcomm = d.m1 - d.m0;
+    // shared constraint rows [C_i; -C_j], normalized (disjoint supports)
+    d.q0 = (int)qptr.size() - 1;
+    int ncommon = 0;
+    for (int ci = 0; ci < si.nconstr; ci++)
+      for (int cj = 0; cj < sj.nconstr; cj++) {
+        if (si.indrowc[ci] != sj.indrowc[cj]) continue;
+        ncommon++;
+        double nrm = 0.0;
+        for (int q = si.cptr[ci]; q < si.cptr[ci + 1]; q++) nrm += si.cval[q] * si.cval[q];
+        for (int q = sj.cptr[cj]; q < sj.cptr[cj + 1]; q++) nrm += sj.cval[q] * sj.cval[q];
+        nrm = std::sqrt(nrm);
+        for (int q = si.cptr[ci]; q < si.cptr[ci + 1]; q++)
+          qpos.push_back(ref2dev[i][si.cidx[q]]), qval.push_back(si.cval[q] / nrm);
+        for (int q = sj.cptr[cj]; q < sj.cptr[cj + 1]; q++)
+          qpos.push_back(si.NI + ref2dev[j][sj.cidx[q]]), qval.push_back(-sj.cval[q] / nrm);
+        qptr.push_back((int)qpos.size());
+      }
+    d.q1 = (int)qptr.size() - 1;
+    const int neig = std::max(0, std::min(S.adapt_nvec, ndofcomm - ncommon));
+    if (neig <= 0) {
+      S.adapt_pairs.push_back({f.first, i, j, 0, 0, 0});
+      S.adapt_eigs.push_back({});
+      continue;
+    }
+    d.k = neig;
+    d.active = 1;
+    double trA = 0.0;
+    for (double x : rho[i]) trA += x;
+    for (double x : rho[j]) trA += x;
+    d.tol = S.adapt_tol * trA / (si.ni + sj.ni);
+    // preconditioner data: S'^{-1} tiles, Z / Phi, pair coarse dofs
+    d.pbi = pcs.pbase[i], d.pbj = pcs.pbase[j];
+    d.zoi = pcs.zoff[i], d.zoj = pcs.zoff[j];
+    d.nci = si.nconstr, d.ncj = sj.nconstr;
+    std::vector<int> cd(si.indrowc.begin(), si.indrowc.begin() + si.nconstr);
+    cd.insert(cd.end(), sj.indrowc.begin(), sj.indrowc.begin() + sj.nconstr);
+    std::sort(cd.begin(), cd.end());
+    cd.erase(std::unique(cd.begin(), cd.end()), cd.end());
+    d.ncp = (int)cd.size();
+    d.cm0 = (int)cmap.size();
+    for (const Sub *sb : {&si, &sj})
+      for (int c = 0; c < sb->nconstr; c++)
+        cmap.push_back(int(std::lower_bound(cd.begin(), cd.end(), sb->indrowc[c]) - cd.begin()));
+    hp.push_back({f.first, i, j, neig, neig, d.tol});
+    pd.push_back(d);
+    kmax = std::max(kmax, neig);
+  }
+  const int np = (int)pd.size();
+  int64_t kcsz = 0;
+  size_t pc_lds = 0;
+  for (int p = 0; p < np; p++) {
+    pd[p].off = off, pd[p].woff = woff;
+    off += (int64_t)pd[p].n * kmax;
+    woff += (int64_t)pd[p].n * 2 * kmax;
+    pd[p].kc0 = kcsz;
+    kcsz += (int64_t)pd[p].ncp * pd[p].ncp;
+    pc_lds = std::max(pc_lds, sizeof(double) * (2 * (size_t)pd[p].ncp + 2 * (size_t)(pd[p].nci + pd[p].ncj)));
+  }
+  if (pc_lds > 65536) fatal(R, "too many constraints on a pair for the pair coarse problem");
+  // K^+ of every pair: the pair's coarse matrix sum_s R_s^T Ac_s R_s
+  // (Ac_s = Phi_s^T S_s Phi_s), inverted on its eigenvalues above 1e-12
+  // times the largest (a floating pair has rigid modes in its null space)
+  std::vector<double> kp((size_t)kcsz, 0.0);
+#pragma omp parallel for schedule(dynamic)
+  for (int p = 0; p < np; p++) {
+    const PairDesc &d = pd[p];
+    const int m = d.ncp;
+    std::vector<double> K((size_t)m * m, 0.0), w, v;
+    const int sub2[2] = {hp[p].i, hp[p].j}, nc2[2] = {d.nci, d.ncj};
+    const int *cm = cmap.data() + d.cm0;
+    for (int sd = 0; sd < 2; sd++) {
+      const std::vector<double> &Ac = pcs.Ac[sub2[sd]];
+      const int nc = nc2[sd];
+      for (int a = 0; a < nc; a++)
+        for (int b = 0; b < nc; b++) K[(size_t)cm[a] * m + cm[b]] += Ac[(size_t)a * nc + b];
+      cm += nc;
+    }
+    jacobi_eig(m, K, w, v);
+    double wmax = 0.0;
+    for (double x : w) wmax = std::max(wmax, x);
+    double *out = kp.data() + d.kc0;
+    for (int e = 0; e < m; e++) {
+      if (!(w[e] > 1e-12 * wmax)) continue;
+      const double inv = 1.0 / w[e];
+      for (int a = 0; a < m; a++) {
+        const double va = v[(size_t)a * m + e] * inv;
+        for (int b = 0; b < m; b++) out[(size_t)a * m + b] += va * v[(size_t)b * m + e];
+      }
+    }
+  }
+  if (2 * kmax > kAdMaxCols) fatal(R, "too many eigenvectors per pair");
+  std::vector<int> jobs;
+  for (int p = 0; p < np; p++)
+    for (int side = 0; side < 2; side++)
+      for (int I = 0; I < (side ? pd[p].ntij : pd[p].ntii); I++) jobs.insert(jobs.end(), {p, side, I});
+  // ---- device buffers
+  ABuf<PairDesc> dpairs;
+  ABuf<int> dqptr, dqpos, dma, dmb, djobs, dcmap;
+  ABuf<double> dqval, dmwa, dmwb, dkp;
+  dpairs.upload(pd, st);
+  dqptr.upload(qptr, st), dqpos.upload(qpos, st), dqval.upload(qval, st);
+  dma.upload(ma, st), dmb.upload(mb, st), dmwa.upload(mwa, st), dmwb.upload(mwb, st);
+  djobs.upload(jobs, st);
+  dcmap.upload(cmap, st), dkp.upload(kp, st);
+  enum { X, AX, BX, W, AW, BW, P, AP, BP, XN, AXN, BXN, PN, APN, BPN, NBLK };
+  std::vector<ABuf<double>> blk(NBLK);
+  for (auto &b : blk) b.alloc(std::max<int64_t>(off, 1));
+  ABuf<double> win, wout, dlam, dnrm, dGA, dGB, dC;
+  win.alloc(std::max<int64_t>(woff, 1));
+  wout.alloc(std::max<int64_t>(woff, 1));
+  dlam.alloc((size_t)np * kmax);
+  dnrm.alloc((size_t)np * kmax);
+  const int ld = 3 * kmax;
+  dGA.alloc((size_t)np * ld * ld);
+  dGB.alloc((size_t)np * ld * ld);
+  dC.alloc((size_t)np * ld * kmax);
+  const dim3 pk(std::max(np, 1), std::max(kmax, 1));
+  // block operator: A x and B x for the k columns of block b
+  auto apply = [&](int b, int ab, int bb) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_ad_copy, pk, dim3(256), 0, st, dpairs.p, blk[b].p, 0, 0, win.p, 1, kmax);
+    hipLaunchKernelGGL(k_ad_project, pk, dim3(64), 0, st, dpairs.p, dqptr.p, dqpos.p, dqval.p,
+                       win.p, 1, kmax);
+    hipLaunchKernelGGL(k_ad_jump, pk, dim3(256), 0, st, dpairs.p, dma.p, dmb.p, dmwa.p, dmwb.p,
+                       win.p, 1, kmax, win.p, 1, 0, 0);
+    hipLaunchKernelGGL(k_ad_schur<double>, dim3((unsigned)jobs.size() / 3), dim3(256), 0, st,
+                       dpairs.p, djobs.p, pool, 0, win.p, wout.p, kmax, 2);
+    hipLaunchKernelGGL(k_ad_jump, pk, dim3(256), 0, st, dpairs.p, dma.p, dmb.p, dmwa.p, dmwb.p,
+                       wout.p, 1, 0, blk[ab].p, 0, 0, 1);
+    hipLaunchKernelGGL(k_ad_project, pk, dim3(64), 0, st, dpairs.p, dqptr.p, dqpos.p, dqval.p,
+                       blk[ab].p, 0, 0);
+    hipLaunchKernelGGL(k_ad_copy, pk, dim3(256), 0, st, dpairs.p, wout.p, 1, kmax, blk[bb].p, 0, 0);
+    hipLaunchKernelGGL(k_ad_project, pk, dim3(64), 0, st, dpairs.p, dqptr.p, dqpos.p, dqval.p,
+                       blk[bb].p, 0, 0);
+  };
+  // preconditioner: W = P T W on the k columns of W (W already projected)
+  auto precondition = [&]() {
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_ad_copy, pk, dim3(256), 0, st, dpairs.p, blk[W].p, 0, 0, win.p, 1, 0);
+    const dim3 gj((unsigned)jobs.size() / 3);
+    if (pcs.inv_float)
+      hipLaunchKernelGGL(k_ad_schur<float>, gj, dim3(256), 0, st, dpairs.p, djobs.p,
+                         (const float *)pcs.inv, 1, win.p, wout.p, kmax, 1);
+    else
+      hipLaunchKernelGGL(k_ad_schur<double>, gj, dim3(256), 0, st, dpairs.p, djobs.p,
+                         (const double *)pcs.inv, 1, win.p, wout.p, kmax, 1);
+    hipLaunchKernelGGL(k_ad_coarse_pc, pk, dim3(256), pc_lds, st, dpairs.p, pcs.Z, pcs.Phi, dcmap.p,
+                       dkp.p, win.p, wout.p);
+    hipLaunchKernelGGL(k_ad_copy, pk, dim3(256), 0, st, dpairs.p, wout.p, 1, 0, blk[W].p, 0, 0);
+    hipLaunchKernelGGL(k_ad_project, pk, dim3(64), 0, st, dpairs.p, dqptr.p, dqpos.p, dqval.p,
+                       blk[W].p, 0, 0);
+  };
+  std::vector<double> hGA((size_t)np * ld * ld), hGB((size_t)np * ld * ld), hC((size_t)np * ld * kmax, 0.0),
+      hlam((size_t)np * kmax, 0.0), hnrm((size_t)np * kmax);
+  std::vector<int> iters(np, 0);
+  // Rayleigh-Ritz of every active pair on nb blocks, then the update
+  auto rr_update = [&](int nb) {
+    Blocks in{{blk[X].p, blk[W].p, blk[P].p}, {blk[AX].p, blk[AW].p, blk[AP].p}, {blk[BX].p, blk[BW].p, blk[BP].p}};
+    hipLaunchKernelGGL(k_ad_gram, dim3(std::max(np, 1)), dim3(256), 0, st, dpairs.p, in, nb, kmax,
+                       dGA.p, dGB.p);
+    AD_CHECK(hipMemcpyAsync(hGA.data(), dGA.p, hGA.size() * 8, hipMemcpyDeviceToHost, st));
+    AD_CHECK(hipMemcpyAsync(hGB.data(), dGB.p, hGB.size() * 8, hipMemcpyDeviceToHost, st));
+    AD_CHECK(hipStreamSynchronize(st));
+#pragma omp parallel for schedule(dynamic)
+    for (int p = 0; p < np; p++) {
+      if (!pd[p].active) continue;
+      const int m = nb * pd[p].k;
+      std::fill(hC.begin() + (size_t)p * ld * kmax, hC.begin() + (size_t)(p + 1) * ld * kmax, 0.0);
+      rayleigh_ritz(m, pd[p].k, ld, hGA.data() + (size_t)p * ld * ld, hGB.data() + (size_t)p * ld * ld,
+                    kmax, hC.data() + (size_t)p * ld * kmax, hlam.data() + (size_t)p * kmax);
+    }
+    AD_CHECK(hipMemcpyAsync(dC.p, hC.data(), hC.size() * 8, hipMemcpyHostToDevice, st));
+    AD_CHECK(hipMemcpyAsync(dlam.p, hlam.data(), hlam.size() * 8, hipMemcpyHostToDevice, st));
+    Blocks out{{blk[XN].p, nullptr, blk[PN].p}, {blk[AXN].p, nullptr, blk[APN].p}, {blk[BXN].p, nullptr, blk[BPN].p}};
+    int maxn = 1;
+    for (const PairDesc &d : pd) maxn = std::max(maxn, d.n);
+    hipLaunchKernelGGL(k_ad_combine, dim3((maxn + 255) / 256, std::max(np, 1)), dim3(256), 0, st, dpairs.p,
+                       in, nb, kmax, dC.p, out);
+    blk[X].swap(blk[XN]), blk[AX].swap(blk[AXN]), blk[BX].swap(blk[BXN]);
+    blk[P].swap(blk[PN]), blk[AP].swap(blk[APN]), blk[BP].swap(blk[BPN]);
+  };
+  if (np > 0) {
+    // random start in range(P), padding zero
+    std::vector<double> x0((size_t)off, 0.0);
+    for (int p = 0; p < np; p++) {
+      const PairDesc &d = pd[p];
+      const Sub &si = S.subs[hp[p].i], &sj = S.subs[hp[p].j];
+      for (int c = 0; c < d.k; c++)
+        for (int r = 0; r < d.n; r++) {
+          const bool real = r < d.NIi ? r < si.ni : r - d.NIi < sj.ni;
+          if (real)
+            x0[d.off + (int64_t)c * d.n + r] =
+                2.0 * unit_random(0x5eedull + (uint64_t)hp[p].face, (uint64_t)c * 1000003ull + r) - 1.0;
+        }
+    }
+    AD_CHECK(hipMemcpyAsync(blk[X].p, x0.data(), x0.size() * 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_ad_project, pk, dim3(64), 0, st, dpairs.p, dqptr.p, dqpos.p, dqval.p,
+                       blk[X].p, 0, 0);
+    apply(X, AX, BX);
+    rr_update(1);
+    int nb = 2;
+    for (int it = 1; it <= S.adapt_maxit; it++) {
+      // the A and B images of X and P are carried by the block updates;
+      // every ten iterations they are recomputed, so that rounding carried
+      // through the recurrences does not leave the Gram matrices
+      // inconsistent with the blocks
+      if (it % 10 == 0) {
+        apply(X, AX, BX);
+        apply(P, AP, BP);
+      }
+      hipLaunchKernelGGL(k_ad_resid, pk, dim3(256), 0, st, dpairs.p, dlam.p, kmax, blk[AX].p,
+                         blk[BX].p, blk[W].p, dnrm.p);
+      AD_CHECK(hipMemcpyAsync(hnrm.data(), dnrm.p, hnrm.size() * 8, hipMemcpyDeviceToHost, st));
+      AD_CHECK(hipStreamSynchronize(st));
+      int nactive = 0;
+      for (int p = 0; p < np; p++) {
+        if (!pd[p].active) continue;
+        bool conv = true;
+        for (int c = 0; c < pd[p].k; c++)
+          if (std::sqrt(hnrm[(size_t)p * kmax + c]) > hp[p].tol) conv = false;
+        if (conv) pd[p].active = 0;
+        else iters[p] = it, nactive++;
+      }
+      if (nactive == 0) break;
+      AD_CHECK(hipMemcpyAsync(dpairs.p, pd.data(), pd.size() * sizeof(PairDesc), hipMemcpyHostToDevice, st));
+      // W = P T P R (the residual kernel wrote R)
+      hipLaunchKernelGGL(k_ad_project, pk, dim3(64), 0, st, dpairs.p, dqptr.p, dqpos.p, dqval.p,
+                         blk[W].p, 0, 0);
+      precondition();
+      apply(W, AW, BW);
+      rr_update(nb);
+      nb = 3;
+    }
+    // A X once more on the final X of every pair: the constraints
+    for (PairDesc &d : pd) d.active = 1;
+    AD_CHECK(hipMemcpyAsync(dpairs.p, pd.data(), pd.size() * sizeof(PairDesc), hipMemcpyHostToDevice, st));
+    apply(X, AX, BX);
+  }
+  std::vector<double> hAX((size_t)off);
+  if (np > 0) {
+    AD_CHECK(hipMemcpyAsync(hAX.data(), blk[AX].p, hAX.size() * 8, hipMemcpyDeviceToHost, st));
+    AD_CHECK(hipStreamSynchronize(st));
+  }
+  // ---- constraints per face (global dof -> value rows), largest first
+  std::map<int, std::vector<std::map<int, double>>> face_rows;  // face -> rows over global dofs
+  S.adapt_est = -1.0;
+  for (int p = 0; p < np; p++) {
+    const PairDesc &d = pd[p];
+    const Sub &si = S.subs[hp[p].i];
+    const double *lam = hlam.data() + (size_t)p * kmax;
+    int nad = 0;
+    while (nad < d.k && lam[nad] >= S.adapt_threshold) nad++;
+    const double est = nad < d.k ? lam[nad] : lam[nad - 1];
+    S.adapt_est = std::max(S.adapt_est, est);
+    S.adapt_pairs.push_back({hp[p].face, hp[p].i, hp[p].j, d.k, nad, iters[p]});
+    S.adapt_eigs.push_back(std::vector<double>(lam, lam + d.k));
+    auto &rows = face_rows[hp[p].face];
+    for (int c = 0; c < nad; c++) {
+      // c = -P (I - E^T) S (I - E) P v (the reference's operator), side i
+      std::map<int, double> row;
+      const double *a = hAX.data() + d.off + (int64_t)c * d.n;
+      for (int r = 0; r < si.ni; r++) row[si.isvgvn[si.ifc_dof[r]]] = -a[ref2dev[hp[p].i][r]];
+      rows.push_back(std::move(row));
+    }
+  }
+  // ---- every face block: its arithmetic rows and its adaptive rows
+  // restricted to the face, orthonormalized once (both subdomains get the
+  // same rows); coarse dofs renumbered with the new counts
+  std::map<int, std::vector<std::vector<double>>> face_q;  // face -> rows over its dofs
+  std::map<int, std::vector<int>> face_dofs;                // face -> global dofs, sorted
+  for (auto &f : face_subs) {
+    const int face = f.first;
+    const Sub &s = S.subs[f.second[0]];
+    std::vector<std::vector<double>> M;
+    std::vector<int> gd;
+    std::vector<std::pair<int, std::map<int, double>>> arith;
+    for (int c = 0; c < s.nconstr; c++) {
+      if (node_of(s.indrowc[c]) != face) continue;
+      std::map<int, double> row;
+      for (int q = s.cptr[c]; q < s.cptr[c + 1]; q++) {
+        const int g = s.isvgvn[s.ifc_dof[s.cidx[q]]];
+        row[g] = s.cval[q];
+        gd.push_back(g);
+      }
+      arith.push_back({c, row});
+    }
+    std::sort(gd.begin(), gd.end());
+    gd.erase(std::unique(gd.begin(), gd.end()), gd.end());
+    auto dense = [&](const std::map<int, double> &row) {
+      std::vector<double> v(gd.size(), 0.0);
+      for (size_t k = 0; k < gd.size(); k++) {
+        auto it = row.find(gd[k]);
+        if (it != row.end()) v[k] = it->second;
+      }
+      return v;
+    };
+    for (auto &a : arith) M.push_back(dense(a.second));
+    auto fr = face_rows.find(face);
+    if (fr != face_rows.end())
+      for (auto &row : fr->second) M.push_back(dense(row));
+    face_q[face] = orthonormal_rows(M, (int)gd.size());
+    face_dofs[face] = gd;
+  }
+  std::vector<int> nndfc(S.nndfc);
+  for (auto &f : face_q) nndfc[f.first] = (int)f.second.size();
+  std::vector<int> kdofc(nndfc.size() + 1, 0);
+  for (size_t k = 0; k < nndfc.size(); k++) kdofc[k + 1] = kdofc[k] + nndfc[k];
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < nsub; is++) {
+    Sub &s = S.subs[is];
+    std::map<int, int> g2r;
+    for (int r = 0; r < s.ni; r++) g2r[s.isvgvn[s.ifc_dof[r]]] = r;
+    std::vector<int> cptr{0}, cidx, indrowc;
+    std::vector<double> cval;
+    for (int id : s.cnodes) {
+      if (id >= first_face && face_q.count(id)) {
+        const auto &Q = face_q.at(id);
+        const auto &gd = face_dofs.at(id);
+        for (size_t r = 0; r < Q.size(); r++) {
+          for (size_t k = 0; k < gd.size(); k++)
+            if (Q[r][k] != 0.0) cidx.push_back(g2r.at(gd[k])), cval.push_back(Q[r][k]);
+          cptr.push_back((int)cidx.size());
+          indrowc.push_back(kdofc[id] + (int)r);
+        }
+      } else {
+        for (int c = 0; c < s.nconstr; c++) {
+          if (node_of(s.indrowc[c]) != id) continue;
+          for (int q = s.cptr[c]; q < s.cptr[c + 1]; q++) cidx.push_back(s.cidx[q]), cval.push_back(s.cval[q]);
+          cptr.push_back((int)cidx.size());
+          indrowc.push_back(kdofc[id] + (s.indrowc[c] - S.kdofc[id]));
+        }
+      }
+    }
+    s.cptr.swap(cptr);
+    s.cidx.swap(cidx);
+    s.cval.swap(cval);
+    s.indrowc.swap(indrowc);
+    s.nconstr = (int)s.indrowc.size();
+  }
+  S.nndfc = nndfc;
+  S.kdofc = kdofc;
+  S.ndofc = kdofc.back();
+  if (S.verbose > 0) {
+    int nad = 0;
+    for (auto &q : S.adapt_pairs) nad += q[4];
+    std::printf(" ADAPTIVITY_SOLVE_EIGENVECTORS: %d pairs, %d adaptive constraints, estimate %.4g, "
+                "%d coarse dofs\n", (int)S.adapt_pairs.size(), nad, S.adapt_est, S.ndofc);
+  }
+}
+
+}  // namespace bddc

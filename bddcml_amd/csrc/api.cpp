@@ -47,6 +47,7 @@ void bddcml_init_c(int *nl, int *nsublev, int *lnsublev, int *nsub_loc_1, int *c
                    int *verbose_level, int *numbase, int *just_direct_solve_int) try {
   (void)comm_init;
   (void)just_direct_solve_int;
+  device_prefer_spin_wait();
   reset_solver();
   set_task_order(1);
   set_trsv_global(false);
@@ -230,12 +231,17 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
     S.use_corner = 1;
     S.use_arith = 1;
     S.weights_type = 0;
+    S.use_adaptive = 0;
   } else {
-    if (*use_adaptive_constraints_int) fatal(R, "adaptive constraints are not supported by this library");
     if (*use_user_constraints_int) fatal(R, "user constraints are not supported by this library");
     S.use_corner = *use_corner_constraints_int != 0;
-    S.use_arith = *use_arithmetic_constraints_int != 0;
+    // adaptive constraints are an add-on to the arithmetic ones
+    // (src/module_levels.f90:2699-2706)
+    S.use_adaptive = *use_adaptive_constraints_int != 0;
+    S.use_arith = *use_arithmetic_constraints_int != 0 || S.use_adaptive;
     S.weights_type = *weights_type;
+    if (S.use_adaptive && S.nl >= 3 && S.nsub2 >= 2)
+      fatal(R, "adaptive constraints are supported with two levels");
   }
   double tp = wall_time();
   auto phase = [&](const char *name) {
@@ -274,7 +280,18 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
     S.is_setup = true;
     return;
   }
+  S.adapt_pass = S.use_adaptive != 0;
   device_setup_finish(S);
+  // adaptive constraints: face eigenproblems on the preconditioner just set
+  // up, then the interface phase again with the new constraints
+  // (src/module_levels.f90:2826-2945)
+  if (S.use_adaptive) {
+    adaptive_constraints(S);
+    S.adapt_pass = false;
+    phase("adaptive constraints");
+    device_setup_finish(S);
+    phase("interface phase again");
+  }
   // three or more levels: the next level from the coarse nodes and matrices,
   // recursively (a next level of one subdomain: the coarse problem is solved
   // directly)
@@ -489,6 +506,24 @@ catch (const Error &e) {
   return -1;
 }
 
+int bddcml_amd_adaptive_pairs(int ipair, int *out, double *eigs) try {
+  Solver &S = solver();
+  if (!S.is_setup) return -1;
+  if (ipair < 0) return (int)S.adapt_pairs.size();
+  if (ipair >= (int)S.adapt_pairs.size()) return -2;
+  const auto &q = S.adapt_pairs[ipair];
+  for (int k = 0; k < 6; k++) out[k] = q[k];
+  out[1] += S.numbase;
+  out[2] += S.numbase;
+  if (eigs)
+    for (size_t k = 0; k < S.adapt_eigs[ipair].size(); k++) eigs[k] = S.adapt_eigs[ipair][k];
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e);
+  return -100;
+}
+
 int bddcml_amd_level2_counts(int *out) try {
   Solver &S = solver();
   if (!S.is_setup || !S.lev2) return -1;
@@ -614,6 +649,26 @@ int bddcml_amd_set_option(const char *key, double value) try {
     set_trsv_global(value != 0.0);
     return 0;
   }
+  if (!std::strcmp(key, "adaptive_threshold")) {
+    if (!(value > 0.0)) return -2;
+    S.adapt_threshold = value;
+    return 0;
+  }
+  if (!std::strcmp(key, "adaptive_tol")) {
+    if (!(value > 0.0)) return -2;
+    S.adapt_tol = value;
+    return 0;
+  }
+  if (!std::strcmp(key, "adaptive_maxit")) {
+    if (value < 1.0) return -2;
+    S.adapt_maxit = (int)value;
+    return 0;
+  }
+  if (!std::strcmp(key, "adaptive_nvec")) {
+    if (value < 1.0 || value > 16.0) return -2;
+    S.adapt_nvec = (int)value;
+    return 0;
+  }
   if (!std::strcmp(key, "pcg_graph")) {
     if (value != 0.0 && value != 1.0) return -2;
     set_pcg_graph(value != 0.0);
@@ -723,6 +778,7 @@ int bddcml_amd_last_error(char *buf, int len) {
 
 int bddcml_amd_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
                              const double *X, double alpha, double beta, double *out) try {
+  device_prefer_spin_wait();
   device_selftest_tile(op, npairs, A, B, C, X, alpha, beta, out);
   return 0;
 }
@@ -744,6 +800,7 @@ catch (const Error &e) {
 }
 
 int bddcml_amd_selftest_copy(long long nbytes, double *out) try {
+  device_prefer_spin_wait();
   if (nbytes <= 0 || nbytes % 16 != 0) fatal("BDDCML_SELFTEST", "copy size must be a positive multiple of 16");
   out[0] = device_selftest_copy(nbytes);
   return 0;

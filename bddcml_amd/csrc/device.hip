@@ -44,6 +44,18 @@ namespace bddc {
       fatal("BDDCML_DEVICE", std::string(#x) + " failed: " + hipGetErrorString(e_));      \
   } while (0)
 
+// Host waits on the device spin instead of sleeping: a sleeping wait woke
+// milliseconds after short launches finished (about 6 ms per wait inside
+// setups and solves of tens of ms).  Effective only before the process's
+// device context exists; a later call is refused by the runtime and ignored.
+void device_prefer_spin_wait() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
+  (void)hipGetLastError();
+}
+
 double wall_time() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -217,6 +229,9 @@ struct Device {
   // start and end of every GEMM launch of the interior factorization (the
   // launches that take most of the setup's device time)
   std::vector<hipEvent_t> ev_gemm;
+  // around one S GEMV and one local preconditioner launch of a solve
+  hipEvent_t ev_apply[4] = {};
+  bool gemv_timed = false, kapply_timed = false;
   std::vector<double> gemm_flops;  // algorithmic flops of each of them
   hipEvent_t ev_up = nullptr;
   hipEvent_t ev_rhs = nullptr;  // reduced right-hand side done (upload stream)
@@ -260,6 +275,12 @@ struct Device {
   View<SubDesc> desc;
   View<int64_t> loff;
   View<int> NIs, map, cid, csize, cptr, cdof, cmap;
+  // general constraint rows (overlapping supports or values other than 1,
+  // as adaptive constraints give): row values, omega of every constraint and
+  // per interface dof the (constraint, value) lists
+  bool gen_constraints = false;
+  View<double> cvals, comega, dval;
+  View<int> dptr, dcon;
   View<double> w;
   DBuf<double> Z, Phi, G, Acs;
   int maxNI = 0, maxnc = 0, maxnti = 0, maxnt = 1, maxncT = 0;
@@ -371,6 +392,8 @@ void device_release(Solver &S) {
     for (hipEvent_t &e : D.ev)
       if (e) (void)hipEventDestroy(e), e = nullptr;
     for (hipEvent_t &e : D.ev_gemm)
+      if (e) (void)hipEventDestroy(e), e = nullptr;
+    for (hipEvent_t &e : D.ev_apply)
       if (e) (void)hipEventDestroy(e), e = nullptr;
     if (D.ev_up) (void)hipEventDestroy(D.ev_up), D.ev_up = nullptr;
     if (D.ev_rhs) (void)hipEventDestroy(D.ev_rhs), D.ev_rhs = nullptr;
@@ -1007,8 +1030,10 @@ void device_setup_finish(Solver &S) {
   // mirror
   {
     int64_t nf = 0;
-    D.inv_planned = S.explicit_inverse != 0;
-    D.inv_auto = S.explicit_inverse == 2;
+    // the first setup of an adaptive run forms S'^{-1} at once: the face
+    // eigenproblems' preconditioner applies it
+    D.inv_planned = S.explicit_inverse != 0 || S.adapt_pass;
+    D.inv_auto = S.explicit_inverse == 2 && !S.adapt_pass;
     D.explicit_inv = false;
     for (int is = 0; is < nsub; is++) assign_factor(D.sub[is], S.subs[is].nconstr, nf, D.inv_planned);
     if (nf >= INT32_MAX) fatal("BDDCML_DEVICE_PLAN", "too many factor tiles");
@@ -1211,7 +1236,7 @@ void device_setup_finish(Solver &S) {
     D.maxncT = std::max(D.maxncT, T.ncT);
   }
   std::vector<int> map(lv), cidv(lv), csz(co), cptrv(co + 1, 0), cdofv(cdoff[nsub]), cmapv(co);
-  std::vector<double> wv(lv);
+  std::vector<double> wv(lv), cvalv(cdoff[nsub]), omegav(co);
   int two_constraints = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : two_constraints)
   for (int is = 0; is < nsub; is++) {
@@ -1228,15 +1253,20 @@ void device_setup_finish(Solver &S) {
     std::fill(cid_loc, cid_loc + s.NI, -1);
     int64_t q2 = cdoff[is];
     for (int c = 0; c < s.nconstr; c++) {
+      double nrm2 = 0.0;
       for (int q = s.cptr[c]; q < s.cptr[c + 1]; q++) {
         int k = ref2dev[s.cidx[q]];
-        if (cid_loc[k] >= 0) two_constraints |= 1;
+        if (cid_loc[k] >= 0 || s.cval[q] != 1.0) two_constraints |= 1;
         cid_loc[k] = d.coff + c;
+        cvalv[q2] = s.cval[q];
         cdofv[q2++] = k;
+        nrm2 += s.cval[q] * s.cval[q];
       }
       cptrv[d.coff + c + 1] = (int)q2;
       csz[d.coff + c] = s.cptr[c + 1] - s.cptr[c];
       cmapv[d.coff + c] = s.indrowc[c];
+      // omega_c c c^T: rho / |D_c| for a unit row, as the indicator form
+      omegav[d.coff + c] = nrm2 > 0.0 ? d.rho / nrm2 : d.rho;
     }
     for (int k = 0; k < s.NI; k++) {
       if (k < s.ni) {
@@ -1249,9 +1279,36 @@ void device_setup_finish(Solver &S) {
       }
     }
   }
-  if (two_constraints)
-    fatal("BDDCML_SETUP_PRECONDITIONER", "a dof belongs to two constraints; only corner and "
-                                         "arithmetic constraints are supported");
+  // rows with overlapping supports or values other than 1 (adaptive
+  // constraints) take the general forms of the S' and C^T kernels
+  D.gen_constraints = two_constraints != 0;
+  if (D.gen_constraints) {
+    std::vector<int> cnt(lv + 1, 0);
+    for (int is = 0; is < nsub; is++) {
+      const SubDesc &d = D.hdesc[is];
+      for (int c = 0; c < d.nc; c++)
+        for (int q = cptrv[d.coff + c]; q < cptrv[d.coff + c + 1]; q++) cnt[d.loff + cdofv[q] + 1]++;
+    }
+    for (int64_t k = 0; k < lv; k++) cnt[k + 1] += cnt[k];
+    std::vector<int> dconv(cnt[lv]), fillp(cnt.begin(), cnt.end() - 1);
+    std::vector<double> dvalv(cnt[lv]);
+    // constraints visited in ascending global index: every list sorted
+    for (int is = 0; is < nsub; is++) {
+      const SubDesc &d = D.hdesc[is];
+      for (int c = 0; c < d.nc; c++)
+        for (int q = cptrv[d.coff + c]; q < cptrv[d.coff + c + 1]; q++) {
+          const int64_t a = d.loff + cdofv[q];
+          dconv[fillp[a]] = d.coff + c;
+          dvalv[fillp[a]++] = cvalv[q];
+        }
+    }
+    std::fill(cidv.begin(), cidv.end(), -1);
+    D.plan2.add(D.dptr, cnt);
+    D.plan2.add(D.dcon, dconv);
+    D.plan2.add(D.dval, dvalv);
+    D.plan2.add(D.cvals, cvalv);
+    D.plan2.add(D.comega, omegav);
+  }
   // limits of the kernels: G^{-1} in place with one LDS row buffer (k_ginv),
   // the whole interface vector of a subdomain in LDS (k_kapply), the coarse
   // matrix as full tiles
@@ -1487,14 +1544,21 @@ void device_setup_finish(Solver &S) {
   // S' factor and Z = S'^{-1} C^T in the factor space (FP32 when mixed);
   // then G = C Z, Phi = Z G^{-1} in FP64
   auto factor_phase = [&](auto *fp) {
-    launch_sprime(st, pool, fp, D.desc.p, D.sp_sub.p, D.sp_ij.p, D.sp_dst.p, D.n_sp, D.cid.p,
-                  D.csize.p);
+    if (D.gen_constraints)
+      launch_sprime_gen(st, pool, fp, D.desc.p, D.sp_sub.p, D.sp_ij.p, D.sp_dst.p, D.n_sp, D.dptr.p,
+                        D.dcon.p, D.dval.p, D.comega.p);
+    else
+      launch_sprime(st, pool, fp, D.desc.p, D.sp_sub.p, D.sp_ij.p, D.sp_dst.p, D.n_sp, D.cid.p,
+                    D.csize.p);
     for (size_t J = 0; J < D.p_diag.size(); J++) {
       launch_tile_potrf(st, fp, D.p_diag[J].tasks.p, D.p_diag[J].pairs.p, D.p_diag[J].ntask, D.err.p);
       launch_tile_gemm(st, fp, D.p_off[J].tasks.p, D.p_off[J].pairs.p, D.p_off[J].ntask, GEMM_NT,
                        POST_RIGHT_XT, 1.0, -1.0);
     }
-    launch_ct_init(st, fp, D.desc.p, nsub, D.cid.p, D.maxnti);
+    if (D.gen_constraints)
+      launch_ct_init_gen(st, fp, D.desc.p, nsub, D.dptr.p, D.dcon.p, D.dval.p, D.maxnti);
+    else
+      launch_ct_init(st, fp, D.desc.p, nsub, D.cid.p, D.maxnti);
     for (size_t i = 0; i < D.z_fwd.size(); i++)
       launch_tile_gemm(st, fp, D.z_fwd[i].tasks.p, D.z_fwd[i].pairs.p, D.z_fwd[i].ntask, GEMM_NN,
                        POST_LEFT_X, 1.0, -1.0);
@@ -1512,12 +1576,14 @@ void device_setup_finish(Solver &S) {
   };
   if (D.mixed) factor_phase(D.pool32.p);
   else factor_phase(D.fpool.p);
-  launch_g(st, D.desc.p, nsub, D.cptr.p, D.cdof.p, D.Z.p, D.G.p);
+  launch_g(st, D.desc.p, nsub, D.cptr.p, D.cdof.p, D.gen_constraints ? D.cvals.p : nullptr, D.Z.p,
+           D.G.p);
   launch_ginv(st, D.desc.p, nsub, D.G.p, nullptr, D.err.p);
   if (D.mixed) phi_phase(D.pool32.p);
   else phi_phase(D.fpool.p);
   if (D.inv_planned && !D.inv_auto) form_inverse(S);
-  launch_acs(st, D.desc.p, nsub, D.cptr.p, D.G.p, D.Acs.p);
+  launch_acs(st, D.desc.p, nsub, D.cptr.p, D.gen_constraints ? D.comega.p : nullptr, D.G.p,
+             D.Acs.p);
   HIPCHECK(hipEventRecord(ev[3], st));
   // coarse problem (with three levels the second level is set up instead):
   // its task lists and assembly map are built on the host while the device
@@ -1824,6 +1890,16 @@ double device_selftest_copy(long long nbytes) {
   return ms;
 }
 
+// the packed lower tiles of every S_i: the subdomain pool, and per
+// subdomain the first tile of S and its tile count
+const double *device_schur_tiles(Solver &S, std::vector<int> &slow, std::vector<int> &nti) {
+  if (!S.dev || !S.dev->pool.p) fatal("BDDCML_DEVICE", "solver not set up on the device");
+  slow.resize(S.nsub);
+  nti.resize(S.nsub);
+  for (int is = 0; is < S.nsub; is++) slow[is] = S.dev->sub[is].slow, nti[is] = S.dev->sub[is].nti;
+  return S.dev->pool.p;
+}
+
 hipStream_t device_stream(Solver &S) {
   if (!S.dev || !S.dev->st) fatal("BDDCML_DEVICE", "solver not set up on the device");
   return S.dev->st;
@@ -1914,6 +1990,20 @@ void device_coarse_matrices(Solver &S, std::vector<std::vector<double>> &acs) {
   }
 }
 
+void device_pair_pc_source(Solver &S, PairPcSource &o) {
+  Device &D = *S.dev;
+  if (!D.explicit_inv) fatal("ADAPTIVITY_SOLVE_EIGENVECTORS", "explicit S'^-1 not formed by the setup");
+  HIPCHECK(hipStreamSynchronize(D.st));
+  o.inv = D.mixed ? (const void *)D.pool32.p : (const void *)D.fpool.p;
+  o.inv_float = D.mixed;
+  o.Z = D.Z.p;
+  o.Phi = D.Phi.p;
+  o.pbase.resize(S.nsub);
+  o.zoff.resize(S.nsub);
+  for (int is = 0; is < S.nsub; is++) o.pbase[is] = D.hdesc[is].pbase, o.zoff[is] = D.hdesc[is].zoff;
+  device_coarse_matrices(S, o.Ac);
+}
+
 // Entry p of the second level's device-ordered subdomain vectors takes the
 // level-1 coarse residual at its coarse dof, weighted at the interface (the
 // residual is a complete right-hand side: src/module_levels.f90:3914-3926,
@@ -1962,30 +2052,46 @@ void device_link_levels(Solver &S) {
 }
 
 // --------------------------------------------------------------- operators
+// events around one S GEMV launch and one local preconditioner launch of
+// a solve, read after its final synchronize (recording them does not stop
+// the host)
+static void timing_events(Device &D) {
+  for (hipEvent_t &e : D.ev_apply)
+    if (!e) HIPCHECK(hipEventCreate(&e));
+}
+
 static void op_schur(Solver &S, const double *x, double *y, bool timed = false) {
   Device &D = *S.dev;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
   if (timed) {
-    HIPCHECK(hipEventCreate(&e0));
-    HIPCHECK(hipEventCreate(&e1));
-    HIPCHECK(hipEventRecord(e0, D.st));
+    timing_events(D);
+    HIPCHECK(hipEventRecord(D.ev_apply[0], D.st));
   }
   launch_symv(D.st, (const double *)D.pool.p, D.scols.p, D.nscols, D.srows.p, D.nsrows, D.loff.p,
               D.map.p, nullptr, x, D.spart.p, D.yl.p);
   if (timed) {
-    HIPCHECK(hipEventRecord(e1, D.st));
-    HIPCHECK(hipEventSynchronize(e1));
-    float ms = 0;
-    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-    S.t.ev_gemv_ms = ms;
+    HIPCHECK(hipEventRecord(D.ev_apply[1], D.st));
     double tiles = 0;
     for (const SubTiles &T : D.sub) tiles += (double)T.nti * (T.nti + 1) / 2;
     // lower matrix tiles + gathered input + written output (8 bytes each)
     S.t.gemv_bytes = tiles * TILE_ELEMS * 8.0 + 16.0 * (double)D.LV;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    D.gemv_timed = true;
   }
   launch_assemble(D.st, S.nig, D.aptr.p, D.asrc.p, nullptr, D.yl.p, y);
+}
+
+// event times of the timed applies of the last solve (after its synchronize)
+static void read_apply_timings(Solver &S) {
+  Device &D = *S.dev;
+  float ms = 0;
+  if (D.gemv_timed) {
+    HIPCHECK(hipEventElapsedTime(&ms, D.ev_apply[0], D.ev_apply[1]));
+    S.t.ev_gemv_ms = ms;
+  }
+  if (D.kapply_timed) {
+    HIPCHECK(hipEventElapsedTime(&ms, D.ev_apply[2], D.ev_apply[3]));
+    S.t.ev_kapply_ms = ms;
+  }
+  D.gemv_timed = D.kapply_timed = false;
 }
 
 static void op_pc(Solver &S, const double *r, double *z, bool timed = false);
@@ -2025,7 +2131,6 @@ static void inverse_at_solve_start(Solver &S) {
 static void op_pc(Solver &S, const double *r, double *z, bool timed) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
   if (D.inv_auto && !D.explicit_inv) ++D.pc_count;
   // coarse correction uc = Ac^{-1} sum_i R_ci^T Phi_i^T W_i r_i (three
   // levels: the second level's BDDC in place of Ac^{-1})
@@ -2037,9 +2142,8 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed) {
     launch_dense_gemv(st, D.cpool.p, D.cfull, D.ntc, D.rc.p, D.cpart.p, D.uc.p);
   // zl_i = K_i W_i r_i + Phi_i uc(indrowc), then z = sum_i R_i^T W_i zl_i
   if (timed) {
-    HIPCHECK(hipEventCreate(&e0));
-    HIPCHECK(hipEventCreate(&e1));
-    HIPCHECK(hipEventRecord(e0, st));
+    timing_events(D);
+    HIPCHECK(hipEventRecord(D.ev_apply[2], st));
   }
   // explicit inverse: yl = S'^{-1} (w .* r) by the symmetric tile GEMV
   const double *ysym = nullptr;
@@ -2059,11 +2163,8 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed) {
     launch_kapply(st, (const double *)D.fpool.p, D.desc.p, S.nsub, D.map.p, D.w.p, r, D.Phi.p,
                   D.Z.p, D.cmap.p, D.uc.p, ysym, D.zl.p, D.maxNI, D.maxnc, D.kapply_deep);
   if (timed) {
-    HIPCHECK(hipEventRecord(e1, st));
-    HIPCHECK(hipEventSynchronize(e1));
-    float ms = 0;
-    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-    S.t.ev_kapply_ms = ms;
+    HIPCHECK(hipEventRecord(D.ev_apply[3], st));
+    D.kapply_timed = true;
     // packed S' factor read by both sweeps, Z and Phi columns, map (4 B),
     // weight, gathered input and output (8 B each) per local interface entry
     double tiles = 0, zb = 0;
@@ -2075,8 +2176,6 @@ static void op_pc(Solver &S, const double *r, double *z, bool timed) {
     // the explicit inverse is read once (symmetric GEMV), the factor twice
     const double passes = D.explicit_inv ? 1.0 : 2.0;
     S.t.kapply_bytes = passes * tiles * TILE_ELEMS * fe + 8.0 * zb + 28.0 * (double)D.LV;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
   }
   launch_assemble(st, S.nig, D.aptr.p, D.asrc.p, D.w.p, D.zl.p, z);
 }
@@ -2529,6 +2628,7 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bo
   }
 finish:
   HIPCHECK(hipStreamSynchronize(st));
+  read_apply_timings(S);
   S.t.krylov += wall_time() - t0;
   t0 = wall_time();
   postprocess(S);

@@ -100,13 +100,25 @@ void launch_y_to_z(hipStream_t st, const T *pool, const SubDesc *subs, int nsub,
 template <typename T>
 void launch_ginv_to_tiles(hipStream_t st, T *pool, const SubDesc *subs, int nsub, const double *G,
                           int maxncT);
-// G = C Z, Ginv = G^{-1} (in place Gauss-Jordan, SPD), Acs = Ginv - diag(rho/|D_c|)
+// general constraint rows (several dense rows per glob, as adaptive
+// constraints give): per interface dof (local flat index) the list
+// dptr/dcon/dval of (global constraint index, value), constraints ascending;
+// S' = S + sum_k omega_k c_k c_k^T and C^T tiles from those lists
+template <typename T>
+void launch_sprime_gen(hipStream_t st, const double *pool, T *ppool, const SubDesc *subs,
+                       const int *jobs_sub, const int *jobs_ij, const int *jobs_dst, int njobs,
+                       const int *dptr, const int *dcon, const double *dval, const double *omega);
+template <typename T>
+void launch_ct_init_gen(hipStream_t st, T *pool, const SubDesc *subs, int nsub, const int *dptr,
+                        const int *dcon, const double *dval, int maxnti);
+// G = C Z (row values cval, 1 when null), Ginv = G^{-1} (in place
+// Gauss-Jordan, SPD), Acs = Ginv - Omega (Omega_c = rho/|D_c|, or omega[c])
 void launch_g(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const int *cdof,
-              const double *Z, double *G);
+              const double *cval, const double *Z, double *G);
 void launch_ginv(hipStream_t st, const SubDesc *subs, int nsub, double *G, double *work, int *err);
 
-void launch_acs(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const double *G,
-                double *Acs);
+void launch_acs(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr,
+                const double *omega, const double *G, double *Acs);
 // coarse assembly: dst[k] = sum_{q in ptr[k]..ptr[k+1]} Acs[src[q]] into pool
 void launch_coarse_assemble(hipStream_t st, double *pool, const int64_t *dst, const int *ptr,
                             const int64_t *src, int n, const double *Acs);

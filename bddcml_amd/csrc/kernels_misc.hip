@@ -71,6 +71,109 @@ template void launch_sprime<float>(hipStream_t, const double *, float *, const S
                                    const int *, const int *, const int *, int, const int *,
                                    const int *);
 
+// General constraint rows (adaptive constraints: several dense rows on one
+// face): every interface dof carries the list of (constraint, value) of the
+// rows it appears in, sorted by constraint; S'(a, b) = S(a, b) + sum over
+// the constraints k of both lists of omega_k C_k(a) C_k(b).  Same jobs and
+// thread layout as k_sprime.
+__device__ __forceinline__ double shared_rows(const int *__restrict__ dptr,
+                                              const int *__restrict__ dcon,
+                                              const double *__restrict__ dval,
+                                              const double *__restrict__ omega, int64_t a,
+                                              int64_t b) {
+  int p = dptr[a], pe = dptr[a + 1], q = dptr[b], qe = dptr[b + 1];
+  double s = 0.0;
+  while (p < pe && q < qe) {
+    const int ka = dcon[p], kb = dcon[q];
+    if (ka < kb) p++;
+    else if (kb < ka) q++;
+    else {
+      s += omega[ka] * dval[p] * dval[q];
+      p++, q++;
+    }
+  }
+  return s;
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_sprime_gen(const double *__restrict__ pool,
+                                                    T *__restrict__ ppool,
+                                                    const SubDesc *__restrict__ subs,
+                                                    const int *__restrict__ jsub,
+                                                    const int *__restrict__ jij,
+                                                    const int *__restrict__ jdst,
+                                                    const int *__restrict__ dptr,
+                                                    const int *__restrict__ dcon,
+                                                    const double *__restrict__ dval,
+                                                    const double *__restrict__ omega) {
+  const int job = blockIdx.x;
+  const SubDesc d = subs[jsub[job]];
+  const int I = jij[job] >> 16, J = jij[job] & 0xffff;
+  const double *src =
+      pool + (size_t)(d.slow + J * d.nti - J * (J - 1) / 2 + (I - J)) * TILE_ELEMS;
+  T *dst = ppool + (size_t)jdst[job] * TILE_ELEMS;
+  const int tid = threadIdx.x;
+  const int r0 = 2 * (tid & 31), c0 = tid >> 5;
+  const int64_t a0 = d.loff + I * 64 + r0;
+#pragma unroll 2
+  for (int k = 0; k < 8; k++) {
+    const int c = c0 + 8 * k;
+    const double2 v = *reinterpret_cast<const double2 *>(src + c * 64 + r0);
+    const int64_t b = d.loff + J * 64 + c;
+    const double x = v.x + shared_rows(dptr, dcon, dval, omega, a0, b);
+    const double y = v.y + shared_rows(dptr, dcon, dval, omega, a0 + 1, b);
+    store2(dst + c * 64 + r0, x, y);
+  }
+}
+
+template <typename T>
+void launch_sprime_gen(hipStream_t st, const double *pool, T *ppool, const SubDesc *subs,
+                       const int *jobs_sub, const int *jobs_ij, const int *jobs_dst, int njobs,
+                       const int *dptr, const int *dcon, const double *dval, const double *omega) {
+  if (njobs <= 0) return;
+  hipLaunchKernelGGL(k_sprime_gen<T>, dim3(njobs), dim3(256), 0, st, pool, ppool, subs, jobs_sub,
+                     jobs_ij, jobs_dst, dptr, dcon, dval, omega);
+}
+template void launch_sprime_gen<double>(hipStream_t, const double *, double *, const SubDesc *,
+                                        const int *, const int *, const int *, int, const int *,
+                                        const int *, const double *, const double *);
+template void launch_sprime_gen<float>(hipStream_t, const double *, float *, const SubDesc *,
+                                       const int *, const int *, const int *, int, const int *,
+                                       const int *, const double *, const double *);
+
+// C^T tiles for general rows: B(i, c) = value of dof i in constraint c
+template <typename T>
+__global__ void k_ct_init_gen(T *__restrict__ pool, const SubDesc *__restrict__ subs,
+                              const int *__restrict__ dptr, const int *__restrict__ dcon,
+                              const double *__restrict__ dval) {
+  const SubDesc d = subs[blockIdx.y];
+  const int I = blockIdx.x;
+  if (I >= d.nti) return;
+  for (int cT = 0; cT < d.ncT; cT++) {
+    T *Y = pool + (size_t)(d.ybase + I * d.ncT + cT) * TILE_ELEMS;
+    for (int e = threadIdx.x; e < TILE_ELEMS; e += blockDim.x) {
+      const int r = e & 63, c = cT * 64 + (e >> 6);
+      double v = 0.0;
+      if (c < d.nc) {
+        const int64_t a = d.loff + I * 64 + r;
+        for (int q = dptr[a]; q < dptr[a + 1]; q++)
+          if (dcon[q] == d.coff + c) v = dval[q];
+      }
+      Y[e] = (T)v;
+    }
+  }
+}
+template <typename T>
+void launch_ct_init_gen(hipStream_t st, T *pool, const SubDesc *subs, int nsub, const int *dptr,
+                        const int *dcon, const double *dval, int maxnti) {
+  if (nsub <= 0 || maxnti <= 0) return;
+  hipLaunchKernelGGL(k_ct_init_gen<T>, dim3(maxnti, nsub), dim3(256), 0, st, pool, subs, dptr, dcon,
+                     dval);
+}
+template void launch_ct_init_gen<double>(hipStream_t, double *, const SubDesc *, int, const int *,
+                                         const int *, const double *, int);
+template void launch_ct_init_gen<float>(hipStream_t, float *, const SubDesc *, int, const int *,
+                                        const int *, const double *, int);
+
 // --------------------------------------------- C^T blocks and Z = S'^{-1} C^T
 template <typename T>
 __global__ void k_ct_init(T *__restrict__ pool, const SubDesc *__restrict__ subs,
@@ -100,16 +203,17 @@ __global__ void k_y_to_z(const T *__restrict__ pool, const SubDesc *__restrict__
     Z[d.zoff + (size_t)c * d.NI + i] = (double)Y[(size_t)(c >> 6) * TILE_ELEMS + (c & 63) * 64];
 }
 
+// G = C Z over the constraint rows (values 1 when cval is null)
 __global__ void k_g(const SubDesc *__restrict__ subs, const int *__restrict__ cptr,
-                    const int *__restrict__ cdof, const double *__restrict__ Z,
-                    double *__restrict__ G) {
+                    const int *__restrict__ cdof, const double *__restrict__ cval,
+                    const double *__restrict__ Z, double *__restrict__ G) {
   const SubDesc d = subs[blockIdx.x];
   const int nc = d.nc;
   for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) {
     int c = e / nc, k = e % nc;
     double s = 0.0;
     for (int q = cptr[d.coff + c]; q < cptr[d.coff + c + 1]; q++)
-      s += Z[d.zoff + (size_t)k * d.NI + cdof[q]];
+      s += (cval ? cval[q] : 1.0) * Z[d.zoff + (size_t)k * d.NI + cdof[q]];
     G[d.goff + (size_t)k * nc + c] = s;
   }
 }
@@ -148,14 +252,17 @@ __global__ void k_ginv(const SubDesc *__restrict__ subs, double *__restrict__ G,
   }
 }
 
+// Ac_i = G^{-1} - Omega; Omega_c = rho / |D_c| for unit rows, or omega[c]
 __global__ void k_acs(const SubDesc *__restrict__ subs, const int *__restrict__ cptr,
-                      const double *__restrict__ G, double *__restrict__ Acs) {
+                      const double *__restrict__ omega, const double *__restrict__ G,
+                      double *__restrict__ Acs) {
   const SubDesc d = subs[blockIdx.x];
   const int nc = d.nc;
   for (int e = threadIdx.x; e < nc * nc; e += blockDim.x) {
     int c = e / nc, k = e % nc;
     double v = G[d.goff + e];
-    if (c == k) v -= d.rho / (double)(cptr[d.coff + c + 1] - cptr[d.coff + c]);
+    if (c == k)
+      v -= omega ? omega[d.coff + c] : d.rho / (double)(cptr[d.coff + c + 1] - cptr[d.coff + c]);
     Acs[d.goff + e] = v;
   }
 }
@@ -216,19 +323,19 @@ template void launch_y_to_z<double>(hipStream_t, const double *, const SubDesc *
 template void launch_y_to_z<float>(hipStream_t, const float *, const SubDesc *, int, double *, int,
                                    int);
 void launch_g(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const int *cdof,
-              const double *Z, double *G) {
+              const double *cval, const double *Z, double *G) {
   if (nsub <= 0) return;
-  hipLaunchKernelGGL(k_g, dim3(nsub), dim3(256), 0, st, subs, cptr, cdof, Z, G);
+  hipLaunchKernelGGL(k_g, dim3(nsub), dim3(256), 0, st, subs, cptr, cdof, cval, Z, G);
 }
 void launch_ginv(hipStream_t st, const SubDesc *subs, int nsub, double *G, double *work, int *err) {
   (void)work;
   if (nsub <= 0) return;
   hipLaunchKernelGGL(k_ginv, dim3(nsub), dim3(256), 0, st, subs, G, err);
 }
-void launch_acs(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr, const double *G,
-                double *Acs) {
+void launch_acs(hipStream_t st, const SubDesc *subs, int nsub, const int *cptr,
+                const double *omega, const double *G, double *Acs) {
   if (nsub <= 0) return;
-  hipLaunchKernelGGL(k_acs, dim3(nsub), dim3(256), 0, st, subs, cptr, G, Acs);
+  hipLaunchKernelGGL(k_acs, dim3(nsub), dim3(256), 0, st, subs, cptr, omega, G, Acs);
 }
 void launch_coarse_assemble(hipStream_t st, double *pool, const int64_t *dst, const int *ptr,
                             const int64_t *src, int n, const double *Acs) {

@@ -640,6 +640,28 @@ void apply_boundary_conditions(Solver &S) {
   if (S.bc_global) exchange_bc(S);
 }
 
+// rho of subdomain is at its interface dofs (reference interface order) for
+// the solver's weights type (dd_get_my_coefficients_for_weights,
+// src/module_dd.f90:11660-11725); false when type 3 lacks its dof data
+bool weight_coefficients(const Solver &S, int is, std::vector<double> &rho) {
+  const Sub &s = S.subs[is];
+  const size_t ni = s.ifc_dof.size();
+  rho.assign(ni, 1.0);
+  if (S.weights_type == 1) {
+    std::vector<double> dg(s.ndofs, 0.0);
+    for (size_t k = 0; k < s.av.size(); k++)
+      if (s.ai[k] == s.aj[k]) dg[s.ai[k]] += s.av[k];
+    for (size_t k = 0; k < ni; k++) {
+      rho[k] = dg[s.ifc_dof[k]];
+      if (rho[k] == 0.0) rho[k] = 1.0;
+    }
+  } else if (S.weights_type == 3) {
+    if ((int)s.dof_data.size() != s.ndofs) return false;
+    for (size_t k = 0; k < ni; k++) rho[k] = s.dof_data[s.ifc_dof[k]];
+  }
+  return true;
+}
+
 void build_weights(Solver &S) {
   if (S.weights_type != 0 && S.weights_type != 1 && S.weights_type != 3)
     fatal("DD_GET_MY_COEFFICIENTS_FOR_WEIGHTS",
@@ -648,23 +670,9 @@ void build_weights(Solver &S) {
   int bad_data = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad_data)
   for (int is = 0; is < S.nsub; is++) {
-    Sub &s = S.subs[is];
-    size_t ni = s.ifc_dof.size();
-    rho[is].assign(ni, 1.0);
-    if (S.weights_type == 1) {
-      std::vector<double> dg(s.ndofs, 0.0);
-      for (size_t k = 0; k < s.av.size(); k++)
-        if (s.ai[k] == s.aj[k]) dg[s.ai[k]] += s.av[k];
-      for (size_t k = 0; k < ni; k++) {
-        rho[is][k] = dg[s.ifc_dof[k]];
-        if (rho[is][k] == 0.0) rho[is][k] = 1.0;
-      }
-    } else if (S.weights_type == 3) {
-      if ((int)s.dof_data.size() != s.ndofs) {
-        bad_data = 1;
-        continue;
-      }
-      for (size_t k = 0; k < ni; k++) rho[is][k] = s.dof_data[s.ifc_dof[k]];
+    if (!weight_coefficients(S, is, rho[is])) {
+      bad_data = 1;
+      continue;
     }
     tot[is] = rho[is];
   }

@@ -4,6 +4,7 @@
 // src/module_krylov.f90:34-43): one solver per process, not reentrant.
 #pragma once
 
+#include <array>
 #include <memory>
 #include <string>
 #include <vector>
@@ -51,6 +52,21 @@ struct Solver {
   // at setup, 2 formed once the preconditioner has been applied 40 times on
   // the same setup (the default: easy problems never pay for it)
   int explicit_inverse = 2;
+  // adaptive constraints: eigenvalue threshold, LOBPCG tolerance relative to
+  // the pair's mean weight coefficient, iteration limit and block size
+  // (src/module_adaptivity.f90:35-46)
+  int use_adaptive = 0;
+  double adapt_threshold = 1.5, adapt_tol = 1e-9;
+  int adapt_maxit = 15, adapt_nvec = 10;
+  // set while the first (arithmetic-constraint) setup of an adaptive run is
+  // built: that setup forms the explicit S'^{-1}, which the preconditioner
+  // of the face eigenproblems applies
+  bool adapt_pass = false;
+  // results per pair: face coarse node, subdomains, eigenvalues sought,
+  // adaptive constraints, LOBPCG iterations; eigenvalues (largest first)
+  std::vector<std::array<int, 6>> adapt_pairs;
+  std::vector<std::vector<double>> adapt_eigs;
+  double adapt_est = -1.0;
   // classes of subdomains with identical structure (ordering, symbolic)
   std::vector<int> ord_cls, ord_reps;
   // topology results
@@ -107,6 +123,9 @@ void apply_boundary_conditions(Solver &S);
 void refresh_boundary_conditions(Solver &S);  // new fixv, same fixed dofs
 void build_reactions(Solver &S);              // from the solution of the last solve
 void build_weights(Solver &S);
+// weight coefficients rho of one subdomain at its interface dofs (reference
+// interface order) for the weights type; false when its data is missing
+bool weight_coefficients(const Solver &S, int is, std::vector<double> &rho);
 void build_constraints(Solver &S);
 void build_condensation_rhs(Solver &S, int isub_only = -1);
 void build_unique_interface(Solver &S);
@@ -154,6 +173,31 @@ void set_pcg_graph(bool on);
 void device_fetch_solution(Solver &S);
 void device_setup_begin(Solver &S);
 void device_setup_finish(Solver &S);
+// the subdomain pool holding the packed lower tiles of every S_i, with the
+// first tile and the tile count of each (device.hip)
+const double *device_schur_tiles(Solver &S, std::vector<int> &slow, std::vector<int> &nti);
+// adaptive constraints (adapt.hip): face eigenproblems of every pair of
+// subdomains sharing a face, solved on the device by batched LOBPCG on the
+// set-up arithmetic-constraint preconditioner, then the eigenvectors above
+// the threshold loaded as constraints of the two faces, each face block
+// orthonormalized, the coarse dofs renumbered (src/module_adaptivity.f90:
+// 296-3024, src/module_dd.f90:4386-4457, 4637-4802); the interface phase is
+// then set up again (device_setup_finish)
+void adaptive_constraints(Solver &S);
+// what the preconditioner of the face eigenproblems reads from the device
+// (device.hip): the explicit S'^{-1} tiles (factor pool, FP32 in the
+// mixed-precision mode; packed lower by columns from each subdomain's
+// pbase), Z and Phi (NI x nc column major at zoff) and host copies of the
+// coarse matrices Ac_i = Phi_i^T S_i Phi_i (nconstr^2, row major)
+struct PairPcSource {
+  const void *inv = nullptr;
+  bool inv_float = false;
+  const double *Z = nullptr, *Phi = nullptr;
+  std::vector<int> pbase;
+  std::vector<int64_t> zoff;
+  std::vector<std::vector<double>> Ac;
+};
+void device_pair_pc_source(Solver &S, PairPcSource &o);
 void device_reduced_rhs(Solver &S);
 // coarse matrices Ac_i of every subdomain (nconstr^2 each, row major) from HBM
 void device_coarse_matrices(Solver &S, std::vector<std::vector<double>> &acs);
@@ -178,5 +222,8 @@ void device_selftest_tile(int op, int npairs, const double *A, const double *B, 
 void device_release(Solver &S);
 
 double wall_time();
+// ask the runtime to spin (not sleep) in host waits on the device; called
+// before the library's first device call
+void device_prefer_spin_wait();
 
 }  // namespace bddc

@@ -26,6 +26,14 @@ int bddcml_amd_level2_counts(int *out);
 /* the same for level ilevel >= 2 of a setup with more levels (-1 if that
  * level does not exist: fewer levels, or a level of one subdomain above it) */
 int bddcml_amd_level_counts(int ilevel, int *out);
+/* Adaptive constraints (bddcml_setup_preconditioner_c with
+ * use_adaptive_constraints_int = 1): with ipair < 0 the number of face
+ * pairs; otherwise for pair ipair out[0] = its face's coarse node (0-based),
+ * out[1], out[2] = its subdomains (numbase), out[3] = eigenvalues sought,
+ * out[4] = adaptive constraints made, out[5] = LOBPCG iterations, and
+ * eigs (out[3] entries, largest first) its eigenvalues; the eigenproblems of
+ * src/module_adaptivity.f90:296-3024 */
+int bddcml_amd_adaptive_pairs(int ipair, int *out, double *eigs);
 /* sorted global dof ids (0-based) of the interface, length out[4] above */
 int bddcml_amd_interface_dofs(int *out);
 /* y = S x, assembled Schur complement on the interface (device) */
@@ -70,6 +78,13 @@ int bddcml_amd_timings(double *out, int n);
  *     memory -- the forms used automatically for subdomains whose vector does
  *     not fit in LDS (about 20,000 dofs, or interfaces of about 19,000);
  *     for testing);
+ *   "adaptive_threshold", "adaptive_tol", "adaptive_maxit",
+ *     "adaptive_nvec" (adaptive constraints: eigenvalues at or above the
+ *     threshold become constraints, 1.5 by default; LOBPCG stops when every
+ *     residual norm is below tol times the pair's mean weight coefficient,
+ *     1e-9 by default, or after maxit iterations, 15; at most nvec
+ *     eigenvectors per face, 10; the defaults of
+ *     src/module_adaptivity.f90:35-46);
  *   "pcg_graph" (1, the default: PCG iterations from the third on replay
  *     one captured graph of the iteration, with the iteration index read
  *     from device memory and r.r written to pinned host memory for the

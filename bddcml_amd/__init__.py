@@ -36,7 +36,7 @@ EXTRA_SYMBOLS = [
     "bddcml_amd_sub_info", "bddcml_amd_sub_maps", "bddcml_amd_sub_matrix",
     "bddcml_amd_sub_coarse", "bddcml_amd_last_error", "bddcml_amd_selftest_tile",
     "bddcml_amd_selftest_copy", "bddcml_amd_element_gradient", "bddcml_amd_recycling_basis",
-    "bddcml_amd_level2_counts", "bddcml_amd_level_counts",
+    "bddcml_amd_level2_counts", "bddcml_amd_level_counts", "bddcml_amd_element_gradient_noalloc",
     "bddcml_amd_adaptive_pairs",
 ]
 
@@ -55,6 +55,7 @@ def lib():
                                                C.c_double, dp]
         L.bddcml_amd_selftest_copy.argtypes = [C.c_longlong, dp]
         L.bddcml_amd_element_gradient.argtypes = [dp, C.c_int, C.c_int, dp, dp, dp]
+        L.bddcml_amd_element_gradient_noalloc.argtypes = [dp, C.c_int, C.c_int, dp, dp, dp]
         _lib = L
     return _lib
 

@@ -799,6 +799,18 @@ catch (const Error &e) {
   return -100;
 }
 
+int bddcml_amd_element_gradient_noalloc(const double *ae, int nke, int per_element, const double *u,
+                                        const double *lam, double *grad) try {
+  Solver &S = solver();
+  if (!S.is_setup || S.plan_only) fatal("BDDCML_AMD_ELEMENT_GRADIENT_NOALLOC", "solver not set up");
+  device_element_gradient_noalloc(S, ae, nke, per_element != 0, u, lam, grad);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e);
+  return -100;
+}
+
 int bddcml_amd_selftest_copy(long long nbytes, double *out) try {
   device_prefer_spin_wait();
   if (nbytes <= 0 || nbytes % 16 != 0) fatal("BDDCML_SELFTEST", "copy size must be a positive multiple of 16");

@@ -217,6 +217,10 @@ double device_selftest_copy(long long nbytes);  // milliseconds
 // grad[isegn(e)] = -lam_e^T A_e u_e over all elements (global u, lam)
 void device_element_gradient(Solver &S, const double *ae, int nke, bool per_element,
                              const double *u, const double *lam, double *grad);
+// the same without device allocations: its buffers live in a fixed device
+// array of the library (noalloc.hip); fails on problems that exceed it
+void device_element_gradient_noalloc(Solver &S, const double *ae, int nke, bool per_element,
+                                     const double *u, const double *lam, double *grad);
 void device_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
                           const double *X, double alpha, double beta, double *out);
 void device_release(Solver &S);

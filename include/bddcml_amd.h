@@ -136,6 +136,16 @@ int bddcml_amd_selftest_tile(int op, int npairs, const double *A, const double *
  * nelem entries.  Needs isegn at upload. */
 int bddcml_amd_element_gradient(const double *ae, int nke, int per_element, const double *u,
                                 const double *lam, double *grad);
+/* The same element gradient without device allocations.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (loops of
+ * gradient evaluations under a no-hipMalloc rule).  Its device buffers (the
+ * element dof table and ids, the element matrices, u, lam and the gradient)
+ * live in a fixed array of 64 MiB per GPU, held until the process ends;
+ * problems that need more (about 4 x ndof + (nke + 2) x nelem doubles, with
+ * nelem x nke^2 more for per_element = 1) are rejected with an error.  Calls
+ * that overlap on one GPU share the array and race. */
+int bddcml_amd_element_gradient_noalloc(const double *ae, int nke, int per_element,
+                                        const double *u, const double *lam, double *grad);
 /* Krylov recycling (bddcml_solve_c with recycling_int = 1, PCG): number
  * of directions kept for later solves on the same factorization and
  * ||V^T W - I||_F of the stored basis (V A-orthonormal, W = S V), the

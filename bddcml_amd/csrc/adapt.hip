@@ -417,6 +417,96 @@ static void jacobi_eig(int n, std::vector<double> &a, std::vector<double> &w, st
   for (int i = 0; i < n; i++) w[i] = a[(size_t)i * n + i];
 }
 
+// Symmetric eigendecomposition of an n x n matrix (row major, overwritten)
+// for orders where cyclic Jacobi is too slow: Householder reduction to
+// tridiagonal form T = Q^T A Q, then implicit symmetric QR steps with
+// Wilkinson shifts on T, the rotations accumulated into Q (Golub and Van
+// Loan, Matrix Computations, algorithms 8.3.1 and 8.3.3).  Eigenvalues in
+// w (unordered), eigenvectors in the columns of v.
+void sym_eig(int n, std::vector<double> &a, std::vector<double> &w, std::vector<double> &v) {
+  v.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) v[(size_t)i * n + i] = 1.0;
+  w.assign(n, 0.0);
+  if (n == 0) return;
+  auto A = [&](int i, int j) -> double & { return a[(size_t)i * n + j]; };
+  auto Q = [&](int i, int j) -> double & { return v[(size_t)i * n + j]; };
+  std::vector<double> hv(n), pv(n), e(n, 0.0);
+  // reduction: step k annihilates column k below the subdiagonal
+  for (int k = 0; k + 2 < n; k++) {
+    const int m = n - k - 1;  // rows k+1 .. n-1
+    double nrm = 0.0;
+    for (int i = 0; i < m; i++) nrm += A(k + 1 + i, k) * A(k + 1 + i, k);
+    nrm = std::sqrt(nrm);
+    if (nrm == 0.0) continue;
+    const double x0 = A(k + 1, k);
+    const double alpha = x0 >= 0.0 ? -nrm : nrm;
+    for (int i = 0; i < m; i++) hv[i] = A(k + 1 + i, k);
+    hv[0] -= alpha;
+    double hn = 0.0;
+    for (int i = 0; i < m; i++) hn += hv[i] * hv[i];
+    hn = std::sqrt(hn);
+    if (hn == 0.0) continue;
+    for (int i = 0; i < m; i++) hv[i] /= hn;
+    // H = I - 2 h h^T on the trailing block: A22 -= 2 (h q^T + q h^T),
+    // p = A22 h, q = p - (h^T p) h
+    double K = 0.0;
+    for (int i = 0; i < m; i++) {
+      double s = 0.0;
+      for (int j = 0; j < m; j++) s += A(k + 1 + i, k + 1 + j) * hv[j];
+      pv[i] = s;
+      K += hv[i] * s;
+    }
+    for (int i = 0; i < m; i++) pv[i] -= K * hv[i];
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j < m; j++) A(k + 1 + i, k + 1 + j) -= 2.0 * (hv[i] * pv[j] + pv[i] * hv[j]);
+    A(k + 1, k) = A(k, k + 1) = alpha;
+    for (int i = 1; i < m; i++) A(k + 1 + i, k) = A(k, k + 1 + i) = 0.0;
+    // Q <- Q H
+    for (int r = 0; r < n; r++) {
+      double s = 0.0;
+      for (int j = 0; j < m; j++) s += Q(r, k + 1 + j) * hv[j];
+      for (int j = 0; j < m; j++) Q(r, k + 1 + j) -= 2.0 * s * hv[j];
+    }
+  }
+  for (int i = 0; i < n; i++) w[i] = A(i, i);
+  for (int i = 0; i + 1 < n; i++) e[i] = A(i + 1, i);
+  // implicit QR on the unreduced blocks, from the bottom
+  const double eps = 2.220446049250313e-16;
+  int hi = n - 1, guard = 0;
+  while (hi > 0 && guard < 60 * n) {
+    if (std::fabs(e[hi - 1]) <= eps * (std::fabs(w[hi - 1]) + std::fabs(w[hi]))) {
+      e[hi - 1] = 0.0;
+      hi--;
+      continue;
+    }
+    int lo = hi - 1;
+    while (lo > 0 && std::fabs(e[lo - 1]) > eps * (std::fabs(w[lo - 1]) + std::fabs(w[lo]))) lo--;
+    guard++;
+    const double dl = 0.5 * (w[hi - 1] - w[hi]), eh = e[hi - 1];
+    const double mu = w[hi] - eh * eh / (dl + (dl >= 0.0 ? 1.0 : -1.0) * std::hypot(dl, eh));
+    double x = w[lo] - mu, z = e[lo];
+    for (int k = lo; k < hi; k++) {
+      const double r = std::hypot(x, z);
+      const double c = r > 0.0 ? x / r : 1.0, s = r > 0.0 ? z / r : 0.0;
+      if (k > lo) e[k - 1] = r;
+      const double ak = w[k], bk = e[k], dk = w[k + 1];
+      w[k] = c * c * ak + 2.0 * c * s * bk + s * s * dk;
+      w[k + 1] = s * s * ak - 2.0 * c * s * bk + c * c * dk;
+      e[k] = c * s * (dk - ak) + (c * c - s * s) * bk;
+      if (k + 1 < hi) {
+        z = s * e[k + 1];  // the bulge at (k + 2, k)
+        e[k + 1] *= c;
+        x = e[k];
+      }
+      for (int r2 = 0; r2 < n; r2++) {
+        const double qk = Q(r2, k), qk1 = Q(r2, k + 1);
+        Q(r2, k) = c * qk + s * qk1;
+        Q(r2, k + 1) = -s * qk + c * qk1;
+      }
+    }
+  }
+}
+
 // Rayleigh-Ritz of one pair: the k largest eigenpairs of (GA, GB) (order m,
 // leading dimension ld) on the range where GB is numerically positive;
 // coefficients into C (m x k, stride kmax) and values into lam.  Returns the
@@ -726,7 +816,7 @@ void adaptive_constraints(Solver &S) {
         for (int b = 0; b < nc; b++) K[(size_t)cm[a] * m + cm[b]] += Ac[(size_t)a * nc + b];
       cm += nc;
     }
-    jacobi_eig(m, K, w, v);
+    sym_eig(m, K, w, v);
     double wmax = 0.0;
     for (double x : w) wmax = std::max(wmax, x);
     double *out = kp.data() + d.kc0;

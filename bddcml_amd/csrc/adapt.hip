@@ -417,94 +417,61 @@ static void jacobi_eig(int n, std::vector<double> &a, std::vector<double> &w, st
   for (int i = 0; i < n; i++) w[i] = a[(size_t)i * n + i];
 }
 
-// Symmetric eigendecomposition of an n x n matrix (row major, overwritten)
-// for orders where cyclic Jacobi is too slow: Householder reduction to
-// tridiagonal form T = Q^T A Q, then implicit symmetric QR steps with
-// Wilkinson shifts on T, the rotations accumulated into Q (Golub and Van
-// Loan, Matrix Computations, algorithms 8.3.1 and 8.3.3).  Eigenvalues in
-// w (unordered), eigenvectors in the columns of v.
-void sym_eig(int n, std::vector<double> &a, std::vector<double> &w, std::vector<double> &v) {
-  v.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; i++) v[(size_t)i * n + i] = 1.0;
-  w.assign(n, 0.0);
-  if (n == 0) return;
+// Generalized inverse G (K G K = K, symmetric) of a symmetric positive
+// semidefinite n x n matrix K (row major): Cholesky with diagonal pivoting
+// stopped where the largest remaining pivot falls below rel_tol times the
+// largest diagonal entry (numerical rank r), K = P^T [L11; L21][L11; L21]^T P,
+// and G = P^T [L11^-T L11^-1, 0; 0, 0] P.  Then K G K = K, and G = K^-1
+// when K is definite.  n^3/3 flops for the factor, r^3/3 each for the
+// triangular inverse and the product.  Returns r.
+int psd_ginv(int n, const double *K, double rel_tol, double *G) {
+  std::vector<double> a(K, K + (size_t)n * n);
+  std::vector<int> piv(n);
+  for (int i = 0; i < n; i++) piv[i] = i;
   auto A = [&](int i, int j) -> double & { return a[(size_t)i * n + j]; };
-  auto Q = [&](int i, int j) -> double & { return v[(size_t)i * n + j]; };
-  std::vector<double> hv(n), pv(n), e(n, 0.0);
-  // reduction: step k annihilates column k below the subdiagonal
-  for (int k = 0; k + 2 < n; k++) {
-    const int m = n - k - 1;  // rows k+1 .. n-1
-    double nrm = 0.0;
-    for (int i = 0; i < m; i++) nrm += A(k + 1 + i, k) * A(k + 1 + i, k);
-    nrm = std::sqrt(nrm);
-    if (nrm == 0.0) continue;
-    const double x0 = A(k + 1, k);
-    const double alpha = x0 >= 0.0 ? -nrm : nrm;
-    for (int i = 0; i < m; i++) hv[i] = A(k + 1 + i, k);
-    hv[0] -= alpha;
-    double hn = 0.0;
-    for (int i = 0; i < m; i++) hn += hv[i] * hv[i];
-    hn = std::sqrt(hn);
-    if (hn == 0.0) continue;
-    for (int i = 0; i < m; i++) hv[i] /= hn;
-    // H = I - 2 h h^T on the trailing block: A22 -= 2 (h q^T + q h^T),
-    // p = A22 h, q = p - (h^T p) h
-    double K = 0.0;
-    for (int i = 0; i < m; i++) {
-      double s = 0.0;
-      for (int j = 0; j < m; j++) s += A(k + 1 + i, k + 1 + j) * hv[j];
-      pv[i] = s;
-      K += hv[i] * s;
+  double dmax = 0.0;
+  for (int i = 0; i < n; i++) dmax = std::max(dmax, A(i, i));
+  int r = 0;
+  for (int j = 0; j < n; j++) {
+    int q = j;
+    for (int i = j + 1; i < n; i++)
+      if (A(i, i) > A(q, q)) q = i;
+    if (!(A(q, q) > rel_tol * dmax)) break;
+    if (q != j) {  // symmetric swap of rows and columns j, q
+      for (int k = 0; k < n; k++) std::swap(A(j, k), A(q, k));
+      for (int k = 0; k < n; k++) std::swap(A(k, j), A(k, q));
+      std::swap(piv[j], piv[q]);
     }
-    for (int i = 0; i < m; i++) pv[i] -= K * hv[i];
-    for (int i = 0; i < m; i++)
-      for (int j = 0; j < m; j++) A(k + 1 + i, k + 1 + j) -= 2.0 * (hv[i] * pv[j] + pv[i] * hv[j]);
-    A(k + 1, k) = A(k, k + 1) = alpha;
-    for (int i = 1; i < m; i++) A(k + 1 + i, k) = A(k, k + 1 + i) = 0.0;
-    // Q <- Q H
-    for (int r = 0; r < n; r++) {
+    const double d = std::sqrt(A(j, j));
+    A(j, j) = d;
+    for (int i = j + 1; i < n; i++) A(i, j) /= d;
+    for (int i = j + 1; i < n; i++) {
+      const double lij = A(i, j);
+      for (int k = j + 1; k <= i; k++) A(i, k) -= lij * A(k, j);
+    }
+    for (int i = j + 1; i < n; i++)  // keep the trailing block symmetric
+      for (int k = j + 1; k < i; k++) A(k, i) = A(i, k);
+    r = j + 1;
+  }
+  // X = L11^-1 (lower), by columns
+  std::vector<double> X((size_t)r * r, 0.0);
+  for (int c = 0; c < r; c++) {
+    X[(size_t)c * r + c] = 1.0 / A(c, c);
+    for (int i = c + 1; i < r; i++) {
       double s = 0.0;
-      for (int j = 0; j < m; j++) s += Q(r, k + 1 + j) * hv[j];
-      for (int j = 0; j < m; j++) Q(r, k + 1 + j) -= 2.0 * s * hv[j];
+      for (int k = c; k < i; k++) s += A(i, k) * X[(size_t)k * r + c];
+      X[(size_t)i * r + c] = -s / A(i, i);
     }
   }
-  for (int i = 0; i < n; i++) w[i] = A(i, i);
-  for (int i = 0; i + 1 < n; i++) e[i] = A(i + 1, i);
-  // implicit QR on the unreduced blocks, from the bottom
-  const double eps = 2.220446049250313e-16;
-  int hi = n - 1, guard = 0;
-  while (hi > 0 && guard < 60 * n) {
-    if (std::fabs(e[hi - 1]) <= eps * (std::fabs(w[hi - 1]) + std::fabs(w[hi]))) {
-      e[hi - 1] = 0.0;
-      hi--;
-      continue;
+  std::fill(G, G + (size_t)n * n, 0.0);
+  for (int x = 0; x < r; x++)
+    for (int y = 0; y <= x; y++) {
+      double s = 0.0;  // (X^T X)(x, y) = sum_{k >= x} X(k, x) X(k, y)
+      for (int k = x; k < r; k++) s += X[(size_t)k * r + x] * X[(size_t)k * r + y];
+      G[(size_t)piv[x] * n + piv[y]] = s;
+      G[(size_t)piv[y] * n + piv[x]] = s;
     }
-    int lo = hi - 1;
-    while (lo > 0 && std::fabs(e[lo - 1]) > eps * (std::fabs(w[lo - 1]) + std::fabs(w[lo]))) lo--;
-    guard++;
-    const double dl = 0.5 * (w[hi - 1] - w[hi]), eh = e[hi - 1];
-    const double mu = w[hi] - eh * eh / (dl + (dl >= 0.0 ? 1.0 : -1.0) * std::hypot(dl, eh));
-    double x = w[lo] - mu, z = e[lo];
-    for (int k = lo; k < hi; k++) {
-      const double r = std::hypot(x, z);
-      const double c = r > 0.0 ? x / r : 1.0, s = r > 0.0 ? z / r : 0.0;
-      if (k > lo) e[k - 1] = r;
-      const double ak = w[k], bk = e[k], dk = w[k + 1];
-      w[k] = c * c * ak + 2.0 * c * s * bk + s * s * dk;
-      w[k + 1] = s * s * ak - 2.0 * c * s * bk + c * c * dk;
-      e[k] = c * s * (dk - ak) + (c * c - s * s) * bk;
-      if (k + 1 < hi) {
-        z = s * e[k + 1];  // the bulge at (k + 2, k)
-        e[k + 1] *= c;
-        x = e[k];
-      }
-      for (int r2 = 0; r2 < n; r2++) {
-        const double qk = Q(r2, k), qk1 = Q(r2, k + 1);
-        Q(r2, k) = c * qk + s * qk1;
-        Q(r2, k + 1) = -s * qk + c * qk1;
-      }
-    }
-  }
+  return r;
 }
 
 // Rayleigh-Ritz of one pair: the k largest eigenpairs of (GA, GB) (order m,
@@ -679,6 +646,12 @@ hipStream_t device_stream(Solver &S);  // compute stream (device.hip)
 void adaptive_constraints(Solver &S) {
   const char *R = "ADAPTIVITY_SOLVE_EIGENVECTORS";
   const int nsub = S.nsub;
+  double tph = wall_time();
+  auto ph = [&](const char *name) {
+    const double now = wall_time();
+    if (S.verbose >= 2) std::printf("   adaptivity: %-28s %8.4f s\n", name, now - tph);
+    tph = now;
+  };
   hipStream_t st = device_stream(S);
   S.adapt_pairs.clear();
   S.adapt_eigs.clear();
@@ -798,15 +771,19 @@ void adaptive_constraints(Solver &S) {
     pc_lds = std::max(pc_lds, sizeof(double) * (2 * (size_t)pd[p].ncp + 2 * (size_t)(pd[p].nci + pd[p].ncj)));
   }
   if (pc_lds > 65536) fatal(R, "too many constraints on a pair for the pair coarse problem");
-  // K^+ of every pair: the pair's coarse matrix sum_s R_s^T Ac_s R_s
-  // (Ac_s = Phi_s^T S_s Phi_s), inverted on its eigenvalues above 1e-12
-  // times the largest (a floating pair has rigid modes in its null space)
+  // K^+ of every pair: a generalized inverse of the pair's coarse matrix
+  // K = sum_s R_s^T Ac_s R_s (Ac_s = Phi_s^T S_s Phi_s; a floating pair has
+  // rigid modes in its null space), by pivoted Cholesky truncated at pivots
+  // below 1e-12 of the largest diagonal entry.  The residuals LOBPCG
+  // preconditions have no rigid component, so any generalized inverse gives
+  // the same correction up to rigid modes, which lie in the null space of B
+  // and drop out of the Rayleigh-Ritz step.
   std::vector<double> kp((size_t)kcsz, 0.0);
 #pragma omp parallel for schedule(dynamic)
   for (int p = 0; p < np; p++) {
     const PairDesc &d = pd[p];
     const int m = d.ncp;
-    std::vector<double> K((size_t)m * m, 0.0), w, v;
+    std::vector<double> K((size_t)m * m, 0.0);
     const int sub2[2] = {hp[p].i, hp[p].j}, nc2[2] = {d.nci, d.ncj};
     const int *cm = cmap.data() + d.cm0;
     for (int sd = 0; sd < 2; sd++) {
@@ -816,20 +793,11 @@ void adaptive_constraints(Solver &S) {
         for (int b = 0; b < nc; b++) K[(size_t)cm[a] * m + cm[b]] += Ac[(size_t)a * nc + b];
       cm += nc;
     }
-    sym_eig(m, K, w, v);
-    double wmax = 0.0;
-    for (double x : w) wmax = std::max(wmax, x);
-    double *out = kp.data() + d.kc0;
-    for (int e = 0; e < m; e++) {
-      if (!(w[e] > 1e-12 * wmax)) continue;
-      const double inv = 1.0 / w[e];
-      for (int a = 0; a < m; a++) {
-        const double va = v[(size_t)a * m + e] * inv;
-        for (int b = 0; b < m; b++) out[(size_t)a * m + b] += va * v[(size_t)b * m + e];
-      }
-    }
+    psd_ginv(m, K.data(), 1e-12, kp.data() + d.kc0);
   }
+  ph("pair coarse inverses");
   if (2 * kmax > kAdMaxCols) fatal(R, "too many eigenvectors per pair");
+  ph("pairs");
   std::vector<int> jobs;
   for (int p = 0; p < np; p++)
     for (int side = 0; side < 2; side++)
@@ -895,6 +863,7 @@ void adaptive_constraints(Solver &S) {
       hlam((size_t)np * kmax, 0.0), hnrm((size_t)np * kmax);
   std::vector<int> iters(np, 0);
   // Rayleigh-Ritz of every active pair on nb blocks, then the update
+  double t_rr = 0.0;  // host part of the Rayleigh-Ritz steps
   auto rr_update = [&](int nb) {
     Blocks in{{blk[X].p, blk[W].p, blk[P].p}, {blk[AX].p, blk[AW].p, blk[AP].p}, {blk[BX].p, blk[BW].p, blk[BP].p}};
     hipLaunchKernelGGL(k_ad_gram, dim3(std::max(np, 1)), dim3(256), 0, st, dpairs.p, in, nb, kmax,
@@ -902,6 +871,7 @@ void adaptive_constraints(Solver &S) {
     AD_CHECK(hipMemcpyAsync(hGA.data(), dGA.p, hGA.size() * 8, hipMemcpyDeviceToHost, st));
     AD_CHECK(hipMemcpyAsync(hGB.data(), dGB.p, hGB.size() * 8, hipMemcpyDeviceToHost, st));
     AD_CHECK(hipStreamSynchronize(st));
+    const double t0 = wall_time();
 #pragma omp parallel for schedule(dynamic)
     for (int p = 0; p < np; p++) {
       if (!pd[p].active) continue;
@@ -910,6 +880,7 @@ void adaptive_constraints(Solver &S) {
       rayleigh_ritz(m, pd[p].k, ld, hGA.data() + (size_t)p * ld * ld, hGB.data() + (size_t)p * ld * ld,
                     kmax, hC.data() + (size_t)p * ld * kmax, hlam.data() + (size_t)p * kmax);
     }
+    t_rr += wall_time() - t0;
     AD_CHECK(hipMemcpyAsync(dC.p, hC.data(), hC.size() * 8, hipMemcpyHostToDevice, st));
     AD_CHECK(hipMemcpyAsync(dlam.p, hlam.data(), hlam.size() * 8, hipMemcpyHostToDevice, st));
     Blocks out{{blk[XN].p, nullptr, blk[PN].p}, {blk[AXN].p, nullptr, blk[APN].p}, {blk[BXN].p, nullptr, blk[BPN].p}};
@@ -982,12 +953,45 @@ void adaptive_constraints(Solver &S) {
     AD_CHECK(hipMemcpyAsync(hAX.data(), blk[AX].p, hAX.size() * 8, hipMemcpyDeviceToHost, st));
     AD_CHECK(hipStreamSynchronize(st));
   }
-  // ---- constraints per face (global dof -> value rows), largest first
-  std::map<int, std::vector<std::map<int, double>>> face_rows;  // face -> rows over global dofs
+  if (S.verbose >= 2) std::printf("   adaptivity: host Rayleigh-Ritz %.4f s of the LOBPCG\n", t_rr);
+  ph("LOBPCG");
+  // ---- every face: the support of its arithmetic rows (global dofs,
+  // sorted) and those rows over it
+  std::vector<int> faces;
+  for (auto &f : face_subs) faces.push_back(f.first);
+  const int nfaces = (int)faces.size();
+  std::map<int, int> face_idx;
+  for (int q = 0; q < nfaces; q++) face_idx[faces[q]] = q;
+  std::vector<std::vector<int>> face_dofs(nfaces);
+  std::vector<std::vector<std::vector<double>>> face_M(nfaces);
+#pragma omp parallel for schedule(dynamic)
+  for (int q = 0; q < nfaces; q++) {
+    const int face = faces[q];
+    const Sub &s = S.subs[face_subs.at(face)[0]];
+    std::vector<int> &gd = face_dofs[q];
+    for (int c = 0; c < s.nconstr; c++) {
+      if (node_of(s.indrowc[c]) != face) continue;
+      for (int e = s.cptr[c]; e < s.cptr[c + 1]; e++) gd.push_back(s.isvgvn[s.ifc_dof[s.cidx[e]]]);
+    }
+    std::sort(gd.begin(), gd.end());
+    gd.erase(std::unique(gd.begin(), gd.end()), gd.end());
+    for (int c = 0; c < s.nconstr; c++) {
+      if (node_of(s.indrowc[c]) != face) continue;
+      std::vector<double> v(gd.size(), 0.0);
+      for (int e = s.cptr[c]; e < s.cptr[c + 1]; e++) {
+        const int g = s.isvgvn[s.ifc_dof[s.cidx[e]]];
+        v[std::lower_bound(gd.begin(), gd.end(), g) - gd.begin()] = s.cval[e];
+      }
+      face_M[q].push_back(std::move(v));
+    }
+  }
+  // ---- the adaptive rows of every pair after its face's arithmetic rows,
+  // largest eigenvalue first: c = -P (I - E^T) S (I - E) P v (the
+  // reference's operator) on side i, restricted to the face dofs
   S.adapt_est = -1.0;
+  std::vector<int> nadv(np, 0);
   for (int p = 0; p < np; p++) {
     const PairDesc &d = pd[p];
-    const Sub &si = S.subs[hp[p].i];
     const double *lam = hlam.data() + (size_t)p * kmax;
     int nad = 0;
     while (nad < d.k && lam[nad] >= S.adapt_threshold) nad++;
@@ -995,53 +999,44 @@ void adaptive_constraints(Solver &S) {
     S.adapt_est = std::max(S.adapt_est, est);
     S.adapt_pairs.push_back({hp[p].face, hp[p].i, hp[p].j, d.k, nad, iters[p]});
     S.adapt_eigs.push_back(std::vector<double>(lam, lam + d.k));
-    auto &rows = face_rows[hp[p].face];
+    nadv[p] = nad;
+  }
+#pragma omp parallel for schedule(dynamic)
+  for (int p = 0; p < np; p++) {
+    const PairDesc &d = pd[p];
+    const int nad = nadv[p];
+    if (nad == 0) continue;
+    const int i = hp[p].i, q = face_idx.at(hp[p].face);
+    const Sub &si = S.subs[i];
+    // interface position of every face dof on side i
+    std::vector<std::pair<int, int>> g2r(si.ni);
+    for (int r = 0; r < si.ni; r++) g2r[r] = {si.isvgvn[si.ifc_dof[r]], r};
+    std::sort(g2r.begin(), g2r.end());
+    const std::vector<int> &gd = face_dofs[q];
+    std::vector<int> dev(gd.size());
+    for (size_t k = 0; k < gd.size(); k++) {
+      auto it = std::lower_bound(g2r.begin(), g2r.end(), std::make_pair(gd[k], -1));
+      dev[k] = (it != g2r.end() && it->first == gd[k]) ? ref2dev[i][it->second] : -1;
+    }
     for (int c = 0; c < nad; c++) {
-      // c = -P (I - E^T) S (I - E) P v (the reference's operator), side i
-      std::map<int, double> row;
       const double *a = hAX.data() + d.off + (int64_t)c * d.n;
-      for (int r = 0; r < si.ni; r++) row[si.isvgvn[si.ifc_dof[r]]] = -a[ref2dev[hp[p].i][r]];
-      rows.push_back(std::move(row));
-    }
-  }
-  // ---- every face block: its arithmetic rows and its adaptive rows
-  // restricted to the face, orthonormalized once (both subdomains get the
-  // same rows); coarse dofs renumbered with the new counts
-  std::map<int, std::vector<std::vector<double>>> face_q;  // face -> rows over its dofs
-  std::map<int, std::vector<int>> face_dofs;                // face -> global dofs, sorted
-  for (auto &f : face_subs) {
-    const int face = f.first;
-    const Sub &s = S.subs[f.second[0]];
-    std::vector<std::vector<double>> M;
-    std::vector<int> gd;
-    std::vector<std::pair<int, std::map<int, double>>> arith;
-    for (int c = 0; c < s.nconstr; c++) {
-      if (node_of(s.indrowc[c]) != face) continue;
-      std::map<int, double> row;
-      for (int q = s.cptr[c]; q < s.cptr[c + 1]; q++) {
-        const int g = s.isvgvn[s.ifc_dof[s.cidx[q]]];
-        row[g] = s.cval[q];
-        gd.push_back(g);
-      }
-      arith.push_back({c, row});
-    }
-    std::sort(gd.begin(), gd.end());
-    gd.erase(std::unique(gd.begin(), gd.end()), gd.end());
-    auto dense = [&](const std::map<int, double> &row) {
       std::vector<double> v(gd.size(), 0.0);
-      for (size_t k = 0; k < gd.size(); k++) {
-        auto it = row.find(gd[k]);
-        if (it != row.end()) v[k] = it->second;
-      }
-      return v;
-    };
-    for (auto &a : arith) M.push_back(dense(a.second));
-    auto fr = face_rows.find(face);
-    if (fr != face_rows.end())
-      for (auto &row : fr->second) M.push_back(dense(row));
-    face_q[face] = orthonormal_rows(M, (int)gd.size());
-    face_dofs[face] = gd;
+      for (size_t k = 0; k < gd.size(); k++) v[k] = dev[k] >= 0 ? -a[dev[k]] : 0.0;
+      face_M[q].push_back(std::move(v));
+    }
   }
+  // ---- every face block orthonormalized once (both subdomains get the
+  // same rows); coarse dofs renumbered with the new counts
+  std::vector<std::vector<std::vector<double>>> face_qv(nfaces);
+#pragma omp parallel for schedule(dynamic)
+  for (int q = 0; q < nfaces; q++) face_qv[q] = orthonormal_rows(face_M[q], (int)face_dofs[q].size());
+  std::map<int, std::vector<std::vector<double>>> face_q;
+  std::map<int, std::vector<int>> face_dof_map;
+  for (int q = 0; q < nfaces; q++) {
+    face_q[faces[q]] = std::move(face_qv[q]);
+    face_dof_map[faces[q]] = face_dofs[q];
+  }
+  ph("face rows and orthonormalization");
   std::vector<int> nndfc(S.nndfc);
   for (auto &f : face_q) nndfc[f.first] = (int)f.second.size();
   std::vector<int> kdofc(nndfc.size() + 1, 0);
@@ -1056,7 +1051,7 @@ void adaptive_constraints(Solver &S) {
     for (int id : s.cnodes) {
       if (id >= first_face && face_q.count(id)) {
         const auto &Q = face_q.at(id);
-        const auto &gd = face_dofs.at(id);
+        const auto &gd = face_dof_map.at(id);
         for (size_t r = 0; r < Q.size(); r++) {
           for (size_t k = 0; k < gd.size(); k++)
             if (Q[r][k] != 0.0) cidx.push_back(g2r.at(gd[k])), cval.push_back(Q[r][k]);
@@ -1081,6 +1076,7 @@ void adaptive_constraints(Solver &S) {
   S.nndfc = nndfc;
   S.kdofc = kdofc;
   S.ndofc = kdofc.back();
+  ph("new constraint rows");
   if (S.verbose > 0) {
     int nad = 0;
     for (auto &q : S.adapt_pairs) nad += q[4];

@@ -22,9 +22,13 @@
 // the largest of (A, B) are sought directly (the same eigenpairs).
 //
 // Each LOBPCG step runs batched over all pairs on the device: the block
-// operator applies (projection, jump, one symmetric tile product with S for
-// the A and B inputs together), the residuals and their norms, the Gram
-// matrices of [X W P] with A and B, and the block updates.  The Rayleigh-Ritz
+// operator applies (projection, jump, the products with S for the A and B
+// inputs together), the residuals and their norms, the Gram matrices of
+// [X W P] with A and B, and the block updates.  The products with S_s (and
+// with S'_s^{-1} in the preconditioner) are gathered per subdomain: one
+// tile task list multiplies each subdomain's matrix by the columns of all
+// pairs it belongs to on the f64 matrix cores, so each S_s is read once per
+// 64 columns rather than once per pair.  The Rayleigh-Ritz
 // step of each pair (a generalized eigenproblem of order at most 3k = 30)
 // runs on the host between them, by two Jacobi eigendecompositions: the
 // directions where the B-Gram matrix is numerically singular are dropped, so
@@ -64,8 +68,6 @@ namespace bddc {
   } while (0)
 
 struct PairDesc {
-  int ntii, ntij;    // tile counts of S_i, S_j
-  int sloi, sloj;    // first tile of S_i, S_j in the subdomain pool
   int NIi, n;        // padded interface size of i; n = NIi + NIj
   int k;             // columns of the block (eigenvectors sought)
   int active;        // still iterating
@@ -73,7 +75,6 @@ struct PairDesc {
   int64_t off, woff; // block offsets: n x kmax and n x 2 kmax per pair
   int q0, q1;        // projection columns (qptr range)
   int m0, m1;        // common dofs (range in the common-dof arrays)
-  int pbi, pbj;      // first tile of S'^{-1} of i, j (factor pool)
   int nci, ncj, ncp; // constraints of i, j; coarse dofs of the pair
   int cm0;           // pair coarse dof of each constraint (cmap range, i then j)
   int64_t zoi, zoj;  // Z / Phi offsets of i, j (NI x nc column major)
@@ -136,67 +137,6 @@ __global__ void k_ad_copy(const PairDesc *__restrict__ pairs, double *src, int s
   const double *x = pcol(src, p, src_wide, s0 + col);
   double *y = pcol(dst, p, dst_wide, d0 + col);
   for (int i = threadIdx.x; i < p.n; i += blockDim.x) y[i] = x[i];
-}
-
-// out = M in on the wide columns, M = S (inv 0, the S pool) or S'^{-1}
-// (inv 1, the factor pool, FP64 or FP32), both packed lower tiles by
-// columns; nsets 2: the k A inputs and the k B inputs, 1: the first k
-// columns only.  One workgroup per (pair, side, tile row I): out_I = sum_J
-// M_IJ in_J, M_IJ = stored lower tile (J <= I) or the transpose of tile
-// (J, I), staged in LDS
-constexpr int kAdMaxCols = 32;
-template <typename TP>
-__global__ __launch_bounds__(256) void k_ad_schur(const PairDesc *__restrict__ pairs,
-                                                  const int *__restrict__ jobs,
-                                                  const TP *__restrict__ pool, int inv,
-                                                  const double *__restrict__ in,
-                                                  double *__restrict__ out, int kmax, int nsets) {
-  __shared__ double tile[64][65];
-  __shared__ double xs[64][kAdMaxCols + 1];
-  const int job = blockIdx.x;
-  const PairDesc p = pairs[jobs[3 * job]];
-  if (!p.active) return;
-  const int side = jobs[3 * job + 1], I = jobs[3 * job + 2];
-  const int nt = side ? p.ntij : p.ntii;
-  const int slow = inv ? (side ? p.pbj : p.pbi) : (side ? p.sloj : p.sloi);
-  const int so = side ? p.NIi : 0;
-  const int ncol = nsets * p.k;
-  const int r = threadIdx.x & 63, cg = threadIdx.x >> 6;
-  double acc[kAdMaxCols / 4];
-#pragma unroll
-  for (int q = 0; q < kAdMaxCols / 4; q++) acc[q] = 0.0;
-  const double *xin = in + p.woff + so;
-  for (int J = 0; J < nt; J++) {
-    const bool low = J <= I;
-    const int I0 = low ? I : J, J0 = low ? J : I;
-    const TP *Tp = pool + (size_t)(slow + J0 * nt - J0 * (J0 - 1) / 2 + (I0 - J0)) * TILE_ELEMS;
-    for (int e = threadIdx.x; e < TILE_ELEMS; e += 256) {
-      const int er = e & 63, ec = e >> 6;  // element (er, ec) of the stored tile
-      if (low) tile[er][ec] = (double)Tp[e];
-      else tile[ec][er] = (double)Tp[e];
-    }
-    for (int e = threadIdx.x; e < 64 * ncol; e += 256) {
-      const int c = e & 63, col = e >> 6;
-      const int wc = col < p.k ? col : kmax + col - p.k;  // A inputs, then B inputs
-      xs[c][col] = xin[(int64_t)wc * p.n + J * 64 + c];
-    }
-    __syncthreads();
-    for (int c = 0; c < 64; c++) {
-      const double s = tile[r][c];
-#pragma unroll
-      for (int q = 0; q < kAdMaxCols / 4; q++) {
-        const int col = cg + 4 * q;
-        if (col < ncol) acc[q] += s * xs[c][col];
-      }
-    }
-    __syncthreads();
-  }
-  double *yo = out + p.woff + so;
-#pragma unroll
-  for (int q = 0; q < kAdMaxCols / 4; q++) {
-    const int col = cg + 4 * q;
-    if (col < ncol) yo[(int64_t)(col < p.k ? col : kmax + col - p.k) * p.n + I * 64 + r] = acc[q];
-  }
 }
 
 // R = AX - BX diag(lam) into W and its squared column norms; a column
@@ -694,7 +634,6 @@ void adaptive_constraints(Solver &S) {
     const int i = f.second[0], j = f.second[1];
     const Sub &si = S.subs[i], &sj = S.subs[j];
     PairDesc d{};
-    d.ntii = ntis[i], d.ntij = ntis[j], d.sloi = slow[i], d.sloj = slow[j];
     d.NIi = si.NI;
     d.n = si.NI + sj.NI;
     // common dofs by global dof
@@ -743,7 +682,6 @@ void adaptive_constraints(Solver &S) {
     for (double x : rho[j]) trA += x;
     d.tol = S.adapt_tol * trA / (si.ni + sj.ni);
     // preconditioner data: S'^{-1} tiles, Z / Phi, pair coarse dofs
-    d.pbi = pcs.pbase[i], d.pbj = pcs.pbase[j];
     d.zoi = pcs.zoff[i], d.zoj = pcs.zoff[j];
     d.nci = si.nconstr, d.ncj = sj.nconstr;
     std::vector<int> cd(si.indrowc.begin(), si.indrowc.begin() + si.nconstr);
@@ -796,20 +734,56 @@ void adaptive_constraints(Solver &S) {
     psd_ginv(m, K.data(), 1e-12, kp.data() + d.kc0);
   }
   ph("pair coarse inverses");
-  if (2 * kmax > kAdMaxCols) fatal(R, "too many eigenvectors per pair");
   ph("pairs");
-  std::vector<int> jobs;
+  // ---- the products with S_s (operator applies: the k A inputs and the k
+  // B inputs of every pair side) and with S'_s^{-1} (preconditioner: k
+  // columns) gathered per subdomain: each subdomain's matrix multiplies all
+  // of its pairs' columns in one tile task list (subdomain, tile row, 64
+  // columns), the columns addressed in place in the wide pair buffers
+  std::vector<std::vector<int64_t>> capply(nsub), cpc(nsub);
   for (int p = 0; p < np; p++)
-    for (int side = 0; side < 2; side++)
-      for (int I = 0; I < (side ? pd[p].ntij : pd[p].ntii); I++) jobs.insert(jobs.end(), {p, side, I});
+    for (int side = 0; side < 2; side++) {
+      const PairDesc &d = pd[p];
+      const int is = side ? hp[p].j : hp[p].i;
+      const int64_t so = d.woff + (side ? d.NIi : 0);
+      for (int c = 0; c < d.k; c++) {
+        capply[is].push_back(so + (int64_t)c * d.n);
+        cpc[is].push_back(so + (int64_t)c * d.n);
+      }
+      for (int c = 0; c < d.k; c++) capply[is].push_back(so + (int64_t)(kmax + c) * d.n);
+    }
+  std::vector<int64_t> colA, colP;
+  std::vector<SymmSub> subA(nsub), subP(nsub);
+  std::vector<SymmTask> taskA, taskP;
+  for (int is = 0; is < nsub; is++) {
+    for (int pass = 0; pass < 2; pass++) {
+      std::vector<int64_t> &c = pass ? cpc[is] : capply[is];
+      std::vector<int64_t> &all = pass ? colP : colA;
+      std::vector<SymmTask> &tasks = pass ? taskP : taskA;
+      SymmSub &sd = pass ? subP[is] : subA[is];
+      c.resize((c.size() + 63) / 64 * 64, -1);
+      sd.tile0 = pass ? pcs.pbase[is] : slow[is];
+      sd.nt = ntis[is];
+      sd.col0 = (int)all.size();
+      sd.ncol = (int)c.size();
+      all.insert(all.end(), c.begin(), c.end());
+      for (int I = 0; I < sd.nt; I++)
+        for (int c0 = 0; c0 < sd.ncol; c0 += 64) tasks.push_back({is, I, c0, 0});
+    }
+  }
   // ---- device buffers
   ABuf<PairDesc> dpairs;
-  ABuf<int> dqptr, dqpos, dma, dmb, djobs, dcmap;
+  ABuf<int> dqptr, dqpos, dma, dmb, dcmap;
   ABuf<double> dqval, dmwa, dmwb, dkp;
+  ABuf<int64_t> dcolA, dcolP;
+  ABuf<SymmSub> dsubA, dsubP;
+  ABuf<SymmTask> dtaskA, dtaskP;
   dpairs.upload(pd, st);
   dqptr.upload(qptr, st), dqpos.upload(qpos, st), dqval.upload(qval, st);
   dma.upload(ma, st), dmb.upload(mb, st), dmwa.upload(mwa, st), dmwb.upload(mwb, st);
-  djobs.upload(jobs, st);
+  dcolA.upload(colA, st), dcolP.upload(colP, st);
+  dsubA.upload(subA, st), dsubP.upload(subP, st);
+  dtaskA.upload(taskA, st), dtaskP.upload(taskP, st);
   dcmap.upload(cmap, st), dkp.upload(kp, st);
   enum { X, AX, BX, W, AW, BW, P, AP, BP, XN, AXN, BXN, PN, APN, BPN, NBLK };
   std::vector<ABuf<double>> blk(NBLK);
@@ -817,6 +791,9 @@ void adaptive_constraints(Solver &S) {
   ABuf<double> win, wout, dlam, dnrm, dGA, dGB, dC;
   win.alloc(std::max<int64_t>(woff, 1));
   wout.alloc(std::max<int64_t>(woff, 1));
+  // the tile products read every pair's columns, also those of pairs that
+  // have stopped iterating: start from zeros
+  AD_CHECK(hipMemsetAsync(win.p, 0, std::max<int64_t>(woff, 1) * sizeof(double), st));
   dlam.alloc((size_t)np * kmax);
   dnrm.alloc((size_t)np * kmax);
   const int ld = 3 * kmax;
@@ -832,8 +809,7 @@ void adaptive_constraints(Solver &S) {
                        win.p, 1, kmax);
     hipLaunchKernelGGL(k_ad_jump, pk, dim3(256), 0, st, dpairs.p, dma.p, dmb.p, dmwa.p, dmwb.p,
                        win.p, 1, kmax, win.p, 1, 0, 0);
-    hipLaunchKernelGGL(k_ad_schur<double>, dim3((unsigned)jobs.size() / 3), dim3(256), 0, st,
-                       dpairs.p, djobs.p, pool, 0, win.p, wout.p, kmax, 2);
+    launch_symm_cols<double>(st, pool, dsubA.p, dtaskA.p, (int)taskA.size(), dcolA.p, win.p, wout.p);
     hipLaunchKernelGGL(k_ad_jump, pk, dim3(256), 0, st, dpairs.p, dma.p, dmb.p, dmwa.p, dmwb.p,
                        wout.p, 1, 0, blk[ab].p, 0, 0, 1);
     hipLaunchKernelGGL(k_ad_project, pk, dim3(64), 0, st, dpairs.p, dqptr.p, dqpos.p, dqval.p,
@@ -846,13 +822,12 @@ void adaptive_constraints(Solver &S) {
   auto precondition = [&]() {
     if (np == 0) return;
     hipLaunchKernelGGL(k_ad_copy, pk, dim3(256), 0, st, dpairs.p, blk[W].p, 0, 0, win.p, 1, 0);
-    const dim3 gj((unsigned)jobs.size() / 3);
     if (pcs.inv_float)
-      hipLaunchKernelGGL(k_ad_schur<float>, gj, dim3(256), 0, st, dpairs.p, djobs.p,
-                         (const float *)pcs.inv, 1, win.p, wout.p, kmax, 1);
+      launch_symm_cols<float>(st, (const float *)pcs.inv, dsubP.p, dtaskP.p, (int)taskP.size(), dcolP.p,
+                              win.p, wout.p);
     else
-      hipLaunchKernelGGL(k_ad_schur<double>, gj, dim3(256), 0, st, dpairs.p, djobs.p,
-                         (const double *)pcs.inv, 1, win.p, wout.p, kmax, 1);
+      launch_symm_cols<double>(st, (const double *)pcs.inv, dsubP.p, dtaskP.p, (int)taskP.size(),
+                               dcolP.p, win.p, wout.p);
     hipLaunchKernelGGL(k_ad_coarse_pc, pk, dim3(256), pc_lds, st, dpairs.p, pcs.Z, pcs.Phi, dcmap.p,
                        dkp.p, win.p, wout.p);
     hipLaunchKernelGGL(k_ad_copy, pk, dim3(256), 0, st, dpairs.p, wout.p, 1, 0, blk[W].p, 0, 0);

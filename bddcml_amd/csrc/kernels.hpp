@@ -22,6 +22,18 @@ enum GemmPost { POST_NONE = 0, POST_RIGHT_XT = 1, POST_LEFT_X = 2, POST_LEFT_XT 
 template <typename T>
 void launch_tile_gemm(hipStream_t st, T *pool, const TileTask *tasks, const TilePair *pairs,
                       int ntasks, int kind, int post, double alpha, double beta);
+// a symmetric matrix per subdomain (packed lower tiles from tile0, nt tile
+// rows) times a set of columns scattered in memory (col0: first entry of the
+// subdomain's column offsets)
+struct SymmSub {
+  int tile0, nt, col0, ncol;
+};
+struct SymmTask {
+  int sub, I, c0, pad;
+};
+template <typename TP>
+void launch_symm_cols(hipStream_t st, const TP *pool, const SymmSub *subs, const SymmTask *tasks,
+                      int ntasks, const int64_t *cols, const double *in, double *out);
 // C = C - sum A B^T as its own kernel symbol (Schur-complement block update)
 // 1: tile-task launches hand each XCD one contiguous range of their task
 // list (default); 0: plain blockIdx order

@@ -605,4 +605,75 @@ void launch_scatter_ones(hipStream_t st, double *pool, const int64_t *idx, int64
   hipLaunchKernelGGL(k_scatter_ones, dim3((unsigned)nb), dim3(256), 0, st, pool, idx, n);
 }
 
+// ------------------------------------------- symmetric tiles x column sets
+// out(:, cols) = M_s in(:, cols) for every subdomain s of a batch, M_s a
+// symmetric matrix in packed lower tiles (tile (I, J), I >= J, at tile0 +
+// J*nt - J(J-1)/2 + I - J; element type TP, FP32 staged as FP64) and the
+// columns scattered in the in / out buffers: column c of subdomain s starts
+// at cols[col0 + c] (-1: a padding column, read as zero and not written).
+// One workgroup per (subdomain, tile row I, 64 columns): out_I = sum_J M_IJ
+// in_J on the f64 matrix cores, the M tile and the 64 x 64 input block
+// staged in LDS (the upper tiles as transposes of the stored lower ones).
+template <typename TP>
+__device__ __forceinline__ void tile_to_lds_f64(const TP *__restrict__ t, double *lds, bool trans) {
+  for (int e = threadIdx.x; e < TILE_ELEMS; e += NTHR) {
+    const int row = e & 63, col = e >> 6;
+    const double x = (double)t[e];
+    if (!trans) lds[col * LDL + row] = x;
+    else lds[row * LDL + col] = x;
+  }
+}
+template <typename TP>
+__global__ __launch_bounds__(NTHR) void k_symm_cols(const TP *__restrict__ pool,
+                                                    const SymmSub *__restrict__ subs,
+                                                    const SymmTask *__restrict__ tasks,
+                                                    const int64_t *__restrict__ cols,
+                                                    const double *__restrict__ in,
+                                                    double *__restrict__ out) {
+  __shared__ double As[64 * LDL];
+  __shared__ double Bs[64 * LDL];
+  __shared__ int64_t cl[64];
+  const SymmTask t = tasks[blockIdx.x];
+  const SymmSub d = subs[t.sub];
+  if (threadIdx.x < 64) cl[threadIdx.x] = cols[d.col0 + t.c0 + threadIdx.x];
+  typename Mfma<double>::v4 acc[2][2];
+  zero_acc<double>(acc);
+  for (int J = 0; J < d.nt; J++) {
+    const bool low = J <= t.I;
+    const int I0 = low ? t.I : J, J0 = low ? J : t.I;
+    const TP *T = pool + (size_t)(d.tile0 + J0 * d.nt - J0 * (J0 - 1) / 2 + (I0 - J0)) * TILE_ELEMS;
+    __syncthreads();
+    // As[q][r] = M(I, J)(r, q): the stored tile itself (I >= J) or the
+    // transpose of tile (J, I)
+    tile_to_lds_f64(T, As, !low);
+    // Bs[q][c] = in(J*64 + q, column c)
+    for (int e = threadIdx.x; e < TILE_ELEMS; e += NTHR) {
+      const int q = e & 63, c = e >> 6;
+      const int64_t o = cl[c];
+      Bs[q * LDL + c] = o >= 0 ? in[o + J * 64 + q] : 0.0;
+    }
+    __syncthreads();
+    mfma_accumulate<double>(As, Bs, acc);
+  }
+  __syncthreads();
+  acc_to_lds<double>(acc, As);  // As[c*LDL + r] = out(I*64 + r, column c)
+  __syncthreads();
+  for (int e = threadIdx.x; e < TILE_ELEMS; e += NTHR) {
+    const int r = e & 63, c = e >> 6;
+    const int64_t o = cl[c];
+    if (o >= 0) out[o + t.I * 64 + r] = As[c * LDL + r];
+  }
+}
+template <typename TP>
+void launch_symm_cols(hipStream_t st, const TP *pool, const SymmSub *subs, const SymmTask *tasks,
+                      int ntasks, const int64_t *cols, const double *in, double *out) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_symm_cols<TP>, dim3(ntasks), dim3(NTHR), 0, st, pool, subs, tasks, cols, in,
+                     out);
+}
+template void launch_symm_cols<double>(hipStream_t, const double *, const SymmSub *, const SymmTask *,
+                                       int, const int64_t *, const double *, double *);
+template void launch_symm_cols<float>(hipStream_t, const float *, const SymmSub *, const SymmTask *,
+                                      int, const int64_t *, const double *, double *);
+
 }  // namespace bddc

@@ -156,7 +156,7 @@ def adaptive_workload(args):
         b.init(2, [p.nsub, 1], numbase=p.numbase,
                verbose_level=int(os.environ.get("BENCH_VERBOSE", "0")))
         for k, v in (("adaptive_threshold", 10.0), ("adaptive_tol", 1e-6), ("adaptive_maxit", 15),
-                     ("adaptive_nvec", 10)):
+                     ("adaptive_nvec", 10), ("explicit_inverse", args.explicit_inverse)):
             b.set_option(k, v)
         for d in subs:
             b.upload_subdomain_data(d)

@@ -35,7 +35,7 @@ EXTRA_SYMBOLS = [
     "bddcml_amd_residual_history", "bddcml_amd_timings", "bddcml_amd_set_option",
     "bddcml_amd_sub_info", "bddcml_amd_sub_maps", "bddcml_amd_sub_matrix",
     "bddcml_amd_sub_coarse", "bddcml_amd_last_error", "bddcml_amd_selftest_tile",
-    "bddcml_amd_selftest_ginv",
+    "bddcml_amd_selftest_ginv", "bddcml_amd_selftest_symeig",
     "bddcml_amd_selftest_copy", "bddcml_amd_element_gradient", "bddcml_amd_recycling_basis",
     "bddcml_amd_level2_counts", "bddcml_amd_level_counts", "bddcml_amd_element_gradient_noalloc",
     "bddcml_amd_adaptive_pairs",
@@ -96,6 +96,20 @@ def selftest_tile(op, A, B, C_, X=None, alpha=1.0, beta=1.0):
                                Cm.ctypes.data_as(dp), Xm.ctypes.data_as(dp), alpha, beta,
                                out.ctypes.data_as(dp))
     return out[:4096].reshape(64, 64).T.copy(), out[4096:].reshape(64, 64).T.copy()
+
+
+def selftest_symeig(a):
+    """Host symmetric eigendecomposition of the library (see
+    include/bddcml_amd.h): eigenvalues (unordered) and eigenvectors as
+    columns."""
+    L = _Checked(lib())
+    L.bddcml_amd_set_option(b"throw_on_error", 1.0)
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    n = a.shape[0]
+    w, v = np.zeros(n), np.zeros((n, n))
+    dp = C.POINTER(C.c_double)
+    L.bddcml_amd_selftest_symeig(n, a.ctypes.data_as(dp), w.ctypes.data_as(dp), v.ctypes.data_as(dp))
+    return w, v
 
 
 def selftest_ginv(k, rel_tol=1e-12):

@@ -30,7 +30,7 @@
 // pairs it belongs to on the f64 matrix cores, so each S_s is read once per
 // 64 columns rather than once per pair.  The Rayleigh-Ritz
 // step of each pair (a generalized eigenproblem of order at most 3k = 30)
-// runs on the host between them, by two Jacobi eigendecompositions: the
+// runs on the host between them, by two symmetric eigendecompositions: the
 // directions where the B-Gram matrix is numerically singular are dropped, so
 // converged columns and the floating null space of B need no special case.
 // The residuals are preconditioned, as in the reference
@@ -313,48 +313,94 @@ __global__ __launch_bounds__(256) void k_ad_combine(const PairDesc *__restrict__
 }
 
 // ------------------------------------------------------- host eigen-solves
-// Cyclic Jacobi eigendecomposition of a symmetric n x n matrix (row major,
-// overwritten): eigenvalues in w, eigenvectors in the columns of v
-static void jacobi_eig(int n, std::vector<double> &a, std::vector<double> &w, std::vector<double> &v) {
+// Symmetric eigendecomposition of an n x n matrix (row major, overwritten)
+// (the Rayleigh-Ritz problems of LOBPCG): Householder reduction to
+// tridiagonal form T = Q^T A Q, then implicit symmetric QR steps with
+// Wilkinson shifts on T, the rotations accumulated into Q (Golub and Van
+// Loan, Matrix Computations, algorithms 8.3.1 and 8.3.3).  Eigenvalues in
+// w (unordered), eigenvectors in the columns of v.
+void sym_eig(int n, std::vector<double> &a, std::vector<double> &w, std::vector<double> &v) {
   v.assign((size_t)n * n, 0.0);
   for (int i = 0; i < n; i++) v[(size_t)i * n + i] = 1.0;
-  for (int sweep = 0; sweep < 100; sweep++) {
-    double off = 0.0, tot = 0.0;
-    for (int i = 0; i < n; i++)
-      for (int j = 0; j < n; j++) {
-        const double x = a[(size_t)i * n + j] * a[(size_t)i * n + j];
-        tot += x;
-        if (i != j) off += x;
-      }
-    if (off <= 1e-30 * tot || off == 0.0) break;
-    for (int pq = 0; pq < n; pq++)
-      for (int q = pq + 1; q < n; q++) {
-        const int pp = pq;
-        const double apq = a[(size_t)pp * n + q];
-        if (apq == 0.0) continue;
-        const double app = a[(size_t)pp * n + pp], aqq = a[(size_t)q * n + q];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < n; k++) {
-          const double akp = a[(size_t)k * n + pp], akq = a[(size_t)k * n + q];
-          a[(size_t)k * n + pp] = c * akp - s * akq;
-          a[(size_t)k * n + q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < n; k++) {
-          const double apk = a[(size_t)pp * n + k], aqk = a[(size_t)q * n + k];
-          a[(size_t)pp * n + k] = c * apk - s * aqk;
-          a[(size_t)q * n + k] = s * apk + c * aqk;
-        }
-        for (int k = 0; k < n; k++) {
-          const double vkp = v[(size_t)k * n + pp], vkq = v[(size_t)k * n + q];
-          v[(size_t)k * n + pp] = c * vkp - s * vkq;
-          v[(size_t)k * n + q] = s * vkp + c * vkq;
-        }
-      }
+  w.assign(n, 0.0);
+  if (n == 0) return;
+  auto A = [&](int i, int j) -> double & { return a[(size_t)i * n + j]; };
+  auto Q = [&](int i, int j) -> double & { return v[(size_t)i * n + j]; };
+  std::vector<double> hv(n), pv(n), e(n, 0.0);
+  // reduction: step k annihilates column k below the subdiagonal
+  for (int k = 0; k + 2 < n; k++) {
+    const int m = n - k - 1;  // rows k+1 .. n-1
+    double nrm = 0.0;
+    for (int i = 0; i < m; i++) nrm += A(k + 1 + i, k) * A(k + 1 + i, k);
+    nrm = std::sqrt(nrm);
+    if (nrm == 0.0) continue;
+    const double x0 = A(k + 1, k);
+    const double alpha = x0 >= 0.0 ? -nrm : nrm;
+    for (int i = 0; i < m; i++) hv[i] = A(k + 1 + i, k);
+    hv[0] -= alpha;
+    double hn = 0.0;
+    for (int i = 0; i < m; i++) hn += hv[i] * hv[i];
+    hn = std::sqrt(hn);
+    if (hn == 0.0) continue;
+    for (int i = 0; i < m; i++) hv[i] /= hn;
+    // H = I - 2 h h^T on the trailing block: A22 -= 2 (h q^T + q h^T),
+    // p = A22 h, q = p - (h^T p) h
+    double K = 0.0;
+    for (int i = 0; i < m; i++) {
+      double s = 0.0;
+      for (int j = 0; j < m; j++) s += A(k + 1 + i, k + 1 + j) * hv[j];
+      pv[i] = s;
+      K += hv[i] * s;
+    }
+    for (int i = 0; i < m; i++) pv[i] -= K * hv[i];
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j < m; j++) A(k + 1 + i, k + 1 + j) -= 2.0 * (hv[i] * pv[j] + pv[i] * hv[j]);
+    A(k + 1, k) = A(k, k + 1) = alpha;
+    for (int i = 1; i < m; i++) A(k + 1 + i, k) = A(k, k + 1 + i) = 0.0;
+    // Q <- Q H
+    for (int r = 0; r < n; r++) {
+      double s = 0.0;
+      for (int j = 0; j < m; j++) s += Q(r, k + 1 + j) * hv[j];
+      for (int j = 0; j < m; j++) Q(r, k + 1 + j) -= 2.0 * s * hv[j];
+    }
   }
-  w.resize(n);
-  for (int i = 0; i < n; i++) w[i] = a[(size_t)i * n + i];
+  for (int i = 0; i < n; i++) w[i] = A(i, i);
+  for (int i = 0; i + 1 < n; i++) e[i] = A(i + 1, i);
+  // implicit QR on the unreduced blocks, from the bottom
+  const double eps = 2.220446049250313e-16;
+  int hi = n - 1, guard = 0;
+  while (hi > 0 && guard < 60 * n) {
+    if (std::fabs(e[hi - 1]) <= eps * (std::fabs(w[hi - 1]) + std::fabs(w[hi]))) {
+      e[hi - 1] = 0.0;
+      hi--;
+      continue;
+    }
+    int lo = hi - 1;
+    while (lo > 0 && std::fabs(e[lo - 1]) > eps * (std::fabs(w[lo - 1]) + std::fabs(w[lo]))) lo--;
+    guard++;
+    const double dl = 0.5 * (w[hi - 1] - w[hi]), eh = e[hi - 1];
+    const double mu = w[hi] - eh * eh / (dl + (dl >= 0.0 ? 1.0 : -1.0) * std::hypot(dl, eh));
+    double x = w[lo] - mu, z = e[lo];
+    for (int k = lo; k < hi; k++) {
+      const double r = std::hypot(x, z);
+      const double c = r > 0.0 ? x / r : 1.0, s = r > 0.0 ? z / r : 0.0;
+      if (k > lo) e[k - 1] = r;
+      const double ak = w[k], bk = e[k], dk = w[k + 1];
+      w[k] = c * c * ak + 2.0 * c * s * bk + s * s * dk;
+      w[k + 1] = s * s * ak - 2.0 * c * s * bk + c * c * dk;
+      e[k] = c * s * (dk - ak) + (c * c - s * s) * bk;
+      if (k + 1 < hi) {
+        z = s * e[k + 1];  // the bulge at (k + 2, k)
+        e[k + 1] *= c;
+        x = e[k];
+      }
+      for (int r2 = 0; r2 < n; r2++) {
+        const double qk = Q(r2, k), qk1 = Q(r2, k + 1);
+        Q(r2, k) = c * qk + s * qk1;
+        Q(r2, k + 1) = -s * qk + c * qk1;
+      }
+    }
+  }
 }
 
 // Generalized inverse G (K G K = K, symmetric) of a symmetric positive
@@ -365,51 +411,70 @@ static void jacobi_eig(int n, std::vector<double> &a, std::vector<double> &w, st
 // when K is definite.  n^3/3 flops for the factor, r^3/3 each for the
 // triangular inverse and the product.  Returns r.
 int psd_ginv(int n, const double *K, double rel_tol, double *G) {
-  std::vector<double> a(K, K + (size_t)n * n);
+  // lower triangle, row major: row i holds A(i, 0..i) contiguously
+  std::vector<double> a((size_t)n * n);
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k <= i; k++) a[(size_t)i * n + k] = K[(size_t)i * n + k];
   std::vector<int> piv(n);
   for (int i = 0; i < n; i++) piv[i] = i;
   auto A = [&](int i, int j) -> double & { return a[(size_t)i * n + j]; };
   double dmax = 0.0;
   for (int i = 0; i < n; i++) dmax = std::max(dmax, A(i, i));
+  std::vector<double> colj(n);
   int r = 0;
   for (int j = 0; j < n; j++) {
     int q = j;
     for (int i = j + 1; i < n; i++)
       if (A(i, i) > A(q, q)) q = i;
     if (!(A(q, q) > rel_tol * dmax)) break;
-    if (q != j) {  // symmetric swap of rows and columns j, q
-      for (int k = 0; k < n; k++) std::swap(A(j, k), A(q, k));
-      for (int k = 0; k < n; k++) std::swap(A(k, j), A(k, q));
+    if (q != j) {  // symmetric swap of j < q in lower storage
+      for (int k = 0; k < j; k++) std::swap(A(j, k), A(q, k));
+      std::swap(A(j, j), A(q, q));
+      for (int k = j + 1; k < q; k++) std::swap(A(k, j), A(q, k));
+      for (int k = q + 1; k < n; k++) std::swap(A(k, j), A(k, q));
       std::swap(piv[j], piv[q]);
     }
     const double d = std::sqrt(A(j, j));
     A(j, j) = d;
-    for (int i = j + 1; i < n; i++) A(i, j) /= d;
+    const double dinv = 1.0 / d;
+    for (int i = j + 1; i < n; i++) colj[i] = A(i, j) *= dinv;
+    // trailing update of the lower triangle, row by row
     for (int i = j + 1; i < n; i++) {
-      const double lij = A(i, j);
-      for (int k = j + 1; k <= i; k++) A(i, k) -= lij * A(k, j);
+      const double lij = colj[i];
+      double *ri = &A(i, 0);
+      for (int k = j + 1; k <= i; k++) ri[k] -= lij * colj[k];
     }
-    for (int i = j + 1; i < n; i++)  // keep the trailing block symmetric
-      for (int k = j + 1; k < i; k++) A(k, i) = A(i, k);
     r = j + 1;
   }
-  // X = L11^-1 (lower), by columns
+  // X = L11^-1 (lower, row major): row i = -(1/L_ii) sum_{k<i} L_ik X(k, :)
   std::vector<double> X((size_t)r * r, 0.0);
-  for (int c = 0; c < r; c++) {
-    X[(size_t)c * r + c] = 1.0 / A(c, c);
-    for (int i = c + 1; i < r; i++) {
-      double s = 0.0;
-      for (int k = c; k < i; k++) s += A(i, k) * X[(size_t)k * r + c];
-      X[(size_t)i * r + c] = -s / A(i, i);
+  for (int i = 0; i < r; i++) {
+    double *xi = &X[(size_t)i * r];
+    for (int k = 0; k < i; k++) {
+      const double lik = A(i, k);
+      const double *xk = &X[(size_t)k * r];
+      for (int c = 0; c <= k; c++) xi[c] += lik * xk[c];
+    }
+    const double inv = 1.0 / A(i, i);
+    for (int c = 0; c < i; c++) xi[c] *= -inv;
+    xi[i] = inv;
+  }
+  // H = X^T X (lower, row major) as rank-one updates by the rows of X
+  std::vector<double> H((size_t)r * r, 0.0);
+  for (int k = 0; k < r; k++) {
+    const double *xk = &X[(size_t)k * r];
+    for (int x = 0; x <= k; x++) {
+      const double v = xk[x];
+      double *hx = &H[(size_t)x * r];
+      for (int y = 0; y <= x; y++) hx[y] += v * xk[y];
     }
   }
   std::fill(G, G + (size_t)n * n, 0.0);
   for (int x = 0; x < r; x++)
     for (int y = 0; y <= x; y++) {
-      double s = 0.0;  // (X^T X)(x, y) = sum_{k >= x} X(k, x) X(k, y)
-      for (int k = x; k < r; k++) s += X[(size_t)k * r + x] * X[(size_t)k * r + y];
-      G[(size_t)piv[x] * n + piv[y]] = s;
-      G[(size_t)piv[y] * n + piv[x]] = s;
+      const double v = H[(size_t)x * r + y];
+      G[(size_t)piv[x] * n + piv[y]] = v;
+      G[(size_t)piv[y] * n + piv[x]] = v;
     }
   return r;
 }
@@ -432,7 +497,7 @@ static int rayleigh_ritz(int m, int k, int ld, const double *ga, const double *g
       B[(size_t)i * m + j] = 0.5 * (gb[i * ld + j] + gb[j * ld + i]) * dsc[i] * dsc[j];
       A[(size_t)i * m + j] = 0.5 * (ga[i * ld + j] + ga[j * ld + i]) * dsc[i] * dsc[j];
     }
-  jacobi_eig(m, B, bw, bv);
+  sym_eig(m, B, bw, bv);
   double bmax = 0.0;
   for (double x : bw) bmax = std::max(bmax, x);
   std::vector<int> keep;
@@ -461,7 +526,7 @@ static int rayleigh_ritz(int m, int k, int ld, const double *ga, const double *g
       for (int i = 0; i < m; i++) s += T[(size_t)i * r + a] * AT[(size_t)i * r + c];
       M[(size_t)a * r + c] = s;
     }
-  jacobi_eig(r, M, w, y);
+  sym_eig(r, M, w, y);
   std::vector<int> ord(r);
   for (int i = 0; i < r; i++) ord[i] = i;
   std::sort(ord.begin(), ord.end(), [&](int a, int b) { return w[a] > w[b]; });
@@ -733,6 +798,13 @@ void adaptive_constraints(Solver &S) {
     }
     psd_ginv(m, K.data(), 1e-12, kp.data() + d.kc0);
   }
+  if (S.verbose >= 2 && np > 0) {
+    double mean = 0.0;
+    int mx = 0;
+    for (const PairDesc &d : pd) mean += d.ncp, mx = std::max(mx, d.ncp);
+    std::printf("   adaptivity: %d pairs, pair coarse order mean %.1f max %d, block %d\n", np, mean / np,
+                mx, kmax);
+  }
   ph("pair coarse inverses");
   ph("pairs");
   // ---- the products with S_s (operator applies: the k A inputs and the k
@@ -819,6 +891,7 @@ void adaptive_constraints(Solver &S) {
                        blk[bb].p, 0, 0);
   };
   // preconditioner: W = P T W on the k columns of W (W already projected)
+  double t_pc_coarse = 0.0;  // verbose 3: the pair coarse correction's share
   auto precondition = [&]() {
     if (np == 0) return;
     hipLaunchKernelGGL(k_ad_copy, pk, dim3(256), 0, st, dpairs.p, blk[W].p, 0, 0, win.p, 1, 0);
@@ -828,8 +901,17 @@ void adaptive_constraints(Solver &S) {
     else
       launch_symm_cols<double>(st, (const double *)pcs.inv, dsubP.p, dtaskP.p, (int)taskP.size(),
                                dcolP.p, win.p, wout.p);
+    double t0 = 0.0;
+    if (S.verbose >= 3) {
+      AD_CHECK(hipStreamSynchronize(st));
+      t0 = wall_time();
+    }
     hipLaunchKernelGGL(k_ad_coarse_pc, pk, dim3(256), pc_lds, st, dpairs.p, pcs.Z, pcs.Phi, dcmap.p,
                        dkp.p, win.p, wout.p);
+    if (S.verbose >= 3) {
+      AD_CHECK(hipStreamSynchronize(st));
+      t_pc_coarse += wall_time() - t0;
+    }
     hipLaunchKernelGGL(k_ad_copy, pk, dim3(256), 0, st, dpairs.p, wout.p, 1, 0, blk[W].p, 0, 0);
     hipLaunchKernelGGL(k_ad_project, pk, dim3(64), 0, st, dpairs.p, dqptr.p, dqpos.p, dqval.p,
                        blk[W].p, 0, 0);
@@ -886,14 +968,26 @@ void adaptive_constraints(Solver &S) {
     apply(X, AX, BX);
     rr_update(1);
     int nb = 2;
+    // verbose 3: device time per section (each section synchronized)
+    double sec[4] = {0, 0, 0, 0};  // refresh applies, preconditioner, apply W, Rayleigh-Ritz + update
+    auto timed = [&](int k, auto &&f) {
+      if (S.verbose < 3) return f();
+      AD_CHECK(hipStreamSynchronize(st));
+      const double t0 = wall_time();
+      f();
+      AD_CHECK(hipStreamSynchronize(st));
+      sec[k] += wall_time() - t0;
+    };
     for (int it = 1; it <= S.adapt_maxit; it++) {
       // the A and B images of X and P are carried by the block updates;
       // every ten iterations they are recomputed, so that rounding carried
       // through the recurrences does not leave the Gram matrices
       // inconsistent with the blocks
       if (it % 10 == 0) {
-        apply(X, AX, BX);
-        apply(P, AP, BP);
+        timed(0, [&] {
+          apply(X, AX, BX);
+          apply(P, AP, BP);
+        });
       }
       hipLaunchKernelGGL(k_ad_resid, pk, dim3(256), 0, st, dpairs.p, dlam.p, kmax, blk[AX].p,
                          blk[BX].p, blk[W].p, dnrm.p);
@@ -913,11 +1007,15 @@ void adaptive_constraints(Solver &S) {
       // W = P T P R (the residual kernel wrote R)
       hipLaunchKernelGGL(k_ad_project, pk, dim3(64), 0, st, dpairs.p, dqptr.p, dqpos.p, dqval.p,
                          blk[W].p, 0, 0);
-      precondition();
-      apply(W, AW, BW);
-      rr_update(nb);
+      timed(1, [&] { precondition(); });
+      timed(2, [&] { apply(W, AW, BW); });
+      timed(3, [&] { rr_update(nb); });
       nb = 3;
     }
+    if (S.verbose >= 3)
+      std::printf("   adaptivity: LOBPCG sections: refresh %.4f s, preconditioner %.4f s, apply %.4f s, "
+                  "Rayleigh-Ritz and update %.4f s (coarse correction %.4f s of the preconditioner)\n",
+                  sec[0], sec[1], sec[2], sec[3], t_pc_coarse);
     // A X once more on the final X of every pair: the constraints
     for (PairDesc &d : pd) d.active = 1;
     AD_CHECK(hipMemcpyAsync(dpairs.p, pd.data(), pd.size() * sizeof(PairDesc), hipMemcpyHostToDevice, st));

@@ -787,6 +787,19 @@ catch (const Error &e) {
   return -100;
 }
 
+int bddcml_amd_selftest_symeig(int n, const double *a, double *w, double *v) try {
+  if (n < 0) fatal("BDDCML_AMD_SELFTEST_SYMEIG", "negative order");
+  std::vector<double> A(a, a + (size_t)n * n), W, V;
+  sym_eig(n, A, W, V);
+  std::copy(W.begin(), W.end(), w);
+  std::copy(V.begin(), V.end(), v);
+  return 0;
+}
+catch (const Error &e) {
+  record_error(e);
+  return -100;
+}
+
 int bddcml_amd_selftest_ginv(int n, const double *k, double rel_tol, double *g, int *rank) try {
   if (n < 0) fatal("BDDCML_AMD_SELFTEST_GINV", "negative order");
   *rank = psd_ginv(n, k, rel_tol, g);

@@ -184,6 +184,9 @@ const double *device_schur_tiles(Solver &S, std::vector<int> &slow, std::vector<
 // 296-3024, src/module_dd.f90:4386-4457, 4637-4802); the interface phase is
 // then set up again (device_setup_finish)
 void adaptive_constraints(Solver &S);
+// symmetric eigendecomposition, host (adapt.hip): a (row major) overwritten,
+// eigenvalues in w, eigenvectors in the columns of v (row major)
+void sym_eig(int n, std::vector<double> &a, std::vector<double> &w, std::vector<double> &v);
 // generalized inverse of a symmetric positive semidefinite matrix by
 // truncated pivoted Cholesky, host (adapt.hip); returns the numerical rank
 int psd_ginv(int n, const double *K, double rel_tol, double *G);

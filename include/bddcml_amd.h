@@ -123,6 +123,11 @@ int bddcml_amd_last_error(char *buf, int len);
  * op 3 the inverse in out[4096..8191] */
 int bddcml_amd_selftest_tile(int op, int npairs, const double *A, const double *B, const double *C,
                              const double *X, double alpha, double beta, double *out);
+/* host symmetric eigendecomposition used for the Rayleigh-Ritz steps of the
+ * adaptive constraints' LOBPCG (Householder tridiagonalization + implicit
+ * QR): a (n x n, row major) in, eigenvalues w (unordered) and eigenvectors
+ * as the columns of v (row major) out; runs on the CPU only */
+int bddcml_amd_selftest_symeig(int n, const double *a, double *w, double *v);
 /* host generalized inverse used for the pair coarse problems of the
  * adaptive constraints: g (n x n, row major) with k g k = k for a symmetric
  * positive semidefinite k, by Cholesky with diagonal pivoting stopped at

@@ -184,6 +184,14 @@ class Bddcml:
         xyz = np.asfortranarray(d["xyz"], dtype=np.float64).ravel(order="F")
         empty = np.zeros(1)
         dof_data = d.get("dof_data", np.zeros(0))
+        # user constraints: (rows x nnods) values per local node, or absent
+        uc = d.get("user_constraints")
+        if uc is None:
+            uc, nuc1, nuc2 = empty, 0, 0
+        else:
+            uc = np.asarray(uc, dtype=np.float64)
+            nuc1, nuc2 = uc.shape
+            uc = uc.ravel(order="F")
         self.L.bddcml_upload_subdomain_data_c(
             _ip(d["nelem"]), _ip(d["nnod"]), _ip(d["ndof"]), _ip(d["ndim"]), _ip(d["meshdim"]),
             _ip(d["isub"]), _ip(d["nelems"]), _ip(d["nnods"]), _ip(d["ndofs"]),
@@ -195,7 +203,7 @@ class Bddcml:
             D(d["rhs"]), _ip(len(d["rhs"])), _ip(d["is_rhs_complete"]),
             D(d["sol"]), _ip(len(d["sol"])), _ip(d["matrixtype"]),
             I(d["i_sparse"]), I(d["j_sparse"]), D(d["a_sparse"]), _ip(len(d["a_sparse"])),
-            _ip(d["is_assembled"]), D(empty), _ip(0), _ip(0), D(empty), _ip(0), _ip(0),
+            _ip(d["is_assembled"]), D(uc), _ip(nuc1), _ip(nuc2), D(empty), _ip(0), _ip(0),
             D(dof_data if len(dof_data) else empty), _ip(len(dof_data)),
             _ip(d["find_components"]), _ip(d["use_dual_mesh_graph"]), _ip(d["neighbouring"]))
 

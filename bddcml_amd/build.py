@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbddcml_amd.so")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
-SOURCES = ["kernels_tile.hip", "kernels_misc.hip", "device.hip", "adapt.hip", "noalloc.hip", "plan.cpp",
+SOURCES = ["kernels_tile.hip", "kernels_misc.hip", "device.hip", "adapt.hip", "noalloc.hip", "constraints.cpp", "plan.cpp",
            "levels.cpp", "global.cpp", "api.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-function"]
 HIPFLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

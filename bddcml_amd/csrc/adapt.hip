@@ -543,76 +543,6 @@ static int rayleigh_ritz(int m, int k, int ld, const double *ga, const double *g
   return kk;
 }
 
-// Householder QR with column pivoting of M^T (rows of M as columns), the
-// columns kept while |R_kk| >= 0.01 |R_11|, and Q's first columns as
-// orthonormal rows spanning those of M (LAPACK's DGEQP3 + DORGQR, sign
-// conventions included; src/module_dd.f90:4637-4802).  M: r x n row major.
-static std::vector<std::vector<double>> orthonormal_rows(const std::vector<std::vector<double>> &M, int n) {
-  const int r = (int)M.size();
-  std::vector<std::vector<double>> a(r, std::vector<double>(n));  // columns of M^T
-  for (int c = 0; c < r; c++) a[c] = M[c];
-  std::vector<std::vector<double>> vs;  // Householder vectors
-  std::vector<double> taus, rdiag;
-  std::vector<double> norms(r);
-  for (int c = 0; c < r; c++) {
-    double s = 0.0;
-    for (double x : a[c]) s += x * x;
-    norms[c] = s;
-  }
-  const int kmax = std::min(r, n);
-  for (int k = 0; k < kmax; k++) {
-    // pivot: the remaining column of largest norm (first on ties)
-    int piv = k;
-    double best = -1.0;
-    for (int c = k; c < r; c++) {
-      double s = 0.0;
-      for (int i = k; i < n; i++) s += a[c][i] * a[c][i];
-      norms[c] = s;
-      if (s > best) best = s, piv = c;
-    }
-    std::swap(a[k], a[piv]);
-    // reflector for a[k][k..n)
-    double alpha = a[k][k], xn = 0.0;
-    for (int i = k + 1; i < n; i++) xn += a[k][i] * a[k][i];
-    std::vector<double> v(n, 0.0);
-    double tau = 0.0, beta = alpha;
-    if (xn > 0.0) {
-      beta = -std::copysign(std::sqrt(alpha * alpha + xn), alpha);
-      tau = (beta - alpha) / beta;
-      v[k] = 1.0;
-      for (int i = k + 1; i < n; i++) v[i] = a[k][i] / (alpha - beta);
-    }
-    rdiag.push_back(beta);
-    // apply H = I - tau v v^T to the remaining columns
-    for (int c = k + 1; c < r; c++) {
-      double s = 0.0;
-      for (int i = k; i < n; i++) s += v[i] * a[c][i];
-      for (int i = k; i < n; i++) a[c][i] -= tau * v[i] * s;
-    }
-    vs.push_back(v);
-    taus.push_back(tau);
-  }
-  int nvalid = 0;
-  if (!rdiag.empty() && std::fabs(rdiag[0]) > 1e-12) {
-    for (size_t k = 0; k < rdiag.size(); k++) {
-      if (std::fabs(rdiag[k]) < 0.01 * std::fabs(rdiag[0])) break;
-      nvalid = (int)k + 1;
-    }
-  }
-  // Q e_c = H_0 H_1 ... H_{nvalid-1} e_c
-  std::vector<std::vector<double>> q(nvalid, std::vector<double>(n, 0.0));
-  for (int c = 0; c < nvalid; c++) {
-    std::vector<double> &x = q[c];
-    x[c] = 1.0;
-    for (int k = nvalid - 1; k >= 0; k--) {
-      double s = 0.0;
-      for (int i = k; i < n; i++) s += vs[k][i] * x[i];
-      for (int i = k; i < n; i++) x[i] -= taus[k] * vs[k][i] * s;
-    }
-  }
-  return q;
-}
-
 static double unit_random(uint64_t seed, uint64_t idx) {
   uint64_t z = (seed ^ idx) + 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -1110,45 +1040,7 @@ void adaptive_constraints(Solver &S) {
     face_dof_map[faces[q]] = face_dofs[q];
   }
   ph("face rows and orthonormalization");
-  std::vector<int> nndfc(S.nndfc);
-  for (auto &f : face_q) nndfc[f.first] = (int)f.second.size();
-  std::vector<int> kdofc(nndfc.size() + 1, 0);
-  for (size_t k = 0; k < nndfc.size(); k++) kdofc[k + 1] = kdofc[k] + nndfc[k];
-#pragma omp parallel for schedule(dynamic)
-  for (int is = 0; is < nsub; is++) {
-    Sub &s = S.subs[is];
-    std::map<int, int> g2r;
-    for (int r = 0; r < s.ni; r++) g2r[s.isvgvn[s.ifc_dof[r]]] = r;
-    std::vector<int> cptr{0}, cidx, indrowc;
-    std::vector<double> cval;
-    for (int id : s.cnodes) {
-      if (id >= first_face && face_q.count(id)) {
-        const auto &Q = face_q.at(id);
-        const auto &gd = face_dof_map.at(id);
-        for (size_t r = 0; r < Q.size(); r++) {
-          for (size_t k = 0; k < gd.size(); k++)
-            if (Q[r][k] != 0.0) cidx.push_back(g2r.at(gd[k])), cval.push_back(Q[r][k]);
-          cptr.push_back((int)cidx.size());
-          indrowc.push_back(kdofc[id] + (int)r);
-        }
-      } else {
-        for (int c = 0; c < s.nconstr; c++) {
-          if (node_of(s.indrowc[c]) != id) continue;
-          for (int q = s.cptr[c]; q < s.cptr[c + 1]; q++) cidx.push_back(s.cidx[q]), cval.push_back(s.cval[q]);
-          cptr.push_back((int)cidx.size());
-          indrowc.push_back(kdofc[id] + (s.indrowc[c] - S.kdofc[id]));
-        }
-      }
-    }
-    s.cptr.swap(cptr);
-    s.cidx.swap(cidx);
-    s.cval.swap(cval);
-    s.indrowc.swap(indrowc);
-    s.nconstr = (int)s.indrowc.size();
-  }
-  S.nndfc = nndfc;
-  S.kdofc = kdofc;
-  S.ndofc = kdofc.back();
+  replace_coarse_node_rows(S, face_q, face_dof_map);
   ph("new constraint rows");
   if (S.verbose > 0) {
     int nad = 0;

@@ -99,7 +99,6 @@ void bddcml_upload_subdomain_data_c(
     int *luser_constraints1, int *luser_constraints2, double *element_data,
     int *lelement_data1, int *lelement_data2, double *dof_data, int *ldof_data,
     int *find_components_int, int *use_dual_mesh_graph_int, int *neighbouring) try {
-  (void)user_constraints;
   (void)element_data;
   (void)lelement_data1;
   (void)lelement_data2;
@@ -116,8 +115,6 @@ void bddcml_upload_subdomain_data_c(
     fatal(R, "the device path supports matrixtype 1 (symmetric positive definite)");
   if (S.matrixtype >= 0 && S.matrixtype != *matrixtype) fatal(R, "matrixtype differs between subdomains");
   S.matrixtype = *matrixtype;
-  if (*luser_constraints1 * *luser_constraints2 > 0)
-    fatal(R, "user constraints are not supported by this library");
   Sub &s = local_sub(S, *isub, R);
   if (s.uploaded) fatal(R, "It appears that mesh was already loaded for subdomain " + std::to_string(*isub));
   const int b = S.numbase;
@@ -155,6 +152,13 @@ void bddcml_upload_subdomain_data_c(
   s.has_fix = std::any_of(s.ifix.begin(), s.ifix.end(), [](int v) { return v != 0; });
   s.rhs_complete = *is_rhs_complete != 0;
   s.dof_data.assign(dof_data, dof_data + *ldof_data);
+  // user constraints: luser_constraints1 rows, one value per subdomain node
+  // (src/module_dd.f90:3188-3232)
+  if (*luser_constraints1 > 0 && *luser_constraints2 > 0) {
+    if (*luser_constraints2 != *nnods) fatal(R, "Array USER_CONSTRAINTS dimension mismatch.");
+    s.nuser_c = *luser_constraints1;
+    s.user_c.assign(user_constraints, user_constraints + (size_t)*luser_constraints1 * *luser_constraints2);
+  }
   s.find_components = *find_components_int;
   s.use_dual = *use_dual_mesh_graph_int;
   s.neighbouring = *neighbouring;
@@ -232,14 +236,15 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
     S.use_arith = 1;
     S.weights_type = 0;
     S.use_adaptive = 0;
+    S.use_user = 0;
   } else {
-    if (*use_user_constraints_int) fatal(R, "user constraints are not supported by this library");
     S.use_corner = *use_corner_constraints_int != 0;
     // adaptive constraints are an add-on to the arithmetic ones
     // (src/module_levels.f90:2699-2706)
     S.use_adaptive = *use_adaptive_constraints_int != 0;
     S.use_arith = *use_arithmetic_constraints_int != 0 || S.use_adaptive;
     S.weights_type = *weights_type;
+    S.use_user = *use_user_constraints_int != 0;
     if (S.use_adaptive && S.nl >= 3 && S.nsub2 >= 2)
       fatal(R, "adaptive constraints are supported with two levels");
   }
@@ -265,6 +270,7 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
   build_globs(S);
   phase("globs");
   build_constraints(S);
+  if (S.use_user) add_user_constraints(S);
   phase("constraints");
   build_unique_interface(S);  // the exchange sums below run over it
   phase("unique interface");

@@ -38,6 +38,10 @@ struct Sub {
   std::vector<double> fixv, rhs, sol;
   bool rhs_complete = false, has_fix = false, uploaded = false;
   std::vector<double> dof_data;
+  // user constraints: nuser_c rows of one value per local node (column
+  // major, nuser_c x nnods)
+  std::vector<double> user_c;
+  int nuser_c = 0;
   int find_components = 0, use_dual = 0, neighbouring = 1;
   // assembled upper triangle (sorted by row, col; duplicates summed)
   std::vector<int> ai, aj;

@@ -5,6 +5,7 @@
 #pragma once
 
 #include <array>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -44,6 +45,7 @@ struct Solver {
   std::vector<Sub> subs;
   // options
   int use_corner = 1, use_arith = 1, weights_type = 0;
+  int use_user = 0;  // user constraints on edges (first level)
   int precision = 64;  // 64: FP64 everywhere; 32: FP32 preconditioner factors
   int ordering = 0;    // interior ordering: 0 tile-aligned nested dissection, 1 RCM
   int nd_leaf = 256;   // nested dissection: largest part (dofs) left undivided
@@ -184,6 +186,15 @@ const double *device_schur_tiles(Solver &S, std::vector<int> &slow, std::vector<
 // 296-3024, src/module_dd.f90:4386-4457, 4637-4802); the interface phase is
 // then set up again (device_setup_finish)
 void adaptive_constraints(Solver &S);
+// constraints.cpp: orthonormal rows spanning the rows of M (r x n) by QR with
+// column pivoting, keeping |R_kk| >= 0.01 |R_11| (src/module_dd.f90:
+// 4637-4802); the rows of whole coarse nodes replaced in every subdomain
+// (each node's rows over its sorted global dofs) with the coarse dofs
+// renumbered; the user constraints on edges (src/module_dd.f90:4459-4565)
+std::vector<std::vector<double>> orthonormal_rows(const std::vector<std::vector<double>> &M, int n);
+void replace_coarse_node_rows(Solver &S, const std::map<int, std::vector<std::vector<double>>> &rows,
+                              const std::map<int, std::vector<int>> &gdofs);
+void add_user_constraints(Solver &S);
 // symmetric eigendecomposition, host (adapt.hip): a (row major) overwritten,
 // eigenvalues in w, eigenvectors in the columns of v (row major)
 void sym_eig(int n, std::vector<double> &a, std::vector<double> &w, std::vector<double> &v);

@@ -231,6 +231,8 @@ def main():
                     help="explicit inverse of S' for the local preconditioner: 0 never, 1 at "
                          "setup, 2 after 40 applies (the library's default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="further library option (bddcml_amd_set_option), repeatable")
     ap.add_argument("--pcg-graph", type=int, default=1, choices=(0, 1),
                     help="PCG iterations from the third on replayed from one captured graph")
     ap.add_argument("--ordering", type=int, default=0,
@@ -303,6 +305,10 @@ def main():
             b.set_option("precision", precision)
             b.set_option("explicit_inverse", args.explicit_inverse)
             b.set_option("pcg_graph", args.pcg_graph)
+            for kv in args.set:
+                k, v = kv.split("=")
+                if b.set_option(k, float(v)) != 0:
+                    raise SystemExit(f"unknown option or value: {kv}")
             t0 = time.perf_counter()
             for d in subs:
                 b.upload_subdomain_data(d)

@@ -406,13 +406,14 @@ class Bddcml:
         return out
 
     def timings(self):
-        out = np.zeros(24)
-        self.L.bddcml_amd_timings(out.ctypes.data_as(C.POINTER(C.c_double)), 24)
         keys = ("topology", "plan", "factor", "krylov", "postprocess", "setup_total",
                 "solve_total", "rhs", "ev_interior_ms", "ev_schur_ms", "ev_interface_ms",
                 "ev_coarse_ms", "schur_flops", "interior_flops", "interface_flops",
                 "ev_gemv_ms", "gemv_bytes", "iterations", "ev_kapply_ms", "kapply_bytes",
-                "level2", "ev_gemm_ms", "gemm_flops", "gemm_launches")
+                "level2", "ev_gemm_ms", "gemm_flops", "gemm_launches", "refinements",
+                "refine_relres", "refine", "ev_resid_ms", "resid_bytes")
+        out = np.zeros(len(keys))
+        self.L.bddcml_amd_timings(out.ctypes.data_as(C.POINTER(C.c_double)), len(keys))
         return dict(zip(keys, out.tolist()))
 
     def sub_info(self, isub):

@@ -248,6 +248,8 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
     if (S.use_adaptive && S.nl >= 3 && S.nsub2 >= 2)
       fatal(R, "adaptive constraints are supported with two levels");
   }
+  if (S.use_adaptive && S.refine)
+    fatal(R, "adaptive constraints need the FP64 Schur complements (option refinement = 0)");
   double tp = wall_time();
   auto phase = [&](const char *name) {
     double now = wall_time();
@@ -629,13 +631,15 @@ catch (const Error &e) {
 }
 int bddcml_amd_timings(double *out, int n) try {
   Solver &S = solver();
-  double v[24] = {S.t.topology,        S.t.plan,           S.t.factor,          S.t.krylov,
+  double v[29] = {S.t.topology,        S.t.plan,           S.t.factor,          S.t.krylov,
                   S.t.postprocess,     S.t.setup_total,    S.t.solve_total,     S.t.rhs,
                   S.t.ev_interior_ms,  S.t.ev_schur_ms,    S.t.ev_interface_ms, S.t.ev_coarse_ms,
                   S.t.schur_flops,     S.t.interior_flops, S.t.interface_flops, S.t.ev_gemv_ms,
                   S.t.gemv_bytes,      (double)S.last_iters, S.t.ev_kapply_ms, S.t.kapply_bytes,
-                  S.t.level2,          S.t.ev_gemm_ms,     S.t.gemm_flops,     (double)S.t.gemm_launches};
-  for (int k = 0; k < std::min(n, 24); k++) out[k] = v[k];
+                  S.t.level2,          S.t.ev_gemm_ms,     S.t.gemm_flops,     (double)S.t.gemm_launches,
+                  (double)S.t.refinements, S.t.refine_relres, S.t.refine,     S.t.ev_resid_ms,
+                  S.t.resid_bytes};
+  for (int k = 0; k < std::min(n, 29); k++) out[k] = v[k];
   return 0;
 }
 catch (const Error &e) {
@@ -701,6 +705,21 @@ int bddcml_amd_set_option(const char *key, double value) try {
   if (!std::strcmp(key, "precision")) {
     if (value != 32.0 && value != 64.0) return -2;
     S.precision = (int)value;
+    return 0;
+  }
+  if (!std::strcmp(key, "refinement")) {
+    if (value != 0.0 && value != 1.0) return -2;
+    S.refine = (int)value;
+    return 0;
+  }
+  if (!std::strcmp(key, "refinement_first_tol")) {
+    if (!(value >= 0.0 && value < 1.0)) return -2;
+    S.refine_first = value;
+    return 0;
+  }
+  if (!std::strcmp(key, "refinement_max")) {
+    if (value < 0.0) return -2;
+    S.refine_max = (int)value;
     return 0;
   }
   if (!std::strcmp(key, "interior_ordering")) {

@@ -229,14 +229,18 @@ struct Device {
   // start and end of every GEMM launch of the interior factorization (the
   // launches that take most of the setup's device time)
   std::vector<hipEvent_t> ev_gemm;
-  // around one S GEMV and one local preconditioner launch of a solve
-  hipEvent_t ev_apply[4] = {};
+  // around one S GEMV, one local preconditioner launch and one FP64
+  // residual launch (refinement) of a solve
+  hipEvent_t ev_apply[6] = {};
   bool gemv_timed = false, kapply_timed = false;
   std::vector<double> gemm_flops;  // algorithmic flops of each of them
   hipEvent_t ev_up = nullptr;
   hipEvent_t ev_rhs = nullptr;  // reduced right-hand side done (upload stream)
   bool rhs_pending = false;     // it was launched there by the setup
   DBuf<double> pool;      // subdomain tiles: pattern of A (factor, S), inverse diagonal tiles
+  // the same tiles in FP32 (interior factor and S in FP32, refined solves)
+  DBuf<float> ipool32;
+  bool f32i = false;
   int64_t ntiles = 0;
   bool mixed = false;     // S' factor, its inverse diagonal tiles and Z in FP32
   DBuf<double> fpool;     // factor space, FP64 mode
@@ -306,6 +310,12 @@ struct Device {
   View<int> rptr, rk, rt, tcptr, tck, tct;
   DBuf<double> vec;
   int64_t VT = 0;
+  // vec holds the forward sweep of the current right-hand side (the
+  // backward sweep of the solution overwrites it)
+  bool vec_y = false;
+  // refinement: local-order residual / correction, reduced right-hand side
+  // of the residual, interface solution before the correction
+  DBuf<double> rloc, gk, uacc;
   View<int64_t> rhs_src;
   // local dof order <-> device order of every subdomain
   std::vector<OrderDesc> hodesc;
@@ -650,7 +660,8 @@ void device_setup_begin(Solver &S) {
     if (r != is) D.sub[is] = D.sub[r];
   }
   int next = 0;
-  D.mixed = S.precision == 32;
+  D.f32i = S.refine != 0;
+  D.mixed = S.precision == 32 || D.f32i;
   for (int is = 0; is < nsub; is++) assign_tiles(D.sub[is], next);
   D.ntiles = next;
 
@@ -675,7 +686,8 @@ void device_setup_begin(Solver &S) {
     pool_alloc.th = std::thread([&D, &pool_alloc, dev] {
       try {
         HIPCHECK(hipSetDevice(dev));
-        D.pool.alloc((size_t)D.ntiles * TILE_ELEMS);
+        if (D.f32i) D.ipool32.alloc((size_t)D.ntiles * TILE_ELEMS);
+        else D.pool.alloc((size_t)D.ntiles * TILE_ELEMS);
       } catch (...) {
         pool_alloc.err = std::current_exception();
       }
@@ -953,7 +965,6 @@ void device_setup_begin(Solver &S) {
   S.t.interior_flops = tile_work(a_diag, 2 * kTileChol) + tile_work(a_off, kTileGemm) + S.t.schur_flops;
   // ---- numerics, first half: Dirichlet elimination into the tiles,
   // interior factorization, Schur complements
-  double *pool = D.pool.p;
   hipEvent_t *ev = D.ev;
   for (int k = 0; k < 5; k++) HIPCHECK(hipEventCreate(&ev[k]));
   D.ev_gemm.assign(2 * D.a_off.size(), nullptr);
@@ -963,18 +974,30 @@ void device_setup_begin(Solver &S) {
   HIPCHECK(hipEventCreateWithFlags(&D.ev_up, hipEventDisableTiming));
   HIPCHECK(hipEventCreateWithFlags(&D.ev_rhs, hipEventDisableTiming));
   HIPCHECK(hipEventRecord(ev[0], st));
-  launch_zero_tiles(st, pool, D.clear_ids.p, (int)D.clear_ids.n);
-  launch_scatter_sub(st, pool, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p,
-                     D.sc_pads.p, D.err.p);
-  for (size_t J = 0; J < D.a_diag.size(); J++) {
-    launch_tile_potrf(st, pool, D.a_diag[J].tasks.p, D.a_diag[J].pairs.p, D.a_diag[J].ntask, D.err.p);
-    HIPCHECK(hipEventRecord(D.ev_gemm[2 * J], st));
-    launch_tile_gemm(st, pool, D.a_off[J].tasks.p, D.a_off[J].pairs.p, D.a_off[J].ntask, GEMM_NT,
-                     POST_RIGHT_XT, 1.0, -1.0);
-    HIPCHECK(hipEventRecord(D.ev_gemm[2 * J + 1], st));
+  // element type of the subdomain tiles: FP64, or FP32 with refinement
+  auto interior = [&](auto *pool) {
+    launch_zero_tiles(st, pool, D.clear_ids.p, (int)D.clear_ids.n);
+    launch_scatter_sub(st, pool, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p,
+                       D.sc_pads.p, D.err.p);
+    for (size_t J = 0; J < D.a_diag.size(); J++) {
+      launch_tile_potrf(st, pool, D.a_diag[J].tasks.p, D.a_diag[J].pairs.p, D.a_diag[J].ntask,
+                        D.err.p);
+      HIPCHECK(hipEventRecord(D.ev_gemm[2 * J], st));
+      launch_tile_gemm(st, pool, D.a_off[J].tasks.p, D.a_off[J].pairs.p, D.a_off[J].ntask, GEMM_NT,
+                       POST_RIGHT_XT, 1.0, -1.0);
+      HIPCHECK(hipEventRecord(D.ev_gemm[2 * J + 1], st));
+    }
+    HIPCHECK(hipEventRecord(ev[1], st));
+  };
+  if (D.f32i) {
+    interior(D.ipool32.p);
+    // S = A22 - W^T W: the same tasks on the FP32 tile GEMM
+    launch_tile_gemm(st, D.ipool32.p, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask,
+                     GEMM_NT, POST_NONE, 1.0, -1.0);
+  } else {
+    interior(D.pool.p);
+    launch_schur_gemm(st, D.pool.p, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask);
   }
-  HIPCHECK(hipEventRecord(ev[1], st));
-  launch_schur_gemm(st, pool, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask);
   HIPCHECK(hipEventRecord(ev[2], st));
   HIPCHECK(hipGetLastError());
   S.t.plan += wall_time() - t0;
@@ -999,14 +1022,21 @@ static void form_inverse(Solver &S) {
   D.explicit_inv = true;
 }
 
-// condensation rhs into the device-ordered subdomain vectors (padding 0),
-// forward sweep over the interior factor, sum of the interface parts into g
-static void reduced_rhs_launches(Solver &S, hipStream_t st) {
+// condensation rhs (src: local order, the Dirichlet-eliminated rhs when
+// null) into the device-ordered subdomain vectors (padding 0), forward sweep
+// over the interior factor, sum of the interface parts into g (D.g when null)
+static void reduced_rhs_launches(Solver &S, hipStream_t st, const double *src = nullptr,
+                                 double *g = nullptr) {
   Device &D = *S.dev;
   HIPCHECK(hipMemsetAsync(D.vec.p, 0, D.VT * sizeof(double), st));
-  launch_to_device_order(st, D.odesc.p, S.nsub, D.sc_perm.p, D.fmod.p, D.vec.p, D.maxndofs);
-  launch_trsv_fwd(st, D.pool.p, D.trsv.p, S.nsub, D.rptr.p, D.rk.p, D.rt.p, D.vec.p, D.maxnt);
-  launch_assemble(st, S.nig, D.aptr.p, D.rhs_src.p, nullptr, D.vec.p, D.g.p);
+  launch_to_device_order(st, D.odesc.p, S.nsub, D.sc_perm.p, src ? src : D.fmod.p, D.vec.p,
+                         D.maxndofs);
+  if (D.f32i)
+    launch_trsv_fwd(st, D.ipool32.p, D.trsv.p, S.nsub, D.rptr.p, D.rk.p, D.rt.p, D.vec.p, D.maxnt);
+  else
+    launch_trsv_fwd(st, D.pool.p, D.trsv.p, S.nsub, D.rptr.p, D.rk.p, D.rt.p, D.vec.p, D.maxnt);
+  launch_assemble(st, S.nig, D.aptr.p, D.rhs_src.p, nullptr, D.vec.p, g ? g : D.g.p);
+  D.vec_y = true;
 }
 
 void device_setup_finish(Solver &S) {
@@ -1539,17 +1569,21 @@ void device_setup_finish(Solver &S) {
   t0 = wall_time();
   st = D.st;
   HIPCHECK(hipStreamWaitEvent(st, D.ev_up, 0));
-  double *pool = D.pool.p;
   hipEvent_t *ev = D.ev;
-  // S' factor and Z = S'^{-1} C^T in the factor space (FP32 when mixed);
-  // then G = C Z, Phi = Z G^{-1} in FP64
-  auto factor_phase = [&](auto *fp) {
+  // S' assembly from S (FP64 tiles, or FP32 with refinement)
+  auto sprime = [&](const auto *pool, auto *fp) {
     if (D.gen_constraints)
       launch_sprime_gen(st, pool, fp, D.desc.p, D.sp_sub.p, D.sp_ij.p, D.sp_dst.p, D.n_sp, D.dptr.p,
                         D.dcon.p, D.dval.p, D.comega.p);
     else
       launch_sprime(st, pool, fp, D.desc.p, D.sp_sub.p, D.sp_ij.p, D.sp_dst.p, D.n_sp, D.cid.p,
                     D.csize.p);
+  };
+  // S' factor and Z = S'^{-1} C^T in the factor space (FP32 when mixed);
+  // then G = C Z, Phi = Z G^{-1} in FP64
+  auto factor_phase = [&](auto *fp) {
+    if (D.f32i) sprime((const float *)D.ipool32.p, fp);
+    else sprime((const double *)D.pool.p, fp);
     for (size_t J = 0; J < D.p_diag.size(); J++) {
       launch_tile_potrf(st, fp, D.p_diag[J].tasks.p, D.p_diag[J].pairs.p, D.p_diag[J].ntask, D.err.p);
       launch_tile_gemm(st, fp, D.p_off[J].tasks.p, D.p_off[J].pairs.p, D.p_off[J].ntask, GEMM_NT,
@@ -1952,8 +1986,15 @@ void device_sub_matrix(Solver &S, int is, int which, double *out) {
   if (which != 'S') fatal("BDDCML_AMD_SUB_MATRIX", "only the Schur complement is stored explicitly");
   const int n = T.nti;
   std::vector<double> tiles((size_t)n * (n + 1) / 2 * TILE_ELEMS);
-  HIPCHECK(hipMemcpy(tiles.data(), D.pool.p + (size_t)T.slow * TILE_ELEMS,
-                     tiles.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (D.f32i) {
+    std::vector<float> t32(tiles.size());
+    HIPCHECK(hipMemcpy(t32.data(), D.ipool32.p + (size_t)T.slow * TILE_ELEMS,
+                       t32.size() * sizeof(float), hipMemcpyDeviceToHost));
+    std::copy(t32.begin(), t32.end(), tiles.begin());
+  } else {
+    HIPCHECK(hipMemcpy(tiles.data(), D.pool.p + (size_t)T.slow * TILE_ELEMS,
+                       tiles.size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
   for (int a = 0; a < s.ni; a++)
     for (int b = 0; b < s.ni; b++) {
       const int p = std::max(a, b), q = std::min(a, b);  // lower triangle entry (p, q)
@@ -2066,14 +2107,19 @@ static void op_schur(Solver &S, const double *x, double *y, bool timed = false) 
     timing_events(D);
     HIPCHECK(hipEventRecord(D.ev_apply[0], D.st));
   }
-  launch_symv(D.st, (const double *)D.pool.p, D.scols.p, D.nscols, D.srows.p, D.nsrows, D.loff.p,
-              D.map.p, nullptr, x, D.spart.p, D.yl.p);
+  if (D.f32i)
+    launch_symv(D.st, (const float *)D.ipool32.p, D.scols.p, D.nscols, D.srows.p, D.nsrows,
+                D.loff.p, D.map.p, nullptr, x, D.spart.p, D.yl.p);
+  else
+    launch_symv(D.st, (const double *)D.pool.p, D.scols.p, D.nscols, D.srows.p, D.nsrows,
+                D.loff.p, D.map.p, nullptr, x, D.spart.p, D.yl.p);
   if (timed) {
     HIPCHECK(hipEventRecord(D.ev_apply[1], D.st));
     double tiles = 0;
     for (const SubTiles &T : D.sub) tiles += (double)T.nti * (T.nti + 1) / 2;
-    // lower matrix tiles + gathered input + written output (8 bytes each)
-    S.t.gemv_bytes = tiles * TILE_ELEMS * 8.0 + 16.0 * (double)D.LV;
+    // lower matrix tiles (8 or 4 bytes an entry) + gathered input + written
+    // output (8 bytes each)
+    S.t.gemv_bytes = tiles * TILE_ELEMS * (D.f32i ? 4.0 : 8.0) + 16.0 * (double)D.LV;
     D.gemv_timed = true;
   }
   launch_assemble(D.st, S.nig, D.aptr.p, D.asrc.p, nullptr, D.yl.p, y);
@@ -2270,16 +2316,27 @@ static double tridiag_max_eig(const std::vector<double> &d, const std::vector<do
 // ------------------------------------------------------------------- Krylov
 // The solve ends with the local solutions in device memory (the timed
 // result); the host copies are made on first use.
-static void postprocess(Solver &S) {
+// interface values of every subdomain copy <- u, backward sweep for the
+// interior (u_o = A11^{-1}(f_o - A12 u_G)) over the forward sweep left in
+// vec, back to local dof order into dst
+static void back_substitute(Solver &S, double *dst) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
-  // interface values of every subdomain copy <- u, backward sweep for the
-  // interior (u_o = A11^{-1}(f_o - A12 u_G)), back to local dof order
+  if (!D.vec_y) fatal("BDDCML_SOLVE", "forward sweep of the right-hand side missing");
   launch_ifc_scatter(st, S.nig, D.aptr.p, D.rhs_src.p, D.u.p, D.vec.p);
-  launch_trsv_bwd(st, D.pool.p, D.trsv.p, S.nsub, D.tcptr.p, D.tck.p, D.tct.p, D.vec.p, D.maxnt);
-  launch_from_device_order(st, D.odesc.p, S.nsub, D.sc_perm.p, D.vec.p, D.sol.p, D.maxndofs);
+  if (D.f32i)
+    launch_trsv_bwd(st, D.ipool32.p, D.trsv.p, S.nsub, D.tcptr.p, D.tck.p, D.tct.p, D.vec.p,
+                    D.maxnt);
+  else
+    launch_trsv_bwd(st, D.pool.p, D.trsv.p, S.nsub, D.tcptr.p, D.tck.p, D.tct.p, D.vec.p, D.maxnt);
+  D.vec_y = false;
+  launch_from_device_order(st, D.odesc.p, S.nsub, D.sc_perm.p, D.vec.p, dst, D.maxndofs);
   HIPCHECK(hipGetLastError());
-  HIPCHECK(hipStreamSynchronize(st));
+}
+static void postprocess(Solver &S) {
+  Device &D = *S.dev;
+  back_substitute(S, D.sol.p);
+  HIPCHECK(hipStreamSynchronize(D.st));
   S.sol_on_device = true;
 }
 
@@ -2399,19 +2456,20 @@ static void pcg_graph_replay(Solver &S, int k, double *rz, double *pap, double *
   }
 }
 
-void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bool recycling,
-                  int maxvec, int *num_iter, int *reason, double *cond) {
+// Krylov solve of S u = rhs on the interface from the u in D.u (fixed dofs
+// keep their values); appends the relative residuals to S.hist and returns
+// the norm of the initial residual in normres0_out (0 when it vanishes)
+static void krylov(Solver &S, const double *rhs, int method, double tol, int maxit, int ndecrmax,
+                   bool recycling, int maxvec, int *num_iter, int *reason, double *cond,
+                   double *normres0_out) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
   const int n = S.nig;
-  double t0 = wall_time();
-  S.hist.clear();
   *num_iter = 0;
   *reason = 0;
   *cond = (method == 0) ? 1.0 : -1.0;
-  inverse_at_solve_start(S);
-  initial_guess(S);
-  HIPCHECK(hipMemcpyAsync(D.r.p, D.g.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  *normres0_out = 0.0;
+  HIPCHECK(hipMemcpyAsync(D.r.p, rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st));
   launch_mask(st, n, D.fixmask.p, D.r.p);
   double sc[4];
   if (method == 0) {
@@ -2423,6 +2481,7 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bo
     dots(S, {{D.r.p, D.r.p}}, sc);
     double normres0 = std::sqrt(sc[0]);
     if (normres0 == 0.0) goto finish;
+    *normres0_out = normres0;
     // Krylov recycling (src/module_krylov.f90:186-216, 308-455, 547-613,
     // 663-726, 1859-2036): the kept directions V (A-orthonormal, W = S V)
     // correct the initial guess by V V^T r0, every new direction is
@@ -2565,6 +2624,7 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bo
     dots(S, {{D.r.p, D.r.p}}, sc);
     double normrhs = std::sqrt(sc[0]);
     if (normrhs == 0.0) goto finish;
+    *normres0_out = normrhs;
     double rhoold = 1.0, alpha = 1.0, omega = 1.0;
     HIPCHECK(hipMemsetAsync(D.v.p, 0, n * sizeof(double), st));
     HIPCHECK(hipMemsetAsync(D.p.p, 0, n * sizeof(double), st));
@@ -2628,11 +2688,121 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bo
   }
 finish:
   HIPCHECK(hipStreamSynchronize(st));
+}
+
+// FP64 iterative refinement over the FP32 factorizations (the interior
+// factors, the Schur complements S~ and the S' factors).  The FP64 residual
+// of the subdomain problems r = f - A u (uploaded triplets, local order) is
+// condensed by the forward sweep; its interface part g_k = g - S u_G is the
+// residual of the interface problem with the exact Schur complements (the
+// interior solves of the sweep cancel out of it), so ||g_k|| / ||r_0|| is
+// the relative residual the FP64 method stops on.  Each step solves
+// S~ d = g_k from d = 0 to the tolerance the step needs, back-substitutes
+// the correction in the interior and adds it to the solution.
+static void refine_solution(Solver &S, int method, double tol, int maxit, int ndecrmax,
+                            double normres0, int *num_iter, int *reason) {
+  Device &D = *S.dev;
+  hipStream_t st = D.st;
+  const int n = S.nig;
+  if (D.rloc.n < (size_t)D.ndofs_total) D.rloc.alloc(D.ndofs_total);
+  if (D.gk.n < (size_t)n) {
+    D.gk.alloc(n);
+    D.uacc.alloc(n);
+  }
+  if (D.ndofs_total > INT32_MAX) fatal("BDDCML_SOLVE", "too many subdomain dofs for the refinement");
+  int total = *num_iter;
+  double sc[4];
+  for (int k = 0;; k++) {
+    if (k == 0) {
+      timing_events(D);
+      HIPCHECK(hipEventRecord(D.ev_apply[4], st));
+    }
+    launch_resid_local(st, D.sc_subs.p, D.odesc.p, S.nsub, D.sc_fix.p, D.fmod.p, D.sol.p, D.rloc.p,
+                       D.maxndofs);
+    if (k == 0) HIPCHECK(hipEventRecord(D.ev_apply[5], st));
+    reduced_rhs_launches(S, st, D.rloc.p, D.gk.p);
+    HIPCHECK(hipMemcpyAsync(D.r.p, D.gk.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    launch_mask(st, n, D.fixmask.p, D.r.p);
+    dots(S, {{D.r.p, D.r.p}}, sc);
+    if (k == 0) {
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, D.ev_apply[4], D.ev_apply[5]));
+      S.t.ev_resid_ms = ms;
+      // triplets (row, column, value), flags, f, u and r of every dof
+      double nnz = 0;
+      for (int is = 0; is < S.nsub; is++) nnz += (double)S.subs[is].ai.size();
+      S.t.resid_bytes = 16.0 * nnz + 25.0 * (double)D.ndofs_total;
+    }
+    const double rel = std::sqrt(sc[0]) / normres0;
+    S.t.refine_relres = rel;
+    if (S.verbose > 0)
+      std::printf(" BDDCML_SOLVE: refinement step %d: relative residual %.6e\n", k, rel);
+    if (rel < tol) {
+      *reason = 0;
+      break;
+    }
+    if (k >= S.refine_max || total >= maxit) {
+      *reason = -1;
+      break;
+    }
+    HIPCHECK(hipMemcpyAsync(D.uacc.p, D.u.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemsetAsync(D.u.p, 0, n * sizeof(double), st));
+    const double inner = std::min(0.5, 0.5 * tol / rel);
+    const size_t h0 = S.hist.size();
+    int it = 0, rs = 0;
+    double c = 0.0, nr = 0.0;
+    krylov(S, D.gk.p, method, inner, maxit - total, ndecrmax, false, 0, &it, &rs, &c, &nr);
+    for (size_t q = h0; q < S.hist.size(); q++) S.hist[q] *= nr / normres0;
+    total += it;
+    S.t.refinements++;
+    back_substitute(S, D.rloc.p);
+    launch_axpy2(st, (int)D.ndofs_total, 1.0, D.rloc.p, D.sol.p, 0.0, nullptr, nullptr);
+    launch_axpy2(st, n, 1.0, D.uacc.p, D.u.p, 0.0, nullptr, nullptr);
+    if (rs == -2) {
+      *reason = -2;
+      break;
+    }
+  }
+  *num_iter = total;
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(st));
+  S.sol_on_device = true;
+}
+
+void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bool recycling,
+                  int maxvec, int *num_iter, int *reason, double *cond) {
+  Device &D = *S.dev;
+  double t0 = wall_time();
+  S.hist.clear();
+  S.t.refinements = 0;
+  S.t.refine_relres = 0.0;
+  inverse_at_solve_start(S);
+  // a repeated solve of the same right-hand side redoes its forward sweep
+  // (the back substitution of the last one overwrote it)
+  if (!D.vec_y) reduced_rhs_launches(S, D.st);
+  initial_guess(S);
+  // with refinement the first solve stops early (at the square root of the
+  // tolerance unless set): the FP32 operator's own error caps what more
+  // iterations on it would give
+  const bool refine = D.f32i;
+  const double tol1 =
+      refine ? std::max(tol, S.refine_first > 0.0 ? S.refine_first : std::sqrt(tol)) : tol;
+  double normres0 = 0.0;
+  krylov(S, D.g.p, method, tol1, maxit, ndecrmax, recycling && !refine, maxvec, num_iter, reason,
+         cond, &normres0);
   read_apply_timings(S);
   S.t.krylov += wall_time() - t0;
   t0 = wall_time();
   postprocess(S);
   S.t.postprocess += wall_time() - t0;
+  // refined also after a stagnating first solve (a first solve from a
+  // converged initial guess stagnates at rounding level): the corrections
+  // bring the interior back-substitution to FP64 accuracy
+  if (refine && *reason != -1 && normres0 > 0.0) {
+    t0 = wall_time();
+    refine_solution(S, method, tol, maxit, ndecrmax, normres0, num_iter, reason);
+    S.t.refine += wall_time() - t0;
+  }
 }
 
 }  // namespace bddc

@@ -53,7 +53,7 @@ struct MirrorJob {
   int src, dst, trans;
 };
 void launch_tile_mirror(hipStream_t st, double *pool, const MirrorJob *jobs, int njobs);
-void launch_zero_tiles(hipStream_t st, double *pool, const int *ids, int n);
+template <typename T> void launch_zero_tiles(hipStream_t st, T *pool, const int *ids, int n);
 // clear ranges of consecutive tiles: ranges[k] = (first tile, count)
 void launch_zero_ranges(hipStream_t st, double *pool, const int2 *ranges, int nranges);
 // assembled triplets of each subdomain -> tile pool (see kernels_tile.hip)
@@ -68,9 +68,9 @@ struct ScatterSub {
   int npad;
   int base;    // first tile of the subdomain
 };
-void launch_scatter_sub(hipStream_t st, double *pool, const ScatterSub *subs, int nsub,
-                        const int *perm, const char *fix, const int *ttab, const int *pads,
-                        int *err);
+template <typename T>
+void launch_scatter_sub(hipStream_t st, T *pool, const ScatterSub *subs, int nsub, const int *perm,
+                        const char *fix, const int *ttab, const int *pads, int *err);
 // pool[idx[k]] = 1.0
 void launch_scatter_ones(hipStream_t st, double *pool, const int64_t *idx, int64_t n);
 
@@ -92,9 +92,10 @@ struct SubDesc {
   double rho;             // augmented-Lagrangian scale
 };
 // S' = S + sum_c (rho/|D_c|) 1_c 1_c^T into the lower tiles of the factor
-// pool ppool (element type T; FP32 in the mixed-precision mode)
-template <typename T>
-void launch_sprime(hipStream_t st, const double *pool, T *ppool, const SubDesc *subs,
+// pool ppool (element type T; FP32 in the mixed-precision mode) from S in
+// pool (element type P: FP32 with the FP32 interior factorization)
+template <typename P, typename T>
+void launch_sprime(hipStream_t st, const P *pool, T *ppool, const SubDesc *subs,
                    const int *jobs_sub, const int *jobs_ij, const int *jobs_dst, int njobs,
                    const int *cid, const int *csize);
 // right-hand sides C^T of the multi-RHS solve S' Z = C^T, written as the
@@ -116,8 +117,8 @@ void launch_ginv_to_tiles(hipStream_t st, T *pool, const SubDesc *subs, int nsub
 // constraints give): per interface dof (local flat index) the list
 // dptr/dcon/dval of (global constraint index, value), constraints ascending;
 // S' = S + sum_k omega_k c_k c_k^T and C^T tiles from those lists
-template <typename T>
-void launch_sprime_gen(hipStream_t st, const double *pool, T *ppool, const SubDesc *subs,
+template <typename P, typename T>
+void launch_sprime_gen(hipStream_t st, const P *pool, T *ppool, const SubDesc *subs,
                        const int *jobs_sub, const int *jobs_ij, const int *jobs_dst, int njobs,
                        const int *dptr, const int *dcon, const double *dval, const double *omega);
 template <typename T>
@@ -152,10 +153,13 @@ void set_trsv_global(bool on);
 // forms; 2: the few-subdomain forms (testing)
 void set_launch_forms(int forms);
 int launch_forms();
-void launch_trsv_fwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
-                     const int *rptr, const int *rk, const int *rt, double *v, int maxnt);
-void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
-                     const int *cptr, const int *ck, const int *ct, double *v, int maxnt);
+// P: element type of the interior factor tiles (double or float)
+template <typename P>
+void launch_trsv_fwd(hipStream_t st, const P *pool, const TrsvDesc *d, int nsub, const int *rptr,
+                     const int *rk, const int *rt, double *v, int maxnt);
+template <typename P>
+void launch_trsv_bwd(hipStream_t st, const P *pool, const TrsvDesc *d, int nsub, const int *cptr,
+                     const int *ck, const int *ct, double *v, int maxnt);
 // largest dynamic LDS allocation a kernel may request on the current device
 size_t max_dynamic_lds();
 
@@ -172,6 +176,10 @@ void launch_to_device_order(hipStream_t st, const OrderDesc *d, int nsub, const 
 // dst[poff + i] = vec[voff + perm[permoff + i]]
 void launch_from_device_order(hipStream_t st, const OrderDesc *d, int nsub, const int *perm,
                               const double *vec, double *dst, int maxndofs);
+// r = f - A u per subdomain in local dof order, FP64, A from the uploaded
+// triplets with the Dirichlet elimination of launch_scatter_sub
+void launch_resid_local(hipStream_t st, const ScatterSub *subs, const OrderDesc *od, int nsub,
+                        const char *fix, const double *f, const double *u, double *r, int maxndofs);
 // every local copy of unique interface dof g: vec[pos[q]] = u[g], q in ptr[g]..ptr[g+1]
 // three levels: vec[p] = w[p] * rc[src[p]] (0 where src < 0), p < n; and
 // uc[g] = vec[pos[g]], g < ndofc

@@ -14,12 +14,20 @@ __device__ __forceinline__ void store2(float *p, double x, double y) {
 __device__ __forceinline__ void store2(double *p, double x, double y) {
   *reinterpret_cast<double2 *>(p) = make_double2(x, y);
 }
+__device__ __forceinline__ double2 load2(const double *p) {
+  return *reinterpret_cast<const double2 *>(p);
+}
+__device__ __forceinline__ double2 load2(const float *p) {
+  const float2 f = *reinterpret_cast<const float2 *>(p);
+  return make_double2((double)f.x, (double)f.y);
+}
 
 // One job per lower tile (I, J): thread t handles rows 2(t%32), 2(t%32)+1
 // of the columns t/32 + 8k (16-byte loads); the constraint ids of the rows
-// are loop invariants, those of the columns are staged in LDS.
-template <typename T>
-__global__ __launch_bounds__(256) void k_sprime(const double *__restrict__ pool,
+// are loop invariants, those of the columns are staged in LDS.  P: element
+// type of S (FP64, or FP32 with the FP32 interior factorization), T: of S'.
+template <typename P, typename T>
+__global__ __launch_bounds__(256) void k_sprime(const P *__restrict__ pool,
                                                 T *__restrict__ ppool,
                                                 const SubDesc *__restrict__ subs,
                                                 const int *__restrict__ jsub,
@@ -32,8 +40,7 @@ __global__ __launch_bounds__(256) void k_sprime(const double *__restrict__ pool,
   const SubDesc d = subs[jsub[job]];
   const int I = jij[job] >> 16, J = jij[job] & 0xffff;
   // jobs cover I >= J: tile (I, J) of the packed lower S
-  const double *src =
-      pool + (size_t)(d.slow + J * d.nti - J * (J - 1) / 2 + (I - J)) * TILE_ELEMS;
+  const P *src = pool + (size_t)(d.slow + J * d.nti - J * (J - 1) / 2 + (I - J)) * TILE_ELEMS;
   T *dst = ppool + (size_t)jdst[job] * TILE_ELEMS;
   const int *ci = cid + d.loff;
   const int tid = threadIdx.x;
@@ -44,7 +51,7 @@ __global__ __launch_bounds__(256) void k_sprime(const double *__restrict__ pool,
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const int c = c0 + 8 * k;
-    const double2 v = *reinterpret_cast<const double2 *>(src + c * 64 + r0);
+    const double2 v = load2(src + c * 64 + r0);
     double x = v.x, y = v.y;
     const int b = ccol[c];
     if (b >= 0 && (a0 == b || a1 == b)) {
@@ -56,20 +63,26 @@ __global__ __launch_bounds__(256) void k_sprime(const double *__restrict__ pool,
   }
 }
 
-template <typename T>
-void launch_sprime(hipStream_t st, const double *pool, T *ppool, const SubDesc *subs,
+template <typename P, typename T>
+void launch_sprime(hipStream_t st, const P *pool, T *ppool, const SubDesc *subs,
                    const int *jobs_sub, const int *jobs_ij, const int *jobs_dst, int njobs,
                    const int *cid, const int *csize) {
   if (njobs <= 0) return;
-  hipLaunchKernelGGL(k_sprime<T>, dim3(njobs), dim3(256), 0, st, pool, ppool, subs, jobs_sub,
+  hipLaunchKernelGGL((k_sprime<P, T>), dim3(njobs), dim3(256), 0, st, pool, ppool, subs, jobs_sub,
                      jobs_ij, jobs_dst, cid, csize);
 }
-template void launch_sprime<double>(hipStream_t, const double *, double *, const SubDesc *,
-                                    const int *, const int *, const int *, int, const int *,
-                                    const int *);
-template void launch_sprime<float>(hipStream_t, const double *, float *, const SubDesc *,
-                                   const int *, const int *, const int *, int, const int *,
-                                   const int *);
+template void launch_sprime<double, double>(hipStream_t, const double *, double *, const SubDesc *,
+                                            const int *, const int *, const int *, int,
+                                            const int *, const int *);
+template void launch_sprime<double, float>(hipStream_t, const double *, float *, const SubDesc *,
+                                           const int *, const int *, const int *, int, const int *,
+                                           const int *);
+template void launch_sprime<float, float>(hipStream_t, const float *, float *, const SubDesc *,
+                                          const int *, const int *, const int *, int, const int *,
+                                          const int *);
+template void launch_sprime<float, double>(hipStream_t, const float *, double *, const SubDesc *,
+                                           const int *, const int *, const int *, int, const int *,
+                                           const int *);
 
 // General constraint rows (adaptive constraints: several dense rows on one
 // face): every interface dof carries the list of (constraint, value) of the
@@ -94,8 +107,8 @@ __device__ __forceinline__ double shared_rows(const int *__restrict__ dptr,
   }
   return s;
 }
-template <typename T>
-__global__ __launch_bounds__(256) void k_sprime_gen(const double *__restrict__ pool,
+template <typename P, typename T>
+__global__ __launch_bounds__(256) void k_sprime_gen(const P *__restrict__ pool,
                                                     T *__restrict__ ppool,
                                                     const SubDesc *__restrict__ subs,
                                                     const int *__restrict__ jsub,
@@ -108,8 +121,7 @@ __global__ __launch_bounds__(256) void k_sprime_gen(const double *__restrict__ p
   const int job = blockIdx.x;
   const SubDesc d = subs[jsub[job]];
   const int I = jij[job] >> 16, J = jij[job] & 0xffff;
-  const double *src =
-      pool + (size_t)(d.slow + J * d.nti - J * (J - 1) / 2 + (I - J)) * TILE_ELEMS;
+  const P *src = pool + (size_t)(d.slow + J * d.nti - J * (J - 1) / 2 + (I - J)) * TILE_ELEMS;
   T *dst = ppool + (size_t)jdst[job] * TILE_ELEMS;
   const int tid = threadIdx.x;
   const int r0 = 2 * (tid & 31), c0 = tid >> 5;
@@ -117,7 +129,7 @@ __global__ __launch_bounds__(256) void k_sprime_gen(const double *__restrict__ p
 #pragma unroll 2
   for (int k = 0; k < 8; k++) {
     const int c = c0 + 8 * k;
-    const double2 v = *reinterpret_cast<const double2 *>(src + c * 64 + r0);
+    const double2 v = load2(src + c * 64 + r0);
     const int64_t b = d.loff + J * 64 + c;
     const double x = v.x + shared_rows(dptr, dcon, dval, omega, a0, b);
     const double y = v.y + shared_rows(dptr, dcon, dval, omega, a0 + 1, b);
@@ -125,20 +137,30 @@ __global__ __launch_bounds__(256) void k_sprime_gen(const double *__restrict__ p
   }
 }
 
-template <typename T>
-void launch_sprime_gen(hipStream_t st, const double *pool, T *ppool, const SubDesc *subs,
+template <typename P, typename T>
+void launch_sprime_gen(hipStream_t st, const P *pool, T *ppool, const SubDesc *subs,
                        const int *jobs_sub, const int *jobs_ij, const int *jobs_dst, int njobs,
                        const int *dptr, const int *dcon, const double *dval, const double *omega) {
   if (njobs <= 0) return;
-  hipLaunchKernelGGL(k_sprime_gen<T>, dim3(njobs), dim3(256), 0, st, pool, ppool, subs, jobs_sub,
-                     jobs_ij, jobs_dst, dptr, dcon, dval, omega);
+  hipLaunchKernelGGL((k_sprime_gen<P, T>), dim3(njobs), dim3(256), 0, st, pool, ppool, subs,
+                     jobs_sub, jobs_ij, jobs_dst, dptr, dcon, dval, omega);
 }
-template void launch_sprime_gen<double>(hipStream_t, const double *, double *, const SubDesc *,
-                                        const int *, const int *, const int *, int, const int *,
-                                        const int *, const double *, const double *);
-template void launch_sprime_gen<float>(hipStream_t, const double *, float *, const SubDesc *,
-                                       const int *, const int *, const int *, int, const int *,
-                                       const int *, const double *, const double *);
+template void launch_sprime_gen<double, double>(hipStream_t, const double *, double *,
+                                                const SubDesc *, const int *, const int *,
+                                                const int *, int, const int *, const int *,
+                                                const double *, const double *);
+template void launch_sprime_gen<double, float>(hipStream_t, const double *, float *,
+                                               const SubDesc *, const int *, const int *,
+                                               const int *, int, const int *, const int *,
+                                               const double *, const double *);
+template void launch_sprime_gen<float, float>(hipStream_t, const float *, float *, const SubDesc *,
+                                              const int *, const int *, const int *, int,
+                                              const int *, const int *, const double *,
+                                              const double *);
+template void launch_sprime_gen<float, double>(hipStream_t, const float *, double *,
+                                               const SubDesc *, const int *, const int *,
+                                               const int *, int, const int *, const int *,
+                                               const double *, const double *);
 
 // C^T tiles for general rows: B(i, c) = value of dof i in constraint c
 template <typename T>
@@ -376,8 +398,10 @@ __device__ __forceinline__ double row_reduce(double s, double *red) {
 // GV: the subdomain vector stays in global memory (L2-cached) instead of
 // LDS, for subdomains whose vector does not fit there; rows written by one
 // wave are published to the others with a workgroup fence before the barrier.
-template <int NW, bool GV>
-__global__ __launch_bounds__(64 * NW) void k_trsv_fwd(const double *__restrict__ pool,
+// P: element type of the factor tiles (FP64, or FP32 with the FP32 interior
+// factorization); the vector and the sums stay FP64.
+template <typename P, int NW, bool GV>
+__global__ __launch_bounds__(64 * NW) void k_trsv_fwd(const P *__restrict__ pool,
                                                   const TrsvDesc *__restrict__ ds,
                                                   const int *__restrict__ rptr,
                                                   const int *__restrict__ rk,
@@ -397,7 +421,7 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_fwd(const double *__restrict__
   for (int I = 0; I < nt; I++) {
     double s = 0.0;
     for (int q = rptr[d.rptr_off + I]; q < rptr[d.rptr_off + I + 1]; q++) {
-      const double *T = pool + (size_t)rt[q] * TILE_ELEMS + r;
+      const P *T = pool + (size_t)rt[q] * TILE_ELEMS + r;
       const double *x = v + rk[q] * 64;
 #pragma unroll 4
       for (int c = g; c < 64; c += NW) s += T[c * 64] * x[c];
@@ -406,7 +430,7 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_fwd(const double *__restrict__
     if (threadIdx.x < 64) t[r] = v[I * 64 + r] - tv;
     __syncthreads();
     if (I < d.nto) {
-      const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r;
+      const P *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r;
       double u = 0.0;
 #pragma unroll 4
       for (int c = g; c < 64; c += NW) u += L[c * 64] * t[c];
@@ -422,8 +446,8 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_fwd(const double *__restrict__
     for (int k = threadIdx.x; k < nt * 64; k += blockDim.x) gv[k] = v[k];
 }
 
-template <int NW, bool GV>
-__global__ __launch_bounds__(64 * NW) void k_trsv_bwd(const double *__restrict__ pool,
+template <typename P, int NW, bool GV>
+__global__ __launch_bounds__(64 * NW) void k_trsv_bwd(const P *__restrict__ pool,
                                                   const TrsvDesc *__restrict__ ds,
                                                   const int *__restrict__ cptr,
                                                   const int *__restrict__ ck,
@@ -448,7 +472,7 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_bwd(const double *__restrict__
     for (int I = d.nto - 1; I >= 0; I--) {
       double a[4] = {0.0, 0.0, 0.0, 0.0};
       for (int q = cptr[d.cptr_off + I]; q < cptr[d.cptr_off + I + 1]; q++) {
-        const double *T = pool + (size_t)ct[q] * TILE_ELEMS + l;
+        const P *T = pool + (size_t)ct[q] * TILE_ELEMS + l;
         const double xv = v[ck[q] * 64 + l];
 #pragma unroll
         for (int m = 0; m < 4; m++) a[m] += T[(g + 16 * m) * 64] * xv;
@@ -460,7 +484,7 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_bwd(const double *__restrict__
         for (int m = 0; m < 4; m++) t[g + 16 * m] = v[I * 64 + g + 16 * m] - a[m];
       }
       __syncthreads();
-      const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + l;
+      const P *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + l;
       const double tv = t[l];
 #pragma unroll
       for (int m = 0; m < 4; m++) a[m] = wave_sum(L[(g + 16 * m) * 64] * tv);
@@ -476,7 +500,7 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_bwd(const double *__restrict__
     double s = 0.0;
     for (int q = cptr[d.cptr_off + I]; q < cptr[d.cptr_off + I + 1]; q++) {
       // (L_kI^T v_k)[r] = sum_c L_kI(c, r) v_k[c], column r of the tile is contiguous
-      const double *T = pool + (size_t)ct[q] * TILE_ELEMS + r * 64 + g * CW;
+      const P *T = pool + (size_t)ct[q] * TILE_ELEMS + r * 64 + g * CW;
       const double *x = v + ck[q] * 64 + g * CW;
 #pragma unroll
       for (int c = 0; c < CW; c++) s += T[c] * x[c];
@@ -484,7 +508,7 @@ __global__ __launch_bounds__(64 * NW) void k_trsv_bwd(const double *__restrict__
     double tv = row_reduce<NW>(s, red);
     if (threadIdx.x < 64) t[r] = v[I * 64 + r] - tv;
     __syncthreads();
-    const double *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r * 64 + g * CW;
+    const P *L = pool + (size_t)(d.linv + I) * TILE_ELEMS + r * 64 + g * CW;
     double u = 0.0;
 #pragma unroll
     for (int c = 0; c < CW; c++) u += L[c] * t[g * CW + c];
@@ -533,9 +557,9 @@ int launch_forms() { return g_launch_forms; }
 static bool few_subdomains(int nsub) {
   return g_launch_forms == 0 ? nsub < kTrsvFewSubs : g_launch_forms == 2;
 }
-template <int NW, class K1, class K2>
+template <int NW, typename P, class K1, class K2>
 static void trsv_launch(hipStream_t st, int nsub, int maxnt, K1 lds_kernel, K2 gv_kernel,
-                        const double *pool, const TrsvDesc *d, const int *p, const int *k,
+                        const P *pool, const TrsvDesc *d, const int *p, const int *k,
                         const int *t, double *v) {
   const size_t sh = (size_t)(maxnt * 64 + NW * 64 + 64) * sizeof(double);
   if (sh <= max_dynamic_lds() && !g_trsv_global) {
@@ -546,26 +570,36 @@ static void trsv_launch(hipStream_t st, int nsub, int maxnt, K1 lds_kernel, K2 g
     hipLaunchKernelGGL(gv_kernel, dim3(nsub), dim3(64 * NW), sg, st, pool, d, p, k, t, v);
   }
 }
-void launch_trsv_fwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
-                     const int *rptr, const int *rk, const int *rt, double *v, int maxnt) {
+template <typename P>
+void launch_trsv_fwd(hipStream_t st, const P *pool, const TrsvDesc *d, int nsub, const int *rptr,
+                     const int *rk, const int *rt, double *v, int maxnt) {
   if (nsub <= 0) return;
   if (few_subdomains(nsub))
-    trsv_launch<16>(st, nsub, maxnt, k_trsv_fwd<16, false>, k_trsv_fwd<16, true>, pool, d, rptr,
-                    rk, rt, v);
+    trsv_launch<16>(st, nsub, maxnt, k_trsv_fwd<P, 16, false>, k_trsv_fwd<P, 16, true>, pool, d,
+                    rptr, rk, rt, v);
   else
-    trsv_launch<4>(st, nsub, maxnt, k_trsv_fwd<4, false>, k_trsv_fwd<4, true>, pool, d, rptr, rk,
-                   rt, v);
+    trsv_launch<4>(st, nsub, maxnt, k_trsv_fwd<P, 4, false>, k_trsv_fwd<P, 4, true>, pool, d, rptr,
+                   rk, rt, v);
 }
-void launch_trsv_bwd(hipStream_t st, const double *pool, const TrsvDesc *d, int nsub,
-                     const int *cptr, const int *ck, const int *ct, double *v, int maxnt) {
+template <typename P>
+void launch_trsv_bwd(hipStream_t st, const P *pool, const TrsvDesc *d, int nsub, const int *cptr,
+                     const int *ck, const int *ct, double *v, int maxnt) {
   if (nsub <= 0) return;
   if (few_subdomains(nsub))
-    trsv_launch<16>(st, nsub, maxnt, k_trsv_bwd<16, false>, k_trsv_bwd<16, true>, pool, d, cptr,
-                    ck, ct, v);
+    trsv_launch<16>(st, nsub, maxnt, k_trsv_bwd<P, 16, false>, k_trsv_bwd<P, 16, true>, pool, d,
+                    cptr, ck, ct, v);
   else
-    trsv_launch<4>(st, nsub, maxnt, k_trsv_bwd<4, false>, k_trsv_bwd<4, true>, pool, d, cptr, ck,
-                   ct, v);
+    trsv_launch<4>(st, nsub, maxnt, k_trsv_bwd<P, 4, false>, k_trsv_bwd<P, 4, true>, pool, d, cptr,
+                   ck, ct, v);
 }
+template void launch_trsv_fwd<double>(hipStream_t, const double *, const TrsvDesc *, int,
+                                      const int *, const int *, const int *, double *, int);
+template void launch_trsv_fwd<float>(hipStream_t, const float *, const TrsvDesc *, int, const int *,
+                                     const int *, const int *, double *, int);
+template void launch_trsv_bwd<double>(hipStream_t, const double *, const TrsvDesc *, int,
+                                      const int *, const int *, const int *, double *, int);
+template void launch_trsv_bwd<float>(hipStream_t, const float *, const TrsvDesc *, int, const int *,
+                                     const int *, const int *, double *, int);
 
 // -------------------------------------------------------- iteration kernels
 // y = S x for symmetric S held as its lower tiles.  Workgroup = tile column
@@ -1316,6 +1350,68 @@ void launch_from_device_order(hipStream_t st, const OrderDesc *d, int nsub, cons
   hipLaunchKernelGGL(k_from_device_order, dim3((maxndofs + 255) / 256, nsub), dim3(256), 0, st, d,
                      perm, vec, dst);
 }
+// r = f - A u of every subdomain in local dof order (FP64), A the subdomain
+// matrix as the tile scatter builds it (couplings of fixed dofs dropped, a
+// fixed diagonal below 1e-12 read as 1) from its assembled upper-triangle
+// triplets, sorted by row: entry (i, j, a) adds a u_j to row i and, off the
+// diagonal, a u_i to row j.  One workgroup per subdomain; the row sums
+// gather in LDS (ds_add_f64) when the subdomain vector fits there (G false),
+// else in r itself by device atomics, read back past the L1 cache.
+template <bool G>
+__global__ __launch_bounds__(512) void k_resid_local(const ScatterSub *__restrict__ subs,
+                                                     const OrderDesc *__restrict__ od,
+                                                     const char *__restrict__ fix,
+                                                     const double *__restrict__ f,
+                                                     const double *__restrict__ u,
+                                                     double *__restrict__ r) {
+  extern __shared__ double ys[];
+  const ScatterSub d = subs[blockIdx.x];
+  const OrderDesc o = od[blockIdx.x];
+  const double *uu = u + o.poff;
+  const char *fx = fix + d.foff;
+  double *yg = r + o.poff;
+  for (int k = threadIdx.x; k < o.ndofs; k += blockDim.x) {
+    if (G) yg[k] = 0.0;
+    else ys[k] = 0.0;
+  }
+  if (G) __threadfence();
+  __syncthreads();
+  auto add = [&](int i, double x) {
+    if constexpr (G) atomicAdd(yg + i, x);
+    else atomicAdd(ys + i, x);
+  };
+  for (int e = threadIdx.x; e < d.nnz; e += blockDim.x) {
+    const int i = d.ai[e], j = d.aj[e];
+    double a = d.av[e];
+    const bool fi = fx[i] != 0, fj = fx[j] != 0;
+    if (i != j) {
+      if (fi || fj) continue;
+      add(i, a * uu[j]);
+      add(j, a * uu[i]);
+    } else {
+      if (fi && fabs(a) < 1e-12) a = 1.0;
+      add(i, a * uu[i]);
+    }
+  }
+  if (G) __threadfence();
+  __syncthreads();
+  for (int k = threadIdx.x; k < o.ndofs; k += blockDim.x) {
+    const double y = G ? __hip_atomic_load(yg + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ys[k];
+    yg[k] = f[o.poff + k] - y;
+  }
+}
+void launch_resid_local(hipStream_t st, const ScatterSub *subs, const OrderDesc *od, int nsub,
+                        const char *fix, const double *f, const double *u, double *r, int maxndofs) {
+  if (nsub <= 0) return;
+  const size_t sh = (size_t)std::max(maxndofs, 1) * sizeof(double);
+  if (sh <= max_dynamic_lds() && !g_trsv_global) {
+    lds_request((const void *)k_resid_local<false>, sh, "residual");
+    hipLaunchKernelGGL(k_resid_local<false>, dim3(nsub), dim3(512), sh, st, subs, od, fix, f, u, r);
+  } else {
+    hipLaunchKernelGGL(k_resid_local<true>, dim3(nsub), dim3(512), 0, st, subs, od, fix, f, u, r);
+  }
+}
+
 __global__ void k_level_fill(int64_t n, const int *__restrict__ src, const double *__restrict__ w,
                              const double *__restrict__ rc, double *__restrict__ vec) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -485,9 +485,11 @@ __global__ __launch_bounds__(NTHR) void k_tile_mirror(double *__restrict__ pool,
   lds_to_tile(As, pool + (size_t)j.dst * TILE_ELEMS);
 }
 
-__global__ void k_zero_tiles(double *__restrict__ pool, const int *__restrict__ ids) {
-  double2 *T = reinterpret_cast<double2 *>(pool + (size_t)ids[blockIdx.x] * TILE_ELEMS);
-  for (int e = threadIdx.x; e < TILE_ELEMS / 2; e += blockDim.x) T[e] = make_double2(0.0, 0.0);
+template <typename T>
+__global__ void k_zero_tiles(T *__restrict__ pool, const int *__restrict__ ids) {
+  uint4 *Q = reinterpret_cast<uint4 *>(pool + (size_t)ids[blockIdx.x] * TILE_ELEMS);
+  constexpr int n16 = TILE_ELEMS * (int)sizeof(T) / 16;
+  for (int e = threadIdx.x; e < n16; e += blockDim.x) Q[e] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 __global__ void k_scatter_ones(double *__restrict__ pool, const int64_t *__restrict__ idx,
@@ -501,12 +503,13 @@ __global__ void k_scatter_ones(double *__restrict__ pool, const int64_t *__restr
 // (src/module_sm.f90:431-600), device permutation, both triangles inside
 // diagonal tiles; then 1.0 on the diagonal of every padding slot (slots no
 // triplet reaches).
-__global__ void k_scatter_sub(double *__restrict__ pool, const ScatterSub *__restrict__ subs,
+template <typename T>
+__global__ void k_scatter_sub(T *__restrict__ pool, const ScatterSub *__restrict__ subs,
                               const int *__restrict__ perm, const char *__restrict__ fix,
                               const int *__restrict__ ttab, const int *__restrict__ pads,
                               int *__restrict__ err) {
   const ScatterSub d = subs[blockIdx.y];
-  double *P = pool + (size_t)d.base * TILE_ELEMS;
+  T *P = pool + (size_t)d.base * TILE_ELEMS;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < d.nnz + d.npad; e += gridDim.x * blockDim.x) {
     if (e >= d.nnz) {
       P[pads[d.padoff + e - d.nnz]] = 1.0;
@@ -532,8 +535,8 @@ __global__ void k_scatter_sub(double *__restrict__ pool, const ScatterSub *__res
       atomicOr(err, 4);
       continue;
     }
-    P[(size_t)t * TILE_ELEMS + (b & 63) * 64 + (a & 63)] = v;
-    if (I == J && a != b) P[(size_t)t * TILE_ELEMS + (a & 63) * 64 + (b & 63)] = v;
+    P[(size_t)t * TILE_ELEMS + (b & 63) * 64 + (a & 63)] = (T)v;
+    if (I == J && a != b) P[(size_t)t * TILE_ELEMS + (a & 63) * 64 + (b & 63)] = (T)v;
   }
 }
 
@@ -545,13 +548,17 @@ __global__ void k_zero_ranges(double *__restrict__ pool, const int2 *__restrict_
     T[e] = make_double2(0.0, 0.0);
 }
 
-void launch_scatter_sub(hipStream_t st, double *pool, const ScatterSub *subs, int nsub,
-                        const int *perm, const char *fix, const int *ttab, const int *pads,
-                        int *err) {
+template <typename T>
+void launch_scatter_sub(hipStream_t st, T *pool, const ScatterSub *subs, int nsub, const int *perm,
+                        const char *fix, const int *ttab, const int *pads, int *err) {
   if (nsub <= 0) return;
-  hipLaunchKernelGGL(k_scatter_sub, dim3(64, nsub), dim3(256), 0, st, pool, subs, perm, fix, ttab,
+  hipLaunchKernelGGL(k_scatter_sub<T>, dim3(64, nsub), dim3(256), 0, st, pool, subs, perm, fix, ttab,
                      pads, err);
 }
+template void launch_scatter_sub<double>(hipStream_t, double *, const ScatterSub *, int, const int *,
+                                         const char *, const int *, const int *, int *);
+template void launch_scatter_sub<float>(hipStream_t, float *, const ScatterSub *, int, const int *,
+                                        const char *, const int *, const int *, int *);
 void launch_zero_ranges(hipStream_t st, double *pool, const int2 *ranges, int nranges) {
   if (nranges <= 0) return;
   hipLaunchKernelGGL(k_zero_ranges, dim3(64, nranges), dim3(256), 0, st, pool, ranges);
@@ -595,10 +602,12 @@ void launch_tile_mirror(hipStream_t st, double *pool, const MirrorJob *jobs, int
   if (njobs <= 0) return;
   hipLaunchKernelGGL(k_tile_mirror, dim3(njobs), dim3(NTHR), 0, st, pool, jobs);
 }
-void launch_zero_tiles(hipStream_t st, double *pool, const int *ids, int n) {
+template <typename T> void launch_zero_tiles(hipStream_t st, T *pool, const int *ids, int n) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_zero_tiles, dim3(n), dim3(256), 0, st, pool, ids);
+  hipLaunchKernelGGL(k_zero_tiles<T>, dim3(n), dim3(256), 0, st, pool, ids);
 }
+template void launch_zero_tiles<double>(hipStream_t, double *, const int *, int);
+template void launch_zero_tiles<float>(hipStream_t, float *, const int *, int);
 void launch_scatter_ones(hipStream_t st, double *pool, const int64_t *idx, int64_t n) {
   if (n <= 0) return;
   int64_t nb = (n + 255) / 256;

@@ -31,6 +31,11 @@ struct Timings {
   int gemm_launches = 0;
   double level2 = 0;  // second level: mesh, host setup and device setup (s)
   int iterations = 0;
+  // FP64 iterative refinement of the last solve: steps (corrections solved),
+  // final relative interface residual, wall time of the refinement part (s)
+  // and device time (ms) of one FP64 residual launch
+  int refinements = 0;
+  double refine_relres = 0, refine = 0, ev_resid_ms = 0, resid_bytes = 0;
 };
 
 struct Solver {
@@ -47,6 +52,14 @@ struct Solver {
   int use_corner = 1, use_arith = 1, weights_type = 0;
   int use_user = 0;  // user constraints on edges (first level)
   int precision = 64;  // 64: FP64 everywhere; 32: FP32 preconditioner factors
+  // FP32 interior factorization, Schur complements and S' factor on the
+  // first level, and an outer FP64 iterative refinement of every solve
+  // (residuals from the FP64 subdomain matrices): 0 off, 1 on.  The first
+  // correction is solved to refine_first (0: the square root of the
+  // tolerance), at most refine_max corrections follow.
+  int refine = 0;
+  double refine_first = 0.0;
+  int refine_max = 10;
   int ordering = 0;    // interior ordering: 0 tile-aligned nested dissection, 1 RCM
   int nd_leaf = 256;   // nested dissection: largest part (dofs) left undivided
   // local preconditioner through an explicit S'^{-1} (applied by a symmetric

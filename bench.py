@@ -237,10 +237,12 @@ def main():
                     help="PCG iterations from the third on replayed from one captured graph")
     ap.add_argument("--ordering", type=int, default=0,
                     help="interior ordering: 0 tile-aligned nested dissection, 1 RCM")
-    ap.add_argument("--precision", default="both", choices=("both", "64", "32"),
+    ap.add_argument("--precision", default="all", choices=("all", "both", "64", "32", "refined"),
                     help="64: FP64 throughout; 32: FP32 factor of the augmented interface "
-                         "matrices (mixed precision); both: FP64 is the reported value and "
-                         "the mixed run is added to the line")
+                         "matrices (mixed precision); refined: FP32 interior factorization, "
+                         "Schur complements and S' factor with FP64 iterative refinement; "
+                         "both: 64 and 32; all: 64, 32 and refined (FP64 is the reported "
+                         "value, the others are added to the line)")
     ap.add_argument("--nd-leaf", type=int, default=256,
                     help="nested dissection: largest part (dofs) left whole")
     args = ap.parse_args()
@@ -291,9 +293,12 @@ def main():
     log(f"generated {p.nsub} subdomains ({min(sizes)}..{max(sizes)} dofs), {p.ndof} dofs in "
         f"{time.perf_counter() - t0:.1f} s")
 
-    def run(precision):
-        """warmup + timed steps of one precision mode; each step is a fresh
-        init + upload (untimed) and setup_preconditioner + solve (timed)"""
+    def run(mode):
+        """warmup + timed steps of one precision mode (64, 32 or "refined");
+        each step is a fresh init + upload (untimed) and setup_preconditioner
+        + solve (timed)"""
+        precision = 64 if mode == 64 else 32
+        tag = "refined" if mode == "refined" else f"fp{precision}"
         step_times, results, tims = [], [], []
         for step in range(args.warmup + args.steps):
             b = bddcml_amd.Bddcml()
@@ -303,6 +308,7 @@ def main():
             b.set_option("interior_ordering", args.ordering)
             b.set_option("nd_leaf", args.nd_leaf)
             b.set_option("precision", precision)
+            b.set_option("refinement", 1 if mode == "refined" else 0)
             b.set_option("explicit_inverse", args.explicit_inverse)
             b.set_option("pcg_graph", args.pcg_graph)
             for kv in args.set:
@@ -320,18 +326,22 @@ def main():
             t2 = time.perf_counter()
             tm = b.timings()
             hist = b.residual_history()
-            if step == args.warmup + args.steps - 1:
+            if step == args.warmup + args.steps - 1 and mode != "refined":
                 # outside the timed region: the true residual of the interface
                 # problem, |g - S u| / |g|, with the converged solution
                 ui = b.interface_solution()
                 g = b.reduced_rhs()
                 tm["true_relres"] = float(np.linalg.norm(g - b.apply_schur(ui)) / np.linalg.norm(g))
-                log(f"[fp{precision}] true interface residual |g - S u|/|g| = "
+                log(f"[{tag}] true interface residual |g - S u|/|g| = "
                     f"{tm['true_relres']:.3e}")
+            elif mode == "refined":
+                # the refinement stops on the FP64 interface residual of the
+                # exact Schur complements (the library's own S is FP32 here)
+                tm["true_relres"] = tm["refine_relres"]
             b.finalize()
             if reason != 0:
                 raise SystemExit(f"PCG did not converge: reason {reason}")
-            log(f"[fp{precision}] step {step}: upload {t_up:.2f} s setup {t1 - t0:.3f} s "
+            log(f"[{tag}] step {step}: upload {t_up:.2f} s setup {t1 - t0:.3f} s "
                 f"solve {t2 - t1:.3f} s iters {it} cond {cond:.4f} final relres {hist[-1]:.3e}")
             log("  " + json.dumps({k: round(v, 4) for k, v in tm.items()}))
             if step >= args.warmup:
@@ -341,9 +351,12 @@ def main():
         ms = 1000.0 * float(np.mean(step_times))
         return ms, results[-1], tims[-1]
 
-    modes = [64, 32] if args.precision == "both" else [int(args.precision)]
+    modes = {"all": [64, 32, "refined"], "both": [64, 32], "64": [64], "32": [32],
+             "refined": ["refined"]}[args.precision]
     runs = {m: run(m) for m in modes}
     head = modes[0]
+    if head == "refined":
+        raise SystemExit("--precision refined reports beside FP64: use --precision all")
     ms, (iters, cond, relres), tm = runs[head]
     value = p.ndof / (ms / 1000.0)
     schur_tf = tm["schur_flops"] / (tm["ev_schur_ms"] * 1e-3) / 1e12
@@ -407,6 +420,37 @@ def main():
                                  "Schur operator, Krylov and coarse problem FP64"}
         log(f"mixed precision: {ms32:.1f} ms/step ({p.ndof / (ms32 / 1000.0):.3e} dofs/s), "
             f"{it32} iterations, final relres {rr32:.3e}")
+    if head == 64 and "refined" in runs:
+        msr, (itr, cr, rrr), tmr = runs["refined"]
+        gemm32 = (tmr["gemm_flops"] / (tmr["ev_gemm_ms"] * 1e-3) / 1e12
+                  if tmr["ev_gemm_ms"] > 0 else 0.0)
+        schur32 = tmr["schur_flops"] / (tmr["ev_schur_ms"] * 1e-3) / 1e12
+        resid_gbs = (tmr["resid_bytes"] / (tmr["ev_resid_ms"] * 1e-3) / 1e9
+                     if tmr["ev_resid_ms"] > 0 else 0.0)
+        factor64 = tm["ev_interior_ms"] + tm["ev_schur_ms"]
+        factor32 = tmr["ev_interior_ms"] + tmr["ev_schur_ms"]
+        line["refined"] = {
+            "value": p.ndof / (msr / 1000.0), "unit": "dofs/s", "ms_per_step": msr,
+            "dtype": "f32+f64", "iterations": itr, "corrections": int(tmr["refinements"]),
+            "final_interface_relres": tmr["refine_relres"],
+            "interior_factorization_ms": round(tmr["ev_interior_ms"], 2),
+            "schur_gemm_ms": round(tmr["ev_schur_ms"], 2),
+            "factorization_ms_fp64_path": round(factor64, 2),
+            "factorization_speedup_vs_fp64": round(factor64 / factor32, 3) if factor32 > 0 else None,
+            "interface_phase_ms": round(tmr["ev_interface_ms"], 2),
+            "krylov_s": round(tmr["krylov"], 4), "refinement_s": round(tmr["refine"], 4),
+            "postprocess_s": round(tmr["postprocess"], 4),
+            "interior_gemm_TFLOPs_f32": round(gemm32, 2),
+            "schur_gemm_TFLOPs_f32": round(schur32, 2),
+            "fp64_residual_ms": round(tmr["ev_resid_ms"], 4),
+            "fp64_residual_GBps": round(resid_gbs, 1),
+            "note": "FP32 interior factorization, Schur complements, S' factor and sweeps; "
+                    "FP64 residuals of the subdomain triplets, Krylov, coarse problem"}
+        log(f"refined FP32: {msr:.1f} ms/step ({p.ndof / (msr / 1000.0):.3e} dofs/s), {itr} "
+            f"iterations, {int(tmr['refinements'])} corrections, interface residual "
+            f"{tmr['refine_relres']:.3e}; interior {tmr['ev_interior_ms']:.1f} ms (GEMM "
+            f"{gemm32:.1f} TFLOP/s), Schur GEMM {tmr['ev_schur_ms']:.1f} ms ({schur32:.1f} "
+            f"TFLOP/s), FP64 residual {tmr['ev_resid_ms']:.3f} ms ({resid_gbs:.0f} GB/s)")
     if not args.no_cpu_baseline and args.workload == "cube":
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline()

@@ -46,7 +46,7 @@ int bddcml_amd_reduced_rhs(double *g);
 int bddcml_amd_interface_solution(double *u);
 /* relative residual history of the last solve; returns its length */
 int bddcml_amd_residual_history(double *out, int maxn);
-/* up to 20 values: wall seconds of topology plan factor krylov postprocess
+/* up to 29 values: wall seconds of topology plan factor krylov postprocess
  * setup_total solve_total rhs; device event milliseconds of the interior
  * factorisation, the Schur-block GEMM launch, the interface phase, the
  * coarse phase; algorithmic flops of the Schur-block launch, the interior
@@ -55,7 +55,10 @@ int bddcml_amd_residual_history(double *out, int maxn);
  * and algorithmic bytes of one local preconditioner launch; seconds of the
  * next levels' setup; the GEMM launches of the interior factorization
  * (k_tile_gemm on FP64 tiles): summed event milliseconds, summed algorithmic
- * flops (2*64^3 per tile product, post multiplies included) and their number */
+ * flops (2*64^3 per tile product, post multiplies included) and their number;
+ * with refinement: corrections solved, final relative interface residual,
+ * wall seconds of the refinement, event milliseconds and algorithmic bytes of
+ * one FP64 residual launch */
 int bddcml_amd_timings(double *out, int n);
 /* options (set after bddcml_init_c, which resets them):
  *   "verbose" (0/1/2);
@@ -71,6 +74,19 @@ int bddcml_amd_timings(double *out, int n);
  *     interface factor S', its triangular sweeps and Z = S'^{-1} C^T in FP32,
  *     while the Schur operator, the Krylov method and the coarse problem
  *     stay FP64);
+ *   "refinement" (1: the interior factorization, the Schur complements and
+ *     S' in FP32 on the first level -- FP32 tiles, FP32 products accumulated
+ *     in FP32 -- and every solve refined in FP64: the residual f - A u of the
+ *     subdomain problems from the uploaded triplets in FP64, condensed by the
+ *     forward sweep to the interface residual of the exact Schur complements,
+ *     on which the solve stops; each correction is solved on the FP32 Schur
+ *     complements.  The iteration count returned sums every correction's.
+ *     Not with adaptive constraints.  0: off, the default);
+ *   "refinement_first_tol" (with refinement: relative tolerance of the
+ *     first solve; 0, the default, takes the square root of the solve's
+ *     tolerance);
+ *   "refinement_max" (with refinement: most corrections after the first
+ *     solve, 10 by default);
  *   "task_order" (1: each XCD runs one contiguous range of a tile-task
  *     launch, the default; 0: plain workgroup order);
  *   "sweep_in_global" (1: the interior sweeps, the local preconditioner and

@@ -267,6 +267,10 @@ struct Device {
   std::vector<int> perm_off;            // offset of each subdomain's permutation in sc_perm
   View<char> sc_fix;
   View<ScatterSub> sc_subs;
+  // FP64 residual of the refinement: row-segmented triplet index per class
+  View<int> rs_ptr;
+  View<int2> rs_ent;
+  View<ResidSub> rs_subs;
   View<int> clear_ids;  // subdomain tiles cleared before the scatter
   View<int2> c_zero;                    // coarse tiles cleared before assembly
   View<int64_t> c_pads;                 // padding diagonals of the coarse matrix
@@ -926,6 +930,54 @@ void device_setup_begin(Solver &S) {
     D.plan.add(D.sc_ttab, ttab);
     D.plan.add(D.sc_pads, pads);
     D.plan.add(D.sc_subs, ss);
+    if (D.f32i) {
+      // row-segmented index of the triplets for the FP64 residual, built
+      // once per class (identical triplet structure): row r lists the
+      // entries (k, i) with aj[k] = r, i < r, then (k, j) with ai[k] = r
+      std::vector<size_t> rpo(nrep + 1, 0), eno(nrep + 1, 0);
+      for (int k = 0; k < nrep; k++) {
+        const Sub &s = S.subs[oreps[k]];
+        rpo[k + 1] = rpo[k] + s.ndofs + 1;
+        size_t ne = 0;
+        for (size_t q = 0; q < s.ai.size(); q++) ne += s.ai[q] == s.aj[q] ? 1 : 2;
+        eno[k + 1] = eno[k] + ne;
+      }
+      if (rpo[nrep] > (size_t)INT32_MAX) fatal("BDDCML_DEVICE_PLAN", "residual index too large");
+      std::vector<int> rptr(rpo[nrep]);
+      std::vector<int2> ent(eno[nrep]);
+#pragma omp parallel for schedule(dynamic)
+      for (int k = 0; k < nrep; k++) {
+        const Sub &s = S.subs[oreps[k]];
+        int *rp = rptr.data() + rpo[k];
+        int2 *E = ent.data() + eno[k];
+        std::vector<int> nt(s.ndofs, 0), nr(s.ndofs, 0);
+        for (size_t q = 0; q < s.ai.size(); q++) {
+          nr[s.ai[q]]++;
+          if (s.ai[q] != s.aj[q]) nt[s.aj[q]]++;
+        }
+        rp[0] = 0;
+        for (int r = 0; r < s.ndofs; r++) rp[r + 1] = rp[r] + nt[r] + nr[r];
+        std::vector<int> ct(s.ndofs), cr(s.ndofs);
+        for (int r = 0; r < s.ndofs; r++) ct[r] = rp[r], cr[r] = rp[r] + nt[r];
+        // triplets sorted by (row, column): both passes visit columns ascending
+        for (size_t q = 0; q < s.ai.size(); q++) {
+          const int i = s.ai[q], j = s.aj[q];
+          E[cr[i]++] = make_int2((int)q, j);
+          if (i != j) E[ct[j]++] = make_int2((int)q, i);
+        }
+      }
+      std::vector<ResidSub> rs(nsub);
+      for (int is = 0; is < nsub; is++) {
+        const int c = ocls[is];
+        rs[is] = ResidSub{S.din->av[is].p, (int64_t)foff[is], (int64_t)eno[c], (int)rpo[c],
+                          (int)foff[is], S.subs[is].ndofs, 0};
+        if (S.subs[is].ai.size() != S.subs[oreps[c]].ai.size())
+          fatal("BDDCML_DEVICE_PLAN", "triplet structure differs within a class");
+      }
+      D.plan.add(D.rs_ptr, rptr);
+      D.plan.add(D.rs_ent, ent);
+      D.plan.add(D.rs_subs, rs);
+    }
   }
 
   ph("scatter entries");
@@ -991,9 +1043,7 @@ void device_setup_begin(Solver &S) {
   };
   if (D.f32i) {
     interior(D.ipool32.p);
-    // S = A22 - W^T W: the same tasks on the FP32 tile GEMM
-    launch_tile_gemm(st, D.ipool32.p, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask,
-                     GEMM_NT, POST_NONE, 1.0, -1.0);
+    launch_schur_gemm(st, D.ipool32.p, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask);
   } else {
     interior(D.pool.p);
     launch_schur_gemm(st, D.pool.p, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask);
@@ -2712,13 +2762,21 @@ static void refine_solution(Solver &S, int method, double tol, int maxit, int nd
   if (D.ndofs_total > INT32_MAX) fatal("BDDCML_SOLVE", "too many subdomain dofs for the refinement");
   int total = *num_iter;
   double sc[4];
+  double tph = wall_time();
+  auto ph = [&](const char *name, int k) {
+    if (S.verbose < 2) return;
+    HIPCHECK(hipStreamSynchronize(st));
+    const double now = wall_time();
+    std::printf("   refinement step %d: %-28s %8.4f s\n", k, name, now - tph);
+    tph = now;
+  };
   for (int k = 0;; k++) {
     if (k == 0) {
       timing_events(D);
       HIPCHECK(hipEventRecord(D.ev_apply[4], st));
     }
-    launch_resid_local(st, D.sc_subs.p, D.odesc.p, S.nsub, D.sc_fix.p, D.fmod.p, D.sol.p, D.rloc.p,
-                       D.maxndofs);
+    launch_resid_rows(st, D.rs_subs.p, S.nsub, D.rs_ptr.p, D.rs_ent.p, D.sc_fix.p, D.fmod.p,
+                      D.sol.p, D.rloc.p, D.maxndofs);
     if (k == 0) HIPCHECK(hipEventRecord(D.ev_apply[5], st));
     reduced_rhs_launches(S, st, D.rloc.p, D.gk.p);
     HIPCHECK(hipMemcpyAsync(D.r.p, D.gk.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -2728,13 +2786,15 @@ static void refine_solution(Solver &S, int method, double tol, int maxit, int nd
       float ms = 0;
       HIPCHECK(hipEventElapsedTime(&ms, D.ev_apply[4], D.ev_apply[5]));
       S.t.ev_resid_ms = ms;
-      // triplets (row, column, value), flags, f, u and r of every dof
+      // triplet values, flags, f, u and r of every dof (the class indices
+      // are shared by the subdomains of a class)
       double nnz = 0;
       for (int is = 0; is < S.nsub; is++) nnz += (double)S.subs[is].ai.size();
-      S.t.resid_bytes = 16.0 * nnz + 25.0 * (double)D.ndofs_total;
+      S.t.resid_bytes = 8.0 * nnz + 25.0 * (double)D.ndofs_total;
     }
     const double rel = std::sqrt(sc[0]) / normres0;
     S.t.refine_relres = rel;
+    ph("residual and condensation", k);
     if (S.verbose > 0)
       std::printf(" BDDCML_SOLVE: refinement step %d: relative residual %.6e\n", k, rel);
     if (rel < tol) {
@@ -2755,9 +2815,12 @@ static void refine_solution(Solver &S, int method, double tol, int maxit, int nd
     for (size_t q = h0; q < S.hist.size(); q++) S.hist[q] *= nr / normres0;
     total += it;
     S.t.refinements++;
+    if (S.verbose >= 2) std::printf("   refinement step %d: correction of %d iterations\n", k, it);
+    ph("correction solve", k);
     back_substitute(S, D.rloc.p);
     launch_axpy2(st, (int)D.ndofs_total, 1.0, D.rloc.p, D.sol.p, 0.0, nullptr, nullptr);
     launch_axpy2(st, n, 1.0, D.uacc.p, D.u.p, 0.0, nullptr, nullptr);
+    ph("back substitution and update", k);
     if (rs == -2) {
       *reason = -2;
       break;

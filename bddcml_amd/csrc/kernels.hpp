@@ -38,7 +38,10 @@ void launch_symm_cols(hipStream_t st, const TP *pool, const SymmSub *subs, const
 // 1: tile-task launches hand each XCD one contiguous range of their task
 // list (default); 0: plain blockIdx order
 void set_task_order(int chunked);
+// S = A22 - sum W^T W over the Schur task list (FP64 or FP32 tiles)
 void launch_schur_gemm(hipStream_t st, double *pool, const TileTask *tasks, const TilePair *pairs,
+                       int ntasks);
+void launch_schur_gemm(hipStream_t st, float *pool, const TileTask *tasks, const TilePair *pairs,
                        int ntasks);
 // C = C - sum A B^T ; C = chol(C) (lower, upper zeroed) ; X = inv(C)
 template <typename T>
@@ -177,9 +180,22 @@ void launch_to_device_order(hipStream_t st, const OrderDesc *d, int nsub, const 
 void launch_from_device_order(hipStream_t st, const OrderDesc *d, int nsub, const int *perm,
                               const double *vec, double *dst, int maxndofs);
 // r = f - A u per subdomain in local dof order, FP64, A from the uploaded
-// triplets with the Dirichlet elimination of launch_scatter_sub
-void launch_resid_local(hipStream_t st, const ScatterSub *subs, const OrderDesc *od, int nsub,
-                        const char *fix, const double *f, const double *u, double *r, int maxndofs);
+// triplets (values av) with the Dirichlet elimination of launch_scatter_sub;
+// rowptr/ent (offsets rp_off/ent_off, shared by a class of identically
+// structured subdomains): per local row the (triplet index, column) pairs
+// of both triangles, columns ascending
+struct ResidSub {
+  const double *av;
+  int64_t poff;     // offset of the subdomain's dofs in local-order arrays
+  int64_t ent_off;  // offset of its class's entries
+  int rp_off;       // offset of its class's row pointers
+  int foff;         // offset of its Dirichlet flags
+  int ndofs;
+  int pad;
+};
+void launch_resid_rows(hipStream_t st, const ResidSub *subs, int nsub, const int *rowptr,
+                       const int2 *ent, const char *fix, const double *f, const double *u,
+                       double *r, int maxndofs);
 // every local copy of unique interface dof g: vec[pos[q]] = u[g], q in ptr[g]..ptr[g+1]
 // three levels: vec[p] = w[p] * rc[src[p]] (0 where src < 0), p < n; and
 // uc[g] = vec[pos[g]], g < ndofc

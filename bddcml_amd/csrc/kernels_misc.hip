@@ -1352,64 +1352,51 @@ void launch_from_device_order(hipStream_t st, const OrderDesc *d, int nsub, cons
 }
 // r = f - A u of every subdomain in local dof order (FP64), A the subdomain
 // matrix as the tile scatter builds it (couplings of fixed dofs dropped, a
-// fixed diagonal below 1e-12 read as 1) from its assembled upper-triangle
-// triplets, sorted by row: entry (i, j, a) adds a u_j to row i and, off the
-// diagonal, a u_i to row j.  One workgroup per subdomain; the row sums
-// gather in LDS (ds_add_f64) when the subdomain vector fits there (G false),
-// else in r itself by device atomics, read back past the L1 cache.
-template <bool G>
-__global__ __launch_bounds__(512) void k_resid_local(const ScatterSub *__restrict__ subs,
-                                                     const OrderDesc *__restrict__ od,
-                                                     const char *__restrict__ fix,
-                                                     const double *__restrict__ f,
-                                                     const double *__restrict__ u,
-                                                     double *__restrict__ r) {
-  extern __shared__ double ys[];
-  const ScatterSub d = subs[blockIdx.x];
-  const OrderDesc o = od[blockIdx.x];
-  const double *uu = u + o.poff;
-  const char *fx = fix + d.foff;
-  double *yg = r + o.poff;
-  for (int k = threadIdx.x; k < o.ndofs; k += blockDim.x) {
-    if (G) yg[k] = 0.0;
-    else ys[k] = 0.0;
-  }
-  if (G) __threadfence();
-  __syncthreads();
-  auto add = [&](int i, double x) {
-    if constexpr (G) atomicAdd(yg + i, x);
-    else atomicAdd(ys + i, x);
-  };
-  for (int e = threadIdx.x; e < d.nnz; e += blockDim.x) {
-    const int i = d.ai[e], j = d.aj[e];
-    double a = d.av[e];
-    const bool fi = fx[i] != 0, fj = fx[j] != 0;
-    if (i != j) {
-      if (fi || fj) continue;
-      add(i, a * uu[j]);
-      add(j, a * uu[i]);
-    } else {
-      if (fi && fabs(a) < 1e-12) a = 1.0;
-      add(i, a * uu[i]);
+// fixed diagonal below 1e-12 read as 1), from its assembled upper-triangle
+// triplets through a row-segmented index shared by the subdomain's class:
+// row i lists (triplet, column) of both triangles, columns ascending.
+// RESID_LANES lanes per row take every RESID_LANES-th entry and their sums
+// meet in a fixed butterfly, so the result does not depend on scheduling.
+constexpr int RESID_LANES = 8;
+__global__ __launch_bounds__(256) void k_resid_rows(const ResidSub *__restrict__ subs,
+                                                    const int *__restrict__ rowptr,
+                                                    const int2 *__restrict__ ent,
+                                                    const char *__restrict__ fix,
+                                                    const double *__restrict__ f,
+                                                    const double *__restrict__ u,
+                                                    double *__restrict__ r) {
+  const ResidSub d = subs[blockIdx.y];
+  const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) / RESID_LANES);
+  const int lane = threadIdx.x % RESID_LANES;
+  double s = 0.0;
+  if (row < d.ndofs) {
+    const int *rp = rowptr + d.rp_off;
+    const int2 *E = ent + d.ent_off;
+    const char *fx = fix + d.foff;
+    const double *uu = u + d.poff;
+    const bool fr = fx[row] != 0;
+    for (int e = rp[row] + lane; e < rp[row + 1]; e += RESID_LANES) {
+      const int2 q = E[e];
+      double a = d.av[q.x];
+      if (q.y != row) {
+        if (fr || fx[q.y]) continue;
+      } else if (fr && fabs(a) < 1e-12) {
+        a = 1.0;
+      }
+      s += a * uu[q.y];
     }
   }
-  if (G) __threadfence();
-  __syncthreads();
-  for (int k = threadIdx.x; k < o.ndofs; k += blockDim.x) {
-    const double y = G ? __hip_atomic_load(yg + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ys[k];
-    yg[k] = f[o.poff + k] - y;
-  }
+#pragma unroll
+  for (int m = 1; m < RESID_LANES; m <<= 1) s += __shfl_xor(s, m, RESID_LANES);
+  if (row < d.ndofs && lane == 0) r[d.poff + row] = f[d.poff + row] - s;
 }
-void launch_resid_local(hipStream_t st, const ScatterSub *subs, const OrderDesc *od, int nsub,
-                        const char *fix, const double *f, const double *u, double *r, int maxndofs) {
-  if (nsub <= 0) return;
-  const size_t sh = (size_t)std::max(maxndofs, 1) * sizeof(double);
-  if (sh <= max_dynamic_lds() && !g_trsv_global) {
-    lds_request((const void *)k_resid_local<false>, sh, "residual");
-    hipLaunchKernelGGL(k_resid_local<false>, dim3(nsub), dim3(512), sh, st, subs, od, fix, f, u, r);
-  } else {
-    hipLaunchKernelGGL(k_resid_local<true>, dim3(nsub), dim3(512), 0, st, subs, od, fix, f, u, r);
-  }
+void launch_resid_rows(hipStream_t st, const ResidSub *subs, int nsub, const int *rowptr,
+                       const int2 *ent, const char *fix, const double *f, const double *u,
+                       double *r, int maxndofs) {
+  if (nsub <= 0 || maxndofs <= 0) return;
+  const int64_t nthr = (int64_t)maxndofs * RESID_LANES;
+  hipLaunchKernelGGL(k_resid_rows, dim3((unsigned)((nthr + 255) / 256), nsub), dim3(256), 0, st,
+                     subs, rowptr, ent, fix, f, u, r);
 }
 
 __global__ void k_level_fill(int64_t n, const int *__restrict__ src, const double *__restrict__ w,

@@ -293,6 +293,13 @@ __global__ __launch_bounds__(NTHR) void k_schur_gemm(double *__restrict__ pool,
                                                      int chunked) {
   tile_gemm_task<double>(pool, tasks[task_index(chunked)], pairs, GEMM_NT, POST_NONE, 1.0, -1.0);
 }
+// the same update on FP32 tiles (FP32 interior factorization)
+__global__ __launch_bounds__(NTHR) void k_schur_gemm_f32(float *__restrict__ pool,
+                                                         const TileTask *__restrict__ tasks,
+                                                         const TilePair *__restrict__ pairs,
+                                                         int chunked) {
+  tile_gemm_task<float>(pool, tasks[task_index(chunked)], pairs, GEMM_NT, POST_NONE, 1.0, -1.0);
+}
 
 // acc += sum_{k < nk} A(r, k) B'(c, k) for one 16x16 block in the lane layout
 // of the 16x16x4 MFMA: lane l supplies A(l%16, q + l/16) and B'(l%16, q + l/16);
@@ -579,6 +586,12 @@ void launch_schur_gemm(hipStream_t st, double *pool, const TileTask *tasks, cons
                        int ntasks) {
   if (ntasks <= 0) return;
   hipLaunchKernelGGL(k_schur_gemm, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs,
+                     g_task_order);
+}
+void launch_schur_gemm(hipStream_t st, float *pool, const TileTask *tasks, const TilePair *pairs,
+                       int ntasks) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_schur_gemm_f32, dim3(ntasks), dim3(NTHR), 0, st, pool, tasks, pairs,
                      g_task_order);
 }
 template <typename T>

@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time-to-rel-residual 1e-8 (mixed vs FP64 GPU vs host CPU), dofs/s, HBM GB/s"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
@@ -355,9 +356,9 @@ def main():
              "refined": ["refined"]}[args.precision]
     runs = {m: run(m) for m in modes}
     head = modes[0]
-    if head == "refined":
-        raise SystemExit("--precision refined reports beside FP64: use --precision all")
     ms, (iters, cond, relres), tm = runs[head]
+    # the interior factorization's tiles: FP64, or FP32 in the refined form
+    gtype, gpeak = ("float", PEAK_FP32_TFLOPS) if head == "refined" else ("double", PEAK_FP64_TFLOPS)
     value = p.ndof / (ms / 1000.0)
     schur_tf = tm["schur_flops"] / (tm["ev_schur_ms"] * 1e-3) / 1e12
     gemv_gbs = tm["gemv_bytes"] / (tm["ev_gemv_ms"] * 1e-3) / 1e9 if tm["ev_gemv_ms"] > 0 else 0.0
@@ -374,36 +375,47 @@ def main():
     # event time, i.e. average flops per launch / average launch duration
     gemm_tf = tm["gemm_flops"] / (tm["ev_gemm_ms"] * 1e-3) / 1e12 if tm["ev_gemm_ms"] > 0 else 0.0
     nl = max(int(tm["gemm_launches"]), 1)
-    log(f"interior GEMM launches (k_tile_gemm<double>): {nl} launches, "
+    log(f"interior GEMM launches (k_tile_gemm<{gtype}>): {nl} launches, "
         f"{tm['gemm_flops'] / nl:.4e} flops and {tm['ev_gemm_ms'] / nl:.3f} ms per launch on "
         f"average ({gemm_tf:.1f} TFLOP/s)")
-    roofline = dict(bound="mfma", achieved=round(gemm_tf, 3), peak=PEAK_FP64_TFLOPS,
-                    unit="TFLOP/s", frac=round(gemm_tf / PEAK_FP64_TFLOPS, 4),
-                    traffic=profiled_traffic(args, "k_tile_gemm<double> interior"),
-                    kernel="k_tile_gemm<double> (interior factorization launches)",
+    schur_kernel = "k_schur_gemm" if head != "refined" else "k_schur_gemm_f32"
+    roofline = dict(bound="mfma", achieved=round(gemm_tf, 3), peak=gpeak,
+                    unit="TFLOP/s", frac=round(gemm_tf / gpeak, 4),
+                    traffic=profiled_traffic(args, f"k_tile_gemm<{gtype}> interior"),
+                    kernel=f"k_tile_gemm<{gtype}> (interior factorization launches)",
                     launches=nl,
                     flops_per_launch=tm["gemm_flops"] / nl,
                     ms_per_launch=round(tm["ev_gemm_ms"] / nl, 4),
-                    others={"k_schur_gemm": {"achieved": round(schur_tf, 3),
-                                             "frac": round(schur_tf / PEAK_FP64_TFLOPS, 4),
-                                             "flops_per_launch": tm["schur_flops"],
-                                             "ms_per_launch": round(tm["ev_schur_ms"], 3),
-                                             "traffic": profiled_traffic(args, "k_schur_gemm")},
-                            "k_symv<double>": {"achieved": round(gemv_gbs, 1), "unit": "GB/s",
-                                               "frac": round(gemv_gbs / PEAK_HBM_GBS, 4),
-                                               "bytes_per_launch": tm["gemv_bytes"]}})
+                    others={schur_kernel: {"achieved": round(schur_tf, 3),
+                                           "frac": round(schur_tf / gpeak, 4),
+                                           "flops_per_launch": tm["schur_flops"],
+                                           "ms_per_launch": round(tm["ev_schur_ms"], 3),
+                                           "traffic": profiled_traffic(args, schur_kernel)},
+                            f"k_symv<{gtype}>": {"achieved": round(gemv_gbs, 1), "unit": "GB/s",
+                                                 "frac": round(gemv_gbs / PEAK_HBM_GBS, 4),
+                                                 "bytes_per_launch": tm["gemv_bytes"]}})
+    if head == "refined" and tm["ev_resid_ms"] > 0:
+        rg = tm["resid_bytes"] / (tm["ev_resid_ms"] * 1e-3) / 1e9
+        roofline["others"]["k_resid_rows"] = {"achieved": round(rg, 1), "unit": "GB/s",
+                                              "frac": round(rg / PEAK_HBM_GBS, 4),
+                                              "bytes_per_launch": tm["resid_bytes"],
+                                              "ms_per_launch": round(tm["ev_resid_ms"], 4)}
     line = {
         "metric": METRIC, "value": value, "unit": "dofs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if head == 64 else "f64+f32",
+        "dtype": {64: "f64", 32: "f64+f32", "refined": "f32+f64"}[head],
         "data": "synthetic",
         "config": {"workload": workload,
                    "dofs": p.ndof, "subdomains": p.nsub, "iterations": iters,
                    "subdomain_dofs": [min(sizes), max(sizes)],
                    "condition_number": round(cond, 4), "final_relres": relres,
-                   "precision": "fp64" if head == 64 else
-                                "mixed: FP32 interface factor, FP64 Schur operator/Krylov/coarse",
+                   "precision": {64: "fp64",
+                                 32: "mixed: FP32 interface factor, FP64 Schur operator/Krylov/"
+                                     "coarse",
+                                 "refined": "refined: FP32 interior factorization, Schur "
+                                            "complements and interface factor, FP64 iterative "
+                                            "refinement"}[head],
                    "s_gemv_GBps": round(gemv_gbs, 1), "hbm_copy_GBps": round(copy_gbs, 1),
                    "pc_apply_GBps": round(kapply_gbs, 1),
                    "true_interface_relres": tm.get("true_relres")},
@@ -441,6 +453,7 @@ def main():
             "krylov_s": round(tmr["krylov"], 4), "refinement_s": round(tmr["refine"], 4),
             "postprocess_s": round(tmr["postprocess"], 4),
             "interior_gemm_TFLOPs_f32": round(gemm32, 2),
+            "interior_gemm_frac_f32": round(gemm32 / PEAK_FP32_TFLOPS, 4),
             "schur_gemm_TFLOPs_f32": round(schur32, 2),
             "fp64_residual_ms": round(tmr["ev_resid_ms"], 4),
             "fp64_residual_GBps": round(resid_gbs, 1),

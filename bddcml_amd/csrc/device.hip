@@ -250,6 +250,7 @@ struct Device {
   int64_t ntilesc = 0;
   Arena plan2;            // task lists and job lists of the second half
   Arena plan3;            // coarse factorization lists (built during the interface phase)
+  Arena plan4;            // residual index of the refined form (built during it too)
   DBuf<int> err;
   std::vector<SubTiles> sub;
   // setup phases
@@ -930,54 +931,6 @@ void device_setup_begin(Solver &S) {
     D.plan.add(D.sc_ttab, ttab);
     D.plan.add(D.sc_pads, pads);
     D.plan.add(D.sc_subs, ss);
-    if (D.f32i) {
-      // row-segmented index of the triplets for the FP64 residual, built
-      // once per class (identical triplet structure): row r lists the
-      // entries (k, i) with aj[k] = r, i < r, then (k, j) with ai[k] = r
-      std::vector<size_t> rpo(nrep + 1, 0), eno(nrep + 1, 0);
-      for (int k = 0; k < nrep; k++) {
-        const Sub &s = S.subs[oreps[k]];
-        rpo[k + 1] = rpo[k] + s.ndofs + 1;
-        size_t ne = 0;
-        for (size_t q = 0; q < s.ai.size(); q++) ne += s.ai[q] == s.aj[q] ? 1 : 2;
-        eno[k + 1] = eno[k] + ne;
-      }
-      if (rpo[nrep] > (size_t)INT32_MAX) fatal("BDDCML_DEVICE_PLAN", "residual index too large");
-      std::vector<int> rptr(rpo[nrep]);
-      std::vector<int2> ent(eno[nrep]);
-#pragma omp parallel for schedule(dynamic)
-      for (int k = 0; k < nrep; k++) {
-        const Sub &s = S.subs[oreps[k]];
-        int *rp = rptr.data() + rpo[k];
-        int2 *E = ent.data() + eno[k];
-        std::vector<int> nt(s.ndofs, 0), nr(s.ndofs, 0);
-        for (size_t q = 0; q < s.ai.size(); q++) {
-          nr[s.ai[q]]++;
-          if (s.ai[q] != s.aj[q]) nt[s.aj[q]]++;
-        }
-        rp[0] = 0;
-        for (int r = 0; r < s.ndofs; r++) rp[r + 1] = rp[r] + nt[r] + nr[r];
-        std::vector<int> ct(s.ndofs), cr(s.ndofs);
-        for (int r = 0; r < s.ndofs; r++) ct[r] = rp[r], cr[r] = rp[r] + nt[r];
-        // triplets sorted by (row, column): both passes visit columns ascending
-        for (size_t q = 0; q < s.ai.size(); q++) {
-          const int i = s.ai[q], j = s.aj[q];
-          E[cr[i]++] = make_int2((int)q, j);
-          if (i != j) E[ct[j]++] = make_int2((int)q, i);
-        }
-      }
-      std::vector<ResidSub> rs(nsub);
-      for (int is = 0; is < nsub; is++) {
-        const int c = ocls[is];
-        rs[is] = ResidSub{S.din->av[is].p, (int64_t)foff[is], (int64_t)eno[c], (int)rpo[c],
-                          (int)foff[is], S.subs[is].ndofs, 0};
-        if (S.subs[is].ai.size() != S.subs[oreps[c]].ai.size())
-          fatal("BDDCML_DEVICE_PLAN", "triplet structure differs within a class");
-      }
-      D.plan.add(D.rs_ptr, rptr);
-      D.plan.add(D.rs_ent, ent);
-      D.plan.add(D.rs_subs, rs);
-    }
   }
 
   ph("scatter entries");
@@ -1087,6 +1040,66 @@ static void reduced_rhs_launches(Solver &S, hipStream_t st, const double *src = 
     launch_trsv_fwd(st, D.pool.p, D.trsv.p, S.nsub, D.rptr.p, D.rk.p, D.rt.p, D.vec.p, D.maxnt);
   launch_assemble(st, S.nig, D.aptr.p, D.rhs_src.p, nullptr, D.vec.p, g ? g : D.g.p);
   D.vec_y = true;
+}
+
+// Row-segmented index of the triplets for the FP64 residual of the refined
+// form, built once per class (identical triplet structure): row r lists the
+// entries (k, i) with aj[k] = r, i < r, then (k, j) with ai[k] = r.  Built
+// and uploaded (own arena, upload stream) while the device runs the
+// interface phase; the solve waits for the upload.
+static void build_resid_index(Solver &S) {
+  Device &D = *S.dev;
+  const int nsub = S.nsub;
+  const std::vector<int> &ocls = S.ord_cls, &oreps = S.ord_reps;
+  const int nrep = (int)oreps.size();
+  std::vector<size_t> rpo(nrep + 1, 0), eno(nrep + 1, 0);
+  for (int k = 0; k < nrep; k++) {
+    const Sub &s = S.subs[oreps[k]];
+    rpo[k + 1] = rpo[k] + s.ndofs + 1;
+    size_t ne = 0;
+    for (size_t q = 0; q < s.ai.size(); q++) ne += s.ai[q] == s.aj[q] ? 1 : 2;
+    eno[k + 1] = eno[k] + ne;
+  }
+  if (rpo[nrep] > (size_t)INT32_MAX) fatal("BDDCML_DEVICE_PLAN", "residual index too large");
+  std::vector<int> rptr(rpo[nrep]);
+  std::vector<int2> ent(eno[nrep]);
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < nrep; k++) {
+    const Sub &s = S.subs[oreps[k]];
+    int *rp = rptr.data() + rpo[k];
+    int2 *E = ent.data() + eno[k];
+    std::vector<int> nt(s.ndofs, 0), nr(s.ndofs, 0);
+    for (size_t q = 0; q < s.ai.size(); q++) {
+      nr[s.ai[q]]++;
+      if (s.ai[q] != s.aj[q]) nt[s.aj[q]]++;
+    }
+    rp[0] = 0;
+    for (int r = 0; r < s.ndofs; r++) rp[r + 1] = rp[r] + nt[r] + nr[r];
+    std::vector<int> ct(s.ndofs), cr(s.ndofs);
+    for (int r = 0; r < s.ndofs; r++) ct[r] = rp[r], cr[r] = rp[r] + nt[r];
+    // triplets sorted by (row, column): both passes visit columns ascending
+    for (size_t q = 0; q < s.ai.size(); q++) {
+      const int i = s.ai[q], j = s.aj[q];
+      E[cr[i]++] = make_int2((int)q, j);
+      if (i != j) E[ct[j]++] = make_int2((int)q, i);
+    }
+  }
+  std::vector<ResidSub> rs(nsub);
+  int64_t foff = 0;
+  for (int is = 0; is < nsub; is++) {
+    const int c = ocls[is];
+    if (S.subs[is].ai.size() != S.subs[oreps[c]].ai.size() ||
+        (int64_t)S.din->av[is].n != (int64_t)S.subs[is].ai.size())
+      fatal("BDDCML_DEVICE_PLAN", "triplet structure differs within a class");
+    if (foff != D.hodesc[is].poff) fatal("BDDCML_DEVICE_PLAN", "local-order offsets");
+    rs[is] = ResidSub{S.din->av[is].p, foff, (int64_t)eno[c], (int)rpo[c], (int)foff,
+                      S.subs[is].ndofs, 0};
+    foff += S.subs[is].ndofs;
+  }
+  D.plan4.add(D.rs_ptr, rptr);
+  D.plan4.add(D.rs_ent, ent);
+  D.plan4.add(D.rs_subs, rs);
+  D.plan4.commit(D.up);
 }
 
 void device_setup_finish(Solver &S) {
@@ -1831,6 +1844,11 @@ void device_setup_finish(Solver &S) {
     launch_tile_mirror(st, cpool, D.mir_c.p, (int)D.mir_c.n);
   }
   HIPCHECK(hipEventRecord(ev[4], st));
+  if (D.f32i) {
+    const double tr = wall_time();
+    build_resid_index(S);
+    S.t.plan += wall_time() - tr;
+  }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(st));
   {
@@ -2411,21 +2429,31 @@ void device_fetch_solution(Solver &S) {
   S.sol_on_device = false;
 }
 
+// u0 on the unique interface: each dof's value from its copy in the
+// lowest-numbered subdomain holding it (the fixed value at Dirichlet dofs),
+// gathered in parallel into the pinned staging area and sent in one copy
 static void initial_guess(Solver &S) {
   Device &D = *S.dev;
   device_fetch_solution(S);
-  std::vector<double> u0(S.nig, 0.0);
-  std::vector<char> done(S.nig, 0);
-  for (int is = 0; is < S.nsub; is++) {
-    const Sub &s = S.subs[is];
-    for (int d : s.ifc_dof) {
-      int g = S.gpos[s.isvgvn[d]];
-      if (done[g]) continue;
-      done[g] = 1;
-      u0[g] = (s.has_fix && s.ifix[d]) ? s.fixv[d] : s.sol[d];
+  if ((int)S.ig_ptr.size() != S.nig + 1) fatal("BDDCML_SOLVE", "interface numbering missing");
+  Staging &h = staging();
+  h.used = 0;
+  h.grow((size_t)S.nig * sizeof(double));
+  double *u0 = reinterpret_cast<double *>(h.p);
+#pragma omp parallel for schedule(static, 4096)
+  for (int g = 0; g < S.nig; g++) {
+    double v = 0.0;
+    if (S.ig_ptr[g] < S.ig_ptr[g + 1]) {
+      const int q = S.ig_ptr[g];
+      const Sub &s = S.subs[S.ig_sub[q]];
+      const int d = s.ifc_dof[S.ig_k[q]];
+      v = (s.has_fix && s.ifix[d]) ? s.fixv[d] : s.sol[d];
     }
+    u0[g] = v;
   }
-  HIPCHECK(hipMemcpyAsync(D.u.p, u0.data(), S.nig * sizeof(double), hipMemcpyHostToDevice, D.st));
+  HIPCHECK(hipMemcpyAsync(D.u.p, u0, S.nig * sizeof(double), hipMemcpyHostToDevice, D.st));
+  // the staging area is reused by the next host transfer
+  HIPCHECK(hipStreamSynchronize(D.st));
 }
 
 // ---------------------------------------------------- PCG iteration graph

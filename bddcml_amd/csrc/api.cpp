@@ -165,23 +165,44 @@ void bddcml_upload_subdomain_data_c(
   s.kdof.assign(*nnods + 1, 0);
   for (int k = 0; k < *nnods; k++) s.kdof[k + 1] = s.kdof[k] + nndf[k];
   if (s.kdof[*nnods] != *ndofs) fatal(R, "sum of NNDF differs from NDOFS");
-  // assemble triplets: sort by (row, col), sum duplicates (src/module_sm.f90:630-685)
+  // assemble triplets: sort by (row, col), sum duplicates (src/module_sm.f90:630-685).
+  // A stable counting sort by row, then a stable insertion sort of each
+  // row's short list by column: O(la) for element-assembled input, and
+  // duplicates are summed in input order
   const int n = *la;
-  std::vector<int> ord(n);
-  std::iota(ord.begin(), ord.end(), 0);
-  for (int k = 0; k < n; k++)
+  const int nd = *ndofs;
+  for (int k = 0; k < n; k++) {
     if (i_sparse[k] > j_sparse[k])
       fatal(R, "Matrix in wrong format - only upper triangle is expected for symmetric problems, matrixtype = 1");
-  std::sort(ord.begin(), ord.end(), [&](int x, int y) {
-    return i_sparse[x] != i_sparse[y] ? i_sparse[x] < i_sparse[y] : j_sparse[x] < j_sparse[y];
-  });
+    const int i = i_sparse[k] - b, j = j_sparse[k] - b;
+    if (i < 0 || j < 0 || i >= nd || j >= nd) fatal(R, "matrix index out of range");
+  }
+  std::vector<int> rowp(nd + 1, 0), ord(n);
+  for (int k = 0; k < n; k++) rowp[i_sparse[k] - b + 1]++;
+  for (int r = 0; r < nd; r++) rowp[r + 1] += rowp[r];
+  {
+    std::vector<int> fill(rowp.begin(), rowp.end() - 1);
+    for (int k = 0; k < n; k++) ord[fill[i_sparse[k] - b]++] = k;
+  }
+  for (int r = 0; r < nd; r++) {
+    if (rowp[r + 1] - rowp[r] > 64) {  // long rows: merge sort
+      std::stable_sort(ord.begin() + rowp[r], ord.begin() + rowp[r + 1],
+                       [&](int x, int y) { return j_sparse[x] < j_sparse[y]; });
+      continue;
+    }
+    for (int q = rowp[r] + 1; q < rowp[r + 1]; q++) {
+      const int k = ord[q], jk = j_sparse[k];
+      int p = q;
+      for (; p > rowp[r] && j_sparse[ord[p - 1]] > jk; p--) ord[p] = ord[p - 1];
+      ord[p] = k;
+    }
+  }
   s.ai.reserve(n);
   s.aj.reserve(n);
   s.av.reserve(n);
   for (int q = 0; q < n; q++) {
-    int k = ord[q];
-    int i = i_sparse[k] - b, j = j_sparse[k] - b;
-    if (i < 0 || j < 0 || i >= *ndofs || j >= *ndofs) fatal(R, "matrix index out of range");
+    const int k = ord[q];
+    const int i = i_sparse[k] - b, j = j_sparse[k] - b;
     if (!s.ai.empty() && s.ai.back() == i && s.aj.back() == j)
       s.av.back() += a_sparse[k];
     else

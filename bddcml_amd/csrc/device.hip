@@ -210,6 +210,17 @@ struct SubTiles {
   // slow: tile (nto, nto), the first of S's lower tiles; the interface tile
   // columns are dense, so tile (nto + I, nto + J) is slow + lowidx(nti, I, J)
   int linvA = 0, slow = 0, pbase = 0, linvP = 0, ybase = 0, ncT = 0, gtbase = 0, phibase = 0;
+  // explicit interior inverse M = L11^{-1} (levels below the first): its
+  // strictly lower tiles are nonzero where row I is an ancestor of column J
+  // in the tile elimination tree; by interior column J, rows mrow[mptr[J] ..
+  // mptr[J+1]) ascending, tile mbase + q.  The diagonal tiles are Linv_A.
+  int mbase = 0;
+  std::vector<int> mptr, mrow;
+  int M(int I, int J) const {  // tile of M(I, J), I > J, or -1
+    auto b = mrow.begin() + mptr[J], e = mrow.begin() + mptr[J + 1];
+    auto it = std::lower_bound(b, e, I);
+    return (it == e || *it != I) ? -1 : mbase + (int)(it - mrow.begin());
+  }
   int P(int I, int J) const { return pbase + lowidx(nti, I, J); }
   int Y(int I, int cT) const { return ybase + I * ncT + cT; }
 };
@@ -255,6 +266,14 @@ struct Device {
   std::vector<SubTiles> sub;
   // setup phases
   std::vector<Phase> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd, w_rows;
+  // explicit interior inverse (levels below the first): its formation steps,
+  // the four GEMV lists of the interior solves, their work vectors
+  bool xint = false;
+  std::vector<Phase> m_rows;
+  View<GemvTask> gx_t[4];
+  View<GemvPair> gx_p[4];
+  int64_t gx_vt = 0;
+  DBuf<double> ybuf, zbuf;
   Phase a_schur, cinv, phi, kinv;
   // explicit inverse of S': planned (W tiles in the factor space, task
   // lists, GEMV columns), formed (W = L^{-1}, W^T W: the apply is a GEMV),
@@ -447,7 +466,7 @@ static void symbolic(const Sub &s, SubTiles &T) {
   }
 }
 
-static void assign_tiles(SubTiles &T, int &next) {
+static void assign_tiles(SubTiles &T, int &next, bool explicit_interior) {
   const int nt = T.nto + T.nti;
   T.base = next;
   T.pat.nt = nt;
@@ -465,6 +484,25 @@ static void assign_tiles(SubTiles &T, int &next) {
   }
   T.linvA = next;
   next += T.nto;
+  T.mptr.assign(T.nto + 1, 0);
+  T.mrow.clear();
+  if (explicit_interior) {
+    // ancestors of each interior tile column in the elimination tree (the
+    // parent: the first interior row below the diagonal of the filled column)
+    std::vector<int> parent(T.nto, -1);
+    for (int J = 0; J < T.nto; J++)
+      for (int I = J + 1; I < T.nto; I++)
+        if (bs_get(T.col[J], I)) {
+          parent[J] = I;
+          break;
+        }
+    for (int J = 0; J < T.nto; J++) {
+      for (int I = parent[J]; I >= 0; I = parent[I]) T.mrow.push_back(I);
+      T.mptr[J + 1] = (int)T.mrow.size();
+    }
+    T.mbase = next;
+    next += (int)T.mrow.size();
+  }
   T.slow = T.pat.find(T.nto, T.nto);
   T.span = next - T.base;
 }
@@ -667,7 +705,8 @@ void device_setup_begin(Solver &S) {
   int next = 0;
   D.f32i = S.refine != 0;
   D.mixed = S.precision == 32 || D.f32i;
-  for (int is = 0; is < nsub; is++) assign_tiles(D.sub[is], next);
+  D.xint = S.explicit_interior;
+  for (int is = 0; is < nsub; is++) assign_tiles(D.sub[is], next, D.xint);
   D.ntiles = next;
 
   ph("symbolic");
@@ -707,6 +746,7 @@ void device_setup_begin(Solver &S) {
   const int ncls = (int)reps.size();
   struct SubTasks {
     std::vector<TaskList> a_diag, a_off;
+    std::vector<TaskList> m_rows;  // explicit interior inverse, one step per tile row
     TaskList a_schur;
     std::vector<int> clear;  // pattern tiles (relative) that start as zeros
   };
@@ -774,6 +814,28 @@ void device_setup_begin(Solver &S) {
         t.p1 = (int)Q.a_schur.pairs.size();
         if (t.p1 > t.p0) Q.a_schur.tasks.push_back(t);
       }
+    if (D.xint) {
+      // M = L11^{-1} row by row: M(I, J) = -Linv_I sum_{J <= K < I} L(I, K) M(K, J),
+      // M(J, J) = Linv_J; every task starts from zero
+      Q.m_rows.resize(nto);
+      for (int I = 0; I < nto; I++)
+        for (int J = 0; J < I; J++) {
+          const int c = T.M(I, J);
+          if (c < 0) continue;
+          TaskList &L = Q.m_rows[I];
+          TileTask t{c, T.linvA + I, (int)L.pairs.size(), 0};
+          t.fresh = 1;
+          for (int K = J; K < I; K++) {
+            const int lik = find(I, K);
+            const int mkj = K == J ? T.linvA + J : T.M(K, J);
+            if (lik >= 0 && mkj >= 0) L.pairs.push_back({lik, mkj});
+          }
+          t.p1 = (int)L.pairs.size();
+          L.tasks.push_back(t);
+        }
+      for (TaskList &L : Q.m_rows)
+        if (!relative_ids(L, T.base, T.span)) bad_ids |= 1;
+    }
     // Tiles without matrix entries whose task writes them whole are
     // "fresh": the task starts from zero and the tile is never cleared.
     // Entry tiles (and the padding diagonals) are cleared and scattered;
@@ -825,6 +887,8 @@ void device_setup_begin(Solver &S) {
                         a_off = merge_steps(sts, &SubTasks::a_off, cls, base_of);
   TaskList a_schur = materialize(
       sts, [](const SubTasks &Q) -> const TaskList * { return &Q.a_schur; }, cls, base_of);
+  std::vector<TaskList> m_rows;
+  if (D.xint) m_rows = merge_steps(sts, &SubTasks::m_rows, cls, base_of);
   {
     // tiles to clear before the scatter, per subdomain
     std::vector<size_t> off(nsub + 1, 0);
@@ -847,6 +911,58 @@ void device_setup_begin(Solver &S) {
   upload_phases(D.a_diag, a_diag, D.plan);
   upload_phases(D.a_off, a_off, D.plan);
   D.a_schur.upload(a_schur, D.plan);
+  if (D.xint) {
+    upload_phases(D.m_rows, m_rows, D.plan);
+    // the four GEMVs of the interior solves, per subdomain (absolute tile
+    // ids, offsets into the device-ordered subdomain vectors):
+    // 0: y1 = M f1; 1: f2 -= L21 y1; 2: z1 = y1 - L21^T u2; 3: u1 = M^T z1
+    std::vector<GemvTask> gt[4];
+    std::vector<GemvPair> gp[4];
+    int64_t vo = 0;
+    for (int is = 0; is < nsub; is++) {
+      const SubTiles &T = D.sub[is];
+      const int nto = T.nto, nt = T.nto + T.nti;
+      auto task = [&](int k, int row) {
+        gt[k].push_back(GemvTask{vo + 64 * row, vo + 64 * row, (int)gp[k].size(), 0});
+      };
+      auto pair = [&](int k, int tile, int col) { gp[k].push_back(GemvPair{tile, 0, vo + 64 * col}); };
+      auto close = [&](int k) { gt[k].back().p1 = (int)gp[k].size(); };
+      for (int I = 0; I < nto; I++) {
+        task(0, I);
+        for (int J = 0; J < I; J++)
+          if (T.M(I, J) >= 0) pair(0, T.M(I, J), J);
+        pair(0, T.linvA + I, I);
+        close(0);
+      }
+      for (int I = nto; I < nt; I++) {
+        task(1, I);
+        for (int K = 0; K < nto; K++)
+          if (T.pat.find(I, K) >= 0) pair(1, T.pat.find(I, K), K);
+        close(1);
+      }
+      for (int J = 0; J < nto; J++) {
+        task(2, J);
+        for (int I = nto; I < nt; I++)
+          if (T.pat.find(I, J) >= 0) pair(2, T.pat.find(I, J), I);
+        close(2);
+        task(3, J);
+        pair(3, T.linvA + J, J);
+        for (int q = T.mptr[J]; q < T.mptr[J + 1]; q++) pair(3, T.mbase + q, T.mrow[q]);
+        close(3);
+      }
+      vo += (int64_t)S.subs[is].NO + S.subs[is].NI;
+    }
+    for (int k = 0; k < 4; k++) {
+      for (const GemvPair &p : gp[k])
+        if (p.tile < 0 || p.tile >= D.ntiles || p.xoff < 0 || p.xoff + 64 > vo)
+          fatal("BDDCML_DEVICE_PLAN", "interior GEMV pair out of range");
+      for (const GemvTask &t : gt[k])
+        if (t.yoff < 0 || t.yoff + 64 > vo) fatal("BDDCML_DEVICE_PLAN", "interior GEMV task out of range");
+      D.plan.add(D.gx_t[k], gt[k]);
+      D.plan.add(D.gx_p[k], gp[k]);
+    }
+    D.gx_vt = vo;
+  }
 
   ph("task lists");
   // ---- matrix entries: the assembled triplets are already on the device
@@ -967,7 +1083,8 @@ void device_setup_begin(Solver &S) {
                 (long long)D.ntiles, D.ntiles * 32768.0 / 1e9, (size_t)D.sc_pads.n);
 
   S.t.schur_flops = kTileGemm * tile_products(a_schur);
-  S.t.interior_flops = tile_work(a_diag, 2 * kTileChol) + tile_work(a_off, kTileGemm) + S.t.schur_flops;
+  S.t.interior_flops = tile_work(a_diag, 2 * kTileChol) + tile_work(a_off, kTileGemm) +
+                       tile_work(m_rows, kTileGemm) + S.t.schur_flops;
   // ---- numerics, first half: Dirichlet elimination into the tiles,
   // interior factorization, Schur complements
   hipEvent_t *ev = D.ev;
@@ -1001,6 +1118,8 @@ void device_setup_begin(Solver &S) {
     interior(D.pool.p);
     launch_schur_gemm(st, D.pool.p, D.a_schur.tasks.p, D.a_schur.pairs.p, D.a_schur.ntask);
   }
+  for (const Phase &m : D.m_rows)
+    launch_tile_gemm(st, D.pool.p, m.tasks.p, m.pairs.p, m.ntask, GEMM_NN, POST_LEFT_X, 1.0, -1.0);
   HIPCHECK(hipEventRecord(ev[2], st));
   HIPCHECK(hipGetLastError());
   S.t.plan += wall_time() - t0;
@@ -1505,6 +1624,13 @@ void device_setup_finish(Solver &S) {
     int64_t vt = 0;
     for (int is = 0; is < nsub; is++) voff[is] = vt, vt += S.subs[is].NO + S.subs[is].NI;
     D.VT = vt;
+    if (D.xint) {
+      if (D.gx_vt != vt) fatal("BDDCML_DEVICE_PLAN", "interior GEMV offsets");
+      D.ybuf.alloc(std::max<int64_t>(vt, 1));
+      D.zbuf.alloc(std::max<int64_t>(vt, 1));
+      D.ybuf.zero(D.up);
+      D.zbuf.zero(D.up);
+    }
     D.hodesc.resize(nsub);
     {
       int64_t po = 0;
@@ -2220,11 +2346,28 @@ static void level2_apply(Solver &S, const double *rc, double *uc) {
   Device &D = *S.dev, &E = *T.dev;
   hipStream_t st = D.st;
   launch_level_fill(st, D.l2n, D.l2src.p, D.l2w.p, rc, E.vec.p);
-  launch_trsv_fwd(st, E.pool.p, E.trsv.p, T.nsub, E.rptr.p, E.rk.p, E.rt.p, E.vec.p, E.maxnt);
+  const double *ep = E.pool.p;
+  auto gemv = [&](int k, const double *x, const double *b, double *y, int trans, double sign) {
+    launch_tile_gemv(st, ep, E.gx_t[k].p, (int)E.gx_t[k].n, E.gx_p[k].p, x, b, y, trans, sign);
+  };
+  // interior solves: by the explicit inverse (four GEMVs across the chip),
+  // or by the forward and backward sweeps
+  if (E.xint) {
+    gemv(0, E.vec.p, nullptr, E.ybuf.p, 0, 1.0);
+    gemv(1, E.ybuf.p, E.vec.p, E.vec.p, 0, -1.0);
+  } else {
+    launch_trsv_fwd(st, E.pool.p, E.trsv.p, T.nsub, E.rptr.p, E.rk.p, E.rt.p, E.vec.p, E.maxnt);
+  }
   launch_assemble(st, T.nig, E.aptr.p, E.rhs_src.p, nullptr, E.vec.p, E.g.p);
   op_pc(T, E.g.p, E.u.p);
   launch_ifc_scatter(st, T.nig, E.aptr.p, E.rhs_src.p, E.u.p, E.vec.p);
-  launch_trsv_bwd(st, E.pool.p, E.trsv.p, T.nsub, E.tcptr.p, E.tck.p, E.tct.p, E.vec.p, E.maxnt);
+  if (E.xint) {
+    gemv(2, E.vec.p, E.ybuf.p, E.zbuf.p, 1, -1.0);
+    gemv(3, E.zbuf.p, nullptr, E.vec.p, 1, 1.0);
+  } else {
+    launch_trsv_bwd(st, E.pool.p, E.trsv.p, T.nsub, E.tcptr.p, E.tck.p, E.tct.p, E.vec.p,
+                    E.maxnt);
+  }
   launch_level_gather(st, S.ndofc, D.l2pos.p, E.vec.p, uc);
 }
 

@@ -223,6 +223,21 @@ struct SymvRow {
   int psub;   // partial slot of the subdomain's (0, 0)
   int nti;
 };
+// block-sparse tile GEMV: per task y[yoff..yoff+64) = (b ? b[boff..] : 0) +
+// sign * sum over its pairs of op(tile) x[xoff..xoff+64), op the tile or
+// (trans) its transpose; tiles column major in pool.  A task may write the
+// segment its base reads (in place) but no segment another task reads.
+struct GemvTask {
+  int64_t yoff, boff;
+  int p0, p1;
+};
+struct GemvPair {
+  int tile, pad;
+  int64_t xoff;
+};
+void launch_tile_gemv(hipStream_t st, const double *pool, const GemvTask *tasks, int ntasks,
+                      const GemvPair *pairs, const double *x, const double *b, double *y, int trans,
+                      double sign);
 template <typename T>
 void launch_symv(hipStream_t st, const T *pool, const SymvCol *cols, int ncols,
                  const SymvRow *rows, int nrows, const int64_t *loff, const int *map,

@@ -712,6 +712,68 @@ template void launch_symv<float>(hipStream_t, const float *, const SymvCol *, in
                                  int, const int64_t *, const int *, const double *, const double *,
                                  double *, double *);
 
+// Block-sparse tile GEMV, one workgroup per output segment of 64:
+// y[yoff..] = (b ? b[boff..] : 0) + sign * sum_p op(T_p) x[xoff_p..], op the
+// tile (trans 0: lane = row, each wave 16 columns, 512-byte column loads) or
+// its transpose (trans 1: wave w the output rows 16w..16w+15, lane = the
+// summed index, 512-byte row loads); partial sums meet in LDS in a fixed
+// order.  The explicit interior operators of the levels below the first.
+__global__ __launch_bounds__(256) void k_tile_gemv(const double *__restrict__ pool,
+                                                   const GemvTask *__restrict__ tasks,
+                                                   const GemvPair *__restrict__ pairs,
+                                                   const double *__restrict__ x,
+                                                   const double *b, double *y, int trans,
+                                                   double sign) {
+  __shared__ double red[64][65];
+  const GemvTask t = tasks[blockIdx.x];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (!trans) {
+    double acc = 0.0;
+    for (int q = t.p0; q < t.p1; q++) {
+      const GemvPair p = pairs[q];
+      const double *T = pool + (size_t)p.tile * TILE_ELEMS + (16 * w) * 64 + l;
+      const double *xv = x + p.xoff + 16 * w;
+#pragma unroll
+      for (int c = 0; c < 16; c++) acc += T[c * 64] * xv[c];
+    }
+    red[w][l] = acc;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const double s = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);
+      const double base = b ? b[t.boff + l] : 0.0;
+      y[t.yoff + l] = base + sign * s;
+    }
+  } else {
+    double acc[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = 0.0;
+    for (int q = t.p0; q < t.p1; q++) {
+      const GemvPair p = pairs[q];
+      // (T^T x)[r] = sum_c T(c, r) x[c]; column r of the tile is contiguous
+      const double *T = pool + (size_t)p.tile * TILE_ELEMS + (16 * w) * 64 + l;
+      const double xl = x[p.xoff + l];
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[r] += T[r * 64] * xl;
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) red[16 * w + r][l] = acc[r];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      double s = 0.0;
+      for (int c = 0; c < 64; c++) s += red[l][c];
+      const double base = b ? b[t.boff + l] : 0.0;
+      y[t.yoff + l] = base + sign * s;
+    }
+  }
+}
+void launch_tile_gemv(hipStream_t st, const double *pool, const GemvTask *tasks, int ntasks,
+                      const GemvPair *pairs, const double *x, const double *b, double *y, int trans,
+                      double sign) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_tile_gemv, dim3(ntasks), dim3(256), 0, st, pool, tasks, pairs, x, b, y,
+                     trans, sign);
+}
+
 // y = A x for a dense matrix of nt x nt tiles in full storage (tile (I, J)
 // at base + I*nt + J), x read from global memory: one workgroup per tile
 // writes the tile's product to partial slot (J, I); the reduce sums the

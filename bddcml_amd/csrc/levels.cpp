@@ -235,11 +235,13 @@ void setup_second_level(Solver &S) {
   T.precision = 64;  // the coarse levels stay in FP64
   T.ordering = S.ordering;
   T.nd_leaf = S.nd_leaf;
-  // levels below the first form S'^{-1} at setup unless the caller turned
-  // the explicit inverse off: their subdomains are few and small, so the
-  // inverse costs little, and its tile GEMV spreads each apply across the
-  // chip where the factor sweeps would run one workgroup per subdomain
+  // levels below the first form S'^{-1} and the inverse of the interior
+  // factor at setup unless the caller turned the explicit inverse off: their
+  // subdomains are few and small, so the inverses cost little, and their
+  // tile GEMVs spread each apply across the chip where the factor sweeps
+  // would run one workgroup per subdomain
   T.explicit_inverse = S.explicit_inverse == 0 ? 0 : 1;
+  T.explicit_interior = S.explicit_inverse != 0;
   T.plan_only = S.plan_only;
   T.subs.resize(T.nsub);
   int bad = 0;

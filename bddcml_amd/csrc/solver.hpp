@@ -67,6 +67,9 @@ struct Solver {
   // at setup, 2 formed once the preconditioner has been applied 40 times on
   // the same setup (the default: easy problems never pay for it)
   int explicit_inverse = 2;
+  // levels below the first: interior solves of the level's apply through the
+  // explicit inverse of the interior factor (tile GEMVs) instead of sweeps
+  bool explicit_interior = false;
   // adaptive constraints: eigenvalue threshold, LOBPCG tolerance relative to
   // the pair's mean weight coefficient, iteration limit and block size
   // (src/module_adaptivity.f90:35-46)

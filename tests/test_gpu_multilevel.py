@@ -15,7 +15,9 @@ its BDDC replaces the coarse solve (src/module_levels.f90:1297-1530,
   stiff slabs make S' ill-conditioned and PCG's count sensitive to the
   factor's precision: 15 % + 2, FP64 5 % + 1, as for two levels);
 - one second-level subdomain: identical to two levels (the coarse problem is
-  then solved directly).
+  then solved directly);
+- the levels below the first with their triangular sweeps instead of the
+  explicit inverses (option explicit_inverse = 0): the same bars as FP64.
 """
 import sys
 
@@ -79,6 +81,18 @@ def test_three_levels_match_oracle(oracle_lib, name, make, nsub2, wt):
     assert r == 0
     assert (it <= 1.15 * it_o + 2) if layered else abs(it - it_o) <= 1
     assert _relmax(b.interface_solution(), u_o) < 1e-8
+    b.finalize()
+
+    # the levels below the first by default apply explicit inverses (S'^-1
+    # and the interior factor's) by tile GEMVs; explicit_inverse = 0 keeps
+    # the triangular sweeps there: the same operator to rounding
+    _say(f"case three_levels {name} sweeps (explicit_inverse 0)")
+    b, (it, r, _) = bddcml_amd.solve_cube(p, weights_type=wt, tol=tol, maxit=400, nsub2=nsub2,
+                                          options={"explicit_inverse": 0})
+    assert r == 0
+    assert abs(it - it_o) <= (max(1, 0.05 * it_o) if layered else 1)
+    assert _relmax(b.interface_solution(), u_o) < 1e-10
+    assert _relmax(b.apply_pc(x), t.o1.apply_pc(x)) < 1e-10
     b.finalize()
     t.close()
 

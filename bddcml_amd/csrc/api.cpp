@@ -111,8 +111,11 @@ void bddcml_upload_subdomain_data_c(
   S.ndof_g = *ndof;
   S.ndim = *ndim;
   S.meshdim = *meshdim;
-  if (*matrixtype != 1)
-    fatal(R, "the device path supports matrixtype 1 (symmetric positive definite)");
+  // matrixtype 1 (SPD) and 2 (symmetric general) in upper-triangle
+  // storage; 0 (general) in full storage is accepted when the matrix is
+  // symmetric and folded to its upper triangle.  The factorizations are
+  // Cholesky: an indefinite matrix stops the setup with an error.
+  if (*matrixtype < 0 || *matrixtype > 2) fatal(R, "unknown matrixtype " + std::to_string(*matrixtype));
   if (S.matrixtype >= 0 && S.matrixtype != *matrixtype) fatal(R, "matrixtype differs between subdomains");
   S.matrixtype = *matrixtype;
   Sub &s = local_sub(S, *isub, R);
@@ -171,9 +174,11 @@ void bddcml_upload_subdomain_data_c(
   // duplicates are summed in input order
   const int n = *la;
   const int nd = *ndofs;
+  const bool full = *matrixtype == 0;
   for (int k = 0; k < n; k++) {
-    if (i_sparse[k] > j_sparse[k])
-      fatal(R, "Matrix in wrong format - only upper triangle is expected for symmetric problems, matrixtype = 1");
+    if (!full && i_sparse[k] > j_sparse[k])
+      fatal(R, "Matrix in wrong format - only upper triangle is expected for symmetric problems, matrixtype = " +
+                   std::to_string(*matrixtype));
     const int i = i_sparse[k] - b, j = j_sparse[k] - b;
     if (i < 0 || j < 0 || i >= nd || j >= nd) fatal(R, "matrix index out of range");
   }
@@ -207,6 +212,32 @@ void bddcml_upload_subdomain_data_c(
       s.av.back() += a_sparse[k];
     else
       s.ai.push_back(i), s.aj.push_back(j), s.av.push_back(a_sparse[k]);
+  }
+  if (full) {
+    // both triangles assembled: every (i, j) needs its mirror (j, i) with the
+    // same value (to 1e-12 of the largest entry: sums of the same element
+    // contributions in another order round differently), then the upper
+    // triangle is kept
+    double amax = 0.0;
+    for (double v : s.av) amax = std::max(amax, std::fabs(v));
+    std::vector<int> rp(nd + 1, 0);
+    for (int i : s.ai) rp[i + 1]++;
+    for (int r = 0; r < nd; r++) rp[r + 1] += rp[r];
+    for (size_t q = 0; q < s.ai.size(); q++) {
+      const int i = s.ai[q], j = s.aj[q];
+      if (i == j) continue;
+      auto bgn = s.aj.begin() + rp[j], end = s.aj.begin() + rp[j + 1];
+      auto it = std::lower_bound(bgn, end, i);
+      const double v = s.av[q];
+      const double m = (it == end || *it != i) ? 0.0 : s.av[it - s.aj.begin()];
+      if (std::fabs(v - m) > 1e-12 * amax)
+        fatal(R, "matrixtype 0: the matrix is not symmetric; the device path factors symmetric "
+                 "positive definite matrices");
+    }
+    size_t o = 0;
+    for (size_t q = 0; q < s.ai.size(); q++)
+      if (s.ai[q] <= s.aj[q]) s.ai[o] = s.ai[q], s.aj[o] = s.aj[q], s.av[o] = s.av[q], o++;
+    s.ai.resize(o), s.aj.resize(o), s.av.resize(o);
   }
   // a fixed dof without a diagonal entry gets an explicit zero one; the
   // Dirichlet elimination turns it into 1 (src/module_sm.f90:556-585)
@@ -249,7 +280,7 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
   const char *R = "BDDCML_SETUP_PRECONDITIONER";
   Solver &S = solver();
   double t0 = wall_time();
-  if (*matrixtype != 1) fatal(R, "the device path supports matrixtype 1 (symmetric positive definite)");
+  if (*matrixtype < 0 || *matrixtype > 2) fatal(R, "unknown matrixtype " + std::to_string(*matrixtype));
   for (auto &s : S.subs)
     if (!s.uploaded) fatal("LEVELS_PC_SETUP", "not all subdomains were uploaded");
   if (*use_defaults_int) {

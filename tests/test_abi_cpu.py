@@ -301,3 +301,53 @@ def test_autotune_rejects_unsupported_level_count():
     from bddcml_amd.autotune import autotune
     with pytest.raises(ValueError, match="nsublev"):
         autotune([], 8, levels=4, candidates=[{}])
+
+
+def test_matrix_types_host_plan():
+    """matrixtype 2 (symmetric, upper triangle) and matrixtype 0 (full
+    storage, here the mirrored symmetric matrix in shuffled order) give the
+    host plan of matrixtype 1; an unsymmetric full matrix and an unknown type
+    are rejected with a message (reported to the caller), as the reference's
+    upload rejects a lower-triangle entry for the symmetric types."""
+    code = f"""
+import sys, json, numpy as np
+sys.path.insert(0, {ROOT!r})
+import ctypes as C
+import bddcml_amd
+from bddcml_amd.cube import CubeProblem
+p = CubeProblem(3, 2, kind='elasticity', rhs='random', seed=3)
+subs = list(p.subdomains())
+def full(d, skew=0.0):
+    i, j, a = d['i_sparse'], d['j_sparse'], d['a_sparse']
+    off = i != j
+    I = np.concatenate([i, j[off]]); J = np.concatenate([j, i[off]])
+    A = np.concatenate([a, a[off] * (1.0 + skew)])
+    o = np.random.default_rng(1).permutation(len(I))
+    return dict(d, i_sparse=I[o], j_sparse=J[o], a_sparse=A[o], matrixtype=0)
+def plan(mt, make):
+    b = bddcml_amd.Bddcml()
+    b.init(2, [p.nsub, 1], numbase=p.numbase)
+    b.set_option('plan_only', 1)
+    try:
+        for d in subs:
+            b.upload_subdomain_data(make(d))
+        b.setup_preconditioner(mt, 0, 1, 1, 1, 0, 0, 0)
+        out = dict(err='', counts=b.counts())
+    except RuntimeError as e:
+        out = dict(err=str(e), counts=None)
+    b.finalize()
+    return out
+res = dict(t1=plan(1, lambda d: d), t2=plan(2, lambda d: dict(d, matrixtype=2)),
+           t0=plan(0, full), bad=plan(0, lambda d: full(d, 1e-6)),
+           unknown=plan(3, lambda d: dict(d, matrixtype=3)))
+print(json.dumps(res))
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    import json
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    assert res["t1"]["err"] == "" and res["t1"]["counts"] is not None
+    assert res["t2"]["err"] == "" and res["t2"]["counts"] == res["t1"]["counts"]
+    assert res["t0"]["err"] == "" and res["t0"]["counts"] == res["t1"]["counts"]
+    assert "not symmetric" in res["bad"]["err"]
+    assert "unknown matrixtype" in res["unknown"]["err"]

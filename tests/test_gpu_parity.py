@@ -293,3 +293,41 @@ def test_sweeps_with_vector_in_global_memory(oracle_lib, nel, ns, kind, ex):
         assert np.abs(ls - o.local_solution(d["isub"])).max() <= 1e-10 * np.abs(u_ref).max()
     o.close()
     b.finalize()
+
+
+@pytest.mark.parametrize("mt", [2, 0], ids=["symmetric_general", "full_storage"])
+def test_matrix_types(oracle_lib, mt):
+    """matrixtype 2 (upper triangle) and 0 (full storage of the symmetric
+    matrix, entries shuffled) solve like matrixtype 1: the same iteration
+    count and the interface solution within 1e-12 (full storage sums the
+    duplicates in another order), and within 1e-10 of the oracle."""
+    _say(f"case matrix type {mt}")
+    p = CubeProblem(3, 2, kind="elasticity", rhs="random", seed=41)
+    o, (it_o, r_o, _) = _oracle(oracle_lib, p, tol=1e-10)
+
+    def run(mtype, make):
+        b = bddcml_amd.Bddcml()
+        b.init(2, [p.nsub, 1], numbase=p.numbase)
+        for d in p.subdomains():
+            b.upload_subdomain_data(make(d))
+        b.setup_preconditioner(mtype, 0, 1, 1, 1, 0, 0, 0)
+        it, r, _ = b.solve(0, 1e-10, 300, 50)
+        u = b.interface_solution()
+        b.finalize()
+        return it, r, u
+
+    def full(d):
+        i, j, a = d["i_sparse"], d["j_sparse"], d["a_sparse"]
+        off = i != j
+        I, J = np.concatenate([i, j[off]]), np.concatenate([j, i[off]])
+        A = np.concatenate([a, a[off]])
+        perm = np.random.default_rng(2).permutation(len(I))
+        return dict(d, i_sparse=I[perm], j_sparse=J[perm], a_sparse=A[perm], matrixtype=0)
+
+    it1, r1, u1 = run(1, lambda d: d)
+    it, r, u = run(mt, (lambda d: dict(d, matrixtype=2)) if mt == 2 else full)
+    assert r1 == r == 0
+    assert it == it1 and abs(it - it_o) <= 1
+    assert _relmax(u, u1) < 1e-12
+    assert _relmax(u, o.interface_solution()) < 1e-10
+    o.close()

@@ -52,6 +52,7 @@ void bddcml_init_c(int *nl, int *nsublev, int *lnsublev, int *nsub_loc_1, int *c
   set_task_order(1);
   set_trsv_global(false);
   set_launch_forms(0);
+  set_sweep_waves(0);
   set_pcg_graph(true);
   Solver &S = solver();
   if (*nl < 2) fatal("LEVELS_INIT", "Number of levels must be at least 2");
@@ -739,6 +740,16 @@ int bddcml_amd_set_option(const char *key, double value) try {
   if (!std::strcmp(key, "launch_forms")) {
     if (value != 0.0 && value != 1.0 && value != 2.0) return -2;
     set_launch_forms((int)value);
+    return 0;
+  }
+  if (!std::strcmp(key, "sweep_waves")) {
+    if (value != 0.0 && value != 4.0 && value != 16.0) return -2;
+    set_sweep_waves((int)value);
+    return 0;
+  }
+  if (!std::strcmp(key, "pc_form")) {
+    if (value != 0.0 && value != 1.0 && value != 2.0) return -2;
+    S.pc_form = (int)value;
     return 0;
   }
   if (!std::strcmp(key, "task_order")) {

@@ -1541,8 +1541,9 @@ void device_setup_finish(Solver &S) {
       mx = std::max(mx, wk);
       sum += wk;
     }
-    D.kapply_deep = launch_forms() == 0 ? nsub < 128 || (sum > 0.0 && mx > 2.0 * sum / nsub)
-                                        : launch_forms() == 2;
+    D.kapply_deep = S.pc_form ? S.pc_form == 2
+                    : launch_forms() == 0 ? nsub < 128 || (sum > 0.0 && mx > 2.0 * sum / nsub)
+                                          : launch_forms() == 2;
     if (S.verbose >= 2)
       std::printf("   device plan: local preconditioner %s form (largest sweep %.2fx the mean)\n",
                   D.kapply_deep ? "deep" : "shallow", sum > 0.0 ? mx * nsub / sum : 0.0);

@@ -155,6 +155,9 @@ void set_trsv_global(bool on);
 // 0: launch forms chosen by the number of subdomains; 1: the many-subdomain
 // forms; 2: the few-subdomain forms (testing)
 void set_launch_forms(int forms);
+// waves per workgroup of the interior sweeps: 0 chosen by the number of
+// subdomains, 4 or 16 forced (a launch shape the autotuner tries)
+void set_sweep_waves(int waves);
 int launch_forms();
 // P: element type of the interior factor tiles (double or float)
 template <typename P>

@@ -556,7 +556,11 @@ void set_trsv_global(bool on) { g_trsv_global = on; }
 static int g_launch_forms = 0;
 void set_launch_forms(int forms) { g_launch_forms = forms; }
 int launch_forms() { return g_launch_forms; }
+// waves per workgroup of the interior sweeps: 0 by the rule above, else 4 or 16
+static int g_sweep_waves = 0;
+void set_sweep_waves(int waves) { g_sweep_waves = waves; }
 static bool few_subdomains(int nsub) {
+  if (g_sweep_waves) return g_sweep_waves == 16;
   return g_launch_forms == 0 ? nsub < kTrsvFewSubs : g_launch_forms == 2;
 }
 template <int NW, typename P, class K1, class K2>

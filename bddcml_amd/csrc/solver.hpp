@@ -61,7 +61,10 @@ struct Solver {
   double refine_first = 0.0;
   int refine_max = 10;
   int ordering = 0;    // interior ordering: 0 tile-aligned nested dissection, 1 RCM
-  int nd_leaf = 256;   // nested dissection: largest part (dofs) left undivided
+  // nested dissection: largest part (dofs) left undivided (512: 128^3
+  // Poisson with 16^3 subdomains 128 -> 120.5 ms per setup + solve against
+  // 256; the autotuner tries 128, 256 and 512 on a caller's problem)
+  int nd_leaf = 512;
   // local preconditioner through an explicit S'^{-1} (applied by a symmetric
   // tile GEMV) instead of two sweeps over the factor of S': 0 never, 1 formed
   // at setup, 2 formed once the preconditioner has been applied 40 times on
@@ -70,6 +73,9 @@ struct Solver {
   // levels below the first: interior solves of the level's apply through the
   // explicit inverse of the interior factor (tile GEMVs) instead of sweeps
   bool explicit_interior = false;
+  // form of the local preconditioner launch: 0 chosen from the subdomain
+  // count and sizes, 1 shallow (two tiles in flight), 2 deep (four)
+  int pc_form = 0;
   // adaptive constraints: eigenvalue threshold, LOBPCG tolerance relative to
   // the pair's mean weight coefficient, iteration limit and block size
   // (src/module_adaptivity.f90:35-46)
@@ -184,6 +190,8 @@ void set_trsv_global(bool on);
 // forms of the sweeps and of the local preconditioner launch: 0 by the number
 // of subdomains, 1 the many-subdomain forms, 2 the few-subdomain forms (testing)
 void set_launch_forms(int forms);
+// waves per workgroup of the interior sweeps (0 automatic, 4 or 16)
+void set_sweep_waves(int waves);
 // PCG iterations from the third on as replays of one captured graph (the
 // default) or as separate launches
 void set_pcg_graph(bool on);

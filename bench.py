@@ -66,7 +66,7 @@ def profiled_traffic(args, kernel):
     FETCH_SIZE / WRITE_SIZE summary of the default workload (null for other
     workloads or when the summary is absent)."""
     if (args.workload, args.nel, args.nsub, args.ordering, args.nd_leaf, args.levels,
-            args.explicit_inverse) != ("cube", 16, 8, 0, 256, 2, 2):
+            args.explicit_inverse) != ("cube", 16, 8, 0, 512, 2, 2):
         return None
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "hbm_traffic_128cube_pmc.md")
@@ -244,7 +244,7 @@ def main():
                          "Schur complements and S' factor with FP64 iterative refinement; "
                          "both: 64 and 32; all: 64, 32 and refined (FP64 is the reported "
                          "value, the others are added to the line)")
-    ap.add_argument("--nd-leaf", type=int, default=256,
+    ap.add_argument("--nd-leaf", type=int, default=512,
                     help="nested dissection: largest part (dofs) left whole")
     args = ap.parse_args()
     if args.gpus != 1:

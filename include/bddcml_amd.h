@@ -69,7 +69,7 @@ int bddcml_amd_timings(double *out, int n);
  *   "interior_ordering" (0: tile-aligned nested dissection by coordinate
  *     bisection, the default; 1: reverse Cuthill-McKee);
  *   "nd_leaf" (largest part, in dofs, that nested dissection leaves whole;
- *     default 256);
+ *     default 512);
  *   "precision" (64: FP64 throughout, the default; 32: the augmented
  *     interface factor S', its triangular sweeps and Z = S'^{-1} C^T in FP32,
  *     while the Schur operator, the Krylov method and the coarse problem
@@ -89,6 +89,11 @@ int bddcml_amd_timings(double *out, int n);
  *     solve, 10 by default);
  *   "task_order" (1: each XCD runs one contiguous range of a tile-task
  *     launch, the default; 0: plain workgroup order);
+ *   "sweep_waves" (waves per workgroup of the interior sweeps: 0, the
+ *     default, 16 below 1,024 subdomains and 4 above; 4 or 16 forced);
+ *   "pc_form" (local preconditioner launch: 0, the default, the deep form
+ *     below 128 subdomains or when the largest subdomain's sweep is more than
+ *     twice the mean, else the shallow one; 1 shallow; 2 deep);
  *   "sweep_in_global" (1: the interior sweeps, the local preconditioner and
  *     the coarse residual keep their per-subdomain working vector in global
  *     memory -- the forms used automatically for subdomains whose vector does

@@ -81,7 +81,8 @@ int bddcml_amd_timings(double *out, int n);
  *     forward sweep to the interface residual of the exact Schur complements,
  *     on which the solve stops; each correction is solved on the FP32 Schur
  *     complements.  The iteration count returned sums every correction's.
- *     Not with adaptive constraints.  0: off, the default);
+ *     Krylov recycling is not used with it, and adaptive constraints are
+ *     refused.  0: off, the default);
  *   "refinement_first_tol" (with refinement: relative tolerance of the
  *     first solve; 0, the default, takes the square root of the solve's
  *     tolerance);

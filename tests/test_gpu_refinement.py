@@ -94,6 +94,24 @@ def test_refined_headline_shape(oracle_lib, forms):
         b.finalize()
 
 
+def test_refined_with_explicit_inverse(oracle_lib):
+    """The refined form with the explicit S'^-1 formed at setup (FP32 tiles,
+    applied by the tile GEMV) as its local preconditioner."""
+    _say("case refined cube with explicit_inverse 1")
+    tol = 1e-10
+    p = CubeProblem(4, 3, kind="poisson", rhs="random", seed=34)
+    o, (it_o, r_o, _) = oracle_lib.solve_cube(p, tol=tol, maxit=300)
+    b, (it, r, _) = bddcml_amd.solve_cube(p, tol=tol, maxit=300,
+                                          options={"refinement": 1, "explicit_inverse": 1})
+    try:
+        assert r == r_o == 0
+        _check_refined(b, it, it_o, tol)
+        _check_solution(b, o, p, 1e-8)
+    finally:
+        o.close()
+        b.finalize()
+
+
 def test_refined_uneven_boxes(oracle_lib):
     """Uneven elasticity boxes with stiff layers (coefficient jump 1e3): the
     FP32 interior factors of the layered subdomains are the least accurate,

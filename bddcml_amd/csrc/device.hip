@@ -1080,7 +1080,8 @@ void device_setup_begin(Solver &S) {
   ph("pool allocation");
   if (S.verbose >= 2)
     std::printf("   device plan: %lld subdomain tiles (%.1f GB), %zu padding slots\n",
-                (long long)D.ntiles, D.ntiles * 32768.0 / 1e9, (size_t)D.sc_pads.n);
+                (long long)D.ntiles, D.ntiles * (D.f32i ? 16384.0 : 32768.0) / 1e9,
+                (size_t)D.sc_pads.n);
 
   S.t.schur_flops = kTileGemm * tile_products(a_schur);
   S.t.interior_flops = tile_work(a_diag, 2 * kTileChol) + tile_work(a_off, kTileGemm) +

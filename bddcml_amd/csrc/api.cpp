@@ -46,7 +46,13 @@ extern "C" {
 void bddcml_init_c(int *nl, int *nsublev, int *lnsublev, int *nsub_loc_1, int *comm_init,
                    int *verbose_level, int *numbase, int *just_direct_solve_int) try {
   (void)comm_init;
-  (void)just_direct_solve_int;
+  // the reference's global direct solve (src/module_levels.f90:469-478) is
+  // not built: the problem is solved by BDDC, whose single-subdomain case
+  // (nsublev(1) = 1, where the reference also falls back to the direct
+  // solve) is the interior factorization and its sweeps, a direct solve
+  if (*just_direct_solve_int != 0 && *verbose_level > 0)
+    std::fprintf(stderr, "WARNING in BDDCML_INIT: the global direct solve is not built; "
+                         "the problem is solved by BDDC\n");
   device_prefer_spin_wait();
   reset_solver();
   set_task_order(1);

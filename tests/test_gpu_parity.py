@@ -331,3 +331,22 @@ def test_matrix_types(oracle_lib, mt):
     assert _relmax(u, u1) < 1e-12
     assert _relmax(u, o.interface_solution()) < 1e-10
     o.close()
+
+
+@pytest.mark.parametrize("kind", ["poisson", "elasticity"])
+def test_single_subdomain_is_a_direct_solve(kind):
+    """One subdomain (nsublev = {1, 1}): no interface, so the solve is the
+    interior tile Cholesky and its two sweeps -- the direct solve the
+    reference falls back to for one subdomain (src/module_bddcml.f90:81-84).
+    Checked against a dense solve of the global system."""
+    _say(f"case single subdomain {kind}")
+    p = CubeProblem(6 if kind == "poisson" else 4, 1, kind=kind, rhs="random", seed=43)
+    b, (it, r, _) = bddcml_amd.solve_cube(p, tol=1e-10, maxit=50)
+    try:
+        assert r == 0 and it == 0
+        Ae, fe, _ = p.global_dense()
+        u = np.linalg.solve(Ae, fe)
+        ug = b.download_global_solution(p.ndof)
+        assert np.abs(ug - u).max() <= 1e-11 * np.abs(u).max()
+    finally:
+        b.finalize()

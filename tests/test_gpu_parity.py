@@ -158,7 +158,7 @@ def test_reactions_and_global_solution(kind):
                                             (3, 2, "elasticity", 0)])
 def test_mixed_precision_solve(oracle_lib, nel, ns, kind, wt):
     """Mixed mode (FP32 factor of S' and Z, FP64 Schur operator, Krylov and
-    coarse problem): iteration count within +-2 of the FP64 oracle and the
+    coarse problem): iteration count within +-1 of the FP64 oracle and the
     solution within 1e-8 relative of it at tolerance 1e-10."""
     _say(f"case mixed nel={nel} ns={ns} kind={kind} wt={wt}")
     p = CubeProblem(nel, ns, kind=kind, rhs="random", seed=6)
@@ -166,7 +166,8 @@ def test_mixed_precision_solve(oracle_lib, nel, ns, kind, wt):
     b, (it, r, c) = bddcml_amd.solve_cube(p, weights_type=wt, tol=1e-10, maxit=300,
                                           options={"precision": 32})
     assert r == r_o == 0
-    assert abs(it - it_o) <= 2
+    _say(f"  iterations {it}, FP64 oracle {it_o}")
+    assert abs(it - it_o) <= 1
     u_ref = o.interface_solution()
     assert _relmax(b.interface_solution(), u_ref) < 1e-8
     for d in p.subdomains():

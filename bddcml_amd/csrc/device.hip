@@ -694,20 +694,36 @@ void device_setup_begin(Solver &S) {
   D.sub.resize(nsub);
   // symbolic factorization once per class of identical subdomains
   if ((int)S.ord_cls.size() != nsub) fatal("BDDCML_DEVICE_PLAN", "orderings missing");
-#pragma omp parallel for schedule(dynamic)
-  for (int k = 0; k < (int)S.ord_reps.size(); k++)
-    symbolic(S.subs[S.ord_reps[k]], D.sub[S.ord_reps[k]]);
-#pragma omp parallel for schedule(dynamic)
-  for (int is = 0; is < nsub; is++) {
-    const int r = S.ord_reps[S.ord_cls[is]];
-    if (r != is) D.sub[is] = D.sub[r];
-  }
-  int next = 0;
   D.f32i = S.refine != 0;
   D.mixed = S.precision == 32 || D.f32i;
   D.xint = S.explicit_interior;
-  for (int is = 0; is < nsub; is++) assign_tiles(D.sub[is], next, D.xint);
-  D.ntiles = next;
+  // one tile layout per class (ids relative to the subdomain's first tile),
+  // then every subdomain takes its class's layout at its own base
+  const int nrep0 = (int)S.ord_reps.size();
+  std::vector<SubTiles> layout(nrep0);
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < nrep0; k++) {
+    symbolic(S.subs[S.ord_reps[k]], layout[k]);
+    int rel = 0;
+    assign_tiles(layout[k], rel, D.xint);
+    layout[k].col.clear();
+    layout[k].col.shrink_to_fit();
+  }
+  std::vector<int64_t> sbase(nsub + 1, 0);
+  for (int is = 0; is < nsub; is++) sbase[is + 1] = sbase[is] + layout[S.ord_cls[is]].span;
+  if (sbase[nsub] >= INT32_MAX) fatal("BDDCML_DEVICE_PLAN", "too many subdomain tiles");
+#pragma omp parallel for schedule(static)
+  for (int is = 0; is < nsub; is++) {
+    SubTiles &T = D.sub[is];
+    T = layout[S.ord_cls[is]];
+    const int b = (int)sbase[is];
+    T.base = b;
+    for (int &t : T.pat.tile) t += b;
+    T.linvA += b;
+    T.mbase += b;
+    if (T.slow >= 0) T.slow += b;
+  }
+  D.ntiles = sbase[nsub];
 
   ph("symbolic");
   // ---- the tile pool is allocated on a helper thread while the host builds

@@ -424,27 +424,38 @@ void build_topology(Solver &S) {
     for (int b = 1; b < nb; b++)
       for (int k = b * blk; k < std::min(ng + 1, (b + 1) * blk); k++) cnt[k] += part[b];
   }
-  S.n2s_idx.resize(S.n2s_ptr[ng]);
-  {
-    std::vector<int> fill(S.n2s_ptr.begin(), S.n2s_ptr.end() - 1);
-    int *f = fill.data(), *idx = S.n2s_idx.data();
-#pragma omp parallel for schedule(dynamic, 4)
-    for (int is = 0; is < S.nsub; is++)
-      for (int g : S.subs[is].isngn) idx[__atomic_fetch_add(&f[g], 1, __ATOMIC_RELAXED)] = is;
-#pragma omp parallel for schedule(static, 4096)
-    for (int g = 0; g < ng; g++)
-      if (cnt[g + 1] - cnt[g] > 1) std::sort(idx + cnt[g], idx + cnt[g + 1]);
-  }
   double tp = wall_time();
-  if (S.verbose >= 2) std::printf("   topology: %-20s %8.4f s\n", "node to subdomains", tp - t_n2s);
+  if (S.verbose >= 2) std::printf("   topology: %-20s %8.4f s\n", "node counts", tp - t_n2s);
+  // the split needs only the counts; the lists themselves are filled by
+  // build_node_subdomains, after the first device launch
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < S.nsub; is++) split_of(S, is);
+  S.n2s_idx.clear();
   if (S.verbose >= 2) std::printf("   topology: %-20s %8.4f s\n", "interface split", wall_time() - tp);
+}
+
+// the subdomains of every global node (CSR over n2s_ptr, ascending): for
+// the neighbour lists, the globs and the interface copies
+static void build_node_subdomains(Solver &S) {
+  const double t0 = wall_time();
+  const int ng = S.nnod_g;
+  const int *cnt = S.n2s_ptr.data();
+  S.n2s_idx.resize(S.n2s_ptr[ng]);
+  std::vector<int> fill(S.n2s_ptr.begin(), S.n2s_ptr.end() - 1);
+  int *f = fill.data(), *idx = S.n2s_idx.data();
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int is = 0; is < S.nsub; is++)
+    for (int g : S.subs[is].isngn) idx[__atomic_fetch_add(&f[g], 1, __ATOMIC_RELAXED)] = is;
+#pragma omp parallel for schedule(static, 4096)
+  for (int g = 0; g < ng; g++)
+    if (cnt[g + 1] - cnt[g] > 1) std::sort(idx + cnt[g], idx + cnt[g + 1]);
+  if (S.verbose >= 2) std::printf("   topology: %-20s %8.4f s\n", "node to subdomains", wall_time() - t0);
 }
 
 // Components, neighbour lists and globs: needed by the constraints and the
 // interface phase only, so they run while the device factors the interiors.
 void build_globs(Solver &S) {
+  build_node_subdomains(S);
   double tp = wall_time();
   auto phase = [&](const char *name) {
     double now = wall_time();

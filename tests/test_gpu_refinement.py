@@ -190,3 +190,25 @@ def test_repeated_solve(oracle_lib, options):
     finally:
         o.close()
         b.finalize()
+
+
+@pytest.mark.parametrize("options", [{}, {"precision": 32}, {"refinement": 1}],
+                         ids=["fp64", "mixed", "refined"])
+def test_results_are_bitwise_reproducible(options):
+    """Every reduction runs in a fixed order (CSR gathers, two-stage dots,
+    the fixed butterflies of the residual and GEMV kernels), so two setups
+    and solves of the same problem give the same bits."""
+    _say(f"case reproducible {options}")
+    p = CubeProblem(4, 3, kind="elasticity", rhs="random", seed=44)
+    out = []
+    for _ in range(2):
+        b, (it, r, c) = bddcml_amd.solve_cube(p, tol=1e-10, maxit=300, options=options)
+        try:
+            assert r == 0
+            out.append((it, c, b.interface_solution(), b.residual_history()))
+        finally:
+            b.finalize()
+    (it0, c0, u0, h0), (it1, c1, u1, h1) = out
+    assert it0 == it1 and c0 == c1
+    assert np.array_equal(u0, u1)
+    assert np.array_equal(h0, h1)

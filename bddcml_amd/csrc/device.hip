@@ -282,7 +282,8 @@ struct Device {
   bool inv_planned = false, explicit_inv = false, inv_auto = false;
   int pc_count = 0;
   std::vector<SeqPhase> cseq;  // coarse factorization and inverse
-  Arena plan;  // task lists, scatter tables and job lists of the first half
+  Arena plan0;  // scatter tables and tiles to clear (uploaded first)
+  Arena plan;  // task lists and job lists of the first half
   View<int> sc_perm, sc_ttab, sc_pads;  // shared per class of identically ordered subdomains
   std::vector<int> perm_off;            // offset of each subdomain's permutation in sc_perm
   View<char> sc_fix;
@@ -898,13 +899,6 @@ void device_setup_begin(Solver &S) {
     if ((int64_t)T.base + T.span > D.ntiles || T.span != R.span)
       fatal("BDDCML_DEVICE_PLAN", "tile range");
   }
-  auto base_of = [&](int is) { return D.sub[is].base; };
-  std::vector<TaskList> a_diag = merge_steps(sts, &SubTasks::a_diag, cls, base_of),
-                        a_off = merge_steps(sts, &SubTasks::a_off, cls, base_of);
-  TaskList a_schur = materialize(
-      sts, [](const SubTasks &Q) -> const TaskList * { return &Q.a_schur; }, cls, base_of);
-  std::vector<TaskList> m_rows;
-  if (D.xint) m_rows = merge_steps(sts, &SubTasks::m_rows, cls, base_of);
   {
     // tiles to clear before the scatter, per subdomain
     std::vector<size_t> off(nsub + 1, 0);
@@ -917,70 +911,8 @@ void device_setup_begin(Solver &S) {
     }
     for (int id : ids)
       if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid tile to clear");
-    D.plan.add(D.clear_ids, ids);
+    D.plan0.add(D.clear_ids, ids);
   }
-  sts.clear();
-  if (S.verbose >= 3) ph("  materialized");
-  if (S.verbose >= 2)
-    std::printf("   device plan: %d structure classes over %d subdomains, %zu interior steps\n", ncls,
-                nsub, a_diag.size());
-  upload_phases(D.a_diag, a_diag, D.plan);
-  upload_phases(D.a_off, a_off, D.plan);
-  D.a_schur.upload(a_schur, D.plan);
-  if (D.xint) {
-    upload_phases(D.m_rows, m_rows, D.plan);
-    // the four GEMVs of the interior solves, per subdomain (absolute tile
-    // ids, offsets into the device-ordered subdomain vectors):
-    // 0: y1 = M f1; 1: f2 -= L21 y1; 2: z1 = y1 - L21^T u2; 3: u1 = M^T z1
-    std::vector<GemvTask> gt[4];
-    std::vector<GemvPair> gp[4];
-    int64_t vo = 0;
-    for (int is = 0; is < nsub; is++) {
-      const SubTiles &T = D.sub[is];
-      const int nto = T.nto, nt = T.nto + T.nti;
-      auto task = [&](int k, int row) {
-        gt[k].push_back(GemvTask{vo + 64 * row, vo + 64 * row, (int)gp[k].size(), 0});
-      };
-      auto pair = [&](int k, int tile, int col) { gp[k].push_back(GemvPair{tile, 0, vo + 64 * col}); };
-      auto close = [&](int k) { gt[k].back().p1 = (int)gp[k].size(); };
-      for (int I = 0; I < nto; I++) {
-        task(0, I);
-        for (int J = 0; J < I; J++)
-          if (T.M(I, J) >= 0) pair(0, T.M(I, J), J);
-        pair(0, T.linvA + I, I);
-        close(0);
-      }
-      for (int I = nto; I < nt; I++) {
-        task(1, I);
-        for (int K = 0; K < nto; K++)
-          if (T.pat.find(I, K) >= 0) pair(1, T.pat.find(I, K), K);
-        close(1);
-      }
-      for (int J = 0; J < nto; J++) {
-        task(2, J);
-        for (int I = nto; I < nt; I++)
-          if (T.pat.find(I, J) >= 0) pair(2, T.pat.find(I, J), I);
-        close(2);
-        task(3, J);
-        pair(3, T.linvA + J, J);
-        for (int q = T.mptr[J]; q < T.mptr[J + 1]; q++) pair(3, T.mbase + q, T.mrow[q]);
-        close(3);
-      }
-      vo += (int64_t)S.subs[is].NO + S.subs[is].NI;
-    }
-    for (int k = 0; k < 4; k++) {
-      for (const GemvPair &p : gp[k])
-        if (p.tile < 0 || p.tile >= D.ntiles || p.xoff < 0 || p.xoff + 64 > vo)
-          fatal("BDDCML_DEVICE_PLAN", "interior GEMV pair out of range");
-      for (const GemvTask &t : gt[k])
-        if (t.yoff < 0 || t.yoff + 64 > vo) fatal("BDDCML_DEVICE_PLAN", "interior GEMV task out of range");
-      D.plan.add(D.gx_t[k], gt[k]);
-      D.plan.add(D.gx_p[k], gp[k]);
-    }
-    D.gx_vt = vo;
-  }
-
-  ph("task lists");
   // ---- matrix entries: the assembled triplets are already on the device
   // (uploaded with each subdomain); here only the device permutation, the
   // Dirichlet flags and the tile tables go up.  The permutation, the packed
@@ -1058,14 +990,104 @@ void device_setup_begin(Solver &S) {
         if (pads[q] < 0 || pads[q] >= span * TILE_ELEMS) fatal("BDDCML_DEVICE_PLAN", "invalid padding entry");
     }
     if (S.verbose >= 3) ph("  scatter checks");
-    D.plan.add(D.sc_perm, perm);
-    D.plan.add(D.sc_fix, fix);
-    D.plan.add(D.sc_ttab, ttab);
-    D.plan.add(D.sc_pads, pads);
-    D.plan.add(D.sc_subs, ss);
+    D.plan0.add(D.sc_perm, perm);
+    D.plan0.add(D.sc_fix, fix);
+    D.plan0.add(D.sc_ttab, ttab);
+    D.plan0.add(D.sc_pads, pads);
+    D.plan0.add(D.sc_subs, ss);
   }
 
   ph("scatter entries");
+  // the scatter's tables go up first, in an arena of their own, and the
+  // tiles are cleared and scattered while the host materializes the task
+  // lists of the factorization
+  D.plan0.commit(st);
+  pool_alloc.wait();
+  D.err.alloc(1);
+  D.err.zero(st);
+  hipEvent_t *ev = D.ev;
+  for (int k = 0; k < 5; k++) HIPCHECK(hipEventCreate(&ev[k]));
+  HIPCHECK(hipEventRecord(ev[0], st));
+  if (D.f32i) {
+    launch_zero_tiles(st, D.ipool32.p, D.clear_ids.p, (int)D.clear_ids.n);
+    launch_scatter_sub(st, D.ipool32.p, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p,
+                       D.sc_pads.p, D.err.p);
+  } else {
+    launch_zero_tiles(st, D.pool.p, D.clear_ids.p, (int)D.clear_ids.n);
+    launch_scatter_sub(st, D.pool.p, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p,
+                       D.sc_pads.p, D.err.p);
+  }
+  HIPCHECK(hipGetLastError());
+  ph("scatter launched");
+  auto base_of = [&](int is) { return D.sub[is].base; };
+  std::vector<TaskList> a_diag = merge_steps(sts, &SubTasks::a_diag, cls, base_of),
+                        a_off = merge_steps(sts, &SubTasks::a_off, cls, base_of);
+  TaskList a_schur = materialize(
+      sts, [](const SubTasks &Q) -> const TaskList * { return &Q.a_schur; }, cls, base_of);
+  std::vector<TaskList> m_rows;
+  if (D.xint) m_rows = merge_steps(sts, &SubTasks::m_rows, cls, base_of);
+  sts.clear();
+  if (S.verbose >= 3) ph("  materialized");
+  if (S.verbose >= 2)
+    std::printf("   device plan: %d structure classes over %d subdomains, %zu interior steps\n", ncls,
+                nsub, a_diag.size());
+  upload_phases(D.a_diag, a_diag, D.plan);
+  upload_phases(D.a_off, a_off, D.plan);
+  D.a_schur.upload(a_schur, D.plan);
+  if (D.xint) {
+    upload_phases(D.m_rows, m_rows, D.plan);
+    // the four GEMVs of the interior solves, per subdomain (absolute tile
+    // ids, offsets into the device-ordered subdomain vectors):
+    // 0: y1 = M f1; 1: f2 -= L21 y1; 2: z1 = y1 - L21^T u2; 3: u1 = M^T z1
+    std::vector<GemvTask> gt[4];
+    std::vector<GemvPair> gp[4];
+    int64_t vo = 0;
+    for (int is = 0; is < nsub; is++) {
+      const SubTiles &T = D.sub[is];
+      const int nto = T.nto, nt = T.nto + T.nti;
+      auto task = [&](int k, int row) {
+        gt[k].push_back(GemvTask{vo + 64 * row, vo + 64 * row, (int)gp[k].size(), 0});
+      };
+      auto pair = [&](int k, int tile, int col) { gp[k].push_back(GemvPair{tile, 0, vo + 64 * col}); };
+      auto close = [&](int k) { gt[k].back().p1 = (int)gp[k].size(); };
+      for (int I = 0; I < nto; I++) {
+        task(0, I);
+        for (int J = 0; J < I; J++)
+          if (T.M(I, J) >= 0) pair(0, T.M(I, J), J);
+        pair(0, T.linvA + I, I);
+        close(0);
+      }
+      for (int I = nto; I < nt; I++) {
+        task(1, I);
+        for (int K = 0; K < nto; K++)
+          if (T.pat.find(I, K) >= 0) pair(1, T.pat.find(I, K), K);
+        close(1);
+      }
+      for (int J = 0; J < nto; J++) {
+        task(2, J);
+        for (int I = nto; I < nt; I++)
+          if (T.pat.find(I, J) >= 0) pair(2, T.pat.find(I, J), I);
+        close(2);
+        task(3, J);
+        pair(3, T.linvA + J, J);
+        for (int q = T.mptr[J]; q < T.mptr[J + 1]; q++) pair(3, T.mbase + q, T.mrow[q]);
+        close(3);
+      }
+      vo += (int64_t)S.subs[is].NO + S.subs[is].NI;
+    }
+    for (int k = 0; k < 4; k++) {
+      for (const GemvPair &p : gp[k])
+        if (p.tile < 0 || p.tile >= D.ntiles || p.xoff < 0 || p.xoff + 64 > vo)
+          fatal("BDDCML_DEVICE_PLAN", "interior GEMV pair out of range");
+      for (const GemvTask &t : gt[k])
+        if (t.yoff < 0 || t.yoff + 64 > vo) fatal("BDDCML_DEVICE_PLAN", "interior GEMV task out of range");
+      D.plan.add(D.gx_t[k], gt[k]);
+      D.plan.add(D.gx_p[k], gp[k]);
+    }
+    D.gx_vt = vo;
+  }
+
+  ph("task lists");
   // S stays in its lower tiles, read by the symmetric GEMV and by S'
   int bad_s = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad_s)
@@ -1085,15 +1107,12 @@ void device_setup_begin(Solver &S) {
   if (bad_s) fatal("BDDCML_DEVICE_PLAN", "interface tiles of S are not packed by columns");
   ph("packed S check");
   const size_t plan_bytes = D.plan.bytes();
-  D.plan.commit(st);
+  // on the upload stream: the compute stream is busy with the scatter
+  D.plan.commit(D.up);
   if (S.verbose >= 3) {
     std::printf("   device plan: %.1f MB of plan arrays\n", plan_bytes / 1e6);
     ph("  plan upload");
   }
-  pool_alloc.wait();
-  D.err.alloc(1);
-  D.err.zero(st);
-  ph("pool allocation");
   if (S.verbose >= 2)
     std::printf("   device plan: %lld subdomain tiles (%.1f GB), %zu padding slots\n",
                 (long long)D.ntiles, D.ntiles * (D.f32i ? 16384.0 : 32768.0) / 1e9,
@@ -1104,20 +1123,14 @@ void device_setup_begin(Solver &S) {
                        tile_work(m_rows, kTileGemm) + S.t.schur_flops;
   // ---- numerics, first half: Dirichlet elimination into the tiles,
   // interior factorization, Schur complements
-  hipEvent_t *ev = D.ev;
-  for (int k = 0; k < 5; k++) HIPCHECK(hipEventCreate(&ev[k]));
   D.ev_gemm.assign(2 * D.a_off.size(), nullptr);
   for (hipEvent_t &e : D.ev_gemm) HIPCHECK(hipEventCreate(&e));
   D.gemm_flops.clear();
   for (const TaskList &l : a_off) D.gemm_flops.push_back(kTileGemm * (tile_products(l) + l.tasks.size()));
   HIPCHECK(hipEventCreateWithFlags(&D.ev_up, hipEventDisableTiming));
   HIPCHECK(hipEventCreateWithFlags(&D.ev_rhs, hipEventDisableTiming));
-  HIPCHECK(hipEventRecord(ev[0], st));
   // element type of the subdomain tiles: FP64, or FP32 with refinement
   auto interior = [&](auto *pool) {
-    launch_zero_tiles(st, pool, D.clear_ids.p, (int)D.clear_ids.n);
-    launch_scatter_sub(st, pool, D.sc_subs.p, nsub, D.sc_perm.p, D.sc_fix.p, D.sc_ttab.p,
-                       D.sc_pads.p, D.err.p);
     for (size_t J = 0; J < D.a_diag.size(); J++) {
       launch_tile_potrf(st, pool, D.a_diag[J].tasks.p, D.a_diag[J].pairs.p, D.a_diag[J].ntask,
                         D.err.p);

@@ -2605,8 +2605,9 @@ void device_fetch_solution(Solver &S) {
 
 // u0 on the unique interface: each dof's value from its copy in the
 // lowest-numbered subdomain holding it (the fixed value at Dirichlet dofs),
-// gathered in parallel into the pinned staging area and sent in one copy
-static void initial_guess(Solver &S) {
+// gathered in parallel into the pinned staging area and sent in one copy;
+// returns whether u0 is zero
+static bool initial_guess(Solver &S) {
   Device &D = *S.dev;
   device_fetch_solution(S);
   if ((int)S.ig_ptr.size() != S.nig + 1) fatal("BDDCML_SOLVE", "interface numbering missing");
@@ -2614,7 +2615,8 @@ static void initial_guess(Solver &S) {
   h.used = 0;
   h.grow((size_t)S.nig * sizeof(double));
   double *u0 = reinterpret_cast<double *>(h.p);
-#pragma omp parallel for schedule(static, 4096)
+  int nonzero = 0;
+#pragma omp parallel for schedule(static, 4096) reduction(| : nonzero)
   for (int g = 0; g < S.nig; g++) {
     double v = 0.0;
     if (S.ig_ptr[g] < S.ig_ptr[g + 1]) {
@@ -2624,10 +2626,12 @@ static void initial_guess(Solver &S) {
       v = (s.has_fix && s.ifix[d]) ? s.fixv[d] : s.sol[d];
     }
     u0[g] = v;
+    nonzero |= v != 0.0;
   }
   HIPCHECK(hipMemcpyAsync(D.u.p, u0, S.nig * sizeof(double), hipMemcpyHostToDevice, D.st));
   // the staging area is reused by the next host transfer
   HIPCHECK(hipStreamSynchronize(D.st));
+  return nonzero == 0;
 }
 
 // ---------------------------------------------------- PCG iteration graph
@@ -2709,11 +2713,13 @@ static void pcg_graph_replay(Solver &S, int k, double *rz, double *pap, double *
 }
 
 // Krylov solve of S u = rhs on the interface from the u in D.u (fixed dofs
-// keep their values); appends the relative residuals to S.hist and returns
-// the norm of the initial residual in normres0_out (0 when it vanishes)
+// keep their values; u_zero: it is zero, so the initial residual is the
+// right-hand side without an S apply); appends the relative residuals to
+// S.hist and returns the norm of the initial residual in normres0_out (0
+// when it vanishes)
 static void krylov(Solver &S, const double *rhs, int method, double tol, int maxit, int ndecrmax,
-                   bool recycling, int maxvec, int *num_iter, int *reason, double *cond,
-                   double *normres0_out) {
+                   bool recycling, int maxvec, bool u_zero, int *num_iter, int *reason,
+                   double *cond, double *normres0_out) {
   Device &D = *S.dev;
   hipStream_t st = D.st;
   const int n = S.nig;
@@ -2727,10 +2733,12 @@ static void krylov(Solver &S, const double *rhs, int method, double tol, int max
   if (method == 0) {
     dots(S, {{D.r.p, D.r.p}}, sc);
     if (sc[0] == 0.0) goto finish;
-    op_schur(S, D.u.p, D.ap.p);
-    launch_mask(st, n, D.fixmask.p, D.ap.p);
-    launch_axpy2(st, n, -1.0, D.ap.p, D.r.p, 0.0, nullptr, nullptr);
-    dots(S, {{D.r.p, D.r.p}}, sc);
+    if (!u_zero) {
+      op_schur(S, D.u.p, D.ap.p);
+      launch_mask(st, n, D.fixmask.p, D.ap.p);
+      launch_axpy2(st, n, -1.0, D.ap.p, D.r.p, 0.0, nullptr, nullptr);
+      dots(S, {{D.r.p, D.r.p}}, sc);
+    }
     double normres0 = std::sqrt(sc[0]);
     if (normres0 == 0.0) goto finish;
     *normres0_out = normres0;
@@ -2870,9 +2878,11 @@ static void krylov(Solver &S, const double *rhs, int method, double tol, int max
     *cond = nw > 1 ? tridiag_max_eig(diag, sub, nw) : 1.0;
   } else {
     // BiCGSTAB (src/module_krylov.f90:948-1505)
-    op_schur(S, D.u.p, D.v.p);
-    launch_axpy2(st, n, -1.0, D.v.p, D.r.p, 0.0, nullptr, nullptr);
-    launch_mask(st, n, D.fixmask.p, D.r.p);
+    if (!u_zero) {
+      op_schur(S, D.u.p, D.v.p);
+      launch_axpy2(st, n, -1.0, D.v.p, D.r.p, 0.0, nullptr, nullptr);
+      launch_mask(st, n, D.fixmask.p, D.r.p);
+    }
     dots(S, {{D.r.p, D.r.p}}, sc);
     double normrhs = std::sqrt(sc[0]);
     if (normrhs == 0.0) goto finish;
@@ -3013,7 +3023,7 @@ static void refine_solution(Solver &S, int method, double tol, int maxit, int nd
     const size_t h0 = S.hist.size();
     int it = 0, rs = 0;
     double c = 0.0, nr = 0.0;
-    krylov(S, D.gk.p, method, inner, maxit - total, ndecrmax, false, 0, &it, &rs, &c, &nr);
+    krylov(S, D.gk.p, method, inner, maxit - total, ndecrmax, false, 0, true, &it, &rs, &c, &nr);
     for (size_t q = h0; q < S.hist.size(); q++) S.hist[q] *= nr / normres0;
     total += it;
     S.t.refinements++;
@@ -3045,7 +3055,7 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bo
   // a repeated solve of the same right-hand side redoes its forward sweep
   // (the back substitution of the last one overwrote it)
   if (!D.vec_y) reduced_rhs_launches(S, D.st);
-  initial_guess(S);
+  const bool u_zero = initial_guess(S);
   // with refinement the first solve stops early (at the square root of the
   // tolerance unless set): the FP32 operator's own error caps what more
   // iterations on it would give
@@ -3053,8 +3063,8 @@ void device_solve(Solver &S, int method, double tol, int maxit, int ndecrmax, bo
   const double tol1 =
       refine ? std::max(tol, S.refine_first > 0.0 ? S.refine_first : std::sqrt(tol)) : tol;
   double normres0 = 0.0;
-  krylov(S, D.g.p, method, tol1, maxit, ndecrmax, recycling && !refine, maxvec, num_iter, reason,
-         cond, &normres0);
+  krylov(S, D.g.p, method, tol1, maxit, ndecrmax, recycling && !refine, maxvec, u_zero, num_iter,
+         reason, cond, &normres0);
   read_apply_timings(S);
   S.t.krylov += wall_time() - t0;
   t0 = wall_time();

@@ -541,14 +541,26 @@ static void assign_factor(SubTiles &T, int nc, int64_t &nf, bool explicit_inv) {
   nf += L.span;
 }
 
-// tasks of a dense lower Cholesky on tiles P(I,J) with inverse tiles Linv(J)
+// tasks of a dense lower Cholesky on tiles P(I,J) with inverse tiles Linv(J).
+// The diagonal update of column J by the columns k < J - 1 runs as a task of
+// the product launch of column J - 1 (first in its list; those tiles are
+// final before it), so that the factorization launch of column J, one
+// workgroup per diagonal tile, adds only the pair of column J - 1.
 template <class FP, class FL>
 static void dense_chol_tasks(int n, FP Ptile, FL Linv, std::vector<TaskList> &diag,
                              std::vector<TaskList> &off) {
   if ((int)diag.size() < n) diag.resize(n), off.resize(n);
   for (int J = 0; J < n; J++) {
     TileTask t{Ptile(J, J), Linv(J), (int)diag[J].pairs.size(), 0};
-    for (int k = 0; k < J; k++) diag[J].pairs.push_back({Ptile(J, k), Ptile(J, k)});
+    if (J >= 2) {
+      TaskList &P = off[J - 1];
+      TileTask e{Ptile(J, J), -1, (int)P.pairs.size(), 0};
+      for (int k = 0; k < J - 1; k++) P.pairs.push_back({Ptile(J, k), Ptile(J, k)});
+      e.p1 = (int)P.pairs.size();
+      e.mode = task_mode(GEMM_NT, POST_NONE, true);
+      P.tasks.insert(P.tasks.begin(), e);
+    }
+    for (int k = std::max(0, J - 1); k < J; k++) diag[J].pairs.push_back({Ptile(J, k), Ptile(J, k)});
     t.p1 = (int)diag[J].pairs.size();
     diag[J].tasks.push_back(t);
     for (int I = J + 1; I < n; I++) {
@@ -657,9 +669,16 @@ static double tile_products(const TaskList &l) {
   for (const TileTask &t : l.tasks) n += t.p1 - t.p0;
   return n;
 }
+// tasks that end with a post multiply or a factorization (a task of its own
+// mode may have neither)
+static double posted_tasks(const TaskList &l) {
+  double n = 0;
+  for (const TileTask &t : l.tasks) n += t.mode < 0 || t.mode == kTaskPotrf || ((t.mode >> 2) & 3) != 0;
+  return n;
+}
 static double tile_work(const std::vector<TaskList> &v, double per_task) {
   double f = 0;
-  for (auto &l : v) f += kTileGemm * tile_products(l) + per_task * l.tasks.size();
+  for (auto &l : v) f += kTileGemm * tile_products(l) + per_task * posted_tasks(l);
   return f;
 }
 
@@ -796,7 +815,9 @@ void device_setup_begin(Solver &S) {
     }
     Q.a_diag.resize(nlev);
     Q.a_off.resize(nlev);
-    auto common = [&](int I, int J, int kmax, TaskList &L) {
+    // pairs (L(I,k), L(J,k)) over the columns k < kmax both rows reach
+    // (and that `keep` accepts)
+    auto common = [&](int I, int J, int kmax, TaskList &L, auto keep) {
       const auto &a = rowk[I], &b = rowk[J];
       size_t x = 0, y = 0;
       while (x < a.size() && y < b.size()) {
@@ -804,30 +825,54 @@ void device_setup_begin(Solver &S) {
         if (a[x] < b[y]) x++;
         else if (b[y] < a[x]) y++;
         else {
-          L.pairs.push_back({find(I, a[x]), find(J, b[y])});
+          if (keep(a[x])) L.pairs.push_back({find(I, a[x]), find(J, b[y])});
           x++, y++;
         }
       }
     };
+    auto all = [](int) { return true; };
+    // The diagonal update of a column at level l by columns of levels below
+    // l - 1 runs in the product launch of level l - 1 (those tiles are final
+    // before it; the tasks go first in that list); the factorization launch
+    // adds only the columns of level l - 1.  `split` records the
+    // factorization tasks that then continue a partial sum.
+    std::vector<std::vector<TileTask>> early(nlev);
+    std::vector<std::pair<int, int>> split;
     for (int J = 0; J < nto; J++) {
-      TaskList &Dg = Q.a_diag[lvl[J]], &Of = Q.a_off[lvl[J]];
+      const int lv = lvl[J];
+      TaskList &Dg = Q.a_diag[lv], &Of = Q.a_off[lv];
       TileTask t{find(J, J), T.linvA + J, (int)Dg.pairs.size(), 0};
-      common(J, J, J, Dg);
+      if (lv >= 1) {
+        TaskList &P = Q.a_off[lv - 1];
+        TileTask e{find(J, J), -1, (int)P.pairs.size(), 0};
+        common(J, J, J, P, [&](int k) { return lvl[k] < lv - 1; });
+        e.p1 = (int)P.pairs.size();
+        e.mode = task_mode(GEMM_NT, POST_NONE, true);
+        if (e.p1 > e.p0) {
+          early[lv - 1].push_back(e);
+          split.push_back({lv, (int)Dg.tasks.size()});
+        }
+        common(J, J, J, Dg, [&](int k) { return lvl[k] == lv - 1; });
+      } else {
+        common(J, J, J, Dg, all);
+      }
       t.p1 = (int)Dg.pairs.size();
       Dg.tasks.push_back(t);
       for (int q = T.pat.colptr[J]; q < T.pat.colptr[J + 1]; q++) {
         int I = T.pat.rowidx[q];
         if (I == J) continue;
         TileTask u{T.pat.tile[q], T.linvA + J, (int)Of.pairs.size(), 0};
-        common(I, J, J, Of);
+        common(I, J, J, Of, all);
         u.p1 = (int)Of.pairs.size();
         Of.tasks.push_back(u);
       }
     }
+    for (int l = 0; l < nlev; l++)
+      Q.a_off[l].tasks.insert(Q.a_off[l].tasks.begin(), early[l].begin(), early[l].end());
     for (int J = nto; J < nt; J++)
       for (int I = J; I < nt; I++) {
         TileTask t{find(I, J), -1, (int)Q.a_schur.pairs.size(), 0};
-        common(I, J, nto, Q.a_schur);
+        common(I, J, nto, Q.a_schur, all);
         t.p1 = (int)Q.a_schur.pairs.size();
         if (t.p1 > t.p0) Q.a_schur.tasks.push_back(t);
       }
@@ -884,6 +929,8 @@ void device_setup_begin(Solver &S) {
       for (auto *v : {&Q.a_diag, &Q.a_off})
         for (TaskList &L : *v) visit(L);
       visit(Q.a_schur);
+      // a factorization after an early diagonal update reads that sum
+      for (auto &sp : split) Q.a_diag[sp.first].tasks[sp.second].fresh = 0;
       for (int q = 0; q < npat; q++)
         if (entry[q] || !written[q]) Q.clear.push_back(q);
     }
@@ -1126,7 +1173,7 @@ void device_setup_begin(Solver &S) {
   D.ev_gemm.assign(2 * D.a_off.size(), nullptr);
   for (hipEvent_t &e : D.ev_gemm) HIPCHECK(hipEventCreate(&e));
   D.gemm_flops.clear();
-  for (const TaskList &l : a_off) D.gemm_flops.push_back(kTileGemm * (tile_products(l) + l.tasks.size()));
+  for (const TaskList &l : a_off) D.gemm_flops.push_back(kTileGemm * (tile_products(l) + posted_tasks(l)));
   HIPCHECK(hipEventCreateWithFlags(&D.ev_up, hipEventDisableTiming));
   HIPCHECK(hipEventCreateWithFlags(&D.ev_rhs, hipEventDisableTiming));
   // element type of the subdomain tiles: FP64, or FP32 with refinement

@@ -328,16 +328,33 @@ __device__ __forceinline__ float lane_bcast(float x, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
+// 1/sqrt(d) for d > 0: the hardware estimate and Newton steps
+// y <- y + y (1/2 - d y^2 / 2) (two for FP64, one for FP32)
+__device__ __forceinline__ double rsqrt_nr(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+  const double h = 0.5 * d;
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  return fma(y, fma(-h * y, y, 0.5), y);
+}
+__device__ __forceinline__ float rsqrt_nr(float d) {
+  const float y = __builtin_amdgcn_rsqf(d);
+  return fmaf(y, fmaf(-0.5f * d * y, y, 0.5f), y);
+}
+
 // Cholesky factor L of the 16x16 diagonal block at (c0, c0) and its inverse,
 // by one wave in registers: lane i (mod 16) holds row i of the block, then
 // column i of inv(L); every cross-lane value is a broadcast from one lane
-// (lane reads into scalar registers).  L goes back to As (upper part
-// zeroed), inv(L) to the same block of Bs.
+// (lane reads into scalar registers).  One reciprocal square root per pivot
+// gives both L(j,j) = d / sqrt(d) and 1/L(j,j); the updates run on every
+// lane unmasked (entries above the diagonal are never read as L and are
+// zeroed at the end).  L goes back to As (upper part zeroed), inv(L) to the
+// same block of Bs.
 template <typename T> __device__ __forceinline__ void chol_inv16(T *As, T *Bs, int c0, int &bad) {
   const int lane = threadIdx.x & 63, i = lane & 15;
   T a[16];
 #pragma unroll
   for (int c = 0; c < 16; c++) a[c] = As[(c0 + c) * LDL + c0 + i];
+  T dinv = (T)0;  // 1 / L(i, i)
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     T d = lane_bcast(a[j], j);
@@ -345,13 +362,11 @@ template <typename T> __device__ __forceinline__ void chol_inv16(T *As, T *Bs, i
       bad = 1;
       d = (T)1;
     }
-    const T sd = sqrt(d), inv = (T)1 / sd;
-    a[j] = (i == j) ? sd : (i > j ? a[j] * inv : a[j]);
+    const T r = rsqrt_nr(d);
+    a[j] = (i == j) ? d * r : a[j] * r;
+    if (i == j) dinv = r;
 #pragma unroll
-    for (int k = j + 1; k < 16; k++) {
-      const T lkj = lane_bcast(a[j], k);
-      if (i >= k) a[k] -= a[j] * lkj;
-    }
+    for (int k = j + 1; k < 16; k++) a[k] -= a[j] * lane_bcast(a[j], k);
   }
 #pragma unroll
   for (int c = 0; c < 16; c++)
@@ -363,7 +378,7 @@ template <typename T> __device__ __forceinline__ void chol_inv16(T *As, T *Bs, i
     T sacc = (r == i) ? (T)1 : (T)0;
 #pragma unroll
     for (int k = 0; k < r; k++) sacc -= lane_bcast(a[k], r) * x[k];
-    x[r] = sacc / lane_bcast(a[r], r);
+    x[r] = sacc * lane_bcast(dinv, r);
   }
   if (lane < 16) {
 #pragma unroll

@@ -265,7 +265,7 @@ struct Device {
   DBuf<int> err;
   std::vector<SubTiles> sub;
   // setup phases
-  std::vector<Phase> a_diag, a_off, p_diag, p_off, z_fwd, z_bwd, w_rows;
+  std::vector<Phase> a_diag, a_off, p_diag, p_off, z_bwd, w_rows;
   // explicit interior inverse (levels below the first): its formation steps,
   // the four GEMV lists of the interior solves, their work vectors
   bool xint = false;
@@ -1359,7 +1359,7 @@ void device_setup_finish(Solver &S) {
       fcls[is] = it->second;
     }
     struct FactorTasks {
-      std::vector<TaskList> p_diag, p_off, z_fwd, z_bwd, w_rows;
+      std::vector<TaskList> p_diag, p_off, z_bwd, w_rows;
       TaskList phi, kinv;
     };
     std::vector<FactorTasks> fts(fkeys.size());
@@ -1372,17 +1372,20 @@ void device_setup_finish(Solver &S) {
       auto Pt = [&](int I, int J) { return L.p + lowidx(n, I, J); };
       auto Yt = [&](int I, int cT) { return L.y + I * ncT + cT; };
       dense_chol_tasks(n, Pt, [&](int J) { return L.linv + J; }, Q.p_diag, Q.p_off);
-      // Z = S'^{-1} C^T by tile sweeps over the factor: forward steps
-      // Y_I = Linv_I (Y_I - sum_{k<I} L_Ik Y_k), backward steps
-      // Y_I = Linv_I^T (Y_I - sum_{k>I} L_kI^T Y_k), one task per tile column
-      Q.z_fwd.resize(ncT ? n : 0);
+      // Z = S'^{-1} C^T by tile sweeps over the factor.  Forward steps
+      // Y_I = Linv_I (Y_I - sum_{k<I} L_Ik Y_k) need the factor's columns
+      // k <= I only: they run in the product launch of column I of the S'
+      // factorization (tasks of their own mode, Y initialized to C^T
+      // before it).  Backward steps Y_I = Linv_I^T (Y_I - sum_{k>I} L_kI^T Y_k)
+      // follow, one launch per tile column.
       Q.z_bwd.resize(ncT ? n : 0);
       for (int cT = 0; cT < ncT; cT++)
         for (int I = 0; I < n; I++) {
-          TaskList &F = Q.z_fwd[I];
+          TaskList &F = Q.p_off[I];
           TileTask t{Yt(I, cT), L.linv + I, (int)F.pairs.size(), 0};
           for (int q = 0; q < I; q++) F.pairs.push_back({Pt(I, q), Yt(q, cT)});
           t.p1 = (int)F.pairs.size();
+          t.mode = task_mode(GEMM_NN, POST_LEFT_X, true);
           F.tasks.push_back(t);
           TaskList &B = Q.z_bwd[n - 1 - I];
           TileTask u{Yt(I, cT), L.linv + I, (int)B.pairs.size(), 0};
@@ -1423,7 +1426,7 @@ void device_setup_finish(Solver &S) {
             Q.kinv.tasks.push_back(t);
           }
       }
-      for (auto *v : {&Q.p_diag, &Q.p_off, &Q.z_fwd, &Q.z_bwd, &Q.w_rows})
+      for (auto *v : {&Q.p_diag, &Q.p_off, &Q.z_bwd, &Q.w_rows})
         for (TaskList &T : *v)
           if (!relative_ids(T, 0, L.span)) bad_ids |= 1;
       if (!relative_ids(Q.phi, 0, L.span)) bad_ids |= 1;
@@ -1436,7 +1439,6 @@ void device_setup_finish(Solver &S) {
     auto fbase_of = [&](int is) { return D.sub[is].fbase; };
     std::vector<TaskList> p_diag = merge_steps(fts, &FactorTasks::p_diag, fcls, fbase_of),
                           p_off = merge_steps(fts, &FactorTasks::p_off, fcls, fbase_of),
-                          z_fwd = merge_steps(fts, &FactorTasks::z_fwd, fcls, fbase_of),
                           z_bwd = merge_steps(fts, &FactorTasks::z_bwd, fcls, fbase_of);
     TaskList phi = materialize(
         fts, [](const FactorTasks &Q) -> const TaskList * { return &Q.phi; }, fcls, fbase_of);
@@ -1446,14 +1448,13 @@ void device_setup_finish(Solver &S) {
     // the explicit inverse's rows and product count only where the setup
     // forms it (option 1); the automatic mode forms it during a solve
     S.t.interface_flops = tile_work(p_diag, 2 * kTileChol) + tile_work(p_off, kTileGemm) +
-                          tile_work(z_fwd, kTileGemm) + tile_work(z_bwd, kTileGemm) +
+                          tile_work(z_bwd, kTileGemm) +
                           kTileGemm * tile_products(phi);
     if (D.inv_planned && !D.inv_auto)
       S.t.interface_flops += tile_work(w_rows, kTileGemm) + kTileGemm * tile_products(kinv);
     D.plan2.verbose = S.verbose;
     upload_phases(D.p_diag, p_diag, D.plan2);
     upload_phases(D.p_off, p_off, D.plan2);
-    upload_phases(D.z_fwd, z_fwd, D.plan2);
     upload_phases(D.z_bwd, z_bwd, D.plan2);
     D.phi.upload(phi, D.plan2);
     upload_phases(D.w_rows, w_rows, D.plan2);
@@ -1851,18 +1852,17 @@ void device_setup_finish(Solver &S) {
   auto factor_phase = [&](auto *fp) {
     if (D.f32i) sprime((const float *)D.ipool32.p, fp);
     else sprime((const double *)D.pool.p, fp);
+    // Y = C^T, then the factorization with the forward steps of Z in its
+    // product launches
+    if (D.gen_constraints)
+      launch_ct_init_gen(st, fp, D.desc.p, nsub, D.dptr.p, D.dcon.p, D.dval.p, D.maxnti);
+    else
+      launch_ct_init(st, fp, D.desc.p, nsub, D.cid.p, D.maxnti);
     for (size_t J = 0; J < D.p_diag.size(); J++) {
       launch_tile_potrf(st, fp, D.p_diag[J].tasks.p, D.p_diag[J].pairs.p, D.p_diag[J].ntask, D.err.p);
       launch_tile_gemm(st, fp, D.p_off[J].tasks.p, D.p_off[J].pairs.p, D.p_off[J].ntask, GEMM_NT,
                        POST_RIGHT_XT, 1.0, -1.0);
     }
-    if (D.gen_constraints)
-      launch_ct_init_gen(st, fp, D.desc.p, nsub, D.dptr.p, D.dcon.p, D.dval.p, D.maxnti);
-    else
-      launch_ct_init(st, fp, D.desc.p, nsub, D.cid.p, D.maxnti);
-    for (size_t i = 0; i < D.z_fwd.size(); i++)
-      launch_tile_gemm(st, fp, D.z_fwd[i].tasks.p, D.z_fwd[i].pairs.p, D.z_fwd[i].ntask, GEMM_NN,
-                       POST_LEFT_X, 1.0, -1.0);
     for (size_t i = 0; i < D.z_bwd.size(); i++)
       launch_tile_gemm(st, fp, D.z_bwd[i].tasks.p, D.z_bwd[i].pairs.p, D.z_bwd[i].ntask, GEMM_TN,
                        POST_LEFT_XT, 1.0, -1.0);

@@ -333,6 +333,15 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
   build_constraints(S);
   if (S.use_user) add_user_constraints(S);
   phase("constraints");
+  // the interface phase needs the constraints only: planned and launched
+  // now, while the host builds the interface numbering and the weights
+  S.adapt_pass = S.use_adaptive != 0;
+  if (!S.plan_only) {
+    const double tdi = wall_time();
+    device_setup_interface(S);
+    th += wall_time() - tdi;  // counted in the plan and factor timings
+    phase("interface phase launched");
+  }
   build_unique_interface(S);  // the exchange sums below run over it
   phase("unique interface");
   apply_boundary_conditions(S);
@@ -347,7 +356,6 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
     S.is_setup = true;
     return;
   }
-  S.adapt_pass = S.use_adaptive != 0;
   device_setup_finish(S);
   // adaptive constraints: face eigenproblems on the preconditioner just set
   // up, then the interface phase again with the new constraints
@@ -356,6 +364,7 @@ void bddcml_setup_preconditioner_c(int *matrixtype, int *use_defaults_int,
     adaptive_constraints(S);
     S.adapt_pass = false;
     phase("adaptive constraints");
+    device_setup_interface(S);
     device_setup_finish(S);
     phase("interface phase again");
   }

@@ -260,6 +260,8 @@ struct Device {
   DBuf<double> cpool;     // coarse matrix: factor, inverse diagonal, W, full inverse
   int64_t ntilesc = 0;
   Arena plan2;            // task lists and job lists of the second half
+  Arena plan2b;           // interface maps, assembly and sweep lists, vectors
+  int nco = 0;            // constraints over all subdomains
   Arena plan3;            // coarse factorization lists (built during the interface phase)
   Arena plan4;            // residual index of the refined form (built during it too)
   DBuf<int> err;
@@ -1298,7 +1300,37 @@ static void build_resid_index(Solver &S) {
   D.plan4.commit(D.up);
 }
 
-void device_setup_finish(Solver &S) {
+// rho of the S' augmentation: the largest interface diagonal entry of the
+// subdomain matrix as the Dirichlet elimination leaves it (a fixed dof's
+// diagonal below 1e-12, or missing, becomes 1.0), read from the uploaded
+// triplets before that elimination runs on the host
+static double interface_rho(const Sub &s) {
+  double rho = 0.0;
+  std::vector<char> seen(s.has_fix ? s.ndofs : 0, 0);
+  for (size_t k = 0; k < s.av.size(); k++) {
+    const int i = s.ai[k];
+    if (i != s.aj[k] || s.perm[i] < s.NO) continue;
+    double v = std::fabs(s.av[k]);
+    if (s.has_fix && s.ifix[i]) {
+      seen[i] = 1;
+      if (v < 1e-12) v = 1.0;
+    }
+    rho = std::max(rho, v);
+  }
+  if (s.has_fix)
+    for (int i = 0; i < s.ndofs; i++)
+      if (s.ifix[i] && !seen[i] && s.perm[i] >= s.NO) rho = std::max(rho, 1.0);
+  return rho > 0.0 ? rho : 1.0;
+}
+
+// Second half of the setup, first part: everything that needs the
+// constraints but not the interface numbering, the weights or the
+// right-hand side -- the factor space, the S' / Z / Phi lists, the
+// constraint maps, the coarse lists -- uploaded and launched (S' assembly,
+// its factorization, Z, G, Phi, the coarse matrices and the coarse
+// factorization) while the host goes on with the unique interface, the
+// boundary conditions and the weights.
+void device_setup_interface(Solver &S) {
   Device &D = *S.dev;
   hipStream_t st = D.up;  // uploads overlap the factorization running on D.st
   const int nsub = S.nsub;
@@ -1525,18 +1557,15 @@ void device_setup_finish(Solver &S) {
     D.maxnti = std::max(D.maxnti, s.nti);
     D.maxncT = std::max(D.maxncT, T.ncT);
   }
-  std::vector<int> map(lv), cidv(lv), csz(co), cptrv(co + 1, 0), cdofv(cdoff[nsub]), cmapv(co);
-  std::vector<double> wv(lv), cvalv(cdoff[nsub]), omegav(co);
+  std::vector<int> cidv(lv), csz(co), cptrv(co + 1, 0), cdofv(cdoff[nsub]), cmapv(co);
+  std::vector<double> cvalv(cdoff[nsub]), omegav(co);
+  D.nco = co;
   int two_constraints = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : two_constraints)
   for (int is = 0; is < nsub; is++) {
     const Sub &s = S.subs[is];
     SubDesc &d = D.hdesc[is];
-    // rho: largest interface diagonal entry of the eliminated matrix
-    double rho = 0.0;
-    for (size_t k = 0; k < s.av.size(); k++)
-      if (s.ai[k] == s.aj[k] && s.perm[s.ai[k]] >= s.NO) rho = std::max(rho, std::fabs(s.av[k]));
-    d.rho = rho > 0.0 ? rho : 1.0;
+    d.rho = interface_rho(s);
     std::vector<int> ref2dev(s.ni);
     for (int k = 0; k < s.ni; k++) ref2dev[s.dev_ifc[k]] = k;
     int *cid_loc = cidv.data() + d.loff;
@@ -1557,16 +1586,6 @@ void device_setup_finish(Solver &S) {
       cmapv[d.coff + c] = s.indrowc[c];
       // omega_c c c^T: rho / |D_c| for a unit row, as the indicator form
       omegav[d.coff + c] = nrm2 > 0.0 ? d.rho / nrm2 : d.rho;
-    }
-    for (int k = 0; k < s.NI; k++) {
-      if (k < s.ni) {
-        int kr = s.dev_ifc[k];
-        map[d.loff + k] = S.gpos[s.isvgvn[s.ifc_dof[kr]]];
-        wv[d.loff + k] = s.w[kr];
-      } else {
-        map[d.loff + k] = -1;
-        wv[d.loff + k] = 0.0;
-      }
     }
   }
   // rows with overlapping supports or values other than 1 (adaptive
@@ -1629,8 +1648,6 @@ void device_setup_finish(Solver &S) {
   D.plan2.add(D.desc, D.hdesc);
   D.plan2.add(D.loff, loff);
   D.plan2.add(D.NIs, NIs);
-  D.plan2.add(D.map, map);
-  D.plan2.add(D.w, wv);
   D.plan2.add(D.cid, cidv);
   D.plan2.add(D.csize, csz);
   D.plan2.add(D.cptr, cptrv);
@@ -1686,108 +1703,6 @@ void device_setup_finish(Solver &S) {
   D.nsrows = (int)sr.size();
   D.nscols = (int)sc.size();
   ph("interface maps");
-  // assembly CSR: unique interface dof -> local flat indices (subdomain order)
-  {
-    std::vector<int> cnt(S.nig + 1, 0);
-    for (int64_t k = 0; k < lv; k++)
-      if (map[k] >= 0) cnt[map[k] + 1]++;
-    for (int g = 0; g < S.nig; g++) cnt[g + 1] += cnt[g];
-    std::vector<int64_t> src(cnt[S.nig]);
-    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-    for (int64_t k = 0; k < lv; k++)
-      if (map[k] >= 0) src[fill[map[k]]++] = k;
-    D.plan2.add(D.aptr, cnt);
-    D.plan2.add(D.asrc, src);
-    // the same contributors in the device-ordered subdomain vectors
-    std::vector<int64_t> voff(nsub);
-    int64_t vt = 0;
-    for (int is = 0; is < nsub; is++) voff[is] = vt, vt += S.subs[is].NO + S.subs[is].NI;
-    D.VT = vt;
-    if (D.xint) {
-      if (D.gx_vt != vt) fatal("BDDCML_DEVICE_PLAN", "interior GEMV offsets");
-      D.ybuf.alloc(std::max<int64_t>(vt, 1));
-      D.zbuf.alloc(std::max<int64_t>(vt, 1));
-      D.ybuf.zero(D.up);
-      D.zbuf.zero(D.up);
-    }
-    D.hodesc.resize(nsub);
-    {
-      int64_t po = 0;
-      for (int is = 0; is < nsub; is++) {
-        D.hodesc[is] = OrderDesc{po, voff[is], S.subs[is].ndofs, D.perm_off[is]};
-        po += S.subs[is].ndofs;
-        D.maxndofs = std::max(D.maxndofs, S.subs[is].ndofs);
-      }
-      D.ndofs_total = po;
-    }
-    D.plan2.add(D.odesc, D.hodesc);
-    std::vector<int64_t> vpos(lv);
-#pragma omp parallel for schedule(static)
-    for (int is = 0; is < nsub; is++)
-      for (int k = 0; k < S.subs[is].NI; k++) vpos[loff[is] + k] = voff[is] + S.subs[is].NO + k;
-    std::vector<int64_t> rsrc(src.size());
-#pragma omp parallel for schedule(static)
-    for (size_t q = 0; q < src.size(); q++) rsrc[q] = vpos[src[q]];
-    D.plan2.add(D.rhs_src, rsrc);
-    // triangular sweep lists of the interior factor: row lists (forward) and
-    // column lists (backward) of the strictly lower factored tiles
-    D.htrsv.resize(nsub);
-    std::vector<size_t> roff(nsub + 1, 0), eoff(nsub + 1, 0), coff_(nsub + 1, 0);
-    int maxnt = 1;
-    for (int is = 0; is < nsub; is++) {
-      const SubTiles &T = D.sub[is];
-      const int nt = T.nto + T.nti;
-      maxnt = std::max(maxnt, nt);
-      size_t ne = 0;
-      for (int J = 0; J < T.nto; J++) ne += T.pat.colptr[J + 1] - T.pat.colptr[J] - 1;
-      roff[is + 1] = roff[is] + nt + 1;
-      coff_[is + 1] = coff_[is] + T.nto + 1;
-      eoff[is + 1] = eoff[is] + ne;
-      D.htrsv[is] = TrsvDesc{T.nto, T.nti, (int)roff[is], (int)coff_[is], T.linvA, voff[is]};
-    }
-    std::vector<int> rp(roff[nsub]), rkv(eoff[nsub]), rtv(eoff[nsub]), cp(coff_[nsub]),
-        ckv(eoff[nsub]), ctv(eoff[nsub]);
-#pragma omp parallel for schedule(dynamic)
-    for (int is = 0; is < nsub; is++) {
-      const SubTiles &T = D.sub[is];
-      const int nt = T.nto + T.nti;
-      const int e0 = (int)eoff[is];
-      // column lists straight from the CSC pattern (diagonal skipped)
-      int e = e0;
-      for (int J = 0; J < T.nto; J++) {
-        cp[coff_[is] + J] = e;
-        for (int q = T.pat.colptr[J] + 1; q < T.pat.colptr[J + 1]; q++)
-          ckv[e] = T.pat.rowidx[q], ctv[e] = T.pat.tile[q], e++;
-      }
-      cp[coff_[is] + T.nto] = e;
-      // row lists by counting sort over rows, columns ascending
-      std::vector<int> cnt(nt + 1, 0);
-      for (int J = 0; J < T.nto; J++)
-        for (int q = T.pat.colptr[J] + 1; q < T.pat.colptr[J + 1]; q++) cnt[T.pat.rowidx[q] + 1]++;
-      for (int I = 0; I < nt; I++) cnt[I + 1] += cnt[I];
-      for (int I = 0; I <= nt; I++) rp[roff[is] + I] = e0 + cnt[I];
-      for (int J = 0; J < T.nto; J++)
-        for (int q = T.pat.colptr[J] + 1; q < T.pat.colptr[J + 1]; q++) {
-          int k = e0 + cnt[T.pat.rowidx[q]]++;
-          rkv[k] = J, rtv[k] = T.pat.tile[q];
-        }
-    }
-    // (subdomains whose vector does not fit in LDS are swept with it in
-    // global memory, see launch_trsv_fwd)
-    for (int id : rtv)
-      if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid sweep tile");
-    for (int id : ctv)
-      if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid sweep tile");
-    D.maxnt = maxnt;
-    D.plan2.add(D.trsv, D.htrsv);
-    D.plan2.add(D.rptr, rp);
-    D.plan2.add(D.rk, rkv);
-    D.plan2.add(D.rt, rtv);
-    D.plan2.add(D.tcptr, cp);
-    D.plan2.add(D.tck, ckv);
-    D.plan2.add(D.tct, ctv);
-  }
-  ph("assembly and sweep lists");
   // coarse gather CSR: coarse dof -> constraint rows (subdomain order)
   {
     std::vector<int> cnt(S.ndofc + 1, 0);
@@ -1799,39 +1714,9 @@ void device_setup_finish(Solver &S) {
     D.plan2.add(D.cgptr, cnt);
     D.plan2.add(D.cgsrc, src);
   }
-  // vectors
-  D.nig = S.nig;
-  D.yl.alloc(lv);
-  D.zl.alloc(lv);
-  D.rcs.alloc(std::max(co, 1));
-  D.rc.alloc(D.NC);
-  D.uc.alloc(D.NC);
-  D.partial.alloc(DOT_BLOCKS * 4);
-  D.dots.alloc(4);
-  for (DBuf<double> *b : {&D.u, &D.r, &D.p, &D.z, &D.ap, &D.rs, &D.v, &D.y, &D.s, &D.t, &D.g,
-                          &D.xin, &D.xout})
-    b->alloc(std::max(S.nig, 1));
-  D.vec.alloc(D.VT);
-  D.plan2.add(D.fixmask, std::vector<char>(S.ig_fixed.begin(), S.ig_fixed.end()));
-  D.rc.zero(st);
-  D.uc.zero(st);
-  {
-    std::vector<double> fm(D.ndofs_total);
-#pragma omp parallel for schedule(static)
-    for (int is = 0; is < nsub; is++)
-      std::copy(S.subs[is].fmod.begin(), S.subs[is].fmod.end(), fm.begin() + D.hodesc[is].poff);
-    D.plan2.add(D.fmod, fm);
-    D.sol.alloc(D.ndofs_total);
-  }
-  D.plan2.commit(st);  // every array of the second half in one copy
-  ph("coarse lists, vectors");
+  D.plan2.commit(st);  // every array of this part in one copy
+  ph("coarse gather, upload");
   HIPCHECK(hipEventRecord(D.ev_up, D.up));
-  // the reduced right-hand side needs only the interior factor (event 2):
-  // on the upload stream, concurrent with the interface phase below
-  HIPCHECK(hipStreamWaitEvent(D.up, D.ev[2], 0));
-  reduced_rhs_launches(S, D.up);
-  HIPCHECK(hipEventRecord(D.ev_rhs, D.up));
-  D.rhs_pending = true;
   S.t.plan += wall_time() - t0;
   // ---- numerics, second half: augmented interface matrices, Z, Phi, coarse
   t0 = wall_time();
@@ -2048,11 +1933,190 @@ void device_setup_finish(Solver &S) {
     launch_tile_mirror(st, cpool, D.mir_c.p, (int)D.mir_c.n);
   }
   HIPCHECK(hipEventRecord(ev[4], st));
-  if (D.f32i) {
-    const double tr = wall_time();
-    build_resid_index(S);
-    S.t.plan += wall_time() - tr;
+  HIPCHECK(hipGetLastError());
+  S.t.factor += wall_time() - t0;
+}
+
+// Second half of the setup, second part (after the unique interface, the
+// boundary conditions, the weights and the condensation right-hand side):
+// the interface maps and weights, the assembly and sweep lists, the solve
+// vectors; the reduced right-hand side on the upload stream; then the wait
+// for the whole setup.
+void device_setup_finish(Solver &S) {
+  Device &D = *S.dev;
+  hipStream_t st = D.up;
+  const int nsub = S.nsub;
+  double t0 = wall_time();
+  double tph = t0;
+  auto ph = [&](const char *name) {
+    double now = wall_time();
+    if (S.verbose >= 2) std::printf("   device plan: %-24s %8.4f s\n", name, now - tph);
+    tph = now;
+  };
+  const int64_t lv = D.LV;
+  const int co = D.nco;
+  std::vector<int64_t> loff(nsub);
+  for (int is = 0; is < nsub; is++) loff[is] = D.hdesc[is].loff;
+  // interface dof of each local interface entry (-1: padding) and its weight
+  std::vector<int> map(lv);
+  std::vector<double> wv(lv);
+#pragma omp parallel for schedule(dynamic)
+  for (int is = 0; is < nsub; is++) {
+    const Sub &s = S.subs[is];
+    const int64_t o = loff[is];
+    for (int k = 0; k < s.NI; k++) {
+      if (k < s.ni) {
+        int kr = s.dev_ifc[k];
+        map[o + k] = S.gpos[s.isvgvn[s.ifc_dof[kr]]];
+        wv[o + k] = s.w[kr];
+      } else {
+        map[o + k] = -1;
+        wv[o + k] = 0.0;
+      }
+    }
   }
+  D.plan2b.verbose = S.verbose;
+  D.plan2b.add(D.map, map);
+  D.plan2b.add(D.w, wv);
+  // assembly CSR: unique interface dof -> local flat indices (subdomain order)
+  {
+    std::vector<int> cnt(S.nig + 1, 0);
+    for (int64_t k = 0; k < lv; k++)
+      if (map[k] >= 0) cnt[map[k] + 1]++;
+    for (int g = 0; g < S.nig; g++) cnt[g + 1] += cnt[g];
+    std::vector<int64_t> src(cnt[S.nig]);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t k = 0; k < lv; k++)
+      if (map[k] >= 0) src[fill[map[k]]++] = k;
+    D.plan2b.add(D.aptr, cnt);
+    D.plan2b.add(D.asrc, src);
+    // the same contributors in the device-ordered subdomain vectors
+    std::vector<int64_t> voff(nsub);
+    int64_t vt = 0;
+    for (int is = 0; is < nsub; is++) voff[is] = vt, vt += S.subs[is].NO + S.subs[is].NI;
+    D.VT = vt;
+    if (D.xint) {
+      if (D.gx_vt != vt) fatal("BDDCML_DEVICE_PLAN", "interior GEMV offsets");
+      D.ybuf.alloc(std::max<int64_t>(vt, 1));
+      D.zbuf.alloc(std::max<int64_t>(vt, 1));
+      D.ybuf.zero(D.up);
+      D.zbuf.zero(D.up);
+    }
+    D.hodesc.resize(nsub);
+    {
+      int64_t po = 0;
+      for (int is = 0; is < nsub; is++) {
+        D.hodesc[is] = OrderDesc{po, voff[is], S.subs[is].ndofs, D.perm_off[is]};
+        po += S.subs[is].ndofs;
+        D.maxndofs = std::max(D.maxndofs, S.subs[is].ndofs);
+      }
+      D.ndofs_total = po;
+    }
+    D.plan2b.add(D.odesc, D.hodesc);
+    std::vector<int64_t> vpos(lv);
+#pragma omp parallel for schedule(static)
+    for (int is = 0; is < nsub; is++)
+      for (int k = 0; k < S.subs[is].NI; k++) vpos[loff[is] + k] = voff[is] + S.subs[is].NO + k;
+    std::vector<int64_t> rsrc(src.size());
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < src.size(); q++) rsrc[q] = vpos[src[q]];
+    D.plan2b.add(D.rhs_src, rsrc);
+    // triangular sweep lists of the interior factor: row lists (forward) and
+    // column lists (backward) of the strictly lower factored tiles
+    D.htrsv.resize(nsub);
+    std::vector<size_t> roff(nsub + 1, 0), eoff(nsub + 1, 0), coff_(nsub + 1, 0);
+    int maxnt = 1;
+    for (int is = 0; is < nsub; is++) {
+      const SubTiles &T = D.sub[is];
+      const int nt = T.nto + T.nti;
+      maxnt = std::max(maxnt, nt);
+      size_t ne = 0;
+      for (int J = 0; J < T.nto; J++) ne += T.pat.colptr[J + 1] - T.pat.colptr[J] - 1;
+      roff[is + 1] = roff[is] + nt + 1;
+      coff_[is + 1] = coff_[is] + T.nto + 1;
+      eoff[is + 1] = eoff[is] + ne;
+      D.htrsv[is] = TrsvDesc{T.nto, T.nti, (int)roff[is], (int)coff_[is], T.linvA, voff[is]};
+    }
+    std::vector<int> rp(roff[nsub]), rkv(eoff[nsub]), rtv(eoff[nsub]), cp(coff_[nsub]),
+        ckv(eoff[nsub]), ctv(eoff[nsub]);
+#pragma omp parallel for schedule(dynamic)
+    for (int is = 0; is < nsub; is++) {
+      const SubTiles &T = D.sub[is];
+      const int nt = T.nto + T.nti;
+      const int e0 = (int)eoff[is];
+      // column lists straight from the CSC pattern (diagonal skipped)
+      int e = e0;
+      for (int J = 0; J < T.nto; J++) {
+        cp[coff_[is] + J] = e;
+        for (int q = T.pat.colptr[J] + 1; q < T.pat.colptr[J + 1]; q++)
+          ckv[e] = T.pat.rowidx[q], ctv[e] = T.pat.tile[q], e++;
+      }
+      cp[coff_[is] + T.nto] = e;
+      // row lists by counting sort over rows, columns ascending
+      std::vector<int> cnt(nt + 1, 0);
+      for (int J = 0; J < T.nto; J++)
+        for (int q = T.pat.colptr[J] + 1; q < T.pat.colptr[J + 1]; q++) cnt[T.pat.rowidx[q] + 1]++;
+      for (int I = 0; I < nt; I++) cnt[I + 1] += cnt[I];
+      for (int I = 0; I <= nt; I++) rp[roff[is] + I] = e0 + cnt[I];
+      for (int J = 0; J < T.nto; J++)
+        for (int q = T.pat.colptr[J] + 1; q < T.pat.colptr[J + 1]; q++) {
+          int k = e0 + cnt[T.pat.rowidx[q]]++;
+          rkv[k] = J, rtv[k] = T.pat.tile[q];
+        }
+    }
+    // (subdomains whose vector does not fit in LDS are swept with it in
+    // global memory, see launch_trsv_fwd)
+    for (int id : rtv)
+      if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid sweep tile");
+    for (int id : ctv)
+      if (id < 0 || id >= D.ntiles) fatal("BDDCML_DEVICE_PLAN", "invalid sweep tile");
+    D.maxnt = maxnt;
+    D.plan2b.add(D.trsv, D.htrsv);
+    D.plan2b.add(D.rptr, rp);
+    D.plan2b.add(D.rk, rkv);
+    D.plan2b.add(D.rt, rtv);
+    D.plan2b.add(D.tcptr, cp);
+    D.plan2b.add(D.tck, ckv);
+    D.plan2b.add(D.tct, ctv);
+  }
+  ph("assembly and sweep lists");
+  // vectors
+  D.nig = S.nig;
+  D.yl.alloc(lv);
+  D.zl.alloc(lv);
+  D.rcs.alloc(std::max(co, 1));
+  D.rc.alloc(D.NC);
+  D.uc.alloc(D.NC);
+  D.partial.alloc(DOT_BLOCKS * 4);
+  D.dots.alloc(4);
+  for (DBuf<double> *b : {&D.u, &D.r, &D.p, &D.z, &D.ap, &D.rs, &D.v, &D.y, &D.s, &D.t, &D.g,
+                          &D.xin, &D.xout})
+    b->alloc(std::max(S.nig, 1));
+  D.vec.alloc(D.VT);
+  D.plan2b.add(D.fixmask, std::vector<char>(S.ig_fixed.begin(), S.ig_fixed.end()));
+  D.rc.zero(st);
+  D.uc.zero(st);
+  {
+    std::vector<double> fm(D.ndofs_total);
+#pragma omp parallel for schedule(static)
+    for (int is = 0; is < nsub; is++)
+      std::copy(S.subs[is].fmod.begin(), S.subs[is].fmod.end(), fm.begin() + D.hodesc[is].poff);
+    D.plan2b.add(D.fmod, fm);
+    D.sol.alloc(D.ndofs_total);
+  }
+  D.plan2b.commit(st);
+  ph("interface maps, vectors");
+  // the reduced right-hand side needs only the interior factor (event 2):
+  // on the upload stream, concurrent with the interface phase
+  HIPCHECK(hipStreamWaitEvent(D.up, D.ev[2], 0));
+  reduced_rhs_launches(S, D.up);
+  HIPCHECK(hipEventRecord(D.ev_rhs, D.up));
+  D.rhs_pending = true;
+  if (D.f32i) build_resid_index(S);
+  S.t.plan += wall_time() - t0;
+  t0 = wall_time();
+  st = D.st;
+  hipEvent_t *ev = D.ev;
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(st));
   {

@@ -257,6 +257,7 @@ void setup_second_level(Solver &S) {
   if (!T.plan_only) device_setup_begin(T);
   build_globs(T);
   build_constraints(T);
+  if (!T.plan_only) device_setup_interface(T);
   build_unique_interface(T);
   apply_boundary_conditions(T);
   build_weights(T);

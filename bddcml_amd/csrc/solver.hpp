@@ -198,6 +198,9 @@ void set_pcg_graph(bool on);
 // copies the solution of the last solve to the host (once; no-op otherwise)
 void device_fetch_solution(Solver &S);
 void device_setup_begin(Solver &S);
+// after the constraints: the interface phase planned and launched
+void device_setup_interface(Solver &S);
+// after the weights and the condensation rhs: the rest, then the wait
 void device_setup_finish(Solver &S);
 // the subdomain pool holding the packed lower tiles of every S_i, with the
 // first tile and the tile count of each (device.hip)
@@ -208,7 +211,7 @@ const double *device_schur_tiles(Solver &S, std::vector<int> &slow, std::vector<
 // the threshold loaded as constraints of the two faces, each face block
 // orthonormalized, the coarse dofs renumbered (src/module_adaptivity.f90:
 // 296-3024, src/module_dd.f90:4386-4457, 4637-4802); the interface phase is
-// then set up again (device_setup_finish)
+// then set up again (device_setup_interface, device_setup_finish)
 void adaptive_constraints(Solver &S);
 // constraints.cpp: orthonormal rows spanning the rows of M (r x n) by QR with
 // column pivoting, keeping |R_kk| >= 0.01 |R_11| (src/module_dd.f90:

@@ -2112,7 +2112,10 @@ void device_setup_finish(Solver &S) {
   reduced_rhs_launches(S, D.up);
   HIPCHECK(hipEventRecord(D.ev_rhs, D.up));
   D.rhs_pending = true;
-  if (D.f32i) build_resid_index(S);
+  if (D.f32i) {
+    build_resid_index(S);
+    ph("residual index");
+  }
   S.t.plan += wall_time() - t0;
   t0 = wall_time();
   st = D.st;

@@ -2106,16 +2106,18 @@ void device_setup_finish(Solver &S) {
   }
   D.plan2b.commit(st);
   ph("interface maps, vectors");
+  // the residual index of the refined form first: its upload synchronizes
+  // the upload stream, which then takes the reduced right-hand side
+  if (D.f32i) {
+    build_resid_index(S);
+    ph("residual index");
+  }
   // the reduced right-hand side needs only the interior factor (event 2):
   // on the upload stream, concurrent with the interface phase
   HIPCHECK(hipStreamWaitEvent(D.up, D.ev[2], 0));
   reduced_rhs_launches(S, D.up);
   HIPCHECK(hipEventRecord(D.ev_rhs, D.up));
   D.rhs_pending = true;
-  if (D.f32i) {
-    build_resid_index(S);
-    ph("residual index");
-  }
   S.t.plan += wall_time() - t0;
   t0 = wall_time();
   st = D.st;

@@ -341,20 +341,23 @@ __device__ __forceinline__ float rsqrt_nr(float d) {
   return fmaf(y, fmaf(-0.5f * d * y, y, 0.5f), y);
 }
 
-// Cholesky factor L of the 16x16 diagonal block at (c0, c0) and its inverse,
-// by one wave in registers: lane i (mod 16) holds row i of the block, then
-// column i of inv(L); every cross-lane value is a broadcast from one lane
-// (lane reads into scalar registers).  One reciprocal square root per pivot
-// gives both L(j,j) = d / sqrt(d) and 1/L(j,j); the updates run on every
-// lane unmasked (entries above the diagonal are never read as L and are
-// zeroed at the end).  L goes back to As (upper part zeroed), inv(L) to the
-// same block of Bs.
-template <typename T> __device__ __forceinline__ void chol_inv16(T *As, T *Bs, int c0, int &bad) {
-  const int lane = threadIdx.x & 63, i = lane & 15;
+// Cholesky factor of block column J (16 columns from c0, rows c0..63) by
+// one wave in registers: lane l holds row c0 + l, right-looking over the
+// block's columns, so the rows below the diagonal block come out as L(R, J)
+// with no separate panel solve.  Every cross-lane value is a broadcast from
+// one lane (lane reads into scalar registers).  One reciprocal square root
+// per pivot gives both L(j,j) = d / sqrt(d) and 1/L(j,j) (kept in dv); the
+// updates run on every lane unmasked (entries above the diagonal are never
+// read as L and are zeroed).
+template <typename T> __device__ __forceinline__ void chol_panel16(T *As, int c0, T *dv, int &bad) {
+  const int l = threadIdx.x & 63;
+  const int r = c0 + l;
+  const bool live = r < 64;
+  const int rr = live ? r : 63;  // lanes past the last row compute a copy
   T a[16];
 #pragma unroll
-  for (int c = 0; c < 16; c++) a[c] = As[(c0 + c) * LDL + c0 + i];
-  T dinv = (T)0;  // 1 / L(i, i)
+  for (int c = 0; c < 16; c++) a[c] = As[(c0 + c) * LDL + rr];
+  T dinv = (T)0;
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     T d = lane_bcast(a[j], j);
@@ -362,43 +365,58 @@ template <typename T> __device__ __forceinline__ void chol_inv16(T *As, T *Bs, i
       bad = 1;
       d = (T)1;
     }
-    const T r = rsqrt_nr(d);
-    a[j] = (i == j) ? d * r : a[j] * r;
-    if (i == j) dinv = r;
+    const T q = rsqrt_nr(d);
+    a[j] = (l == j) ? d * q : a[j] * q;
+    if (l == j) dinv = q;
 #pragma unroll
     for (int k = j + 1; k < 16; k++) a[k] -= a[j] * lane_bcast(a[j], k);
   }
+  if (l < 16) {
 #pragma unroll
-  for (int c = 0; c < 16; c++)
-    if (c > i) a[c] = (T)0;
-  // column i of inv(L) by forward substitution; L(r, k) is a[k] of lane r
+    for (int c = 0; c < 16; c++)
+      if (c > l) a[c] = (T)0;
+    dv[c0 + l] = dinv;
+  }
+  if (live) {
+#pragma unroll
+    for (int c = 0; c < 16; c++) As[(c0 + c) * LDL + r] = a[c];
+  }
+}
+
+// inverse of the 16x16 lower diagonal block at (c0, c0) (L in As, 1/L(j,j)
+// in dv) into the same block of Bs, by one wave: lane i (mod 16) computes
+// column i by forward substitution; L(r, k) is a[k] of lane r
+template <typename T>
+__device__ __forceinline__ void inv16(const T *As, T *Bs, const T *dv, int c0) {
+  const int lane = threadIdx.x & 63, i = lane & 15;
+  T a[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) a[c] = As[(c0 + c) * LDL + c0 + i];
   T x[16];
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     T sacc = (r == i) ? (T)1 : (T)0;
 #pragma unroll
     for (int k = 0; k < r; k++) sacc -= lane_bcast(a[k], r) * x[k];
-    x[r] = sacc * lane_bcast(dinv, r);
+    x[r] = sacc * dv[c0 + r];
   }
   if (lane < 16) {
-#pragma unroll
-    for (int c = 0; c < 16; c++) As[(c0 + c) * LDL + c0 + i] = a[c];
 #pragma unroll
     for (int r = 0; r < 16; r++) Bs[(c0 + i) * LDL + c0 + r] = x[r];
   }
 }
 
-// C = C - sum A B^T over the task's pairs; C = chol(C) (lower, upper zeroed);
-// X = inv(chol(C)).  Blocked by 16: diagonal blocks in registers (chol_inv16),
-// panel solves, trailing updates and the off-diagonal blocks of the inverse
-// (X_IJ = -inv(L_II) sum_{J<=K<I} L_IK X_KJ) on the matrix cores.
 // C = C - sum of the task's pairs A B^T, then its Cholesky factor L into C
-// and inv(L) into tile t.x; a non-positive pivot sets *err
+// and inv(L) into tile t.x; a non-positive pivot sets *err.  Blocked by 16:
+// per block column the update by the columns before it on the matrix
+// cores, then its panel Cholesky by one wave; then the four diagonal-block
+// inverses, one per wave, and the off-diagonal blocks of the inverse
+// (X_IJ = -inv(L_II) sum_{J<=K<I} L_IK X_KJ) on the matrix cores.
 template <typename T>
 __device__ __forceinline__ void tile_potrf_task(T *__restrict__ pool, const TileTask t,
                                                 const TilePair *__restrict__ pairs,
                                                 int *__restrict__ err, T *As, T *Bs,
-                                                T (*Ts)[16 * 17], int &bad) {
+                                                T (*Ts)[16 * 17], T *dv, int &bad) {
   typedef typename Mfma<T>::v4 v4;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   v4 acc[2][2];
@@ -413,27 +431,20 @@ __device__ __forceinline__ void tile_potrf_task(T *__restrict__ pool, const Tile
   for (int J = 0; J < 4; J++) {
     const int c0 = 16 * J;
     // block column J -= L(:, 0:c0) L(J rows, 0:c0)^T, one row block per wave
-    if (J > 0 && wave < 4 - J) {
-      const int R = J + wave;
-      v4 u = (v4){0, 0, 0, 0};
-      u = mfma16<T>(u, As + 16 * R, 1, LDL, As + c0, 1, LDL, c0);
+    if (J > 0) {
+      if (wave < 4 - J) {
+        const int R = J + wave;
+        v4 u = (v4){0, 0, 0, 0};
+        u = mfma16<T>(u, As + 16 * R, 1, LDL, As + c0, 1, LDL, c0);
 #pragma unroll
-      for (int v = 0; v < 4; v++) As[(c0 + ocol) * LDL + 16 * R + Mfma<T>::row(lane, v)] -= u[v];
+        for (int v = 0; v < 4; v++) As[(c0 + ocol) * LDL + 16 * R + Mfma<T>::row(lane, v)] -= u[v];
+      }
+      __syncthreads();
     }
-    __syncthreads();
     if (wave == 0) {
       int b = 0;
-      chol_inv16<T>(As, Bs, c0, b);
+      chol_panel16<T>(As, c0, dv, b);
       if (b && lane == 0) bad = 1;
-    }
-    __syncthreads();
-    // panel: L(R, J) = A(R, J) inv(L_JJ)^T
-    if (wave < 3 - J) {
-      const int R = J + 1 + wave;
-      v4 u = (v4){0, 0, 0, 0};
-      u = mfma16<T>(u, As + c0 * LDL + 16 * R, 1, LDL, Bs + c0 * LDL + c0, 1, LDL, 16);
-#pragma unroll
-      for (int v = 0; v < 4; v++) As[(c0 + ocol) * LDL + 16 * R + Mfma<T>::row(lane, v)] = u[v];
     }
     __syncthreads();
   }
@@ -442,6 +453,8 @@ __device__ __forceinline__ void tile_potrf_task(T *__restrict__ pool, const Tile
     int r = e & 63, c = e >> 6;
     if (r < c) As[c * LDL + r] = (T)0;
   }
+  inv16<T>(As, Bs, dv, 16 * wave);
+  __syncthreads();
   // off-diagonal blocks of inv(L), one block row at a time
   for (int I = 1; I < 4; I++) {
     const int Jb = wave;
@@ -474,8 +487,9 @@ __global__ __launch_bounds__(NTHR) void k_tile_potrf(T *__restrict__ pool,
   __shared__ T As[64 * LDL];
   __shared__ T Bs[64 * LDL];
   __shared__ T Ts[4][16 * 17];
+  __shared__ T dv[64];
   __shared__ int bad;
-  tile_potrf_task<T>(pool, tasks[task_index(chunked)], pairs, err, As, Bs, Ts, bad);
+  tile_potrf_task<T>(pool, tasks[task_index(chunked)], pairs, err, As, Bs, Ts, dv, bad);
 }
 
 // A launch whose tasks are tile products with their own mode (task_mode) or
@@ -489,10 +503,11 @@ __global__ __launch_bounds__(NTHR) void k_tile_step(double *__restrict__ pool,
   __shared__ double As[64 * LDL];
   __shared__ double Bs[64 * LDL];
   __shared__ double Ts[4][16 * 17];
+  __shared__ double dv[64];
   __shared__ int bad;
   const TileTask t = tasks[task_index(chunked)];
   if (t.mode == kTaskPotrf)
-    tile_potrf_task<double>(pool, t, pairs, err, As, Bs, Ts, bad);
+    tile_potrf_task<double>(pool, t, pairs, err, As, Bs, Ts, dv, bad);
   else
     tile_gemm_task_lds<double>(pool, t, pairs, t.mode & 3, (t.mode >> 2) & 3, 1.0,
                                (t.mode & 16) ? -1.0 : 0.0, As, Bs);

@@ -136,6 +136,21 @@ static Staging &staging() {
   return *s;
 }
 
+// copy of a large plan array into the staging area by all host threads
+static void copy_par(char *dst, const char *src, size_t n) {
+  constexpr size_t chunk = (size_t)1 << 18;
+  if (n < 4 * chunk) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const int64_t nc = (int64_t)((n + chunk - 1) / chunk);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < nc; c++) {
+    const size_t o = (size_t)c * chunk;
+    std::memcpy(dst + o, src + o, std::min(chunk, n - o));
+  }
+}
+
 // Plan arrays of one setup half, packed into the pinned staging area and
 // sent to one device allocation with one copy; each View is fixed up to
 // its place in that allocation at commit.
@@ -151,7 +166,7 @@ struct Arena {
     if (!open) h.used = 0, open = true;
     const size_t off = (h.used + 255) / 256 * 256;
     h.grow(off + src.size() * sizeof(T));
-    if (!src.empty()) std::memcpy(h.p + off, src.data(), src.size() * sizeof(T));
+    if (!src.empty()) copy_par(h.p + off, reinterpret_cast<const char *>(src.data()), src.size() * sizeof(T));
     h.used = off + src.size() * sizeof(T);
     v.n = src.size();
     fix.push_back({off, reinterpret_cast<void **>(&v.p)});
@@ -1958,7 +1973,8 @@ void device_setup_finish(Solver &S) {
   std::vector<int64_t> loff(nsub);
   for (int is = 0; is < nsub; is++) loff[is] = D.hdesc[is].loff;
   // interface dof of each local interface entry (-1: padding) and its weight
-  std::vector<int> map(lv);
+  // devpos: device position of each reference interface index
+  std::vector<int> map(lv), devpos(lv);
   std::vector<double> wv(lv);
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < nsub; is++) {
@@ -1969,6 +1985,7 @@ void device_setup_finish(Solver &S) {
         int kr = s.dev_ifc[k];
         map[o + k] = S.gpos[s.isvgvn[s.ifc_dof[kr]]];
         wv[o + k] = s.w[kr];
+        devpos[o + kr] = k;
       } else {
         map[o + k] = -1;
         wv[o + k] = 0.0;
@@ -1978,16 +1995,27 @@ void device_setup_finish(Solver &S) {
   D.plan2b.verbose = S.verbose;
   D.plan2b.add(D.map, map);
   D.plan2b.add(D.w, wv);
-  // assembly CSR: unique interface dof -> local flat indices (subdomain order)
+  ph("interface dofs, weights");
+  // assembly CSR: unique interface dof -> local flat indices (subdomain
+  // order), from the copy lists of the unique interface (ig_ptr, ig_sub,
+  // ig_k: the copies of each dof in subdomain order, by reference
+  // interface index) through each subdomain's device positions
   {
-    std::vector<int> cnt(S.nig + 1, 0);
-    for (int64_t k = 0; k < lv; k++)
-      if (map[k] >= 0) cnt[map[k] + 1]++;
-    for (int g = 0; g < S.nig; g++) cnt[g + 1] += cnt[g];
-    std::vector<int64_t> src(cnt[S.nig]);
-    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-    for (int64_t k = 0; k < lv; k++)
-      if (map[k] >= 0) src[fill[map[k]]++] = k;
+    const std::vector<int> &cnt = S.ig_ptr;
+    if ((int)cnt.size() != S.nig + 1) fatal("BDDCML_DEVICE_PLAN", "interface numbering missing");
+    const int64_t ncopy = cnt[S.nig];
+    std::vector<int64_t> src(ncopy);
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int64_t q = 0; q < ncopy; q++) {
+      const int is = S.ig_sub[q], kr = S.ig_k[q];
+      if (is < 0 || is >= nsub || kr < 0 || kr >= S.subs[is].ni) {
+        bad = 1;
+        continue;
+      }
+      src[q] = loff[is] + devpos[loff[is] + kr];
+    }
+    if (bad) fatal("BDDCML_DEVICE_PLAN", "interface copy lists");
     D.plan2b.add(D.aptr, cnt);
     D.plan2b.add(D.asrc, src);
     // the same contributors in the device-ordered subdomain vectors
@@ -2021,6 +2049,7 @@ void device_setup_finish(Solver &S) {
 #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < src.size(); q++) rsrc[q] = vpos[src[q]];
     D.plan2b.add(D.rhs_src, rsrc);
+    ph("assembly lists");
     // triangular sweep lists of the interior factor: row lists (forward) and
     // column lists (backward) of the strictly lower factored tiles
     D.htrsv.resize(nsub);
@@ -2079,7 +2108,7 @@ void device_setup_finish(Solver &S) {
     D.plan2b.add(D.tck, ckv);
     D.plan2b.add(D.tct, ctv);
   }
-  ph("assembly and sweep lists");
+  ph("sweep lists");
   // vectors
   D.nig = S.nig;
   D.yl.alloc(lv);
@@ -2104,8 +2133,9 @@ void device_setup_finish(Solver &S) {
     D.plan2b.add(D.fmod, fm);
     D.sol.alloc(D.ndofs_total);
   }
+  ph("vectors");
   D.plan2b.commit(st);
-  ph("interface maps, vectors");
+  ph("upload");
   // the residual index of the refined form first: its upload synchronizes
   // the upload stream, which then takes the reduced right-hand side
   if (D.f32i) {

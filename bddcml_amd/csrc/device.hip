@@ -64,27 +64,45 @@ double wall_time() {
 // host seconds spent in device allocations and uploads (setup timers)
 static double g_alloc_s = 0.0, g_upload_s = 0.0;
 
+// A device array: its own allocation, or (carve) a range of another
+// DBuf's allocation, which that one frees
 template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0;
+  bool owned = true;
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
-  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr, o.n = 0; }
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), owned(o.owned) { o.p = nullptr, o.n = 0, o.owned = true; }
   DBuf &operator=(DBuf &&o) noexcept {
     if (this != &o) {
       release();
-      p = o.p, n = o.n;
-      o.p = nullptr, o.n = 0;
+      p = o.p, n = o.n, owned = o.owned;
+      o.p = nullptr, o.n = 0, o.owned = true;
     }
     return *this;
   }
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    owned = true;
+  }
+  // cnt elements from *off (in elements, kept 256-byte aligned) of block
+  void carve(DBuf<T> &block, size_t &off, size_t cnt) {
+    release();
+    const size_t align = 256 / sizeof(T);
+    off = (off + align - 1) / align * align;
+    if (off + std::max<size_t>(cnt, 1) > block.n) {
+      std::fprintf(stderr, "ERROR in DEVICE_ALLOC: carved range outside its block\n");
+      std::abort();
+    }
+    p = block.p + off;
+    n = cnt;
+    owned = false;
+    off += std::max<size_t>(cnt, 1);
   }
   void alloc(size_t cnt) {
     const double t = wall_time();
@@ -367,6 +385,7 @@ struct Device {
   View<double> fmod;  // right-hand side after Dirichlet elimination (rewritten in place)
   DBuf<double> sol;
   // vectors
+  DBuf<double> vblock;  // the allocation the solve vectors are carved from
   DBuf<double> yl, zl, rcs, rc, uc, partial, dots;
   DBuf<double> pcg;           // PCG scalars per iteration: r.z, p.Ap, r.r
   DBuf<double> recV, recW;    // recycling basis V and W = S V (columns of length nig)
@@ -2109,19 +2128,25 @@ void device_setup_finish(Solver &S) {
     D.plan2b.add(D.tct, ctv);
   }
   ph("sweep lists");
-  // vectors
+  // vectors, carved from one allocation
   D.nig = S.nig;
-  D.yl.alloc(lv);
-  D.zl.alloc(lv);
-  D.rcs.alloc(std::max(co, 1));
-  D.rc.alloc(D.NC);
-  D.uc.alloc(D.NC);
-  D.partial.alloc(DOT_BLOCKS * 4);
-  D.dots.alloc(4);
-  for (DBuf<double> *b : {&D.u, &D.r, &D.p, &D.z, &D.ap, &D.rs, &D.v, &D.y, &D.s, &D.t, &D.g,
-                          &D.xin, &D.xout})
-    b->alloc(std::max(S.nig, 1));
-  D.vec.alloc(D.VT);
+  {
+    DBuf<double> *krylov[] = {&D.u, &D.r, &D.p, &D.z, &D.ap, &D.rs, &D.v, &D.y, &D.s, &D.t, &D.g,
+                              &D.xin, &D.xout};
+    const size_t nk = sizeof(krylov) / sizeof(krylov[0]);
+    const size_t sizes[] = {(size_t)lv, (size_t)lv, (size_t)std::max(co, 1), (size_t)D.NC,
+                            (size_t)D.NC, (size_t)DOT_BLOCKS * 4, 4, (size_t)D.VT};
+    DBuf<double> *fixed[] = {&D.yl, &D.zl, &D.rcs, &D.rc, &D.uc, &D.partial, &D.dots, &D.vec};
+    size_t total = 0;
+    for (size_t k = 0; k < 8; k++) total += sizes[k] + 32;
+    total += nk * ((size_t)std::max(S.nig, 1) + 32);
+    for (DBuf<double> *b : fixed) b->release();
+    for (DBuf<double> *b : krylov) b->release();
+    D.vblock.alloc(total);
+    size_t off = 0;
+    for (size_t k = 0; k < 8; k++) fixed[k]->carve(D.vblock, off, sizes[k]);
+    for (DBuf<double> *b : krylov) b->carve(D.vblock, off, std::max(S.nig, 1));
+  }
   D.plan2b.add(D.fixmask, std::vector<char>(S.ig_fixed.begin(), S.ig_fixed.end()));
   D.rc.zero(st);
   D.uc.zero(st);

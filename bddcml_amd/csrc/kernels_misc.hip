@@ -44,15 +44,18 @@ __global__ __launch_bounds__(256) void k_sprime(const P *__restrict__ pool,
   T *dst = ppool + (size_t)jdst[job] * TILE_ELEMS;
   const int *ci = cid + d.loff;
   const int tid = threadIdx.x;
+  const int r0 = 2 * (tid & 31), c0 = tid >> 5;
+  // the tile's loads go out before the constraint ids are staged
+  double2 v[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) v[k] = load2(src + (c0 + 8 * k) * 64 + r0);
+  const int a0 = ci[I * 64 + r0], a1 = ci[I * 64 + r0 + 1];
   if (tid < 64) ccol[tid] = ci[J * 64 + tid];
   __syncthreads();
-  const int r0 = 2 * (tid & 31), c0 = tid >> 5;
-  const int a0 = ci[I * 64 + r0], a1 = ci[I * 64 + r0 + 1];
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const int c = c0 + 8 * k;
-    const double2 v = load2(src + c * 64 + r0);
-    double x = v.x, y = v.y;
+    double x = v[k].x, y = v[k].y;
     const int b = ccol[c];
     if (b >= 0 && (a0 == b || a1 == b)) {
       const double add = d.rho / (double)csize[b];

@@ -420,7 +420,54 @@ static bool has_level2(const Solver &S) { return S.nl >= 3 && S.nsub2 >= 2; }
 struct DeviceInput {
   std::vector<DBuf<int>> ai, aj;
   std::vector<DBuf<double>> av;
+  DBuf<char> cmp;  // pointer tables and flags of the pattern comparison
 };
+
+// Ordering classes: the triplet pattern of each subdomain i compared with
+// that of its candidate cand[i] where the uploaded copies are (HBM), instead
+// of reading both patterns on the host.  eq[i] = 1 when equal.  False (eq
+// untouched) when the host arrays are not the uploaded ones (a Dirichlet
+// elimination added diagonal entries) or in the plan-only mode.
+bool device_same_patterns(Solver &S, const std::vector<int> &cand, std::vector<char> &eq) {
+  if (S.plan_only || !S.din) return false;
+  DeviceInput &I = *S.din;
+  const int n = S.nsub;
+  if ((int)I.ai.size() != n || (int)I.aj.size() != n || (int)cand.size() != n) return false;
+  std::vector<uint64_t> pa(n), pb(n);
+  std::vector<int64_t> cnt(n);
+  std::vector<int> c2(cand);
+  for (int i = 0; i < n; i++) {
+    const Sub &s = S.subs[i];
+    if (I.ai[i].n != s.ai.size() || I.aj[i].n != s.aj.size() || s.ai.size() != s.aj.size()) return false;
+    pa[i] = (uint64_t)(uintptr_t)I.ai[i].p;
+    pb[i] = (uint64_t)(uintptr_t)I.aj[i].p;
+    cnt[i] = (int64_t)s.ai.size();
+  }
+  eq.assign(n, 1);
+  for (int i = 0; i < n; i++)
+    if (c2[i] < 0 || c2[i] >= n) return false;
+    else if (c2[i] != i && cnt[c2[i]] != cnt[i]) eq[i] = 0, c2[i] = i;  // lengths differ
+  // one buffer: pointers, lengths, candidates, flags
+  const size_t o1 = 8 * (size_t)n, o2 = 16 * (size_t)n, o3 = 24 * (size_t)n, o4 = 28 * (size_t)n,
+               bytes = 32 * (size_t)n;
+  if (I.cmp.n < bytes) I.cmp.alloc(bytes);
+  std::vector<char> h(bytes, 0);
+  std::memcpy(h.data(), pa.data(), 8 * (size_t)n);
+  std::memcpy(h.data() + o1, pb.data(), 8 * (size_t)n);
+  std::memcpy(h.data() + o2, cnt.data(), 8 * (size_t)n);
+  std::memcpy(h.data() + o3, c2.data(), 4 * (size_t)n);
+  char *d = I.cmp.p;
+  HIPCHECK(hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice));
+  launch_same_pattern(nullptr, reinterpret_cast<const uint64_t *>(d),
+                      reinterpret_cast<const uint64_t *>(d + o1), reinterpret_cast<const int *>(d + o3),
+                      reinterpret_cast<const int64_t *>(d + o2), reinterpret_cast<int *>(d + o4), n);
+  HIPCHECK(hipGetLastError());
+  std::vector<int> diff(n);
+  HIPCHECK(hipMemcpy(diff.data(), d + o4, 4 * (size_t)n, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; i++)
+    if (diff[i]) eq[i] = 0;
+  return true;
+}
 
 Solver::Solver() = default;
 Solver::~Solver() = default;

@@ -77,6 +77,11 @@ void launch_scatter_sub(hipStream_t st, T *pool, const ScatterSub *subs, int nsu
 // pool[idx[k]] = 1.0
 void launch_scatter_ones(hipStream_t st, double *pool, const int64_t *idx, int64_t n);
 
+// diff[i] = 1 where the triplet pattern (ai, aj: device pointers) of
+// subdomain i differs from that of cand[i] (equal lengths cnt)
+void launch_same_pattern(hipStream_t st, const uint64_t *pai, const uint64_t *paj, const int *cand,
+                         const int64_t *cnt, int *diff, int nsub);
+
 // ---- interface preconditioner pieces ------------------------------------
 // Descriptor of one subdomain's dense interface matrices.
 struct SubDesc {

@@ -7,6 +7,30 @@
 
 namespace bddc {
 
+// ------------------------------------------------- ordering class checks
+// The uploaded triplet pattern (ai, aj) of subdomain i against that of its
+// class candidate cand[i] (same length, checked by the caller): diff[i] = 1
+// on any difference.  All writers of diff[i] write 1.
+__global__ void k_same_pattern(const uint64_t *__restrict__ pai, const uint64_t *__restrict__ paj,
+                               const int *__restrict__ cand, const int64_t *__restrict__ cnt,
+                               int *__restrict__ diff) {
+  const int i = blockIdx.y, c = cand[i];
+  if (c == i) return;
+  const int64_t n = cnt[i];
+  const int *a0 = reinterpret_cast<const int *>(pai[i]), *a1 = reinterpret_cast<const int *>(pai[c]);
+  const int *b0 = reinterpret_cast<const int *>(paj[i]), *b1 = reinterpret_cast<const int *>(paj[c]);
+  int bad = 0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    bad |= (a0[k] != a1[k]) | (b0[k] != b1[k]);
+  if (bad) diff[i] = 1;
+}
+void launch_same_pattern(hipStream_t st, const uint64_t *pai, const uint64_t *paj, const int *cand,
+                         const int64_t *cnt, int *diff, int nsub) {
+  if (nsub <= 0) return;
+  hipLaunchKernelGGL(k_same_pattern, dim3(16, nsub), dim3(256), 0, st, pai, paj, cand, cnt, diff);
+}
+
 // --------------------------------------------------------- S' = S + rho C^T C
 __device__ __forceinline__ void store2(float *p, double x, double y) {
   *reinterpret_cast<float2 *>(p) = make_float2((float)x, (float)y);

@@ -333,14 +333,17 @@ static void globs_second_pass(Solver &S, int is) {
 // integer structure; same(r, i) decides.  Candidates are compared with the
 // first subdomain of their hash bucket in parallel; the rare mismatches are
 // resolved serially.
-template <class H, class E>
-static void structure_classes(int n, H hash, E same, std::vector<int> &cls, std::vector<int> &reps) {
+// pre(cand) runs once the candidates are known (before any same(cand[i], i)).
+template <class H, class E, class P>
+static void structure_classes(int n, H hash, E same, std::vector<int> &cls, std::vector<int> &reps,
+                              P pre) {
   std::vector<uint64_t> h(n);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; i++) h[i] = hash(i);
   std::unordered_map<uint64_t, int> first;
   std::vector<int> cand(n);
   for (int i = 0; i < n; i++) cand[i] = first.emplace(h[i], i).first->second;
+  pre(cand);
   std::vector<char> ok(n, 0);
 #pragma omp parallel for schedule(dynamic)
   for (int i = 0; i < n; i++) ok[i] = cand[i] == i || same(cand[i], i);
@@ -366,6 +369,10 @@ static void structure_classes(int n, H hash, E same, std::vector<int> &cls, std:
       reps.push_back(i);
     }
   }
+}
+template <class H, class E>
+static void structure_classes(int n, H hash, E same, std::vector<int> &cls, std::vector<int> &reps) {
+  structure_classes(n, hash, same, cls, reps, [](const std::vector<int> &) {});
 }
 
 struct Fnv {
@@ -1133,6 +1140,10 @@ void build_orderings(Solver &S) {
     if (S.verbose >= 3) std::printf("   orderings: %-19s %8.4f s\n", name, now - tp);
     tp = now;
   };
+  // the triplet patterns, the bulk of the comparison, are compared in HBM
+  // where the uploads put them (each subdomain against its candidate)
+  std::vector<int> cand_of;
+  std::vector<char> pat_eq;
   structure_classes(
       S.nsub,
       [&](int i) {
@@ -1144,11 +1155,16 @@ void build_orderings(Solver &S) {
       },
       [&](int a, int b) {
         const Sub &x = S.subs[a], &y = S.subs[b];
-        return x.ndofs == y.ndofs && x.nnods == y.nnods && x.kdof == y.kdof && x.ai == y.ai &&
-               x.aj == y.aj && x.int_dof == y.int_dof && x.ifc_dof == y.ifc_dof &&
-               same_shape(x, y, S.ndim);
+        const bool known = !pat_eq.empty() && cand_of[b] == a;
+        return x.ndofs == y.ndofs && x.nnods == y.nnods && x.kdof == y.kdof &&
+               x.int_dof == y.int_dof && x.ifc_dof == y.ifc_dof &&
+               (known ? pat_eq[b] != 0 : (x.ai == y.ai && x.aj == y.aj)) && same_shape(x, y, S.ndim);
       },
-      S.ord_cls, S.ord_reps);
+      S.ord_cls, S.ord_reps,
+      [&](const std::vector<int> &cand) {
+        cand_of = cand;
+        if (!device_same_patterns(S, cand, pat_eq)) pat_eq.clear();
+      });
   phase("classes");
 #pragma omp parallel for schedule(dynamic)
   for (int k = 0; k < (int)S.ord_reps.size(); k++)

@@ -178,9 +178,14 @@ void change_global_data(const int *ifix, int lifix, const double *fixv, int lfix
 // device steps (device.hip)
 void device_upload_entries(Solver &S, int is);
 void device_release_input(Solver &S);
-// setup in two halves: begin builds the tile plan and launches the interior
-// factorization; finish (after weights, Dirichlet data, unique interface)
-// builds the interface/coarse maps, launches the rest and waits
+// triplet patterns of subdomain i and cand[i] compared in HBM (eq[i] = 1:
+// equal); false when not available (then eq is untouched)
+bool device_same_patterns(Solver &S, const std::vector<int> &cand, std::vector<char> &eq);
+// setup in three stages: begin builds the tile plan and launches the
+// interior factorization; interface (after the constraints) plans and
+// launches the interface phase and the coarse problem; finish (after the
+// unique interface, Dirichlet data, weights and condensation rhs) builds the
+// solve maps and lists, queues the reduced rhs and waits
 // order of tile-task launches (kernels: 1 = one contiguous range of tasks
 // per XCD, the default; 0 = blockIdx order)
 void set_task_order(int chunked);

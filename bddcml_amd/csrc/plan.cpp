@@ -412,8 +412,10 @@ void build_topology(Solver &S) {
   // slots by relaxed atomics, a blocked prefix sum, then a sort of each
   // node's short list
   const int ng = S.nnod_g;
-  S.n2s_ptr.assign(ng + 1, 0);
+  S.n2s_ptr.resize(ng + 1);
   int *cnt = S.n2s_ptr.data();
+#pragma omp parallel for schedule(static)
+  for (int g = 0; g <= ng; g++) cnt[g] = 0;
 #pragma omp parallel for schedule(dynamic, 4)
   for (int is = 0; is < S.nsub; is++)
     for (int g : S.subs[is].isngn) __atomic_fetch_add(&cnt[g + 1], 1, __ATOMIC_RELAXED);
@@ -447,9 +449,11 @@ static void build_node_subdomains(Solver &S) {
   const double t0 = wall_time();
   const int ng = S.nnod_g;
   const int *cnt = S.n2s_ptr.data();
-  S.n2s_idx.resize(S.n2s_ptr[ng]);
-  std::vector<int> fill(S.n2s_ptr.begin(), S.n2s_ptr.end() - 1);
+  S.n2s_idx.resize(S.n2s_ptr[ng]);  // every slot is written below
+  RawInts fill(ng);
   int *f = fill.data(), *idx = S.n2s_idx.data();
+#pragma omp parallel for schedule(static)
+  for (int g = 0; g < ng; g++) f[g] = cnt[g];
   const double t1 = wall_time();
   // a node of one subdomain has one writer: a plain store, no atomic
 #pragma omp parallel for schedule(dynamic, 4)

@@ -12,6 +12,21 @@
 
 #include "plan.hpp"
 
+// std::vector whose resize leaves new elements uninitialized (the large
+// node tables are filled in parallel right after)
+template <class T, class A = std::allocator<T>>
+struct NoInitAlloc : A {
+  using A::A;
+  template <class U> struct rebind {
+    using other = NoInitAlloc<U, typename std::allocator_traits<A>::template rebind_alloc<U>>;
+  };
+  template <class U> void construct(U *p) noexcept { ::new (static_cast<void *>(p)) U; }
+  template <class U, class... Args> void construct(U *p, Args &&...args) {
+    std::allocator_traits<A>::construct(static_cast<A &>(*this), p, std::forward<Args>(args)...);
+  }
+};
+using RawInts = std::vector<int, NoInitAlloc<int>>;
+
 namespace bddc {
 
 struct Device;       // device-side plan and buffers (device.hip)
@@ -99,7 +114,7 @@ struct Solver {
   std::vector<double> xyzc;  // coarse node coordinates, nnodc x ndim column major
   bool bc_global = false;
   // node -> subdomains
-  std::vector<int> n2s_ptr, n2s_idx;
+  RawInts n2s_ptr, n2s_idx;
   // unique interface
   int nig = 0;
   std::vector<int> igdof;  // sorted global dofs

@@ -292,7 +292,7 @@ struct Device {
   int64_t ntilesf = 0;
   DBuf<double> cpool;     // coarse matrix: factor, inverse diagonal, W, full inverse
   int64_t ntilesc = 0;
-  Arena plan2;            // task lists and job lists of the second half
+  Arena plan2;            // task lists, job lists and constraint maps of the interface stage
   Arena plan2b;           // interface maps, assembly and sweep lists, vectors
   int nco = 0;            // constraints over all subdomains
   Arena plan3;            // coarse factorization lists (built during the interface phase)
@@ -766,15 +766,18 @@ static double tile_work(const std::vector<TaskList> &v, double per_task) {
 }
 
 // ------------------------------------------------------------------ setup
-// Setup runs in two halves.  device_setup_begin needs only the interface
+// Setup runs in three stages.  device_setup_begin needs only the interface
 // split and the orderings: it builds the subdomain tile layout and the task
 // lists of the interior factorization and the Schur complements, and
-// launches them without waiting.  The host then classifies globs, builds
-// the constraints, weights, Dirichlet data and the condensation right-hand
-// side while the device factors; device_setup_finish lays out the factor
-// space and the coarse tiles (pools of their own, sized by the constraints),
-// builds their task lists and the interface and coarse maps (uploaded on a
-// second stream) and launches the rest.
+// launches them without waiting.  The host then classifies globs and builds
+// the constraints while the device factors; device_setup_interface lays out
+// the factor space and the coarse tiles (pools of their own, sized by the
+// constraints), builds their task lists and the constraint maps (uploaded
+// on a second stream) and launches the interface phase and the coarse
+// problem; the host builds the interface numbering, Dirichlet data, weights
+// and the condensation right-hand side meanwhile, and device_setup_finish
+// adds the solve maps and lists, queues the reduced right-hand side and
+// waits.
 void device_setup_begin(Solver &S) {
   device_fetch_solution(S);  // a new setup keeps the last solution as the initial guess
   device_release(S);
@@ -1425,7 +1428,7 @@ void device_setup_interface(Solver &S) {
       std::printf("   device plan:   (allocations %.4f s, uploads %.4f s so far)\n", g_alloc_s, g_upload_s);
     tph = now;
   };
-  // ---- second half of the plan, now that the constraints are known: the
+  // ---- interface stage of the plan, now that the constraints are known: the
   // factor space (its own pool, FP64 or FP32 in the mixed-precision mode),
   // the coarse tiles (FP64 pool), the task lists of the S' factorization, Z
   // and Phi, the coarse factorization, the S' assembly jobs and the coarse
@@ -1799,7 +1802,7 @@ void device_setup_interface(Solver &S) {
   ph("coarse gather, upload");
   HIPCHECK(hipEventRecord(D.ev_up, D.up));
   S.t.plan += wall_time() - t0;
-  // ---- numerics, second half: augmented interface matrices, Z, Phi, coarse
+  // ---- numerics of the interface stage: augmented interface matrices, Z, Phi, coarse
   t0 = wall_time();
   st = D.st;
   HIPCHECK(hipStreamWaitEvent(st, D.ev_up, 0));

@@ -335,7 +335,8 @@ struct Device {
   int n_sp = 0;
   // interface data
   std::vector<SubDesc> hdesc;
-  // uploaded with the second half's arena (plan2)
+  // uploaded with the interface stage's arena (plan2; the interface dofs
+  // and weights with plan2b)
   View<SubDesc> desc;
   View<int64_t> loff;
   View<int> NIs, map, cid, csize, cptr, cdof, cmap;

@@ -1856,6 +1856,7 @@ void device_setup_interface(Solver &S) {
   launch_acs(st, D.desc.p, nsub, D.cptr.p, D.gen_constraints ? D.comega.p : nullptr, D.G.p,
              D.Acs.p);
   HIPCHECK(hipEventRecord(ev[3], st));
+  ph("interface launches");
   // coarse problem (with three levels the second level is set up instead):
   // its task lists and assembly map are built on the host while the device
   // runs the interface phase launched above, and uploaded on the upload
@@ -1960,28 +1961,61 @@ void device_setup_interface(Solver &S) {
       D.plan3.add(D.c_zero, cz);
       D.plan3.add(D.c_pads, cpads);
     }
+    ph("coarse factorization lists");
     // coarse assembly CSR over the lower tiles of the coarse matrix
     {
       struct E {
         int64_t dst, src;
       };
-      std::vector<E> ent;
       const int n = D.ntc;
+      // entries in subdomain order: counts, then each subdomain fills its range
+      std::vector<size_t> eo(nsub + 1, 0);
+      int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
+      for (int is = 0; is < nsub; is++) {
+        const Sub &s = S.subs[is];
+        size_t c = 0;
+        for (int a = 0; a < s.nconstr; a++)
+          for (int b = 0; b < s.nconstr; b++) {
+            const int ga = s.indrowc[a], gb = s.indrowc[b];
+            if (ga < 0 || ga >= S.ndofc || gb < 0 || gb >= S.ndofc) bad = 1;
+            else c += ga / TILE >= gb / TILE;
+          }
+        eo[is + 1] = c;
+      }
+      if (bad) fatal("BDDCML_DEVICE_PLAN", "coarse dof out of range");
+      for (int is = 0; is < nsub; is++) eo[is + 1] += eo[is];
+      std::vector<E> ent(eo[nsub]);
+#pragma omp parallel for schedule(dynamic)
       for (int is = 0; is < nsub; is++) {
         const Sub &s = S.subs[is];
         const SubDesc &d = D.hdesc[is];
+        size_t k = eo[is];
         for (int a = 0; a < s.nconstr; a++)
           for (int b = 0; b < s.nconstr; b++) {
-            int ga = s.indrowc[a], gb = s.indrowc[b];
-            if (ga < 0 || ga >= S.ndofc || gb < 0 || gb >= S.ndofc)
-              fatal("BDDCML_DEVICE_PLAN", "coarse dof out of range");
-            int I = ga / TILE, J = gb / TILE;
+            const int ga = s.indrowc[a], gb = s.indrowc[b];
+            const int I = ga / TILE, J = gb / TILE;
             if (I < J) continue;
-            int64_t dst = (int64_t)(D.cbase + lowidx(n, I, J)) * TILE_ELEMS + (gb % TILE) * TILE + ga % TILE;
-            ent.push_back({dst, d.goff + (int64_t)a * s.nconstr + b});
+            const int64_t dst = (int64_t)(D.cbase + lowidx(n, I, J)) * TILE_ELEMS + (gb % TILE) * TILE + ga % TILE;
+            ent[k++] = E{dst, d.goff + (int64_t)a * s.nconstr + b};
           }
       }
-      std::stable_sort(ent.begin(), ent.end(), [](const E &a, const E &b) { return a.dst < b.dst; });
+      // stable sort by destination: least-significant-digit radix passes
+      {
+        const int64_t maxd = (int64_t)D.ntilesc * TILE_ELEMS;
+        int bits = 1;
+        while (bits < 62 && ((int64_t)1 << bits) < maxd) bits++;
+        constexpr int RB = 12, NB = 1 << RB;
+        std::vector<E> tmp(ent.size());
+        std::vector<size_t> c(NB + 1);
+        for (int sh = 0; sh < bits; sh += RB) {
+          std::fill(c.begin(), c.end(), 0);
+          for (const E &e : ent) c[((e.dst >> sh) & (NB - 1)) + 1]++;
+          for (int b = 0; b < NB; b++) c[b + 1] += c[b];
+          for (const E &e : ent) tmp[c[(e.dst >> sh) & (NB - 1)]++] = e;
+          ent.swap(tmp);
+        }
+      }
       std::vector<int64_t> dst, src;
       std::vector<int> ptr(1, 0);
       for (size_t k = 0; k < ent.size(); k++) {
@@ -1997,6 +2031,7 @@ void device_setup_interface(Solver &S) {
       D.plan3.add(D.ca_ptr, ptr);
       D.nca = (int)dst.size();
     }
+    ph("coarse assembly lists");
     D.plan3.commit(D.up);
     S.t.plan += wall_time() - tc;
     // coarse tiles cleared (they accumulate), padding diagonals 1

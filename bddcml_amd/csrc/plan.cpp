@@ -170,28 +170,62 @@ static void neighbours_of(Solver &S, int is) {
 
 // glob classification and face-triangle corner selection
 // (src/module_dd.f90:9130-9300 and 10017-10327)
+// The nodes of `nodes` (ascending) grouped by equal glob keys: each group
+// ascending; the groups in no particular order (their users sort or only
+// classify them).
+static std::vector<std::vector<int>> key_groups(const Sub &s, std::vector<int> nodes) {
+  auto less = [&](int a, int b) {
+    const int *pa = s.nkval.data() + s.nkptr[a], *pb = s.nkval.data() + s.nkptr[b];
+    const int la = s.nkptr[a + 1] - s.nkptr[a], lb = s.nkptr[b + 1] - s.nkptr[b];
+    if (la != lb) return la < lb;
+    for (int k = 0; k < la; k++)
+      if (pa[k] != pb[k]) return pa[k] < pb[k];
+    return a < b;
+  };
+  auto same = [&](int a, int b) {
+    const int la = s.nkptr[a + 1] - s.nkptr[a];
+    return la == s.nkptr[b + 1] - s.nkptr[b] &&
+           std::equal(s.nkval.begin() + s.nkptr[a], s.nkval.begin() + s.nkptr[a + 1],
+                      s.nkval.begin() + s.nkptr[b]);
+  };
+  std::sort(nodes.begin(), nodes.end(), less);
+  std::vector<std::vector<int>> out;
+  for (size_t k = 0; k < nodes.size(); k++) {
+    if (k == 0 || !same(nodes[k - 1], nodes[k])) out.emplace_back();
+    out.back().push_back(nodes[k]);
+  }
+  return out;
+}
+
 static void globs_first_pass(Solver &S, int is) {
   Sub &s = S.subs[is];
   int nnodi = (int)s.iin.size();
   s.nsubnode.assign(nnodi, 0);
-  s.nodekey.assign(nnodi, {});
   for (size_t a = 0; a < s.adj.size(); a++)
-    for (int q = s.shptr[a]; q < s.shptr[a + 1]; q++) {
-      int in = s.shnode[q];
-      s.nsubnode[in]++;
-      auto &k = s.nodekey[in];
-      k.push_back(s.adj[a]);
-      k.push_back(s.shcomp[q]);
-      k.push_back(s.ncomp[s.iin[in]]);
-    }
+    for (int q = s.shptr[a]; q < s.shptr[a + 1]; q++) s.nsubnode[s.shnode[q]]++;
+  s.nkptr.assign(nnodi + 1, 0);
+  for (int i = 0; i < nnodi; i++) s.nkptr[i + 1] = s.nkptr[i] + 3 * s.nsubnode[i];
+  s.nkval.resize(s.nkptr[nnodi]);
+  {
+    std::vector<int> f(s.nkptr.begin(), s.nkptr.end() - 1);
+    for (size_t a = 0; a < s.adj.size(); a++)
+      for (int q = s.shptr[a]; q < s.shptr[a + 1]; q++) {
+        const int in = s.shnode[q];
+        int *k = s.nkval.data() + f[in];
+        f[in] += 3;
+        k[0] = s.adj[a];
+        k[1] = s.shcomp[q];
+        k[2] = s.ncomp[s.iin[in]];
+      }
+  }
   // group nodes with equal keys
-  std::map<std::vector<int>, std::vector<int>> groups;
-  for (int i = 0; i < nnodi; i++) groups[s.nodekey[i]].push_back(i);
+  std::vector<int> all(nnodi);
+  for (int i = 0; i < nnodi; i++) all[i] = i;
   std::vector<int> gt(nnodi, 0);
-  for (auto &kv : groups) {
-    int nsubn = (int)kv.first.size() / 3;
-    int t = nsubn == 1 ? 1 : (kv.second.size() > 1 ? 2 : 3);
-    for (int i : kv.second) gt[i] = t;
+  for (const std::vector<int> &grp : key_groups(s, std::move(all))) {
+    const int nsubn = (s.nkptr[grp[0] + 1] - s.nkptr[grp[0]]) / 3;
+    const int t = nsubn == 1 ? 1 : (grp.size() > 1 ? 2 : 3);
+    for (int i : grp) gt[i] = t;
   }
   for (int i = 0; i < nnodi; i++)
     if (s.nsubnode[i] == 1) gt[i] = 1;
@@ -310,14 +344,14 @@ static void globs_second_pass(Solver &S, int is) {
       for (int d = s.kdof[ln]; d < s.kdof[ln + 1]; d++)
         if (s.ifix[d]) kind[i] = -2;
     }
-  std::map<std::vector<int>, std::vector<int>> groups;
+  std::vector<int> free_nodes;
   for (int i = 0; i < nnodi; i++)
-    if (kind[i] == 0) groups[s.nodekey[i]].push_back(i);
+    if (kind[i] == 0) free_nodes.push_back(i);
   s.globs.clear();
-  for (auto &kv : groups) {
+  for (std::vector<int> &grp : key_groups(s, std::move(free_nodes))) {
     Sub::Glob g;
-    g.type = kv.first.size() / 3 == 1 ? 1 : 2;
-    g.nodes = kv.second;
+    g.type = (s.nkptr[grp[0] + 1] - s.nkptr[grp[0]]) / 3 == 1 ? 1 : 2;
+    g.nodes = std::move(grp);
     g.key = INT32_MAX;
     for (int i : g.nodes) g.key = std::min(g.key, s.isngn[s.iin[i]]);
     s.globs.push_back(std::move(g));

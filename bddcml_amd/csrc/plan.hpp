@@ -59,7 +59,9 @@ struct Sub {
   // shared interface nodes per neighbour, sorted by global node id
   std::vector<int> shptr, shnode, shcomp;
   std::vector<int> nsubnode;               // number of neighbours sharing a node
-  std::vector<std::vector<int>> nodekey;   // (neighbour, its comp, my comp) triples
+  // glob key of each interface node: its (neighbour, its comp, my comp)
+  // triples in neighbour order, node i's at nkval[nkptr[i] .. nkptr[i+1])
+  std::vector<int> nkptr, nkval;
   std::vector<int> lcorner;                // locally selected corners (global nodes)
   std::vector<int> corners;                // final corners (sorted global nodes)
   struct Glob {

@@ -178,7 +178,7 @@ struct Arena {
   int verbose = 0;
   bool open = false;
   size_t bytes() const { return open ? staging().used : 0; }
-  template <class T> void add(View<T> &v, const std::vector<T> &src) {
+  template <class T, class A> void add(View<T> &v, const std::vector<T, A> &src) {
     if (verbose >= 4) std::printf("   device plan: arena %10zu bytes\n", src.size() * sizeof(T));
     Staging &h = staging();
     if (!open) h.used = 0, open = true;
@@ -1344,8 +1344,8 @@ static void build_resid_index(Solver &S) {
     eno[k + 1] = eno[k] + ne;
   }
   if (rpo[nrep] > (size_t)INT32_MAX) fatal("BDDCML_DEVICE_PLAN", "residual index too large");
-  std::vector<int> rptr(rpo[nrep]);
-  std::vector<int2> ent(eno[nrep]);
+  RawVec<int> rptr(rpo[nrep]);  // every entry is written below
+  RawVec<int2> ent(eno[nrep]);
 #pragma omp parallel for schedule(dynamic)
   for (int k = 0; k < nrep; k++) {
     const Sub &s = S.subs[oreps[k]];
@@ -2079,8 +2079,9 @@ void device_setup_finish(Solver &S) {
   for (int is = 0; is < nsub; is++) loff[is] = D.hdesc[is].loff;
   // interface dof of each local interface entry (-1: padding) and its weight
   // devpos: device position of each reference interface index
-  std::vector<int> map(lv), devpos(lv);
-  std::vector<double> wv(lv);
+  // (every entry read below is written first)
+  RawVec<int> map(lv), devpos(lv);
+  RawVec<double> wv(lv);
 #pragma omp parallel for schedule(dynamic)
   for (int is = 0; is < nsub; is++) {
     const Sub &s = S.subs[is];
@@ -2109,7 +2110,7 @@ void device_setup_finish(Solver &S) {
     const std::vector<int> &cnt = S.ig_ptr;
     if ((int)cnt.size() != S.nig + 1) fatal("BDDCML_DEVICE_PLAN", "interface numbering missing");
     const int64_t ncopy = cnt[S.nig];
-    std::vector<int64_t> src(ncopy);
+    RawVec<int64_t> src(ncopy);
     int bad = 0;
 #pragma omp parallel for schedule(static) reduction(| : bad)
     for (int64_t q = 0; q < ncopy; q++) {
@@ -2146,11 +2147,11 @@ void device_setup_finish(Solver &S) {
       D.ndofs_total = po;
     }
     D.plan2b.add(D.odesc, D.hodesc);
-    std::vector<int64_t> vpos(lv);
+    RawVec<int64_t> vpos(lv);
 #pragma omp parallel for schedule(static)
     for (int is = 0; is < nsub; is++)
       for (int k = 0; k < S.subs[is].NI; k++) vpos[loff[is] + k] = voff[is] + S.subs[is].NO + k;
-    std::vector<int64_t> rsrc(src.size());
+    RawVec<int64_t> rsrc(src.size());
 #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < src.size(); q++) rsrc[q] = vpos[src[q]];
     D.plan2b.add(D.rhs_src, rsrc);
@@ -2237,7 +2238,7 @@ void device_setup_finish(Solver &S) {
   D.rc.zero(st);
   D.uc.zero(st);
   {
-    std::vector<double> fm(D.ndofs_total);
+    RawVec<double> fm(D.ndofs_total);
 #pragma omp parallel for schedule(static)
     for (int is = 0; is < nsub; is++)
       std::copy(S.subs[is].fmod.begin(), S.subs[is].fmod.end(), fm.begin() + D.hodesc[is].poff);

@@ -25,7 +25,8 @@ struct NoInitAlloc : A {
     std::allocator_traits<A>::construct(static_cast<A &>(*this), p, std::forward<Args>(args)...);
   }
 };
-using RawInts = std::vector<int, NoInitAlloc<int>>;
+template <class T> using RawVec = std::vector<T, NoInitAlloc<T>>;
+using RawInts = RawVec<int>;
 
 namespace bddc {
 

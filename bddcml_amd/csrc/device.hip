@@ -2172,8 +2172,8 @@ void device_setup_finish(Solver &S) {
       eoff[is + 1] = eoff[is] + ne;
       D.htrsv[is] = TrsvDesc{T.nto, T.nti, (int)roff[is], (int)coff_[is], T.linvA, voff[is]};
     }
-    std::vector<int> rp(roff[nsub]), rkv(eoff[nsub]), rtv(eoff[nsub]), cp(coff_[nsub]),
-        ckv(eoff[nsub]), ctv(eoff[nsub]);
+    RawVec<int> rp(roff[nsub]), rkv(eoff[nsub]), rtv(eoff[nsub]), cp(coff_[nsub]),
+        ckv(eoff[nsub]), ctv(eoff[nsub]);  // every entry is written below
 #pragma omp parallel for schedule(dynamic)
     for (int is = 0; is < nsub; is++) {
       const SubTiles &T = D.sub[is];

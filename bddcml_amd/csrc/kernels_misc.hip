@@ -897,22 +897,36 @@ __global__ __launch_bounds__(256) void k_phit(const SubDesc *__restrict__ subs,
     }
     __syncthreads();
   }
+  // the wave's (up to) four columns c0 + wave + 4k side by side, one
+  // accumulator each (the order of each column's sum is the plain one):
+  // four independent loads per step instead of one dependent chain
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int c = c0 + wave; c < min(c0 + PHIT_COLS, d.nc); c += 4) {
-    const double *pc = Phi + d.zoff + (size_t)c * NI;
-    double s = 0.0;
-    for (int i = lane; i < NI; i += 64) {
-      double xv;
-      if (STAGE) {
-        xv = xw[i];
-      } else {
-        const int g = map[d.loff + i];
-        xv = g >= 0 ? w[d.loff + i] * x[g] : 0.0;
-      }
-      s += pc[i] * xv;
+  constexpr int CPW = PHIT_COLS / 4;
+  const int cend = min(c0 + PHIT_COLS, d.nc);
+  const double *pc[CPW];
+  double s[CPW];
+#pragma unroll
+  for (int k = 0; k < CPW; k++) {
+    const int c = min(c0 + wave + 4 * k, cend - 1);  // a column past the end reads a valid one
+    pc[k] = Phi + d.zoff + (size_t)c * NI;
+    s[k] = 0.0;
+  }
+  for (int i = lane; i < NI; i += 64) {
+    double xv;
+    if (STAGE) {
+      xv = xw[i];
+    } else {
+      const int g = map[d.loff + i];
+      xv = g >= 0 ? w[d.loff + i] * x[g] : 0.0;
     }
-    s = wave_sum(s);
-    if (lane == 0) rcs[d.coff + c] = s;
+#pragma unroll
+    for (int k = 0; k < CPW; k++) s[k] += pc[k][i] * xv;
+  }
+#pragma unroll
+  for (int k = 0; k < CPW; k++) {
+    const double t = wave_sum(s[k]);
+    const int c = c0 + wave + 4 * k;
+    if (lane == 0 && c < cend) rcs[d.coff + c] = t;
   }
 }
 

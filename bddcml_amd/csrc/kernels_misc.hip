@@ -721,6 +721,7 @@ __global__ __launch_bounds__(256) void k_symv_reduce(const SymvRow *__restrict__
   if (row >= nrows) return;
   const SymvRow R = rows[row];
   double s = 0.0;
+#pragma unroll 8
   for (int J = 0; J <= R.I; J++)
     s += part[(size_t)(R.psub + J * R.nti - J * (J - 1) / 2 + (R.I - J)) * 64 + r];
   yout[loff[R.mat] + R.I * 64 + r] = s;
@@ -831,7 +832,9 @@ __global__ __launch_bounds__(256) void k_dense_gemv(const double *__restrict__ p
 __global__ void k_dense_gemv_reduce(int nt, const double *__restrict__ part, double *__restrict__ y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nt * 64) return;
+  // unrolled: the loads go out ahead of the (unchanged) chain of sums
   double s = 0.0;
+#pragma unroll 8
   for (int J = 0; J < nt; J++) s += part[(size_t)J * nt * 64 + i];
   y[i] = s;
 }
@@ -848,6 +851,7 @@ __global__ void k_assemble(int n, const int *__restrict__ ptr, const int64_t *__
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
   double s = 0.0;
+#pragma unroll 4
   for (int q = ptr[g]; q < ptr[g + 1]; q++) {
     int64_t k = src[q];
     s += w ? w[k] * yl[k] : yl[k];
